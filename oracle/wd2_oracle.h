@@ -1,0 +1,139 @@
+/*
+ * wd2_oracle.h -- CPU restatement of the wolfd2 per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity checker for the HIP path in
+ * wolfd2_amd: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it, and only to check or time against.  The product never
+ * links it and has no CPU fallback.
+ *
+ * Every function restates one reference subroutine statement for statement,
+ * in the same floating-point operation order (build with -ffp-contract=off),
+ * so that on identical inputs it reproduces the reference's FP64 results bit
+ * for bit.  Argument lists follow the Fortran ones (src/momentum.f,
+ * src/pressure.f, src/utility.f, src/bound_cond.f) with the caller's array
+ * layout made explicit:
+ *   - every field is a double array dimensioned (0:mnx,0:mny) column-major;
+ *     element (i,j) lives at f[i + ld*j] with ld = mnx+1;
+ *   - region tables are Fortran arrays: nRegBrd(mgri,mgrj,4),
+ *     nRegType(mgri,mgrj), nMomBdTp(mgri,mgrj,4), dBCVal(mgri,mgrj,4,4),
+ *     dPR*(mgri,mgrj); nReg(2) holds the region counts in i and j.
+ * Functions return 0, or a negative code for a bad boundary/solver flag
+ * (where the reference prints a message and STOPs).
+ */
+#ifndef WD2_ORACLE_H
+#define WD2_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Region tables and grid extents shared by every hot-path call. */
+typedef struct {
+    int nx, ny;            /* grid nodes (grid.f:220-221) */
+    int ld;                /* leading dimension, mnx+1 */
+    int mgri, mgrj;        /* region table extents (config.f:21-22) */
+    const int *nReg;       /* (2) */
+    const int *nRegBrd;    /* (mgri,mgrj,4) */
+    const int *nRegType;   /* (mgri,mgrj) */
+    const int *nMomBdTp;   /* (mgri,mgrj,4) */
+    const double *dBCVal;  /* (mgri,mgrj,4,4) */
+    const double *dPRporos, *dPRporc1, *dPRporc2;   /* (mgri,mgrj) */
+} wd2o_regions;
+
+/* The metric arrays produced by grid.f:Metric (names as in main.f:103-118). */
+typedef struct {
+    const double *ran, *rbn, *rgn, *rac, *rbc, *rgc;
+    const double *rau, *rbu, *rbv, *rgv;
+    const double *dju, *djv, *djc, *djn;
+    const double *xen, *yen, *xzn, *yzn;
+    const double *xec, *yec, *xzc, *yzc;
+    const double *xeu, *yeu, *xzv, *yzv;
+    const double *xzu, *yzu, *xev, *yev;
+} wd2o_metrics;
+
+/* grid.f:Grid 31-127 without the file read: scale the nodes gx,gy (given on
+ * i=1..nx, j=1..ny) by 1/dlref, mirror, interpolate, compute the metrics.
+ * Output arrays must be zero-filled by the caller; entries outside
+ * 1..nx x 1..ny are left as they are, as in the reference. */
+void wd2o_grid(int nx, int ny, int ld, double dlref, double *gx, double *gy,
+               double *rau, double *rbu, double *rbv, double *rgv,
+               double *ran, double *rbn, double *rgn,
+               double *rac, double *rbc, double *rgc,
+               double *dju, double *djv, double *djc, double *djn,
+               double *xen, double *yen, double *xzn, double *yzn,
+               double *xec, double *yec, double *xzc, double *yzc,
+               double *xeu, double *yeu, double *xzv, double *yzv,
+               double *xzu, double *yzu, double *xev, double *yev);
+
+/* momentum.f:AltTridLU 1307-1339 (a is a(3,n): sub, diag, super per row). */
+void wd2o_alt_trid_lu(int n, double *a, double *b);
+
+/* momentum.f:ConvCoef 864-981 */
+int wd2o_conv_coef(int nx, int ny, int ld, int ncomp, int njacob,
+                   const double *xzi, const double *xet, const double *yzi, const double *yet,
+                   const double *u, const double *v, double *cc1, double *cc2);
+
+/* momentum.f:XMomentum 199-514 / YMomentum 520-838 (dus / dvs written on
+ * the reference's output range only). */
+int wd2o_x_momentum(const wd2o_regions *r, double dk, double re, const wd2o_metrics *m,
+                    const double *us, const double *vs, const double *un, const double *vn,
+                    double *dus);
+int wd2o_y_momentum(const wd2o_regions *r, double dk, double re, double fr,
+                    const wd2o_metrics *m, const double *d, const double *dn,
+                    const double *us, const double *vs, const double *un, const double *vn,
+                    double *dvs);
+
+/* momentum.f:nAuxMomentum 33-193; returns m on convergence, -1 otherwise,
+ * or a code <= -100 for a bad flag. */
+int wd2o_naux_momentum(const wd2o_regions *r, int mqiter, double dk, double re, double fr,
+                       double qtol, const wd2o_metrics *m, const double *d, const double *dn,
+                       const double *un, const double *vn, double *us, double *vs);
+
+/* bound_cond.f:VelBoundCond 511-847, PresBoundCond 853-1024, VelOutflowBCs 1656-1874 */
+int wd2o_vel_bound_cond(const wd2o_regions *r, double *u, double *v);
+int wd2o_pres_bound_cond(const wd2o_regions *r, double *p);
+int wd2o_vel_outflow_bcs(const wd2o_regions *r, double *u, double *v);
+
+/* pressure.f:Divergence 265-323 (nloc 1 or 2) */
+int wd2o_divergence(int nx, int ny, int ld, int nloc,
+                    const double *xet, const double *yet, const double *xzi, const double *yzi,
+                    const double *u, const double *v, double *div);
+
+/* pressure.f:Ppe 30-249; nSorConv receives the iteration count (msorit when
+ * not converged, as the reference sets it after its warning). */
+int wd2o_ppe(const wd2o_regions *r, int lCartesGrid, int nPpeSolver, int msorit,
+             int *nSorConv, double dk, double sortol, double sorrel,
+             const wd2o_metrics *m, const double *u, const double *v, double *p);
+
+/* utility.f:Project 253-440 */
+int wd2o_project(const wd2o_regions *r, double dk, const wd2o_metrics *m,
+                 const double *p, double *u, double *v);
+
+/* utility.f:DMaxNorm 479-507, DiffMaxNorm 446-473 */
+double wd2o_dmax_norm(int nx, int ny, int ld, const double *u);
+double wd2o_diff_max_norm(int nx, int ny, int ld, const double *un, const double *u);
+
+/* One time step of main.f:690-972 for cold flow (nthermen=0, nsmallscl=0,
+ * no filters, nmeiter=1): pn,un,vn copies, nAuxMomentum, VelBoundCond,
+ * PresBoundCond, Ppe, PresBoundCond, Project, VelBoundCond, PresBoundCond,
+ * u=us, v=vs, VelBoundCond, PresBoundCond, DiffMaxNorm.  Work arrays
+ * (pn,un,vn,us,vs: ld*(mny+1) doubles each) are caller-owned.  out_iter
+ * receives {nQLiter, nSorConv}; out_dif the PrintDiff norms
+ * {|p1-pn|,|u1-un|,|v1-vn|}. */
+int wd2o_step(const wd2o_regions *r, const wd2o_metrics *m, int lCartesGrid,
+              int mqiter, double qtol, int nPpeSolver, int msorit, double sortol,
+              double sorrel, double dk, double re, double fr,
+              const double *d, double *dn,
+              double *u, double *v, double *p,
+              double *pn, double *un, double *vn, double *us, double *vs,
+              int *out_iter, double *out_dif);
+
+/* Cold-start projection, main.f:606-641. */
+int wd2o_cold_start(const wd2o_regions *r, const wd2o_metrics *m, int lCartesGrid,
+                    int nPpeSolver, int msorit, double sortol, double sorrel, double dk,
+                    double *u, double *v, double *p, int *nSorConv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

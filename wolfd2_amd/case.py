@@ -1,0 +1,357 @@
+"""Case set-up for the wolfd2 hot path: grid metrics, region tables, scalings.
+
+This is the part of the reference's initialisation that produces the inputs of
+the per-timestep calls (main.f:376-601): the metric arrays of grid.f, the
+region/boundary tables of bound_cond.f:SetUpBCs and parse.f:InitBCFlags, and
+the non-dimensional parameters of file_manip.f:ReadParam.  It runs once per
+case on the host, in numpy, with the reference's operation order so that the
+arrays are bit-identical to the reference's.
+
+Array convention (the reference's): a field dimensioned (0:mnx,0:mny) in
+column-major order is a C-contiguous numpy array of shape (mny+1, mnx+1)
+indexed [j, i]; the leading dimension is ld = mnx+1.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# wolfd2.h:15-42
+RM_BLOCKG, RM_INTERN, RM_POROUS = 0, 1, 2
+BM_INTERN, BM_WALL1, BM_WALL2, BM_INLET, BM_OUTLT1, BM_OUTLT2 = 0, 1, 2, 3, 4, 5
+WEST, EAST, SOUTH, NORTH = 1, 2, 3, 4
+VAR_U, VAR_V, VAR_P, VAR_T = 1, 2, 3, 4
+SIDES = {"west": WEST, "east": EAST, "south": SOUTH, "north": NORTH}
+
+MGRI, MGRJ = 20, 10          # config.f:21-22
+
+METRIC_NAMES = (
+    "rau", "rbu", "rbv", "rgv", "ran", "rbn", "rgn", "rac", "rbc", "rgc",
+    "dju", "djv", "djc", "djn", "xen", "yen", "xzn", "yzn", "xec", "yec",
+    "xzc", "yzc", "xeu", "yeu", "xzv", "yzv", "xzu", "yzu", "xev", "yev",
+)
+
+
+def field_array(ld: int, ncol: int, dtype=np.float64) -> np.ndarray:
+    """Zero-filled (0:ld-1, 0:ncol-1) Fortran field as a [j, i] numpy array."""
+    return np.zeros((ncol, ld), dtype=dtype)
+
+
+@dataclass
+class Grid:
+    """Nodes and metric arrays of grid.f:Grid 31-127.
+
+    ``x``, ``y`` are the node coordinates on i=1..nx, j=1..ny as (ny, nx)
+    arrays.  Metric arrays are (mny+1, mnx+1) with ld = mnx+1 and are zero
+    outside i=1..nx, j=1..ny, exactly as the reference leaves them.
+    """
+    nx: int
+    ny: int
+    ld: int
+    ncol: int
+    metrics: dict = field(default_factory=dict)
+    gx: np.ndarray | None = None
+    gy: np.ndarray | None = None
+
+    def __getattr__(self, name):
+        m = self.__dict__.get("metrics")
+        if m is not None and name in m:
+            return m[name]
+        raise AttributeError(name)
+
+
+def compute_grid(x: np.ndarray, y: np.ndarray, dlref: float = 1.0,
+                 mnx: int | None = None, mny: int | None = None) -> Grid:
+    """grid.f:Grid without the file read: scale, MirrorPts, FullGrid, Metric."""
+    ny, nx = x.shape
+    mnx = nx + 1 if mnx is None else mnx
+    mny = ny + 1 if mny is None else mny
+    if nx + 1 > mnx or ny + 1 > mny:
+        raise ValueError(f"grid {nx}x{ny} needs mnx>={nx + 1}, mny>={ny + 1} (grid.f:CheckGridSize)")
+    ld, ncol = mnx + 1, mny + 1
+    gx = field_array(ld, ncol)
+    gy = field_array(ld, ncol)
+    gx[1:ny + 1, 1:nx + 1] = x
+    gy[1:ny + 1, 1:nx + 1] = y
+    # grid.f:100-106
+    gx[1:ny + 1, 1:nx + 1] = gx[1:ny + 1, 1:nx + 1] / dlref
+    gy[1:ny + 1, 1:nx + 1] = gy[1:ny + 1, 1:nx + 1] / dlref
+
+    # grid.f:MirrorPts 258-306 (arrays indexed [j, i])
+    for a in (gx, gy):
+        a[0, 1:nx + 1] = 2.0 * a[1, 1:nx + 1] - a[2, 1:nx + 1]
+        a[ny + 1, 1:nx + 1] = 2.0 * a[ny, 1:nx + 1] - a[ny - 1, 1:nx + 1]
+    for a in (gx, gy):
+        a[1:ny + 1, 0] = 2.0 * a[1:ny + 1, 1] - a[1:ny + 1, 2]
+        a[1:ny + 1, nx + 1] = 2.0 * a[1:ny + 1, nx] - a[1:ny + 1, nx - 1]
+    for a in (gx, gy):
+        a[0, 0] = 2.0 * a[1, 1] - a[2, 2]
+        a[ny + 1, 0] = 2.0 * a[ny, 1] - a[ny - 1, 2]
+        a[0, nx + 1] = 2.0 * a[1, nx] - a[2, nx - 1]
+        a[ny + 1, nx + 1] = 2.0 * a[ny, nx] - a[ny - 1, nx - 1]
+
+    # grid.f:FullGrid 312-362
+    xu = field_array(ld, ncol); yu = field_array(ld, ncol)
+    xv = field_array(ld, ncol); yv = field_array(ld, ncol)
+    xc = field_array(ld, ncol); yc = field_array(ld, ncol)
+    J1 = slice(1, ny + 2); J0 = slice(0, ny + 1)
+    I = slice(0, nx + 2)
+    xu[J1, I] = 0.5 * (gx[J1, I] + gx[J0, I])
+    yu[J1, I] = 0.5 * (gy[J1, I] + gy[J0, I])
+    Ja = slice(0, ny + 2); I1 = slice(1, nx + 2); I0 = slice(0, nx + 1)
+    xv[Ja, I1] = 0.5 * (gx[Ja, I1] + gx[Ja, I0])
+    yv[Ja, I1] = 0.5 * (gy[Ja, I1] + gy[Ja, I0])
+    xc[J1, I1] = 0.5 * (xu[J1, I1] + xu[J1, I0])
+    yc[J1, I1] = 0.5 * (yv[J1, I1] + yv[J0, I1])
+
+    # grid.f:Metric 368-535 on i=1..nx, j=1..ny
+    m = {nm: field_array(ld, ncol) for nm in METRIC_NAMES}
+    J = slice(1, ny + 1); Jp = slice(2, ny + 2); Jm = slice(0, ny)
+    Ic = slice(1, nx + 1); Ip = slice(2, nx + 2); Im = slice(0, nx)
+
+    def inner(a, b, c, d):
+        return a * b + c * d
+
+    # N
+    xzn = xv[J, Ip] - xv[J, Ic]; xen = xu[Jp, Ic] - xu[J, Ic]
+    yzn = yv[J, Ip] - yv[J, Ic]; yen = yu[Jp, Ic] - yu[J, Ic]
+    djn = 1.0 / (xzn * yen - xen * yzn)
+    g11 = inner(xzn, xzn, yzn, yzn); g12 = inner(xzn, xen, yzn, yen); g22 = inner(xen, xen, yen, yen)
+    m["xzn"][J, Ic], m["xen"][J, Ic], m["yzn"][J, Ic], m["yen"][J, Ic] = xzn, xen, yzn, yen
+    m["djn"][J, Ic] = djn
+    m["ran"][J, Ic] = djn * g22
+    m["rbn"][J, Ic] = -djn * g12 * 0.25
+    m["rgn"][J, Ic] = djn * g11
+    # U
+    xzu = xc[J, Ip] - xc[J, Ic]; xeu = gx[J, Ic] - gx[Jm, Ic]
+    yzu = yc[J, Ip] - yc[J, Ic]; yeu = gy[J, Ic] - gy[Jm, Ic]
+    dju = 1.0 / (xzu * yeu - xeu * yzu)
+    g12 = inner(xzu, xeu, yzu, yeu); g22 = inner(xeu, xeu, yeu, yeu)
+    m["xzu"][J, Ic], m["xeu"][J, Ic], m["yzu"][J, Ic], m["yeu"][J, Ic] = xzu, xeu, yzu, yeu
+    m["dju"][J, Ic] = dju
+    m["rau"][J, Ic] = dju * g22
+    m["rbu"][J, Ic] = -dju * g12 * 0.25
+    # V
+    xzv = gx[J, Ic] - gx[J, Im]; xev = xc[Jp, Ic] - xc[J, Ic]
+    yzv = gy[J, Ic] - gy[J, Im]; yev = yc[Jp, Ic] - yc[J, Ic]
+    djv = 1.0 / (xzv * yev - xev * yzv)
+    g11 = inner(xzv, xzv, yzv, yzv); g12 = inner(xzv, xev, yzv, yev)
+    m["xzv"][J, Ic], m["xev"][J, Ic], m["yzv"][J, Ic], m["yev"][J, Ic] = xzv, xev, yzv, yev
+    m["djv"][J, Ic] = djv
+    m["rbv"][J, Ic] = -djv * g12 * 0.25
+    m["rgv"][J, Ic] = djv * g11
+    # C
+    xzc = xu[J, Ic] - xu[J, Im]; xec = xv[J, Ic] - xv[Jm, Ic]
+    yzc = yu[J, Ic] - yu[J, Im]; yec = yv[J, Ic] - yv[Jm, Ic]
+    djc = 1.0 / (xzc * yec - xec * yzc)
+    g11 = inner(xzc, xzc, yzc, yzc); g12 = inner(xzc, xec, yzc, yec); g22 = inner(xec, xec, yec, yec)
+    m["xzc"][J, Ic], m["xec"][J, Ic], m["yzc"][J, Ic], m["yec"][J, Ic] = xzc, xec, yzc, yec
+    m["djc"][J, Ic] = djc
+    m["rac"][J, Ic] = djc * g22
+    m["rbc"][J, Ic] = -djc * g12 * 0.25
+    m["rgc"][J, Ic] = djc * g11
+    return Grid(nx=nx, ny=ny, ld=ld, ncol=ncol, metrics=m, gx=gx, gy=gy)
+
+
+def uniform_nodes(nx: int, ny: int, lx: float = 1.0, ly: float = 1.0):
+    """Nodes x=(i-1)/(nx-1)*Lx, y=(j-1)/(ny-1)*Ly as (ny, nx) arrays."""
+    i = np.arange(nx, dtype=np.float64)
+    j = np.arange(ny, dtype=np.float64)
+    x = np.broadcast_to(i[None, :] / (nx - 1) * lx, (ny, nx)).copy()
+    y = np.broadcast_to(j[:, None] / (ny - 1) * ly, (ny, nx)).copy()
+    return x, y
+
+
+def skewed_nodes(nx: int, ny: int, stretch: float = 0.15, skew: float = 0.2, seed: int = 0):
+    """A stretched, sheared, slightly perturbed grid: every cross metric is non-zero."""
+    rng = np.random.default_rng(seed)
+    s = np.arange(nx, dtype=np.float64) / (nx - 1)
+    t = np.arange(ny, dtype=np.float64) / (ny - 1)
+    xs = s + stretch * np.sin(np.pi * s) * s
+    yt = t + 0.5 * stretch * np.sin(np.pi * t)
+    x = xs[None, :] + skew * yt[:, None] * (1.0 + 0.1 * s[None, :])
+    y = yt[:, None] * (1.0 + 0.05 * np.cos(np.pi * s[None, :])) + 0.03 * s[None, :]
+    jit = 0.02 / max(nx, ny)
+    x = x + jit * rng.standard_normal((ny, nx)) * (np.arange(ny)[:, None] % (ny - 1) > 0) * (np.arange(nx)[None, :] % (nx - 1) > 0)
+    y = y + jit * rng.standard_normal((ny, nx)) * (np.arange(ny)[:, None] % (ny - 1) > 0) * (np.arange(nx)[None, :] % (nx - 1) > 0)
+    return x, y
+
+
+@dataclass
+class Regions:
+    """Region and boundary tables as built by bound_cond.f:SetUpBCs 34-443.
+
+    All tables are Fortran-ordered int32/float64 arrays of the reference's
+    declared extents (mgri, mgrj): nRegBrd(mgri,mgrj,4), nRegType(mgri,mgrj),
+    nMomBdTp(mgri,mgrj,4), dBCVal(mgri,mgrj,4,4), dPR*(mgri,mgrj).
+    """
+    nx: int
+    ny: int
+    nreg: np.ndarray
+    nRegBrd: np.ndarray
+    nRegType: np.ndarray
+    nMomBdTp: np.ndarray
+    dBCVal: np.ndarray
+    dPRporos: np.ndarray
+    dPRporc1: np.ndarray
+    dPRporc2: np.ndarray
+    mgri: int = MGRI
+    mgrj: int = MGRJ
+
+    def border(self, ir, jr, side):
+        return int(self.nRegBrd[ir - 1, jr - 1, side - 1])
+
+    def btype(self, ir, jr, side):
+        return int(self.nMomBdTp[ir - 1, jr - 1, side - 1])
+
+    def rtype(self, ir, jr):
+        return int(self.nRegType[ir - 1, jr - 1])
+
+
+def make_regions(nx: int, ny: int, nreg=(1, 1), i_borders=(), j_borders=(),
+                 walls=(), inlets=(), outlets=(), blockages=(), porous=(),
+                 mgri: int = MGRI, mgrj: int = MGRJ) -> Regions:
+    """Region tables for ``nreg`` regions, as the boundary_conditions section builds them.
+
+    ``walls`` items: (ireg, jreg, side, kind, tangent_vel, press) with kind
+    'no_slip'|'no_stress' (parse.f TWALL); ``inlets``: (ireg, jreg, side, u, v,
+    press); ``outlets``: (ireg, jreg, side, kind 'fully_dev'|'mass_cons', u, v,
+    press); ``blockages``: (ireg, jreg); ``porous``: (ireg, jreg, poros, c1, c2)
+    with already non-dimensional coefficients.  Velocities are non-dimensional
+    (already divided by uref, parse.f:1532).
+    """
+    ni, nj = nreg
+    if not (1 <= ni <= mgri and 1 <= nj <= mgrj):
+        raise ValueError("too many regions")
+    f = dict(order="F")
+    brd = np.zeros((mgri, mgrj, 4), dtype=np.int32, **f)
+    rtype = np.zeros((mgri, mgrj), dtype=np.int32, **f)
+    btp = np.zeros((mgri, mgrj, 4), dtype=np.int32, **f)
+    bcv = np.zeros((mgri, mgrj, 4, 4), dtype=np.float64, **f)
+    poros = np.zeros((mgri, mgrj), dtype=np.float64, **f)
+    porc1 = np.zeros((mgri, mgrj), dtype=np.float64, **f)
+    porc2 = np.zeros((mgri, mgrj), dtype=np.float64, **f)
+
+    for k, b in enumerate(i_borders):
+        brd[k + 1, 0, WEST - 1] = b
+    for k, b in enumerate(j_borders):
+        brd[0, k + 1, SOUTH - 1] = b
+    # parse.f:InitBCFlags 2257-2379
+    rtype[:ni, :nj] = RM_INTERN
+    btp[:ni, :nj, :] = BM_INTERN
+    btp[0, :nj, WEST - 1] = BM_WALL1
+    btp[ni - 1, :nj, EAST - 1] = BM_WALL1
+    btp[:ni, 0, SOUTH - 1] = BM_WALL1
+    btp[:ni, nj - 1, NORTH - 1] = BM_WALL1
+
+    for (ir, jr) in blockages:        # parse.f TBLOCKAGE
+        rtype[ir - 1, jr - 1] = RM_BLOCKG
+        btp[ir - 1, jr - 1, :] = BM_WALL1
+        if ir > 1:
+            btp[ir - 2, jr - 1, EAST - 1] = BM_WALL1
+        if ir < ni:
+            btp[ir, jr - 1, WEST - 1] = BM_WALL1
+        if jr > 1:
+            btp[ir - 1, jr - 2, NORTH - 1] = BM_WALL1
+        if jr < nj:
+            btp[ir - 1, jr, SOUTH - 1] = BM_WALL1
+    for (ir, jr, por, c1, c2) in porous:
+        rtype[ir - 1, jr - 1] = RM_POROUS
+        poros[ir - 1, jr - 1] = por
+        porc1[ir - 1, jr - 1] = c1
+        porc2[ir - 1, jr - 1] = c2
+    for (ir, jr, side, kind, tv, pr) in walls:
+        s = SIDES[side] if isinstance(side, str) else side
+        btp[ir - 1, jr - 1, s - 1] = BM_WALL1 if kind == "no_slip" else BM_WALL2
+        if s in (WEST, EAST):
+            bcv[ir - 1, jr - 1, s - 1, VAR_V - 1] = tv
+        else:
+            bcv[ir - 1, jr - 1, s - 1, VAR_U - 1] = tv
+        bcv[ir - 1, jr - 1, s - 1, VAR_P - 1] = pr
+    for (ir, jr, side, u, v, pr) in inlets:
+        s = SIDES[side] if isinstance(side, str) else side
+        btp[ir - 1, jr - 1, s - 1] = BM_INLET
+        bcv[ir - 1, jr - 1, s - 1, VAR_U - 1] = u
+        bcv[ir - 1, jr - 1, s - 1, VAR_V - 1] = v
+        bcv[ir - 1, jr - 1, s - 1, VAR_P - 1] = pr
+    for (ir, jr, side, kind, u, v, pr) in outlets:
+        s = SIDES[side] if isinstance(side, str) else side
+        btp[ir - 1, jr - 1, s - 1] = BM_OUTLT1 if kind == "fully_dev" else BM_OUTLT2
+        bcv[ir - 1, jr - 1, s - 1, VAR_U - 1] = u
+        bcv[ir - 1, jr - 1, s - 1, VAR_V - 1] = v
+        bcv[ir - 1, jr - 1, s - 1, VAR_P - 1] = pr
+
+    # bound_cond.f:165-193 border assignment
+    for jr in range(1, nj + 1):
+        brd[0, jr - 1, WEST - 1] = 1
+        brd[ni - 1, jr - 1, EAST - 1] = nx
+    for ir in range(1, ni + 1):
+        brd[ir - 1, 0, SOUTH - 1] = 1
+        brd[ir - 1, nj - 1, NORTH - 1] = ny
+    for jr in range(2, nj + 1):
+        for ir in range(2, ni + 1):
+            brd[ir - 1, jr - 1, WEST - 1] = brd[ir - 1, 0, WEST - 1]
+            brd[ir - 1, jr - 1, SOUTH - 1] = brd[0, jr - 1, SOUTH - 1]
+    for jr in range(1, nj + 1):
+        for ir in range(1, ni):
+            brd[ir - 1, jr - 1, EAST - 1] = brd[ir, 0, WEST - 1]
+    for jr in range(1, nj):
+        for ir in range(1, ni + 1):
+            brd[ir - 1, jr - 1, NORTH - 1] = brd[0, jr, SOUTH - 1]
+    # bound_cond.f:196-230 consistency of inlet/wall flags across interfaces
+    for jr in range(1, nj + 1):
+        for ir in range(1, ni):
+            if btp[ir - 1, jr - 1, EAST - 1] == BM_INLET:
+                btp[ir, jr - 1, WEST - 1] = BM_INLET
+            if btp[ir, jr - 1, WEST - 1] == BM_INLET:
+                btp[ir - 1, jr - 1, EAST - 1] = BM_INLET
+            if btp[ir - 1, jr - 1, EAST - 1] == BM_WALL1:
+                btp[ir, jr - 1, WEST - 1] = BM_WALL1
+            if btp[ir, jr - 1, WEST - 1] == BM_WALL1:
+                btp[ir - 1, jr - 1, EAST - 1] = BM_WALL1
+    for jr in range(1, nj):
+        for ir in range(1, ni + 1):
+            if btp[ir - 1, jr, SOUTH - 1] == BM_INLET:
+                btp[ir - 1, jr - 1, NORTH - 1] = BM_INLET
+            if btp[ir - 1, jr - 1, NORTH - 1] == BM_INLET:
+                btp[ir - 1, jr, SOUTH - 1] = BM_INLET
+            if btp[ir - 1, jr, SOUTH - 1] == BM_WALL1:
+                btp[ir - 1, jr - 1, NORTH - 1] = BM_WALL1
+            if btp[ir - 1, jr - 1, NORTH - 1] == BM_WALL1:
+                btp[ir - 1, jr, SOUTH - 1] = BM_WALL1
+    nreg_arr = np.array([ni, nj], dtype=np.int32)
+    return Regions(nx=nx, ny=ny, nreg=nreg_arr, nRegBrd=brd, nRegType=rtype, nMomBdTp=btp,
+                   dBCVal=bcv, dPRporos=poros, dPRporc1=porc1, dPRporc2=porc2, mgri=mgri, mgrj=mgrj)
+
+
+def cavity_regions(nx: int, ny: int, lid: float = 1.0) -> Regions:
+    """One region, no-slip walls, north wall moving at ``lid`` (lid-driven cavity)."""
+    return make_regions(nx, ny, walls=[(1, 1, "north", "no_slip", lid, 0.0)])
+
+
+@dataclass
+class Params:
+    """Non-dimensional run parameters (file_manip.f:ReadParam 120-159, parse.f:117-193)."""
+    dk: float
+    re: float
+    fr: float = 1.0 / 9.81
+    mqiter: int = 20
+    qtol: float = 1e-4
+    nPpeSolver: int = 1
+    msorit: int = 2000
+    sortol: float = 1e-8
+    sorrel: float = 1.0
+    cartesian: bool = True
+
+    @staticmethod
+    def from_physical(dt: float, rho: float, mu: float, dlref: float = 1.0, uref: float = 1.0,
+                      **kw) -> "Params":
+        dk = dt * uref / dlref                     # file_manip.f:134
+        dkinvisc = mu / rho                        # file_manip.f:145
+        re = dlref * uref / dkinvisc               # file_manip.f:157
+        fr = uref ** 2.0 / (dlref * 9.81)          # file_manip.f:159
+        return Params(dk=dk, re=re, fr=fr, **kw)
+
+
+PPE_SOLVERS = {"sor": 1, "lsor": 2, "rb_lsor": 3, "par_rb_lsor": 4, "rb_sor": 5, "par_rb_sor": 6}
