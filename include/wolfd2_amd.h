@@ -1,0 +1,166 @@
+/*
+ * wolfd2_amd.h -- C ABI of the MI355X (gfx950) implementation of the wolfd2
+ * per-timestep hot path.
+ *
+ * Two layers:
+ *
+ * 1. Drop-in entry points, one per reference subroutine on the hot path, with
+ *    the reference's argument lists (all arrays are the caller's host arrays,
+ *    dimensioned (0:mnx,0:mny) column-major) plus the caller's compile-time
+ *    extents made explicit: ld = mnx+1 (leading dimension of every field) and
+ *    mgri, mgrj (extents of the region tables).  Each call uploads its inputs,
+ *    runs on the GPU and downloads its outputs before returning; the library
+ *    keeps device buffers (sized by nx, ny on first use) but no host pointer.
+ *    The Fortran binding in wolfd2_amd/fortran/wd2_shim.f exports these under
+ *    the reference's own names (nauxmomentum_, ppe_, project_, ...).
+ *
+ * 2. A device-resident context (wd2_ctx_*) that keeps the fields in HBM
+ *    across time steps and runs whole steps of main.f:690-972 on the GPU.
+ *
+ * Status codes: WD2_OK, or a negative WD2_ERR_*; wd2_last_error() describes
+ * the last failure.  A bad boundary or solver flag, where the reference
+ * prints a message and STOPs, returns WD2_ERR_FLAG.
+ *
+ * Calls are synchronous and not reentrant (the reference keeps static
+ * scratch too).  FP64 results equal the reference's bit for bit.
+ */
+#ifndef WOLFD2_AMD_H
+#define WOLFD2_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WD2_OK 0
+#define WD2_ERR_FLAG (-1)
+#define WD2_ERR_ARG (-2)
+#define WD2_ERR_HIP (-3)
+#define WD2_ERR_UNSUPPORTED (-4)
+
+const char *wd2_last_error(void);
+/* Number of times the parallel AltTridLU fell back to its sequential path. */
+int wd2_trid_fallbacks(void);
+
+/* momentum.f:nAuxMomentum 33-48.  *nQLiter receives the function value: the
+ * number of q-l iterations, or -1 when not converged in mqiter iterations. */
+int wd2_naux_momentum(int nx, int ny, int ld, int mgri, int mgrj, int mqiter,
+                      const int *nReg, const int *nRegBrd, const int *nRegType, const int *nMomBdTp,
+                      double dk, double re, double fr, double qtol,
+                      const double *dPRporos, const double *dPRporc1, const double *dPRporc2,
+                      const double *dBCVal,
+                      const double *ran, const double *rbn, const double *rgn,
+                      const double *rac, const double *rbc, const double *rgc,
+                      const double *dju, const double *djv,
+                      const double *xec, const double *yec, const double *xzn, const double *yzn,
+                      const double *xen, const double *yen, const double *xzc, const double *yzc,
+                      const double *xeu, const double *yeu, const double *xzu, const double *yzu,
+                      const double *xev, const double *yev, const double *xzv, const double *yzv,
+                      const double *d, const double *dn,
+                      const double *un, const double *vn, double *us, double *vs, int *nQLiter);
+
+/* momentum.f:XMomentum 199-514 / YMomentum 520-838 (one ADI pass; the
+ * increment is written on the reference's output range only). */
+int wd2_x_momentum(int nx, int ny, int ld, int mgri, int mgrj,
+                   const int *nReg, const int *nRegBrd, const int *nRegType, const int *nMomBdTp,
+                   double dk, double re,
+                   const double *dPRporos, const double *dPRporc1, const double *dPRporc2,
+                   const double *rbn, const double *rgn, const double *rac, const double *rbc, const double *dju,
+                   const double *xec, const double *yec, const double *xzn, const double *yzn,
+                   const double *xeu, const double *yeu, const double *xzu, const double *yzu,
+                   const double *us, const double *vs, const double *un, const double *vn, double *dus);
+int wd2_y_momentum(int nx, int ny, int ld, int mgri, int mgrj,
+                   const int *nReg, const int *nRegBrd, const int *nRegType, const int *nMomBdTp,
+                   double dk, double re, double fr,
+                   const double *dPRporos, const double *dPRporc1, const double *dPRporc2,
+                   const double *ran, const double *rbn, const double *rbc, const double *rgc, const double *djv,
+                   const double *xen, const double *yen, const double *xzc, const double *yzc,
+                   const double *xev, const double *yev, const double *xzv, const double *yzv,
+                   const double *d, const double *dn,
+                   const double *us, const double *vs, const double *un, const double *vn, double *dvs);
+
+/* momentum.f:AltTridLU 1307-1339: a is a(3,n) (sub, diagonal, super per row),
+ * b the right-hand side, overwritten by the solution.  a is not modified. */
+int wd2_alt_trid_lu(int n, const double *a, double *b);
+
+/* pressure.f:Ppe 30-38.  lCartesGrid is a Fortran logical (non-zero = true).
+ * nPpeSolver 1 sor, 2 lsor, 3 rb_lsor, 4 par_rb_lsor, 5 rb_sor, 6 par_rb_sor.
+ * *nSorConv receives the iteration count (msorit when not converged, after the
+ * reference's warning). */
+int wd2_ppe(int nx, int ny, int ld, int mgri, int mgrj,
+            const int *nReg, const int *nRegBrd, const int *nRegType,
+            int lCartesGrid, int nPpeSolver, int msorit, int *nSorConv,
+            double dk, double sortol, double sorrel,
+            const double *rau, const double *rbu, const double *rbv, const double *rgv,
+            const double *xeu, const double *yeu, const double *xzv, const double *yzv,
+            const double *u, const double *v, double *p);
+
+/* pressure.f:Divergence 265-323 (nloc 1: pressure-cell centres, 2: nodes). */
+int wd2_divergence(int nx, int ny, int ld, int nloc,
+                   const double *xet, const double *yet, const double *xzi, const double *yzi,
+                   const double *u, const double *v, double *div);
+
+/* utility.f:Project 253-259 */
+int wd2_project(int nx, int ny, int ld, int mgri, int mgrj,
+                const int *nReg, const int *nRegBrd, const int *nRegType, const int *nMomBdTp,
+                double dk, const double *dju, const double *djv,
+                const double *yeu, const double *xzv, const double *yzu, const double *xev,
+                const double *p, double *u, double *v);
+
+/* bound_cond.f:VelBoundCond 511-513, PresBoundCond 853-856, VelOutflowBCs 1656-1659 */
+int wd2_vel_bound_cond(int nx, int ny, int ld, int mgri, int mgrj,
+                       const int *nReg, const int *nRegBrd, const int *nMomBdTp, const double *dBCVal,
+                       double *u, double *v);
+int wd2_pres_bound_cond(int nx, int ny, int ld, int mgri, int mgrj,
+                        const int *nReg, const int *nRegBrd, const int *nRegType, const int *nMomBdTp,
+                        const double *dBCVal, double *p);
+int wd2_vel_outflow_bcs(int nx, int ny, int ld, int mgri, int mgrj,
+                        const int *nReg, const int *nRegBrd, const int *nMomBdTp, const double *dBCVal,
+                        double *u, double *v);
+
+/* utility.f:DMaxNorm 479-507, DiffMaxNorm 446-473 (result in *out). */
+int wd2_dmax_norm(int nx, int ny, int ld, const double *u, double *out);
+int wd2_diff_max_norm(int nx, int ny, int ld, const double *un, const double *u, double *out);
+
+/* ----------------------------------------------------------------------
+ * Device-resident context.
+ */
+typedef struct wd2_ctx wd2_ctx;
+
+/* Run parameters (non-dimensional, as file_manip.f:ReadParam leaves them). */
+typedef struct {
+    int lCartesGrid;   /* non-zero: Cartesian grid (RhsPpe once per solve) */
+    int mqiter;        /* max_ql_iter */
+    int nPpeSolver;    /* 1..6 as for wd2_ppe */
+    int msorit;        /* max_sor_iter */
+    double qtol, sortol, sorrel;
+    double dk, re, fr;
+} wd2_params;
+
+/* metrics: the 30 arrays in grid.f:Grid argument order
+ * (rau, rbu, rbv, rgv, ran, rbn, rgn, rac, rbc, rgc, dju, djv, djc, djn,
+ *  xen, yen, xzn, yzn, xec, yec, xzc, yzc, xeu, yeu, xzv, yzv, xzu, yzu, xev, yev),
+ * each a host (0:mnx,0:mny) array with leading dimension ld. */
+int wd2_ctx_create(wd2_ctx **ctx, int nx, int ny, int ld, int mgri, int mgrj,
+                   const int *nReg, const int *nRegBrd, const int *nRegType, const int *nMomBdTp,
+                   const double *dBCVal, const double *dPRporos, const double *dPRporc1,
+                   const double *dPRporc2, const double *const *metrics);
+int wd2_ctx_destroy(wd2_ctx *ctx);
+/* Upload u, v, p (host, leading dimension ld) to the context. */
+int wd2_ctx_set_fields(wd2_ctx *ctx, int ld, const double *u, const double *v, const double *p);
+int wd2_ctx_get_fields(wd2_ctx *ctx, int ld, double *u, double *v, double *p);
+/* main.f:606-641: VelBoundCond, Ppe, PresBoundCond, Project, VelBoundCond. */
+int wd2_ctx_cold_start(wd2_ctx *ctx, const wd2_params *prm, int *nSorConv);
+/* nsteps of main.f:690-972 (cold flow, one momentum/energy iteration).
+ * Per step k (if the pointers are non-null): ql_iter[k], ppe_iter[k], and
+ * dif[3k..3k+2] = DiffMaxNorm of p, u, v against the previous step. */
+int wd2_ctx_step(wd2_ctx *ctx, const wd2_params *prm, int nsteps, int *ql_iter, int *ppe_iter, double *dif);
+/* Device pointers of the context's u, v, p fields and their leading
+ * dimension (the device layout pads rows to a multiple of 16 elements). */
+int wd2_ctx_device_fields(wd2_ctx *ctx, double **u, double **v, double **p, int *ld);
+/* The HIP stream the context runs on. */
+void *wd2_ctx_stream(wd2_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

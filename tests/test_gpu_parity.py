@@ -1,0 +1,148 @@
+"""Parity of the HIP path with the reference (through the C ABI).
+
+FP64 results must be bit-identical to the reference's: every comparison here
+is np.array_equal (tolerance 0) against the golden fixtures the reference
+wrote, or against the oracle (the C restatement) on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+import helpers
+from wolfd2_amd import api, case
+
+pytestmark = pytest.mark.gpu
+
+
+def _kernel_case(fx):
+    g = case.compute_grid(fx["x"], fx["y"])
+    return int(fx["nx"]), int(fx["ny"]), g, helpers.regions_from(fx)
+
+
+@pytest.mark.parametrize("name", helpers.kernel_fixture_names())
+def test_kernels_bitwise_vs_reference_goldens(name):
+    fx = helpers.load(name)
+    nx, ny, g, reg = _kernel_case(fx)
+    ld, nc = g.ld, g.ncol
+    E = lambda a: helpers.embed(a, ld, nc)  # noqa: E731
+    S = np.s_[: ny + 2, : nx + 2]
+    print(f"case {name} nx={nx} ny={ny}", flush=True)
+
+    u, v = E(fx["vbc_u_in"]), E(fx["vbc_v_in"])
+    api.vel_bound_cond(reg, u, v)
+    assert np.array_equal(u[S], fx["vbc_u_out"]) and np.array_equal(v[S], fx["vbc_v_out"])
+    u, v = E(fx["vob_u_in"]), E(fx["vob_v_in"])
+    api.vel_outflow_bcs(reg, u, v)
+    assert np.array_equal(u[S], fx["vob_u_out"]) and np.array_equal(v[S], fx["vob_v_out"])
+    p = E(fx["pbc_p_in"])
+    api.pres_bound_cond(reg, p)
+    assert np.array_equal(p[S], fx["pbc_p_out"])
+    u, v, p = E(fx["prj_u_in"]), E(fx["prj_v_in"]), E(fx["prj_p"])
+    api.project(reg, g, float(fx["prj_dk"]), p, u, v)
+    assert np.array_equal(u[S], fx["prj_u_out"]) and np.array_equal(v[S], fx["prj_v_out"])
+    div = np.zeros_like(u)
+    api.divergence(g, 1, E(fx["div_u"]), E(fx["div_v"]), div)
+    assert np.array_equal(div[S], fx["div_out"])
+    us, vs, un, vn = (E(fx[f"mom_{k}"]) for k in ("us", "vs", "un", "vn"))
+    prm = case.Params(dk=0.01, re=100.0)
+    dus, dvs, d0 = np.zeros_like(us), np.zeros_like(us), np.zeros_like(us)
+    api.x_momentum(reg, g, prm, us, vs, un, vn, dus)
+    api.y_momentum(reg, g, prm, d0, d0.copy(), us, vs, un, vn, dvs)
+    assert np.array_equal(dus[S], fx["mom_dus"]), "XMomentum"
+    assert np.array_equal(dvs[S], fx["mom_dvs"]), "YMomentum"
+    for mq, qt in ((1, 1e-4), (30, 1e-10)):
+        us2, vs2 = np.zeros_like(us), np.zeros_like(us)
+        n = api.naux_momentum(reg, g, prm, d0, d0.copy(), un, vn, us2, vs2, mqiter=mq, qtol=qt)
+        assert n == int(fx[f"naux{mq}_n"])
+        assert np.array_equal(us2[S], fx[f"naux{mq}_us"]) and np.array_equal(vs2[S], fx[f"naux{mq}_vs"])
+    assert api.dmax_norm(nx, ny, dus) == float(fx["dmax"])
+    assert api.diff_max_norm(nx, ny, us, vs) == float(fx["diffmax"])
+    x = api.alt_trid_lu(fx["trid_a"], fx["trid_b"])
+    assert np.array_equal(x, fx["trid_x"])
+
+
+@pytest.mark.parametrize("name", helpers.kernel_fixture_names())
+@pytest.mark.parametrize("solver", [1, 2, 3, 4, 5, 6])
+def test_ppe_solvers_bitwise_vs_reference_goldens(name, solver):
+    fx = helpers.load(name)
+    nx, ny, g, reg = _kernel_case(fx)
+    E = lambda a: helpers.embed(a, g.ld, g.ncol)  # noqa: E731
+    S = np.s_[: ny + 2, : nx + 2]
+    for cart in (1, 0):
+        key = f"ppe{solver}c{cart}"
+        print(f"case {name} {key}", flush=True)
+        p = E(fx[f"{key}_p_in"])
+        prm = case.Params(dk=0.01, re=100.0, nPpeSolver=solver, msorit=25, sortol=1e-8, sorrel=1.3,
+                          cartesian=bool(cart))
+        n = api.ppe(reg, g, prm, E(fx[f"{key}_u"]), E(fx[f"{key}_v"]), p)
+        assert n == int(fx[f"{key}_n"])
+        assert np.array_equal(p[S], fx[f"{key}_p_out"]), key
+
+
+def _chain(n, rng, weak=False):
+    a = np.zeros((n, 3))
+    a[:, 0] = rng.uniform(-1, 0, n)
+    a[:, 2] = rng.uniform(-1, 0, n)
+    a[:, 1] = (np.abs(a[:, 0]) + np.abs(a[:, 2])) * (1.0001 if weak else 1.3) + (0 if weak else 0.05)
+    b = rng.standard_normal(n)
+    return a, b
+
+
+@pytest.mark.parametrize("n,weak", [(2, False), (3, False), (17, False), (4095, False), (4096, False),
+                                    (4097, False), (100_003, False), (1_049_600, False), (20_000, True)])
+def test_alt_trid_lu_bitwise_vs_oracle(oracle, n, weak):
+    rng = np.random.default_rng(n)
+    a, b = _chain(n, rng, weak)
+    print(f"chain n={n} weak={weak}", flush=True)
+    x_gpu = api.alt_trid_lu(a, b)
+    _, x_ref = oracle.alt_trid_lu(a, b)
+    assert np.array_equal(x_gpu, x_ref)
+
+
+@pytest.mark.parametrize("name", helpers.run_fixture_names())
+def test_whole_runs_bitwise_vs_reference_program(name):
+    fx = helpers.load(name)
+    g, reg, prm = helpers.run_case(fx)
+    nx, ny = g.nx, g.ny
+    print(f"run {name} nx={nx} ny={ny} solver={prm.nPpeSolver} steps={int(fx['nts'])}", flush=True)
+    sol = api.Solver(g, reg, prm)
+    z = np.zeros((g.ncol, g.ld))
+    sol.set_fields(z, z, z)
+    sol.cold_start()
+    table = sol.step(int(fx["nts"]))
+    u, v, p = sol.get_fields()
+    sol.close()
+    ref = fx["table"]
+    n = min(len(ref), len(table))
+    assert np.array_equal(table[:n, 0], ref[:n, 2]), "QlIt differs"
+    assert np.array_equal(table[:n, 1], ref[:n, 3]), "PpeIt differs"
+    fin = np.isfinite(ref[:n, 4:7])
+    np.testing.assert_allclose(table[:n, 2:5][fin], ref[:n, 4:7][fin], rtol=6e-6, atol=1e-300)
+    S = np.s_[: ny + 2, : nx + 2]
+    if np.all(np.isfinite(fx["u"])):
+        assert np.array_equal(u[S], fx["u"]), "u"
+        assert np.array_equal(v[S], fx["v"]), "v"
+        assert np.array_equal(p[S], fx["p"]), "p"
+
+
+@pytest.mark.parametrize("solver", [5, 1])
+def test_full_size_cavity_steps_bitwise_vs_oracle(oracle, solver):
+    """1024^2 cells (the benchmark grid): a cold start and two time steps with
+    a capped Poisson solve, GPU against the oracle, bit for bit."""
+    nx = ny = 1025
+    x, y = case.uniform_nodes(nx, ny)
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(nx, ny)
+    prm = case.Params.from_physical(2e-4, 1.0, 1e-3, nPpeSolver=solver, msorit=12 if solver == 5 else 4)
+    print(f"full-size cavity solver={solver}", flush=True)
+    sol = api.Solver(g, reg, prm)
+    z = np.zeros((g.ncol, g.ld))
+    sol.set_fields(z, z, z)
+    sol.cold_start()
+    tg = sol.step(2)
+    u1, v1, p1 = sol.get_fields()
+    sol.close()
+    u, v, p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
+    oracle.cold_start(reg, g, prm, u, v, p)
+    to = oracle.run(reg, g, prm, u, v, p, 2)
+    assert np.array_equal(tg[:, :2], to[:, :2])
+    assert np.array_equal(u1, u) and np.array_equal(v1, v) and np.array_equal(p1, p)

@@ -1,0 +1,188 @@
+"""The reference's hot-path call interface, backed by the HIP library.
+
+Each function mirrors one subroutine of the reference with its argument
+meaning (momentum.f:nAuxMomentum/XMomentum/YMomentum/AltTridLU,
+pressure.f:Ppe/Divergence, utility.f:Project/DMaxNorm/DiffMaxNorm,
+bound_cond.f:VelBoundCond/PresBoundCond/VelOutflowBCs): fields are numpy
+float64 arrays of shape (mny+1, mnx+1) indexed [j, i] (the reference's
+(0:mnx,0:mny) column-major arrays), updated in place; region tables and
+metrics come from :mod:`wolfd2_amd.case`.  Every call runs on the GPU through
+the C ABI of include/wolfd2_amd.h; a bad boundary or solver flag raises
+:class:`Wd2Error` where the reference would STOP.
+
+:class:`Solver` is the device-resident driver: it keeps u, v, p in HBM and
+advances whole time steps (main.f:690-972).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import case
+from ._lib import Params, Wd2Error, check, dp, fp, ip, lib
+
+__all__ = ["vel_bound_cond", "pres_bound_cond", "vel_outflow_bcs", "project", "divergence", "ppe",
+           "x_momentum", "y_momentum", "naux_momentum", "alt_trid_lu", "dmax_norm", "diff_max_norm",
+           "Solver", "Wd2Error", "make_params"]
+
+
+def _ld(a: np.ndarray) -> int:
+    return a.shape[1]
+
+
+def vel_bound_cond(reg: case.Regions, u: np.ndarray, v: np.ndarray) -> None:
+    check(lib().wd2_vel_bound_cond(reg.nx, reg.ny, _ld(u), reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                                   ip(reg.nMomBdTp), fp(reg.dBCVal), dp(u), dp(v)))
+
+
+def vel_outflow_bcs(reg: case.Regions, u: np.ndarray, v: np.ndarray) -> None:
+    check(lib().wd2_vel_outflow_bcs(reg.nx, reg.ny, _ld(u), reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                                    ip(reg.nMomBdTp), fp(reg.dBCVal), dp(u), dp(v)))
+
+
+def pres_bound_cond(reg: case.Regions, p: np.ndarray) -> None:
+    check(lib().wd2_pres_bound_cond(reg.nx, reg.ny, _ld(p), reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                                    ip(reg.nRegType), ip(reg.nMomBdTp), fp(reg.dBCVal), dp(p)))
+
+
+def project(reg: case.Regions, g: case.Grid, dk: float, p, u, v) -> None:
+    check(lib().wd2_project(reg.nx, reg.ny, g.ld, reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                            ip(reg.nRegType), ip(reg.nMomBdTp), C.c_double(dk), dp(g.dju), dp(g.djv), dp(g.yeu),
+                            dp(g.xzv), dp(g.yzu), dp(g.xev), dp(p), dp(u), dp(v)))
+
+
+def divergence(g: case.Grid, nloc: int, u, v, div) -> None:
+    if nloc == 1:
+        ms = (g.xeu, g.yeu, g.xzv, g.yzv)
+    else:
+        ms = (g.xev, g.yev, g.xzu, g.yzu)
+    check(lib().wd2_divergence(g.nx, g.ny, g.ld, nloc, *[dp(m) for m in ms], dp(u), dp(v), dp(div)))
+
+
+def ppe(reg: case.Regions, g: case.Grid, prm: case.Params, u, v, p, solver=None, msorit=None, sortol=None,
+        sorrel=None) -> int:
+    """pressure.f:Ppe; returns nSorConv."""
+    n = C.c_int(0)
+    check(lib().wd2_ppe(reg.nx, reg.ny, g.ld, reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd), ip(reg.nRegType),
+                        int(prm.cartesian), prm.nPpeSolver if solver is None else solver,
+                        prm.msorit if msorit is None else msorit, C.byref(n), C.c_double(prm.dk),
+                        C.c_double(prm.sortol if sortol is None else sortol),
+                        C.c_double(prm.sorrel if sorrel is None else sorrel),
+                        dp(g.rau), dp(g.rbu), dp(g.rbv), dp(g.rgv), dp(g.xeu), dp(g.yeu), dp(g.xzv), dp(g.yzv),
+                        dp(u), dp(v), dp(p)))
+    return n.value
+
+
+def x_momentum(reg, g, prm, us, vs, un, vn, dus) -> None:
+    check(lib().wd2_x_momentum(reg.nx, reg.ny, g.ld, reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                               ip(reg.nRegType), ip(reg.nMomBdTp), C.c_double(prm.dk), C.c_double(prm.re),
+                               fp(reg.dPRporos), fp(reg.dPRporc1), fp(reg.dPRporc2),
+                               *[dp(g.metrics[k]) for k in ("rbn", "rgn", "rac", "rbc", "dju", "xec", "yec", "xzn",
+                                                            "yzn", "xeu", "yeu", "xzu", "yzu")],
+                               dp(us), dp(vs), dp(un), dp(vn), dp(dus)))
+
+
+def y_momentum(reg, g, prm, d, dn, us, vs, un, vn, dvs) -> None:
+    check(lib().wd2_y_momentum(reg.nx, reg.ny, g.ld, reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                               ip(reg.nRegType), ip(reg.nMomBdTp), C.c_double(prm.dk), C.c_double(prm.re),
+                               C.c_double(prm.fr), fp(reg.dPRporos), fp(reg.dPRporc1), fp(reg.dPRporc2),
+                               *[dp(g.metrics[k]) for k in ("ran", "rbn", "rbc", "rgc", "djv", "xen", "yen", "xzc",
+                                                            "yzc", "xev", "yev", "xzv", "yzv")],
+                               dp(d), dp(dn), dp(us), dp(vs), dp(un), dp(vn), dp(dvs)))
+
+
+def naux_momentum(reg, g, prm, d, dn, un, vn, us, vs, mqiter=None, qtol=None) -> int:
+    """momentum.f:nAuxMomentum; returns the q-l iteration count or -1."""
+    n = C.c_int(0)
+    check(lib().wd2_naux_momentum(
+        reg.nx, reg.ny, g.ld, reg.mgri, reg.mgrj, prm.mqiter if mqiter is None else mqiter,
+        ip(reg.nreg), ip(reg.nRegBrd), ip(reg.nRegType), ip(reg.nMomBdTp),
+        C.c_double(prm.dk), C.c_double(prm.re), C.c_double(prm.fr),
+        C.c_double(prm.qtol if qtol is None else qtol),
+        fp(reg.dPRporos), fp(reg.dPRporc1), fp(reg.dPRporc2), fp(reg.dBCVal),
+        *[dp(g.metrics[k]) for k in ("ran", "rbn", "rgn", "rac", "rbc", "rgc", "dju", "djv", "xec", "yec", "xzn",
+                                     "yzn", "xen", "yen", "xzc", "yzc", "xeu", "yeu", "xzu", "yzu", "xev", "yev",
+                                     "xzv", "yzv")],
+        dp(d), dp(dn), dp(un), dp(vn), dp(us), dp(vs), C.byref(n)))
+    return n.value
+
+
+def alt_trid_lu(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """momentum.f:AltTridLU on an (n, 3) row array (sub, diag, super); returns x."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    x = np.array(b, dtype=np.float64, copy=True)
+    check(lib().wd2_alt_trid_lu(x.size, dp(a), dp(x)))
+    return x
+
+
+def dmax_norm(nx: int, ny: int, u: np.ndarray) -> float:
+    out = C.c_double(0)
+    check(lib().wd2_dmax_norm(nx, ny, _ld(u), dp(u), C.byref(out)))
+    return out.value
+
+
+def diff_max_norm(nx: int, ny: int, un: np.ndarray, u: np.ndarray) -> float:
+    out = C.c_double(0)
+    check(lib().wd2_diff_max_norm(nx, ny, _ld(u), dp(un), dp(u), C.byref(out)))
+    return out.value
+
+
+def make_params(prm: case.Params) -> Params:
+    return Params(int(prm.cartesian), prm.mqiter, prm.nPpeSolver, prm.msorit, prm.qtol, prm.sortol, prm.sorrel,
+                  prm.dk, prm.re, prm.fr)
+
+
+class Solver:
+    """Device-resident driver of main.f:600-972 for one case (wd2_ctx_*)."""
+
+    def __init__(self, g: case.Grid, reg: case.Regions, prm: case.Params):
+        self.g, self.reg, self.prm = g, reg, prm
+        self.ctx = C.c_void_p()
+        mets = (C.POINTER(C.c_double) * 30)(*[dp(g.metrics[k]) for k in case.METRIC_NAMES])
+        self._mets = mets
+        check(lib().wd2_ctx_create(C.byref(self.ctx), g.nx, g.ny, g.ld, reg.mgri, reg.mgrj, ip(reg.nreg),
+                                   ip(reg.nRegBrd), ip(reg.nRegType), ip(reg.nMomBdTp), fp(reg.dBCVal),
+                                   fp(reg.dPRporos), fp(reg.dPRporc1), fp(reg.dPRporc2), mets))
+        self.params = make_params(prm)
+
+    def close(self):
+        if self.ctx:
+            lib().wd2_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, prm: case.Params):
+        self.prm = prm
+        self.params = make_params(prm)
+
+    def set_fields(self, u, v, p):
+        check(lib().wd2_ctx_set_fields(self.ctx, self.g.ld, dp(u), dp(v), dp(p)))
+
+    def get_fields(self):
+        g = self.g
+        u, v, p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
+        check(lib().wd2_ctx_get_fields(self.ctx, g.ld, dp(u), dp(v), dp(p)))
+        return u, v, p
+
+    def cold_start(self) -> int:
+        n = C.c_int(0)
+        check(lib().wd2_ctx_cold_start(self.ctx, C.byref(self.params), C.byref(n)))
+        return n.value
+
+    def step(self, nsteps: int = 1) -> np.ndarray:
+        """Advance nsteps; returns rows (ql_iter, ppe_iter, |p1-pn|, |u1-un|, |v1-vn|)."""
+        ql = (C.c_int * nsteps)()
+        pi = (C.c_int * nsteps)()
+        dif = (C.c_double * (3 * nsteps))()
+        check(lib().wd2_ctx_step(self.ctx, C.byref(self.params), nsteps, ql, pi, dif))
+        out = np.zeros((nsteps, 5))
+        out[:, 0] = np.frombuffer(ql, dtype=np.int32)
+        out[:, 1] = np.frombuffer(pi, dtype=np.int32)
+        out[:, 2:] = np.frombuffer(dif, dtype=np.float64).reshape(nsteps, 3)
+        return out

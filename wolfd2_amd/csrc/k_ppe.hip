@@ -1,0 +1,317 @@
+// k_ppe.hip -- the pressure Poisson relaxations of pressure.f (Ppe 30-249).
+//
+//   SorRB / SorRBP (pressure.f:457-656): two colour kernels per sweep.  Points of
+//     one colour only read points of the other, so every point's arithmetic is
+//     the reference's regardless of order; the unrelaxed |s| is max-reduced.
+//   Sor (pressure.f:384-450): lexicographic Gauss-Seidel.  A point needs its
+//     west and south neighbours of this sweep and its east and north neighbours
+//     of the last one, so 64x64 tiles run in wavefront order (each workgroup
+//     takes the next tile in row-major order from an atomic ticket and waits for
+//     its west and south tiles), and inside a tile the points run along
+//     anti-diagonals through LDS.
+//   Slor / SlorRB / SlorRBP (pressure.f:673-1138): each j-line is an (nx-1)-row
+//     AltTridLU; the lines of one colour are solved as one batch of chains.
+//
+// Sweep control stays on the device: the last workgroup of a sweep folds the
+// maxima, counts the sweep and raises sor_done when m > 1 and dif < sortol
+// (pressure.f:443) or m reaches msorit; every later kernel of the solve returns
+// at entry, so the host only has to look at the flag every few dozen sweeps.
+#include "wd2_dev.hpp"
+#include "wd2_launch.hpp"
+
+namespace wd2 {
+
+constexpr int RB_T = 256;
+
+__device__ inline void sor_finish(Scalars *sc, double dif, double sortol, int msorit) {
+    const int m = sc->sor_m + 1;
+    sc->sor_m = m;
+    sc->sor_dif = dif;
+    if (m > 1 && dif < sortol) {
+        sc->sor_done = 1;
+        sc->sor_nconv = m;
+    } else if (m >= msorit) {
+        sc->sor_done = 1;
+    }
+}
+
+__device__ inline bool last_arrival(unsigned int *counter, unsigned int nblocks, int tid) {
+    __shared__ bool amlast;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned int t = atomicAdd(counter, 1u);
+        amlast = (t == nblocks - 1);
+        if (amlast) *counter = 0;
+    }
+    __syncthreads();
+    return amlast;
+}
+
+// One point update of Sor / SorRB / SorRBP (pressure.f:425-431); returns |s|.
+template <bool COEF>
+__device__ inline double sor_point(int ld, int i, int j, const double *__restrict__ rau,
+                                   const double *__restrict__ rgv, const PpeCoef &cf, const double *__restrict__ b,
+                                   double sorrel, double *p) {
+    const size_t k = WD2_IX(i, j);
+    double a1, a2, a3, a4, a5;
+    if (COEF) {
+        a1 = cf.a1[k]; a2 = cf.a2[k]; a3 = cf.a3[k]; a4 = cf.a4[k]; a5 = cf.a5[k];
+    } else {
+        a1 = rgv[WD2_IX(i, j - 1)];
+        a2 = rau[WD2_IX(i - 1, j)];
+        a4 = rau[k];
+        a5 = rgv[k];
+        a3 = -a4 - a2 - a5 - a1;
+    }
+    double sum = b[k] - a1 * p[WD2_IX(i, j - 1)] - a2 * p[WD2_IX(i - 1, j)] - a4 * p[WD2_IX(i + 1, j)] -
+                 a5 * p[WD2_IX(i, j + 1)];
+    const double pij = p[k];
+    sum = sum / a3 - pij;
+    p[k] = pij + sorrel * sum;
+    return fabs(sum);
+}
+
+template <int COLOR, bool COEF>
+__global__ __launch_bounds__(RB_T) void k_sor_rb(int nx, int ny, int ld, const double *__restrict__ rau,
+                                                 const double *__restrict__ rgv, PpeCoef cf,
+                                                 const double *__restrict__ b, double sorrel, double sortol,
+                                                 int msorit, double *p, double *__restrict__ partials, Scalars *sc) {
+    if (sc->sor_done) return;
+    __shared__ double sh[RB_T / 64];
+    const int tid = threadIdx.x;
+    double mx = 0.0;
+    for (int j = 2 + blockIdx.y; j <= ny; j += gridDim.y) {
+        const int i0 = COLOR == 0 ? 2 + (j % 2) : 2 + ((j + 1) % 2);
+        const int i = i0 + 2 * (blockIdx.x * RB_T + tid);
+        if (i <= nx) mx = dmaxf(mx, sor_point<COEF>(ld, i, j, rau, rgv, cf, b, sorrel, p));
+    }
+    mx = block_max<RB_T>(mx, sh, tid);
+    const unsigned int nb = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
+    if (tid == 0) partials[COLOR * nb + bid] = mx;
+    if (COLOR == 1 && last_arrival(&sc->counter[7], nb, tid)) {
+        double r = 0.0;
+        for (unsigned int k = tid; k < 2 * nb; k += RB_T) r = dmaxf(r, ld_coherent(&partials[k]));
+        r = block_max<RB_T>(r, sh, tid);
+        if (tid == 0) sor_finish(sc, r, sortol, msorit);
+    }
+}
+
+void launch_sor_rb_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
+                         double sorrel, double sortol, int msorit, double *p, double *partials, Scalars *sc,
+                         hipStream_t s) {
+    const int per_row = (nx / 2 + RB_T) / RB_T;
+    int rows = ny - 1;
+    while ((long)per_row * rows * 2 > RED_MAX_BLOCKS) rows = (rows + 1) / 2;
+    const dim3 g(per_row, rows);
+    PpeCoef cf = coef ? *coef : PpeCoef{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (coef) {
+        hipLaunchKernelGGL((k_sor_rb<0, true>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel, sortol,
+                           msorit, p, partials, sc);
+        hipLaunchKernelGGL((k_sor_rb<1, true>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel, sortol,
+                           msorit, p, partials, sc);
+    } else {
+        hipLaunchKernelGGL((k_sor_rb<0, false>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel,
+                           sortol, msorit, p, partials, sc);
+        hipLaunchKernelGGL((k_sor_rb<1, false>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel,
+                           sortol, msorit, p, partials, sc);
+    }
+}
+
+__global__ void k_sor_reset(Scalars *sc) {
+    sc->sor_done = 0;
+    sc->sor_nconv = 0;
+    sc->sor_m = 0;
+    sc->sor_dif = 0.0;
+    sc->val[6] = 0.0;
+}
+void launch_sor_reset(Scalars *sc, hipStream_t s) { hipLaunchKernelGGL(k_sor_reset, dim3(1), dim3(1), 0, s, sc); }
+
+// ------------------------------------------------------- lexicographic Sor
+constexpr int LX_T = 64;          // one wavefront per tile
+constexpr int LX_TI = 64, LX_TJ = 64;
+
+template <bool COEF>
+__global__ __launch_bounds__(LX_T) void k_sor_lex(int nx, int ny, int ld, const double *__restrict__ rau,
+                                                  const double *__restrict__ rgv, PpeCoef cf,
+                                                  const double *__restrict__ b, double sorrel, double sortol,
+                                                  int msorit, double *p, int *tile_flags, int sweep,
+                                                  double *__restrict__ partials, Scalars *sc) {
+    if (sc->sor_done) return;
+    __shared__ int ticket;
+    __shared__ double sh[1];
+    const int t = threadIdx.x;
+    const int ntx = (nx - 1 + LX_TI - 1) / LX_TI, nty = (ny - 1 + LX_TJ - 1) / LX_TJ;
+    const unsigned int ntiles = ntx * nty;
+    if (t == 0) ticket = (int)atomicAdd(&sc->counter[5], 1u);
+    __syncthreads();
+    const int tile = ticket;
+    const int TI = tile % ntx, TJ = tile / ntx;
+    // wait for the west and south tiles of this sweep
+    if (t == 0) {
+        if (TI > 0)
+            while (__hip_atomic_load(&tile_flags[tile - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < sweep)
+                __builtin_amdgcn_s_sleep(1);
+        if (TJ > 0)
+            while (__hip_atomic_load(&tile_flags[tile - ntx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < sweep)
+                __builtin_amdgcn_s_sleep(1);
+    }
+    __syncthreads();
+    const int i0 = 2 + TI * LX_TI, j0 = 2 + TJ * LX_TJ;
+    const int ni = min(LX_TI, nx - i0 + 1), nj = min(LX_TJ, ny - j0 + 1);
+    // column i0 + t, rows j0 .. j0+nj-1, walked along anti-diagonals
+    double mx = 0.0;
+    for (int step = 0; step < nj + LX_T - 1; step++) {
+        const int jj = step - t;
+        if (t < ni && jj >= 0 && jj < nj) {
+            const int i = i0 + t, j = j0 + jj;
+            const size_t k = WD2_IX(i, j);
+            double a1, a2, a3, a4, a5;
+            if (COEF) {
+                a1 = cf.a1[k]; a2 = cf.a2[k]; a3 = cf.a3[k]; a4 = cf.a4[k]; a5 = cf.a5[k];
+            } else {
+                a1 = rgv[WD2_IX(i, j - 1)]; a2 = rau[WD2_IX(i - 1, j)]; a4 = rau[k]; a5 = rgv[k];
+                a3 = -a4 - a2 - a5 - a1;
+            }
+            // neighbours written during this sweep by other lanes or tiles are
+            // read through the coherent path
+            const double pS = ld_coherent(&p[WD2_IX(i, j - 1)]);
+            const double pW = ld_coherent(&p[WD2_IX(i - 1, j)]);
+            const double pE = p[WD2_IX(i + 1, j)], pN = p[WD2_IX(i, j + 1)];
+            double sum = b[k] - a1 * pS - a2 * pW - a4 * pE - a5 * pN;
+            const double pij = p[k];
+            sum = sum / a3 - pij;
+            const double pnew = pij + sorrel * sum;
+            __hip_atomic_store((unsigned long long *)&p[k], dbits(pnew), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            mx = dmaxf(mx, fabs(sum));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = dmaxf(mx, __shfl_xor(mx, off));
+    if (t == 0) {
+        partials[tile] = mx;
+        __hip_atomic_store(&tile_flags[tile], sweep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (last_arrival(&sc->counter[6], ntiles, t)) {
+        double r = 0.0;
+        for (unsigned int k = t; k < ntiles; k += LX_T) r = dmaxf(r, ld_coherent(&partials[k]));
+        for (int off = 32; off > 0; off >>= 1) r = dmaxf(r, __shfl_xor(r, off));
+        if (t == 0) {
+            sc->counter[5] = 0;
+            sor_finish(sc, r, sortol, msorit);
+        }
+    }
+    (void)sh;
+}
+
+void launch_sor_lex_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
+                          double sorrel, double sortol, int msorit, double *p, int *tile_flags, int sweep,
+                          double *partials, Scalars *sc, hipStream_t s) {
+    const int ntx = (nx - 1 + LX_TI - 1) / LX_TI, nty = (ny - 1 + LX_TJ - 1) / LX_TJ;
+    PpeCoef cf = coef ? *coef : PpeCoef{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (coef)
+        hipLaunchKernelGGL(k_sor_lex<true>, dim3(ntx * nty), dim3(LX_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel,
+                           sortol, msorit, p, tile_flags, sweep, partials, sc);
+    else
+        hipLaunchKernelGGL(k_sor_lex<false>, dim3(ntx * nty), dim3(LX_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b,
+                           sorrel, sortol, msorit, p, tile_flags, sweep, partials, sc);
+}
+
+// ------------------------------------------------------------- line SOR
+// Chains of the lines j = first_line + line_step*q (q < nlines): row r of line q
+// is i = 2 + r (r < nx-1), with (a, d, c) = (a2, a3, a4) and
+// b = rhs - (a1*p(i,j-1) + a5*p(i,j+1))   (pressure.f:746-771)
+template <bool COEF>
+__global__ __launch_bounds__(256) void k_slor_assemble(int nx, int ny, int ld, const double *__restrict__ rau,
+                                                       const double *__restrict__ rgv, PpeCoef cf,
+                                                       const double *__restrict__ b, const double *__restrict__ p,
+                                                       int first_line, int line_step, int nlines,
+                                                       double *__restrict__ ca, double *__restrict__ cd,
+                                                       double *__restrict__ cc, double *__restrict__ cb,
+                                                       const Scalars *sc) {
+    if (sc->sor_done) return;
+    const int L = nx - 1;
+    const long r = (long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= (long)L * nlines) return;
+    const int q = (int)(r / L), i = 2 + (int)(r % L);
+    const int j = first_line + line_step * q;
+    const size_t k = WD2_IX(i, j);
+    double a1, a2, a3, a4, a5;
+    if (COEF) {
+        a1 = cf.a1[k]; a2 = cf.a2[k]; a3 = cf.a3[k]; a4 = cf.a4[k]; a5 = cf.a5[k];
+    } else {
+        a1 = rgv[WD2_IX(i, j - 1)]; a2 = rau[WD2_IX(i - 1, j)]; a4 = rau[k]; a5 = rgv[k];
+        a3 = -a4 - a2 - a5 - a1;
+    }
+    ca[r] = a2;
+    cd[r] = a3;
+    cc[r] = a4;
+    cb[r] = b[k] - (a1 * p[WD2_IX(i, j - 1)] + a5 * p[WD2_IX(i, j + 1)]);
+}
+
+void launch_slor_assemble(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
+                          const double *p, int first_line, int line_step, int nlines, double *ca, double *cd,
+                          double *cc, double *cb, const Scalars *sc, hipStream_t s) {
+    const long n = (long)(nx - 1) * nlines;
+    PpeCoef cf = coef ? *coef : PpeCoef{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (coef)
+        hipLaunchKernelGGL(k_slor_assemble<true>, dim3((n + 255) / 256), dim3(256), 0, s, nx, ny, ld, m.rau, m.rgv, cf,
+                           b, p, first_line, line_step, nlines, ca, cd, cc, cb, sc);
+    else
+        hipLaunchKernelGGL(k_slor_assemble<false>, dim3((n + 255) / 256), dim3(256), 0, s, nx, ny, ld, m.rau, m.rgv,
+                           cf, b, p, first_line, line_step, nlines, ca, cd, cc, cb, sc);
+}
+
+// Relaxation of the solved lines: p = plold + sorrel*(x - plold).  dif_mode 0:
+// max |x - plold| (Slor, SlorRB); 1: max |plold - pnew| (SlorRBP, pressure.f:1122).
+__global__ __launch_bounds__(256) void k_slor_relax(int nx, int ny, int ld, int first_line, int line_step,
+                                                    int nlines, const double *__restrict__ x, double sorrel,
+                                                    int dif_mode, double *__restrict__ p, Scalars *sc, int slot) {
+    if (sc->sor_done) return;
+    __shared__ double sh[4];
+    const int L = nx - 1;
+    const long r = (long)blockIdx.x * 256 + threadIdx.x;
+    double mx = 0.0;
+    if (r < (long)L * nlines) {
+        const int q = (int)(r / L), i = 2 + (int)(r % L);
+        const int j = first_line + line_step * q;
+        const size_t k = WD2_IX(i, j);
+        const double plold = p[k];
+        const double sum = x[r] - plold;
+        const double pnew = plold + sorrel * sum;
+        p[k] = pnew;
+        mx = dif_mode == 0 ? fabs(sum) : fabs(plold - pnew);
+    }
+    mx = block_max<256>(mx, sh, threadIdx.x);
+    if (threadIdx.x == 0) atomicMax((unsigned long long *)&sc->val[slot], dbits(mx));
+}
+
+void launch_slor_relax(int nx, int ny, int ld, int first_line, int line_step, int nlines, const double *x,
+                       double sorrel, int dif_mode, double *p, const double *pn, double *partials, Scalars *sc,
+                       int slot, hipStream_t s) {
+    const long n = (long)(nx - 1) * nlines;
+    (void)pn;
+    (void)partials;
+    hipLaunchKernelGGL(k_slor_relax, dim3((n + 255) / 256), dim3(256), 0, s, nx, ny, ld, first_line, line_step,
+                       nlines, x, sorrel, dif_mode, p, sc, slot);
+}
+
+__global__ void k_slor_finish(double sortol, int msorit, Scalars *sc) {
+    if (sc->sor_done) return;
+    const double dif = sc->val[6];
+    sc->val[6] = 0.0;
+    sor_finish(sc, dif, sortol, msorit);
+}
+
+void launch_slor_finish(int nparts, double sortol, int msorit, double *partials, Scalars *sc, hipStream_t s) {
+    (void)nparts;
+    (void)partials;
+    hipLaunchKernelGGL(k_slor_finish, dim3(1), dim3(1), 0, s, sortol, msorit, sc);
+}
+
+void launch_slor_rbp_relax(int, int, int, int, double, const double *, double *, hipStream_t) {}
+
+}  // namespace wd2

@@ -1,0 +1,784 @@
+// wd2_api.hip -- host side of the wolfd2 HIP path: device context, the
+// reference's call sequences (nAuxMomentum, Ppe, main.f time step) and the C
+// ABI of include/wolfd2_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wolfd2_amd.h"
+#include "wd2_dev.hpp"
+#include "wd2_launch.hpp"
+
+using namespace wd2;
+
+static thread_local std::string g_err;
+static int g_fallbacks = 0;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(WD2_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));   \
+    } while (0)
+
+// Metric order of grid.f:Grid (and of wd2_ctx_create's metrics argument).
+enum {
+    M_RAU, M_RBU, M_RBV, M_RGV, M_RAN, M_RBN, M_RGN, M_RAC, M_RBC, M_RGC, M_DJU, M_DJV, M_DJC, M_DJN,
+    M_XEN, M_YEN, M_XZN, M_YZN, M_XEC, M_YEC, M_XZC, M_YZC, M_XEU, M_YEU, M_XZV, M_YZV, M_XZU, M_YZU,
+    M_XEV, M_YEV, M_COUNT
+};
+
+struct wd2_ctx {
+    int nx = 0, ny = 0, ld = 0;
+    size_t nf = 0;                 // elements per field
+    hipStream_t s = nullptr;
+    // region tables
+    int mgri = 0, mgrj = 0;
+    std::vector<int> h_nreg, h_brd, h_rtype, h_btp;
+    std::vector<double> h_bcv, h_por, h_pc1, h_pc2;
+    int *d_int = nullptr;
+    double *d_dbl = nullptr;
+    Regions R{};
+    bool any_outlet = false, any_porous = false, any_blockage = false;
+    unsigned char *prow = nullptr;
+    // metrics
+    double *metric_store = nullptr;
+    double *met[M_COUNT] = {};
+    Metrics M{};
+    // fields
+    double *store = nullptr;
+    double *u, *v, *p, *un, *vn, *pn, *us, *vs, *dus, *dvs, *d, *dn, *div, *rhs, *beff;
+    double *pc[5];
+    MomTemps T{};
+    Chain ch{};
+    TridWork tw{};
+    double *partials = nullptr;
+    Scalars *sc = nullptr;
+    int *tile_flags = nullptr;
+    int sweep_id = 0;
+    bool coef_ready = false;
+};
+
+static inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+static void ctx_free(wd2_ctx *c) {
+    if (!c) return;
+    (void)hipFree(c->d_int);
+    (void)hipFree(c->d_dbl);
+    (void)hipFree(c->prow);
+    (void)hipFree(c->metric_store);
+    (void)hipFree(c->store);
+    (void)hipFree(c->partials);
+    (void)hipFree(c->sc);
+    (void)hipFree(c->tile_flags);
+    (void)hipFree(c->tw.cp);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+}
+
+static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
+    if (nx < 5 || ny < 5) return fail(WD2_ERR_ARG, "grid %dx%d too small (need nx, ny >= 5)", nx, ny);
+    wd2_ctx *c = new wd2_ctx();
+    c->nx = nx;
+    c->ny = ny;
+    c->ld = round_up(nx + 2, 16);
+    c->nf = (size_t)c->ld * (ny + 2);
+    HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    const int NFIELDS = 15 + 5 + 13;
+    const size_t nchain = (size_t)nx * ny + 64;
+    const size_t total = NFIELDS * c->nf + 5 * nchain;
+    HIPCHK(hipMalloc(&c->store, total * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c->store, 0, total * sizeof(double), c->s));
+    double *q = c->store;
+    auto take = [&](size_t n) { double *r = q; q += n; return r; };
+    double **f[] = {&c->u, &c->v, &c->p, &c->un, &c->vn, &c->pn, &c->us, &c->vs, &c->dus, &c->dvs,
+                    &c->d, &c->dn, &c->div, &c->rhs, &c->beff};
+    for (double **pp : f) *pp = take(c->nf);
+    for (int k = 0; k < 5; k++) c->pc[k] = take(c->nf);
+    double **t[] = {&c->T.cj1, &c->T.cj2, &c->T.c1s, &c->T.c2s, &c->T.c1n, &c->T.c2n, &c->T.cnvs,
+                    &c->T.cnvn, &c->T.difs, &c->T.difn, &c->T.cps, &c->T.cpn, &c->T.cpj};
+    for (double **pp : t) *pp = take(c->nf);
+    c->ch.a = take(nchain);
+    c->ch.d = take(nchain);
+    c->ch.c = take(nchain);
+    c->ch.b = take(nchain);
+    c->ch.x = take(nchain);
+    const int nb = trid_blocks((long)nchain) + 2;
+    HIPCHK(hipMalloc(&c->tw.cp, (2 * nchain + 3 * nb) * sizeof(double) + 2 * sizeof(int) + 64));
+    c->tw.bp = c->tw.cp + nchain;
+    c->tw.blk_c = c->tw.bp + nchain;
+    c->tw.blk_b = c->tw.blk_c + nb;
+    c->tw.blk_x = c->tw.blk_b + nb;
+    c->tw.fail = (int *)(c->tw.blk_x + nb);
+    c->tw.nfallback = c->tw.fail + 1;
+    HIPCHK(hipMemsetAsync(c->tw.fail, 0, 2 * sizeof(int), c->s));
+    const size_t npart = 4 * (size_t)RED_MAX_BLOCKS + 2 * (size_t)(c->nf / 256 + 64);
+    HIPCHK(hipMalloc(&c->partials, npart * sizeof(double)));
+    HIPCHK(hipMalloc(&c->sc, sizeof(Scalars)));
+    HIPCHK(hipMemsetAsync(c->sc, 0, sizeof(Scalars), c->s));
+    const int ntiles = ((nx + 62) / 64 + 1) * ((ny + 62) / 64 + 1);
+    HIPCHK(hipMalloc(&c->tile_flags, ntiles * sizeof(int)));
+    HIPCHK(hipMemsetAsync(c->tile_flags, 0, ntiles * sizeof(int), c->s));
+    HIPCHK(hipMalloc(&c->metric_store, (size_t)M_COUNT * c->nf * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c->metric_store, 0, (size_t)M_COUNT * c->nf * sizeof(double), c->s));
+    for (int k = 0; k < M_COUNT; k++) c->met[k] = c->metric_store + (size_t)k * c->nf;
+    Metrics &M = c->M;
+    M.ran = c->met[M_RAN]; M.rbn = c->met[M_RBN]; M.rgn = c->met[M_RGN];
+    M.rac = c->met[M_RAC]; M.rbc = c->met[M_RBC]; M.rgc = c->met[M_RGC];
+    M.rau = c->met[M_RAU]; M.rbu = c->met[M_RBU]; M.rbv = c->met[M_RBV]; M.rgv = c->met[M_RGV];
+    M.dju = c->met[M_DJU]; M.djv = c->met[M_DJV];
+    M.xen = c->met[M_XEN]; M.yen = c->met[M_YEN]; M.xzn = c->met[M_XZN]; M.yzn = c->met[M_YZN];
+    M.xec = c->met[M_XEC]; M.yec = c->met[M_YEC]; M.xzc = c->met[M_XZC]; M.yzc = c->met[M_YZC];
+    M.xeu = c->met[M_XEU]; M.yeu = c->met[M_YEU]; M.xzv = c->met[M_XZV]; M.yzv = c->met[M_YZV];
+    M.xzu = c->met[M_XZU]; M.yzu = c->met[M_YZU]; M.xev = c->met[M_XEV]; M.yev = c->met[M_YEV];
+    HIPCHK(hipStreamSynchronize(c->s));
+    *out = c;
+    return WD2_OK;
+}
+
+// host (ld_h) -> device (c->ld) field copy of columns 0..ny+1, rows 0..nx+1
+static int up(wd2_ctx *c, const double *h, int ld_h, double *dst) {
+    if (!h) return WD2_OK;
+    HIPCHK(hipMemcpy2DAsync(dst, (size_t)c->ld * 8, h, (size_t)ld_h * 8, (size_t)(c->nx + 2) * 8, c->ny + 2,
+                            hipMemcpyHostToDevice, c->s));
+    return WD2_OK;
+}
+static int down(wd2_ctx *c, const double *src, double *h, int ld_h) {
+    HIPCHK(hipMemcpy2DAsync(h, (size_t)ld_h * 8, src, (size_t)c->ld * 8, (size_t)(c->nx + 2) * 8, c->ny + 2,
+                            hipMemcpyDeviceToHost, c->s));
+    return WD2_OK;
+}
+
+static bool valid_btype(int t) { return t >= BM_INTERN && t <= BM_OUTLT2; }
+
+// Region tables (host copies + device copies + derived flags and the PPE
+// identity-row mask of pressure.f:109-196).
+static int set_regions(wd2_ctx *c, int mgri, int mgrj, const int *nReg, const int *nRegBrd, const int *nRegType,
+                       const int *nMomBdTp, const double *dBCVal, const double *por, const double *pc1,
+                       const double *pc2) {
+    if (!nReg || !nRegBrd || !nMomBdTp) return fail(WD2_ERR_ARG, "region tables missing");
+    const int ni = nReg[0], nj = nReg[1];
+    if (mgri < 1 || mgrj < 1 || ni < 1 || nj < 1 || ni > mgri || nj > mgrj)
+        return fail(WD2_ERR_ARG, "bad region counts %d x %d (tables %d x %d)", ni, nj, mgri, mgrj);
+    const size_t n2 = (size_t)mgri * mgrj;
+    c->mgri = mgri;
+    c->mgrj = mgrj;
+    c->h_nreg.assign(nReg, nReg + 2);
+    c->h_brd.assign(nRegBrd, nRegBrd + 4 * n2);
+    c->h_rtype.assign(n2, RM_INTERN);
+    if (nRegType) c->h_rtype.assign(nRegType, nRegType + n2);
+    c->h_btp.assign(nMomBdTp, nMomBdTp + 4 * n2);
+    c->h_bcv.assign(16 * n2, 0.0);
+    if (dBCVal) c->h_bcv.assign(dBCVal, dBCVal + 16 * n2);
+    c->h_por.assign(n2, 1.0);
+    c->h_pc1.assign(n2, 0.0);
+    c->h_pc2.assign(n2, 0.0);
+    if (por) c->h_por.assign(por, por + n2);
+    if (pc1) c->h_pc1.assign(pc1, pc1 + n2);
+    if (pc2) c->h_pc2.assign(pc2, pc2 + n2);
+    if (!c->d_int) {
+        HIPCHK(hipMalloc(&c->d_int, (2 + 9 * n2) * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_dbl, 19 * n2 * sizeof(double)));
+    }
+    int *di = c->d_int;
+    double *dd = c->d_dbl;
+    HIPCHK(hipMemcpyAsync(di, c->h_brd.data(), 4 * n2 * sizeof(int), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(di + 4 * n2, c->h_rtype.data(), n2 * sizeof(int), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(di + 5 * n2, c->h_btp.data(), 4 * n2 * sizeof(int), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dd, c->h_bcv.data(), 16 * n2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dd + 16 * n2, c->h_por.data(), n2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dd + 17 * n2, c->h_pc1.data(), n2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dd + 18 * n2, c->h_pc2.data(), n2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    Regions &R = c->R;
+    R.nx = c->nx; R.ny = c->ny; R.ni = ni; R.nj = nj; R.mgri = mgri; R.mgrj = mgrj;
+    R.brd = di; R.rtype = di + 4 * n2; R.btp = di + 5 * n2;
+    R.bcv = dd; R.poros = dd + 16 * n2; R.porc1 = dd + 17 * n2; R.porc2 = dd + 18 * n2;
+    // host-side scans
+    auto brd = [&](int ir, int jr, int s) { return c->h_brd[(ir - 1) + mgri * ((jr - 1) + mgrj * (s - 1))]; };
+    auto btp = [&](int ir, int jr, int s) { return c->h_btp[(ir - 1) + mgri * ((jr - 1) + mgrj * (s - 1))]; };
+    auto rty = [&](int ir, int jr) { return c->h_rtype[(ir - 1) + mgri * (jr - 1)]; };
+    c->any_outlet = c->any_porous = c->any_blockage = false;
+    for (int jr = 1; jr <= nj; jr++)
+        for (int ir = 1; ir <= ni; ir++) {
+            for (int s = 1; s <= 4; s++) {
+                const int t = btp(ir, jr, s);
+                if (!valid_btype(t)) return fail(WD2_ERR_FLAG, "Wrong nBdType flag %d in region %d,%d", t, ir, jr);
+                if (t == BM_OUTLT1 || t == BM_OUTLT2) c->any_outlet = true;
+                const int b = brd(ir, jr, s);
+                if (b < 1 || b > ((s <= 2) ? c->nx : c->ny))
+                    return fail(WD2_ERR_ARG, "region %d,%d border %d out of range", ir, jr, b);
+            }
+            if (rty(ir, jr) == RM_POROUS) c->any_porous = true;
+            if (rty(ir, jr) == RM_BLOCKG) c->any_blockage = true;
+        }
+    if (c->any_blockage) {
+        const int nx = c->nx, ny = c->ny, ld = c->ld;
+        std::vector<unsigned char> m(c->nf, 0);
+        auto mark = [&](int i, int j) {
+            if (i >= 2 && i <= nx && j >= 2 && j <= ny) m[(size_t)i + (size_t)ld * j] = 1;
+        };
+        for (int jr = 1; jr <= nj; jr++)
+            for (int ir = 1; ir <= ni; ir++) {
+                if (rty(ir, jr) != RM_BLOCKG) continue;
+                const int iW = brd(ir, jr, WEST), iE = brd(ir, jr, EAST), jS = brd(ir, jr, SOUTH),
+                          jN = brd(ir, jr, NORTH);
+                for (int j = jS + 2; j <= jN - 1; j++)
+                    for (int i = iW + 2; i <= iE - 1; i++) mark(i, j);
+                if (ir > 1 && rty(ir - 1, jr) == RM_BLOCKG)
+                    for (int j = jS + 1; j <= jN; j++) mark(iW + 1, j);
+                if (ir < ni && rty(ir + 1, jr) == RM_BLOCKG)
+                    for (int j = jS + 1; j <= jN; j++) mark(iE, j);
+                if (jr > 1 && rty(ir, jr - 1) == RM_BLOCKG)
+                    for (int i = iW + 1; i <= iE; i++) mark(i, jS + 1);
+                if (jr < nj && rty(ir, jr + 1) == RM_BLOCKG)
+                    for (int i = iW + 1; i <= iE; i++) mark(i, jN);
+            }
+        if (!c->prow) HIPCHK(hipMalloc(&c->prow, c->nf));
+        HIPCHK(hipMemcpyAsync(c->prow, m.data(), c->nf, hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    }
+    c->coef_ready = false;
+    return WD2_OK;
+}
+
+static int check_err(wd2_ctx *c) {
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, &c->sc->err, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (e) {
+        const int z = 0;
+        HIPCHK(hipMemcpy(&c->sc->err, &z, sizeof(int), hipMemcpyHostToDevice));
+        return fail(WD2_ERR_FLAG, "bad boundary flag (code %d)", e);
+    }
+    return WD2_OK;
+}
+
+// -------------------------------------------------------------- momentum
+static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
+                    const double *un, const double *vn, double *out) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    const Metrics &M = c->M;
+    hipStream_t s = c->s;
+    if (comp == 1) {
+        launch_conv_coef(nx, ny, ld, 4, 1, M.xzu, M.xeu, M.yzu, M.yeu, us, vs, c->T.cj1, c->T.cj2, s);
+        launch_conv_coef(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, us, vs, c->T.c1s, c->T.c2s, s);
+        launch_conv_coef(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, un, vn, c->T.c1n, c->T.c2n, s);
+    } else {
+        launch_conv_coef(nx, ny, ld, 5, 1, M.xzv, M.xev, M.yzv, M.yev, us, vs, c->T.cj1, c->T.cj2, s);
+        launch_conv_coef(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, us, vs, c->T.c1s, c->T.c2s, s);
+        launch_conv_coef(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, un, vn, c->T.c1n, c->T.c2n, s);
+    }
+    if (c->any_porous) launch_porous_div(c->R, ld, comp, c->T, s);
+    const double *fs = comp == 1 ? us : vs, *fn = comp == 1 ? un : vn;
+    launch_conv_diff(nx, ny, ld, comp, M, fs, fn, c->T, s);
+    if (c->any_porous) launch_poros_coef(c->R, ld, comp, us, vs, un, vn, c->T, s);
+    const long n = comp == 1 ? (long)nx * (ny - 1) : (long)(nx - 1) * ny;
+    launch_assemble(c->R, ld, comp, 1, dk, re, fr, M, c->T, fs, fn, c->d, c->dn, nullptr, c->ch, s, &c->sc->err);
+    launch_trid(c->tw, n, n, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
+    launch_assemble(c->R, ld, comp, 2, dk, re, fr, M, c->T, fs, fn, c->d, c->dn, c->ch.x, c->ch, s, &c->sc->err);
+    launch_trid(c->tw, n, n, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
+    launch_scatter(nx, ny, ld, comp, c->ch.x, out, s);
+    return WD2_OK;
+}
+
+// momentum.f:nAuxMomentum 33-193 on the context's un, vn -> us, vs.
+static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr, double qtol, int *nql) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    launch_copy(ld, 1, nx + 1, 1, ny + 1, c->un, c->us, c->s);
+    launch_copy(ld, 1, nx + 1, 1, ny + 1, c->vn, c->vs, c->s);
+    *nql = -1;
+    for (int m = 1; m <= mqiter; m++) {
+        if (c->any_outlet) launch_vel_bc<double>(c->R, ld, c->us, c->vs, true, &c->sc->err, c->s);
+        int st = momentum(c, 1, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dus);
+        if (st) return st;
+        st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs);
+        if (st) return st;
+        launch_update_norms(nx, ny, ld, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, c->s);
+        double nv[2];
+        HIPCHK(hipMemcpyAsync(nv, c->sc->val, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        const double difmax = nv[0] > nv[1] ? nv[0] : nv[1];
+        if (difmax <= qtol) {
+            *nql = m;
+            break;
+        }
+    }
+    return check_err(c);
+}
+
+// ------------------------------------------------------------------- Ppe
+static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double sortol, double sorrel,
+               const double *u, const double *v, double *p, int *nconv_out) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    if (solver < 1 || solver > 6) {
+        // pressure.f:234-236 prints and leaves p untouched; Ppe then reports msorit
+        fprintf(stdout, " Wrong nPpeSolver flag passed to Ppe\n");
+        *nconv_out = msorit;
+        return WD2_OK;
+    }
+    launch_divergence(nx, ny, ld, 1, c->M, u, v, c->div, s);
+    launch_ppe_rhs(nx, ny, ld, dk, c->div, c->any_blockage ? c->prow : nullptr, c->rhs, s);
+    const PpeCoef cf{c->pc[0], c->pc[1], c->pc[2], c->pc[3], c->pc[4]};
+    const PpeCoef *coef = nullptr;
+    if (c->any_blockage || solver == 2 || solver == 3 || solver == 4) {
+        if (!c->coef_ready) {
+            launch_ppe_coef(nx, ny, ld, c->M, c->any_blockage ? c->prow : nullptr, c->pc[0], c->pc[1], c->pc[2],
+                            c->pc[3], c->pc[4], s);
+            c->coef_ready = true;
+        }
+        coef = &cf;
+    }
+    launch_sor_reset(c->sc, s);
+    const double *b = c->rhs;
+    const int batch = (solver == 2) ? 1 : 32;
+    int done = 0;
+    for (int it = 0; it < msorit && !done;) {
+        for (int q = 0; q < batch && it < msorit; q++, it++) {
+            if (solver == 5 || solver == 6) {
+                if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
+                launch_sor_rb_sweep(nx, ny, ld, c->M, coef, b, sorrel, sortol, msorit, p, c->partials, c->sc, s);
+            } else if (solver == 1) {
+                if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
+                launch_sor_lex_sweep(nx, ny, ld, c->M, coef, b, sorrel, sortol, msorit, p, c->tile_flags,
+                                     ++c->sweep_id, c->partials, c->sc, s);
+            } else if (solver == 3 || solver == 4) {
+                for (int k = 2; k <= 3; k++) {
+                    if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
+                    const int nl = (ny - k) / 2 + 1;
+                    if (nl < 1) continue;
+                    const long n = (long)(nx - 1) * nl;
+                    launch_slor_assemble(nx, ny, ld, c->M, coef, b, p, k, 2, nl, c->ch.a, c->ch.d, c->ch.c,
+                                         c->ch.b, c->sc, s);
+                    launch_trid(c->tw, n, nx - 1, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
+                    launch_slor_relax(nx, ny, ld, k, 2, nl, c->ch.x, sorrel, solver == 4 ? 1 : 0, p, nullptr,
+                                      nullptr, c->sc, 6, s);
+                }
+                launch_slor_finish(0, sortol, msorit, nullptr, c->sc, s);
+            } else {   // solver 2: lines in order j = 2..ny
+                if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
+                for (int j = 2; j <= ny; j++) {
+                    launch_slor_assemble(nx, ny, ld, c->M, coef, b, p, j, 1, 1, c->ch.a, c->ch.d, c->ch.c,
+                                         c->ch.b, c->sc, s);
+                    launch_trid(c->tw, nx - 1, nx - 1, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
+                    launch_slor_relax(nx, ny, ld, j, 1, 1, c->ch.x, sorrel, 0, p, nullptr, nullptr, c->sc, 6, s);
+                }
+                launch_slor_finish(0, sortol, msorit, nullptr, c->sc, s);
+            }
+        }
+        HIPCHK(hipMemcpyAsync(&done, &c->sc->sor_done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    int nconv = 0;
+    HIPCHK(hipMemcpyAsync(&nconv, &c->sc->sor_nconv, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (nconv < 1) {
+        fprintf(stdout, " Warning: SOR iterations did not converge after %6d iterations.\n", msorit);
+        nconv = msorit;
+    }
+    *nconv_out = nconv;
+    return WD2_OK;
+}
+
+// ------------------------------------------------------------- main.f step
+static int cold_start(wd2_ctx *c, const wd2_params *P, int *nsor) {
+    launch_vel_bc<double>(c->R, c->ld, c->u, c->v, false, &c->sc->err, c->s);
+    int st = ppe(c, P->lCartesGrid, P->nPpeSolver, P->msorit, P->dk, P->sortol, P->sorrel, c->u, c->v, c->p, nsor);
+    if (st) return st;
+    launch_pres_bc<double>(c->R, c->ld, c->p, &c->sc->err, c->s);
+    launch_project(c->R, c->ld, P->dk, c->M, c->p, c->u, c->v, c->s);
+    launch_vel_bc<double>(c->R, c->ld, c->u, c->v, false, &c->sc->err, c->s);
+    return check_err(c);
+}
+
+static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double *dif3) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    // main.f:696-704, 741-747
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->p, c->pn, s);
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->u, c->un, s);
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->v, c->vn, s);
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->d, c->dn, s);
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->u, c->us, s);
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->v, c->vs, s);
+    int st = naux_momentum(c, P->mqiter, P->dk, P->re, P->fr, P->qtol, nql);
+    if (st) return st;
+    launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    st = ppe(c, P->lCartesGrid, P->nPpeSolver, P->msorit, P->dk, P->sortol, P->sorrel, c->us, c->vs, c->p, nsor);
+    if (st) return st;
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    launch_project(c->R, ld, P->dk, c->M, c->p, c->us, c->vs, s);
+    launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    // main.f:864-870: u = us, v = vs (swap the buffers: same values)
+    std::swap(c->u, c->us);
+    std::swap(c->v, c->vs);
+    launch_vel_bc<double>(c->R, ld, c->u, c->v, false, &c->sc->err, s);
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    // main.f:962-965
+    launch_max_norm(nx, ny, ld, c->pn, c->p, c->partials, c->sc, 2, s);
+    launch_max_norm(nx, ny, ld, c->un, c->u, c->partials + RED_MAX_BLOCKS, c->sc, 3, s);
+    launch_max_norm(nx, ny, ld, c->vn, c->v, c->partials + 2 * RED_MAX_BLOCKS, c->sc, 4, s);
+    if (dif3) {
+        HIPCHK(hipMemcpyAsync(dif3, &c->sc->val[2], 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    return check_err(c);
+}
+
+static int fallbacks(wd2_ctx *c) {
+    int n = 0;
+    if (hipMemcpy(&n, c->tw.nfallback, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    return n;
+}
+
+// ================================================================ C ABI
+
+static wd2_ctx *g_ctx = nullptr;   // drop-in calls share one context per grid size
+
+static int dropin_ctx(int nx, int ny, int ld) {
+    if (ld < nx + 2) return fail(WD2_ERR_ARG, "ld=%d < nx+2=%d", ld, nx + 2);
+    if (g_ctx && g_ctx->nx == nx && g_ctx->ny == ny) return WD2_OK;
+    if (g_ctx) {
+        g_fallbacks += fallbacks(g_ctx);
+        ctx_free(g_ctx);
+        g_ctx = nullptr;
+    }
+    return ctx_alloc(&g_ctx, nx, ny);
+}
+
+static int up_metric(int k, const double *h, int ld) { return up(g_ctx, h, ld, g_ctx->met[k]); }
+
+extern "C" {
+
+const char *wd2_last_error(void) { return g_err.c_str(); }
+
+int wd2_trid_fallbacks(void) { return g_fallbacks + (g_ctx ? fallbacks(g_ctx) : 0); }
+
+int wd2_vel_bound_cond(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                       const int *nMomBdTp, const double *dBCVal, double *u, double *v) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nullptr, nMomBdTp, dBCVal, nullptr, nullptr, nullptr)))
+        return st;
+    if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v))) return st;
+    launch_vel_bc<double>(c->R, c->ld, c->u, c->v, false, &c->sc->err, c->s);
+    if ((st = check_err(c))) return st;
+    if ((st = down(c, c->u, u, ld)) || (st = down(c, c->v, v, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_vel_outflow_bcs(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                        const int *nMomBdTp, const double *dBCVal, double *u, double *v) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nullptr, nMomBdTp, dBCVal, nullptr, nullptr, nullptr)))
+        return st;
+    if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v))) return st;
+    launch_vel_bc<double>(c->R, c->ld, c->u, c->v, true, &c->sc->err, c->s);
+    if ((st = check_err(c))) return st;
+    if ((st = down(c, c->u, u, ld)) || (st = down(c, c->v, v, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_pres_bound_cond(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                        const int *nRegType, const int *nMomBdTp, const double *dBCVal, double *p) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, nMomBdTp, dBCVal, nullptr, nullptr, nullptr)))
+        return st;
+    if ((st = up(c, p, ld, c->p))) return st;
+    launch_pres_bc<double>(c->R, c->ld, c->p, &c->sc->err, c->s);
+    if ((st = check_err(c))) return st;
+    if ((st = down(c, c->p, p, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_project(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                const int *nRegType, const int *nMomBdTp, double dk, const double *dju, const double *djv,
+                const double *yeu, const double *xzv, const double *yzu, const double *xev, const double *p,
+                double *u, double *v) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, nMomBdTp, nullptr, nullptr, nullptr, nullptr)))
+        return st;
+    if ((st = up_metric(M_DJU, dju, ld)) || (st = up_metric(M_DJV, djv, ld)) || (st = up_metric(M_YEU, yeu, ld)) ||
+        (st = up_metric(M_XZV, xzv, ld)) || (st = up_metric(M_YZU, yzu, ld)) || (st = up_metric(M_XEV, xev, ld)))
+        return st;
+    if ((st = up(c, p, ld, c->p)) || (st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v))) return st;
+    launch_project(c->R, c->ld, dk, c->M, c->p, c->u, c->v, c->s);
+    if ((st = down(c, c->u, u, ld)) || (st = down(c, c->v, v, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_divergence(int nx, int ny, int ld, int nloc, const double *xet, const double *yet, const double *xzi,
+                   const double *yzi, const double *u, const double *v, double *div) {
+    int st;
+    if (nloc != 1 && nloc != 2) return fail(WD2_ERR_FLAG, "Wrong location flag passed to Divergence: %d", nloc);
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    const int kx = nloc == 1 ? M_XEU : M_XEV, ky = nloc == 1 ? M_YEU : M_YEV;
+    const int kzx = nloc == 1 ? M_XZV : M_XZU, kzy = nloc == 1 ? M_YZV : M_YZU;
+    if ((st = up_metric(kx, xet, ld)) || (st = up_metric(ky, yet, ld)) || (st = up_metric(kzx, xzi, ld)) ||
+        (st = up_metric(kzy, yzi, ld)))
+        return st;
+    if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v)) || (st = up(c, div, ld, c->div))) return st;
+    launch_divergence(nx, ny, c->ld, nloc, c->M, c->u, c->v, c->div, c->s);
+    if ((st = down(c, c->div, div, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_ppe(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd, const int *nRegType,
+            int lCartesGrid, int nPpeSolver, int msorit, int *nSorConv, double dk, double sortol, double sorrel,
+            const double *rau, const double *rbu, const double *rbv, const double *rgv, const double *xeu,
+            const double *yeu, const double *xzv, const double *yzv, const double *u, const double *v,
+            double *p) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    std::vector<int> btp(4 * (size_t)mgri * mgrj, BM_INTERN);   // Ppe takes no boundary types
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, btp.data(), nullptr, nullptr, nullptr, nullptr)))
+        return st;
+    if ((st = up_metric(M_RAU, rau, ld)) || (st = up_metric(M_RBU, rbu, ld)) || (st = up_metric(M_RBV, rbv, ld)) ||
+        (st = up_metric(M_RGV, rgv, ld)) || (st = up_metric(M_XEU, xeu, ld)) || (st = up_metric(M_YEU, yeu, ld)) ||
+        (st = up_metric(M_XZV, xzv, ld)) || (st = up_metric(M_YZV, yzv, ld)))
+        return st;
+    if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v)) || (st = up(c, p, ld, c->p))) return st;
+    if ((st = ppe(c, lCartesGrid != 0, nPpeSolver, msorit, dk, sortol, sorrel, c->u, c->v, c->p, nSorConv)))
+        return st;
+    if ((st = down(c, c->p, p, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_x_momentum(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                   const int *nRegType, const int *nMomBdTp, double dk, double re, const double *dPRporos,
+                   const double *dPRporc1, const double *dPRporc2, const double *rbn, const double *rgn,
+                   const double *rac, const double *rbc, const double *dju, const double *xec, const double *yec,
+                   const double *xzn, const double *yzn, const double *xeu, const double *yeu, const double *xzu,
+                   const double *yzu, const double *us, const double *vs, const double *un, const double *vn,
+                   double *dus) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, nMomBdTp, nullptr, dPRporos, dPRporc1,
+                          dPRporc2)))
+        return st;
+    const int ks[] = {M_RBN, M_RGN, M_RAC, M_RBC, M_DJU, M_XEC, M_YEC, M_XZN, M_YZN, M_XEU, M_YEU, M_XZU, M_YZU};
+    const double *hs[] = {rbn, rgn, rac, rbc, dju, xec, yec, xzn, yzn, xeu, yeu, xzu, yzu};
+    for (int k = 0; k < 13; k++)
+        if ((st = up_metric(ks[k], hs[k], ld))) return st;
+    if ((st = up(c, us, ld, c->us)) || (st = up(c, vs, ld, c->vs)) || (st = up(c, un, ld, c->un)) ||
+        (st = up(c, vn, ld, c->vn)) || (st = up(c, dus, ld, c->dus)))
+        return st;
+    if ((st = momentum(c, 1, dk, re, 1.0, c->us, c->vs, c->un, c->vn, c->dus))) return st;
+    if ((st = check_err(c))) return st;
+    if ((st = down(c, c->dus, dus, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_y_momentum(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                   const int *nRegType, const int *nMomBdTp, double dk, double re, double fr,
+                   const double *dPRporos, const double *dPRporc1, const double *dPRporc2, const double *ran,
+                   const double *rbn, const double *rbc, const double *rgc, const double *djv, const double *xen,
+                   const double *yen, const double *xzc, const double *yzc, const double *xev, const double *yev,
+                   const double *xzv, const double *yzv, const double *d, const double *dn, const double *us,
+                   const double *vs, const double *un, const double *vn, double *dvs) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, nMomBdTp, nullptr, dPRporos, dPRporc1,
+                          dPRporc2)))
+        return st;
+    const int ks[] = {M_RAN, M_RBN, M_RBC, M_RGC, M_DJV, M_XEN, M_YEN, M_XZC, M_YZC, M_XEV, M_YEV, M_XZV, M_YZV};
+    const double *hs[] = {ran, rbn, rbc, rgc, djv, xen, yen, xzc, yzc, xev, yev, xzv, yzv};
+    for (int k = 0; k < 13; k++)
+        if ((st = up_metric(ks[k], hs[k], ld))) return st;
+    if ((st = up(c, us, ld, c->us)) || (st = up(c, vs, ld, c->vs)) || (st = up(c, un, ld, c->un)) ||
+        (st = up(c, vn, ld, c->vn)) || (st = up(c, d, ld, c->d)) || (st = up(c, dn, ld, c->dn)) ||
+        (st = up(c, dvs, ld, c->dvs)))
+        return st;
+    if ((st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs))) return st;
+    if ((st = check_err(c))) return st;
+    if ((st = down(c, c->dvs, dvs, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_naux_momentum(int nx, int ny, int ld, int mgri, int mgrj, int mqiter, const int *nReg, const int *nRegBrd,
+                      const int *nRegType, const int *nMomBdTp, double dk, double re, double fr, double qtol,
+                      const double *dPRporos, const double *dPRporc1, const double *dPRporc2,
+                      const double *dBCVal, const double *ran, const double *rbn, const double *rgn,
+                      const double *rac, const double *rbc, const double *rgc, const double *dju, const double *djv,
+                      const double *xec, const double *yec, const double *xzn, const double *yzn, const double *xen,
+                      const double *yen, const double *xzc, const double *yzc, const double *xeu, const double *yeu,
+                      const double *xzu, const double *yzu, const double *xev, const double *yev, const double *xzv,
+                      const double *yzv, const double *d, const double *dn, const double *un, const double *vn,
+                      double *us, double *vs, int *nQLiter) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, nMomBdTp, dBCVal, dPRporos, dPRporc1,
+                          dPRporc2)))
+        return st;
+    const int ks[] = {M_RAN, M_RBN, M_RGN, M_RAC, M_RBC, M_RGC, M_DJU, M_DJV, M_XEC, M_YEC, M_XZN, M_YZN,
+                      M_XEN, M_YEN, M_XZC, M_YZC, M_XEU, M_YEU, M_XZU, M_YZU, M_XEV, M_YEV, M_XZV, M_YZV};
+    const double *hs[] = {ran, rbn, rgn, rac, rbc, rgc, dju, djv, xec, yec, xzn, yzn,
+                          xen, yen, xzc, yzc, xeu, yeu, xzu, yzu, xev, yev, xzv, yzv};
+    for (int k = 0; k < 24; k++)
+        if ((st = up_metric(ks[k], hs[k], ld))) return st;
+    if ((st = up(c, d, ld, c->d)) || (st = up(c, dn, ld, c->dn)) || (st = up(c, un, ld, c->un)) ||
+        (st = up(c, vn, ld, c->vn)) || (st = up(c, us, ld, c->us)) || (st = up(c, vs, ld, c->vs)))
+        return st;
+    // nAuxMomentum zeroes the increments on 0..nx+1 x 0..ny+1 each iteration
+    launch_fill(c->ld, 0, nx + 1, 0, ny + 1, 0.0, c->dus, c->s);
+    launch_fill(c->ld, 0, nx + 1, 0, ny + 1, 0.0, c->dvs, c->s);
+    int nql = -1;
+    if ((st = naux_momentum(c, mqiter, dk, re, fr, qtol, &nql))) return st;
+    if (nQLiter) *nQLiter = nql;
+    if ((st = down(c, c->us, us, ld)) || (st = down(c, c->vs, vs, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_alt_trid_lu(int n, const double *a, double *b) {
+    if (n < 2) return fail(WD2_ERR_ARG, "AltTridLU needs n >= 2");
+    int st;
+    const int side = (int)std::ceil(std::sqrt((double)n)) + 8;
+    if ((st = dropin_ctx(side, side, side + 2))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((size_t)n > (size_t)c->nx * c->ny) return fail(WD2_ERR_ARG, "chain too long");
+    std::vector<double> ad(n), dd(n), cd(n);
+    for (int k = 0; k < n; k++) {
+        ad[k] = a[3 * k];
+        dd[k] = a[3 * k + 1];
+        cd[k] = a[3 * k + 2];
+    }
+    HIPCHK(hipMemcpyAsync(c->ch.a, ad.data(), n * 8, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->ch.d, dd.data(), n * 8, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->ch.c, cd.data(), n * 8, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->ch.b, b, n * 8, hipMemcpyHostToDevice, c->s));
+    launch_trid(c->tw, n, n, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, c->s);
+    HIPCHK(hipMemcpyAsync(b, c->ch.x, n * 8, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+static int norm_common(int nx, int ny, int ld, const double *a, const double *b, double *out) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = up(c, a, ld, c->un)) || (st = up(c, b, ld, c->u))) return st;
+    launch_max_norm(nx, ny, c->ld, c->un, b ? c->u : nullptr, c->partials, c->sc, 2, c->s);
+    HIPCHK(hipMemcpyAsync(out, &c->sc->val[2], sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_dmax_norm(int nx, int ny, int ld, const double *u, double *out) { return norm_common(nx, ny, ld, u, nullptr, out); }
+int wd2_diff_max_norm(int nx, int ny, int ld, const double *un, const double *u, double *out) {
+    return norm_common(nx, ny, ld, un, u, out);
+}
+
+// ------------------------------------------------------------ context API
+int wd2_ctx_create(wd2_ctx **out, int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                   const int *nRegType, const int *nMomBdTp, const double *dBCVal, const double *dPRporos,
+                   const double *dPRporc1, const double *dPRporc2, const double *const *metrics) {
+    if (!out || !metrics) return fail(WD2_ERR_ARG, "null argument");
+    if (ld < nx + 2) return fail(WD2_ERR_ARG, "ld=%d < nx+2=%d", ld, nx + 2);
+    wd2_ctx *c = nullptr;
+    int st = ctx_alloc(&c, nx, ny);
+    if (st) return st;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, nMomBdTp, dBCVal, dPRporos, dPRporc1, dPRporc2))) {
+        ctx_free(c);
+        return st;
+    }
+    for (int k = 0; k < M_COUNT; k++)
+        if ((st = up(c, metrics[k], ld, c->met[k]))) {
+            ctx_free(c);
+            return st;
+        }
+    if (hipStreamSynchronize(c->s) != hipSuccess) {
+        ctx_free(c);
+        return fail(WD2_ERR_HIP, "metric upload failed");
+    }
+    *out = c;
+    return WD2_OK;
+}
+
+int wd2_ctx_destroy(wd2_ctx *c) {
+    if (c) g_fallbacks += fallbacks(c);
+    ctx_free(c);
+    return WD2_OK;
+}
+
+int wd2_ctx_set_fields(wd2_ctx *c, int ld, const double *u, const double *v, const double *p) {
+    int st;
+    if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v)) || (st = up(c, p, ld, c->p))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_ctx_get_fields(wd2_ctx *c, int ld, double *u, double *v, double *p) {
+    int st;
+    if (u && (st = down(c, c->u, u, ld))) return st;
+    if (v && (st = down(c, c->v, v, ld))) return st;
+    if (p && (st = down(c, c->p, p, ld))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_ctx_cold_start(wd2_ctx *c, const wd2_params *P, int *nSorConv) {
+    int n = 0;
+    int st = cold_start(c, P, &n);
+    if (nSorConv) *nSorConv = n;
+    return st;
+}
+
+int wd2_ctx_step(wd2_ctx *c, const wd2_params *P, int nsteps, int *ql_iter, int *ppe_iter, double *dif) {
+    for (int k = 0; k < nsteps; k++) {
+        int nql = 0, nsor = 0;
+        int st = one_step(c, P, &nql, &nsor, dif ? dif + 3 * k : nullptr);
+        if (st) return st;
+        if (ql_iter) ql_iter[k] = nql;
+        if (ppe_iter) ppe_iter[k] = nsor;
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_ctx_device_fields(wd2_ctx *c, double **u, double **v, double **p, int *ld) {
+    if (u) *u = c->u;
+    if (v) *v = c->v;
+    if (p) *p = c->p;
+    if (ld) *ld = c->ld;
+    return WD2_OK;
+}
+
+void *wd2_ctx_stream(wd2_ctx *c) { return (void *)c->s; }
+
+}  // extern "C"

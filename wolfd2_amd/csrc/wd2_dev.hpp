@@ -1,0 +1,84 @@
+// wd2_dev.hpp -- shared device-side definitions of the wolfd2 HIP path (gfx950).
+//
+// Layout on the device: every field is a double (or float) array of ldd*(ny+2)
+// elements, element (i,j) at f[i + ldd*j] for i = 0..nx+1, j = 0..ny+1 -- the
+// reference's (0:mnx,0:mny) column-major layout with the leading dimension
+// rounded up to a multiple of 16 elements so that every grid row starts on a
+// 128-byte boundary.  Fortran (1-based) indices are used throughout.
+//
+// FP64 kernels reproduce the reference's arithmetic bit for bit: the library is
+// compiled with -ffp-contract=off and every expression keeps the reference's
+// evaluation order (left to right, as written in momentum.f / pressure.f /
+// utility.f / bound_cond.f).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wd2 {
+
+// wolfd2.h:15-42
+enum { RM_BLOCKG = 0, RM_INTERN = 1, RM_POROUS = 2 };
+enum { BM_INTERN = 0, BM_WALL1 = 1, BM_WALL2 = 2, BM_INLET = 3, BM_OUTLT1 = 4, BM_OUTLT2 = 5 };
+enum { WEST = 1, EAST = 2, SOUTH = 3, NORTH = 4 };
+enum { VU = 1, VV = 2, VP = 3, VT = 4 };
+
+// Region tables on the device (Fortran-ordered copies of the caller's tables).
+struct Regions {
+    int nx, ny, ni, nj, mgri, mgrj;
+    const int *brd, *rtype, *btp;
+    const double *bcv, *poros, *porc1, *porc2;
+
+    __host__ __device__ int border(int ir, int jr, int side) const {
+        return brd[(ir - 1) + mgri * ((jr - 1) + mgrj * (side - 1))];
+    }
+    __host__ __device__ int btype(int ir, int jr, int side) const {
+        return btp[(ir - 1) + mgri * ((jr - 1) + mgrj * (side - 1))];
+    }
+    __host__ __device__ int type(int ir, int jr) const { return rtype[(ir - 1) + mgri * (jr - 1)]; }
+    __host__ __device__ double bc(int ir, int jr, int side, int var) const {
+        return bcv[(ir - 1) + mgri * ((jr - 1) + mgrj * ((side - 1) + 4 * (var - 1)))];
+    }
+    __host__ __device__ double tab(const double *t, int ir, int jr) const { return t[(ir - 1) + mgri * (jr - 1)]; }
+};
+
+// Metric arrays used on the hot path (grid.f:Metric names).
+struct Metrics {
+    const double *ran, *rbn, *rgn, *rac, *rbc, *rgc, *rau, *rbu, *rbv, *rgv;
+    const double *dju, *djv, *xen, *yen, *xzn, *yzn, *xec, *yec, *xzc, *yzc;
+    const double *xeu, *yeu, *xzv, *yzv, *xzu, *yzu, *xev, *yev;
+};
+
+#define WD2_IX(i, j) ((size_t)(i) + (size_t)ld * (size_t)(j))
+
+// Non-negative doubles order like their bit patterns read as unsigned integers.
+__device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ inline double dmaxf(double a, double b) { return a > b ? a : b; }
+
+// Load that bypasses the (non-coherent) per-CU cache: used to read values that
+// other workgroups of the same kernel published before an atomic ticket.
+__device__ inline double ld_coherent(const double *p) {
+    return bitsd(__hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Block-wide max of a double over NT threads (NT a multiple of 64, tid the
+// flattened thread index); every thread gets the result.
+template <int NT>
+__device__ inline double block_max(double v, double *sh, int tid) {
+    for (int off = 32; off > 0; off >>= 1) v = dmaxf(v, __shfl_xor(v, off));
+    int lane = tid & 63, w = tid >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double m = sh[0];
+        for (int k = 1; k < NT / 64; k++) m = dmaxf(m, sh[k]);
+        sh[0] = m;
+    }
+    __syncthreads();
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace wd2

@@ -1,0 +1,106 @@
+// wd2_launch.hpp -- host-side launchers of the wolfd2 HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wd2_dev.hpp"
+
+namespace wd2 {
+
+// Scratch of the parallel AltTridLU for chains of up to nmax rows.
+struct TridWork {
+    double *cp, *bp;              // forward-swept super-diagonal / right-hand side
+    double *blk_c, *blk_b, *blk_x;  // per-workgroup boundary states
+    int *fail, *nfallback;        // device flags
+};
+int trid_blocks(long n);
+void launch_trid(const TridWork &w, long n, long L, const double *A, const double *D, const double *C,
+                 const double *B, double *X, hipStream_t s);
+
+// Boundary fills (one workgroup).  err receives 1000+side for a bad flag.
+template <typename T>
+void launch_vel_bc(const Regions &R, int ld, T *u, T *v, bool outflow_only, int *err, hipStream_t s);
+template <typename T>
+void launch_pres_bc(const Regions &R, int ld, T *p, int *err, hipStream_t s);
+
+// Device scalars shared by the reductions (one cache line each).
+struct Scalars {
+    double val[8];                 // reduction results
+    unsigned int counter[8];       // last-workgroup tickets (always returned to 0)
+    int sor_done, sor_nconv, sor_m, pad0;
+    double sor_dif;
+    int err;                       // bad-flag code from a kernel (0 = none)
+    int pad1[15];
+};
+
+constexpr int RED_MAX_BLOCKS = 4096;   // partial-maximum slots per reduction
+
+// utility.f:Project
+void launch_project(const Regions &R, int ld, double dk, const Metrics &m, const double *p, double *u,
+                    double *v, hipStream_t s);
+// pressure.f:Divergence (nloc 1 or 2)
+void launch_divergence(int nx, int ny, int ld, int nloc, const Metrics &m, const double *u, const double *v,
+                       double *div, hipStream_t s);
+// Ppe right-hand side: rhs = div/dk with blockage rows zeroed (prow != 0 marks them).
+void launch_ppe_rhs(int nx, int ny, int ld, double dk, const double *div, const unsigned char *prow,
+                    double *rhs, hipStream_t s);
+// RhsPpe mixed-derivative terms for curvilinear grids: beff = rhs - cross(p).
+void launch_rhs_cross(int nx, int ny, int ld, const Metrics &m, const double *rhs, const double *p,
+                      double *beff, hipStream_t s);
+// Five PPE coefficient fields with the blockage identity rows.
+void launch_ppe_coef(int nx, int ny, int ld, const Metrics &m, const unsigned char *prow, double *a1,
+                     double *a2, double *a3, double *a4, double *a5, hipStream_t s);
+// utility.f:DMaxNorm / DiffMaxNorm into sc->val[slot] (b may be null for DMaxNorm).
+void launch_max_norm(int nx, int ny, int ld, const double *a, const double *b, double *partials, Scalars *sc,
+                     int slot, hipStream_t s);
+// dst = src on i in [i0,i1], j in [j0,j1]
+void launch_copy(int ld, int i0, int i1, int j0, int j1, const double *src, double *dst, hipStream_t s);
+void launch_fill(int ld, int i0, int i1, int j0, int j1, double val, double *dst, hipStream_t s);
+
+// SOR family.  coef: null -> coefficients from rau/rgv; else five fields.
+struct PpeCoef {
+    const double *a1, *a2, *a3, *a4, *a5;
+};
+void launch_sor_rb_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
+                         double sorrel, double sortol, int msorit, double *p, double *partials, Scalars *sc,
+                         hipStream_t s);
+void launch_sor_reset(Scalars *sc, hipStream_t s);
+// Lexicographic SOR sweep (pressure.f:Sor), tiled wavefront.
+void launch_sor_lex_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
+                          double sorrel, double sortol, int msorit, double *p, int *tile_flags, int sweep,
+                          double *partials, Scalars *sc, hipStream_t s);
+// Line SOR: assemble the chains of one colour (first_line, step 2 lines, or one line), then relax.
+void launch_slor_assemble(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
+                          const double *p, int first_line, int line_step, int nlines, double *ca, double *cd,
+                          double *cc, double *cb, const Scalars *sc, hipStream_t s);
+void launch_slor_relax(int nx, int ny, int ld, int first_line, int line_step, int nlines, const double *x,
+                       double sorrel, int dif_mode, double *p, const double *pn, double *partials, Scalars *sc,
+                       int slot, hipStream_t s);
+void launch_slor_finish(int nparts, double sortol, int msorit, double *partials, Scalars *sc, hipStream_t s);
+// SlorRBP: whole-grid relaxation of one colour's lines and |pn - p| max.
+void launch_slor_rbp_relax(int nx, int ny, int ld, int first_line, double sorrel, const double *pn, double *p,
+                           hipStream_t s);
+
+// Momentum (momentum.f)
+struct MomTemps {
+    double *cj1, *cj2, *c1s, *c2s, *c1n, *c2n, *cnvs, *cnvn, *difs, *difn, *cps, *cpn, *cpj;
+};
+struct Chain {
+    double *a, *d, *c, *b, *x;
+};
+void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const double *xzi, const double *xet,
+                      const double *yzi, const double *yet, const double *u, const double *v, double *cc1,
+                      double *cc2, hipStream_t s);
+void launch_porous_div(const Regions &R, int ld, int comp, const MomTemps &t, hipStream_t s);
+void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, const double *vs,
+                       const double *un, const double *vn, const MomTemps &t, hipStream_t s);
+void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *un,
+                      const MomTemps &t, hipStream_t s);
+void launch_assemble(const Regions &R, int ld, int comp, int split, double dk, double re, double fr,
+                     const Metrics &m, const MomTemps &t, const double *us, const double *un, const double *d,
+                     const double *dn, const double *xprev, const Chain &ch, hipStream_t s, int *err);
+void launch_scatter(int nx, int ny, int ld, int comp, const double *x, double *out, hipStream_t s);
+void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const double *dus, const double *dvs,
+                         double *partials, Scalars *sc, hipStream_t s);
+
+}  // namespace wd2
