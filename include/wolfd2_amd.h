@@ -40,6 +40,13 @@ extern "C" {
 const char *wd2_last_error(void);
 /* Number of times the parallel AltTridLU fell back to its sequential path. */
 int wd2_trid_fallbacks(void);
+/* Red-black SOR on Cartesian grids without blockages runs several sweeps per
+ * launch on LDS tiles (on by default); 0 selects one launch per colour and
+ * sweep.  Both give the reference's iterates bit for bit.  Returns the
+ * previous setting. */
+int wd2_set_tiled_sor(int enable);
+/* Red-black SOR sweeps per launch of its dominant kernel (1 when not tiled). */
+int wd2_sor_sweeps_per_launch(void);
 
 /* momentum.f:nAuxMomentum 33-48.  *nQLiter receives the function value: the
  * number of q-l iterations, or -1 when not converged in mqiter iterations. */
@@ -159,6 +166,14 @@ int wd2_ctx_step(wd2_ctx *ctx, const wd2_params *prm, int nsteps, int *ql_iter, 
 int wd2_ctx_device_fields(wd2_ctx *ctx, double **u, double **v, double **p, int *ld);
 /* The HIP stream the context runs on. */
 void *wd2_ctx_stream(wd2_ctx *ctx);
+/* Timing of the red-black SOR kernels (the tiled multi-sweep kernel, or the
+ * colour kernels) with HIP events on the context's stream: enable=1 starts
+ * recording an event pair around every launch,
+ * enable=0 stops and returns the mean launch duration (microseconds, while
+ * the kernel occupies the stream, excluding launch gaps) and the launch count.
+ * Sweeps of a solve that has already converged are early-exit launches and
+ * are counted too. */
+int wd2_ctx_kernel_timing(wd2_ctx *ctx, int enable, double *avg_us, long *launches);
 
 #ifdef __cplusplus
 }

@@ -1,4 +1,5 @@
-"""Input decks, grids and restart files in the reference solver's formats.
+"""Input decks, grids and restart files in the reference solver's formats.  TEST INFRASTRUCTURE
+(drives the compiled reference as a checker and CPU baseline).
 
 Writes what the reference program reads (the keyword deck of
 src/parse.f:lReadIPFile 32-878 and lReadBCFile 1004-1951, the fluid table of

@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, HERE)
 
-import decks  # noqa: E402
+import refdecks as decks  # noqa: E402
 from pyoracle import Reference, run_reference_program  # noqa: E402
 from wolfd2_amd import case  # noqa: E402
 

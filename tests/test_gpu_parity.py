@@ -98,9 +98,19 @@ def test_alt_trid_lu_bitwise_vs_oracle(oracle, n, weak):
     assert np.array_equal(x_gpu, x_ref)
 
 
+@pytest.fixture(params=[1, 0], ids=["tiled_sor", "colour_sor"])
+def sor_tiling(request):
+    from wolfd2_amd._lib import lib
+    old = lib().wd2_set_tiled_sor(request.param)
+    yield request.param
+    lib().wd2_set_tiled_sor(old)
+
+
 @pytest.mark.parametrize("name", helpers.run_fixture_names())
-def test_whole_runs_bitwise_vs_reference_program(name):
+def test_whole_runs_bitwise_vs_reference_program(name, sor_tiling):
     fx = helpers.load(name)
+    if not sor_tiling and int(fx["ppe_solver"]) not in (5, 6):
+        pytest.skip("the tiling switch only changes the red-black SOR path")
     g, reg, prm = helpers.run_case(fx)
     nx, ny = g.nx, g.ny
     print(f"run {name} nx={nx} ny={ny} solver={prm.nPpeSolver} steps={int(fx['nts'])}", flush=True)
@@ -125,15 +135,18 @@ def test_whole_runs_bitwise_vs_reference_program(name):
 
 
 @pytest.mark.parametrize("solver", [5, 1])
-def test_full_size_cavity_steps_bitwise_vs_oracle(oracle, solver):
+def test_full_size_cavity_steps_bitwise_vs_oracle(oracle, solver, sor_tiling):
     """1024^2 cells (the benchmark grid): a cold start and two time steps with
     a capped Poisson solve, GPU against the oracle, bit for bit."""
     nx = ny = 1025
     x, y = case.uniform_nodes(nx, ny)
     g = case.compute_grid(x, y)
     reg = case.cavity_regions(nx, ny)
-    prm = case.Params.from_physical(2e-4, 1.0, 1e-3, nPpeSolver=solver, msorit=12 if solver == 5 else 4)
-    print(f"full-size cavity solver={solver}", flush=True)
+    if not sor_tiling and solver != 5:
+        pytest.skip("the tiling switch only changes the red-black SOR path")
+    # 13 sweeps: three tiled launches of four and a one-sweep tail
+    prm = case.Params.from_physical(2e-4, 1.0, 1e-3, nPpeSolver=solver, msorit=13 if solver == 5 else 4)
+    print(f"full-size cavity solver={solver} tiled={sor_tiling}", flush=True)
     sol = api.Solver(g, reg, prm)
     z = np.zeros((g.ncol, g.ld))
     sol.set_fields(z, z, z)

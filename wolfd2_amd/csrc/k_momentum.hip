@@ -376,53 +376,56 @@ void launch_scatter(int nx, int ny, int ld, int comp, const double *x, double *o
 
 // ------------------------------------------- q-l update and increment norms
 // us += dus, vs += dvs on i=1..nx, j=1..ny (momentum.f:171-176) and
-// DMaxNorm of dus -> val[0], of dvs -> val[1] (momentum.f:179-180).
+// DMaxNorm of dus -> val[0], of dvs -> val[1] (momentum.f:179-180), folded by
+// a one-workgroup kernel.
 __global__ __launch_bounds__(MX *MY) void k_update_norms(int nx, int ny, int ld, double *__restrict__ us,
                                                          double *__restrict__ vs, const double *__restrict__ dus,
-                                                         const double *__restrict__ dvs, double *__restrict__ partials,
-                                                         Scalars *sc) {
+                                                         const double *__restrict__ dvs, double *__restrict__ partials) {
     __shared__ double sh[MX * MY / 64];
-    __shared__ bool amlast;
     const int tid = threadIdx.x + MX * threadIdx.y;
-    const int i = 1 + blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
     double m1 = 0.0, m2 = 0.0;
-    if (i <= nx && j <= ny) {
-        const size_t k = WD2_IX(i, j);
-        const double du = dus[k], dv = dvs[k];
-        us[k] = us[k] + du;
-        vs[k] = vs[k] + dv;
-        if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1) { m1 = fabs(du); m2 = fabs(dv); }
+    const int i = 1 + blockIdx.x * MX + threadIdx.x;
+    for (int j = 1 + blockIdx.y * MY + threadIdx.y; j <= ny; j += gridDim.y * MY) {
+        if (i <= nx) {
+            const size_t k = WD2_IX(i, j);
+            const double du = dus[k], dv = dvs[k];
+            us[k] = us[k] + du;
+            vs[k] = vs[k] + dv;
+            if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1) {
+                m1 = dmaxf(m1, fabs(du));
+                m2 = dmaxf(m2, fabs(dv));
+            }
+        }
     }
     m1 = block_max<MX * MY>(m1, sh, tid);
     m2 = block_max<MX * MY>(m2, sh, tid);
-    const unsigned int nb = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned int bid = blockIdx.x + gridDim.x * blockIdx.y;
     if (tid == 0) { partials[2 * bid] = m1; partials[2 * bid + 1] = m2; }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned int tk = atomicAdd(&sc->counter[0], 1u);
-        amlast = (tk == nb - 1);
-        if (amlast) sc->counter[0] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_update_norms_fold(int ld, const double *__restrict__ dus,
+                                                           const double *__restrict__ dvs,
+                                                           const double *__restrict__ partials, int nb, Scalars *sc) {
+    __shared__ double sh[4];
+    double r1 = 0.0, r2 = 0.0;
+    for (int q = threadIdx.x; q < nb; q += 256) {
+        r1 = dmaxf(r1, partials[2 * q]);
+        r2 = dmaxf(r2, partials[2 * q + 1]);
     }
-    __syncthreads();
-    if (amlast) {
-        double r1 = 0.0, r2 = 0.0;
-        for (unsigned int q = tid; q < nb; q += MX * MY) {
-            r1 = dmaxf(r1, ld_coherent(&partials[2 * q]));
-            r2 = dmaxf(r2, ld_coherent(&partials[2 * q + 1]));
-        }
-        r1 = block_max<MX * MY>(r1, sh, tid);
-        r2 = block_max<MX * MY>(r2, sh, tid);
-        if (tid == 0) {
-            sc->val[0] = dmaxf(fabs(dus[WD2_IX(5, 5)]), r1);
-            sc->val[1] = dmaxf(fabs(dvs[WD2_IX(5, 5)]), r2);
-        }
+    r1 = block_max<256>(r1, sh, threadIdx.x);
+    r2 = block_max<256>(r2, sh, threadIdx.x);
+    if (threadIdx.x == 0) {
+        sc->val[0] = dmaxf(fabs(dus[WD2_IX(5, 5)]), r1);
+        sc->val[1] = dmaxf(fabs(dvs[WD2_IX(5, 5)]), r2);
     }
 }
 
 void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const double *dus, const double *dvs,
                          double *partials, Scalars *sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_update_norms, mgrid(nx, ny), dim3(MX, MY), 0, s, nx, ny, ld, us, vs, dus, dvs, partials, sc);
+    const int gx = (nx + MX - 1) / MX;
+    const int gy = min((ny + MY - 1) / MY, max(1, 1024 / gx));
+    hipLaunchKernelGGL(k_update_norms, dim3(gx, gy), dim3(MX, MY), 0, s, nx, ny, ld, us, vs, dus, dvs, partials);
+    hipLaunchKernelGGL(k_update_norms_fold, dim3(1), dim3(256), 0, s, ld, dus, dvs, partials, gx * gy, sc);
 }
 
 }  // namespace wd2

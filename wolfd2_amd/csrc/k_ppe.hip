@@ -8,7 +8,7 @@
 //     of the last one, so 64x64 tiles run in wavefront order (each workgroup
 //     takes the next tile in row-major order from an atomic ticket and waits for
 //     its west and south tiles), and inside a tile the points run along
-//     anti-diagonals through LDS.
+//     anti-diagonals (a barrier between diagonals).
 //   Slor / SlorRB / SlorRBP (pressure.f:673-1138): each j-line is an (nx-1)-row
 //     AltTridLU; the lines of one colour are solved as one batch of chains.
 //
@@ -72,14 +72,40 @@ __device__ inline double sor_point(int ld, int i, int j, const double *__restric
     return fabs(sum);
 }
 
+// Max of n published partial maxima (written by earlier kernels), folded by
+// one workgroup; every thread gets the result.
+__device__ inline double fold_partials(const double *__restrict__ part, int n, double *sh, int tid) {
+    double r = 0.0;
+    for (int k = tid; k < n; k += RB_T) r = dmaxf(r, part[k]);
+    return block_max<RB_T>(r, sh, tid);
+}
+
+// Sweep m of SorRB/SorRBP, one colour.  Before doing any work every
+// workgroup evaluates the stopping test of sweep m-1 from that sweep's
+// published partial maxima (pressure.f:534: m-1 > 1 and dif < sortol); all
+// workgroups of both colour kernels reach the same decision, so the solve
+// stops after exactly the sweep the reference stops after.  No device-scope
+// fence is needed: partials cross workgroups only through kernel boundaries.
 template <int COLOR, bool COEF>
 __global__ __launch_bounds__(RB_T) void k_sor_rb(int nx, int ny, int ld, const double *__restrict__ rau,
                                                  const double *__restrict__ rgv, PpeCoef cf,
-                                                 const double *__restrict__ b, double sorrel, double sortol,
-                                                 int msorit, double *p, double *__restrict__ partials, Scalars *sc) {
+                                                 const double *__restrict__ b, double sorrel, double sortol, int m,
+                                                 double *p, const double *__restrict__ prev_part, int nprev,
+                                                 double *__restrict__ cur_part, Scalars *sc) {
     if (sc->sor_done) return;
     __shared__ double sh[RB_T / 64];
     const int tid = threadIdx.x;
+    if (m - 1 > 1) {
+        const double d = fold_partials(prev_part, nprev, sh, tid);
+        if (d < sortol) {
+            if (COLOR == 0 && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+                sc->sor_done = 1;
+                sc->sor_nconv = m - 1;
+                sc->sor_dif = d;
+            }
+            return;
+        }
+    }
     double mx = 0.0;
     for (int j = 2 + blockIdx.y; j <= ny; j += gridDim.y) {
         const int i0 = COLOR == 0 ? 2 + (j % 2) : 2 + ((j + 1) % 2);
@@ -88,34 +114,208 @@ __global__ __launch_bounds__(RB_T) void k_sor_rb(int nx, int ny, int ld, const d
     }
     mx = block_max<RB_T>(mx, sh, tid);
     const unsigned int nb = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
-    if (tid == 0) partials[COLOR * nb + bid] = mx;
-    if (COLOR == 1 && last_arrival(&sc->counter[7], nb, tid)) {
-        double r = 0.0;
-        for (unsigned int k = tid; k < 2 * nb; k += RB_T) r = dmaxf(r, ld_coherent(&partials[k]));
-        r = block_max<RB_T>(r, sh, tid);
-        if (tid == 0) sor_finish(sc, r, sortol, msorit);
+    if (tid == 0) cur_part[COLOR * nb + bid] = mx;
+}
+
+// Stopping test of the last launched sweep m (one workgroup).
+__global__ __launch_bounds__(RB_T) void k_sor_check(const double *__restrict__ part, int n, int m, double sortol,
+                                                    int msorit, Scalars *sc) {
+    if (sc->sor_done) return;
+    __shared__ double sh[RB_T / 64];
+    const double d = fold_partials(part, n, sh, threadIdx.x);
+    if (threadIdx.x == 0) {
+        sc->sor_dif = d;
+        if (m > 1 && d < sortol) {
+            sc->sor_done = 1;
+            sc->sor_nconv = m;
+        } else if (m >= msorit) {
+            sc->sor_done = 1;
+        }
     }
 }
 
+static dim3 rb_grid(int nx, int ny) {
+    const int per_row = (nx / 2 + RB_T - 1) / RB_T;
+    const int rows = min(ny - 1, max(1, 512 / per_row));
+    return dim3(per_row, rows);
+}
+
+int sor_rb_partials(int nx, int ny) {
+    const dim3 g = rb_grid(nx, ny);
+    return 2 * g.x * g.y;
+}
+
 void launch_sor_rb_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
-                         double sorrel, double sortol, int msorit, double *p, double *partials, Scalars *sc,
-                         hipStream_t s) {
-    const int per_row = (nx / 2 + RB_T) / RB_T;
-    int rows = ny - 1;
-    while ((long)per_row * rows * 2 > RED_MAX_BLOCKS) rows = (rows + 1) / 2;
-    const dim3 g(per_row, rows);
+                         double sorrel, double sortol, int sweep, double *p, double *partials, Scalars *sc,
+                         hipStream_t s, hipEvent_t *ev) {
+    const dim3 g = rb_grid(nx, ny);
+    const int np = 2 * g.x * g.y;
+    const double *prev = partials + (size_t)((sweep - 1) & 1) * np;
+    double *cur = partials + (size_t)(sweep & 1) * np;
     PpeCoef cf = coef ? *coef : PpeCoef{nullptr, nullptr, nullptr, nullptr, nullptr};
-    if (coef) {
-        hipLaunchKernelGGL((k_sor_rb<0, true>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel, sortol,
-                           msorit, p, partials, sc);
-        hipLaunchKernelGGL((k_sor_rb<1, true>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel, sortol,
-                           msorit, p, partials, sc);
-    } else {
-        hipLaunchKernelGGL((k_sor_rb<0, false>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel,
-                           sortol, msorit, p, partials, sc);
-        hipLaunchKernelGGL((k_sor_rb<1, false>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel,
-                           sortol, msorit, p, partials, sc);
+    for (int color = 0; color < 2; color++) {
+        if (ev) (void)hipEventRecord(ev[2 * color], s);
+        if (coef) {
+            if (color == 0)
+                hipLaunchKernelGGL((k_sor_rb<0, true>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel,
+                                   sortol, sweep, p, prev, np, cur, sc);
+            else
+                hipLaunchKernelGGL((k_sor_rb<1, true>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b, sorrel,
+                                   sortol, sweep, p, prev, np, cur, sc);
+        } else {
+            if (color == 0)
+                hipLaunchKernelGGL((k_sor_rb<0, false>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b,
+                                   sorrel, sortol, sweep, p, prev, np, cur, sc);
+            else
+                hipLaunchKernelGGL((k_sor_rb<1, false>), g, dim3(RB_T), 0, s, nx, ny, ld, m.rau, m.rgv, cf, b,
+                                   sorrel, sortol, sweep, p, prev, np, cur, sc);
+        }
+        if (ev) (void)hipEventRecord(ev[2 * color + 1], s);
     }
+}
+
+void launch_sor_rb_check(int nx, int ny, int sweep, double sortol, int msorit, const double *partials,
+                         Scalars *sc, hipStream_t s) {
+    const int np = sor_rb_partials(nx, ny);
+    hipLaunchKernelGGL(k_sor_check, dim3(1), dim3(RB_T), 0, s, partials + (size_t)(sweep & 1) * np, np, sweep,
+                       sortol, msorit, sc);
+}
+
+// --------------------------------------------- temporally blocked SorRB
+// Cartesian grids without blockages: the right-hand side and the five
+// coefficients do not change during the solve, so a workgroup can run KT
+// sweeps on an LDS tile.  The tile holds TT x TT owned points plus a halo of
+// TH = 2*KT cells: a red-black sweep moves the dependence front by two cells
+// (black points read red neighbours, red points read the new black ones), so
+// after KT sweeps every owned point has seen exactly the values it sees in
+// the global sweep order, and its arithmetic is the reference's.  Points of
+// the outermost ring are never updated; points outside the array are never
+// read by an updated point.  p, the right-hand side, rau and rgv of the tile
+// sit in LDS (4 x 64 x 65 doubles); in each half-sweep the 1024 lanes take
+// one point of that colour from each of two horizontal pairs.
+//
+// The kernel reads p_in and writes the owned points to p_out (tiles read
+// their neighbours' points as halo, so it cannot work in place), and
+// publishes the max |s| of each of its sweeps over its owned points.
+constexpr int KT = 4, TH = 2 * KT, TW = 64, TT = TW - 2 * TH;   // 48 owned
+constexpr int TP_T = 1024;
+constexpr int TLD = TW + 1;               // padded LDS row
+
+__global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, const double *__restrict__ rau,
+                                                    const double *__restrict__ rgv, const double *__restrict__ b,
+                                                    double sorrel, int nsweep, const double *__restrict__ p_in,
+                                                    double *__restrict__ p_out, double *__restrict__ part,
+                                                    int ntx, int ntiles, const Scalars *sc, int force) {
+    if (!force && sc->sor_done) return;
+    __shared__ double P[TW * TLD], B[TW * TLD], RA[TW * TLD], RG[TW * TLD];
+    __shared__ double red[KT][TP_T / 64];
+    // XCD-aware tile order: consecutive workgroups go to different XCDs, so
+    // give each XCD a contiguous band of tiles (their halos then share an L2).
+    const int per = (ntiles + 7) / 8;
+    const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (tile >= ntiles) return;
+    const int TI = tile % ntx, TJ = tile / ntx;
+    const int i0 = 2 + TI * TT - TH, j0 = 2 + TJ * TT - TH;   // tile origin
+    const int t = threadIdx.x;
+    for (int q = t; q < TW * TW; q += TP_T) {
+        const int x = q % TW, y = q / TW, i = i0 + x, j = j0 + y;
+        const bool inarr = i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1;
+        const size_t k = inarr ? WD2_IX(i, j) : 0;
+        P[y * TLD + x] = inarr ? p_in[k] : 0.0;
+        B[y * TLD + x] = inarr ? b[k] : 0.0;
+        RA[y * TLD + x] = inarr ? rau[k] : 0.0;
+        RG[y * TLD + x] = inarr ? rgv[k] : 0.0;
+    }
+    __syncthreads();
+    const int lane = t & 63, w = t >> 6;
+    for (int s = 0; s < nsweep; s++) {
+        double mx = 0.0;
+        for (int color = 0; color < 2; color++) {
+#pragma unroll
+            for (int r = 0; r < TW * TW / 2 / TP_T; r++) {
+                const int pq = t + TP_T * r;
+                const int y = pq / (TW / 2), xp = 2 * (pq % (TW / 2));
+                // black iff (i + j) even: pick the pair's point of this colour
+                const int x = xp + (((i0 + xp + j0 + y) & 1) ^ color);
+                const int i = i0 + x, j = j0 + y;
+                if (i >= 2 && i <= nx && j >= 2 && j <= ny && x >= 1 && x <= TW - 2 && y >= 1 && y <= TW - 2) {
+                    const int c = y * TLD + x;
+                    const double a1 = RG[c - TLD], a2 = RA[c - 1], a4 = RA[c], a5 = RG[c];
+                    const double a3 = -a4 - a2 - a5 - a1;
+                    const double pij = P[c];
+                    double sum = B[c] - a1 * P[c - TLD] - a2 * P[c - 1] - a4 * P[c + 1] - a5 * P[c + TLD];
+                    sum = sum / a3 - pij;
+                    P[c] = pij + sorrel * sum;
+                    if (x >= TH && x < TH + TT && y >= TH && y < TH + TT) mx = dmaxf(mx, fabs(sum));
+                }
+            }
+            __syncthreads();
+        }
+        for (int off = 32; off > 0; off >>= 1) mx = dmaxf(mx, __shfl_xor(mx, off));
+        if (lane == 0) red[s][w] = mx;
+    }
+    // owned points out
+    for (int q = t; q < TT * TT; q += TP_T) {
+        const int x = TH + q % TT, y = TH + q / TT, i = i0 + x, j = j0 + y;
+        if (i <= nx && j <= ny) p_out[WD2_IX(i, j)] = P[y * TLD + x];
+    }
+    __syncthreads();
+    if (t < nsweep) {
+        double v = 0.0;
+        for (int k = 0; k < TP_T / 64; k++) v = dmaxf(v, red[t][k]);
+        part[(size_t)t * ntiles + tile] = v;
+    }
+}
+
+// Stopping test over the nsweep sweeps of one tiled launch that started after
+// sweep m0 (one workgroup).  Records the sweep count and, when the solve
+// stopped inside the launch, how many of its sweeps must be redone.
+__global__ __launch_bounds__(RB_T) void k_sor_tiled_check(const double *__restrict__ part, int ntiles, int nsweep,
+                                                          int m0, int launch, double sortol, int msorit, Scalars *sc) {
+    if (sc->sor_done) return;
+    __shared__ double sh[RB_T / 64];
+    for (int s = 0; s < nsweep; s++) {
+        const double d = fold_partials(part + (size_t)s * ntiles, ntiles, sh, threadIdx.x);
+        const int m = m0 + s + 1;
+        bool stop = false;
+        if (m > 1 && d < sortol) stop = true;
+        if (stop || m >= msorit) {
+            if (threadIdx.x == 0) {
+                sc->sor_done = 1;
+                sc->sor_nconv = stop ? m : 0;
+                sc->sor_dif = d;
+                sc->sor_m = m;
+                sc->redo_launch = launch;
+                sc->redo_sweeps = s + 1;
+            }
+            return;
+        }
+        if (threadIdx.x == 0) sc->sor_dif = d;
+    }
+}
+
+int sor_tiled_tiles(int nx, int ny, int *ntx) {
+    const int tx = (nx - 1 + TT - 1) / TT, ty = (ny - 1 + TT - 1) / TT;
+    if (ntx) *ntx = tx;
+    return tx * ty;
+}
+int sor_tiled_sweeps() { return KT; }
+
+void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
+                      const double *p_in, double *p_out, double *part, const Scalars *sc, int force,
+                      hipStream_t s) {
+    int ntx = 0;
+    const int nt = sor_tiled_tiles(nx, ny, &ntx);
+    const int grid = 8 * ((nt + 7) / 8);
+    hipLaunchKernelGGL(k_sor_tiled, dim3(grid), dim3(TP_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel, nsweep, p_in,
+                       p_out, part, ntx, nt, sc, force);
+}
+
+void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, double sortol, int msorit,
+                            const double *part, Scalars *sc, hipStream_t s) {
+    const int nt = sor_tiled_tiles(nx, ny, nullptr);
+    hipLaunchKernelGGL(k_sor_tiled_check, dim3(1), dim3(RB_T), 0, s, part, nt, nsweep, m0, launch, sortol, msorit,
+                       sc);
 }
 
 __global__ void k_sor_reset(Scalars *sc) {
@@ -124,6 +324,8 @@ __global__ void k_sor_reset(Scalars *sc) {
     sc->sor_m = 0;
     sc->sor_dif = 0.0;
     sc->val[6] = 0.0;
+    sc->redo_launch = 0;
+    sc->redo_sweeps = 0;
 }
 void launch_sor_reset(Scalars *sc, hipStream_t s) { hipLaunchKernelGGL(k_sor_reset, dim3(1), dim3(1), 0, s, sc); }
 

@@ -4,7 +4,7 @@
 //
 // All are HBM-bound one-pass kernels over a 2-D grid of 64x4 threads (x along
 // i, the contiguous direction).  Reductions write one partial per workgroup;
-// the last workgroup to finish (atomic ticket) folds the partials.
+// a one-workgroup kernel folds the partials.
 #include "wd2_dev.hpp"
 #include "wd2_launch.hpp"
 
@@ -13,20 +13,6 @@ namespace wd2 {
 constexpr int SX = 64, SY = 4;
 
 static inline dim3 grid2(int ni, int nj) { return dim3((ni + SX - 1) / SX, (nj + SY - 1) / SY); }
-
-// The last workgroup to arrive gets true; the ticket is returned to zero.
-__device__ inline bool last_block_arrives(unsigned int *counter, unsigned int nblocks, int tid) {
-    __shared__ bool amlast;
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned int t = atomicAdd(counter, 1u);
-        amlast = (t == nblocks - 1);
-        if (amlast) *counter = 0;
-    }
-    __syncthreads();
-    return amlast;
-}
 
 // ------------------------------------------------------------------ Project
 __global__ __launch_bounds__(SX *SY) void k_project(Regions R, int ld, double dk, Metrics m,
@@ -192,34 +178,41 @@ void launch_ppe_coef(int nx, int ny, int ld, const Metrics &m, const unsigned ch
 // ------------------------------------------------------------ max norms
 // DMaxNorm (b == null): max(|a(5,5)|, max_{2..nx-1, 2..ny-1} |a|)
 // DiffMaxNorm:          max(|a(2,2)-b(2,2)|, max_{2..nx-1, 2..ny-1} |a-b|)
+// Workgroups publish partial maxima; a one-workgroup kernel folds them with
+// the seed (the kernel boundary orders the two, no device-scope fence).
 __global__ __launch_bounds__(SX *SY) void k_max_norm(int nx, int ny, int ld, const double *__restrict__ a,
-                                                     const double *__restrict__ b, double *__restrict__ partials,
-                                                     Scalars *sc, int slot) {
+                                                     const double *__restrict__ b, double *__restrict__ partials) {
     __shared__ double sh[SX * SY / 64];
     const int tid = threadIdx.x + SX * threadIdx.y;
-    const int i = 2 + blockIdx.x * SX + threadIdx.x, j = 2 + blockIdx.y * SY + threadIdx.y;
     double mx = 0.0;
-    if (i <= nx - 1 && j <= ny - 1) mx = b ? fabs(a[WD2_IX(i, j)] - b[WD2_IX(i, j)]) : fabs(a[WD2_IX(i, j)]);
+    for (int j = 2 + blockIdx.y * SY + threadIdx.y; j <= ny - 1; j += gridDim.y * SY) {
+        const int i = 2 + blockIdx.x * SX + threadIdx.x;
+        if (i <= nx - 1) mx = dmaxf(mx, b ? fabs(a[WD2_IX(i, j)] - b[WD2_IX(i, j)]) : fabs(a[WD2_IX(i, j)]));
+    }
     mx = block_max<SX * SY>(mx, sh, tid);
-    const unsigned int nb = gridDim.x * gridDim.y, bid = blockIdx.x + gridDim.x * blockIdx.y;
-    if (tid == 0) partials[bid] = mx;
-    if (last_block_arrives(&sc->counter[slot], nb, tid)) {
-        double r = 0.0;
-        for (unsigned int k = tid; k < nb; k += SX * SY) r = dmaxf(r, ld_coherent(&partials[k]));
-        r = block_max<SX * SY>(r, sh, tid);
-        if (tid == 0) {
-            const double seed = b ? fabs(a[WD2_IX(2, 2)] - b[WD2_IX(2, 2)]) : fabs(a[WD2_IX(5, 5)]);
-            // Fortran max(diff, |x|) folded from the seed: the max of a set
-            // does not depend on the order of the folds.
-            sc->val[slot] = dmaxf(seed, r);
-        }
+    if (tid == 0) partials[blockIdx.x + gridDim.x * blockIdx.y] = mx;
+}
+
+__global__ __launch_bounds__(256) void k_max_norm_fold(int ld, const double *__restrict__ a,
+                                                       const double *__restrict__ b,
+                                                       const double *__restrict__ partials, int n, Scalars *sc,
+                                                       int slot) {
+    __shared__ double sh[4];
+    double r = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) r = dmaxf(r, partials[k]);
+    r = block_max<256>(r, sh, threadIdx.x);
+    if (threadIdx.x == 0) {
+        const double seed = b ? fabs(a[WD2_IX(2, 2)] - b[WD2_IX(2, 2)]) : fabs(a[WD2_IX(5, 5)]);
+        sc->val[slot] = dmaxf(seed, r);
     }
 }
 
 void launch_max_norm(int nx, int ny, int ld, const double *a, const double *b, double *partials, Scalars *sc,
                      int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_max_norm, grid2(nx - 2 > 0 ? nx - 2 : 1, ny - 2 > 0 ? ny - 2 : 1), dim3(SX, SY), 0, s, nx,
-                       ny, ld, a, b, partials, sc, slot);
+    const int gx = (max(nx - 2, 1) + SX - 1) / SX;
+    const int gy = min((max(ny - 2, 1) + SY - 1) / SY, max(1, 1024 / gx));
+    hipLaunchKernelGGL(k_max_norm, dim3(gx, gy), dim3(SX, SY), 0, s, nx, ny, ld, a, b, partials);
+    hipLaunchKernelGGL(k_max_norm_fold, dim3(1), dim3(256), 0, s, ld, a, b, partials, gx * gy, sc, slot);
 }
 
 // -------------------------------------------------------------- copies

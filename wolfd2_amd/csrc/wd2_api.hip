@@ -3,6 +3,7 @@
 // ABI of include/wolfd2_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -18,6 +19,7 @@ using namespace wd2;
 
 static thread_local std::string g_err;
 static int g_fallbacks = 0;
+static int g_tiled_sor = 1;   // temporally blocked SorRB when the grid allows it
 
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -61,7 +63,7 @@ struct wd2_ctx {
     Metrics M{};
     // fields
     double *store = nullptr;
-    double *u, *v, *p, *un, *vn, *pn, *us, *vs, *dus, *dvs, *d, *dn, *div, *rhs, *beff;
+    double *u, *v, *p, *un, *vn, *pn, *us, *vs, *dus, *dvs, *d, *dn, *div, *rhs, *beff, *pscr;
     double *pc[5];
     MomTemps T{};
     Chain ch{};
@@ -71,12 +73,17 @@ struct wd2_ctx {
     int *tile_flags = nullptr;
     int sweep_id = 0;
     bool coef_ready = false;
+    // optional HIP-event timing of the red-black SOR colour kernels
+    bool kt_on = false;
+    std::vector<hipEvent_t> kt_ev;
+    size_t kt_used = 0;
 };
 
 static inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
 static void ctx_free(wd2_ctx *c) {
     if (!c) return;
+    for (hipEvent_t e : c->kt_ev) (void)hipEventDestroy(e);
     (void)hipFree(c->d_int);
     (void)hipFree(c->d_dbl);
     (void)hipFree(c->prow);
@@ -98,7 +105,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->ld = round_up(nx + 2, 16);
     c->nf = (size_t)c->ld * (ny + 2);
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-    const int NFIELDS = 15 + 5 + 13;
+    const int NFIELDS = 16 + 5 + 13;
     const size_t nchain = (size_t)nx * ny + 64;
     const size_t total = NFIELDS * c->nf + 5 * nchain;
     HIPCHK(hipMalloc(&c->store, total * sizeof(double)));
@@ -106,7 +113,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     double *q = c->store;
     auto take = [&](size_t n) { double *r = q; q += n; return r; };
     double **f[] = {&c->u, &c->v, &c->p, &c->un, &c->vn, &c->pn, &c->us, &c->vs, &c->dus, &c->dvs,
-                    &c->d, &c->dn, &c->div, &c->rhs, &c->beff};
+                    &c->d, &c->dn, &c->div, &c->rhs, &c->beff, &c->pscr};
     for (double **pp : f) *pp = take(c->nf);
     for (int k = 0; k < 5; k++) c->pc[k] = take(c->nf);
     double **t[] = {&c->T.cj1, &c->T.cj2, &c->T.c1s, &c->T.c2s, &c->T.c1n, &c->T.c2n, &c->T.cnvs,
@@ -345,13 +352,72 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
     }
     launch_sor_reset(c->sc, s);
     const double *b = c->rhs;
-    const int batch = (solver == 2) ? 1 : 32;
+    if ((solver == 5 || solver == 6) && cart && !c->any_blockage && g_tiled_sor) {
+        // temporally blocked sweeps, ping-pong between p and the scratch field
+        const int K = sor_tiled_sweeps();
+        const int Q = 32;   // launches between two looks at the stop flag
+        launch_copy(ld, 0, nx + 1, 0, ny + 1, p, c->pscr, s);
+        int m0 = 0, L = 0, done = 0;
+        while (m0 < msorit && !done) {
+            for (int q = 0; q < Q && m0 < msorit; q++, L++) {
+                const int kr = std::min(K, msorit - m0);
+                const double *in = (L % 2 == 0) ? p : c->pscr;
+                double *out = (L % 2 == 0) ? c->pscr : p;
+                hipEvent_t *ev = nullptr;
+                if (c->kt_on) {
+                    while (c->kt_ev.size() < c->kt_used + 2) {
+                        hipEvent_t e;
+                        HIPCHK(hipEventCreate(&e));
+                        c->kt_ev.push_back(e);
+                    }
+                    ev = &c->kt_ev[c->kt_used];
+                    c->kt_used += 2;
+                    HIPCHK(hipEventRecord(ev[0], s));
+                }
+                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, kr, in, out, c->partials, c->sc, 0, s);
+                if (ev) HIPCHK(hipEventRecord(ev[1], s));
+                launch_sor_tiled_check(nx, ny, kr, m0, L, sortol, msorit, c->partials, c->sc, s);
+                m0 += kr;
+            }
+            HIPCHK(hipMemcpyAsync(&done, &c->sc->sor_done, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        Scalars h;
+        HIPCHK(hipMemcpy(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        const int Ld = h.redo_launch, keep = h.redo_sweeps;
+        const int krd = std::min(K, msorit - K * Ld);
+        const double *in = (Ld % 2 == 0) ? p : c->pscr;
+        double *out = (Ld % 2 == 0) ? c->pscr : p;
+        if (L > 0) {
+            if (keep < krd)   // stopped inside the launch: redo it with exactly `keep` sweeps
+                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, keep, in, out, c->partials, c->sc, 1, s);
+            if (out != p) launch_copy(ld, 0, nx + 1, 0, ny + 1, out, p, s);
+        }
+        int nconv = h.sor_nconv;
+        if (nconv < 1) {
+            fprintf(stdout, " Warning: SOR iterations did not converge after %6d iterations.\n", msorit);
+            nconv = msorit;
+        }
+        *nconv_out = nconv;
+        return WD2_OK;
+    }
+    const int batch = (solver == 2) ? 1 : 64;
     int done = 0;
     for (int it = 0; it < msorit && !done;) {
         for (int q = 0; q < batch && it < msorit; q++, it++) {
             if (solver == 5 || solver == 6) {
                 if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
-                launch_sor_rb_sweep(nx, ny, ld, c->M, coef, b, sorrel, sortol, msorit, p, c->partials, c->sc, s);
+                hipEvent_t *ev = nullptr;
+                if (c->kt_on) {
+                    while (c->kt_ev.size() < c->kt_used + 4) {
+                        hipEvent_t e;
+                        HIPCHK(hipEventCreate(&e));
+                        c->kt_ev.push_back(e);
+                    }
+                    ev = &c->kt_ev[c->kt_used];
+                    c->kt_used += 4;
+                }
+                launch_sor_rb_sweep(nx, ny, ld, c->M, coef, b, sorrel, sortol, it + 1, p, c->partials, c->sc, s, ev);
             } else if (solver == 1) {
                 if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
                 launch_sor_lex_sweep(nx, ny, ld, c->M, coef, b, sorrel, sortol, msorit, p, c->tile_flags,
@@ -380,6 +446,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
                 launch_slor_finish(0, sortol, msorit, nullptr, c->sc, s);
             }
         }
+        if (solver == 5 || solver == 6) launch_sor_rb_check(nx, ny, it, sortol, msorit, c->partials, c->sc, s);
         HIPCHK(hipMemcpyAsync(&done, &c->sc->sor_done, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -466,6 +533,14 @@ static int up_metric(int k, const double *h, int ld) { return up(g_ctx, h, ld, g
 extern "C" {
 
 const char *wd2_last_error(void) { return g_err.c_str(); }
+
+int wd2_sor_sweeps_per_launch(void) { return g_tiled_sor ? sor_tiled_sweeps() : 1; }
+
+int wd2_set_tiled_sor(int enable) {
+    const int old = g_tiled_sor;
+    g_tiled_sor = enable ? 1 : 0;
+    return old;
+}
 
 int wd2_trid_fallbacks(void) { return g_fallbacks + (g_ctx ? fallbacks(g_ctx) : 0); }
 
@@ -780,5 +855,27 @@ int wd2_ctx_device_fields(wd2_ctx *c, double **u, double **v, double **p, int *l
 }
 
 void *wd2_ctx_stream(wd2_ctx *c) { return (void *)c->s; }
+
+int wd2_ctx_kernel_timing(wd2_ctx *c, int enable, double *avg_us, long *launches) {
+    if (enable) {
+        c->kt_on = true;
+        c->kt_used = 0;
+        return WD2_OK;
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    double tot = 0.0;
+    long n = 0;
+    for (size_t k = 0; k + 1 < c->kt_used; k += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->kt_ev[k], c->kt_ev[k + 1]));
+        tot += ms;
+        n++;
+    }
+    c->kt_on = false;
+    c->kt_used = 0;
+    if (avg_us) *avg_us = n ? 1e3 * tot / n : 0.0;
+    if (launches) *launches = n;
+    return WD2_OK;
+}
 
 }  // extern "C"

@@ -30,7 +30,8 @@ struct Scalars {
     int sor_done, sor_nconv, sor_m, pad0;
     double sor_dif;
     int err;                       // bad-flag code from a kernel (0 = none)
-    int pad1[15];
+    int redo_launch, redo_sweeps;  // temporally blocked SOR: launch to redo, sweeps to keep
+    int pad1[13];
 };
 
 constexpr int RED_MAX_BLOCKS = 4096;   // partial-maximum slots per reduction
@@ -61,10 +62,26 @@ void launch_fill(int ld, int i0, int i1, int j0, int j1, double val, double *dst
 struct PpeCoef {
     const double *a1, *a2, *a3, *a4, *a5;
 };
+// Sweep number `sweep` (1-based) of SorRB/SorRBP; partials holds two sweeps'
+// worth of sor_rb_partials() maxima.  ev: null, or four events recorded around
+// the black and the red kernel.  launch_sor_rb_check evaluates the stopping
+// test of the last launched sweep.
+int sor_rb_partials(int nx, int ny);
 void launch_sor_rb_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
-                         double sorrel, double sortol, int msorit, double *p, double *partials, Scalars *sc,
-                         hipStream_t s);
+                         double sorrel, double sortol, int sweep, double *p, double *partials, Scalars *sc,
+                         hipStream_t s, hipEvent_t *ev);
+void launch_sor_rb_check(int nx, int ny, int sweep, double sortol, int msorit, const double *partials,
+                         Scalars *sc, hipStream_t s);
 void launch_sor_reset(Scalars *sc, hipStream_t s);
+// Temporally blocked SorRB/SorRBP (Cartesian, no blockage): nsweep <= sor_tiled_sweeps()
+// sweeps of p_in into p_out; part receives nsweep x tiles maxima.
+int sor_tiled_tiles(int nx, int ny, int *ntx);
+int sor_tiled_sweeps();
+void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
+                      const double *p_in, double *p_out, double *part, const Scalars *sc, int force,
+                      hipStream_t s);
+void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, double sortol, int msorit,
+                            const double *part, Scalars *sc, hipStream_t s);
 // Lexicographic SOR sweep (pressure.f:Sor), tiled wavefront.
 void launch_sor_lex_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
                           double sorrel, double sortol, int msorit, double *p, int *tile_flags, int sweep,
