@@ -92,6 +92,7 @@ def main() -> int:
                                    "run with --gpus 1", "n_gpus": args.gpus}))
         return 2
 
+    lib().wd2_set_verbose(0)      # stdout carries the JSON line only
     nx = ny = args.n + 1
     x, y = case.uniform_nodes(nx, ny)
     g = case.compute_grid(x, y)

@@ -45,6 +45,9 @@ int wd2_trid_fallbacks(void);
  * sweep.  Both give the reference's iterates bit for bit.  Returns the
  * previous setting. */
 int wd2_set_tiled_sor(int enable);
+/* Non-zero (default): print the reference's warnings on standard output
+ * (Ppe non-convergence, bad solver flag).  Returns the previous setting. */
+int wd2_set_verbose(int enable);
 /* Red-black SOR sweeps per launch of its dominant kernel (1 when not tiled). */
 int wd2_sor_sweeps_per_launch(void);
 

@@ -190,25 +190,78 @@ void launch_sor_rb_check(int nx, int ny, int sweep, double sortol, int msorit, c
 // after KT sweeps every owned point has seen exactly the values it sees in
 // the global sweep order, and its arithmetic is the reference's.  Points of
 // the outermost ring are never updated; points outside the array are never
-// read by an updated point.  p, the right-hand side, rau and rgv of the tile
-// sit in LDS (4 x 64 x 65 doubles); in each half-sweep the 1024 lanes take
-// one point of that colour from each of two horizontal pairs.
+// read by an updated point.
+//
+// Only p lives in LDS (64 x 65 doubles).  Each of the 512 lanes owns four
+// horizontal pairs of points, with their right-hand side and coefficients
+// (b, a1 = rgv(i,j-1), a2 = rau(i-1,j), a4 = rau(i,j), a5 = rgv(i,j)) in
+// registers; in each half-sweep a lane updates the point of that colour of
+// each of its pairs.
 //
 // The kernel reads p_in and writes the owned points to p_out (tiles read
-// their neighbours' points as halo, so it cannot work in place), and
-// publishes the max |s| of each of its sweeps over its owned points.
+// their neighbours' points as halo, so it cannot work in place).  The
+// stopping test of sweep m is max|s| < sortol over the whole grid
+// (pressure.f:534), which holds exactly when it holds on every tile: each tile
+// publishes a bit mask of its sweeps whose owned-point maximum is below
+// sortol, and the test of a launch is the AND of the masks.  Before any work,
+// every workgroup evaluates the previous launch's test; all reach the same
+// decision, so a solve that stopped is never advanced further.
 constexpr int KT = 4, TH = 2 * KT, TW = 64, TT = TW - 2 * TH;   // 48 owned
-constexpr int TP_T = 1024;
-constexpr int TLD = TW + 1;               // padded LDS row
+constexpr int TP_T = 512;
+constexpr int TP_PAIRS = TW * TW / 2 / TP_T;   // 4
+constexpr int TLD = TW + 1;                    // padded LDS row
+
+// AND of the ntiles masks (every thread gets it).
+__device__ inline unsigned tiled_mask(const unsigned *__restrict__ masks, int ntiles, unsigned *sh) {
+    unsigned m = ~0u;
+    for (int k = threadIdx.x; k < ntiles; k += blockDim.x) m &= masks[k];
+    for (int off = 32; off > 0; off >>= 1) m &= (unsigned)__shfl_xor((int)m, off);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    unsigned r = ~0u;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r &= sh[w];
+    __syncthreads();
+    return r;
+}
+
+// 1-based sweep of a launch (nsweep sweeps after sweep m0) after which the
+// solve stops, or 0; *conv tells whether it stopped converged.
+__device__ inline int tiled_stop(unsigned mask, int nsweep, int m0, int msorit, int *conv) {
+    for (int s = 0; s < nsweep; s++) {
+        const int m = m0 + s + 1;
+        const bool ok = m > 1 && ((mask >> s) & 1u);
+        if (ok || m >= msorit) {
+            *conv = ok ? 1 : 0;
+            return s + 1;
+        }
+    }
+    return 0;
+}
 
 __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, const double *__restrict__ rau,
                                                     const double *__restrict__ rgv, const double *__restrict__ b,
                                                     double sorrel, int nsweep, const double *__restrict__ p_in,
-                                                    double *__restrict__ p_out, double *__restrict__ part,
-                                                    int ntx, int ntiles, const Scalars *sc, int force) {
+                                                    double *__restrict__ p_out, unsigned *__restrict__ masks,
+                                                    const unsigned *__restrict__ prev_masks, int prev_nsweep,
+                                                    int prev_m0, int prev_launch, double sortol, int msorit,
+                                                    int ntx, int ntiles, Scalars *sc, int force) {
     if (!force && sc->sor_done) return;
-    __shared__ double P[TW * TLD], B[TW * TLD], RA[TW * TLD], RG[TW * TLD];
-    __shared__ double red[KT][TP_T / 64];
+    __shared__ double P[TW * TLD];
+    __shared__ unsigned sh[TP_T / 64];
+    if (!force && prev_nsweep > 0) {
+        int conv = 0;
+        const int k = tiled_stop(tiled_mask(prev_masks, ntiles, sh), prev_nsweep, prev_m0, msorit, &conv);
+        if (k > 0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                sc->sor_done = 1;
+                sc->sor_nconv = conv ? prev_m0 + k : 0;
+                sc->sor_m = prev_m0 + k;
+                sc->redo_launch = prev_launch;
+                sc->redo_sweeps = k;
+            }
+            return;
+        }
+    }
     // XCD-aware tile order: consecutive workgroups go to different XCDs, so
     // give each XCD a contiguous band of tiles (their halos then share an L2).
     const int per = (ntiles + 7) / 8;
@@ -217,80 +270,99 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
     const int TI = tile % ntx, TJ = tile / ntx;
     const int i0 = 2 + TI * TT - TH, j0 = 2 + TJ * TT - TH;   // tile origin
     const int t = threadIdx.x;
-    for (int q = t; q < TW * TW; q += TP_T) {
-        const int x = q % TW, y = q / TW, i = i0 + x, j = j0 + y;
-        const bool inarr = i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1;
-        const size_t k = inarr ? WD2_IX(i, j) : 0;
-        P[y * TLD + x] = inarr ? p_in[k] : 0.0;
-        B[y * TLD + x] = inarr ? b[k] : 0.0;
-        RA[y * TLD + x] = inarr ? rau[k] : 0.0;
-        RG[y * TLD + x] = inarr ? rgv[k] : 0.0;
+    double cb[TP_PAIRS][2], c1[TP_PAIRS][2], c2[TP_PAIRS][2], c4[TP_PAIRS][2], c5[TP_PAIRS][2];
+    int flags[TP_PAIRS];
+#define PX(r) (2 * ((t + TP_T * (r)) % (TW / 2)))
+#define PY(r) ((t + TP_T * (r)) / (TW / 2))
+#pragma unroll
+    for (int r = 0; r < TP_PAIRS; r++) {
+        const int x = PX(r), y = PY(r);
+        int f = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = i0 + x + h, j = j0 + y;
+            const bool inarr = i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1;
+            P[y * TLD + x + h] = inarr ? p_in[WD2_IX(i, j)] : 0.0;
+            const bool upd = i >= 2 && i <= nx && j >= 2 && j <= ny && x + h >= 1 && x + h <= TW - 2 && y >= 1 &&
+                             y <= TW - 2;
+            const bool own = upd && x + h >= TH && x + h < TH + TT && y >= TH && y < TH + TT;
+            double bb = 0, a1 = 0, a2 = 0, a4 = 0, a5 = 1;
+            if (upd) {
+                const size_t k = WD2_IX(i, j);
+                bb = b[k];
+                a1 = rgv[WD2_IX(i, j - 1)];
+                a2 = rau[WD2_IX(i - 1, j)];
+                a4 = rau[k];
+                a5 = rgv[k];
+            }
+            cb[r][h] = bb; c1[r][h] = a1; c2[r][h] = a2; c4[r][h] = a4; c5[r][h] = a5;
+            f |= (upd ? 1 : 0) << h;
+            f |= (own ? 1 : 0) << (2 + h);
+        }
+        f |= ((i0 + x + j0 + y) & 1) << 4;   // black iff (i + j) even
+        flags[r] = f;
     }
     __syncthreads();
-    const int lane = t & 63, w = t >> 6;
+    unsigned below = 0;   // bit s: this lane's owned max |s| of sweep s is < sortol
     for (int s = 0; s < nsweep; s++) {
         double mx = 0.0;
+#pragma unroll
         for (int color = 0; color < 2; color++) {
 #pragma unroll
-            for (int r = 0; r < TW * TW / 2 / TP_T; r++) {
-                const int pq = t + TP_T * r;
-                const int y = pq / (TW / 2), xp = 2 * (pq % (TW / 2));
-                // black iff (i + j) even: pick the pair's point of this colour
-                const int x = xp + (((i0 + xp + j0 + y) & 1) ^ color);
-                const int i = i0 + x, j = j0 + y;
-                if (i >= 2 && i <= nx && j >= 2 && j <= ny && x >= 1 && x <= TW - 2 && y >= 1 && y <= TW - 2) {
-                    const int c = y * TLD + x;
-                    const double a1 = RG[c - TLD], a2 = RA[c - 1], a4 = RA[c], a5 = RG[c];
+            for (int r = 0; r < TP_PAIRS; r++) {
+                const int h = ((flags[r] >> 4) & 1) ^ color;   // the pair's point of this colour
+                if ((flags[r] >> h) & 1) {
+                    const int c = PY(r) * TLD + PX(r) + h;
+                    // register selects (a runtime index into the arrays would go to scratch)
+                    const double a1 = h ? c1[r][1] : c1[r][0], a2 = h ? c2[r][1] : c2[r][0];
+                    const double a4 = h ? c4[r][1] : c4[r][0], a5 = h ? c5[r][1] : c5[r][0];
+                    const double bb = h ? cb[r][1] : cb[r][0];
                     const double a3 = -a4 - a2 - a5 - a1;
                     const double pij = P[c];
-                    double sum = B[c] - a1 * P[c - TLD] - a2 * P[c - 1] - a4 * P[c + 1] - a5 * P[c + TLD];
+                    double sum = bb - a1 * P[c - TLD] - a2 * P[c - 1] - a4 * P[c + 1] - a5 * P[c + TLD];
                     sum = sum / a3 - pij;
                     P[c] = pij + sorrel * sum;
-                    if (x >= TH && x < TH + TT && y >= TH && y < TH + TT) mx = dmaxf(mx, fabs(sum));
+                    if ((flags[r] >> (2 + h)) & 1) mx = dmaxf(mx, fabs(sum));
                 }
             }
             __syncthreads();
         }
-        for (int off = 32; off > 0; off >>= 1) mx = dmaxf(mx, __shfl_xor(mx, off));
-        if (lane == 0) red[s][w] = mx;
+        if (mx < sortol) below |= 1u << s;
     }
-    // owned points out
-    for (int q = t; q < TT * TT; q += TP_T) {
-        const int x = TH + q % TT, y = TH + q / TT, i = i0 + x, j = j0 + y;
-        if (i <= nx && j <= ny) p_out[WD2_IX(i, j)] = P[y * TLD + x];
-    }
+#pragma unroll
+    for (int r = 0; r < TP_PAIRS; r++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            if ((flags[r] >> (2 + h)) & 1) {
+                const int x = PX(r) + h, y = PY(r);
+                p_out[WD2_IX(i0 + x, j0 + y)] = P[y * TLD + x];
+            }
+#undef PX
+#undef PY
+    // tile mask = AND over lanes
+    for (int off = 32; off > 0; off >>= 1) below &= (unsigned)__shfl_xor((int)below, off);
+    if ((t & 63) == 0) sh[t >> 6] = below;
     __syncthreads();
-    if (t < nsweep) {
-        double v = 0.0;
-        for (int k = 0; k < TP_T / 64; k++) v = dmaxf(v, red[t][k]);
-        part[(size_t)t * ntiles + tile] = v;
+    if (t == 0) {
+        unsigned r = ~0u;
+        for (int w = 0; w < TP_T / 64; w++) r &= sh[w];
+        masks[tile] = r;
     }
 }
 
-// Stopping test over the nsweep sweeps of one tiled launch that started after
-// sweep m0 (one workgroup).  Records the sweep count and, when the solve
-// stopped inside the launch, how many of its sweeps must be redone.
-__global__ __launch_bounds__(RB_T) void k_sor_tiled_check(const double *__restrict__ part, int ntiles, int nsweep,
-                                                          int m0, int launch, double sortol, int msorit, Scalars *sc) {
+// Stopping test of the last launch of a batch (one workgroup).
+__global__ __launch_bounds__(RB_T) void k_sor_tiled_check(const unsigned *__restrict__ masks, int ntiles,
+                                                          int nsweep, int m0, int launch, int msorit, Scalars *sc) {
     if (sc->sor_done) return;
-    __shared__ double sh[RB_T / 64];
-    for (int s = 0; s < nsweep; s++) {
-        const double d = fold_partials(part + (size_t)s * ntiles, ntiles, sh, threadIdx.x);
-        const int m = m0 + s + 1;
-        bool stop = false;
-        if (m > 1 && d < sortol) stop = true;
-        if (stop || m >= msorit) {
-            if (threadIdx.x == 0) {
-                sc->sor_done = 1;
-                sc->sor_nconv = stop ? m : 0;
-                sc->sor_dif = d;
-                sc->sor_m = m;
-                sc->redo_launch = launch;
-                sc->redo_sweeps = s + 1;
-            }
-            return;
-        }
-        if (threadIdx.x == 0) sc->sor_dif = d;
+    __shared__ unsigned sh[RB_T / 64];
+    int conv = 0;
+    const int k = tiled_stop(tiled_mask(masks, ntiles, sh), nsweep, m0, msorit, &conv);
+    if (threadIdx.x == 0 && k > 0) {
+        sc->sor_done = 1;
+        sc->sor_nconv = conv ? m0 + k : 0;
+        sc->sor_m = m0 + k;
+        sc->redo_launch = launch;
+        sc->redo_sweeps = k;
     }
 }
 
@@ -302,20 +374,20 @@ int sor_tiled_tiles(int nx, int ny, int *ntx) {
 int sor_tiled_sweeps() { return KT; }
 
 void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
-                      const double *p_in, double *p_out, double *part, const Scalars *sc, int force,
-                      hipStream_t s) {
+                      const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
+                      int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
+                      int force, hipStream_t s) {
     int ntx = 0;
     const int nt = sor_tiled_tiles(nx, ny, &ntx);
     const int grid = 8 * ((nt + 7) / 8);
     hipLaunchKernelGGL(k_sor_tiled, dim3(grid), dim3(TP_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel, nsweep, p_in,
-                       p_out, part, ntx, nt, sc, force);
+                       p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit, ntx, nt, sc, force);
 }
 
-void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, double sortol, int msorit,
-                            const double *part, Scalars *sc, hipStream_t s) {
+void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, int msorit, const unsigned *masks,
+                            Scalars *sc, hipStream_t s) {
     const int nt = sor_tiled_tiles(nx, ny, nullptr);
-    hipLaunchKernelGGL(k_sor_tiled_check, dim3(1), dim3(RB_T), 0, s, part, nt, nsweep, m0, launch, sortol, msorit,
-                       sc);
+    hipLaunchKernelGGL(k_sor_tiled_check, dim3(1), dim3(RB_T), 0, s, masks, nt, nsweep, m0, launch, msorit, sc);
 }
 
 __global__ void k_sor_reset(Scalars *sc) {

@@ -20,6 +20,7 @@ using namespace wd2;
 static thread_local std::string g_err;
 static int g_fallbacks = 0;
 static int g_tiled_sor = 1;   // temporally blocked SorRB when the grid allows it
+static int g_verbose = 1;     // print the reference's warnings (pressure.f:242-246, 234)
 
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -334,7 +335,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
     hipStream_t s = c->s;
     if (solver < 1 || solver > 6) {
         // pressure.f:234-236 prints and leaves p untouched; Ppe then reports msorit
-        fprintf(stdout, " Wrong nPpeSolver flag passed to Ppe\n");
+        if (g_verbose) fprintf(stdout, " Wrong nPpeSolver flag passed to Ppe\n");
         *nconv_out = msorit;
         return WD2_OK;
     }
@@ -358,6 +359,9 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         const int Q = 32;   // launches between two looks at the stop flag
         launch_copy(ld, 0, nx + 1, 0, ny + 1, p, c->pscr, s);
         int m0 = 0, L = 0, done = 0;
+        const int nt = sor_tiled_tiles(nx, ny, nullptr);
+        unsigned *part[2] = {(unsigned *)c->partials, (unsigned *)c->partials + nt};
+        int prev_k = 0, prev_m0 = 0;
         while (m0 < msorit && !done) {
             for (int q = 0; q < Q && m0 < msorit; q++, L++) {
                 const int kr = std::min(K, msorit - m0);
@@ -374,11 +378,15 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
                     c->kt_used += 2;
                     HIPCHK(hipEventRecord(ev[0], s));
                 }
-                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, kr, in, out, c->partials, c->sc, 0, s);
+                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, kr, in, out, part[L % 2], part[(L + 1) % 2], prev_k,
+                                 prev_m0, L - 1, sortol, msorit, c->sc, 0, s);
                 if (ev) HIPCHK(hipEventRecord(ev[1], s));
-                launch_sor_tiled_check(nx, ny, kr, m0, L, sortol, msorit, c->partials, c->sc, s);
+                prev_k = kr;
+                prev_m0 = m0;
                 m0 += kr;
             }
+            // the stopping test of the batch's last launch
+            launch_sor_tiled_check(nx, ny, prev_k, prev_m0, L - 1, msorit, part[(L - 1) % 2], c->sc, s);
             HIPCHK(hipMemcpyAsync(&done, &c->sc->sor_done, sizeof(int), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
         }
@@ -390,12 +398,13 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         double *out = (Ld % 2 == 0) ? c->pscr : p;
         if (L > 0) {
             if (keep < krd)   // stopped inside the launch: redo it with exactly `keep` sweeps
-                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, keep, in, out, c->partials, c->sc, 1, s);
+                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, keep, in, out, part[Ld % 2], nullptr, 0, 0, 0,
+                                 sortol, msorit, c->sc, 1, s);
             if (out != p) launch_copy(ld, 0, nx + 1, 0, ny + 1, out, p, s);
         }
         int nconv = h.sor_nconv;
         if (nconv < 1) {
-            fprintf(stdout, " Warning: SOR iterations did not converge after %6d iterations.\n", msorit);
+            if (g_verbose) fprintf(stdout, " Warning: SOR iterations did not converge after %6d iterations.\n", msorit);
             nconv = msorit;
         }
         *nconv_out = nconv;
@@ -454,7 +463,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
     HIPCHK(hipMemcpyAsync(&nconv, &c->sc->sor_nconv, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (nconv < 1) {
-        fprintf(stdout, " Warning: SOR iterations did not converge after %6d iterations.\n", msorit);
+        if (g_verbose) fprintf(stdout, " Warning: SOR iterations did not converge after %6d iterations.\n", msorit);
         nconv = msorit;
     }
     *nconv_out = nconv;
@@ -535,6 +544,12 @@ extern "C" {
 const char *wd2_last_error(void) { return g_err.c_str(); }
 
 int wd2_sor_sweeps_per_launch(void) { return g_tiled_sor ? sor_tiled_sweeps() : 1; }
+
+int wd2_set_verbose(int enable) {
+    const int old = g_verbose;
+    g_verbose = enable ? 1 : 0;
+    return old;
+}
 
 int wd2_set_tiled_sor(int enable) {
     const int old = g_tiled_sor;

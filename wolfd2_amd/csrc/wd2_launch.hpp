@@ -74,14 +74,17 @@ void launch_sor_rb_check(int nx, int ny, int sweep, double sortol, int msorit, c
                          Scalars *sc, hipStream_t s);
 void launch_sor_reset(Scalars *sc, hipStream_t s);
 // Temporally blocked SorRB/SorRBP (Cartesian, no blockage): nsweep <= sor_tiled_sweeps()
-// sweeps of p_in into p_out; part receives nsweep x tiles maxima.
+// sweeps of p_in into p_out; masks receives one stopping-test bit mask per
+// tile.  The launch first applies the stopping test to the previous launch
+// (prev_*; prev_nsweep = 0 for none).
 int sor_tiled_tiles(int nx, int ny, int *ntx);
 int sor_tiled_sweeps();
 void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
-                      const double *p_in, double *p_out, double *part, const Scalars *sc, int force,
-                      hipStream_t s);
-void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, double sortol, int msorit,
-                            const double *part, Scalars *sc, hipStream_t s);
+                      const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
+                      int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
+                      int force, hipStream_t s);
+void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, int msorit, const unsigned *masks,
+                            Scalars *sc, hipStream_t s);
 // Lexicographic SOR sweep (pressure.f:Sor), tiled wavefront.
 void launch_sor_lex_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
                           double sorrel, double sortol, int msorit, double *p, int *tile_flags, int sweep,
