@@ -1,16 +1,25 @@
-"""Throughput of the wolfd2 time step on one MI355X.
+"""Throughput of the wolfd2 time step on one MI355X: mixed precision against FP64.
 
 Workload: lid-driven square cavity, 1024x1024 cells (1025^2 nodes), Re = 1000,
 dt = 2e-4, the reference's default tolerances (q-l 1e-4 / 20 iterations,
-Poisson 1e-8 / 2000 sweeps) with red-black SOR, FP64, fluid at rest with the
-lid at u = 1.  A step is one pass of main.f:690-972 over the whole grid:
+Poisson 1e-8 / 2000 sweeps) with red-black SOR, fluid at rest with the lid at
+u = 1.  A step is one pass of main.f:690-972 over the whole grid:
 nAuxMomentum (Douglas-Gunn ADI q-l iterations), the boundary fills, Ppe and
-Project.  The timed region starts after a cold start and W warm-up steps,
-with every field resident in HBM; it ends after K steps and a device sync.
+Project.
 
-Prints one JSON line (metric, value = Mcell-steps/s, roofline of the dominant
-kernel, and the CPU baseline: the reference program compiled at -O3 on one
-host core, timed on a one-step sample of the same cavity).
+Both arithmetic paths start from the same cold-start state: the FP64 path
+(the reference's arithmetic, bit for bit) and the mixed path (FP64 residual
+and pressure accumulation, FP32 red-black SOR sweeps).  Each timed region
+starts after W warm-up steps with every field resident in HBM and ends after
+K steps and a device sync.  value = the mixed path's Mcell-steps/s; the FP64
+path's throughput and the per-field relative L2 / max error of the mixed
+path's u, v, p against the FP64 path's after the same steps are reported
+beside it.
+
+Prints one JSON line: metric, value, roofline of the dominant kernel (the
+mixed path's tiled SOR kernel; the FP64 one under fp64_roofline), and the CPU
+baseline: the reference program compiled at -O3 on one host core, timed on a
+one-step sample of the same cavity.
 """
 from __future__ import annotations
 
@@ -79,6 +88,71 @@ def cpu_baseline(nx: int, ny: int, g, reg, prm, u0, v0, p0, timeout: float = 600
                        f"QlIt={int(tab[0, 2]) if len(tab) else -1}, PpeIt={int(tab[0, 3]) if len(tab) else -1}")}
 
 
+def copy_calibration(mib: int) -> int:
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst, src))
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 5, "ok": ok}))
+    return 0 if ok else 1
+
+
+def _traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        return json.load(open(path)).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def run_path(sol, precision: str, fields0, warmup: int, steps: int, cells: int) -> dict:
+    """W warm-up steps and K timed steps of one arithmetic path from the same
+    state, then K more steps with the dominant kernel's launches timed by HIP
+    events on the context's stream."""
+    import ctypes as C
+    L = lib()
+    sol.set_precision(precision)
+    sol.set_fields(*fields0)
+    if warmup:
+        sol.step(warmup)
+    t0 = time.perf_counter()
+    tab = sol.step(steps)                 # returns after a device synchronisation
+    secs = time.perf_counter() - t0
+    fields = sol.get_fields()
+    avg_us, nl = C.c_double(0), C.c_long(0)
+    L.wd2_ctx_kernel_timing(sol.ctx, 1, None, None)
+    sol.step(steps)
+    L.wd2_ctx_kernel_timing(sol.ctx, 0, C.byref(avg_us), C.byref(nl))
+    if precision == "mixed":
+        # one launch: FP64 defect correction (read p and b, write p: 24 B per
+        # cell) around k FP32 sweeps of the correction (read e and r, write e:
+        # 12 B per cell and sweep)
+        k = L.wd2_sor_mixed_sweeps_per_launch()
+        bpl = (24.0 + 12.0 * k) * cells
+        kern = f"k_sor_tiled_mx ({k} FP32 red-black SOR sweeps per launch on an FP64 residual)"
+        traffic = _traffic("k_sor_tiled_mx_cavity1024_traffic.json")
+    else:
+        # a sweep reads p and the right-hand side and writes p: 24 B per cell
+        k = L.wd2_sor_sweeps_per_launch()
+        bpl = (24.0 * k if k > 1 else 12.0) * cells
+        kern = (f"k_sor_tiled ({k} red-black SOR sweeps per launch)" if k > 1
+                else "k_sor_rb (one colour of a sweep)")
+        traffic = _traffic("k_sor_tiled_cavity1024_traffic.json")
+    achieved = bpl / (avg_us.value * 1e-6) / 1e9 if avg_us.value > 0 else 0.0
+    return {"value": cells * steps / secs / 1e6, "ms_per_step": 1e3 * secs / steps, "fields": fields,
+            "ql": float(np.mean(tab[:, 0])), "ppe": float(np.mean(tab[:, 1])),
+            "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                         "avg_launch_us": round(avg_us.value, 3), "launches": int(nl.value),
+                         "bytes_per_launch": bpl}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,7 +160,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1024, help="cells per side")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the reference kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only (the hot path is not sharded); "
                                    "run with --gpus 1", "n_gpus": args.gpus}))
@@ -102,73 +181,52 @@ def main() -> int:
     z = np.zeros((g.ncol, g.ld))
     sol.set_fields(z, z, z)
     sol.cold_start()
-    u0, v0, p0 = sol.get_fields()
-    if args.warmup:
-        sol.step(args.warmup)
-    t0 = time.perf_counter()
-    tab = sol.step(args.steps)            # returns after a device synchronisation
-    t1 = time.perf_counter()
-    secs = t1 - t0
+    fields0 = sol.get_fields()
     cells = (nx - 1) * (ny - 1)
-    value = cells * args.steps / secs / 1e6
-
-    # dominant kernel: the red-black SOR kernel (k_sor_tiled, several sweeps per
-    # launch on LDS tiles).  Mean launch duration from HIP events on the
-    # context's stream over a second timed region of the same length.
-    L = lib()
-    import ctypes as C
-    sweeps_per_launch = L.wd2_sor_sweeps_per_launch()
-    avg_us, nl = C.c_double(0), C.c_long(0)
-    L.wd2_ctx_kernel_timing(sol.ctx, 1, None, None)
-    tab2 = sol.step(args.steps)
-    L.wd2_ctx_kernel_timing(sol.ctx, 0, C.byref(avg_us), C.byref(nl))
-    # algorithmic bytes per launch: a sweep reads p and the right-hand side and
-    # writes p once per cell, 3 x 8 B per cell-sweep, times the sweeps per
-    # launch (one colour kernel of the untiled path is half a sweep: 12 B)
-    bytes_per_launch = (24.0 * sweeps_per_launch if sweeps_per_launch > 1 else 12.0) * cells
-    achieved = bytes_per_launch / (avg_us.value * 1e-6) / 1e9 if avg_us.value > 0 else 0.0
-    # roofline traffic from rocprofv3 counters (FETCH_SIZE + WRITE_SIZE per
-    # dispatch, scaled by the copy-kernel calibration): see profiles/
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "sor_rb_1024_traffic.json")
-    if os.path.exists(prof):
-        try:
-            traffic = json.load(open(prof)).get("bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
-
+    res = {prec: run_path(sol, prec, fields0, args.warmup, args.steps, cells) for prec in ("fp64", "mixed")}
+    mx, f64 = res["mixed"], res["fp64"]
+    S = np.s_[: ny + 2, : nx + 2]
+    err = {}
+    for name, a, b in zip("uvp", mx["fields"], f64["fields"]):
+        a, b = a[S], b[S]
+        nb = float(np.linalg.norm(b))
+        err[name] = {"rel_l2": float(np.linalg.norm(a - b)) / (nb if nb > 0 else 1.0),
+                     "max_abs": float(np.max(np.abs(a - b)))}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(nx, ny, g, reg, prm, u0, v0, p0)
+        cpu = cpu_baseline(nx, ny, g, reg, prm, *fields0)
     sol.close()
 
     out = {
         "metric": METRIC,
-        "value": round(value, 4),
+        "value": round(mx["value"], 4),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * secs / args.steps, 4),
+        "ms_per_step": round(mx["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic (cold-start lid-driven cavity)",
         "config": {"workload": "lid-driven cavity 1024x1024 cells, Re=1000, dt=2e-4, rb_sor, reference "
-                               "tolerances (qtol 1e-4/20, sortol 1e-8/2000), FP64 bit-exact path",
-                   "cells": cells, "path": "fp64",
-                   "ql_iter_mean": float(np.mean(tab[:, 0])), "ppe_iter_mean": float(np.mean(tab[:, 1]))},
-        "roofline": {"bound": "hbm", "kernel": (f"k_sor_tiled ({sweeps_per_launch} red-black SOR sweeps per launch)"
-                                                if sweeps_per_launch > 1 else "k_sor_rb (one colour of a sweep)"),
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "avg_launch_us": round(avg_us.value, 3), "launches": int(nl.value),
-                     "bytes_per_launch": bytes_per_launch},
+                               "tolerances (qtol 1e-4/20, sortol 1e-8/2000); mixed path: FP64 residual and "
+                               "pressure accumulation with FP32 red-black SOR sweeps, FP64 momentum",
+                   "cells": cells, "path": "mixed",
+                   "fp64_value": round(f64["value"], 4), "fp64_ms_per_step": round(f64["ms_per_step"], 4),
+                   "speedup_vs_fp64": round(mx["value"] / f64["value"], 3),
+                   "error_vs_fp64": err,
+                   "within_1e-6_l2": all(e["rel_l2"] <= 1e-6 for e in err.values()),
+                   "ql_iter_mean": mx["ql"], "ppe_iter_mean": mx["ppe"],
+                   "fp64_ql_iter_mean": f64["ql"], "fp64_ppe_iter_mean": f64["ppe"]},
+        "roofline": mx["roofline"],
+        "fp64_roofline": f64["roofline"],
         "cpu_baseline": cpu,
     }
     if cpu:
-        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        out["gpu_over_cpu"] = round(mx["value"] / cpu["value"], 1)
+        out["fp64_gpu_over_cpu"] = round(f64["value"] / cpu["value"], 1)
     print(json.dumps(out))
     return 0
 

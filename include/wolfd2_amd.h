@@ -48,8 +48,10 @@ int wd2_set_tiled_sor(int enable);
 /* Non-zero (default): print the reference's warnings on standard output
  * (Ppe non-convergence, bad solver flag).  Returns the previous setting. */
 int wd2_set_verbose(int enable);
-/* Red-black SOR sweeps per launch of its dominant kernel (1 when not tiled). */
+/* Red-black SOR sweeps per launch of its dominant kernel (1 when not tiled),
+ * FP64 path and mixed-precision path. */
 int wd2_sor_sweeps_per_launch(void);
+int wd2_sor_mixed_sweeps_per_launch(void);
 
 /* momentum.f:nAuxMomentum 33-48.  *nQLiter receives the function value: the
  * number of q-l iterations, or -1 when not converged in mqiter iterations. */
@@ -169,13 +171,25 @@ int wd2_ctx_step(wd2_ctx *ctx, const wd2_params *prm, int nsteps, int *ql_iter, 
 int wd2_ctx_device_fields(wd2_ctx *ctx, double **u, double **v, double **p, int *ld);
 /* The HIP stream the context runs on. */
 void *wd2_ctx_stream(wd2_ctx *ctx);
-/* Timing of the red-black SOR kernels (the tiled multi-sweep kernel, or the
- * colour kernels) with HIP events on the context's stream: enable=1 starts
- * recording an event pair around every launch,
- * enable=0 stops and returns the mean launch duration (microseconds, while
- * the kernel occupies the stream, excluding launch gaps) and the launch count.
- * Sweeps of a solve that has already converged are early-exit launches and
- * are counted too. */
+
+/* Arithmetic of the context's time steps (no counterpart in the reference).
+ * WD2_PREC_FP64 (default): the reference's FP64 arithmetic, bit for bit.
+ * WD2_PREC_MIXED: the red-black SOR Poisson solve (nPpeSolver 5/6 on a
+ * Cartesian grid without blockages) runs as defect correction: FP64 residual
+ * and pressure accumulation, FP32 relaxation sweeps.  It follows the same
+ * iteration (sweep order, stopping test, iteration cap) as the FP64 path, and
+ * its iterates differ from FP64 by the rounding of the FP32 corrections.
+ * Other solvers and grids keep FP64. */
+#define WD2_PREC_FP64 0
+#define WD2_PREC_MIXED 1
+int wd2_ctx_set_precision(wd2_ctx *ctx, int precision);
+/* Timing of the red-black SOR kernels with HIP events on the context's
+ * stream.  enable=1 starts recording: one event pair around each batch of
+ * back-to-back launches of the tiled multi-sweep kernel, or around each
+ * colour kernel of the untiled path.  enable=0 stops and returns the mean
+ * launch duration (microseconds: the bracketed time over the launches it
+ * holds) and the launch count.  Launches after a solve has converged exit
+ * early and are counted too. */
 int wd2_ctx_kernel_timing(wd2_ctx *ctx, int enable, double *avg_us, long *launches);
 
 #ifdef __cplusplus

@@ -1,0 +1,117 @@
+"""Mixed-precision path (FP64 residual and pressure accumulation, FP32
+red-black SOR relaxation) against the FP64 path on the same inputs.
+
+Bar: the relative L2 error of u, v and p against the FP64 path is at most
+1e-6 per field after the steps run (the tolerance of the mixed-precision
+target), the per-step iteration counts agree, and a tightly converged Poisson
+solve reaches the FP64 solution (the residual that drives the FP32 sweeps is
+formed in FP64, so both share one fixed point).
+"""
+import numpy as np
+import pytest
+
+import helpers
+from wolfd2_amd import api, case
+
+pytestmark = pytest.mark.gpu
+
+TOL_L2 = 1e-6
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _run(g, reg, prm, precision, nsteps, fields=None):
+    sol = api.Solver(g, reg, prm)
+    sol.set_precision(precision)
+    if fields is None:
+        z = np.zeros((g.ncol, g.ld))
+        sol.set_fields(z, z, z)
+        sol.cold_start()
+    else:
+        sol.set_fields(*fields)
+    tab = sol.step(nsteps)
+    out = sol.get_fields()
+    sol.close()
+    return tab, out
+
+
+def _cavity(n, re, dt, **kw):
+    nx = ny = n + 1
+    x, y = case.uniform_nodes(nx, ny)
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(nx, ny)
+    prm = case.Params.from_physical(dt, 1.0, 1.0 / re, nPpeSolver=5, **kw)
+    return g, reg, prm
+
+
+@pytest.mark.parametrize("n,re,dt,steps", [(128, 1000.0, 1e-3, 40), (64, 100.0, 1e-3, 100), (200, 400.0, 5e-4, 10)])
+def test_mixed_follows_fp64_cavity(n, re, dt, steps):
+    g, reg, prm = _cavity(n, re, dt)
+    print(f"mixed vs fp64 cavity {n}^2 Re={re} steps={steps}", flush=True)
+    t64, f64 = _run(g, reg, prm, "fp64", steps)
+    tmx, fmx = _run(g, reg, prm, "mixed", steps)
+    assert np.array_equal(t64[:, 0], tmx[:, 0]), "q-l iteration counts differ"
+    # the Poisson sweep counts may differ by a sweep where max|s| sits at sortol
+    assert np.all(np.abs(t64[:, 1] - tmx[:, 1]) <= 1)
+    S = np.s_[: g.ny + 2, : g.nx + 2]
+    for name, a, b in zip("uvp", fmx, f64):
+        e = rel_l2(a[S], b[S])
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= TOL_L2, name
+
+
+def test_mixed_follows_fp64_channel_fixture():
+    """Channel with inlet and mass-conserving outlet (the run fixture's deck)."""
+    fx = helpers.load("run_chan_64x16.npz")
+    g, reg, prm = helpers.run_case(fx)
+    steps = 50
+    t64, f64 = _run(g, reg, prm, "fp64", steps)
+    tmx, fmx = _run(g, reg, prm, "mixed", steps)
+    assert np.array_equal(t64[:, 0], tmx[:, 0])
+    S = np.s_[: g.ny + 2, : g.nx + 2]
+    for name, a, b in zip("uvp", fmx, f64):
+        assert rel_l2(a[S], b[S]) <= TOL_L2, name
+
+
+def test_mixed_poisson_reaches_fp64_solution():
+    """A Poisson solve driven to a tight tolerance (the cold start: VelBoundCond,
+    Ppe, PresBoundCond, Project, VelBoundCond) from a random velocity field:
+    both precisions stop below the cap and agree far below the FP32 rounding."""
+    g, reg, prm = _cavity(96, 100.0, 1e-3, msorit=200000, sortol=1e-13, sorrel=1.9)
+    rng = np.random.default_rng(7)
+    u, v, p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
+    S = np.s_[1: g.ny + 1, 1: g.nx + 1]
+    u[S] = rng.standard_normal(u[S].shape) * 0.1
+    v[S] = rng.standard_normal(v[S].shape) * 0.1
+    n, out = {}, {}
+    for prec in ("fp64", "mixed"):
+        sol = api.Solver(g, reg, prm)
+        sol.set_precision(prec)
+        sol.set_fields(u, v, p)
+        n[prec] = sol.cold_start()
+        out[prec] = sol.get_fields()
+        sol.close()
+    print(f"sweeps fp64={n['fp64']} mixed={n['mixed']}", flush=True)
+    assert n["fp64"] < prm.msorit and n["mixed"] < prm.msorit
+    assert abs(n["fp64"] - n["mixed"]) <= 8
+    P = np.s_[2: g.ny + 1, 2: g.nx + 1]
+    e = rel_l2(out["mixed"][2][P], out["fp64"][2][P])
+    print(f"p rel L2 {e:.3e}", flush=True)
+    assert e <= 1e-9
+
+
+def test_mixed_full_size_cavity_steps():
+    """1024^2 cells, reference tolerances (2000-sweep cap): two steps."""
+    g, reg, prm = _cavity(1024, 1000.0, 2e-4)
+    print("mixed vs fp64 cavity 1024^2, 2 steps", flush=True)
+    t64, f64 = _run(g, reg, prm, "fp64", 2)
+    tmx, fmx = _run(g, reg, prm, "mixed", 2)
+    assert np.array_equal(t64[:, 0], tmx[:, 0])
+    S = np.s_[: g.ny + 2, : g.nx + 2]
+    for name, a, b in zip("uvp", fmx, f64):
+        e = rel_l2(a[S], b[S])
+        print(f"  {name}: rel L2 {e:.3e}", flush=True)
+        assert e <= TOL_L2, name

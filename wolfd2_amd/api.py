@@ -161,6 +161,11 @@ class Solver:
         self.prm = prm
         self.params = make_params(prm)
 
+    def set_precision(self, precision: str):
+        """"fp64" (the reference's arithmetic, bit for bit) or "mixed" (FP64
+        residual and accumulation, FP32 relaxation in the red-black SOR)."""
+        check(lib().wd2_ctx_set_precision(self.ctx, {"fp64": 0, "mixed": 1}[precision]))
+
     def set_fields(self, u, v, p):
         check(lib().wd2_ctx_set_fields(self.ctx, self.g.ld, dp(u), dp(v), dp(p)))
 

@@ -1,8 +1,12 @@
 // k_ppe.hip -- the pressure Poisson relaxations of pressure.f (Ppe 30-249).
 //
-//   SorRB / SorRBP (pressure.f:457-656): two colour kernels per sweep.  Points of
-//     one colour only read points of the other, so every point's arithmetic is
-//     the reference's regardless of order; the unrelaxed |s| is max-reduced.
+//   SorRB / SorRBP (pressure.f:457-656), Cartesian grid without blockages:
+//     k_sor_tiled runs KT sweeps per launch on LDS tiles (temporal blocking,
+//     below), bit for bit the reference's iterates.
+//   SorRB / SorRBP otherwise: two colour kernels per sweep.  Points of one
+//     colour only read points of the other, so every point's arithmetic is
+//     the reference's regardless of order; the unrelaxed |s| is max-reduced
+//     through per-workgroup partials that the next sweep's kernel folds.
 //   Sor (pressure.f:384-450): lexicographic Gauss-Seidel.  A point needs its
 //     west and south neighbours of this sweep and its east and north neighbours
 //     of the last one, so 64x64 tiles run in wavefront order (each workgroup
@@ -12,12 +16,13 @@
 //   Slor / SlorRB / SlorRBP (pressure.f:673-1138): each j-line is an (nx-1)-row
 //     AltTridLU; the lines of one colour are solved as one batch of chains.
 //
-// Sweep control stays on the device: the last workgroup of a sweep folds the
-// maxima, counts the sweep and raises sor_done when m > 1 and dif < sortol
-// (pressure.f:443) or m reaches msorit; every later kernel of the solve returns
-// at entry, so the host only has to look at the flag every few dozen sweeps.
+// Sweep control stays on the device: the stopping test of a sweep (m > 1 and
+// dif < sortol, pressure.f:443/534, or m = msorit) is evaluated on the device
+// and raises sor_done; every later kernel of the solve returns at entry, so
+// the host only looks at the flag every few dozen sweeps.
 #include "wd2_dev.hpp"
 #include "wd2_launch.hpp"
+#include "wd2_sor.hpp"
 
 namespace wd2 {
 
@@ -211,33 +216,6 @@ constexpr int TP_T = 512;
 constexpr int TP_PAIRS = TW * TW / 2 / TP_T;   // 4
 constexpr int TLD = TW + 1;                    // padded LDS row
 
-// AND of the ntiles masks (every thread gets it).
-__device__ inline unsigned tiled_mask(const unsigned *__restrict__ masks, int ntiles, unsigned *sh) {
-    unsigned m = ~0u;
-    for (int k = threadIdx.x; k < ntiles; k += blockDim.x) m &= masks[k];
-    for (int off = 32; off > 0; off >>= 1) m &= (unsigned)__shfl_xor((int)m, off);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-    __syncthreads();
-    unsigned r = ~0u;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r &= sh[w];
-    __syncthreads();
-    return r;
-}
-
-// 1-based sweep of a launch (nsweep sweeps after sweep m0) after which the
-// solve stops, or 0; *conv tells whether it stopped converged.
-__device__ inline int tiled_stop(unsigned mask, int nsweep, int m0, int msorit, int *conv) {
-    for (int s = 0; s < nsweep; s++) {
-        const int m = m0 + s + 1;
-        const bool ok = m > 1 && ((mask >> s) & 1u);
-        if (ok || m >= msorit) {
-            *conv = ok ? 1 : 0;
-            return s + 1;
-        }
-    }
-    return 0;
-}
-
 __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, const double *__restrict__ rau,
                                                     const double *__restrict__ rgv, const double *__restrict__ b,
                                                     double sorrel, int nsweep, const double *__restrict__ p_in,
@@ -384,10 +362,9 @@ void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b,
                        p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit, ntx, nt, sc, force);
 }
 
-void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, int msorit, const unsigned *masks,
+void launch_sor_tiled_check(int ntiles, int nsweep, int m0, int launch, int msorit, const unsigned *masks,
                             Scalars *sc, hipStream_t s) {
-    const int nt = sor_tiled_tiles(nx, ny, nullptr);
-    hipLaunchKernelGGL(k_sor_tiled_check, dim3(1), dim3(RB_T), 0, s, masks, nt, nsweep, m0, launch, msorit, sc);
+    hipLaunchKernelGGL(k_sor_tiled_check, dim3(1), dim3(RB_T), 0, s, masks, ntiles, nsweep, m0, launch, msorit, sc);
 }
 
 __global__ void k_sor_reset(Scalars *sc) {

@@ -74,11 +74,28 @@ struct wd2_ctx {
     int *tile_flags = nullptr;
     int sweep_id = 0;
     bool coef_ready = false;
-    // optional HIP-event timing of the red-black SOR colour kernels
+    int precision = WD2_PREC_FP64;
+    // optional HIP-event timing of the red-black SOR kernels: event pairs,
+    // each around kt_cnt[pair] back-to-back launches
     bool kt_on = false;
     std::vector<hipEvent_t> kt_ev;
+    std::vector<int> kt_cnt;
     size_t kt_used = 0;
 };
+
+// npairs new event pairs for the kernel timing (nullptr when it is off)
+static hipEvent_t *kt_events(wd2_ctx *c, int npairs) {
+    if (!c->kt_on) return nullptr;
+    while (c->kt_ev.size() < c->kt_used + 2 * npairs) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->kt_ev.push_back(e);
+    }
+    hipEvent_t *ev = &c->kt_ev[c->kt_used];
+    c->kt_used += 2 * npairs;
+    for (int k = 0; k < npairs; k++) c->kt_cnt.push_back(1);
+    return ev;
+}
 
 static inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
@@ -353,40 +370,40 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
     }
     launch_sor_reset(c->sc, s);
     const double *b = c->rhs;
-    if ((solver == 5 || solver == 6) && cart && !c->any_blockage && g_tiled_sor) {
-        // temporally blocked sweeps, ping-pong between p and the scratch field
-        const int K = sor_tiled_sweeps();
-        const int Q = 32;   // launches between two looks at the stop flag
+    if ((solver == 5 || solver == 6) && cart && !c->any_blockage && (g_tiled_sor || c->precision == WD2_PREC_MIXED)) {
+        // temporally blocked sweeps, ping-pong between p and the scratch field;
+        // mixed precision: FP64 residual + FP32 sweeps + FP64 accumulation
+        const bool mx = c->precision == WD2_PREC_MIXED;
+        auto sweep = mx ? launch_sor_tiled_mx : launch_sor_tiled;
+        const int K = mx ? sor_tiled_mx_sweeps() : sor_tiled_sweeps();
+        const int Q = mx ? 16 : 32;   // launches between two looks at the stop flag
         launch_copy(ld, 0, nx + 1, 0, ny + 1, p, c->pscr, s);
         int m0 = 0, L = 0, done = 0;
-        const int nt = sor_tiled_tiles(nx, ny, nullptr);
+        const int nt = mx ? sor_tiled_mx_tiles(nx, ny, nullptr) : sor_tiled_tiles(nx, ny, nullptr);
         unsigned *part[2] = {(unsigned *)c->partials, (unsigned *)c->partials + nt};
         int prev_k = 0, prev_m0 = 0;
         while (m0 < msorit && !done) {
-            for (int q = 0; q < Q && m0 < msorit; q++, L++) {
+            // timing: one event pair around the batch's launches (no marker
+            // between two launches, so their durations are not inflated)
+            hipEvent_t *ev = kt_events(c, 1);
+            if (ev) HIPCHK(hipEventRecord(ev[0], s));
+            int nb = 0;
+            for (int q = 0; q < Q && m0 < msorit; q++, L++, nb++) {
                 const int kr = std::min(K, msorit - m0);
                 const double *in = (L % 2 == 0) ? p : c->pscr;
                 double *out = (L % 2 == 0) ? c->pscr : p;
-                hipEvent_t *ev = nullptr;
-                if (c->kt_on) {
-                    while (c->kt_ev.size() < c->kt_used + 2) {
-                        hipEvent_t e;
-                        HIPCHK(hipEventCreate(&e));
-                        c->kt_ev.push_back(e);
-                    }
-                    ev = &c->kt_ev[c->kt_used];
-                    c->kt_used += 2;
-                    HIPCHK(hipEventRecord(ev[0], s));
-                }
-                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, kr, in, out, part[L % 2], part[(L + 1) % 2], prev_k,
-                                 prev_m0, L - 1, sortol, msorit, c->sc, 0, s);
-                if (ev) HIPCHK(hipEventRecord(ev[1], s));
+                sweep(nx, ny, ld, c->M, b, sorrel, kr, in, out, part[L % 2], part[(L + 1) % 2], prev_k, prev_m0,
+                      L - 1, sortol, msorit, c->sc, 0, s);
                 prev_k = kr;
                 prev_m0 = m0;
                 m0 += kr;
             }
+            if (ev) {
+                HIPCHK(hipEventRecord(ev[1], s));
+                c->kt_cnt.back() = nb;
+            }
             // the stopping test of the batch's last launch
-            launch_sor_tiled_check(nx, ny, prev_k, prev_m0, L - 1, msorit, part[(L - 1) % 2], c->sc, s);
+            launch_sor_tiled_check(nt, prev_k, prev_m0, L - 1, msorit, part[(L - 1) % 2], c->sc, s);
             HIPCHK(hipMemcpyAsync(&done, &c->sc->sor_done, sizeof(int), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
         }
@@ -398,8 +415,8 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         double *out = (Ld % 2 == 0) ? c->pscr : p;
         if (L > 0) {
             if (keep < krd)   // stopped inside the launch: redo it with exactly `keep` sweeps
-                launch_sor_tiled(nx, ny, ld, c->M, b, sorrel, keep, in, out, part[Ld % 2], nullptr, 0, 0, 0,
-                                 sortol, msorit, c->sc, 1, s);
+                sweep(nx, ny, ld, c->M, b, sorrel, keep, in, out, part[Ld % 2], nullptr, 0, 0, 0, sortol, msorit,
+                      c->sc, 1, s);
             if (out != p) launch_copy(ld, 0, nx + 1, 0, ny + 1, out, p, s);
         }
         int nconv = h.sor_nconv;
@@ -416,16 +433,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         for (int q = 0; q < batch && it < msorit; q++, it++) {
             if (solver == 5 || solver == 6) {
                 if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
-                hipEvent_t *ev = nullptr;
-                if (c->kt_on) {
-                    while (c->kt_ev.size() < c->kt_used + 4) {
-                        hipEvent_t e;
-                        HIPCHK(hipEventCreate(&e));
-                        c->kt_ev.push_back(e);
-                    }
-                    ev = &c->kt_ev[c->kt_used];
-                    c->kt_used += 4;
-                }
+                hipEvent_t *ev = kt_events(c, 2);
                 launch_sor_rb_sweep(nx, ny, ld, c->M, coef, b, sorrel, sortol, it + 1, p, c->partials, c->sc, s, ev);
             } else if (solver == 1) {
                 if (!cart) { launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s); b = c->beff; }
@@ -544,6 +552,7 @@ extern "C" {
 const char *wd2_last_error(void) { return g_err.c_str(); }
 
 int wd2_sor_sweeps_per_launch(void) { return g_tiled_sor ? sor_tiled_sweeps() : 1; }
+int wd2_sor_mixed_sweeps_per_launch(void) { return sor_tiled_mx_sweeps(); }
 
 int wd2_set_verbose(int enable) {
     const int old = g_verbose;
@@ -871,10 +880,19 @@ int wd2_ctx_device_fields(wd2_ctx *c, double **u, double **v, double **p, int *l
 
 void *wd2_ctx_stream(wd2_ctx *c) { return (void *)c->s; }
 
+int wd2_ctx_set_precision(wd2_ctx *c, int precision) {
+    if (!c) return fail(WD2_ERR_ARG, "wd2_ctx_set_precision: null context");
+    if (precision != WD2_PREC_FP64 && precision != WD2_PREC_MIXED)
+        return fail(WD2_ERR_ARG, "wd2_ctx_set_precision: unknown precision %d", precision);
+    c->precision = precision;
+    return WD2_OK;
+}
+
 int wd2_ctx_kernel_timing(wd2_ctx *c, int enable, double *avg_us, long *launches) {
     if (enable) {
         c->kt_on = true;
         c->kt_used = 0;
+        c->kt_cnt.clear();
         return WD2_OK;
     }
     HIPCHK(hipStreamSynchronize(c->s));
@@ -884,10 +902,11 @@ int wd2_ctx_kernel_timing(wd2_ctx *c, int enable, double *avg_us, long *launches
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, c->kt_ev[k], c->kt_ev[k + 1]));
         tot += ms;
-        n++;
+        n += c->kt_cnt[k / 2];
     }
     c->kt_on = false;
     c->kt_used = 0;
+    c->kt_cnt.clear();
     if (avg_us) *avg_us = n ? 1e3 * tot / n : 0.0;
     if (launches) *launches = n;
     return WD2_OK;

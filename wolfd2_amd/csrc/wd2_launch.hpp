@@ -83,7 +83,15 @@ void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b,
                       const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
                       int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
                       int force, hipStream_t s);
-void launch_sor_tiled_check(int nx, int ny, int nsweep, int m0, int launch, int msorit, const unsigned *masks,
+// Mixed precision (FP64 residual and accumulation, FP32 sweeps), same contract.
+int sor_tiled_mx_tiles(int nx, int ny, int *ntx);
+int sor_tiled_mx_sweeps();
+void launch_sor_tiled_mx(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
+                         const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
+                         int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
+                         int force, hipStream_t s);
+// Stopping test of a launch over ntiles tile masks (the last launch of a batch).
+void launch_sor_tiled_check(int ntiles, int nsweep, int m0, int launch, int msorit, const unsigned *masks,
                             Scalars *sc, hipStream_t s);
 // Lexicographic SOR sweep (pressure.f:Sor), tiled wavefront.
 void launch_sor_lex_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
