@@ -98,6 +98,23 @@ def test_alt_trid_lu_bitwise_vs_oracle(oracle, n, weak):
     assert np.array_equal(x_gpu, x_ref)
 
 
+def test_alt_trid_lu_sequential_fallback_bitwise(oracle):
+    """The 1-D Laplacian chain (d = 2, a = c = -1) barely contracts, so the
+    speculative segments never coincide bit for bit with the repaired ones:
+    the solver must detect it and fall back to the sequential sweep."""
+    from wolfd2_amd._lib import lib
+    n = 5000
+    a = np.zeros((n, 3))
+    a[:, 0], a[:, 1], a[:, 2] = -1.0, 2.0, -1.0
+    b = np.random.default_rng(11).standard_normal(n)
+    before = lib().wd2_trid_fallbacks()
+    x_gpu = api.alt_trid_lu(a, b)
+    after = lib().wd2_trid_fallbacks()
+    _, x_ref = oracle.alt_trid_lu(a, b)
+    assert np.array_equal(x_gpu, x_ref)
+    assert after > before, "expected the sequential fallback to run"
+
+
 @pytest.fixture(params=[1, 0], ids=["tiled_sor", "colour_sor"])
 def sor_tiling(request):
     from wolfd2_amd._lib import lib

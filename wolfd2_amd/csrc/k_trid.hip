@@ -10,15 +10,18 @@
 //
 // Both recurrences contract (the systems are diagonally dominant), so two
 // runs of a recurrence started from different states agree bit for bit after
-// a few dozen rows.  Each thread therefore runs its own 16-row segment from a
+// a few dozen rows.  Each thread therefore runs its own 8-row segment from a
 // guessed state (speculation), then re-runs it from its predecessor's end
 // state until its values coincide bit for bit with the speculative ones
 // (fix-up).  From the coalescence point on the two runs are identical, so the
 // result equals the sequential sweep exactly.  Fix-ups are iterated inside a
-// workgroup until no segment end changes; across workgroups a second kernel
-// repairs each block's head from its predecessor's end state.  A segment that
-// never coalesces raises a flag and a one-lane kernel recomputes the whole
-// system sequentially, so the answer is exact in every case.
+// workgroup until no segment end changes.  Across workgroups, the backward
+// kernel repairs each block's forward head from its predecessor's end state
+// before using it, and a one-workgroup kernel repairs each block's backward
+// tail from its successor's first value.  A repair that never coalesces raises
+// a flag and the same kernel recomputes the whole system sequentially, so the
+// answer is exact in every case.  Segments are loaded and stored through LDS
+// so that global accesses are unit-stride.
 //
 // A batch of independent chains of equal length L is solved at once: row k
 // starts a chain when k % L == 0 and ends one when k % L == L-1.
@@ -28,8 +31,9 @@
 namespace wd2 {
 
 constexpr int TR_T = 256;   // threads per workgroup
-constexpr int TR_S = 16;    // rows per thread
+constexpr int TR_S = 8;     // rows per thread
 constexpr int TR_R = TR_T * TR_S;
+constexpr int TR_P = TR_S + 1;   // padded LDS stride of a thread's rows
 
 __device__ inline bool same(double x, double y) { return dbits(x) == dbits(y); }
 
@@ -51,38 +55,82 @@ __device__ inline int row_kind(long k, long L) {
     return r == 0 ? 1 : (r == L - 1 ? 2 : 0);
 }
 
+// Segment loads and stores go through LDS so that global accesses are
+// unit-stride across the workgroup (row blk0 + t + TR_T*m in pass m), while
+// each thread works on TR_S consecutive rows.
+__device__ inline void stage_in(const double *__restrict__ G, long blk0, int nrow, double *sbuf, double (&r)[TR_S]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < TR_S; m++) {
+        const int k = t + TR_T * m;
+        if (k < nrow) sbuf[(k / TR_S) * TR_P + k % TR_S] = G[blk0 + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TR_S; q++) r[q] = sbuf[t * TR_P + q];
+    __syncthreads();
+}
+__device__ inline void stage_out(double *__restrict__ G, long blk0, int nrow, double *sbuf, const double (&r)[TR_S]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < TR_S; q++) sbuf[t * TR_P + q] = r[q];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < TR_S; m++) {
+        const int k = t + TR_T * m;
+        if (k < nrow) G[blk0 + k] = sbuf[(k / TR_S) * TR_P + k % TR_S];
+    }
+    __syncthreads();
+}
+
+// Kinds of the TR_S rows from seg0 on, two bits per row (one 64-bit modulo).
+__device__ inline unsigned seg_kinds(long seg0, long L) {
+    long r = seg0 % L;
+    unsigned m = 0;
+#pragma unroll
+    for (int s = 0; s < TR_S; s++) {
+        const unsigned k = r == 0 ? 1u : (r == L - 1 ? 2u : 0u);
+        m |= k << (2 * s);
+        if (++r == L) r = 0;
+    }
+    return m;
+}
+#define SKIND(m, s) ((int)(((m) >> (2 * (s))) & 3u))
+
 __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const double *__restrict__ A,
                                                    const double *__restrict__ D, const double *__restrict__ Cc,
                                                    const double *__restrict__ B, double *__restrict__ CP,
                                                    double *__restrict__ BP, double *__restrict__ blk_c,
                                                    double *__restrict__ blk_b) {
+    __shared__ double sbuf[TR_T * TR_P];
     __shared__ double sc[TR_T], sb[TR_T];
     __shared__ int changed;
     const int t = threadIdx.x;
-    const long seg0 = (long)blockIdx.x * TR_R + (long)t * TR_S;
-    const int len = (int)max(0L, min((long)TR_S, n - seg0));
+    const long blk0 = (long)blockIdx.x * TR_R;
+    const int nrow = (int)min((long)TR_R, n - blk0);
+    const long seg0 = blk0 + (long)t * TR_S;
+    const int len = max(0, min(TR_S, nrow - t * TR_S));
     double a[TR_S], d[TR_S], c[TR_S], b[TR_S], cp[TR_S], bp[TR_S];
-#pragma unroll
-    for (int s = 0; s < TR_S; s++) {
-        if (s < len) {
-            a[s] = A[seg0 + s]; d[s] = D[seg0 + s]; c[s] = Cc[seg0 + s]; b[s] = B[seg0 + s];
-        } else {
-            a[s] = 0; d[s] = 1; c[s] = 0; b[s] = 0;
-        }
-    }
+    stage_in(A, blk0, nrow, sbuf, a);
+    stage_in(D, blk0, nrow, sbuf, d);
+    stage_in(Cc, blk0, nrow, sbuf, c);
+    stage_in(B, blk0, nrow, sbuf, b);
+    const unsigned kinds = seg_kinds(seg0, L);
     // the first-pivot quirk needs the original diagonal of the row after a chain start
-#define DNEXT(s) (row_kind(seg0 + (s), L) == 1 ? D[seg0 + (s) + 1] : 1.0)
+#define DNEXT(s) (SKIND(kinds, s) == 1 ? ((s) + 1 < len ? d[((s) + 1) % TR_S] : D[seg0 + (s) + 1]) : 1.0)
     // speculative run from the zero state
     double cpv = 0.0, bpv = 0.0;
 #pragma unroll
     for (int s = 0; s < TR_S; s++) {
         if (s < len) {
-            fwd_row(row_kind(seg0 + s, L), a[s], d[s], c[s], b[s], DNEXT(s), cpv, bpv, cp[s], bp[s]);
+            fwd_row(SKIND(kinds, s), a[s], d[s], c[s], b[s], DNEXT(s), cpv, bpv, cp[s], bp[s]);
             cpv = cp[s]; bpv = bp[s];
+        } else {
+            cp[s] = 0.0; bp[s] = 0.0;
         }
     }
     // fix-up rounds inside the workgroup
-    const bool dep = len > 0 && t > 0 && row_kind(seg0, L) != 1;
+    const bool dep = len > 0 && t > 0 && SKIND(kinds, 0) != 1;
     for (int round = 0; round <= TR_T; round++) {
         if (len > 0) { sc[t] = cp[len - 1]; sb[t] = bp[len - 1]; }
         if (t == 0) changed = 0;
@@ -94,7 +142,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const double 
             for (int s = 0; s < TR_S; s++) {
                 if (live && s < len) {
                     double ck, bk;
-                    fwd_row(row_kind(seg0 + s, L), a[s], d[s], c[s], b[s], DNEXT(s), sc0, sb0, ck, bk);
+                    fwd_row(SKIND(kinds, s), a[s], d[s], c[s], b[s], DNEXT(s), sc0, sb0, ck, bk);
                     if (same(ck, cp[s]) && same(bk, bp[s])) {
                         live = false;
                     } else {
@@ -108,67 +156,79 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const double 
         if (!changed) break;
         __syncthreads();
     }
-#pragma unroll
-    for (int s = 0; s < TR_S; s++)
-        if (s < len) { CP[seg0 + s] = cp[s]; BP[seg0 + s] = bp[s]; }
 #undef DNEXT
     // block end state = state after the block's last valid row
-    const long blk0 = (long)blockIdx.x * TR_R;
-    const long last = min(blk0 + TR_R, n) - 1;
-    if (last >= seg0 && last < seg0 + len) {
-        blk_c[blockIdx.x] = cp[last - seg0];
-        blk_b[blockIdx.x] = bp[last - seg0];
+    if (len > 0 && t * TR_S + len == nrow) {
+        blk_c[blockIdx.x] = cp[len - 1];
+        blk_b[blockIdx.x] = bp[len - 1];
     }
+    stage_out(CP, blk0, nrow, sbuf, cp);
+    stage_out(BP, blk0, nrow, sbuf, bp);
 }
 
-// Repair each workgroup's head from its predecessor's end state.
-__global__ void k_trid_fwd_fix(long n, long L, int nblk, const double *__restrict__ A,
-                               const double *__restrict__ D, const double *__restrict__ Cc,
-                               const double *__restrict__ B, double *__restrict__ CP, double *__restrict__ BP,
-                               const double *__restrict__ blk_c, const double *__restrict__ blk_b,
-                               int *__restrict__ fail) {
-    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blk < 1 || blk >= nblk) return;
-    const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
-    if (row_kind(k0, L) == 1) return;
-    double cpv = blk_c[blk - 1], bpv = blk_b[blk - 1];
-    for (long k = k0; k < k1; k++) {
-        const int kind = row_kind(k, L);
-        const double dn = (kind == 1) ? D[k + 1] : 1.0;
-        double ck, bk;
-        fwd_row(kind, A[k], D[k], Cc[k], B[k], dn, cpv, bpv, ck, bk);
-        if (same(ck, CP[k]) && same(bk, BP[k])) return;
-        CP[k] = ck; BP[k] = bk; cpv = ck; bpv = bk;
-    }
-    atomicOr(fail, 1);
-}
-
-__global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double *__restrict__ CP,
-                                                   const double *__restrict__ BP, double *__restrict__ X,
-                                                   double *__restrict__ blk_x) {
+// Backward pass.  A workgroup first repairs the head of its block's forward
+// values from the predecessor block's end state (one lane, row by row until
+// the recomputed values coincide bit for bit with the speculative ones; the
+// repaired rows are written back), then back-substitutes its segments with
+// the same speculation and in-workgroup fix-up rounds.
+__global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double *__restrict__ A,
+                                                   const double *__restrict__ D, const double *__restrict__ Cc,
+                                                   const double *__restrict__ B, double *__restrict__ CP,
+                                                   double *__restrict__ BP, const double *__restrict__ blk_c,
+                                                   const double *__restrict__ blk_b, double *__restrict__ X,
+                                                   double *__restrict__ blk_x, int *__restrict__ fail) {
+    __shared__ double scp[TR_T * TR_P], sbp[TR_T * TR_P];
     __shared__ double sx[TR_T];
     __shared__ int changed;
     const int t = threadIdx.x;
-    const long seg0 = (long)blockIdx.x * TR_R + (long)t * TR_S;
-    const int len = (int)max(0L, min((long)TR_S, n - seg0));
+    const long blk0 = (long)blockIdx.x * TR_R;
+    const int nrow = (int)min((long)TR_R, n - blk0);
+    const long seg0 = blk0 + (long)t * TR_S;
+    const int len = max(0, min(TR_S, nrow - t * TR_S));
+#pragma unroll
+    for (int m = 0; m < TR_S; m++) {
+        const int k = t + TR_T * m;
+        if (k < nrow) {
+            scp[(k / TR_S) * TR_P + k % TR_S] = CP[blk0 + k];
+            sbp[(k / TR_S) * TR_P + k % TR_S] = BP[blk0 + k];
+        }
+    }
+    __syncthreads();
+    if (t == 0 && blockIdx.x > 0) {
+        long r = blk0 % L;   // position of the block's first row in its chain
+        if (r != 0) {
+            double cpv = blk_c[blockIdx.x - 1], bpv = blk_b[blockIdx.x - 1];
+            int k = 0;
+            for (; k < nrow; k++, r = (r + 1 == L) ? 0 : r + 1) {
+                const int kind = r == 0 ? 1 : (r == L - 1 ? 2 : 0);
+                const long g = blk0 + k;
+                const double dn = (kind == 1) ? D[g + 1] : 1.0;
+                double ck, bk;
+                fwd_row(kind, A[g], D[g], Cc[g], B[g], dn, cpv, bpv, ck, bk);
+                const int q = (k / TR_S) * TR_P + k % TR_S;
+                if (same(ck, scp[q]) && same(bk, sbp[q])) break;
+                scp[q] = ck; sbp[q] = bk; CP[g] = ck; BP[g] = bk;
+                cpv = ck; bpv = bk;
+            }
+            if (k == nrow) atomicOr(fail, 1);
+        }
+    }
+    __syncthreads();
     double cp[TR_S], bp[TR_S], x[TR_S];
 #pragma unroll
-    for (int s = 0; s < TR_S; s++) {
-        if (s < len) { cp[s] = CP[seg0 + s]; bp[s] = BP[seg0 + s]; } else { cp[s] = 0; bp[s] = 0; }
-    }
+    for (int q = 0; q < TR_S; q++) { cp[q] = scp[t * TR_P + q]; bp[q] = sbp[t * TR_P + q]; }
+    const unsigned kinds = seg_kinds(seg0, L);
     double xn = 0.0;
 #pragma unroll
     for (int s = TR_S - 1; s >= 0; s--) {
         if (s < len) {
-            x[s] = (row_kind(seg0 + s, L) == 2) ? bp[s] : bp[s] - (cp[s] * xn);
+            x[s] = (SKIND(kinds, s) == 2) ? bp[s] : bp[s] - (cp[s] * xn);
             xn = x[s];
         } else {
             x[s] = 0;
         }
     }
-    const long lastrow = seg0 + len - 1;
-    const bool dep = len > 0 && row_kind(lastrow, L) != 2 && lastrow < n - 1 && t < TR_T - 1 &&
-                     seg0 + TR_S < n;
+    const bool dep = len > 0 && SKIND(kinds, len - 1) != 2 && t * TR_S + len < nrow;
     for (int round = 0; round <= TR_T; round++) {
         if (len > 0) sx[t] = x[0];
         if (t == 0) changed = 0;
@@ -179,7 +239,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double 
 #pragma unroll
             for (int s = TR_S - 1; s >= 0; s--) {
                 if (live && s < len) {
-                    const double xk = (row_kind(seg0 + s, L) == 2) ? bp[s] : bp[s] - (cp[s] * xv);
+                    const double xk = (SKIND(kinds, s) == 2) ? bp[s] : bp[s] - (cp[s] * xv);
                     if (same(xk, x[s])) live = false; else { x[s] = xk; xv = xk; }
                 }
             }
@@ -189,37 +249,37 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double 
         if (!changed) break;
         __syncthreads();
     }
-#pragma unroll
-    for (int s = 0; s < TR_S; s++)
-        if (s < len) X[seg0 + s] = x[s];
     if (t == 0 && len > 0) blk_x[blockIdx.x] = x[0];
+    stage_out(X, blk0, nrow, scp, x);
 }
 
-// Repair each workgroup's tail from its successor's first value.
-__global__ void k_trid_bwd_fix(long n, long L, int nblk, const double *__restrict__ CP,
-                               const double *__restrict__ BP, double *__restrict__ X,
-                               const double *__restrict__ blk_x, int *__restrict__ fail) {
-    const int blk = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blk >= nblk - 1) return;
-    const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
-    if (row_kind(k1 - 1, L) == 2) return;
-    double xv = blk_x[blk + 1];
-    for (long k = k1 - 1; k >= k0; k--) {
-        const double xk = (row_kind(k, L) == 2) ? BP[k] : BP[k] - (CP[k] * xv);
-        if (same(xk, X[k])) return;
-        X[k] = xk;
-        xv = xk;
+// One workgroup: repair each block's tail from its successor's first value,
+// then, if any repair failed to coalesce inside its block, solve every chain
+// sequentially (exact in every case).
+constexpr int TRF_T = 1024;
+__global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk, const double *__restrict__ A,
+                                                       const double *__restrict__ D, const double *__restrict__ Cc,
+                                                       const double *__restrict__ B, double *__restrict__ CP,
+                                                       double *__restrict__ BP, double *__restrict__ X,
+                                                       const double *__restrict__ blk_x, int *__restrict__ fail,
+                                                       int *__restrict__ nfallback) {
+    for (int blk = threadIdx.x; blk < nblk - 1; blk += TRF_T) {
+        const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
+        long r = (k1 - 1) % L;   // position of row k in its chain
+        if (r == L - 1 && r != 0) continue;
+        double xv = blk_x[blk + 1];
+        long k = k1 - 1;
+        for (; k >= k0; k--, r = (r == 0) ? L - 1 : r - 1) {
+            const double xk = (r == L - 1 && r != 0) ? BP[k] : BP[k] - (CP[k] * xv);
+            if (same(xk, X[k])) break;
+            X[k] = xk;
+            xv = xk;
+        }
+        if (k < k0) atomicOr(fail, 1);
     }
-    atomicOr(fail, 1);
-}
-
-// Sequential AltTridLU of every chain, used only if a segment never coalesced.
-__global__ void k_trid_fallback(long n, long L, const double *__restrict__ A, const double *__restrict__ D,
-                                const double *__restrict__ Cc, const double *__restrict__ B,
-                                double *__restrict__ CP, double *__restrict__ BP, double *__restrict__ X,
-                                int *__restrict__ fail, int *__restrict__ nfallback) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (*fail == 0) return;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
     for (long c0 = 0; c0 < n; c0 += L) {
         double cpv = 0, bpv = 0;
         for (long k = c0; k < c0 + L; k++) {
@@ -243,15 +303,10 @@ void launch_trid(const TridWork &w, long n, long L, const double *A, const doubl
                  const double *B, double *X, hipStream_t s) {
     const int nb = trid_blocks(n);
     hipLaunchKernelGGL(k_trid_fwd, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b);
-    if (nb > 1)
-        hipLaunchKernelGGL(k_trid_fwd_fix, dim3((nb + 63) / 64), dim3(64), 0, s, n, L, nb, A, D, C, B, w.cp, w.bp,
-                           w.blk_c, w.blk_b, w.fail);
-    hipLaunchKernelGGL(k_trid_bwd, dim3(nb), dim3(TR_T), 0, s, n, L, w.cp, w.bp, X, w.blk_x);
-    if (nb > 1)
-        hipLaunchKernelGGL(k_trid_bwd_fix, dim3((nb + 63) / 64), dim3(64), 0, s, n, L, nb, w.cp, w.bp, X, w.blk_x,
-                           w.fail);
-    hipLaunchKernelGGL(k_trid_fallback, dim3(1), dim3(64), 0, s, n, L, A, D, C, B, w.cp, w.bp, X, w.fail,
-                       w.nfallback);
+    hipLaunchKernelGGL(k_trid_bwd, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b, X,
+                       w.blk_x, w.fail);
+    hipLaunchKernelGGL(k_trid_finish, dim3(1), dim3(TRF_T), 0, s, n, L, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
+                       w.fail, w.nfallback);
 }
 
 }  // namespace wd2
