@@ -174,12 +174,13 @@ void *wd2_ctx_stream(wd2_ctx *ctx);
 
 /* Arithmetic of the context's time steps (no counterpart in the reference).
  * WD2_PREC_FP64 (default): the reference's FP64 arithmetic, bit for bit.
- * WD2_PREC_MIXED: the red-black SOR Poisson solve (nPpeSolver 5/6 on a
- * Cartesian grid without blockages) runs as defect correction: FP64 residual
- * and pressure accumulation, FP32 relaxation sweeps.  It follows the same
- * iteration (sweep order, stopping test, iteration cap) as the FP64 path, and
- * its iterates differ from FP64 by the rounding of the FP32 corrections.
- * Other solvers and grids keep FP64. */
+ * WD2_PREC_MIXED: the fields stay FP64; the momentum stencils and the ADI
+ * chain solves run in FP32 and their increments are added in FP64; the
+ * red-black SOR Poisson solve (nPpeSolver 5/6 on a Cartesian grid without
+ * blockages) runs as defect correction: FP64 residual and pressure
+ * accumulation, FP32 relaxation sweeps.  The iteration (q-l loop, sweep
+ * order, stopping tests, caps) is the reference's; the results differ from
+ * FP64 by FP32 rounding.  The other Poisson solvers keep FP64. */
 #define WD2_PREC_FP64 0
 #define WD2_PREC_MIXED 1
 int wd2_ctx_set_precision(wd2_ctx *ctx, int precision);

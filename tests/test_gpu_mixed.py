@@ -103,6 +103,32 @@ def test_mixed_poisson_reaches_fp64_solution():
     assert e <= 1e-9
 
 
+def test_precision_switch_on_one_context():
+    """FP64, then mixed, then FP64 again on one context from the same state:
+    each run equals a fresh context's run of that precision (the FP32 views
+    share memory with the FP64 temporaries, which must not leak across)."""
+    g, reg, prm = _cavity(96, 1000.0, 1e-3)
+    steps = 5
+    t64, f64 = _run(g, reg, prm, "fp64", steps)
+    tmx, fmx = _run(g, reg, prm, "mixed", steps)
+    sol = api.Solver(g, reg, prm)
+    z = np.zeros((g.ncol, g.ld))
+    sol.set_fields(z, z, z)
+    sol.cold_start()
+    f0 = sol.get_fields()
+    out = []
+    for prec in ("fp64", "mixed", "fp64"):
+        sol.set_precision(prec)
+        sol.set_fields(*f0)
+        tab = sol.step(steps)
+        out.append((tab, sol.get_fields()))
+    sol.close()
+    for (tab, f), (tr, fr) in zip(out, [(t64, f64), (tmx, fmx), (t64, f64)]):
+        assert np.array_equal(tab[:, :2], tr[:, :2])
+        for a, b in zip(f, fr):
+            assert np.array_equal(a, b)
+
+
 def test_mixed_full_size_cavity_steps():
     """1024^2 cells, reference tolerances (2000-sweep cap): two steps."""
     g, reg, prm = _cavity(1024, 1000.0, 2e-4)

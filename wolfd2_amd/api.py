@@ -163,7 +163,8 @@ class Solver:
 
     def set_precision(self, precision: str):
         """"fp64" (the reference's arithmetic, bit for bit) or "mixed" (FP64
-        residual and accumulation, FP32 relaxation in the red-black SOR)."""
+        fields; FP32 momentum stencils and ADI chains; red-black SOR as FP64
+        defect correction around FP32 sweeps)."""
         check(lib().wd2_ctx_set_precision(self.ctx, {"fp64": 0, "mixed": 1}[precision]))
 
     def set_fields(self, u, v, p):

@@ -12,6 +12,13 @@
 // The chains themselves go to the parallel AltTridLU of k_trid.hip.  Region
 // loops whose ranges overlap are resolved per point in the reference's
 // region order, so the result is the reference's whatever the thread order.
+//
+// Every kernel is a template on T, the type of the temporaries and chains
+// and of the arithmetic; the fields (u, v, metrics) stay FP64 and are read
+// through cv<T>.  T = double is the reference's arithmetic, operation for
+// operation (the casts are no-ops).  T = float is the mixed-precision path's
+// momentum: FP32 stencils and ADI chains on FP64 fields, with the difference
+// fn - fs of the right-hand side formed in FP64 before rounding.
 #include "wd2_dev.hpp"
 #include "wd2_launch.hpp"
 
@@ -21,18 +28,21 @@ constexpr int MX = 64, MY = 4;
 static inline dim3 mgrid(int ni, int nj) { return dim3((ni + MX - 1) / MX, (nj + MY - 1) / MY); }
 
 // ------------------------------------------------------------- ConvCoef
-template <int NCOMP>
-__global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, double djac,
+template <typename T>
+__device__ inline T cv(double x) { return (T)x; }
+
+template <int NCOMP, typename T>
+__global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, double djac_,
                                                       const double *__restrict__ xzi, const double *__restrict__ xet,
                                                       const double *__restrict__ yzi, const double *__restrict__ yet,
                                                       const double *__restrict__ u, const double *__restrict__ v,
-                                                      double *__restrict__ cc1, double *__restrict__ cc2) {
+                                                      T *__restrict__ cc1, T *__restrict__ cc2) {
     const int i = blockIdx.x * MX + threadIdx.x, j = blockIdx.y * MY + threadIdx.y;   // 0-based cover
     if (i > nx + 1 || j > ny + 1) return;
-    const double dHalf = 0.5, dFour = 4.0;
-#define U(a, b) u[WD2_IX(a, b)]
-#define V(a, b) v[WD2_IX(a, b)]
-#define M(f, a, b) f[WD2_IX(a, b)]
+    const T dHalf = T(0.5), dFour = T(4.0), djac = cv<T>(djac_);
+#define U(a, b) cv<T>(u[WD2_IX(a, b)])
+#define V(a, b) cv<T>(v[WD2_IX(a, b)])
+#define M(f, a, b) cv<T>(f[WD2_IX(a, b)])
     const size_t k = WD2_IX(i, j);
     if (NCOMP == 1) {
         if (i >= 1 && j >= 1)
@@ -64,22 +74,29 @@ __global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, do
 #undef M
 }
 
+template <typename T>
 void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const double *xzi, const double *xet,
-                      const double *yzi, const double *yet, const double *u, const double *v, double *cc1,
-                      double *cc2, hipStream_t s) {
+                      const double *yzi, const double *yet, const double *u, const double *v, T *cc1, T *cc2,
+                      hipStream_t s) {
     const double djac = njacob == 1 ? 2.0 : 1.0;
     const dim3 g = mgrid(nx + 2, ny + 2), b(MX, MY);
     switch (ncomp) {
-    case 1: hipLaunchKernelGGL(k_conv_coef<1>, g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
-    case 2: hipLaunchKernelGGL(k_conv_coef<2>, g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
-    case 4: hipLaunchKernelGGL(k_conv_coef<4>, g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
-    case 5: hipLaunchKernelGGL(k_conv_coef<5>, g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
+    case 1: hipLaunchKernelGGL((k_conv_coef<1, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
+    case 2: hipLaunchKernelGGL((k_conv_coef<2, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
+    case 4: hipLaunchKernelGGL((k_conv_coef<4, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
+    case 5: hipLaunchKernelGGL((k_conv_coef<5, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
     default: break;
     }
 }
+template void launch_conv_coef<double>(int, int, int, int, int, const double *, const double *, const double *,
+                                       const double *, const double *, const double *, double *, double *,
+                                       hipStream_t);
+template void launch_conv_coef<float>(int, int, int, int, int, const double *, const double *, const double *,
+                                      const double *, const double *, const double *, float *, float *, hipStream_t);
 
 // --------------------------------------------------- porous-region division
-__global__ __launch_bounds__(MX *MY) void k_porous_div(Regions R, int ld, int comp, MomTemps t) {
+template <typename T>
+__global__ __launch_bounds__(MX *MY) void k_porous_div(Regions R, int ld, int comp, MomTempsT<T> t) {
     const int i = blockIdx.x * MX + threadIdx.x, j = blockIdx.y * MY + threadIdx.y;
     if (i > R.nx + 1 || j > R.ny + 1) return;
     const size_t k = WD2_IX(i, j);
@@ -91,7 +108,7 @@ __global__ __launch_bounds__(MX *MY) void k_porous_div(Regions R, int ld, int co
             const bool in = comp == 1 ? (j >= jS + 1 && j <= jN && i >= iW && i <= iE)
                                       : (j >= jS && j <= jN && i >= iW + 1 && i <= iE);
             if (!in) continue;
-            const double por = R.tab(R.poros, ir, jr);
+            const T por = cv<T>(R.tab(R.poros, ir, jr));
             t.cj1[k] = t.cj1[k] / por;
             t.cj2[k] = t.cj2[k] / por;
             t.c1s[k] = t.c1s[k] / por;
@@ -101,35 +118,42 @@ __global__ __launch_bounds__(MX *MY) void k_porous_div(Regions R, int ld, int co
         }
 }
 
-void launch_porous_div(const Regions &R, int ld, int comp, const MomTemps &t, hipStream_t s) {
-    hipLaunchKernelGGL(k_porous_div, mgrid(R.nx + 2, R.ny + 2), dim3(MX, MY), 0, s, R, ld, comp, t);
+template <typename T>
+void launch_porous_div(const Regions &R, int ld, int comp, const MomTempsT<T> &t, hipStream_t s) {
+    hipLaunchKernelGGL(k_porous_div<T>, mgrid(R.nx + 2, R.ny + 2), dim3(MX, MY), 0, s, R, ld, comp, t);
 }
+template void launch_porous_div<double>(const Regions &, int, int, const MomTempsT<double> &, hipStream_t);
+template void launch_porous_div<float>(const Regions &, int, int, const MomTempsT<float> &, hipStream_t);
 
 // -------------------------------------------------------------- PorosCoef
 // Each point takes the value of the last region (in loop order) whose write
 // range covers it: zero for a non-porous region, the drag formula otherwise.
-__device__ inline double poros_val(int comp, int njacob, int ld, int i, int j, double porc1, double porc2,
-                                   const double *__restrict__ u, const double *__restrict__ v) {
-    const double dFour = 4.0;
-    double unorm, unrm1 = 0.0;
+template <typename T>
+__device__ inline T poros_val(int comp, int njacob, int ld, int i, int j, T porc1, T porc2,
+                              const double *__restrict__ u, const double *__restrict__ v) {
+    const T dFour = T(4.0);
+    T unorm, unrm1 = T(0.0);
     if (comp == 1) {
-        const double uu = u[WD2_IX(i, j)];
-        const double w = v[WD2_IX(i, j)] + v[WD2_IX(i + 1, j)] + v[WD2_IX(i, j - 1)] + v[WD2_IX(i + 1, j - 1)] / dFour;
+        const T uu = cv<T>(u[WD2_IX(i, j)]);
+        const T w = cv<T>(v[WD2_IX(i, j)]) + cv<T>(v[WD2_IX(i + 1, j)]) + cv<T>(v[WD2_IX(i, j - 1)]) +
+                    cv<T>(v[WD2_IX(i + 1, j - 1)]) / dFour;
         unorm = sqrt(uu * uu + w * w);
-        if (unorm > 1.e-8) unrm1 = (uu * uu) / unorm;
+        if (unorm > T(1.e-8)) unrm1 = (uu * uu) / unorm;
     } else {
-        const double vv = v[WD2_IX(i, j)];
-        const double w = (u[WD2_IX(i - 1, j + 1)] + u[WD2_IX(i, j + 1)] + u[WD2_IX(i - 1, j)] + u[WD2_IX(i, j)]) / dFour;
+        const T vv = cv<T>(v[WD2_IX(i, j)]);
+        const T w = (cv<T>(u[WD2_IX(i - 1, j + 1)]) + cv<T>(u[WD2_IX(i, j + 1)]) + cv<T>(u[WD2_IX(i - 1, j)]) +
+                     cv<T>(u[WD2_IX(i, j)])) / dFour;
         unorm = sqrt(w * w + vv * vv);
-        if (unorm > 1.e-8) unrm1 = (vv * vv) / unorm;
+        if (unorm > T(1.e-8)) unrm1 = (vv * vv) / unorm;
     }
     if (njacob == 1) unorm = unrm1 + unorm;
     return porc1 + porc2 * unorm;
 }
 
+template <typename T>
 __global__ __launch_bounds__(MX *MY) void k_poros_coef(Regions R, int ld, int comp, const double *__restrict__ us,
                                                        const double *__restrict__ vs, const double *__restrict__ un,
-                                                       const double *__restrict__ vn, MomTemps t) {
+                                                       const double *__restrict__ vn, MomTempsT<T> t) {
     const int i = 1 + blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
     if (i > R.nx || j > R.ny) return;
     int wir = 0, wjr = 0, wpor = 0;
@@ -147,87 +171,102 @@ __global__ __launch_bounds__(MX *MY) void k_poros_coef(Regions R, int ld, int co
     if (wir == 0) return;
     const size_t k = WD2_IX(i, j);
     if (!wpor) {
-        t.cpj[k] = 0.0; t.cps[k] = 0.0; t.cpn[k] = 0.0;
+        t.cpj[k] = T(0.0); t.cps[k] = T(0.0); t.cpn[k] = T(0.0);
         return;
     }
-    const double c1 = R.tab(R.porc1, wir, wjr), c2 = R.tab(R.porc2, wir, wjr);
-    t.cpj[k] = poros_val(comp, 1, ld, i, j, c1, c2, us, vs);
-    t.cps[k] = poros_val(comp, 0, ld, i, j, c1, c2, us, vs);
-    t.cpn[k] = poros_val(comp, 0, ld, i, j, c1, c2, un, vn);
+    const T c1 = cv<T>(R.tab(R.porc1, wir, wjr)), c2 = cv<T>(R.tab(R.porc2, wir, wjr));
+    t.cpj[k] = poros_val<T>(comp, 1, ld, i, j, c1, c2, us, vs);
+    t.cps[k] = poros_val<T>(comp, 0, ld, i, j, c1, c2, us, vs);
+    t.cpn[k] = poros_val<T>(comp, 0, ld, i, j, c1, c2, un, vn);
 }
 
-void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, const double *vs,
-                       const double *un, const double *vn, const MomTemps &t, hipStream_t s) {
-    hipLaunchKernelGGL(k_poros_coef, mgrid(R.nx, R.ny), dim3(MX, MY), 0, s, R, ld, comp, us, vs, un, vn, t);
+template <typename T>
+void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, const double *vs, const double *un,
+                       const double *vn, const MomTempsT<T> &t, hipStream_t s) {
+    hipLaunchKernelGGL(k_poros_coef<T>, mgrid(R.nx, R.ny), dim3(MX, MY), 0, s, R, ld, comp, us, vs, un, vn, t);
 }
+template void launch_poros_coef<double>(const Regions &, int, int, const double *, const double *, const double *,
+                                        const double *, const MomTempsT<double> &, hipStream_t);
+template void launch_poros_coef<float>(const Regions &, int, int, const double *, const double *, const double *,
+                                       const double *, const MomTempsT<float> &, hipStream_t);
 
 // ---------------------------------------------------- convection/diffusion
-__device__ inline double dconv_u(int ld, int i, int j, const double *c1, const double *c2, const double *u) {
+template <typename T>
+__device__ inline T dconv_u(int ld, int i, int j, const T *c1, const T *c2, const double *u) {
 #define C1(a, b) c1[WD2_IX(a, b)]
 #define C2(a, b) c2[WD2_IX(a, b)]
-#define U(a, b) u[WD2_IX(a, b)]
+#define U(a, b) cv<T>(u[WD2_IX(a, b)])
     return -C2(i, j - 1) * U(i, j - 1) - C1(i, j) * U(i - 1, j) +
            (C1(i + 1, j) - C1(i, j) + C2(i, j) - C2(i, j - 1)) * U(i, j) + C1(i + 1, j) * U(i + 1, j) +
            C2(i, j) * U(i, j + 1);
 }
-__device__ inline double dconv_v(int ld, int i, int j, const double *c1, const double *c2, const double *u) {
+template <typename T>
+__device__ inline T dconv_v(int ld, int i, int j, const T *c1, const T *c2, const double *u) {
     return -C2(i, j) * U(i, j - 1) - C1(i - 1, j) * U(i - 1, j) +
            (C1(i, j) - C1(i - 1, j) + C2(i, j + 1) - C2(i, j)) * U(i, j) + C1(i, j) * U(i + 1, j) +
            C2(i, j + 1) * U(i, j + 1);
 #undef C1
 #undef C2
 }
-__device__ inline double ddiff_u(int ld, int i, int j, const double *ac, const double *bc, const double *bn,
-                                 const double *gn, const double *u) {
-#define F(f, a, b) f[WD2_IX(a, b)]
-    const double s1 = F(ac, i + 1, j) * (U(i + 1, j) - U(i, j)) - F(ac, i, j) * (U(i, j) - U(i - 1, j)) +
-                      F(bc, i + 1, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
-                      F(bc, i, j) * (U(i, j + 1) + U(i - 1, j + 1) - U(i, j - 1) - U(i - 1, j - 1));
-    const double s2 = F(bn, i, j) * (U(i + 1, j + 1) + U(i + 1, j) - U(i - 1, j + 1) - U(i - 1, j)) -
-                      F(bn, i, j - 1) * (U(i + 1, j) + U(i + 1, j - 1) - U(i - 1, j) - U(i - 1, j - 1)) +
-                      F(gn, i, j) * (U(i, j + 1) - U(i, j)) - F(gn, i, j - 1) * (U(i, j) - U(i, j - 1));
+template <typename T>
+__device__ inline T ddiff_u(int ld, int i, int j, const double *ac, const double *bc, const double *bn,
+                            const double *gn, const double *u) {
+#define F(f, a, b) cv<T>(f[WD2_IX(a, b)])
+    const T s1 = F(ac, i + 1, j) * (U(i + 1, j) - U(i, j)) - F(ac, i, j) * (U(i, j) - U(i - 1, j)) +
+                 F(bc, i + 1, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
+                 F(bc, i, j) * (U(i, j + 1) + U(i - 1, j + 1) - U(i, j - 1) - U(i - 1, j - 1));
+    const T s2 = F(bn, i, j) * (U(i + 1, j + 1) + U(i + 1, j) - U(i - 1, j + 1) - U(i - 1, j)) -
+                 F(bn, i, j - 1) * (U(i + 1, j) + U(i + 1, j - 1) - U(i - 1, j) - U(i - 1, j - 1)) +
+                 F(gn, i, j) * (U(i, j + 1) - U(i, j)) - F(gn, i, j - 1) * (U(i, j) - U(i, j - 1));
     return s1 + s2;
 }
-__device__ inline double ddiff_v(int ld, int i, int j, const double *an, const double *bc, const double *bn,
-                                 const double *gc, const double *u) {
-    const double s1 = F(an, i, j) * (U(i + 1, j) - U(i, j)) - F(an, i - 1, j) * (U(i, j) - U(i - 1, j)) +
-                      F(bn, i, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
-                      F(bn, i - 1, j) * (U(i, j + 1) + U(i - 1, j + 1) - U(i, j - 1) - U(i - 1, j - 1));
-    const double s2 = F(bc, i, j + 1) * (U(i + 1, j + 1) + U(i + 1, j) - U(i - 1, j + 1) - U(i - 1, j)) -
-                      F(bc, i, j) * (U(i + 1, j) + U(i + 1, j - 1) - U(i - 1, j) - U(i - 1, j - 1)) +
-                      F(gc, i, j + 1) * (U(i, j + 1) - U(i, j)) - F(gc, i, j) * (U(i, j) - U(i, j - 1));
+template <typename T>
+__device__ inline T ddiff_v(int ld, int i, int j, const double *an, const double *bc, const double *bn,
+                            const double *gc, const double *u) {
+    const T s1 = F(an, i, j) * (U(i + 1, j) - U(i, j)) - F(an, i - 1, j) * (U(i, j) - U(i - 1, j)) +
+                 F(bn, i, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
+                 F(bn, i - 1, j) * (U(i, j + 1) + U(i - 1, j + 1) - U(i, j - 1) - U(i - 1, j - 1));
+    const T s2 = F(bc, i, j + 1) * (U(i + 1, j + 1) + U(i + 1, j) - U(i - 1, j + 1) - U(i - 1, j)) -
+                 F(bc, i, j) * (U(i + 1, j) + U(i + 1, j - 1) - U(i - 1, j) - U(i - 1, j - 1)) +
+                 F(gc, i, j + 1) * (U(i, j + 1) - U(i, j)) - F(gc, i, j) * (U(i, j) - U(i, j - 1));
     return s1 + s2;
 #undef F
 #undef U
 }
 
+template <typename T>
 __global__ __launch_bounds__(MX *MY) void k_conv_diff(int nx, int ny, int ld, int comp, Metrics m,
                                                       const double *__restrict__ fs, const double *__restrict__ fn,
-                                                      MomTemps t) {
+                                                      MomTempsT<T> t) {
     if (comp == 1) {
         const int i = 1 + blockIdx.x * MX + threadIdx.x, j = 2 + blockIdx.y * MY + threadIdx.y;
         if (i > nx || j > ny) return;
         const size_t k = WD2_IX(i, j);
-        t.cnvs[k] = dconv_u(ld, i, j, t.c1s, t.c2s, fs);
-        t.cnvn[k] = dconv_u(ld, i, j, t.c1n, t.c2n, fn);
-        t.difs[k] = ddiff_u(ld, i, j, m.rac, m.rbc, m.rbn, m.rgn, fs);
-        t.difn[k] = ddiff_u(ld, i, j, m.rac, m.rbc, m.rbn, m.rgn, fn);
+        t.cnvs[k] = dconv_u<T>(ld, i, j, t.c1s, t.c2s, fs);
+        t.cnvn[k] = dconv_u<T>(ld, i, j, t.c1n, t.c2n, fn);
+        t.difs[k] = ddiff_u<T>(ld, i, j, m.rac, m.rbc, m.rbn, m.rgn, fs);
+        t.difn[k] = ddiff_u<T>(ld, i, j, m.rac, m.rbc, m.rbn, m.rgn, fn);
     } else {
         const int i = 2 + blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
         if (i > nx || j > ny) return;
         const size_t k = WD2_IX(i, j);
-        t.cnvs[k] = dconv_v(ld, i, j, t.c1s, t.c2s, fs);
-        t.cnvn[k] = dconv_v(ld, i, j, t.c1n, t.c2n, fn);
-        t.difs[k] = ddiff_v(ld, i, j, m.ran, m.rbc, m.rbn, m.rgc, fs);
-        t.difn[k] = ddiff_v(ld, i, j, m.ran, m.rbc, m.rbn, m.rgc, fn);
+        t.cnvs[k] = dconv_v<T>(ld, i, j, t.c1s, t.c2s, fs);
+        t.cnvn[k] = dconv_v<T>(ld, i, j, t.c1n, t.c2n, fn);
+        t.difs[k] = ddiff_v<T>(ld, i, j, m.ran, m.rbc, m.rbn, m.rgc, fs);
+        t.difn[k] = ddiff_v<T>(ld, i, j, m.ran, m.rbc, m.rbn, m.rgc, fn);
     }
 }
 
+template <typename T>
 void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const double *fs, const double *fn,
-                      const MomTemps &t, hipStream_t s) {
+                      const MomTempsT<T> &t, hipStream_t s) {
     const dim3 g = comp == 1 ? mgrid(nx, ny - 1) : mgrid(nx - 1, ny);
-    hipLaunchKernelGGL(k_conv_diff, g, dim3(MX, MY), 0, s, nx, ny, ld, comp, m, fs, fn, t);
+    hipLaunchKernelGGL(k_conv_diff<T>, g, dim3(MX, MY), 0, s, nx, ny, ld, comp, m, fs, fn, t);
 }
+template void launch_conv_diff<double>(int, int, int, int, const Metrics &, const double *, const double *,
+                                       const MomTempsT<double> &, hipStream_t);
+template void launch_conv_diff<float>(int, int, int, int, const Metrics &, const double *, const double *,
+                                      const MomTempsT<float> &, hipStream_t);
 
 // ------------------------------------------------------------ assembly
 // comp 1 (x-momentum): rows (i=1..nx, j=2..ny), k = (j-2)*nx + i-1
@@ -235,11 +274,12 @@ void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const 
 // split 1: first split step with its right-hand side; split 2: second split
 // step (y-direction coefficients on the same x-ordered chain), right-hand
 // side = first-split solution, wall and blockage identity rows.
-__global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp, int split, double dk, double dk2,
-                                                     double re1, double fr, Metrics m, MomTemps t,
+template <typename T>
+__global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp, int split, double dk_,
+                                                     double dk2_, double re1_, double fr_, Metrics m, MomTempsT<T> t,
                                                      const double *__restrict__ fs, const double *__restrict__ fn,
                                                      const double *__restrict__ d, const double *__restrict__ dn,
-                                                     const double *__restrict__ xprev, Chain ch, int *err) {
+                                                     const T *__restrict__ xprev, ChainT<T> ch, int *err) {
     const int nx = R.nx, ny = R.ny;
     int i, j;
     long k;
@@ -252,36 +292,37 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
         if (i > nx || j > ny) return;
         k = (long)(j - 1) * (nx - 1) + (i - 2);
     }
-    const double dZero = 0.0, dOne = 1.0, dFour = 4.0;
+    const T dZero = T(0.0), dOne = T(1.0), dFour = T(4.0);
+    const T dk = cv<T>(dk_), dk2 = cv<T>(dk2_), re1 = cv<T>(re1_), fr = cv<T>(fr_);
     const size_t p = WD2_IX(i, j);
-#define T(f, a, b) t.f[WD2_IX(a, b)]
-#define MM(f, a, b) m.f[WD2_IX(a, b)]
-    double a1, a2, a3;
+#define TT(f, a, b) t.f[WD2_IX(a, b)]
+#define MM(f, a, b) cv<T>(m.f[WD2_IX(a, b)])
+    T a1, a2, a3;
     if (comp == 1) {
-        const double rkj = dk2 * m.dju[p];
+        const T rkj = dk2 * cv<T>(m.dju[p]);
         if (split == 1) {
             if (t.cj1[p] >= dZero) {
-                a1 = rkj * (-T(cj1, i - 1, j) - re1 * MM(rac, i, j));
-                a2 = dOne + rkj * (T(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * t.cpj[p];
+                a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(rac, i, j));
+                a2 = dOne + rkj * (TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * t.cpj[p];
                 a3 = rkj * (-re1 * MM(rac, i + 1, j));
             } else {
                 a1 = rkj * (-re1 * MM(rac, i, j));
-                a2 = dOne + rkj * (-T(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * t.cpj[p];
-                a3 = rkj * (T(cj1, i + 1, j) - re1 * MM(rac, i + 1, j));
+                a2 = dOne + rkj * (-TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * t.cpj[p];
+                a3 = rkj * (TT(cj1, i + 1, j) - re1 * MM(rac, i + 1, j));
             }
-            ch.b[k] = fn[p] - fs[p] + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
-                      dk2 * (t.cps[p] * fs[p] + t.cpn[p] * fn[p]);
+            ch.b[k] = cv<T>(fn[p] - fs[p]) + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
+                      dk2 * (t.cps[p] * cv<T>(fs[p]) + t.cpn[p] * cv<T>(fn[p]));
         } else {
             if (t.cj2[p] >= dZero) {
-                a1 = rkj * (-T(cj2, i, j - 1) - re1 * MM(rgn, i, j - 1));
-                a2 = dOne + rkj * (T(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * t.cpj[p];
+                a1 = rkj * (-TT(cj2, i, j - 1) - re1 * MM(rgn, i, j - 1));
+                a2 = dOne + rkj * (TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * t.cpj[p];
                 a3 = rkj * (-re1 * MM(rgn, i, j));
             } else {
                 a1 = rkj * (-re1 * MM(rgn, i, j - 1));
-                a2 = dOne + rkj * (-T(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * t.cpj[p];
-                a3 = rkj * (T(cj2, i, j + 1) - re1 * MM(rgn, i, j));
+                a2 = dOne + rkj * (-TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * t.cpj[p];
+                a3 = rkj * (TT(cj2, i, j + 1) - re1 * MM(rgn, i, j));
             }
-            double bb = xprev[k];
+            T bb = xprev[k];
             bool ident = false;
             for (int jr = 1; jr <= R.nj; jr++)
                 for (int ir = 1; ir <= R.ni; ir++) {
@@ -298,31 +339,32 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
             ch.b[k] = bb;
         }
     } else {
-        const double rkj = dk2 * m.djv[p];
+        const T rkj = dk2 * cv<T>(m.djv[p]);
         if (split == 1) {
             if (t.cj1[p] >= dZero) {
-                a1 = rkj * (-T(cj1, i - 1, j) - re1 * MM(ran, i - 1, j));
-                a2 = rkj * (T(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * t.cpj[p] + dOne;
+                a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(ran, i - 1, j));
+                a2 = rkj * (TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * t.cpj[p] + dOne;
                 a3 = rkj * (-re1 * MM(ran, i, j));
             } else {
                 a1 = rkj * (-re1 * MM(ran, i - 1, j));
-                a2 = rkj * (-T(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * t.cpj[p] + dOne;
-                a3 = rkj * (T(cj1, i + 1, j) - re1 * MM(ran, i, j));
+                a2 = rkj * (-TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * t.cpj[p] + dOne;
+                a3 = rkj * (TT(cj1, i + 1, j) - re1 * MM(ran, i, j));
             }
-            const double buoy = dk * (d[WD2_IX(i, j + 1)] + d[p] + dn[WD2_IX(i, j + 1)] + dn[p]) / (dFour * fr);
-            ch.b[k] = fn[p] - fs[p] + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
-                      dk2 * (t.cps[p] * fs[p] + t.cpn[p] * fn[p]) - buoy;
+            const T buoy = dk * (cv<T>(d[WD2_IX(i, j + 1)]) + cv<T>(d[p]) + cv<T>(dn[WD2_IX(i, j + 1)]) + cv<T>(dn[p])) /
+                           (dFour * fr);
+            ch.b[k] = cv<T>(fn[p] - fs[p]) + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
+                      dk2 * (t.cps[p] * cv<T>(fs[p]) + t.cpn[p] * cv<T>(fn[p])) - buoy;
         } else {
             if (t.cj2[p] >= dZero) {
-                a1 = rkj * (-T(cj2, i, j - 1) - re1 * MM(rgc, i, j));
-                a2 = rkj * (T(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * t.cpj[p] + dOne;
+                a1 = rkj * (-TT(cj2, i, j - 1) - re1 * MM(rgc, i, j));
+                a2 = rkj * (TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * t.cpj[p] + dOne;
                 a3 = rkj * (-re1 * MM(rgc, i, j + 1));
             } else {
                 a1 = rkj * (-re1 * MM(rgc, i, j));
-                a2 = rkj * (-T(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * t.cpj[p] + dOne;
-                a3 = rkj * (T(cj2, i, j + 1) - re1 * MM(rgc, i, j + 1));
+                a2 = rkj * (-TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * t.cpj[p] + dOne;
+                a3 = rkj * (TT(cj2, i, j + 1) - re1 * MM(rgc, i, j + 1));
             }
-            double bb = xprev[k];
+            T bb = xprev[k];
             bool ident = false;
             for (int jr = 1; jr <= R.nj; jr++)
                 for (int ir = 1; ir <= R.ni; ir++) {
@@ -339,40 +381,51 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
             ch.b[k] = bb;
         }
     }
-#undef T
+#undef TT
 #undef MM
     ch.a[k] = a1;
     ch.d[k] = a2;
     ch.c[k] = a3;
 }
 
+template <typename T>
 void launch_assemble(const Regions &R, int ld, int comp, int split, double dk, double re, double fr,
-                     const Metrics &m, const MomTemps &t, const double *fs, const double *fn, const double *d,
-                     const double *dn, const double *xprev, const Chain &ch, hipStream_t s, int *err) {
+                     const Metrics &m, const MomTempsT<T> &t, const double *fs, const double *fn, const double *d,
+                     const double *dn, const T *xprev, const ChainT<T> &ch, hipStream_t s, int *err) {
     const double dk2 = dk * 0.5, re1 = 1.0 / re;
     const dim3 g = comp == 1 ? mgrid(R.nx, R.ny - 1) : mgrid(R.nx - 1, R.ny);
-    hipLaunchKernelGGL(k_assemble, g, dim3(MX, MY), 0, s, R, ld, comp, split, dk, dk2, re1, fr, m, t, fs, fn, d, dn,
-                       xprev, ch, err);
+    hipLaunchKernelGGL(k_assemble<T>, g, dim3(MX, MY), 0, s, R, ld, comp, split, dk, dk2, re1, fr, m, t, fs, fn, d,
+                       dn, xprev, ch, err);
 }
+template void launch_assemble<double>(const Regions &, int, int, int, double, double, double, const Metrics &,
+                                      const MomTempsT<double> &, const double *, const double *, const double *,
+                                      const double *, const double *, const ChainT<double> &, hipStream_t, int *);
+template void launch_assemble<float>(const Regions &, int, int, int, double, double, double, const Metrics &,
+                                     const MomTempsT<float> &, const double *, const double *, const double *,
+                                     const double *, const float *, const ChainT<float> &, hipStream_t, int *);
 
 // ------------------------------------------------------------- scatter
-__global__ __launch_bounds__(MX *MY) void k_scatter(int nx, int ny, int ld, int comp, const double *__restrict__ x,
+template <typename T>
+__global__ __launch_bounds__(MX *MY) void k_scatter(int nx, int ny, int ld, int comp, const T *__restrict__ x,
                                                     double *__restrict__ out) {
     if (comp == 1) {
         const int i = 1 + blockIdx.x * MX + threadIdx.x, j = 2 + blockIdx.y * MY + threadIdx.y;
         if (i > nx || j > ny) return;
-        out[WD2_IX(i, j)] = x[(long)(j - 2) * nx + (i - 1)];
+        out[WD2_IX(i, j)] = (double)x[(long)(j - 2) * nx + (i - 1)];
     } else {
         const int i = 2 + blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
         if (i > nx || j > ny) return;
-        out[WD2_IX(i, j)] = x[(long)(j - 1) * (nx - 1) + (i - 2)];
+        out[WD2_IX(i, j)] = (double)x[(long)(j - 1) * (nx - 1) + (i - 2)];
     }
 }
 
-void launch_scatter(int nx, int ny, int ld, int comp, const double *x, double *out, hipStream_t s) {
+template <typename T>
+void launch_scatter(int nx, int ny, int ld, int comp, const T *x, double *out, hipStream_t s) {
     const dim3 g = comp == 1 ? mgrid(nx, ny - 1) : mgrid(nx - 1, ny);
-    hipLaunchKernelGGL(k_scatter, g, dim3(MX, MY), 0, s, nx, ny, ld, comp, x, out);
+    hipLaunchKernelGGL(k_scatter<T>, g, dim3(MX, MY), 0, s, nx, ny, ld, comp, x, out);
 }
+template void launch_scatter<double>(int, int, int, int, const double *, double *, hipStream_t);
+template void launch_scatter<float>(int, int, int, int, const float *, double *, hipStream_t);
 
 // ------------------------------------------- q-l update and increment norms
 // us += dus, vs += dvs on i=1..nx, j=1..ny (momentum.f:171-176) and

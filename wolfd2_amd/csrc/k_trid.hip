@@ -36,16 +36,17 @@ constexpr int TR_R = TR_T * TR_S;
 constexpr int TR_P = TR_S + 1;   // padded LDS stride of a thread's rows
 
 __device__ inline bool same(double x, double y) { return dbits(x) == dbits(y); }
+__device__ inline bool same(float x, float y) { return __float_as_uint(x) == __float_as_uint(y); }
 
 // One forward row.  kind: 0 interior, 1 chain start, 2 chain end.
-__device__ inline void fwd_row(int kind, double a, double d, double c, double b, double dnext,
-                               double cpv, double bpv, double &cpk, double &bpk) {
+template <typename T>
+__device__ inline void fwd_row(int kind, T a, T d, T c, T b, T dnext, T cpv, T bpv, T &cpk, T &bpk) {
     if (kind == 1) {
         cpk = c / dnext;
         bpk = b / d;
     } else {
-        const double dp = d - (a * cpv);
-        cpk = (kind == 2) ? 0.0 : c / dp;
+        const T dp = d - (a * cpv);
+        cpk = (kind == 2) ? T(0) : c / dp;
         bpk = (b - a * bpv) / dp;
     }
 }
@@ -58,7 +59,8 @@ __device__ inline int row_kind(long k, long L) {
 // Segment loads and stores go through LDS so that global accesses are
 // unit-stride across the workgroup (row blk0 + t + TR_T*m in pass m), while
 // each thread works on TR_S consecutive rows.
-__device__ inline void stage_in(const double *__restrict__ G, long blk0, int nrow, double *sbuf, double (&r)[TR_S]) {
+template <typename T>
+__device__ inline void stage_in(const T *__restrict__ G, long blk0, int nrow, T *sbuf, T (&r)[TR_S]) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int m = 0; m < TR_S; m++) {
@@ -70,7 +72,8 @@ __device__ inline void stage_in(const double *__restrict__ G, long blk0, int nro
     for (int q = 0; q < TR_S; q++) r[q] = sbuf[t * TR_P + q];
     __syncthreads();
 }
-__device__ inline void stage_out(double *__restrict__ G, long blk0, int nrow, double *sbuf, const double (&r)[TR_S]) {
+template <typename T>
+__device__ inline void stage_out(T *__restrict__ G, long blk0, int nrow, T *sbuf, const T (&r)[TR_S]) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < TR_S; q++) sbuf[t * TR_P + q] = r[q];
@@ -97,36 +100,70 @@ __device__ inline unsigned seg_kinds(long seg0, long L) {
 }
 #define SKIND(m, s) ((int)(((m) >> (2 * (s))) & 3u))
 
-__global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const double *__restrict__ A,
-                                                   const double *__restrict__ D, const double *__restrict__ Cc,
-                                                   const double *__restrict__ B, double *__restrict__ CP,
-                                                   double *__restrict__ BP, double *__restrict__ blk_c,
-                                                   double *__restrict__ blk_b) {
-    __shared__ double sbuf[TR_T * TR_P];
-    __shared__ double sc[TR_T], sb[TR_T];
+// Speculated states at block boundaries: the forward state entering row blk0
+// and the backward value at row k1 are guessed by running the recurrence over
+// the TR_W rows beyond the boundary from a zero state.  The recurrences
+// contract, so the guess is usually the exact state bit for bit; it is always
+// verified (and repaired if not) against the neighbour block's own result.
+constexpr int TR_W = 16;
+template <typename T>
+__device__ inline void warm_fwd(const T *__restrict__ A, const T *__restrict__ D, const T *__restrict__ Cc,
+                                const T *__restrict__ B, long blk0, long L, T &cpv, T &bpv) {
+    const long k0 = blk0 > TR_W ? blk0 - TR_W : 0;
+    long r = k0 % L;
+    T c = T(0), bb = T(0);
+    for (long k = k0; k < blk0; k++, r = (r + 1 == L) ? 0 : r + 1) {
+        const int kind = r == 0 ? 1 : (r == L - 1 ? 2 : 0);
+        T ck, bk;
+        fwd_row<T>(kind, A[k], D[k], Cc[k], B[k], kind == 1 ? D[k + 1] : T(1), c, bb, ck, bk);
+        c = ck;
+        bb = bk;
+    }
+    cpv = c;
+    bpv = bb;
+}
+template <typename T>
+__device__ inline T warm_bwd(const T *CP, const T *BP, long n, long k1, long L) {
+    const long k2 = min(n, k1 + TR_W);
+    long r = (k2 - 1) % L;
+    T xw = T(0);
+    for (long k = k2 - 1; k >= k1; k--, r = (r == 0) ? L - 1 : r - 1)
+        xw = (r == L - 1 && r != 0) ? BP[k] : BP[k] - (CP[k] * xw);
+    return xw;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__restrict__ A, const T *__restrict__ D,
+                                                   const T *__restrict__ Cc, const T *__restrict__ B,
+                                                   T *__restrict__ CP, T *__restrict__ BP, T *__restrict__ blk_c,
+                                                   T *__restrict__ blk_b) {
+    __shared__ T sbuf[TR_T * TR_P];
+    __shared__ T sc[TR_T], sb[TR_T];
     __shared__ int changed;
     const int t = threadIdx.x;
     const long blk0 = (long)blockIdx.x * TR_R;
     const int nrow = (int)min((long)TR_R, n - blk0);
     const long seg0 = blk0 + (long)t * TR_S;
     const int len = max(0, min(TR_S, nrow - t * TR_S));
-    double a[TR_S], d[TR_S], c[TR_S], b[TR_S], cp[TR_S], bp[TR_S];
+    T a[TR_S], d[TR_S], c[TR_S], b[TR_S], cp[TR_S], bp[TR_S];
     stage_in(A, blk0, nrow, sbuf, a);
     stage_in(D, blk0, nrow, sbuf, d);
     stage_in(Cc, blk0, nrow, sbuf, c);
     stage_in(B, blk0, nrow, sbuf, b);
     const unsigned kinds = seg_kinds(seg0, L);
     // the first-pivot quirk needs the original diagonal of the row after a chain start
-#define DNEXT(s) (SKIND(kinds, s) == 1 ? ((s) + 1 < len ? d[((s) + 1) % TR_S] : D[seg0 + (s) + 1]) : 1.0)
-    // speculative run from the zero state
-    double cpv = 0.0, bpv = 0.0;
+#define DNEXT(s) (SKIND(kinds, s) == 1 ? ((s) + 1 < len ? d[((s) + 1) % TR_S] : D[seg0 + (s) + 1]) : T(1))
+    // speculative run: lane 0 from the warmed-up state entering the block,
+    // the others from the zero state
+    T cpv = T(0), bpv = T(0);
+    if (t == 0 && blk0 > 0) warm_fwd<T>(A, D, Cc, B, blk0, L, cpv, bpv);
 #pragma unroll
     for (int s = 0; s < TR_S; s++) {
         if (s < len) {
-            fwd_row(SKIND(kinds, s), a[s], d[s], c[s], b[s], DNEXT(s), cpv, bpv, cp[s], bp[s]);
+            fwd_row<T>(SKIND(kinds, s), a[s], d[s], c[s], b[s], DNEXT(s), cpv, bpv, cp[s], bp[s]);
             cpv = cp[s]; bpv = bp[s];
         } else {
-            cp[s] = 0.0; bp[s] = 0.0;
+            cp[s] = T(0); bp[s] = T(0);
         }
     }
     // fix-up rounds inside the workgroup
@@ -136,13 +173,13 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const double 
         if (t == 0) changed = 0;
         __syncthreads();
         if (dep) {
-            double sc0 = sc[t - 1], sb0 = sb[t - 1];
+            T sc0 = sc[t - 1], sb0 = sb[t - 1];
             bool live = true;
 #pragma unroll
             for (int s = 0; s < TR_S; s++) {
                 if (live && s < len) {
-                    double ck, bk;
-                    fwd_row(SKIND(kinds, s), a[s], d[s], c[s], b[s], DNEXT(s), sc0, sb0, ck, bk);
+                    T ck, bk;
+                    fwd_row<T>(SKIND(kinds, s), a[s], d[s], c[s], b[s], DNEXT(s), sc0, sb0, ck, bk);
                     if (same(ck, cp[s]) && same(bk, bp[s])) {
                         live = false;
                     } else {
@@ -171,14 +208,15 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const double 
 // the recomputed values coincide bit for bit with the speculative ones; the
 // repaired rows are written back), then back-substitutes its segments with
 // the same speculation and in-workgroup fix-up rounds.
-__global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double *__restrict__ A,
-                                                   const double *__restrict__ D, const double *__restrict__ Cc,
-                                                   const double *__restrict__ B, double *__restrict__ CP,
-                                                   double *__restrict__ BP, const double *__restrict__ blk_c,
-                                                   const double *__restrict__ blk_b, double *__restrict__ X,
-                                                   double *__restrict__ blk_x, int *__restrict__ fail) {
-    __shared__ double scp[TR_T * TR_P], sbp[TR_T * TR_P];
-    __shared__ double sx[TR_T];
+template <typename T>
+__global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__restrict__ A, const T *__restrict__ D,
+                                                   const T *__restrict__ Cc, const T *__restrict__ B,
+                                                   T *__restrict__ CP, T *__restrict__ BP,
+                                                   const T *__restrict__ blk_c, const T *__restrict__ blk_b,
+                                                   T *__restrict__ X, T *__restrict__ blk_x, T *__restrict__ blk_xe,
+                                                   int *__restrict__ fail) {
+    __shared__ T scp[TR_T * TR_P], sbp[TR_T * TR_P];
+    __shared__ T sx[TR_T];
     __shared__ int changed;
     const int t = threadIdx.x;
     const long blk0 = (long)blockIdx.x * TR_R;
@@ -197,14 +235,14 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double 
     if (t == 0 && blockIdx.x > 0) {
         long r = blk0 % L;   // position of the block's first row in its chain
         if (r != 0) {
-            double cpv = blk_c[blockIdx.x - 1], bpv = blk_b[blockIdx.x - 1];
+            T cpv = blk_c[blockIdx.x - 1], bpv = blk_b[blockIdx.x - 1];
             int k = 0;
             for (; k < nrow; k++, r = (r + 1 == L) ? 0 : r + 1) {
                 const int kind = r == 0 ? 1 : (r == L - 1 ? 2 : 0);
                 const long g = blk0 + k;
-                const double dn = (kind == 1) ? D[g + 1] : 1.0;
-                double ck, bk;
-                fwd_row(kind, A[g], D[g], Cc[g], B[g], dn, cpv, bpv, ck, bk);
+                const T dn = (kind == 1) ? D[g + 1] : T(1);
+                T ck, bk;
+                fwd_row<T>(kind, A[g], D[g], Cc[g], B[g], dn, cpv, bpv, ck, bk);
                 const int q = (k / TR_S) * TR_P + k % TR_S;
                 if (same(ck, scp[q]) && same(bk, sbp[q])) break;
                 scp[q] = ck; sbp[q] = bk; CP[g] = ck; BP[g] = bk;
@@ -214,18 +252,24 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double 
         }
     }
     __syncthreads();
-    double cp[TR_S], bp[TR_S], x[TR_S];
+    T cp[TR_S], bp[TR_S], x[TR_S];
 #pragma unroll
     for (int q = 0; q < TR_S; q++) { cp[q] = scp[t * TR_P + q]; bp[q] = sbp[t * TR_P + q]; }
     const unsigned kinds = seg_kinds(seg0, L);
-    double xn = 0.0;
+    // the lane with the block's last row starts from the warmed-up value of the
+    // next block's first row (recorded for the check in k_trid_finish)
+    T xn = T(0);
+    if (len > 0 && t * TR_S + len == nrow && blk0 + nrow < n) {
+        xn = warm_bwd<T>(CP, BP, n, blk0 + nrow, L);
+        blk_xe[blockIdx.x] = xn;
+    }
 #pragma unroll
     for (int s = TR_S - 1; s >= 0; s--) {
         if (s < len) {
             x[s] = (SKIND(kinds, s) == 2) ? bp[s] : bp[s] - (cp[s] * xn);
             xn = x[s];
         } else {
-            x[s] = 0;
+            x[s] = T(0);
         }
     }
     const bool dep = len > 0 && SKIND(kinds, len - 1) != 2 && t * TR_S + len < nrow;
@@ -234,12 +278,12 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double 
         if (t == 0) changed = 0;
         __syncthreads();
         if (dep) {
-            double xv = sx[t + 1];
+            T xv = sx[t + 1];
             bool live = true;
 #pragma unroll
             for (int s = TR_S - 1; s >= 0; s--) {
                 if (live && s < len) {
-                    const double xk = (SKIND(kinds, s) == 2) ? bp[s] : bp[s] - (cp[s] * xv);
+                    const T xk = (SKIND(kinds, s) == 2) ? bp[s] : bp[s] - (cp[s] * xv);
                     if (same(xk, x[s])) live = false; else { x[s] = xk; xv = xk; }
                 }
             }
@@ -257,20 +301,22 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const double 
 // then, if any repair failed to coalesce inside its block, solve every chain
 // sequentially (exact in every case).
 constexpr int TRF_T = 1024;
-__global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk, const double *__restrict__ A,
-                                                       const double *__restrict__ D, const double *__restrict__ Cc,
-                                                       const double *__restrict__ B, double *__restrict__ CP,
-                                                       double *__restrict__ BP, double *__restrict__ X,
-                                                       const double *__restrict__ blk_x, int *__restrict__ fail,
-                                                       int *__restrict__ nfallback) {
+template <typename T>
+__global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk, const T *__restrict__ A,
+                                                       const T *__restrict__ D, const T *__restrict__ Cc,
+                                                       const T *__restrict__ B, T *__restrict__ CP,
+                                                       T *__restrict__ BP, T *__restrict__ X,
+                                                       const T *__restrict__ blk_x, const T *__restrict__ blk_xe,
+                                                       int *__restrict__ fail, int *__restrict__ nfallback) {
     for (int blk = threadIdx.x; blk < nblk - 1; blk += TRF_T) {
         const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
         long r = (k1 - 1) % L;   // position of row k in its chain
         if (r == L - 1 && r != 0) continue;
-        double xv = blk_x[blk + 1];
+        T xv = blk_x[blk + 1];
+        if (same(xv, blk_xe[blk])) continue;   // the block started from the exact value
         long k = k1 - 1;
         for (; k >= k0; k--, r = (r == 0) ? L - 1 : r - 1) {
-            const double xk = (r == L - 1 && r != 0) ? BP[k] : BP[k] - (CP[k] * xv);
+            const T xk = (r == L - 1 && r != 0) ? BP[k] : BP[k] - (CP[k] * xv);
             if (same(xk, X[k])) break;
             X[k] = xk;
             xv = xk;
@@ -281,13 +327,13 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
     if (threadIdx.x != 0) return;
     if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
     for (long c0 = 0; c0 < n; c0 += L) {
-        double cpv = 0, bpv = 0;
+        T cpv = T(0), bpv = T(0);
         for (long k = c0; k < c0 + L; k++) {
             const int kind = row_kind(k, L);
-            fwd_row(kind, A[k], D[k], Cc[k], B[k], kind == 1 ? D[k + 1] : 1.0, cpv, bpv, CP[k], BP[k]);
+            fwd_row<T>(kind, A[k], D[k], Cc[k], B[k], kind == 1 ? D[k + 1] : T(1), cpv, bpv, CP[k], BP[k]);
             cpv = CP[k]; bpv = BP[k];
         }
-        double xv = 0;
+        T xv = T(0);
         for (long k = c0 + L - 1; k >= c0; k--) {
             X[k] = (row_kind(k, L) == 2) ? BP[k] : BP[k] - (CP[k] * xv);
             xv = X[k];
@@ -299,14 +345,19 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
 
 int trid_blocks(long n) { return (int)((n + TR_R - 1) / TR_R); }
 
-void launch_trid(const TridWork &w, long n, long L, const double *A, const double *D, const double *C,
-                 const double *B, double *X, hipStream_t s) {
+template <typename T>
+void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
+                 hipStream_t s) {
     const int nb = trid_blocks(n);
-    hipLaunchKernelGGL(k_trid_fwd, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b);
-    hipLaunchKernelGGL(k_trid_bwd, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b, X,
-                       w.blk_x, w.fail);
-    hipLaunchKernelGGL(k_trid_finish, dim3(1), dim3(TRF_T), 0, s, n, L, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
-                       w.fail, w.nfallback);
+    hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b);
+    hipLaunchKernelGGL(k_trid_bwd<T>, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
+                       X, w.blk_x, w.blk_xe, w.fail);
+    hipLaunchKernelGGL(k_trid_finish<T>, dim3(1), dim3(TRF_T), 0, s, n, L, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
+                       w.blk_xe, w.fail, w.nfallback);
 }
+template void launch_trid<double>(const TridWorkT<double> &, long, long, const double *, const double *,
+                                  const double *, const double *, double *, hipStream_t);
+template void launch_trid<float>(const TridWorkT<float> &, long, long, const float *, const float *, const float *,
+                                 const float *, float *, hipStream_t);
 
 }  // namespace wd2

@@ -69,6 +69,10 @@ struct wd2_ctx {
     MomTemps T{};
     Chain ch{};
     TridWork tw{};
+    // FP32 views of the same buffers for the mixed-precision momentum
+    MomTempsT<float> Tf{};
+    ChainT<float> chf{};
+    TridWorkT<float> twf{};
     double *partials = nullptr;
     Scalars *sc = nullptr;
     int *tile_flags = nullptr;
@@ -143,13 +147,21 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->ch.b = take(nchain);
     c->ch.x = take(nchain);
     const int nb = trid_blocks((long)nchain) + 2;
-    HIPCHK(hipMalloc(&c->tw.cp, (2 * nchain + 3 * nb) * sizeof(double) + 2 * sizeof(int) + 64));
+    HIPCHK(hipMalloc(&c->tw.cp, (2 * nchain + 4 * nb) * sizeof(double) + 2 * sizeof(int) + 64));
     c->tw.bp = c->tw.cp + nchain;
     c->tw.blk_c = c->tw.bp + nchain;
     c->tw.blk_b = c->tw.blk_c + nb;
     c->tw.blk_x = c->tw.blk_b + nb;
-    c->tw.fail = (int *)(c->tw.blk_x + nb);
+    c->tw.blk_xe = c->tw.blk_x + nb;
+    c->tw.fail = (int *)(c->tw.blk_xe + nb);
     c->tw.nfallback = c->tw.fail + 1;
+    auto F = [](double *q) { return reinterpret_cast<float *>(q); };
+    c->Tf = MomTempsT<float>{F(c->T.cj1),  F(c->T.cj2),  F(c->T.c1s),  F(c->T.c2s), F(c->T.c1n),
+                             F(c->T.c2n),  F(c->T.cnvs), F(c->T.cnvn), F(c->T.difs), F(c->T.difn),
+                             F(c->T.cps),  F(c->T.cpn),  F(c->T.cpj)};
+    c->chf = ChainT<float>{F(c->ch.a), F(c->ch.d), F(c->ch.c), F(c->ch.b), F(c->ch.x)};
+    c->twf = TridWorkT<float>{F(c->tw.cp),    F(c->tw.bp),     F(c->tw.blk_c), F(c->tw.blk_b),
+                              F(c->tw.blk_x), F(c->tw.blk_xe), c->tw.fail,     c->tw.nfallback};
     HIPCHK(hipMemsetAsync(c->tw.fail, 0, 2 * sizeof(int), c->s));
     const size_t npart = 4 * (size_t)RED_MAX_BLOCKS + 2 * (size_t)(c->nf / 256 + 64);
     HIPCHK(hipMalloc(&c->partials, npart * sizeof(double)));
@@ -293,31 +305,42 @@ static int check_err(wd2_ctx *c) {
 }
 
 // -------------------------------------------------------------- momentum
+// XMomentum (comp 1) / YMomentum (comp 2) on temporaries and chains of type T
+// (double: the reference's arithmetic; float: the mixed-precision momentum).
+template <typename T>
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
-                    const double *un, const double *vn, double *out) {
+                    const double *un, const double *vn, const MomTempsT<T> &t, const ChainT<T> &ch,
+                    const TridWorkT<T> &tw, double *out) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     const Metrics &M = c->M;
     hipStream_t s = c->s;
     if (comp == 1) {
-        launch_conv_coef(nx, ny, ld, 4, 1, M.xzu, M.xeu, M.yzu, M.yeu, us, vs, c->T.cj1, c->T.cj2, s);
-        launch_conv_coef(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, us, vs, c->T.c1s, c->T.c2s, s);
-        launch_conv_coef(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, un, vn, c->T.c1n, c->T.c2n, s);
+        launch_conv_coef<T>(nx, ny, ld, 4, 1, M.xzu, M.xeu, M.yzu, M.yeu, us, vs, t.cj1, t.cj2, s);
+        launch_conv_coef<T>(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, us, vs, t.c1s, t.c2s, s);
+        launch_conv_coef<T>(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, un, vn, t.c1n, t.c2n, s);
     } else {
-        launch_conv_coef(nx, ny, ld, 5, 1, M.xzv, M.xev, M.yzv, M.yev, us, vs, c->T.cj1, c->T.cj2, s);
-        launch_conv_coef(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, us, vs, c->T.c1s, c->T.c2s, s);
-        launch_conv_coef(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, un, vn, c->T.c1n, c->T.c2n, s);
+        launch_conv_coef<T>(nx, ny, ld, 5, 1, M.xzv, M.xev, M.yzv, M.yev, us, vs, t.cj1, t.cj2, s);
+        launch_conv_coef<T>(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, us, vs, t.c1s, t.c2s, s);
+        launch_conv_coef<T>(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, un, vn, t.c1n, t.c2n, s);
     }
-    if (c->any_porous) launch_porous_div(c->R, ld, comp, c->T, s);
+    if (c->any_porous) launch_porous_div<T>(c->R, ld, comp, t, s);
     const double *fs = comp == 1 ? us : vs, *fn = comp == 1 ? un : vn;
-    launch_conv_diff(nx, ny, ld, comp, M, fs, fn, c->T, s);
-    if (c->any_porous) launch_poros_coef(c->R, ld, comp, us, vs, un, vn, c->T, s);
+    launch_conv_diff<T>(nx, ny, ld, comp, M, fs, fn, t, s);
+    if (c->any_porous) launch_poros_coef<T>(c->R, ld, comp, us, vs, un, vn, t, s);
     const long n = comp == 1 ? (long)nx * (ny - 1) : (long)(nx - 1) * ny;
-    launch_assemble(c->R, ld, comp, 1, dk, re, fr, M, c->T, fs, fn, c->d, c->dn, nullptr, c->ch, s, &c->sc->err);
-    launch_trid(c->tw, n, n, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
-    launch_assemble(c->R, ld, comp, 2, dk, re, fr, M, c->T, fs, fn, c->d, c->dn, c->ch.x, c->ch, s, &c->sc->err);
-    launch_trid(c->tw, n, n, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
-    launch_scatter(nx, ny, ld, comp, c->ch.x, out, s);
+    launch_assemble<T>(c->R, ld, comp, 1, dk, re, fr, M, t, fs, fn, c->d, c->dn, nullptr, ch, s, &c->sc->err);
+    launch_trid<T>(tw, n, n, ch.a, ch.d, ch.c, ch.b, ch.x, s);
+    launch_assemble<T>(c->R, ld, comp, 2, dk, re, fr, M, t, fs, fn, c->d, c->dn, ch.x, ch, s, &c->sc->err);
+    launch_trid<T>(tw, n, n, ch.a, ch.d, ch.c, ch.b, ch.x, s);
+    launch_scatter<T>(nx, ny, ld, comp, ch.x, out, s);
     return WD2_OK;
+}
+
+static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
+                    const double *un, const double *vn, double *out) {
+    if (c->precision == WD2_PREC_MIXED)
+        return momentum<float>(c, comp, dk, re, fr, us, vs, un, vn, c->Tf, c->chf, c->twf, out);
+    return momentum<double>(c, comp, dk, re, fr, us, vs, un, vn, c->T, c->ch, c->tw, out);
 }
 
 // momentum.f:nAuxMomentum 33-193 on the context's un, vn -> us, vs.
@@ -884,6 +907,14 @@ int wd2_ctx_set_precision(wd2_ctx *c, int precision) {
     if (!c) return fail(WD2_ERR_ARG, "wd2_ctx_set_precision: null context");
     if (precision != WD2_PREC_FP64 && precision != WD2_PREC_MIXED)
         return fail(WD2_ERR_ARG, "wd2_ctx_set_precision: unknown precision %d", precision);
+    if (precision != c->precision) {
+        // The FP32 momentum views alias the FP64 temporaries and chains.  Entries
+        // the momentum kernels never write are read as 0 (as in the reference),
+        // so clear the whole block when the arithmetic changes.
+        const size_t nchain = (size_t)c->nx * c->ny + 64;
+        HIPCHK(hipMemsetAsync(c->T.cj1, 0, (13 * c->nf + 5 * nchain) * sizeof(double), c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    }
     c->precision = precision;
     return WD2_OK;
 }

@@ -7,15 +7,20 @@
 
 namespace wd2 {
 
-// Scratch of the parallel AltTridLU for chains of up to nmax rows.
-struct TridWork {
-    double *cp, *bp;              // forward-swept super-diagonal / right-hand side
-    double *blk_c, *blk_b, *blk_x;  // per-workgroup boundary states
-    int *fail, *nfallback;        // device flags
+// Scratch of the parallel AltTridLU for chains of up to nmax rows (T: double
+// for the reference's arithmetic, float for the mixed-precision momentum).
+template <typename T>
+struct TridWorkT {
+    T *cp, *bp;              // forward-swept super-diagonal / right-hand side
+    T *blk_c, *blk_b, *blk_x;  // per-workgroup boundary states
+    T *blk_xe;                 // per-workgroup speculated x of the next block's first row
+    int *fail, *nfallback;     // device flags
 };
+using TridWork = TridWorkT<double>;
 int trid_blocks(long n);
-void launch_trid(const TridWork &w, long n, long L, const double *A, const double *D, const double *C,
-                 const double *B, double *X, hipStream_t s);
+template <typename T>
+void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
+                 hipStream_t s);
 
 // Boundary fills (one workgroup).  err receives 1000+side for a bad flag.
 template <typename T>
@@ -109,25 +114,36 @@ void launch_slor_finish(int nparts, double sortol, int msorit, double *partials,
 void launch_slor_rbp_relax(int nx, int ny, int ld, int first_line, double sorrel, const double *pn, double *p,
                            hipStream_t s);
 
-// Momentum (momentum.f)
-struct MomTemps {
-    double *cj1, *cj2, *c1s, *c2s, *c1n, *c2n, *cnvs, *cnvn, *difs, *difn, *cps, *cpn, *cpj;
+// Momentum (momentum.f).  T: type of the temporaries, chains and arithmetic
+// (double: the reference's; float: the mixed-precision path).
+template <typename T>
+struct MomTempsT {
+    T *cj1, *cj2, *c1s, *c2s, *c1n, *c2n, *cnvs, *cnvn, *difs, *difn, *cps, *cpn, *cpj;
 };
-struct Chain {
-    double *a, *d, *c, *b, *x;
+template <typename T>
+struct ChainT {
+    T *a, *d, *c, *b, *x;
 };
+using MomTemps = MomTempsT<double>;
+using Chain = ChainT<double>;
+template <typename T>
 void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const double *xzi, const double *xet,
-                      const double *yzi, const double *yet, const double *u, const double *v, double *cc1,
-                      double *cc2, hipStream_t s);
-void launch_porous_div(const Regions &R, int ld, int comp, const MomTemps &t, hipStream_t s);
-void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, const double *vs,
-                       const double *un, const double *vn, const MomTemps &t, hipStream_t s);
+                      const double *yzi, const double *yet, const double *u, const double *v, T *cc1, T *cc2,
+                      hipStream_t s);
+template <typename T>
+void launch_porous_div(const Regions &R, int ld, int comp, const MomTempsT<T> &t, hipStream_t s);
+template <typename T>
+void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, const double *vs, const double *un,
+                       const double *vn, const MomTempsT<T> &t, hipStream_t s);
+template <typename T>
 void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *un,
-                      const MomTemps &t, hipStream_t s);
+                      const MomTempsT<T> &t, hipStream_t s);
+template <typename T>
 void launch_assemble(const Regions &R, int ld, int comp, int split, double dk, double re, double fr,
-                     const Metrics &m, const MomTemps &t, const double *us, const double *un, const double *d,
-                     const double *dn, const double *xprev, const Chain &ch, hipStream_t s, int *err);
-void launch_scatter(int nx, int ny, int ld, int comp, const double *x, double *out, hipStream_t s);
+                     const Metrics &m, const MomTempsT<T> &t, const double *us, const double *un, const double *d,
+                     const double *dn, const T *xprev, const ChainT<T> &ch, hipStream_t s, int *err);
+template <typename T>
+void launch_scatter(int nx, int ny, int ld, int comp, const T *x, double *out, hipStream_t s);
 void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const double *dus, const double *dvs,
                          double *partials, Scalars *sc, hipStream_t s);
 
