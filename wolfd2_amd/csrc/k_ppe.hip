@@ -255,29 +255,42 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
 #pragma unroll
     for (int r = 0; r < TP_PAIRS; r++) {
         const int x = PX(r), y = PY(r);
+        const int i = i0 + x, j = j0 + y;   // i is even: the pair (i, i+1) is 16-byte aligned
+        const double2 pv = load_pair(p_in, ld, nx, ny, i, j);
+        P[y * TLD + x] = pv.x;
+        P[y * TLD + x + 1] = pv.y;
+        const bool rowin = j >= 2 && j <= ny && y >= 1 && y <= TW - 2;
+        const bool upd0 = rowin && i >= 2 && i <= nx && x >= 1;
+        const bool upd1 = rowin && i + 1 >= 2 && i + 1 <= nx && x + 1 <= TW - 2;
+        // coefficients as aligned pairs (i >= 2 and i + 1 <= nx + 1 when a point updates)
+        double2 bv = make_double2(0.0, 0.0), g0 = make_double2(0.0, 0.0), g1 = make_double2(1.0, 1.0);
+        double2 av = make_double2(0.0, 0.0);
+        double aw = 0.0;
+        if (upd0 || upd1) {
+            bv = *reinterpret_cast<const double2 *>(b + WD2_IX(i, j));
+            g0 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j - 1));
+            g1 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j));
+            av = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
+            aw = rau[WD2_IX(i - 1, j)];
+        }
         int f = 0;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const int i = i0 + x + h, j = j0 + y;
-            const bool inarr = i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1;
-            P[y * TLD + x + h] = inarr ? p_in[WD2_IX(i, j)] : 0.0;
-            const bool upd = i >= 2 && i <= nx && j >= 2 && j <= ny && x + h >= 1 && x + h <= TW - 2 && y >= 1 &&
-                             y <= TW - 2;
+            const bool upd = h ? upd1 : upd0;
             const bool own = upd && x + h >= TH && x + h < TH + TT && y >= TH && y < TH + TT;
             double bb = 0, a1 = 0, a2 = 0, a4 = 0, a5 = 1;
             if (upd) {
-                const size_t k = WD2_IX(i, j);
-                bb = b[k];
-                a1 = rgv[WD2_IX(i, j - 1)];
-                a2 = rau[WD2_IX(i - 1, j)];
-                a4 = rau[k];
-                a5 = rgv[k];
+                bb = h ? bv.y : bv.x;
+                a1 = h ? g0.y : g0.x;
+                a2 = h ? av.x : aw;
+                a4 = h ? av.y : av.x;
+                a5 = h ? g1.y : g1.x;
             }
             cb[r][h] = bb; c1[r][h] = a1; c2[r][h] = a2; c4[r][h] = a4; c5[r][h] = a5;
             f |= (upd ? 1 : 0) << h;
             f |= (own ? 1 : 0) << (2 + h);
         }
-        f |= ((i0 + x + j0 + y) & 1) << 4;   // black iff (i + j) even
+        f |= ((i + j) & 1) << 4;   // black iff (i + j) even
         flags[r] = f;
     }
     __syncthreads();

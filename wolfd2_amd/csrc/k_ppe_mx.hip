@@ -38,8 +38,8 @@ constexpr int MLD = MW + 1;                                       // padded LDS 
 static_assert(MW * MW / 2 == MX_PAIRS * MX_T, "tile pairs must divide evenly");
 
 __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, const double *__restrict__ rau,
-                                                       const double *__restrict__ rgv,
-                                                       const double *__restrict__ b, double sorrel, int nsweep,
+                                                       const double *__restrict__ rgv, const double *__restrict__ b,
+                                                       double sorrel, int nsweep,
                                                        const double *__restrict__ p_in, double *__restrict__ p_out,
                                                        unsigned *__restrict__ masks,
                                                        const unsigned *__restrict__ prev_masks, int prev_nsweep,
@@ -74,26 +74,16 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
 #define QX(r) (2 * ((t + MX_T * (r)) % (MW / 2)))
 #define QY(r) ((t + MX_T * (r)) / (MW / 2))
 
-    // p into LDS (FP64), e = 0
+    // p of the tile into LDS (FP64; 0 outside the array) as aligned pairs, e = 0
 #pragma unroll
     for (int r = 0; r < MX_PAIRS; r++) {
         const int x = QX(r), y = QY(r);
-        const int i = i0 + x, j = j0 + y;
-        double v0 = 0.0, v1 = 0.0;
-        if (j >= 0 && j <= ny + 1) {
-            if (i >= 0 && i + 1 <= nx + 1) {
-                const double2 pv = *reinterpret_cast<const double2 *>(p_in + WD2_IX(i, j));
-                v0 = pv.x;
-                v1 = pv.y;
-            } else {
-                if (i >= 0 && i <= nx + 1) v0 = p_in[WD2_IX(i, j)];
-                if (i + 1 >= 0 && i + 1 <= nx + 1) v1 = p_in[WD2_IX(i + 1, j)];
-            }
-        }
-        P[y * MLD + x] = v0;
-        P[y * MLD + x + 1] = v1;
-        E[y * MLD + x] = 0.f;
-        E[y * MLD + x + 1] = 0.f;
+        const int c = y * MLD + x;
+        const double2 pv = load_pair(p_in, ld, nx, ny, i0 + x, j0 + y);
+        P[c] = pv.x;
+        P[c + 1] = pv.y;
+        E[c] = 0.f;
+        E[c + 1] = 0.f;
     }
     __syncthreads();
 
@@ -105,15 +95,22 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     for (int r = 0; r < MX_PAIRS; r++) {
         const int x = QX(r), y = QY(r);
         const int i = i0 + x, j = j0 + y;
+        const int c0 = y * MLD + x;
         int f = 0;
-        double ra[3] = {0.0, 0.0, 0.0};
         const bool rowin = j >= 2 && j <= ny && y >= 1 && y <= MW - 2;
         const bool upd0 = rowin && i >= 2 && i <= nx && x >= 1;
         const bool upd1 = rowin && i + 1 >= 2 && i + 1 <= nx && x + 1 <= MW - 2;
+        double ra[3] = {0.0, 0.0, 0.0};
+        double2 bv = make_double2(0.0, 0.0), g0 = bv, g1 = bv;
         if (upd0 || upd1) {
+            // i >= 1 and i + 1 <= nx + 1 here: the pairs are inside the array
             ra[0] = rau[WD2_IX(i - 1, j)];
-            ra[1] = rau[WD2_IX(i, j)];
-            ra[2] = rau[WD2_IX(i + 1, j)];
+            const double2 av = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
+            ra[1] = av.x;
+            ra[2] = av.y;
+            bv = *reinterpret_cast<const double2 *>(b + WD2_IX(i, j));
+            g0 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j - 1));
+            g1 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j));
         }
         cw[r] = (float)ra[0];
         cc[r] = (float)ra[1];
@@ -124,17 +121,16 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
             const bool own = upd && x + h >= MH && x + h < MH + MT && y >= MH && y < MH + MT;
             float rr = 0.f, a1f = 0.f, a5f = 0.f, i3f = 0.f;
             if (upd) {
-                const size_t k = WD2_IX(i + h, j);
-                const double a1 = rgv[WD2_IX(i + h, j - 1)], a5 = rgv[k];
+                const double a1 = h ? g0.y : g0.x, a5 = h ? g1.y : g1.x;
                 const double a2 = ra[h], a4 = ra[h + 1];
                 const double a3 = -a4 - a2 - a5 - a1;
-                const int c = y * MLD + x + h;
-                const double res = b[k] - a1 * P[c - MLD] - a2 * P[c - 1] - a3 * P[c] - a4 * P[c + 1] -
-                                   a5 * P[c + MLD];
+                const int c = c0 + h;
+                const double res = (h ? bv.y : bv.x) - a1 * P[c - MLD] - a2 * P[c - 1] - a3 * P[c] -
+                                   a4 * P[c + 1] - a5 * P[c + MLD];
                 rr = (float)res;
                 a1f = (float)a1;
                 a5f = (float)a5;
-                i3f = (float)(1.0 / a3);
+                i3f = 1.0f / (float)a3;   // only scales the FP32 correction sweeps
             }
             cr[r][h] = rr; c1[r][h] = a1f; c5[r][h] = a5f; ci3[r][h] = i3f;
             f |= (upd ? 1 : 0) << h;

@@ -56,6 +56,18 @@ __device__ inline unsigned long long dbits(double x) { return (unsigned long lon
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ inline double dmaxf(double a, double b) { return a > b ? a : b; }
 
+// f(i,j), f(i+1,j) for even i (a 16-byte aligned pair: ld is a multiple of 16
+// and every field starts 128-byte aligned), 0 for points outside the array
+// (i outside 0..nx+1 or j outside 0..ny+1).
+__device__ inline double2 load_pair(const double *__restrict__ f, int ld, int nx, int ny, int i, int j) {
+    double2 v = make_double2(0.0, 0.0);
+    if (j < 0 || j > ny + 1) return v;
+    if (i >= 0 && i + 1 <= nx + 1) return *reinterpret_cast<const double2 *>(f + (size_t)i + (size_t)ld * j);
+    if (i >= 0 && i <= nx + 1) v.x = f[(size_t)i + (size_t)ld * j];
+    if (i + 1 >= 0 && i + 1 <= nx + 1) v.y = f[(size_t)i + 1 + (size_t)ld * j];
+    return v;
+}
+
 // Load that bypasses the (non-coherent) per-CU cache: used to read values that
 // other workgroups of the same kernel published before an atomic ticket.
 __device__ inline double ld_coherent(const double *p) {
