@@ -197,11 +197,11 @@ void launch_sor_rb_check(int nx, int ny, int sweep, double sortol, int msorit, c
 // the outermost ring are never updated; points outside the array are never
 // read by an updated point.
 //
-// Only p lives in LDS (64 x 65 doubles).  Each of the 512 lanes owns four
-// horizontal pairs of points, with their right-hand side and coefficients
-// (b, a1 = rgv(i,j-1), a2 = rau(i-1,j), a4 = rau(i,j), a5 = rgv(i,j)) in
-// registers; in each half-sweep a lane updates the point of that colour of
-// each of its pairs.
+// Only p lives in LDS (64 x 64 doubles, stored split by colour).  Each of
+// the 512 lanes owns four horizontal pairs of points, with their right-hand
+// side and coefficients (b, a1 = rgv(i,j-1), a2 = rau(i-1,j), a4 = rau(i,j),
+// a5 = rgv(i,j)) in registers; in each half-sweep a lane updates the point of
+// that colour of each of its pairs.
 //
 // The kernel reads p_in and writes the owned points to p_out (tiles read
 // their neighbours' points as halo, so it cannot work in place).  The
@@ -214,7 +214,7 @@ void launch_sor_rb_check(int nx, int ny, int sweep, double sortol, int msorit, c
 constexpr int KT = 4, TH = 2 * KT, TW = 64, TT = TW - 2 * TH;   // 48 owned
 constexpr int TP_T = 512;
 constexpr int TP_PAIRS = TW * TW / 2 / TP_T;   // 4
-constexpr int TLD = TW + 1;                    // padded LDS row
+constexpr int TK = TW / 2;                     // pairs per tile row
 
 __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, const double *__restrict__ rau,
                                                     const double *__restrict__ rgv, const double *__restrict__ b,
@@ -224,7 +224,10 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
                                                     int prev_m0, int prev_launch, double sortol, int msorit,
                                                     int ntx, int ntiles, Scalars *sc, int force) {
     if (!force && sc->sor_done) return;
-    __shared__ double P[TW * TLD];
+    // p split by colour: PC[c][y*TK + k] is the point of colour c ((i+j) & 1)
+    // in pair k of tile row y, so that the neighbours a colour reads are
+    // unit-stride across the lanes of a wavefront
+    __shared__ double PC[2][TW * TK];
     __shared__ unsigned sh[TP_T / 64];
     if (!force && prev_nsweep > 0) {
         int conv = 0;
@@ -250,15 +253,17 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
     const int t = threadIdx.x;
     double cb[TP_PAIRS][2], c1[TP_PAIRS][2], c2[TP_PAIRS][2], c4[TP_PAIRS][2], c5[TP_PAIRS][2];
     int flags[TP_PAIRS];
-#define PX(r) (2 * ((t + TP_T * (r)) % (TW / 2)))
-#define PY(r) ((t + TP_T * (r)) / (TW / 2))
+#define PX(r) (2 * ((t + TP_T * (r)) % TK))
+#define PY(r) ((t + TP_T * (r)) / TK)
+#define PB(r) (t + TP_T * (r))   // y*TK + k of the lane's pair r
 #pragma unroll
     for (int r = 0; r < TP_PAIRS; r++) {
         const int x = PX(r), y = PY(r);
         const int i = i0 + x, j = j0 + y;   // i is even: the pair (i, i+1) is 16-byte aligned
         const double2 pv = load_pair(p_in, ld, nx, ny, i, j);
-        P[y * TLD + x] = pv.x;
-        P[y * TLD + x + 1] = pv.y;
+        const int lc = (i + j) & 1;   // colour of the left point
+        PC[lc][PB(r)] = pv.x;
+        PC[lc ^ 1][PB(r)] = pv.y;
         const bool rowin = j >= 2 && j <= ny && y >= 1 && y <= TW - 2;
         const bool upd0 = rowin && i >= 2 && i <= nx && x >= 1;
         const bool upd1 = rowin && i + 1 >= 2 && i + 1 <= nx && x + 1 <= TW - 2;
@@ -299,20 +304,24 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
         double mx = 0.0;
 #pragma unroll
         for (int color = 0; color < 2; color++) {
+            double *__restrict__ dst = PC[color];
+            const double *__restrict__ src = PC[1 - color];
 #pragma unroll
             for (int r = 0; r < TP_PAIRS; r++) {
-                const int h = ((flags[r] >> 4) & 1) ^ color;   // the pair's point of this colour
+                // h: the pair's point of this colour (0 left, 1 right); its west
+                // neighbour is the other colour in pair k-1+h, its east one in k+h
+                const int h = ((flags[r] >> 4) & 1) ^ color;
                 if ((flags[r] >> h) & 1) {
-                    const int c = PY(r) * TLD + PX(r) + h;
+                    const int q = PB(r);
                     // register selects (a runtime index into the arrays would go to scratch)
                     const double a1 = h ? c1[r][1] : c1[r][0], a2 = h ? c2[r][1] : c2[r][0];
                     const double a4 = h ? c4[r][1] : c4[r][0], a5 = h ? c5[r][1] : c5[r][0];
                     const double bb = h ? cb[r][1] : cb[r][0];
                     const double a3 = -a4 - a2 - a5 - a1;
-                    const double pij = P[c];
-                    double sum = bb - a1 * P[c - TLD] - a2 * P[c - 1] - a4 * P[c + 1] - a5 * P[c + TLD];
+                    const double pij = dst[q];
+                    double sum = bb - a1 * src[q - TK] - a2 * src[q - 1 + h] - a4 * src[q + h] - a5 * src[q + TK];
                     sum = sum / a3 - pij;
-                    P[c] = pij + sorrel * sum;
+                    dst[q] = pij + sorrel * sum;
                     if ((flags[r] >> (2 + h)) & 1) mx = dmaxf(mx, fabs(sum));
                 }
             }
@@ -321,15 +330,18 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
         if (mx < sortol) below |= 1u << s;
     }
 #pragma unroll
-    for (int r = 0; r < TP_PAIRS; r++)
+    for (int r = 0; r < TP_PAIRS; r++) {
+        const int lc = (flags[r] >> 4) & 1;
 #pragma unroll
         for (int h = 0; h < 2; h++)
             if ((flags[r] >> (2 + h)) & 1) {
                 const int x = PX(r) + h, y = PY(r);
-                p_out[WD2_IX(i0 + x, j0 + y)] = P[y * TLD + x];
+                p_out[WD2_IX(i0 + x, j0 + y)] = PC[lc ^ h][PB(r)];
             }
+    }
 #undef PX
 #undef PY
+#undef PB
     // tile mask = AND over lanes
     for (int off = 32; off > 0; off >>= 1) below &= (unsigned)__shfl_xor((int)below, off);
     if ((t & 63) == 0) sh[t >> 6] = below;

@@ -34,7 +34,8 @@ namespace wd2 {
 constexpr int KM = 8, MH = 2 * KM, MW = 96, MT = MW - 2 * MH;   // 64 owned per side
 constexpr int MX_T = 768;                                         // 12 wavefronts
 constexpr int MX_PAIRS = MW * MW / 2 / MX_T;                      // 6 pairs per lane
-constexpr int MLD = MW + 1;                                       // padded LDS row
+constexpr int MLD = MW + 1;                                       // padded LDS row of p
+constexpr int MK = MW / 2;                                        // pairs per tile row
 static_assert(MW * MW / 2 == MX_PAIRS * MX_T, "tile pairs must divide evenly");
 
 __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, const double *__restrict__ rau,
@@ -47,7 +48,10 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
                                                        int ntx, int ntiles, Scalars *sc, int force) {
     if (!force && sc->sor_done) return;
     __shared__ double P[MW * MLD];
-    __shared__ float E[MW * MLD];
+    // the correction e split by colour: EC[c][y*MK + k] is the point of colour
+    // c ((i+j) & 1) in pair k of tile row y, so that the neighbours a colour
+    // reads are unit-stride across the lanes of a wavefront
+    __shared__ float EC[2][MW * MK];
     __shared__ unsigned sh[MX_T / 64];
     if (!force && prev_nsweep > 0) {
         int conv = 0;
@@ -71,8 +75,9 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     const int TI = tile % ntx, TJ = tile / ntx;
     const int i0 = 2 + TI * MT - MH, j0 = 2 + TJ * MT - MH;   // tile origin (even: i0 + x is 16-byte aligned)
     const int t = threadIdx.x;
-#define QX(r) (2 * ((t + MX_T * (r)) % (MW / 2)))
-#define QY(r) ((t + MX_T * (r)) / (MW / 2))
+#define QX(r) (2 * ((t + MX_T * (r)) % MK))
+#define QY(r) ((t + MX_T * (r)) / MK)
+#define QB(r) (t + MX_T * (r))   // y*MK + k of the lane's pair r
 
     // p of the tile into LDS (FP64; 0 outside the array) as aligned pairs, e = 0
 #pragma unroll
@@ -82,8 +87,8 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
         const double2 pv = load_pair(p_in, ld, nx, ny, i0 + x, j0 + y);
         P[c] = pv.x;
         P[c + 1] = pv.y;
-        E[c] = 0.f;
-        E[c + 1] = 0.f;
+        EC[0][QB(r)] = 0.f;
+        EC[1][QB(r)] = 0.f;
     }
     __syncthreads();
 
@@ -147,18 +152,23 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
         float mx = 0.f;
 #pragma unroll
         for (int color = 0; color < 2; color++) {
+            float *__restrict__ dst = EC[color];
+            const float *__restrict__ src = EC[1 - color];
 #pragma unroll
             for (int r = 0; r < MX_PAIRS; r++) {
-                const int h = ((flags[r] >> 4) & 1) ^ color;   // the pair's point of this colour
+                // h: the pair's point of this colour (0 left, 1 right); its west
+                // neighbour is the other colour in pair k-1+h, its east one in k+h
+                const int h = ((flags[r] >> 4) & 1) ^ color;
                 if ((flags[r] >> h) & 1) {
-                    const int c = QY(r) * MLD + QX(r) + h;
+                    const int q = QB(r);
                     // register selects (a runtime index into the arrays would go to scratch)
                     const float a1 = h ? c1[r][1] : c1[r][0], a5 = h ? c5[r][1] : c5[r][0];
                     const float a2 = h ? cc[r] : cw[r], a4 = h ? ce[r] : cc[r];
                     const float rr = h ? cr[r][1] : cr[r][0], i3 = h ? ci3[r][1] : ci3[r][0];
-                    const float e = E[c];
-                    const float sum = (rr - a1 * E[c - MLD] - a2 * E[c - 1] - a4 * E[c + 1] - a5 * E[c + MLD]) * i3 - e;
-                    E[c] = e + w * sum;
+                    const float e = dst[q];
+                    const float sum =
+                        (rr - a1 * src[q - MK] - a2 * src[q - 1 + h] - a4 * src[q + h] - a5 * src[q + MK]) * i3 - e;
+                    dst[q] = e + w * sum;
                     if ((flags[r] >> (2 + h)) & 1) mx = fmaxf(mx, fabsf(sum));
                 }
             }
@@ -171,15 +181,17 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
 #pragma unroll
     for (int r = 0; r < MX_PAIRS; r++) {
         const int x = QX(r), y = QY(r);
+        const int lc = (flags[r] >> 4) & 1;   // colour of the pair's left point
 #pragma unroll
         for (int h = 0; h < 2; h++)
             if ((flags[r] >> (2 + h)) & 1) {
                 const int c = y * MLD + x + h;
-                p_out[WD2_IX(i0 + x + h, j0 + y)] = P[c] + (double)E[c];
+                p_out[WD2_IX(i0 + x + h, j0 + y)] = P[c] + (double)EC[lc ^ h][QB(r)];
             }
     }
 #undef QX
 #undef QY
+#undef QB
     for (int off = 32; off > 0; off >>= 1) below &= (unsigned)__shfl_xor((int)below, off);
     if ((t & 63) == 0) sh[t >> 6] = below;
     __syncthreads();
