@@ -79,6 +79,16 @@ struct wd2_ctx {
     int sweep_id = 0;
     bool coef_ready = false;
     int precision = WD2_PREC_FP64;
+    // pinned copies of the stop flag and events of the SOR batches in flight
+    int *h_flag = nullptr;
+    hipEvent_t bev[2] = {nullptr, nullptr};
+    // host transfers: two pinned staging buffers of one packed field each (the
+    // caller's arrays are pageable and strided), and the last uploaded content
+    // of every metric array of the drop-in calls (re-sent only when it changes)
+    double *h_stage[2] = {nullptr, nullptr};
+    hipEvent_t sev[2] = {nullptr, nullptr};
+    int stage_i = 0;
+    std::vector<std::vector<double>> met_shadow;
     // optional HIP-event timing of the red-black SOR kernels: event pairs,
     // each around kt_cnt[pair] back-to-back launches
     bool kt_on = false;
@@ -106,6 +116,13 @@ static inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 static void ctx_free(wd2_ctx *c) {
     if (!c) return;
     for (hipEvent_t e : c->kt_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->bev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->h_flag) (void)hipHostFree(c->h_flag);
+    for (int k = 0; k < 2; k++) {
+        if (c->sev[k]) (void)hipEventDestroy(c->sev[k]);
+        if (c->h_stage[k]) (void)hipHostFree(c->h_stage[k]);
+    }
     (void)hipFree(c->d_int);
     (void)hipFree(c->d_dbl);
     (void)hipFree(c->prow);
@@ -127,6 +144,13 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->ld = round_up(nx + 2, 16);
     c->nf = (size_t)c->ld * (ny + 2);
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void **)&c->h_flag, 2 * sizeof(int), hipHostMallocDefault));
+    for (hipEvent_t &e : c->bev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (int k = 0; k < 2; k++) {
+        HIPCHK(hipHostMalloc((void **)&c->h_stage[k], (size_t)(nx + 2) * (ny + 2) * sizeof(double),
+                             hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&c->sev[k], hipEventDisableTiming));
+    }
     const int NFIELDS = 16 + 5 + 13;
     const size_t nchain = (size_t)nx * ny + 64;
     const size_t total = NFIELDS * c->nf + 5 * nchain;
@@ -188,15 +212,31 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
 }
 
 // host (ld_h) -> device (c->ld) field copy of columns 0..ny+1, rows 0..nx+1
+// The rows are packed into a pinned staging buffer on the host, then sent in
+// one DMA copy; the two buffers alternate, so packing the next field overlaps
+// the previous copy.
 static int up(wd2_ctx *c, const double *h, int ld_h, double *dst) {
     if (!h) return WD2_OK;
-    HIPCHK(hipMemcpy2DAsync(dst, (size_t)c->ld * 8, h, (size_t)ld_h * 8, (size_t)(c->nx + 2) * 8, c->ny + 2,
-                            hipMemcpyHostToDevice, c->s));
+    const size_t w = (size_t)c->nx + 2, rows = (size_t)c->ny + 2;
+    const int k = c->stage_i;
+    c->stage_i ^= 1;
+    HIPCHK(hipEventSynchronize(c->sev[k]));   // the buffer's previous copy has finished
+    double *st = c->h_stage[k];
+    for (size_t j = 0; j < rows; j++) std::memcpy(st + j * w, h + j * (size_t)ld_h, w * sizeof(double));
+    HIPCHK(hipMemcpy2DAsync(dst, (size_t)c->ld * 8, st, w * 8, w * 8, rows, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipEventRecord(c->sev[k], c->s));
     return WD2_OK;
 }
+// device (c->ld) -> host (ld_h); the host array is filled when this returns
 static int down(wd2_ctx *c, const double *src, double *h, int ld_h) {
-    HIPCHK(hipMemcpy2DAsync(h, (size_t)ld_h * 8, src, (size_t)c->ld * 8, (size_t)(c->nx + 2) * 8, c->ny + 2,
-                            hipMemcpyDeviceToHost, c->s));
+    const size_t w = (size_t)c->nx + 2, rows = (size_t)c->ny + 2;
+    const int k = c->stage_i;
+    c->stage_i ^= 1;
+    HIPCHK(hipEventSynchronize(c->sev[k]));
+    double *st = c->h_stage[k];
+    HIPCHK(hipMemcpy2DAsync(st, w * 8, src, (size_t)c->ld * 8, w * 8, rows, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (size_t j = 0; j < rows; j++) std::memcpy(h + j * (size_t)ld_h, st + j * w, w * sizeof(double));
     return WD2_OK;
 }
 
@@ -399,36 +439,51 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         const bool mx = c->precision == WD2_PREC_MIXED;
         auto sweep = mx ? launch_sor_tiled_mx : launch_sor_tiled;
         const int K = mx ? sor_tiled_mx_sweeps() : sor_tiled_sweeps();
-        const int Q = mx ? 16 : 32;   // launches between two looks at the stop flag
         launch_copy(ld, 0, nx + 1, 0, ny + 1, p, c->pscr, s);
-        int m0 = 0, L = 0, done = 0;
+        int m0 = 0, L = 0;
         const int nt = mx ? sor_tiled_mx_tiles(nx, ny, nullptr) : sor_tiled_tiles(nx, ny, nullptr);
         unsigned *part[2] = {(unsigned *)c->partials, (unsigned *)c->partials + nt};
         int prev_k = 0, prev_m0 = 0;
-        while (m0 < msorit && !done) {
-            // timing: one event pair around the batch's launches (no marker
-            // between two launches, so their durations are not inflated)
-            hipEvent_t *ev = kt_events(c, 1);
-            if (ev) HIPCHK(hipEventRecord(ev[0], s));
-            int nb = 0;
-            for (int q = 0; q < Q && m0 < msorit; q++, L++, nb++) {
-                const int kr = std::min(K, msorit - m0);
-                const double *in = (L % 2 == 0) ? p : c->pscr;
-                double *out = (L % 2 == 0) ? c->pscr : p;
-                sweep(nx, ny, ld, c->M, b, sorrel, kr, in, out, part[L % 2], part[(L + 1) % 2], prev_k, prev_m0,
-                      L - 1, sortol, msorit, c->sc, 0, s);
-                prev_k = kr;
-                prev_m0 = m0;
-                m0 += kr;
+        // Batches of launches with a stopping check at the end of each, two
+        // batches in flight: the host reads batch n's flag (copied to pinned
+        // memory) while batch n+1 runs.  A batch issued after the solve
+        // stopped returns at entry in every launch.  Batches grow from 8 to 64
+        // launches, so a short solve is not overrun and a long one rarely waits.
+        int issued = 0, checked = 0, stop = 0;
+        while (!stop) {
+            if (m0 < msorit && issued - checked < 2) {
+                const int Q = std::min(64, 8 << std::min(issued, 3));
+                // timing: one event pair around the batch's launches (no marker
+                // between two launches, so their durations are not inflated)
+                hipEvent_t *ev = kt_events(c, 1);
+                if (ev) HIPCHK(hipEventRecord(ev[0], s));
+                int nb = 0;
+                for (int q = 0; q < Q && m0 < msorit; q++, L++, nb++) {
+                    const int kr = std::min(K, msorit - m0);
+                    const double *in = (L % 2 == 0) ? p : c->pscr;
+                    double *out = (L % 2 == 0) ? c->pscr : p;
+                    sweep(nx, ny, ld, c->M, b, sorrel, kr, in, out, part[L % 2], part[(L + 1) % 2], prev_k, prev_m0,
+                          L - 1, sortol, msorit, c->sc, 0, s);
+                    prev_k = kr;
+                    prev_m0 = m0;
+                    m0 += kr;
+                }
+                if (ev) {
+                    HIPCHK(hipEventRecord(ev[1], s));
+                    c->kt_cnt.back() = nb;
+                }
+                // the stopping test of the batch's last launch
+                launch_sor_tiled_check(nt, prev_k, prev_m0, L - 1, msorit, part[(L - 1) % 2], c->sc, s);
+                HIPCHK(hipMemcpyAsync(&c->h_flag[issued % 2], &c->sc->sor_done, sizeof(int), hipMemcpyDeviceToHost,
+                                      s));
+                HIPCHK(hipEventRecord(c->bev[issued % 2], s));
+                issued++;
+                continue;
             }
-            if (ev) {
-                HIPCHK(hipEventRecord(ev[1], s));
-                c->kt_cnt.back() = nb;
-            }
-            // the stopping test of the batch's last launch
-            launch_sor_tiled_check(nt, prev_k, prev_m0, L - 1, msorit, part[(L - 1) % 2], c->sc, s);
-            HIPCHK(hipMemcpyAsync(&done, &c->sc->sor_done, sizeof(int), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+            if (checked == issued) break;
+            HIPCHK(hipEventSynchronize(c->bev[checked % 2]));
+            stop = c->h_flag[checked % 2];
+            checked++;
         }
         Scalars h;
         HIPCHK(hipMemcpy(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
@@ -568,7 +623,25 @@ static int dropin_ctx(int nx, int ny, int ld) {
     return ctx_alloc(&g_ctx, nx, ny);
 }
 
-static int up_metric(int k, const double *h, int ld) { return up(g_ctx, h, ld, g_ctx->met[k]); }
+// Metric arrays of the drop-in calls: compared with the content last sent and
+// uploaded only when it differs (the driver's metrics never change, but the
+// library cannot assume it).
+static int up_metric(int k, const double *h, int ld) {
+    wd2_ctx *c = g_ctx;
+    if (!h) return WD2_OK;
+    const size_t w = (size_t)c->nx + 2, rows = (size_t)c->ny + 2;
+    if (c->met_shadow.size() < (size_t)M_COUNT) c->met_shadow.resize(M_COUNT);
+    std::vector<double> &sh = c->met_shadow[k];
+    if (sh.size() == w * rows) {
+        bool same = true;
+        for (size_t j = 0; j < rows && same; j++)
+            same = std::memcmp(h + j * (size_t)ld, sh.data() + j * w, w * sizeof(double)) == 0;
+        if (same) return WD2_OK;
+    }
+    sh.resize(w * rows);
+    for (size_t j = 0; j < rows; j++) std::memcpy(sh.data() + j * w, h + j * (size_t)ld, w * sizeof(double));
+    return up(c, h, ld, c->met[k]);
+}
 
 extern "C" {
 
