@@ -12,7 +12,6 @@ from __future__ import annotations
 
 import os
 import re
-import struct
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -121,46 +120,8 @@ class Deck:
             f.write(self.input_text())
 
 
-def _records(buf: bytes):
-    off = 0
-    while off < len(buf):
-        (n,) = struct.unpack_from("<i", buf, off)
-        rec = buf[off + 4: off + 4 + n]
-        (n2,) = struct.unpack_from("<i", buf, off + 4 + n)
-        assert n == n2, "bad record marker"
-        yield rec
-        off += 8 + n
-
-
-def read_restart(path: str) -> dict:
-    """Unformatted restart (file_manip.f:387-398): k,dtime / nx,ny / p,u,v,t,pss,uss,vss,tss.
-
-    Each field is returned with shape (ny+2, nx+2), element [j, i] = f(i, j).
-    """
-    with open(path, "rb") as f:
-        recs = list(_records(f.read()))
-    k, dtime = struct.unpack("<id", recs[0][:12])
-    nx, ny = struct.unpack("<ii", recs[1][:8])
-    data = np.frombuffer(recs[2], dtype="<f8")
-    n = (nx + 2) * (ny + 2)
-    names = ["p", "u", "v", "t", "pss", "uss", "vss", "tss"]
-    out = {"k": k, "dtime": dtime, "nx": nx, "ny": ny}
-    for m, name in enumerate(names):
-        out[name] = data[m * n:(m + 1) * n].reshape(ny + 2, nx + 2).copy()
-    return out
-
-
-def write_restart(path: str, k: int, dtime: float, nx: int, ny: int, fields: dict) -> None:
-    """Inverse of read_restart; missing fields are written as zeros."""
-    def rec(b: bytes) -> bytes:
-        return struct.pack("<i", len(b)) + b + struct.pack("<i", len(b))
-    zeros = np.zeros((ny + 2, nx + 2))
-    body = b"".join(np.ascontiguousarray(fields.get(nm, zeros), dtype="<f8").tobytes()
-                    for nm in ["p", "u", "v", "t", "pss", "uss", "vss", "tss"])
-    with open(path, "wb") as f:
-        f.write(rec(struct.pack("<id", k, dtime)))
-        f.write(rec(struct.pack("<ii", nx, ny)))
-        f.write(rec(body))
+# the restart format is the product's own reader/writer (wolfd2_amd.deck)
+from wolfd2_amd.deck import read_restart, write_restart  # noqa: E402,F401
 
 
 _DIFF_RE = re.compile(r"^\s*(\d+)\s*([-+0-9.Ee]+)\s*(-?\d+)([ *])\s*(\d+)\s+([-+0-9.Ee]+)\s+([-+0-9.Ee]+)\s+([-+0-9.Ee]+)")

@@ -1,0 +1,84 @@
+"""The deck runner (python -m wolfd2_amd.run): the reference's own case files
+in, the reference's PrintDiff table and restart file out, with the whole run on
+the GPU.  Each fixture deck must reproduce the reference program's table and
+final fields bit for bit, and a run continued from its restart file must equal
+the uninterrupted run."""
+import io
+import os
+
+import numpy as np
+import pytest
+
+import helpers
+import refdecks
+from wolfd2_amd import deck
+from wolfd2_amd.run import run_case
+
+pytestmark = pytest.mark.gpu
+
+
+def write_case(rd, fx, text=None):
+    refdecks.write_grid(os.path.join(rd, "grid.xyz"), fx["x"], fx["y"])
+    refdecks.write_fluid_table(os.path.join(rd, "fluid.dat"), "fl01", float(fx["rho"]), float(fx["mu"]))
+    with open(os.path.join(rd, "input.dat"), "w") as f:
+        f.write(str(fx["deck"]) if text is None else text)
+
+
+# every deck except the two 1000-step / line-SOR decks whose whole runs
+# test_gpu_parity.py already checks through the same context (they are slow)
+RUNNER_CASES = [n for n in helpers.run_fixture_names() if n not in ("run_c1_sor_64.npz", "run_c1_lsor_32.npz")]
+
+
+@pytest.mark.parametrize("name", RUNNER_CASES)
+def test_runner_reproduces_the_reference_program(name, tmp_path):
+    fx = helpers.load(name)
+    write_case(str(tmp_path), fx)
+    print(f"runner {name} steps={int(fx['nts'])}", flush=True)
+    buf = io.StringIO()
+    r = run_case(str(tmp_path), out=buf)
+    ref = fx["table"]
+    tab = refdecks.parse_print_diff(buf.getvalue())
+    fin = np.all(np.isfinite(ref), axis=1)
+    n = min(len(tab), len(ref))
+    assert n > 0
+    assert np.array_equal(tab[:n][fin[:n]], ref[:n][fin[:n]]), "PrintDiff table differs"
+    if np.all(np.isfinite(fx["u"])):
+        rst = deck.read_restart(str(tmp_path / "out.rst"))
+        assert rst["k"] == int(fx["k"]) and rst["dtime"] == float(fx["dtime"])
+        for f in ("u", "v", "p"):
+            assert np.array_equal(rst[f], fx[f]), f
+
+
+def test_runner_restart_continues_the_run(tmp_path):
+    fx = helpers.load("run_c5_re1000_128.npz")
+    text = str(fx["deck"])
+    full, first, second = (tmp_path / d for d in ("full", "first", "second"))
+    for d in (full, first, second):
+        d.mkdir()
+    write_case(str(full), fx, text.replace("n_time_steps 50", "n_time_steps 30"))
+    write_case(str(first), fx, text.replace("n_time_steps 50", "n_time_steps 12"))
+    second_text = text.replace("n_time_steps 50", "n_time_steps 18").replace(
+        "write_restart out.rst", "restart in.rst\nwrite_restart out.rst")
+    write_case(str(second), fx, second_text)
+    b_full, b1, b2 = io.StringIO(), io.StringIO(), io.StringIO()
+    r_full = run_case(str(full), out=b_full)
+    run_case(str(first), out=b1)
+    os.replace(first / "out.rst", second / "in.rst")
+    r2 = run_case(str(second), out=b2)
+    t_full = refdecks.parse_print_diff(b_full.getvalue())
+    t_split = np.vstack([refdecks.parse_print_diff(b1.getvalue()), refdecks.parse_print_diff(b2.getvalue())])
+    assert np.array_equal(t_full, t_split)
+    a, b = deck.read_restart(str(full / "out.rst")), deck.read_restart(str(second / "out.rst"))
+    assert (a["k"], a["dtime"]) == (b["k"], b["dtime"]) == (30, r_full["dtime"])
+    for f in ("u", "v", "p"):
+        assert np.array_equal(a[f], b[f])
+
+
+def test_runner_mixed_precision(tmp_path):
+    fx = helpers.load("run_c5_re1000_128.npz")
+    write_case(str(tmp_path), fx)
+    r64 = run_case(str(tmp_path), out=io.StringIO())
+    rmx = run_case(str(tmp_path), precision="mixed", out=io.StringIO())
+    for f in ("u", "v", "p"):
+        a, b = rmx[f], r64[f]
+        assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b), f

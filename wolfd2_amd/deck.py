@@ -1,0 +1,353 @@
+"""The reference solver's input and output files, for the hot-path configurations.
+
+Reads what a wolfd2 run reads:
+
+- the keyword deck (parse.f:lReadIPFile 32-878, lReadBCFile 1004-1951);
+- the fluid property table (parse.f:lReadFldProps 2027-2249);
+- the Plot3D grid (grid.f:ReadGrid 150-252);
+- the unformatted restart file (file_manip.f:ReadRestart).
+
+Writes what it writes: the restart file (file_manip.f:SaveRestart 364-401) and
+the per-step PrintDiff table (string.f:PrintDiff 467-570).
+
+Only the hot path's options are accepted. These features of the reference are
+outside that path, and a deck that turns one on is refused:
+
+- thermal energy, buoyancy;
+- small scales, trajectories;
+- porous regions, whose coefficients come from the reference's permeability
+  correlations;
+- thermal boundary regions.
+
+The reference program linked against this library through the Fortran binding
+serves those cases.
+"""
+from __future__ import annotations
+
+import math
+import os
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import case
+
+# parse.f:117-193 defaults of the keywords the hot path uses
+DEFAULTS = dict(nts=0, mqiter=20, nPpeSolver=1, msorit=2000, qtol=1.0e-4, sortol=1.0e-8, sorrel=1.0,
+                dlref=1.0, uref=1.0, tref=2.8357e2, tmax=1.0e3, dk=0.0,
+                flPropFn="fluidprop.dat", flName="air01", resReadFl="restart.rst", resWriteFl="restart.rst",
+                nrestart=0, nreswrite=0, nWrtRstFq=100, lPrntDiff=0, nPrDfFreq=1, lPrDfBann=0, nPrDfBnFq=20,
+                cartesian=False, grid_file=None)
+
+# keywords that switch on physics outside the hot path (parse.f:905-973)
+REFUSED = {"thermal_energy", "buoyancy", "small_scale", "trajectories"}
+# keywords of outputs this runner does not produce; accepted and ignored
+IGNORED = {"max_me_iter", "me_tolerance", "filter_u", "filter_v", "filter_t", "time_average_mode",
+           "output_format", "output_prefix", "snapshots_prefix", "snapshots_freq", "snapshots_first",
+           "time_series_prefix", "time_series_freq", "time_series_point", "atd_cu0", "atd_tscf", "atd_hscf",
+           "atd_bnumcrit", "atd_rmapmax", "atd_rmapexp", "atd_temcf", "atd_ppe_solver", "atd_max_sor_iter",
+           "atd_sor_tolerance", "atd_sor_relaxation", "atd_filter_u", "atd_filter_v", "atd_filter_t",
+           "max_ref_temperature"}
+
+
+class DeckError(ValueError):
+    pass
+
+
+@dataclass
+class Deck:
+    """Settings of one run (the keyword names of parse.f, values as parsed)."""
+    settings: dict = field(default_factory=lambda: dict(DEFAULTS))
+    nreg: tuple = (1, 1)
+    i_borders: tuple = ()
+    j_borders: tuple = ()
+    bc: list = field(default_factory=list)   # (keyword, ireg, jreg, [side], words)
+
+
+def _words(line: str) -> list:
+    return line.split()
+
+
+def read_input(path: str) -> Deck:
+    """The input_parameters and boundary_conditions sections of an input deck."""
+    d = Deck()
+    s = d.settings
+    section = None
+    with open(path) as f:
+        for ln, raw in enumerate(f, 1):
+            w = _words(raw.split("#")[0])
+            if not w:
+                continue
+            key = w[0].lower()
+            if key == "section":
+                section = w[1].lower() if len(w) > 1 else None
+                continue
+            if key == "end":
+                section = None
+                continue
+            try:
+                if section == "input_parameters":
+                    _input_keyword(s, key, w[1:])
+                elif section == "boundary_conditions":
+                    _bc_keyword(d, key, w[1:])
+                else:
+                    raise DeckError(f"keyword '{key}' outside a section")
+            except (IndexError, ValueError) as e:
+                raise DeckError(f"{path}:{ln}: cannot parse '{raw.strip()}' ({e})") from None
+    if not s["grid_file"]:
+        raise DeckError("no grid_file in the deck")
+    return d
+
+
+def _input_keyword(s: dict, key: str, a: list) -> None:
+    if key in REFUSED:
+        if a and a[0].lower() in ("off", "no", "0", "false"):
+            return
+        raise DeckError(f"'{key}' is outside the GPU hot path (use the Fortran drop-in build for it)")
+    if key in IGNORED:
+        return
+    if key == "grid_file":
+        s["grid_file"] = a[0]
+        s["cartesian"] = len(a) > 1 and a[1].lower() == "cartesian_grid"
+    elif key == "n_time_steps":
+        s["nts"] = int(a[0])
+    elif key == "time_step_size":
+        s["dk"] = float(a[0])
+    elif key == "max_ql_iter":
+        s["mqiter"] = int(a[0])
+    elif key == "ql_tolerance":
+        s["qtol"] = float(a[0])
+    elif key == "ppe_solver":
+        if a[0].lower() not in case.PPE_SOLVERS:
+            raise ValueError(f"unknown ppe_solver {a[0]}")
+        s["nPpeSolver"] = case.PPE_SOLVERS[a[0].lower()]
+    elif key == "max_sor_iter":
+        s["msorit"] = int(a[0])
+    elif key == "sor_tolerance":
+        s["sortol"] = float(a[0])
+    elif key == "sor_relaxation":
+        s["sorrel"] = float(a[0])
+    elif key == "ref_length":
+        s["dlref"] = float(a[0])
+    elif key == "ref_velocity":
+        s["uref"] = float(a[0])
+    elif key == "ref_temperature":
+        s["tref"] = float(a[0])
+    elif key == "fluid_prop_file":
+        s["flPropFn"] = a[0]
+    elif key == "fluid_prop_table":
+        s["flName"] = a[0]
+    elif key == "restart":
+        s["nrestart"] = 1
+        s["resReadFl"] = a[0]
+    elif key == "write_restart":           # parse.f:679-700
+        s["resWriteFl"] = a[0]
+        if len(a) > 1:
+            s["nreswrite"] = 2
+            s["nWrtRstFq"] = int(a[1])
+        else:
+            s["nreswrite"] = 1
+    elif key == "print_diff_freq":
+        s["lPrntDiff"] = 1
+        s["nPrDfFreq"] = int(a[0])
+    elif key == "print_bann_freq":
+        s["lPrDfBann"] = 1
+        s["nPrDfBnFq"] = int(a[0])
+    else:
+        raise ValueError(f"unknown keyword '{key}'")
+
+
+def _bc_keyword(d: Deck, key: str, a: list) -> None:
+    if key == "number_of_regions":
+        d.nreg = (int(a[0]), int(a[1]))
+    elif key == "i_borders":
+        d.i_borders = tuple(int(x) for x in a)
+    elif key == "j_borders":
+        d.j_borders = tuple(int(x) for x in a)
+    elif key in ("wall", "inlet", "outlet"):
+        d.bc.append((key, int(a[0]), int(a[1]), a[2].lower(), [x.lower() for x in a[3:]]))
+    elif key == "blockage":
+        d.bc.append((key, int(a[0]), int(a[1]), None, []))
+    elif key in ("porous_region", "heat_generation_region", "fixed_temperature_region", "fixed_temperature_wall",
+                 "heat_flux_wall"):
+        raise DeckError(f"'{key}' is outside the GPU hot path (use the Fortran drop-in build for it)")
+    else:
+        raise ValueError(f"unknown keyword '{key}'")
+
+
+def _opt(words: list, names: tuple):
+    """Value following the first of ``names`` in words, or None."""
+    for k, w in enumerate(words):
+        if w in names and k + 1 < len(words):
+            return float(words[k + 1])
+    return None
+
+
+def read_fluid(path: str, name: str, tref: float) -> np.ndarray:
+    """parse.f:lReadFldProps: the fluid's row (T, rho, cp, mu, k, Pr) at tref,
+    interpolated linearly (utility.f:dLinInterp) between the two rows whose
+    temperatures bracket it."""
+    rows, found = [], False
+    with open(path) as f:
+        for raw in f:
+            w = raw.split()
+            if not w:
+                continue
+            if w[0].lower() == "fluid":
+                found = w[1] == name
+                rows = []
+                continue
+            if w[0].lower() == "end":
+                if found:
+                    break
+                continue
+            if found:
+                rows.append([float(x) for x in w[:6]])
+    if not found or not rows:
+        raise DeckError(f"fluid '{name}' not found in {path}")
+    if rows[0][0] > tref:
+        raise DeckError(f"tref {tref} is below the first entry of fluid '{name}'")
+    prev = rows[0]
+    for cur in rows[1:]:
+        if prev[0] <= tref <= cur[0]:
+            out = [tref]
+            for p in range(1, 6):
+                y1, y2, x1, x2 = cur[p], prev[p], cur[0], prev[0]
+                out.append(y1 + (y2 - y1) * (tref - x1) / (x2 - x1))
+            return np.array(out)
+        prev = cur
+    raise DeckError(f"tref {tref} is outside the table of fluid '{name}'")
+
+
+def read_grid(path: str):
+    """grid.f:ReadGrid: 'nx ny' (or 'nx ny nz') then x, then y.  For a 3-D file
+    every k-plane is read into the same 2-D array, so the last plane is kept."""
+    with open(path) as f:
+        head = f.readline().split()
+        vals = np.array(f.read().split(), dtype=np.float64)
+    if len(head) not in (2, 3):
+        raise DeckError(f"syntax error in first line of grid file {path}")
+    nx, ny = int(head[0]), int(head[1])
+    nz = int(head[2]) if len(head) == 3 else 1
+    n = nx * ny
+    if vals.size < 2 * n * nz:
+        raise DeckError(f"grid file {path} holds {vals.size} values, needs {2 * n * nz}")
+    x = vals[(nz - 1) * n: nz * n].reshape(ny, nx).copy()
+    y = vals[nz * n + (nz - 1) * n: nz * n + nz * n].reshape(ny, nx).copy()
+    return x, y
+
+
+def build_case(d: Deck, rho: float, nx: int, ny: int) -> case.Regions:
+    """Region tables of the boundary_conditions section, with the reference's
+    scalings: velocities / uref (parse.f:1532, 1630), pressures / (rho uref^2)
+    (parse.f:1550, 1739)."""
+    uref = d.settings["uref"]
+    pscale = rho * uref * uref
+    walls, inlets, outlets, blocks = [], [], [], []
+    for key, ir, jr, side, w in d.bc:
+        if key == "blockage":
+            blocks.append((ir, jr))
+            continue
+        if side not in case.SIDES:
+            raise DeckError(f"bad side '{side}' for {key} {ir} {jr}")
+        press = _opt(w, ("press", "p"))
+        pr = 0.0 if press is None else press / pscale
+        if key == "wall":
+            kind = "no_stress" if "no_stress" in w else "no_slip"
+            tv = _opt(w, ("tangent_vel", "tv"))
+            walls.append((ir, jr, side, kind, 0.0 if tv is None else tv / uref, pr))
+        elif key == "inlet":
+            nv = _opt(w, ("normal_vel", "nv"))
+            tv = _opt(w, ("tangent_vel", "tv"))
+            nv = 0.0 if nv is None else nv / uref
+            tv = 0.0 if tv is None else tv / uref
+            u, v = (nv, tv) if side in ("west", "east") else (tv, nv)
+            inlets.append((ir, jr, side, u, v, 0.0))
+        else:
+            kind = "fully_dev" if ("fully_dev" in w or "fd" in w) else "mass_cons"
+            outlets.append((ir, jr, side, kind, 0.0, 0.0, pr))
+    # make_regions applies blockages, then walls, inlets and outlets; the
+    # reference applies the lines in deck order, which differs only for a deck
+    # that gives one region face two conditions
+    return case.make_regions(nx, ny, nreg=d.nreg, i_borders=d.i_borders, j_borders=d.j_borders, walls=walls,
+                             inlets=inlets, outlets=outlets, blockages=blocks)
+
+
+# ------------------------------------------------------------ restart files
+def _records(buf: bytes):
+    off = 0
+    while off < len(buf):
+        (n,) = struct.unpack_from("<i", buf, off)
+        rec = buf[off + 4: off + 4 + n]
+        (n2,) = struct.unpack_from("<i", buf, off + 4 + n)
+        if n != n2:
+            raise DeckError("bad record marker in restart file")
+        yield rec
+        off += 8 + n
+
+
+RESTART_FIELDS = ("p", "u", "v", "t", "pss", "uss", "vss", "tss")
+
+
+def read_restart(path: str) -> dict:
+    """Unformatted restart (file_manip.f:387-398): k,dtime / nx,ny / p,u,v,t,pss,uss,vss,tss.
+
+    Each field is returned with shape (ny+2, nx+2), element [j, i] = f(i, j).
+    """
+    with open(path, "rb") as f:
+        recs = list(_records(f.read()))
+    k, dtime = struct.unpack("<id", recs[0][:12])
+    nx, ny = struct.unpack("<ii", recs[1][:8])
+    data = np.frombuffer(recs[2], dtype="<f8")
+    n = (nx + 2) * (ny + 2)
+    out = {"k": k, "dtime": dtime, "nx": nx, "ny": ny}
+    for m, name in enumerate(RESTART_FIELDS):
+        out[name] = data[m * n:(m + 1) * n].reshape(ny + 2, nx + 2).copy()
+    return out
+
+
+def write_restart(path: str, k: int, dtime: float, nx: int, ny: int, fields: dict) -> None:
+    """Inverse of read_restart; missing fields are written as zeros."""
+    def rec(b: bytes) -> bytes:
+        return struct.pack("<i", len(b)) + b + struct.pack("<i", len(b))
+    zeros = np.zeros((ny + 2, nx + 2))
+    body = b"".join(np.ascontiguousarray(fields.get(nm, zeros), dtype="<f8").tobytes() for nm in RESTART_FIELDS)
+    with open(path, "wb") as f:
+        f.write(rec(struct.pack("<id", k, dtime)))
+        f.write(rec(struct.pack("<ii", nx, ny)))
+        f.write(rec(body))
+
+
+# ------------------------------------------------------------ PrintDiff table
+def fortran_e(x: float, w: int, d: int) -> str:
+    """Fortran Ew.d editing (scale factor 0): 0.ddd...E+ee, right-justified."""
+    if math.isnan(x):
+        s = "NaN"
+    elif math.isinf(x):
+        s = "Infinity" if x > 0 else "-Infinity"
+    elif x == 0.0:
+        s = ("-" if math.copysign(1.0, x) < 0 else "") + "0." + "0" * d + "E+00"
+    else:
+        m, e = f"{abs(x):.{d - 1}e}".split("e")
+        digits = m.replace(".", "")
+        ex = int(e) + 1
+        es = f"E{'+' if ex >= 0 else '-'}{abs(ex):02d}" if abs(ex) <= 99 else f"{'+' if ex >= 0 else '-'}{abs(ex):03d}"
+        s = ("-" if x < 0 else "") + "0." + digits + es
+    if len(s) > w and s.startswith("0."):
+        s = s[1:]
+    if len(s) > w and s.startswith("-0."):
+        s = "-" + s[2:]
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+PRINTDIFF_RULE = " " + "-" * 40 + " " * 30      # a character*70 variable holding 40 dashes
+PRINTDIFF_HEAD = " " + f"{'Tstp':>6}{'Time   ':>13}{'QlIt':>5}{'PpeIt':>7}{'|p1-pn|':>10}" + \
+    "".join(f"{h:>13}" for h in ("|u1-un|", "|v1-vn|"))
+
+
+def print_diff_row(k: int, time: float, nql: int, nsor: int, dif) -> str:
+    """string.f:PrintDiff row '(1x,i6,e13.6,i4,a1,i6,4(e13.6))' (no thermal)."""
+    mark = "*" if nql < 0 else " "
+    return (" " + f"{k:6d}" + fortran_e(time, 13, 6) + f"{nql:4d}" + mark + f"{nsor:6d}" +
+            "".join(fortran_e(float(v), 13, 6) for v in dif[:3]))

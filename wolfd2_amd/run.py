@@ -1,0 +1,121 @@
+"""Run a wolfd2 case deck on the GPU: ``python -m wolfd2_amd.run CASE_DIR``.
+
+Reads the case directory's input.dat, grid, fluid table and restart file in
+the reference's formats (wolfd2_amd.deck).  It then runs the reference's
+program flow on the device-resident context:
+
+1. the cold start of main.f:606-641, or the restart read (bound_cond.f:InitialCond);
+2. the time loop of main.f:690-972;
+3. the checkpoint and final restart writes (main.f:1040-1046, 1310-1313).
+
+It prints the PrintDiff table (string.f:PrintDiff) on standard output and
+stops on divergence (main.f:966-970), as the reference program does.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+from . import api, case, deck
+
+
+def _embed(a: np.ndarray, ld: int, ncol: int) -> np.ndarray:
+    f = np.zeros((ncol, ld))
+    f[: a.shape[0], : a.shape[1]] = a
+    return f
+
+
+def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) -> dict:
+    """Run the deck in case_dir; returns the PrintDiff rows and final state."""
+    out = out or sys.stdout
+    d = deck.read_input(os.path.join(case_dir, "input.dat"))
+    s = d.settings
+    x, y = deck.read_grid(os.path.join(case_dir, s["grid_file"]))
+    ny, nx = x.shape
+    fl = deck.read_fluid(os.path.join(case_dir, s["flPropFn"]), s["flName"], s["tref"])
+    rho, mu = float(fl[1]), float(fl[3])
+    g = case.compute_grid(x, y, dlref=s["dlref"])
+    reg = deck.build_case(d, rho, nx, ny)
+    prm = case.Params.from_physical(s["dk"], rho, mu, dlref=s["dlref"], uref=s["uref"], mqiter=s["mqiter"],
+                                    qtol=s["qtol"], nPpeSolver=s["nPpeSolver"], msorit=s["msorit"],
+                                    sortol=s["sortol"], sorrel=s["sorrel"], cartesian=s["cartesian"])
+    sol = api.Solver(g, reg, prm)
+    try:
+        sol.set_precision(precision)
+        if s["nrestart"]:
+            r = deck.read_restart(os.path.join(case_dir, s["resReadFl"]))
+            if (r["nx"], r["ny"]) != (nx, ny):
+                raise deck.DeckError(f"restart file is {r['nx']}x{r['ny']}, grid is {nx}x{ny}")
+            ks, dtime = r["k"], r["dtime"]
+            sol.set_fields(*(_embed(r[f], g.ld, g.ncol) for f in ("u", "v", "p")))
+        else:
+            ks, dtime = 0, 0.0
+            z = np.zeros((g.ncol, g.ld))
+            sol.set_fields(z, z, z)
+            sol.cold_start()
+        nts = s["nts"]
+        rows, ban = [], 0
+        k = ks
+        diverged = False
+
+        def save(kk, dt):
+            u, v, p = sol.get_fields()
+            S = np.s_[: ny + 2, : nx + 2]
+            deck.write_restart(os.path.join(case_dir, s["resWriteFl"]), kk, dt, nx, ny,
+                               {"u": u[S], "v": v[S], "p": p[S]})
+
+        while k < ks + nts and not diverged:
+            n = min(chunk, ks + nts - k)
+            if s["nreswrite"] == 2:   # stop at the next checkpoint
+                n = min(n, s["nWrtRstFq"] - (k - ks) % s["nWrtRstFq"])
+            tab = sol.step(n)
+            for m in range(n):
+                k += 1
+                dtime = dtime + prm.dk             # main.f:693
+                ql, nsor, dif = int(tab[m, 0]), int(tab[m, 1]), tab[m, 2:5]
+                rows.append([k, dtime, ql, nsor, *dif])
+                if max(dif) > 1.0e12:              # main.f:966-970
+                    print("* Solution diverged. Please reduce CFL number.", file=out)
+                    diverged = True
+                    break
+                if s["lPrntDiff"] == 1 and (k - ks) % s["nPrDfFreq"] == 0:
+                    if s["lPrDfBann"] == 1:
+                        if ban == 0 or ban >= s["nPrDfBnFq"]:
+                            print(deck.PRINTDIFF_RULE, file=out)
+                            print(deck.PRINTDIFF_HEAD, file=out)
+                            print(deck.PRINTDIFF_RULE, file=out)
+                            ban = 1
+                        else:
+                            ban += 1
+                    rdim = dtime * s["dlref"] / s["uref"]
+                    print(deck.print_diff_row(k, rdim, ql, nsor, dif), file=out)
+            if diverged:
+                break
+            if s["nreswrite"] == 2 and (k - ks) % s["nWrtRstFq"] == 0:
+                save(k - 1, dtime)                 # main.f:1044 writes k-1
+        if not diverged and s["nreswrite"] == 1:
+            save(k, dtime)                         # main.f:1312, k-1 after the loop
+        u, v, p = sol.get_fields()
+    finally:
+        sol.close()
+    return {"rows": np.array(rows), "u": u, "v": v, "p": p, "k": k, "dtime": dtime, "diverged": diverged}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("case_dir")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    args = ap.parse_args(argv)
+    try:
+        r = run_case(args.case_dir, args.precision)
+    except deck.DeckError as e:
+        print(f"* {e}", file=sys.stderr)
+        return 2
+    return 1 if r["diverged"] else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
