@@ -45,6 +45,11 @@ int wd2_trid_fallbacks(void);
  * sweep.  Both give the reference's iterates bit for bit.  Returns the
  * previous setting. */
 int wd2_set_tiled_sor(int enable);
+/* Lexicographic Sor (nPpeSolver 1) on grids of up to 1024 pressure rows runs
+ * as one single-workgroup launch per solve, a grid row per lane (on by
+ * default); 0 selects the tiled wavefront, one launch per sweep.  Both give
+ * the reference's iterates bit for bit.  Returns the previous setting. */
+int wd2_set_lex_rows(int enable);
 /* Non-zero (default): print the reference's warnings on standard output
  * (Ppe non-convergence, bad solver flag).  Returns the previous setting. */
 int wd2_set_verbose(int enable);

@@ -78,6 +78,29 @@ def test_ppe_solvers_bitwise_vs_reference_goldens(name, solver):
         assert np.array_equal(p[S], fx[f"{key}_p_out"]), key
 
 
+@pytest.mark.parametrize("name", helpers.kernel_fixture_names())
+def test_lex_sor_tiled_wavefront_path_bitwise(name):
+    """Sor on the tiled wavefront (one launch per sweep) instead of the
+    single-workgroup row-per-lane kernel: the same reference goldens."""
+    from wolfd2_amd._lib import lib
+    fx = helpers.load(name)
+    nx, ny, g, reg = _kernel_case(fx)
+    E = lambda a: helpers.embed(a, g.ld, g.ncol)  # noqa: E731
+    S = np.s_[: ny + 2, : nx + 2]
+    old = lib().wd2_set_lex_rows(0)
+    try:
+        for cart in (1, 0):
+            key = f"ppe1c{cart}"
+            p = E(fx[f"{key}_p_in"])
+            prm = case.Params(dk=0.01, re=100.0, nPpeSolver=1, msorit=25, sortol=1e-8, sorrel=1.3,
+                              cartesian=bool(cart))
+            n = api.ppe(reg, g, prm, E(fx[f"{key}_u"]), E(fx[f"{key}_v"]), p)
+            assert n == int(fx[f"{key}_n"])
+            assert np.array_equal(p[S], fx[f"{key}_p_out"]), key
+    finally:
+        lib().wd2_set_lex_rows(old)
+
+
 def _chain(n, rng, weak=False):
     a = np.zeros((n, 3))
     a[:, 0] = rng.uniform(-1, 0, n)

@@ -483,6 +483,152 @@ __global__ __launch_bounds__(LX_T) void k_sor_lex(int nx, int ny, int ld, const 
     (void)sh;
 }
 
+// Lexicographic Sor, whole solve in one workgroup (grids of up to LR_MAXROWS
+// rows).  Each lane owns one grid row j = 2 + lane and walks it from i = 2 to
+// nx; the lanes of a wavefront run skewed by one column per lane, so at every
+// step lane L updates the point whose south neighbour lane L-1 updated in the
+// previous step (passed by a lane shuffle) and whose west neighbour is the
+// lane's own previous point: exactly the reference's i-then-j order.  East,
+// north and the point's old value are read from the previous sweep's field.
+// Wavefront w starts LR_LAG steps after wavefront w-1 and takes the south
+// values of its first row from a ring that w-1's last lane fills; with a
+// workgroup barrier every LR_K steps a ring value is always at least one
+// barrier old when it is read.  The stopping test (pressure.f:443) is applied
+// in the kernel after every sweep, so the whole solve is one launch.
+// Non-Cartesian grids re-form the right-hand side before every sweep
+// (pressure.f:RhsPpe, as in the reference).
+constexpr int LR_K = 16;                // steps between two barriers
+constexpr int LR_LAG = 64 + LR_K;       // step lag between consecutive wavefronts
+constexpr int LR_RING = 64;             // ring slots (> LR_K + 1)
+constexpr int LR_D = 4;                 // steps of operand prefetch (divides LR_K)
+constexpr int LR_MAXW = 8;
+constexpr int LR_MAXROWS = 64 * LR_MAXW;
+
+template <bool COEF>
+__global__ __launch_bounds__(64 * LR_MAXW) void k_sor_lex_rows(int nx, int ny, int ld, const double *__restrict__ rau,
+                                                              const double *__restrict__ rgv, PpeCoef cf,
+                                                              const double *__restrict__ rhs,
+                                                              const double *__restrict__ rbu,
+                                                              const double *__restrict__ rbv, double *beff, int cart,
+                                                              double sorrel, double sortol, int msorit, double *p,
+                                                              Scalars *sc) {
+    __shared__ double ring[LR_MAXW][LR_RING];
+    __shared__ double red[LR_MAXW];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int nw = blockDim.x >> 6;
+    const int j = 2 + t;
+    const bool rowok = j <= ny;
+    const int nc = nx - 1;                       // columns i = 2 .. nx
+    // steps of a sweep, rounded up to whole blocks (the extra steps find no point)
+    const int nsteps = (nc + LR_LAG * (nw - 1) + 63 + LR_K - 1) / LR_K * LR_K;
+    const double *bb = cart ? rhs : beff;
+    for (int m = 1;; m++) {
+        if (!cart) {   // pressure.f:RhsPpe from the field at the start of the sweep
+#define P(a, b) p[WD2_IX(a, b)]
+            for (int q = t; q < (nx - 1) * (ny - 1); q += blockDim.x) {
+                const int i = 2 + q % (nx - 1), jj = 2 + q / (nx - 1);
+                beff[WD2_IX(i, jj)] =
+                    rhs[WD2_IX(i, jj)] -
+                    (rbu[WD2_IX(i, jj)] * (P(i + 1, jj + 1) + P(i, jj + 1) - P(i + 1, jj - 1) - P(i, jj - 1)) -
+                     rbu[WD2_IX(i - 1, jj)] * (P(i, jj + 1) + P(i - 1, jj + 1) - P(i, jj - 1) - P(i - 1, jj - 1)) +
+                     rbv[WD2_IX(i, jj)] * (P(i + 1, jj + 1) + P(i + 1, jj) - P(i - 1, jj + 1) - P(i - 1, jj)) -
+                     rbv[WD2_IX(i, jj - 1)] * (P(i + 1, jj) + P(i + 1, jj - 1) - P(i - 1, jj) - P(i - 1, jj - 1)));
+            }
+#undef P
+            __syncthreads();
+        }
+        // operands of the point LR_D steps ahead are loaded while the current
+        // step runs (slot d holds column c + d of this lane's row)
+        // (the point's old value is the previous column's east value, and on
+        // the unit-coefficient path its a2 the previous column's a4)
+        double oE[LR_D], oN[LR_D], oB[LR_D], o1[LR_D], o4[LR_D], o5[LR_D];
+        double o2[COEF ? LR_D : 1], o3[COEF ? LR_D : 1];
+        auto fetch = [&](int cc, int d) {
+            if (rowok && cc >= 0 && cc < nc) {
+                const int i = 2 + cc;
+                const size_t k = WD2_IX(i, j);
+                oE[d] = p[WD2_IX(i + 1, j)];
+                oN[d] = p[WD2_IX(i, j + 1)];
+                oB[d] = bb[k];
+                if (COEF) {
+                    o1[d] = cf.a1[k]; o2[d] = cf.a2[k]; o3[d] = cf.a3[k]; o4[d] = cf.a4[k]; o5[d] = cf.a5[k];
+                } else {
+                    o1[d] = rgv[WD2_IX(i, j - 1)];
+                    o4[d] = rau[k];
+                    o5[d] = rgv[k];
+                }
+            }
+        };
+        const int off = w * LR_LAG + lane;   // the lane's point at step st is column st - off
+#pragma unroll
+        for (int d = 0; d < LR_D; d++) fetch(d - off, d);
+        double mx = 0.0, res = 0.0, prevE = 0.0, prevA4 = 0.0;
+        for (int s0 = 0; s0 < nsteps; s0 += LR_K) {
+#pragma unroll
+            for (int q = 0; q < LR_K; q++) {
+                const int st = s0 + q;
+                const int d = q % LR_D;                   // slot of this step (LR_K is a multiple of LR_D)
+                const int c = st - off;
+                const double up = __shfl_up(res, 1);      // lane-1's point of the previous step
+                if (rowok && c >= 0 && c < nc) {
+                    const int i = 2 + c;
+                    double pS;
+                    if (lane > 0) pS = up;
+                    else if (w > 0) pS = ring[w - 1][c % LR_RING];
+                    else pS = p[WD2_IX(i, j - 1)];            // ghost row j = 1
+                    const double pW = c > 0 ? res : p[WD2_IX(i - 1, j)];   // ghost column i = 1
+                    const double pij = c > 0 ? prevE : p[WD2_IX(i, j)];
+                    const double a1 = o1[d], a4 = o4[d], a5 = o5[d];
+                    const double a2 = COEF ? o2[COEF ? d : 0] : (c > 0 ? prevA4 : rau[WD2_IX(i - 1, j)]);
+                    const double a3 = COEF ? o3[COEF ? d : 0] : -a4 - a2 - a5 - a1;
+                    double sum = oB[d] - a1 * pS - a2 * pW - a4 * oE[d] - a5 * oN[d];
+                    sum = sum / a3 - pij;
+                    res = pij + sorrel * sum;
+                    p[WD2_IX(i, j)] = res;
+                    mx = dmaxf(mx, fabs(sum));
+                    if (lane == 63) ring[w][c % LR_RING] = res;
+                }
+                prevE = oE[d];        // the next column's old value and a2
+                prevA4 = o4[d];
+                fetch(c + LR_D, d);   // the slot is free again: column c + LR_D
+            }
+            __syncthreads();
+        }
+        // dif of the sweep and the stopping test (pressure.f:434-449)
+        for (int off = 32; off > 0; off >>= 1) mx = dmaxf(mx, __shfl_xor(mx, off));
+        if (lane == 0) red[w] = mx;
+        __syncthreads();
+        double dif = 0.0;
+        for (int q = 0; q < nw; q++) dif = dmaxf(dif, red[q]);
+        __syncthreads();
+        const bool conv = m > 1 && dif < sortol;
+        if (conv || m >= msorit) {
+            if (t == 0) {
+                sc->sor_m = m;
+                sc->sor_dif = dif;
+                sc->sor_nconv = conv ? m : 0;
+                sc->sor_done = 1;
+            }
+            return;
+        }
+    }
+}
+
+int sor_lex_rows_max() { return LR_MAXROWS; }
+
+void launch_sor_lex_rows(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *rhs,
+                         double *beff, int cart, double sorrel, double sortol, int msorit, double *p, Scalars *sc,
+                         hipStream_t s) {
+    const int nw = (ny - 1 + 63) / 64;
+    const PpeCoef cf = coef ? *coef : PpeCoef{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (coef)
+        hipLaunchKernelGGL(k_sor_lex_rows<true>, dim3(1), dim3(64 * nw), 0, s, nx, ny, ld, m.rau, m.rgv, cf, rhs,
+                           m.rbu, m.rbv, beff, cart, sorrel, sortol, msorit, p, sc);
+    else
+        hipLaunchKernelGGL(k_sor_lex_rows<false>, dim3(1), dim3(64 * nw), 0, s, nx, ny, ld, m.rau, m.rgv, cf, rhs,
+                           m.rbu, m.rbv, beff, cart, sorrel, sortol, msorit, p, sc);
+}
+
 void launch_sor_lex_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
                           double sorrel, double sortol, int msorit, double *p, int *tile_flags, int sweep,
                           double *partials, Scalars *sc, hipStream_t s) {

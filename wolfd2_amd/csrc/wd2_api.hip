@@ -20,7 +20,8 @@ using namespace wd2;
 static thread_local std::string g_err;
 static int g_fallbacks = 0;
 static int g_tiled_sor = 1;   // temporally blocked SorRB when the grid allows it
-static int g_verbose = 1;     // print the reference's warnings (pressure.f:242-246, 234)
+static int g_verbose = 1;
+static int g_lex_rows = 1;    // lexicographic Sor as one single-workgroup launch when the rows fit     // print the reference's warnings (pressure.f:242-246, 234)
 
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -507,6 +508,11 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
     }
     const int batch = (solver == 2) ? 1 : 64;
     int done = 0;
+    if (solver == 1 && g_lex_rows && ny - 1 <= sor_lex_rows_max()) {
+        launch_sor_lex_rows(nx, ny, ld, c->M, coef, c->rhs, c->beff, cart ? 1 : 0, sorrel, sortol, msorit, p, c->sc,
+                            s);
+        done = 1;
+    }
     for (int it = 0; it < msorit && !done;) {
         for (int q = 0; q < batch && it < msorit; q++, it++) {
             if (solver == 5 || solver == 6) {
@@ -653,6 +659,12 @@ int wd2_sor_mixed_sweeps_per_launch(void) { return sor_tiled_mx_sweeps(); }
 int wd2_set_verbose(int enable) {
     const int old = g_verbose;
     g_verbose = enable ? 1 : 0;
+    return old;
+}
+
+int wd2_set_lex_rows(int enable) {
+    const int old = g_lex_rows;
+    g_lex_rows = enable ? 1 : 0;
     return old;
 }
 

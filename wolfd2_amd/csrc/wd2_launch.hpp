@@ -102,6 +102,11 @@ void launch_sor_tiled_check(int ntiles, int nsweep, int m0, int launch, int msor
 void launch_sor_lex_sweep(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
                           double sorrel, double sortol, int msorit, double *p, int *tile_flags, int sweep,
                           double *partials, Scalars *sc, hipStream_t s);
+// Lexicographic Sor, whole solve in one launch (one workgroup; ny - 1 <= sor_lex_rows_max()).
+int sor_lex_rows_max();
+void launch_sor_lex_rows(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *rhs,
+                         double *beff, int cart, double sorrel, double sortol, int msorit, double *p, Scalars *sc,
+                         hipStream_t s);
 // Line SOR: assemble the chains of one colour (first_line, step 2 lines, or one line), then relax.
 void launch_slor_assemble(int nx, int ny, int ld, const Metrics &m, const PpeCoef *coef, const double *b,
                           const double *p, int first_line, int line_step, int nlines, double *ca, double *cd,
