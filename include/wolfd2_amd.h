@@ -189,6 +189,18 @@ void *wd2_ctx_stream(wd2_ctx *ctx);
 #define WD2_PREC_FP64 0
 #define WD2_PREC_MIXED 1
 int wd2_ctx_set_precision(wd2_ctx *ctx, int precision);
+
+/* Boundary profile of one region side (no counterpart in the reference, whose
+ * dBCVal(mgri,mgrj,4,4) holds one constant per region side and variable, set
+ * from the deck at parse.f:1524-1540 for walls and bound_cond.f:585-593 for
+ * inlets).  var is 1 (u), 2 (v) or 3 (p); values[k] replaces
+ * dBCVal(ireg,jreg,side,var) on grid line k: k = j = 0..ny+1 on the west and
+ * east sides (n = ny+2), k = i = 0..nx+1 on the south and north sides
+ * (n = nx+2).  Every boundary formula of VelBoundCond, PresBoundCond and
+ * VelOutflowBCs is kept, with the constant replaced by the line's value, so
+ * a profile of constant values gives the constant's results bit for bit.
+ * values = NULL removes the profile.  Applies to this context's later steps. */
+int wd2_ctx_set_bc_profile(wd2_ctx *ctx, int ireg, int jreg, int side, int var, const double *values, int n);
 /* Timing of the red-black SOR kernels with HIP events on the context's
  * stream.  enable=1 starts recording: one event pair around each batch of
  * back-to-back launches of the tiled multi-sweep kernel, or around each

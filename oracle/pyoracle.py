@@ -53,7 +53,8 @@ def _i(a: np.ndarray):
 class _Regions(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("ld", C.c_int), ("mgri", C.c_int), ("mgrj", C.c_int),
                 ("nReg", _ip), ("nRegBrd", _ip), ("nRegType", _ip), ("nMomBdTp", _ip),
-                ("dBCVal", _dp), ("dPRporos", _dp), ("dPRporc1", _dp), ("dPRporc2", _dp)]
+                ("dBCVal", _dp), ("dPRporos", _dp), ("dPRporc1", _dp), ("dPRporc2", _dp),
+                ("poff", _ip), ("pval", _dp)]
 
 
 class _Metrics(C.Structure):
@@ -85,6 +86,10 @@ class Oracle:
         s = _Regions(reg.nx, reg.ny, ld, reg.mgri, reg.mgrj, _i(reg.nreg), _i(reg.nRegBrd),
                      _i(reg.nRegType), _i(reg.nMomBdTp), _d(reg.dBCVal), _d(reg.dPRporos),
                      _d(reg.dPRporc1), _d(reg.dPRporc2))
+        poff, pval = reg.profile_tables()
+        if poff is not None:
+            s.keep = (poff, pval)   # alive as long as the structure
+            s.poff, s.pval = _i(poff), _d(pval)
         return s
 
     def metrics(self, grid):

@@ -42,6 +42,16 @@ static inline int rtype(const wd2o_regions *r, int ir, int jr) {
 static inline double bcv(const wd2o_regions *r, int ir, int jr, int side, int var) {
     return r->dBCVal[(ir - 1) + r->mgri * ((jr - 1) + r->mgrj * ((side - 1) + 4 * (var - 1)))];
 }
+/* Boundary value on grid line k of a side (k = j on the west and east sides,
+ * i on the south and north ones): the side's profile when it has one (an
+ * extension, not in the reference), else the reference's constant. */
+static inline double bcvk(const wd2o_regions *r, int ir, int jr, int side, int var, int k) {
+    if (r->poff) {
+        const int o = r->poff[(ir - 1) + r->mgri * ((jr - 1) + r->mgrj * ((side - 1) + 4 * (var - 1)))];
+        if (o >= 0) return r->pval[o + k];
+    }
+    return bcv(r, ir, jr, side, var);
+}
 static inline double tab2(const wd2o_regions *r, const double *t, int ir, int jr) {
     return t[(ir - 1) + r->mgri * (jr - 1)];
 }
@@ -727,18 +737,18 @@ int wd2o_vel_bound_cond(const wd2o_regions *r, double *u, double *v)
             case BM_INTERN: break;
             case BM_WALL1:
                 for (j = jS; j <= jN; j++) F(u, iW, j) = 0.0;
-                for (j = jS + 1; j <= jN; j++) F(v, iW, j) = dTwo * bcv(r, ir, jr, WEST, VV) - F(v, iW + 1, j);
+                for (j = jS + 1; j <= jN; j++) F(v, iW, j) = dTwo * bcvk(r, ir, jr, WEST, VV, j) - F(v, iW + 1, j);
                 break;
             case BM_WALL2:
                 for (j = jS; j <= jN; j++) F(u, iW, j) = 0.0;
                 for (j = jS + 1; j <= jN; j++) F(v, iW, j) = F(v, iW + 1, j);
                 break;
             case BM_INLET:
-                for (j = jS; j <= jN; j++) F(u, iW, j) = bcv(r, ir, jr, WEST, VU);
-                for (j = jS + 1; j <= jN; j++) F(v, iW, j) = dTwo * bcv(r, ir, jr, WEST, VV) - F(v, iW + 1, j);
+                for (j = jS; j <= jN; j++) F(u, iW, j) = bcvk(r, ir, jr, WEST, VU, j);
+                for (j = jS + 1; j <= jN; j++) F(v, iW, j) = dTwo * bcvk(r, ir, jr, WEST, VV, j) - F(v, iW + 1, j);
                 break;
             case BM_OUTLT1:
-                for (j = jS; j <= jN; j++) F(u, iW - 1, j) = bcv(r, ir, jr, WEST, VU) + F(u, iW, j);
+                for (j = jS; j <= jN; j++) F(u, iW - 1, j) = bcvk(r, ir, jr, WEST, VU, j) + F(u, iW, j);
                 for (j = jS + 1; j <= jN; j++) F(v, iW, j) = -F(v, iW + 1, j);
                 break;
             case BM_OUTLT2:
@@ -755,18 +765,18 @@ int wd2o_vel_bound_cond(const wd2o_regions *r, double *u, double *v)
             case BM_INTERN: break;
             case BM_WALL1:
                 for (j = jS; j <= jN; j++) F(u, iE, j) = 0.0;
-                for (j = jS + 1; j <= jN; j++) F(v, iE + 1, j) = dTwo * bcv(r, ir, jr, EAST, VV) - F(v, iE, j);
+                for (j = jS + 1; j <= jN; j++) F(v, iE + 1, j) = dTwo * bcvk(r, ir, jr, EAST, VV, j) - F(v, iE, j);
                 break;
             case BM_WALL2:
                 for (j = jS; j <= jN; j++) F(u, iE, j) = 0.0;
                 for (j = jS + 1; j <= jN; j++) F(v, iE + 1, j) = F(v, iE, j);
                 break;
             case BM_INLET:
-                for (j = jS; j <= jN; j++) F(u, iE, j) = bcv(r, ir, jr, EAST, VU);
-                for (j = jS + 1; j <= jN; j++) F(v, iE + 1, j) = dTwo * bcv(r, ir, jr, EAST, VV) - F(v, iE, j);
+                for (j = jS; j <= jN; j++) F(u, iE, j) = bcvk(r, ir, jr, EAST, VU, j);
+                for (j = jS + 1; j <= jN; j++) F(v, iE + 1, j) = dTwo * bcvk(r, ir, jr, EAST, VV, j) - F(v, iE, j);
                 break;
             case BM_OUTLT1:
-                for (j = jS; j <= jN; j++) F(u, iE + 1, j) = bcv(r, ir, jr, EAST, VU) + F(u, iE, j);
+                for (j = jS; j <= jN; j++) F(u, iE + 1, j) = bcvk(r, ir, jr, EAST, VU, j) + F(u, iE, j);
                 for (j = jS + 1; j <= jN; j++) F(v, iE + 1, j) = -F(v, iE, j);
                 break;
             case BM_OUTLT2:
@@ -782,7 +792,7 @@ int wd2o_vel_bound_cond(const wd2o_regions *r, double *u, double *v)
             switch (btp(r, ir, jr, SOUTH)) {
             case BM_INTERN: break;
             case BM_WALL1:
-                for (i = iW + 1; i <= iE; i++) F(u, i, jS) = dTwo * bcv(r, ir, jr, SOUTH, VU) - F(u, i, jS + 1);
+                for (i = iW + 1; i <= iE; i++) F(u, i, jS) = dTwo * bcvk(r, ir, jr, SOUTH, VU, i) - F(u, i, jS + 1);
                 for (i = iW; i <= iE; i++) F(v, i, jS) = 0.0;
                 break;
             case BM_WALL2:
@@ -790,12 +800,12 @@ int wd2o_vel_bound_cond(const wd2o_regions *r, double *u, double *v)
                 for (i = iW; i <= iE; i++) F(v, i, jS) = 0.0;
                 break;
             case BM_INLET:
-                for (i = iW + 1; i <= iE; i++) F(u, i, jS) = dTwo * bcv(r, ir, jr, SOUTH, VU) - F(u, i, jS + 1);
-                for (i = iW; i <= iE; i++) F(v, i, jS) = bcv(r, ir, jr, SOUTH, VV);
+                for (i = iW + 1; i <= iE; i++) F(u, i, jS) = dTwo * bcvk(r, ir, jr, SOUTH, VU, i) - F(u, i, jS + 1);
+                for (i = iW; i <= iE; i++) F(v, i, jS) = bcvk(r, ir, jr, SOUTH, VV, i);
                 break;
             case BM_OUTLT1:
                 for (i = iW + 1; i <= iE; i++) F(u, i, jS) = -F(u, i, jS + 1);
-                for (i = iW; i <= iE; i++) F(v, i, jS) = bcv(r, ir, jr, SOUTH, VV) + F(v, i, jS);
+                for (i = iW; i <= iE; i++) F(v, i, jS) = bcvk(r, ir, jr, SOUTH, VV, i) + F(v, i, jS);
                 break;
             case BM_OUTLT2:
                 for (i = iW + 1; i <= iE; i++) F(v, i, jS) = F(v, i, jS + 1) + (F(u, i, jS + 1) - F(u, i - 1, jS + 1));
@@ -810,7 +820,7 @@ int wd2o_vel_bound_cond(const wd2o_regions *r, double *u, double *v)
             switch (btp(r, ir, jr, NORTH)) {
             case BM_INTERN: break;
             case BM_WALL1:
-                for (i = iW + 1; i <= iE; i++) F(u, i, jN + 1) = dTwo * bcv(r, ir, jr, NORTH, VU) - F(u, i, jN);
+                for (i = iW + 1; i <= iE; i++) F(u, i, jN + 1) = dTwo * bcvk(r, ir, jr, NORTH, VU, i) - F(u, i, jN);
                 for (i = iW; i <= iE; i++) F(v, i, jN) = 0.0;
                 break;
             case BM_WALL2:
@@ -818,12 +828,12 @@ int wd2o_vel_bound_cond(const wd2o_regions *r, double *u, double *v)
                 for (i = iW; i <= iE; i++) F(v, i, jN) = 0.0;
                 break;
             case BM_INLET:
-                for (i = iW + 1; i <= iE; i++) F(u, i, jN + 1) = dTwo * bcv(r, ir, jr, NORTH, VU) - F(u, i, jN);
-                for (i = iW; i <= iE; i++) F(v, i, jN) = bcv(r, ir, jr, NORTH, VV);
+                for (i = iW + 1; i <= iE; i++) F(u, i, jN + 1) = dTwo * bcvk(r, ir, jr, NORTH, VU, i) - F(u, i, jN);
+                for (i = iW; i <= iE; i++) F(v, i, jN) = bcvk(r, ir, jr, NORTH, VV, i);
                 break;
             case BM_OUTLT1:
                 for (i = iW + 1; i <= iE; i++) F(u, i, jN + 1) = -F(u, i, jN);
-                for (i = iW; i <= iE; i++) F(v, i, jN + 1) = bcv(r, ir, jr, NORTH, VV) + F(v, i, jN);
+                for (i = iW; i <= iE; i++) F(v, i, jN + 1) = bcvk(r, ir, jr, NORTH, VV, i) + F(v, i, jN);
                 break;
             case BM_OUTLT2:
                 for (i = iW; i <= iE; i++) F(v, i, jN) = F(v, i, jN - 1) - (F(u, i, jN) - F(u, i - 1, jN));
@@ -850,19 +860,19 @@ int wd2o_pres_bound_cond(const wd2o_regions *r, double *p)
             if (rtype(r, ir, jr) == RM_BLOCKG) {
                 for (j = jS + 1; j <= jN; j++)
                     for (i = iW + 1; i <= iE; i++) F(p, i, j) = 0.0;
-                for (j = jS + 1; j <= jN; j++) F(p, iW + 1, j) = bcv(r, ir, jr, WEST, VP) + F(p, iW, j);
-                for (j = jS + 1; j <= jN; j++) F(p, iE, j) = bcv(r, ir, jr, EAST, VP) + F(p, iE + 1, j);
-                for (i = iW + 1; i <= iE; i++) F(p, i, jS + 1) = bcv(r, ir, jr, SOUTH, VP) + F(p, i, jS);
-                for (i = iW + 1; i <= iE; i++) F(p, i, jN) = bcv(r, ir, jr, NORTH, VP) + F(p, i, jN + 1);
+                for (j = jS + 1; j <= jN; j++) F(p, iW + 1, j) = bcvk(r, ir, jr, WEST, VP, j) + F(p, iW, j);
+                for (j = jS + 1; j <= jN; j++) F(p, iE, j) = bcvk(r, ir, jr, EAST, VP, j) + F(p, iE + 1, j);
+                for (i = iW + 1; i <= iE; i++) F(p, i, jS + 1) = bcvk(r, ir, jr, SOUTH, VP, i) + F(p, i, jS);
+                for (i = iW + 1; i <= iE; i++) F(p, i, jN) = bcvk(r, ir, jr, NORTH, VP, i) + F(p, i, jN + 1);
                 continue;
             }
             switch (btp(r, ir, jr, WEST)) {
             case BM_INTERN: break;
             case BM_WALL1: case BM_WALL2: case BM_INLET:
-                for (j = jS + 1; j <= jN; j++) F(p, iW, j) = bcv(r, ir, jr, WEST, VP) + F(p, iW + 1, j);
+                for (j = jS + 1; j <= jN; j++) F(p, iW, j) = bcvk(r, ir, jr, WEST, VP, j) + F(p, iW + 1, j);
                 break;
             case BM_OUTLT1: case BM_OUTLT2:
-                for (j = jS + 1; j <= jN; j++) F(p, iW, j) = dTwo * bcv(r, ir, jr, WEST, VP) - F(p, iW + 1, j);
+                for (j = jS + 1; j <= jN; j++) F(p, iW, j) = dTwo * bcvk(r, ir, jr, WEST, VP, j) - F(p, iW + 1, j);
                 break;
             default:
                 fprintf(stderr, "Wrong nBdTypeW flag in region %d,%d\n", ir, jr);
@@ -871,10 +881,10 @@ int wd2o_pres_bound_cond(const wd2o_regions *r, double *p)
             switch (btp(r, ir, jr, EAST)) {
             case BM_INTERN: break;
             case BM_WALL1: case BM_WALL2: case BM_INLET:
-                for (j = jS + 1; j <= jN; j++) F(p, iE + 1, j) = bcv(r, ir, jr, EAST, VP) + F(p, iE, j);
+                for (j = jS + 1; j <= jN; j++) F(p, iE + 1, j) = bcvk(r, ir, jr, EAST, VP, j) + F(p, iE, j);
                 break;
             case BM_OUTLT1: case BM_OUTLT2:
-                for (j = jS + 1; j <= jN; j++) F(p, iE + 1, j) = dTwo * bcv(r, ir, jr, EAST, VP) - F(p, iE, j);
+                for (j = jS + 1; j <= jN; j++) F(p, iE + 1, j) = dTwo * bcvk(r, ir, jr, EAST, VP, j) - F(p, iE, j);
                 break;
             default:
                 fprintf(stderr, "Wrong nBdTypeE flag in region %d,%d\n", ir, jr);
@@ -883,10 +893,10 @@ int wd2o_pres_bound_cond(const wd2o_regions *r, double *p)
             switch (btp(r, ir, jr, SOUTH)) {
             case BM_INTERN: break;
             case BM_WALL1: case BM_WALL2: case BM_INLET:
-                for (i = iW + 1; i <= iE; i++) F(p, i, jS) = bcv(r, ir, jr, SOUTH, VP) + F(p, i, jS + 1);
+                for (i = iW + 1; i <= iE; i++) F(p, i, jS) = bcvk(r, ir, jr, SOUTH, VP, i) + F(p, i, jS + 1);
                 break;
             case BM_OUTLT1: case BM_OUTLT2:
-                for (i = iW + 1; i <= iE; i++) F(p, i, jS) = dTwo * bcv(r, ir, jr, SOUTH, VP) - F(p, i, jS + 1);
+                for (i = iW + 1; i <= iE; i++) F(p, i, jS) = dTwo * bcvk(r, ir, jr, SOUTH, VP, i) - F(p, i, jS + 1);
                 break;
             default:
                 fprintf(stderr, "Wrong nBdTypeS flag in region %d,%d\n", ir, jr);
@@ -895,10 +905,10 @@ int wd2o_pres_bound_cond(const wd2o_regions *r, double *p)
             switch (btp(r, ir, jr, NORTH)) {
             case BM_INTERN: break;
             case BM_WALL1: case BM_WALL2: case BM_INLET:
-                for (i = iW + 1; i <= iE; i++) F(p, i, jN + 1) = bcv(r, ir, jr, NORTH, VP) + F(p, i, jN);
+                for (i = iW + 1; i <= iE; i++) F(p, i, jN + 1) = bcvk(r, ir, jr, NORTH, VP, i) + F(p, i, jN);
                 break;
             case BM_OUTLT1: case BM_OUTLT2:
-                for (i = iW + 1; i <= iE; i++) F(p, i, jN + 1) = dTwo * bcv(r, ir, jr, NORTH, VP) - F(p, i, jN);
+                for (i = iW + 1; i <= iE; i++) F(p, i, jN + 1) = dTwo * bcvk(r, ir, jr, NORTH, VP, i) - F(p, i, jN);
                 break;
             default:
                 fprintf(stderr, "Wrong nBdTypeN flag in region %d,%d\n", ir, jr);
@@ -919,7 +929,7 @@ int wd2o_vel_outflow_bcs(const wd2o_regions *r, double *u, double *v)
             switch (btp(r, ir, jr, WEST)) {
             case BM_INTERN: case BM_WALL1: case BM_WALL2: case BM_INLET: break;
             case BM_OUTLT1:
-                for (j = jS; j <= jN; j++) F(u, iW - 1, j) = bcv(r, ir, jr, WEST, VU) + F(u, iW, j);
+                for (j = jS; j <= jN; j++) F(u, iW - 1, j) = bcvk(r, ir, jr, WEST, VU, j) + F(u, iW, j);
                 for (j = jS + 1; j <= jN; j++) F(v, iW, j) = -F(v, iW + 1, j);
                 break;
             case BM_OUTLT2:
@@ -936,7 +946,7 @@ int wd2o_vel_outflow_bcs(const wd2o_regions *r, double *u, double *v)
             switch (btp(r, ir, jr, EAST)) {
             case BM_INTERN: case BM_WALL1: case BM_WALL2: case BM_INLET: break;
             case BM_OUTLT1:
-                for (j = jS; j <= jN; j++) F(u, iE + 1, j) = bcv(r, ir, jr, EAST, VU) + F(u, iE, j);
+                for (j = jS; j <= jN; j++) F(u, iE + 1, j) = bcvk(r, ir, jr, EAST, VU, j) + F(u, iE, j);
                 for (j = jS + 1; j <= jN; j++) F(v, iE + 1, j) = -F(v, iE, j);
                 break;
             case BM_OUTLT2:
@@ -953,7 +963,7 @@ int wd2o_vel_outflow_bcs(const wd2o_regions *r, double *u, double *v)
             case BM_INTERN: case BM_WALL1: case BM_WALL2: case BM_INLET: break;
             case BM_OUTLT1:
                 for (i = iW + 1; i <= iE; i++) F(u, i, jS) = -F(u, i, jS + 1);
-                for (i = iW; i <= iE; i++) F(v, i, jS) = bcv(r, ir, jr, SOUTH, VV) + F(v, i, jS);
+                for (i = iW; i <= iE; i++) F(v, i, jS) = bcvk(r, ir, jr, SOUTH, VV, i) + F(v, i, jS);
                 break;
             case BM_OUTLT2:
                 for (i = iW + 1; i <= iE; i++) F(v, i, jS) = F(v, i, jS + 1) + (F(u, i, jS + 1) - F(u, i - 1, jS + 1));
@@ -969,7 +979,7 @@ int wd2o_vel_outflow_bcs(const wd2o_regions *r, double *u, double *v)
             case BM_INTERN: case BM_WALL1: case BM_WALL2: case BM_INLET: break;
             case BM_OUTLT1:
                 for (i = iW + 1; i <= iE; i++) F(u, i, jN + 1) = -F(u, i, jN);
-                for (i = iW; i <= iE; i++) F(v, i, jN + 1) = bcv(r, ir, jr, NORTH, VV) + F(v, i, jN);
+                for (i = iW; i <= iE; i++) F(v, i, jN + 1) = bcvk(r, ir, jr, NORTH, VV, i) + F(v, i, jN);
                 break;
             case BM_OUTLT2:
                 for (i = iW; i <= iE; i++) F(v, i, jN) = F(v, i, jN - 1) - (F(u, i, jN) - F(u, i - 1, jN));

@@ -38,6 +38,15 @@ typedef struct {
     const int *nMomBdTp;   /* (mgri,mgrj,4) */
     const double *dBCVal;  /* (mgri,mgrj,4,4) */
     const double *dPRporos, *dPRporc1, *dPRporc2;   /* (mgri,mgrj) */
+    /* Boundary profiles, an extension with no counterpart in the reference
+     * (which holds one constant per region side in dBCVal): NULL, or
+     * poff(mgri,mgrj,4,3) for the variables u, v, p, each -1 (the side uses
+     * dBCVal) or the offset in pval of one value per grid line along the
+     * side, indexed by j (0..ny+1) on the west/east sides and by i (0..nx+1)
+     * on the south/north sides.  Every formula of bound_cond.f is kept, with
+     * the constant replaced by the line's value. */
+    const int *poff;
+    const double *pval;
 } wd2o_regions;
 
 /* The metric arrays produced by grid.f:Metric (names as in main.f:103-118). */

@@ -145,6 +145,19 @@ class Solver:
                                    ip(reg.nRegBrd), ip(reg.nRegType), ip(reg.nMomBdTp), fp(reg.dBCVal),
                                    fp(reg.dPRporos), fp(reg.dPRporc1), fp(reg.dPRporc2), mets))
         self.params = make_params(prm)
+        for (ir, jr, side, var), values in reg.profiles.items():
+            self.set_bc_profile(ir, jr, side, var, values)
+
+    def set_bc_profile(self, ir: int, jr: int, side: int, var: int, values) -> None:
+        """One value per grid line in place of the constant boundary value of
+        variable ``var`` (case.VAR_U/V/P) on side ``side`` of region (ir, jr):
+        indexed by j = 0..ny+1 on the west/east sides, by i = 0..nx+1 on the
+        south/north sides.  None restores the constant."""
+        if values is None:
+            check(lib().wd2_ctx_set_bc_profile(self.ctx, ir, jr, side, var, None, 0))
+            return
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().wd2_ctx_set_bc_profile(self.ctx, ir, jr, side, var, dp(a), a.size))
 
     def close(self):
         if self.ctx:

@@ -185,6 +185,12 @@ class Regions:
     All tables are Fortran-ordered int32/float64 arrays of the reference's
     declared extents (mgri, mgrj): nRegBrd(mgri,mgrj,4), nRegType(mgri,mgrj),
     nMomBdTp(mgri,mgrj,4), dBCVal(mgri,mgrj,4,4), dPR*(mgri,mgrj).
+
+    ``profiles`` is an extension with no counterpart in the reference, which
+    holds one constant per region side and variable: {(ir, jr, side, var):
+    values} replaces the constant dBCVal(ir,jr,side,var) of u, v or p by one
+    value per grid line along the side, indexed by j = 0..ny+1 on the west and
+    east sides and by i = 0..nx+1 on the south and north sides.
     """
     nx: int
     ny: int
@@ -198,6 +204,7 @@ class Regions:
     dPRporc2: np.ndarray
     mgri: int = MGRI
     mgrj: int = MGRJ
+    profiles: dict = field(default_factory=dict)
 
     def border(self, ir, jr, side):
         return int(self.nRegBrd[ir - 1, jr - 1, side - 1])
@@ -207,6 +214,88 @@ class Regions:
 
     def rtype(self, ir, jr):
         return int(self.nRegType[ir - 1, jr - 1])
+
+    def profile_length(self, side: int) -> int:
+        return self.ny + 2 if side in (WEST, EAST) else self.nx + 2
+
+    def set_profile(self, ir: int, jr: int, side: int, var: int, values) -> None:
+        """Give side ``side`` of region (ir, jr) a profile of variable ``var``
+        (VAR_U, VAR_V or VAR_P); ``values`` None removes it."""
+        if not (1 <= ir <= self.nreg[0] and 1 <= jr <= self.nreg[1]):
+            raise ValueError(f"region {ir},{jr} outside the {self.nreg[0]}x{self.nreg[1]} regions")
+        if side not in (WEST, EAST, SOUTH, NORTH) or var not in (VAR_U, VAR_V, VAR_P):
+            raise ValueError(f"bad side {side} or variable {var}")
+        key = (ir, jr, side, var)
+        if values is None:
+            self.profiles.pop(key, None)
+            return
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if a.shape != (self.profile_length(side),):
+            raise ValueError(f"profile of side {side} needs {self.profile_length(side)} values, got {a.shape}")
+        self.profiles[key] = a.copy()
+
+    def profile_tables(self):
+        """(poff, pval): poff(mgri,mgrj,4,3) Fortran-ordered int32, -1 where a
+        side uses its constant, else the offset of its values in pval; both
+        None without profiles."""
+        if not self.profiles:
+            return None, None
+        poff = np.full((self.mgri, self.mgrj, 4, 3), -1, dtype=np.int32, order="F")
+        parts, off = [], 0
+        for (ir, jr, side, var), a in sorted(self.profiles.items()):
+            poff[ir - 1, jr - 1, side - 1, var - 1] = off
+            parts.append(a)
+            off += a.size
+        return poff, np.concatenate(parts)
+
+
+def side_positions(x: np.ndarray, y: np.ndarray, reg: Regions, ir: int, jr: int, side: int,
+                   var: int) -> np.ndarray:
+    """Where each grid line's boundary value of a region side applies, as the
+    fraction s of the side's length (0 at its first border node, 1 at its
+    last), for k = 0..ny+1 (west/east sides) or 0..nx+1 (south/north).
+
+    Along a west or east side u and p sit at cell centres (between nodes j-1
+    and j) and v on node j; along a south or north side v and p sit at cell
+    centres (between nodes i-1 and i) and u on node i.  Lengths are measured
+    along the side's grid line; the ghost lines 0 and n+1 lie one spacing
+    beyond the end nodes.  x, y are the (ny, nx) node arrays."""
+    if side in (WEST, EAST):
+        col = reg.border(ir, jr, side) - 1
+        px, py = x[:, col], y[:, col]
+        lo, hi = reg.border(ir, jr, SOUTH), reg.border(ir, jr, NORTH)
+        centred = var in (VAR_U, VAR_P)
+    else:
+        row = reg.border(ir, jr, side) - 1
+        px, py = x[row, :], y[row, :]
+        lo, hi = reg.border(ir, jr, WEST), reg.border(ir, jr, EAST)
+        centred = var in (VAR_V, VAR_P)
+    arc = np.concatenate(([0.0], np.cumsum(np.hypot(np.diff(px), np.diff(py)))))
+    n = arc.size                                  # nodes 1..n at arc[0..n-1]
+    node = np.empty(n + 2)                        # node index 0..n+1
+    node[1:n + 1] = arc
+    node[0] = 2.0 * arc[0] - arc[1]
+    node[n + 1] = 2.0 * arc[-1] - arc[-2]
+    if centred:                                   # line k between nodes k-1 and k
+        pos = np.empty(n + 2)
+        pos[1:] = 0.5 * (node[:-1] + node[1:])
+        pos[0] = 2.0 * node[0] - pos[1]
+    else:
+        pos = node
+    return (pos - node[lo]) / (node[hi] - node[lo])
+
+
+def parabolic_profile(s: np.ndarray, peak: float) -> np.ndarray:
+    """4 peak s (1 - s): zero at both ends of the side, ``peak`` at its middle."""
+    return 4.0 * peak * s * (1.0 - s)
+
+
+def sine_profile(s: np.ndarray, amplitudes) -> np.ndarray:
+    """sum_k a_k sin(k pi s), k = 1..len(amplitudes)."""
+    out = np.zeros_like(s)
+    for k, a in enumerate(amplitudes, start=1):
+        out += a * np.sin(k * np.pi * s)
+    return out
 
 
 def make_regions(nx: int, ny: int, nreg=(1, 1), i_borders=(), j_borders=(),

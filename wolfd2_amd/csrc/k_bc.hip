@@ -7,6 +7,11 @@
 // walks the regions and sides in the reference's order; each loop whose
 // iterations are independent is spread over the workgroup, the recurrences
 // run on one lane, and a barrier separates consecutive loops.
+//
+// Boundary values: the reference's dBCVal holds one constant per region side
+// and variable.  A side may instead carry a profile, one value per grid line
+// (Regions::bcp); every formula is the reference's with the constant replaced
+// by the line's value, so a profile of constant values gives the same bits.
 #include "wd2_dev.hpp"
 #include "wd2_launch.hpp"
 
@@ -33,18 +38,20 @@ __global__ __launch_bounds__(BC_THREADS) void k_vel_bc(Regions R, int ld, T *__r
             // ---- WEST
             {
                 const int t = R.btype(ir, jr, WEST);
-                const T bu = (T)R.bc(ir, jr, WEST, VU), bv = (T)R.bc(ir, jr, WEST, VV);
+                const int ou = R.prof(ir, jr, WEST, VU), ov = R.prof(ir, jr, WEST, VV);
+                auto bu = [&](int k) { return (T)R.bcp(ou, ir, jr, WEST, VU, k); };
+                auto bv = [&](int k) { return (T)R.bcp(ov, ir, jr, WEST, VV, k); };
                 if (!outflow_only && t == BM_WALL1) {
                     PAR(j, jS, jN) U(iW, j) = 0;
-                    PAR(j, jS + 1, jN) V(iW, j) = dTwo * bv - V(iW + 1, j);
+                    PAR(j, jS + 1, jN) V(iW, j) = dTwo * bv(j) - V(iW + 1, j);
                 } else if (!outflow_only && t == BM_WALL2) {
                     PAR(j, jS, jN) U(iW, j) = 0;
                     PAR(j, jS + 1, jN) V(iW, j) = V(iW + 1, j);
                 } else if (!outflow_only && t == BM_INLET) {
-                    PAR(j, jS, jN) U(iW, j) = bu;
-                    PAR(j, jS + 1, jN) V(iW, j) = dTwo * bv - V(iW + 1, j);
+                    PAR(j, jS, jN) U(iW, j) = bu(j);
+                    PAR(j, jS + 1, jN) V(iW, j) = dTwo * bv(j) - V(iW + 1, j);
                 } else if (t == BM_OUTLT1) {
-                    PAR(j, jS, jN) U(iW - 1, j) = bu + U(iW, j);
+                    PAR(j, jS, jN) U(iW - 1, j) = bu(j) + U(iW, j);
                     PAR(j, jS + 1, jN) V(iW, j) = -V(iW + 1, j);
                 } else if (t == BM_OUTLT2) {
                     if (!outflow_only) {
@@ -64,18 +71,20 @@ __global__ __launch_bounds__(BC_THREADS) void k_vel_bc(Regions R, int ld, T *__r
             // ---- EAST
             {
                 const int t = R.btype(ir, jr, EAST);
-                const T bu = (T)R.bc(ir, jr, EAST, VU), bv = (T)R.bc(ir, jr, EAST, VV);
+                const int ou = R.prof(ir, jr, EAST, VU), ov = R.prof(ir, jr, EAST, VV);
+                auto bu = [&](int k) { return (T)R.bcp(ou, ir, jr, EAST, VU, k); };
+                auto bv = [&](int k) { return (T)R.bcp(ov, ir, jr, EAST, VV, k); };
                 if (!outflow_only && t == BM_WALL1) {
                     PAR(j, jS, jN) U(iE, j) = 0;
-                    PAR(j, jS + 1, jN) V(iE + 1, j) = dTwo * bv - V(iE, j);
+                    PAR(j, jS + 1, jN) V(iE + 1, j) = dTwo * bv(j) - V(iE, j);
                 } else if (!outflow_only && t == BM_WALL2) {
                     PAR(j, jS, jN) U(iE, j) = 0;
                     PAR(j, jS + 1, jN) V(iE + 1, j) = V(iE, j);
                 } else if (!outflow_only && t == BM_INLET) {
-                    PAR(j, jS, jN) U(iE, j) = bu;
-                    PAR(j, jS + 1, jN) V(iE + 1, j) = dTwo * bv - V(iE, j);
+                    PAR(j, jS, jN) U(iE, j) = bu(j);
+                    PAR(j, jS + 1, jN) V(iE + 1, j) = dTwo * bv(j) - V(iE, j);
                 } else if (t == BM_OUTLT1) {
-                    PAR(j, jS, jN) U(iE + 1, j) = bu + U(iE, j);
+                    PAR(j, jS, jN) U(iE + 1, j) = bu(j) + U(iE, j);
                     PAR(j, jS + 1, jN) V(iE + 1, j) = -V(iE, j);
                 } else if (t == BM_OUTLT2) {
                     PAR(j, jS + 1, jN) U(iE, j) = U(iE - 1, j) - (V(iE, j) - V(iE, j - 1));
@@ -91,19 +100,21 @@ __global__ __launch_bounds__(BC_THREADS) void k_vel_bc(Regions R, int ld, T *__r
             // ---- SOUTH
             {
                 const int t = R.btype(ir, jr, SOUTH);
-                const T bu = (T)R.bc(ir, jr, SOUTH, VU), bv = (T)R.bc(ir, jr, SOUTH, VV);
+                const int ou = R.prof(ir, jr, SOUTH, VU), ov = R.prof(ir, jr, SOUTH, VV);
+                auto bu = [&](int k) { return (T)R.bcp(ou, ir, jr, SOUTH, VU, k); };
+                auto bv = [&](int k) { return (T)R.bcp(ov, ir, jr, SOUTH, VV, k); };
                 if (!outflow_only && t == BM_WALL1) {
-                    PAR(i, iW + 1, iE) U(i, jS) = dTwo * bu - U(i, jS + 1);
+                    PAR(i, iW + 1, iE) U(i, jS) = dTwo * bu(i) - U(i, jS + 1);
                     PAR(i, iW, iE) V(i, jS) = 0;
                 } else if (!outflow_only && t == BM_WALL2) {
                     PAR(i, iW + 1, iE) U(i, jS) = U(i, jS + 1);
                     PAR(i, iW, iE) V(i, jS) = 0;
                 } else if (!outflow_only && t == BM_INLET) {
-                    PAR(i, iW + 1, iE) U(i, jS) = dTwo * bu - U(i, jS + 1);
-                    PAR(i, iW, iE) V(i, jS) = bv;
+                    PAR(i, iW + 1, iE) U(i, jS) = dTwo * bu(i) - U(i, jS + 1);
+                    PAR(i, iW, iE) V(i, jS) = bv(i);
                 } else if (t == BM_OUTLT1) {
                     PAR(i, iW + 1, iE) U(i, jS) = -U(i, jS + 1);
-                    PAR(i, iW, iE) V(i, jS) = bv + V(i, jS);
+                    PAR(i, iW, iE) V(i, jS) = bv(i) + V(i, jS);
                 } else if (t == BM_OUTLT2) {
                     PAR(i, iW + 1, iE) V(i, jS) = V(i, jS + 1) + (U(i, jS + 1) - U(i - 1, jS + 1));
                     SYNC();
@@ -118,19 +129,21 @@ __global__ __launch_bounds__(BC_THREADS) void k_vel_bc(Regions R, int ld, T *__r
             // ---- NORTH
             {
                 const int t = R.btype(ir, jr, NORTH);
-                const T bu = (T)R.bc(ir, jr, NORTH, VU), bv = (T)R.bc(ir, jr, NORTH, VV);
+                const int ou = R.prof(ir, jr, NORTH, VU), ov = R.prof(ir, jr, NORTH, VV);
+                auto bu = [&](int k) { return (T)R.bcp(ou, ir, jr, NORTH, VU, k); };
+                auto bv = [&](int k) { return (T)R.bcp(ov, ir, jr, NORTH, VV, k); };
                 if (!outflow_only && t == BM_WALL1) {
-                    PAR(i, iW + 1, iE) U(i, jN + 1) = dTwo * bu - U(i, jN);
+                    PAR(i, iW + 1, iE) U(i, jN + 1) = dTwo * bu(i) - U(i, jN);
                     PAR(i, iW, iE) V(i, jN) = 0;
                 } else if (!outflow_only && t == BM_WALL2) {
                     PAR(i, iW + 1, iE) U(i, jN + 1) = U(i, jN);
                     PAR(i, iW, iE) V(i, jN) = 0;
                 } else if (!outflow_only && t == BM_INLET) {
-                    PAR(i, iW + 1, iE) U(i, jN + 1) = dTwo * bu - U(i, jN);
-                    PAR(i, iW, iE) V(i, jN) = bv;
+                    PAR(i, iW + 1, iE) U(i, jN + 1) = dTwo * bu(i) - U(i, jN);
+                    PAR(i, iW, iE) V(i, jN) = bv(i);
                 } else if (t == BM_OUTLT1) {
                     PAR(i, iW + 1, iE) U(i, jN + 1) = -U(i, jN);
-                    PAR(i, iW, iE) V(i, jN + 1) = bv + V(i, jN);
+                    PAR(i, iW, iE) V(i, jN + 1) = bv(i) + V(i, jN);
                 } else if (t == BM_OUTLT2) {
                     PAR(i, iW, iE) V(i, jN) = V(i, jN - 1) - (U(i, jN) - U(i - 1, jN));
                     SYNC();
@@ -157,56 +170,54 @@ __global__ __launch_bounds__(BC_THREADS) void k_pres_bc(Regions R, int ld, T *__
         for (int ir = 1; ir <= R.ni; ir++) {
             const int iW = R.border(ir, jr, WEST), iE = R.border(ir, jr, EAST);
             const int jS = R.border(ir, jr, SOUTH), jN = R.border(ir, jr, NORTH);
+            // boundary value of the pressure on line k of a side
+            auto bp = [&](int side, int k) { return (T)R.bcp(R.prof(ir, jr, side, VP), ir, jr, side, VP, k); };
             if (R.type(ir, jr) == RM_BLOCKG) {
                 const int w = iE - iW, h = jN - jS;
                 for (int k = threadIdx.x; k < w * h; k += BC_THREADS) P(iW + 1 + k % w, jS + 1 + k / w) = 0;
                 SYNC();
-                PAR(j, jS + 1, jN) P(iW + 1, j) = (T)R.bc(ir, jr, WEST, VP) + P(iW, j);
+                PAR(j, jS + 1, jN) P(iW + 1, j) = bp(WEST, j) + P(iW, j);
                 SYNC();
-                PAR(j, jS + 1, jN) P(iE, j) = (T)R.bc(ir, jr, EAST, VP) + P(iE + 1, j);
+                PAR(j, jS + 1, jN) P(iE, j) = bp(EAST, j) + P(iE + 1, j);
                 SYNC();
-                PAR(i, iW + 1, iE) P(i, jS + 1) = (T)R.bc(ir, jr, SOUTH, VP) + P(i, jS);
+                PAR(i, iW + 1, iE) P(i, jS + 1) = bp(SOUTH, i) + P(i, jS);
                 SYNC();
-                PAR(i, iW + 1, iE) P(i, jN) = (T)R.bc(ir, jr, NORTH, VP) + P(i, jN + 1);
+                PAR(i, iW + 1, iE) P(i, jN) = bp(NORTH, i) + P(i, jN + 1);
                 SYNC();
                 continue;
             }
             int t = R.btype(ir, jr, WEST);
-            T b = (T)R.bc(ir, jr, WEST, VP);
             if (t == BM_WALL1 || t == BM_WALL2 || t == BM_INLET) {
-                PAR(j, jS + 1, jN) P(iW, j) = b + P(iW + 1, j);
+                PAR(j, jS + 1, jN) P(iW, j) = bp(WEST, j) + P(iW + 1, j);
             } else if (t == BM_OUTLT1 || t == BM_OUTLT2) {
-                PAR(j, jS + 1, jN) P(iW, j) = dTwo * b - P(iW + 1, j);
+                PAR(j, jS + 1, jN) P(iW, j) = dTwo * bp(WEST, j) - P(iW + 1, j);
             } else if (t != BM_INTERN && threadIdx.x == 0) {
                 *err = 1000 + WEST;
             }
             SYNC();
             t = R.btype(ir, jr, EAST);
-            b = (T)R.bc(ir, jr, EAST, VP);
             if (t == BM_WALL1 || t == BM_WALL2 || t == BM_INLET) {
-                PAR(j, jS + 1, jN) P(iE + 1, j) = b + P(iE, j);
+                PAR(j, jS + 1, jN) P(iE + 1, j) = bp(EAST, j) + P(iE, j);
             } else if (t == BM_OUTLT1 || t == BM_OUTLT2) {
-                PAR(j, jS + 1, jN) P(iE + 1, j) = dTwo * b - P(iE, j);
+                PAR(j, jS + 1, jN) P(iE + 1, j) = dTwo * bp(EAST, j) - P(iE, j);
             } else if (t != BM_INTERN && threadIdx.x == 0) {
                 *err = 1000 + EAST;
             }
             SYNC();
             t = R.btype(ir, jr, SOUTH);
-            b = (T)R.bc(ir, jr, SOUTH, VP);
             if (t == BM_WALL1 || t == BM_WALL2 || t == BM_INLET) {
-                PAR(i, iW + 1, iE) P(i, jS) = b + P(i, jS + 1);
+                PAR(i, iW + 1, iE) P(i, jS) = bp(SOUTH, i) + P(i, jS + 1);
             } else if (t == BM_OUTLT1 || t == BM_OUTLT2) {
-                PAR(i, iW + 1, iE) P(i, jS) = dTwo * b - P(i, jS + 1);
+                PAR(i, iW + 1, iE) P(i, jS) = dTwo * bp(SOUTH, i) - P(i, jS + 1);
             } else if (t != BM_INTERN && threadIdx.x == 0) {
                 *err = 1000 + SOUTH;
             }
             SYNC();
             t = R.btype(ir, jr, NORTH);
-            b = (T)R.bc(ir, jr, NORTH, VP);
             if (t == BM_WALL1 || t == BM_WALL2 || t == BM_INLET) {
-                PAR(i, iW + 1, iE) P(i, jN + 1) = b + P(i, jN);
+                PAR(i, iW + 1, iE) P(i, jN + 1) = bp(NORTH, i) + P(i, jN);
             } else if (t == BM_OUTLT1 || t == BM_OUTLT2) {
-                PAR(i, iW + 1, iE) P(i, jN + 1) = dTwo * b - P(i, jN);
+                PAR(i, iW + 1, iE) P(i, jN + 1) = dTwo * bp(NORTH, i) - P(i, jN);
             } else if (t != BM_INTERN && threadIdx.x == 0) {
                 *err = 1000 + NORTH;
             }

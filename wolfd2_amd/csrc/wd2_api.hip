@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -20,8 +21,8 @@ using namespace wd2;
 static thread_local std::string g_err;
 static int g_fallbacks = 0;
 static int g_tiled_sor = 1;   // temporally blocked SorRB when the grid allows it
-static int g_verbose = 1;
-static int g_lex_rows = 1;    // lexicographic Sor as one single-workgroup launch when the rows fit     // print the reference's warnings (pressure.f:242-246, 234)
+static int g_verbose = 1;     // print the reference's warnings (pressure.f:242-246, 234)
+static int g_lex_rows = 1;    // lexicographic Sor as one single-workgroup launch when the rows fit
 
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -57,6 +58,12 @@ struct wd2_ctx {
     int *d_int = nullptr;
     double *d_dbl = nullptr;
     Regions R{};
+    // boundary profiles by their index in the (mgri,mgrj,4,3) offset table,
+    // and the device copies of that table and of the values
+    std::map<int, std::vector<double>> h_prof;
+    int *d_poff = nullptr;
+    double *d_pval = nullptr;
+    size_t pval_cap = 0;
     bool any_outlet = false, any_porous = false, any_blockage = false;
     unsigned char *prow = nullptr;
     // metrics
@@ -126,6 +133,8 @@ static void ctx_free(wd2_ctx *c) {
     }
     (void)hipFree(c->d_int);
     (void)hipFree(c->d_dbl);
+    (void)hipFree(c->d_poff);
+    (void)hipFree(c->d_pval);
     (void)hipFree(c->prow);
     (void)hipFree(c->metric_store);
     (void)hipFree(c->store);
@@ -1001,6 +1010,52 @@ int wd2_ctx_set_precision(wd2_ctx *c, int precision) {
         HIPCHK(hipStreamSynchronize(c->s));
     }
     c->precision = precision;
+    return WD2_OK;
+}
+
+int wd2_ctx_set_bc_profile(wd2_ctx *c, int ireg, int jreg, int side, int var, const double *values, int n) {
+    if (!c) return fail(WD2_ERR_ARG, "wd2_ctx_set_bc_profile: null context");
+    if (ireg < 1 || ireg > c->h_nreg[0] || jreg < 1 || jreg > c->h_nreg[1])
+        return fail(WD2_ERR_ARG, "wd2_ctx_set_bc_profile: region %d,%d outside the %d x %d regions", ireg, jreg,
+                    c->h_nreg[0], c->h_nreg[1]);
+    if (side < WEST || side > NORTH || var < VU || var > VP)
+        return fail(WD2_ERR_ARG, "wd2_ctx_set_bc_profile: bad side %d or variable %d", side, var);
+    const int want = side <= EAST ? c->ny + 2 : c->nx + 2;
+    const int key = (ireg - 1) + c->mgri * ((jreg - 1) + c->mgrj * ((side - 1) + 4 * (var - 1)));
+    if (values && n != want)
+        return fail(WD2_ERR_ARG, "wd2_ctx_set_bc_profile: side %d needs %d values (one per grid line), got %d", side,
+                    want, n);
+    // kernels in flight may still read the current tables
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (values)
+        c->h_prof[key].assign(values, values + n);
+    else
+        c->h_prof.erase(key);
+    if (c->h_prof.empty()) {
+        c->R.poff = nullptr;
+        c->R.pval = nullptr;
+        return WD2_OK;
+    }
+    const size_t n2 = (size_t)c->mgri * c->mgrj;
+    std::vector<int> off(12 * n2, -1);
+    std::vector<double> val;
+    for (const auto &kv : c->h_prof) {
+        off[kv.first] = (int)val.size();
+        val.insert(val.end(), kv.second.begin(), kv.second.end());
+    }
+    if (!c->d_poff) HIPCHK(hipMalloc(&c->d_poff, 12 * n2 * sizeof(int)));
+    if (val.size() > c->pval_cap) {
+        (void)hipFree(c->d_pval);
+        c->d_pval = nullptr;
+        c->pval_cap = 0;
+        HIPCHK(hipMalloc(&c->d_pval, val.size() * sizeof(double)));
+        c->pval_cap = val.size();
+    }
+    HIPCHK(hipMemcpyAsync(c->d_poff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_pval, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->R.poff = c->d_poff;
+    c->R.pval = c->d_pval;
     return WD2_OK;
 }
 

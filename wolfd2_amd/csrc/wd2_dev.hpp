@@ -24,10 +24,19 @@ enum { WEST = 1, EAST = 2, SOUTH = 3, NORTH = 4 };
 enum { VU = 1, VV = 2, VP = 3, VT = 4 };
 
 // Region tables on the device (Fortran-ordered copies of the caller's tables).
+//
+// Boundary profiles (an extension: the reference holds one constant per
+// region side and variable, dBCVal): poff[(ir-1) + mgri*((jr-1) + mgrj*((side-1)
+// + 4*(var-1)))] for var = VU, VV, VP is -1 for a side that uses the constant,
+// or the offset in pval of one value per grid line along the side -- indexed
+// by j (0..ny+1) on the west and east sides, by i (0..nx+1) on the south and
+// north sides.  bcp() returns the value for line k.
 struct Regions {
     int nx, ny, ni, nj, mgri, mgrj;
     const int *brd, *rtype, *btp;
     const double *bcv, *poros, *porc1, *porc2;
+    const int *poff;      // nullptr: no profiles
+    const double *pval;
 
     __host__ __device__ int border(int ir, int jr, int side) const {
         return brd[(ir - 1) + mgri * ((jr - 1) + mgrj * (side - 1))];
@@ -40,6 +49,13 @@ struct Regions {
         return bcv[(ir - 1) + mgri * ((jr - 1) + mgrj * ((side - 1) + 4 * (var - 1)))];
     }
     __host__ __device__ double tab(const double *t, int ir, int jr) const { return t[(ir - 1) + mgri * (jr - 1)]; }
+    __host__ __device__ int prof(int ir, int jr, int side, int var) const {
+        return poff ? poff[(ir - 1) + mgri * ((jr - 1) + mgrj * ((side - 1) + 4 * (var - 1)))] : -1;
+    }
+    // boundary value on grid line k of a side: the profile's, else the constant
+    __host__ __device__ double bcp(int off, int ir, int jr, int side, int var, int k) const {
+        return off >= 0 ? pval[off + k] : bc(ir, jr, side, var);
+    }
 };
 
 // Metric arrays used on the hot path (grid.f:Metric names).

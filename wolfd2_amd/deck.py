@@ -63,6 +63,7 @@ class Deck:
     i_borders: tuple = ()
     j_borders: tuple = ()
     bc: list = field(default_factory=list)   # (keyword, ireg, jreg, [side], words)
+    profiles: list = field(default_factory=list)   # bc_profile lines: (ireg, jreg, side, var, words)
 
 
 def _words(line: str) -> list:
@@ -169,6 +170,10 @@ def _bc_keyword(d: Deck, key: str, a: list) -> None:
         d.bc.append((key, int(a[0]), int(a[1]), a[2].lower(), [x.lower() for x in a[3:]]))
     elif key == "blockage":
         d.bc.append((key, int(a[0]), int(a[1]), None, []))
+    elif key == "bc_profile":
+        if a[2].lower() not in case.SIDES or a[3].lower() not in PROFILE_VARS or a[4].lower() not in PROFILE_SHAPES:
+            raise ValueError("expected bc_profile IREG JREG SIDE u|v|p parabolic|sine|file ...")
+        d.profiles.append((int(a[0]), int(a[1]), a[2].lower(), a[3].lower(), [a[4].lower()] + a[5:]))
     elif key in ("porous_region", "heat_generation_region", "fixed_temperature_region", "fixed_temperature_wall",
                  "heat_flux_wall"):
         raise DeckError(f"'{key}' is outside the GPU hot path (use the Fortran drop-in build for it)")
@@ -238,10 +243,10 @@ def read_grid(path: str):
     return x, y
 
 
-def build_case(d: Deck, rho: float, nx: int, ny: int) -> case.Regions:
+def build_case(d: Deck, rho: float, nx: int, ny: int, x=None, y=None, case_dir: str = ".") -> case.Regions:
     """Region tables of the boundary_conditions section, with the reference's
     scalings: velocities / uref (parse.f:1532, 1630), pressures / (rho uref^2)
-    (parse.f:1550, 1739)."""
+    (parse.f:1550, 1739).  bc_profile lines need the grid nodes x, y."""
     uref = d.settings["uref"]
     pscale = rho * uref * uref
     walls, inlets, outlets, blocks = [], [], [], []
@@ -270,8 +275,49 @@ def build_case(d: Deck, rho: float, nx: int, ny: int) -> case.Regions:
     # make_regions applies blockages, then walls, inlets and outlets; the
     # reference applies the lines in deck order, which differs only for a deck
     # that gives one region face two conditions
-    return case.make_regions(nx, ny, nreg=d.nreg, i_borders=d.i_borders, j_borders=d.j_borders, walls=walls,
-                             inlets=inlets, outlets=outlets, blockages=blocks)
+    reg = case.make_regions(nx, ny, nreg=d.nreg, i_borders=d.i_borders, j_borders=d.j_borders, walls=walls,
+                            inlets=inlets, outlets=outlets, blockages=blocks)
+    if d.profiles:
+        if x is None or y is None:
+            raise DeckError("bc_profile needs the grid nodes")
+        for ir, jr, side, var, w in d.profiles:
+            _apply_profile(reg, x, y, ir, jr, case.SIDES[side], PROFILE_VARS[var], w,
+                           pscale if var == "p" else uref, case_dir)
+    return reg
+
+
+# ------------------------------------------------------- boundary profiles
+# An extension with no counterpart in the reference, whose boundary values are
+# one constant per region side (dBCVal).  In the boundary_conditions section:
+#
+#   bc_profile IREG JREG SIDE VAR parabolic PEAK     4 PEAK s (1 - s)
+#   bc_profile IREG JREG SIDE VAR sine A1 A2 ... An   sum_k Ak sin(k pi s)
+#   bc_profile IREG JREG SIDE VAR file NAME           one value per grid line
+#
+# VAR is u, v or p; s is the position along the side (0 at its first border
+# node, 1 at its last) of the point each grid line's value applies to
+# (case.side_positions).  A file lists ny+2 values (west/east sides, j = 0..ny+1)
+# or nx+2 (south/north, i = 0..nx+1).  Values are dimensional and scaled like
+# the constants: velocities / uref, pressures / (rho uref^2).
+PROFILE_VARS = {"u": case.VAR_U, "v": case.VAR_V, "p": case.VAR_P}
+PROFILE_SHAPES = ("parabolic", "sine", "file")
+
+
+def _apply_profile(reg, x, y, ir, jr, side, var, w, scale, case_dir):
+    shape, args = w[0], w[1:]
+    if shape == "file":
+        with open(os.path.join(case_dir, args[0])) as f:
+            vals = np.array(f.read().split(), dtype=np.float64)
+    else:
+        sp = case.side_positions(x, y, reg, ir, jr, side, var)
+        if shape == "parabolic":
+            vals = case.parabolic_profile(sp, float(args[0]))
+        else:
+            vals = case.sine_profile(sp, [float(a) for a in args])
+    try:
+        reg.set_profile(ir, jr, side, var, vals / scale)
+    except ValueError as e:
+        raise DeckError(f"bc_profile {ir} {jr}: {e}") from None
 
 
 # ------------------------------------------------------------ restart files

@@ -38,7 +38,7 @@ def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) 
     fl = deck.read_fluid(os.path.join(case_dir, s["flPropFn"]), s["flName"], s["tref"])
     rho, mu = float(fl[1]), float(fl[3])
     g = case.compute_grid(x, y, dlref=s["dlref"])
-    reg = deck.build_case(d, rho, nx, ny)
+    reg = deck.build_case(d, rho, nx, ny, x=x, y=y, case_dir=case_dir)
     prm = case.Params.from_physical(s["dk"], rho, mu, dlref=s["dlref"], uref=s["uref"], mqiter=s["mqiter"],
                                     qtol=s["qtol"], nPpeSolver=s["nPpeSolver"], msorit=s["msorit"],
                                     sortol=s["sortol"], sorrel=s["sorrel"], cartesian=s["cartesian"])
