@@ -138,6 +138,38 @@ int wd2_vel_outflow_bcs(int nx, int ny, int ld, int mgri, int mgrj,
 int wd2_dmax_norm(int nx, int ny, int ld, const double *u, double *out);
 int wd2_diff_max_norm(int nx, int ny, int ld, const double *un, const double *u, double *out);
 
+/* Thermal energy.  Thermal tables: nTRgType(mgri,mgrj) region types
+ * (0 no source, 1 heat generation, 2 fixed temperature), nTemBdTp(mgri,mgrj,4)
+ * boundary types (0 internal, 1 fixed temperature, 2 heat flux),
+ * dTRgVal(mgri,mgrj) region temperatures, dHGSTval(mgri,mgrj) heat sources;
+ * boundary temperatures are dBCVal(.,.,side,4).  A bad nTemBdTp flag returns
+ * WD2_ERR_FLAG (the reference STOPs). */
+/* bound_cond.f:TempBoundCond 1030-1033 */
+int wd2_temp_bound_cond(int nx, int ny, int ld, int mgri, int mgrj,
+                        const int *nReg, const int *nRegBrd, const int *nTRgType, const int *nTemBdTp,
+                        const double *dTRgVal, const double *dBCVal, double *t);
+/* thermal.f:ThermEnergy 24-32: t is advanced in place (it also receives
+ * TempBoundCond's fills); un, vn, tn are time level n, u, v the current
+ * velocities. */
+int wd2_therm_energy(int nx, int ny, int ld, int mgri, int mgrj,
+                     const int *nReg, const int *nRegBrd, const int *nTRgType, const int *nTemBdTp,
+                     double dk, double pe, const double *dTRgVal, const double *dHGSTval,
+                     const double *dBCVal,
+                     const double *rau, const double *rbu, const double *rbv, const double *rgv,
+                     const double *djc, const double *xeu, const double *yeu, const double *xzv,
+                     const double *yzv, const double *xec, const double *yec, const double *xzc,
+                     const double *yzc, const double *un, const double *vn, const double *u,
+                     const double *v, const double *tn, double *t);
+/* thermal.f:EqState 280-282: den on i = 2..nx, j = 2..ny (the rest of den is
+ * left as passed). */
+int wd2_eq_state(int nx, int ny, int ld, double uref, double densref, double tmax, double tref,
+                 double rconst, const double *p, const double *t, double *den);
+/* utility.f:Filter 31-34: ncomp 1 (u), 2 (v) or 4 (t); qu filtered in place.
+ * Another ncomp returns WD2_ERR_FLAG (the reference STOPs). */
+int wd2_filter(int nx, int ny, int ld, int mgri, int mgrj, int ncomp,
+               const int *nReg, const int *nRegBrd, const int *nRegType, const int *nMomBdTp,
+               const int *nTRgType, double fp, double *qu);
+
 /* ----------------------------------------------------------------------
  * Device-resident context.
  */

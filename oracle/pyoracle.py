@@ -57,6 +57,10 @@ class _Regions(C.Structure):
                 ("poff", _ip), ("pval", _dp)]
 
 
+class _Thermal(C.Structure):
+    _fields_ = [("nTRgType", _ip), ("nTemBdTp", _ip), ("dTRgVal", _dp), ("dHGSTval", _dp)]
+
+
 class _Metrics(C.Structure):
     _fields_ = [(nm, _dp) for nm in METRIC_ORDER]
 
@@ -94,6 +98,27 @@ class Oracle:
 
     def metrics(self, grid):
         return _Metrics(*[_d(grid.metrics[nm]) for nm in METRIC_ORDER])
+
+    @staticmethod
+    def thermal(th):
+        return _Thermal(_i(th.nTRgType), _i(th.nTemBdTp), _d(th.dTRgVal), _d(th.dHGSTval))
+
+    # -- thermal routines ----------------------------------------------------
+    def temp_bound_cond(self, reg, th, ld, t):
+        return self.L.wd2o_temp_bound_cond(C.byref(self.regions(reg, ld)), C.byref(self.thermal(th)), _d(t))
+
+    def therm_energy(self, reg, th, grid, dk, pe, un, vn, u, v, tn, t):
+        return self.L.wd2o_therm_energy(C.byref(self.regions(reg, grid.ld)), C.byref(self.thermal(th)),
+                                        C.c_double(dk), C.c_double(pe), C.byref(self.metrics(grid)),
+                                        _d(un), _d(vn), _d(u), _d(v), _d(tn), _d(t))
+
+    def eq_state(self, grid, uref, densref, tmax, tref, rconst, p, t, den):
+        self.L.wd2o_eq_state(grid.nx, grid.ny, grid.ld, *[C.c_double(x) for x in (uref, densref, tmax, tref, rconst)],
+                             _d(p), _d(t), _d(den))
+
+    def filter(self, reg, th, ld, ncomp, fp, q):
+        return self.L.wd2o_filter(C.byref(self.regions(reg, ld)), C.byref(self.thermal(th)), ncomp,
+                                  C.c_double(fp), _d(q))
 
     # -- hot-path routines ---------------------------------------------------
     def alt_trid_lu(self, a: np.ndarray, b: np.ndarray):
@@ -305,6 +330,26 @@ class Reference:
                                         "xec", "yec", "xzn", "yzn", "xen", "yen", "xzc", "yzc",
                                         "xeu", "yeu", "xzu", "yzu", "xev", "yev", "xzv", "yzv")],
                  _d(d), _d(dn), _d(un), _d(vn), _d(us), _d(vs))
+
+    def temp_bound_cond(self, reg, th, t):
+        self.L.tempboundcond_(self._ir(reg.nx), self._ir(reg.ny), _i(reg.nreg), _i(reg.nRegBrd),
+                              _i(th.nTRgType), _i(th.nTemBdTp), _d(th.dTRgVal), _d(reg.dBCVal), _d(t))
+
+    def therm_energy(self, reg, th, g, dk, pe, un, vn, u, v, tn, t):
+        self.L.thermenergy_(self._ir(reg.nx), self._ir(reg.ny), _i(reg.nreg), _i(reg.nRegBrd),
+                            _i(th.nTRgType), _i(th.nTemBdTp), self._dr(dk), self._dr(pe),
+                            _d(th.dTRgVal), _d(th.dHGSTval), _d(reg.dBCVal),
+                            *[_d(g[nm]) for nm in ("rau", "rbu", "rbv", "rgv", "djc", "xeu", "yeu", "xzv", "yzv",
+                                                   "xec", "yec", "xzc", "yzc")],
+                            _d(un), _d(vn), _d(u), _d(v), _d(tn), _d(t))
+
+    def eq_state(self, nx, ny, uref, densref, tmax, tref, rconst, p, t, den):
+        self.L.eqstate_(self._ir(nx), self._ir(ny), *[self._dr(x) for x in (uref, densref, tmax, tref, rconst)],
+                        _d(p), _d(t), _d(den))
+
+    def filter(self, reg, th, ncomp, fp, q):
+        self.L.filter_(self._ir(reg.nx), self._ir(reg.ny), self._ir(ncomp), _i(reg.nreg), _i(reg.nRegBrd),
+                       _i(reg.nRegType), _i(reg.nMomBdTp), _i(th.nTRgType), self._dr(fp), _d(q))
 
     def dmax_norm(self, nx, ny, u):
         return self.L.dmaxnorm_(self._ir(nx), self._ir(ny), _d(u))

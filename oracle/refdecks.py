@@ -41,10 +41,11 @@ def uniform_grid(nx: int, ny: int, lx: float = 1.0, ly: float = 1.0):
     return x, y
 
 
-def write_fluid_table(path: str, name: str, rho: float, mu: float) -> None:
-    """Two identical rows bracketing tref, so the interpolation is exact."""
+def write_fluid_table(path: str, name: str, rho: float, mu: float, gas_constant: float | None = None) -> None:
+    """Two identical rows bracketing tref, so the interpolation is exact; the
+    gas constant (parse.f:2134-2143) is given on the 'fluid' line."""
     with open(path, "w") as f:
-        f.write(f"fluid {name}\n")
+        f.write(f"fluid {name}" + (f" gasconstant {gas_constant!r}" if gas_constant is not None else "") + "\n")
         for t in (200.0, 400.0):
             f.write(f"{t:.1f} {rho!r} 1.0 {mu!r} 0.025 0.7\n")
         f.write("end\n")
@@ -72,6 +73,8 @@ class Deck:
     i_borders: tuple = ()
     j_borders: tuple = ()
     bc_lines: list = field(default_factory=lambda: ["wall 1 1 north tangent_vel 1.0"])
+    extra_input: list = field(default_factory=list)   # more input_parameters lines
+    gas_constant: float | None = None
 
     def input_text(self) -> str:
         grid = f"grid_file {self.grid_file}" + (" cartesian_grid" if self.cartesian else "")
@@ -94,6 +97,7 @@ class Deck:
             kw.append(f"sor_tolerance {self.sor_tolerance!r}")
         if self.sor_relaxation is not None:
             kw.append(f"sor_relaxation {self.sor_relaxation!r}")
+        kw.extend(self.extra_input)
         if self.restart_in:
             kw.append(f"restart {self.restart_in}")
         kw.append(f"write_restart {self.restart_out}")
@@ -115,7 +119,7 @@ class Deck:
     def write(self, rundir: str, x: np.ndarray, y: np.ndarray) -> None:
         os.makedirs(rundir, exist_ok=True)
         write_grid(os.path.join(rundir, self.grid_file), x, y)
-        write_fluid_table(os.path.join(rundir, "fluid.dat"), "fl01", self.rho, self.mu)
+        write_fluid_table(os.path.join(rundir, "fluid.dat"), "fl01", self.rho, self.mu, self.gas_constant)
         with open(os.path.join(rundir, "input.dat"), "w") as f:
             f.write(self.input_text())
 

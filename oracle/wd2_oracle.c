@@ -1398,3 +1398,265 @@ int wd2o_step(const wd2o_regions *r, const wd2o_metrics *m, int lCartesGrid,
     out_dif[2] = wd2o_diff_max_norm(nx, ny, ld, vn, v);
     return 0;
 }
+
+/* ------------------------------------------------------------------ thermal */
+
+/* wolfd2.h:24-31 */
+enum { RT_NOSRCE = 0, RT_HEATGN = 1, RT_TEMPER = 2 };
+enum { BT_INTERN = 0, BT_TEMPER = 1, BT_HTFLUX = 2 };
+
+static inline int trtype(const wd2o_thermal *th, const wd2o_regions *r, int ir, int jr) {
+    return th->nTRgType[(ir - 1) + r->mgri * (jr - 1)];
+}
+static inline int tbtp(const wd2o_thermal *th, const wd2o_regions *r, int ir, int jr, int side) {
+    return th->nTemBdTp[(ir - 1) + r->mgri * ((jr - 1) + r->mgrj * (side - 1))];
+}
+
+/* bound_cond.f:TempBoundCond 1030-1190 */
+int wd2o_temp_bound_cond(const wd2o_regions *r, const wd2o_thermal *th, double *t)
+{
+    int ld = r->ld, ir, jr, i, j;
+    const double dTwo = 2.0;
+    for (jr = 1; jr <= r->nReg[1]; jr++)
+        for (ir = 1; ir <= r->nReg[0]; ir++) {
+            int iW = brd(r, ir, jr, WEST), iE = brd(r, ir, jr, EAST);
+            int jS = brd(r, ir, jr, SOUTH), jN = brd(r, ir, jr, NORTH);
+            if (trtype(th, r, ir, jr) == RT_TEMPER) {
+                const double tv = tab2(r, th->dTRgVal, ir, jr);
+                for (j = jS + 1; j <= jN; j++)
+                    for (i = iW + 1; i <= iE; i++) F(t, i, j) = tv;
+                for (j = jS + 1; j <= jN; j++) F(t, iW + 1, j) = dTwo * bcv(r, ir, jr, WEST, VT) - F(t, iW, j);
+                for (j = jS + 1; j <= jN; j++) F(t, iE, j) = dTwo * bcv(r, ir, jr, EAST, VT) - F(t, iE + 1, j);
+                for (i = iW + 1; i <= iE; i++) F(t, i, jS + 1) = dTwo * bcv(r, ir, jr, SOUTH, VT) - F(t, i, jS);
+                for (i = iW + 1; i <= iE; i++) F(t, i, jN) = dTwo * bcv(r, ir, jr, NORTH, VT) - F(t, i, jN + 1);
+                continue;
+            }
+            switch (tbtp(th, r, ir, jr, WEST)) {
+            case BT_INTERN: break;
+            case BT_TEMPER:
+                for (j = jS + 1; j <= jN; j++) F(t, iW, j) = dTwo * bcv(r, ir, jr, WEST, VT) - F(t, iW + 1, j);
+                break;
+            case BT_HTFLUX:
+                for (j = jS + 1; j <= jN; j++) F(t, iW, j) = bcv(r, ir, jr, WEST, VT) + F(t, iW + 1, j);
+                break;
+            default:
+                fprintf(stderr, "Wrong nTemBdTp W flag in region %d,%d\n", ir, jr);
+                return ERR_FLAG;
+            }
+            switch (tbtp(th, r, ir, jr, EAST)) {
+            case BT_INTERN: break;
+            case BT_TEMPER:
+                for (j = jS + 1; j <= jN; j++) F(t, iE + 1, j) = dTwo * bcv(r, ir, jr, EAST, VT) - F(t, iE, j);
+                break;
+            case BT_HTFLUX:
+                for (j = jS + 1; j <= jN; j++) F(t, iE + 1, j) = bcv(r, ir, jr, EAST, VT) + F(t, iE, j);
+                break;
+            default:
+                fprintf(stderr, "Wrong nTemBdTp E flag in region %d,%d\n", ir, jr);
+                return ERR_FLAG;
+            }
+            switch (tbtp(th, r, ir, jr, SOUTH)) {
+            case BT_INTERN: break;
+            case BT_TEMPER:
+                for (i = iW + 1; i <= iE; i++) F(t, i, jS) = dTwo * bcv(r, ir, jr, SOUTH, VT) - F(t, i, jS + 1);
+                break;
+            case BT_HTFLUX:
+                for (i = iW + 1; i <= iE; i++) F(t, i, jS) = bcv(r, ir, jr, SOUTH, VT) + F(t, i, jS + 1);
+                break;
+            default:
+                fprintf(stderr, "Wrong nTemBdTp S flag in region %d,%d\n", ir, jr);
+                return ERR_FLAG;
+            }
+            switch (tbtp(th, r, ir, jr, NORTH)) {
+            case BT_INTERN: break;
+            case BT_TEMPER:
+                for (i = iW + 1; i <= iE; i++) F(t, i, jN + 1) = dTwo * bcv(r, ir, jr, NORTH, VT) - F(t, i, jN);
+                break;
+            case BT_HTFLUX:
+                for (i = iW + 1; i <= iE; i++) F(t, i, jN + 1) = bcv(r, ir, jr, NORTH, VT) + F(t, i, jN);
+                break;
+            default:
+                fprintf(stderr, "Wrong nTemBdTp N flag in region %d,%d\n", ir, jr);
+                return ERR_FLAG;
+            }
+        }
+    return 0;
+}
+
+/* thermal.f:ThermEnergy 24-274.  Both splits hand AltTridLU one chain of
+ * (nx-1)(ny-1) rows ordered i fastest (ind = (j-2)(nx-1)+i-1); the second
+ * split's rows carry the j-direction coefficients in that same i-fastest
+ * order (thermal.f:210-245), which this restatement keeps. */
+int wd2o_therm_energy(const wd2o_regions *r, const wd2o_thermal *th, double dk, double pe,
+                      const wd2o_metrics *m, const double *un, const double *vn,
+                      const double *u, const double *v, const double *tn, double *t)
+{
+    const int nx = r->nx, ny = r->ny, ld = r->ld;
+    const size_t nf = (size_t)ld * (ny + 2 > 0 ? (size_t)(ny + 2) : 1);
+    const int n = (nx - 1) * (ny - 1);
+    const double dZero = 0.0, dOne = 1.0, dTwo = 2.0, dHalf = 0.5;
+    const double *rau = m->rau, *rbu = m->rbu, *rbv = m->rbv, *rgv = m->rgv, *djc = m->djc;
+    int ir, jr, i, j, st;
+    double *cu1 = zalloc(nf), *cun = zalloc(nf), *cv1 = zalloc(nf), *cvn = zalloc(nf), *s = zalloc(nf);
+    double *a = zalloc(3 * (size_t)n), *b = zalloc((size_t)n);
+    const double pe1 = dOne / pe;
+#define A(k, ind) a[3 * ((ind) - 1) + (k) - 1]
+#define B(ind) b[(ind) - 1]
+    st = wd2o_temp_bound_cond(r, th, t);
+    if (st) goto out;
+    wd2o_conv_coef(nx, ny, ld, 6, 0, m->xzc, m->xec, m->yzc, m->yec, u, v, cu1, cv1);
+    wd2o_conv_coef(nx, ny, ld, 3, 0, m->xzv, m->xeu, m->yzv, m->yeu, un, vn, cun, cvn);
+    /* heat generation term */
+    for (jr = 1; jr <= r->nReg[1]; jr++)
+        for (ir = 1; ir <= r->nReg[0]; ir++) {
+            int iW = brd(r, ir, jr, WEST), iE = brd(r, ir, jr, EAST);
+            int jS = brd(r, ir, jr, SOUTH), jN = brd(r, ir, jr, NORTH);
+            const double sv = trtype(th, r, ir, jr) == RT_HEATGN ? tab2(r, th->dHGSTval, ir, jr) : dZero;
+            for (j = jS + 1; j <= jN; j++)
+                for (i = iW + 1; i <= iE; i++) F(s, i, j) = sv;
+        }
+    /* first split step */
+    for (j = 2; j <= ny; j++)
+        for (i = 2; i <= nx; i++) {
+            const int ind = (j - 2) * (nx - 1) + i - 1;
+            const double rkj = dk * F(djc, i, j) * dHalf;
+            if (F(cu1, i, j) >= dZero) {
+                A(1, ind) = rkj * (-F(cu1, i - 1, j) - pe1 * F(rau, i - 1, j));
+                A(2, ind) = dOne + rkj * (F(cu1, i, j) + pe1 * (F(rau, i, j) + F(rau, i - 1, j)));
+                A(3, ind) = rkj * (-pe1 * F(rau, i, j));
+            } else {
+                A(1, ind) = rkj * (-pe1 * F(rau, i - 1, j));
+                A(2, ind) = dOne + rkj * (-F(cu1, i, j) + pe1 * (F(rau, i, j) + F(rau, i - 1, j)));
+                A(3, ind) = rkj * (F(cu1, i + 1, j) - pe1 * F(rau, i, j));
+            }
+            const double c = -F(cvn, i, j - 1) * F(tn, i, j - 1) - F(cun, i - 1, j) * F(tn, i - 1, j)
+                             + (F(cun, i, j) - F(cun, i - 1, j) + F(cvn, i, j) - F(cvn, i, j - 1)) * F(tn, i, j)
+                             + F(cun, i, j) * F(tn, i + 1, j) + F(cvn, i, j) * F(tn, i, j + 1);
+            const double d = F(rau, i, j) * (F(tn, i + 1, j) - F(tn, i, j))
+                             - F(rau, i - 1, j) * (F(tn, i, j) - F(tn, i - 1, j))
+                             + F(rbu, i, j) * (F(tn, i + 1, j + 1) + F(tn, i, j + 1) - F(tn, i + 1, j - 1) - F(tn, i, j - 1))
+                             - F(rbu, i - 1, j) * (F(tn, i, j + 1) + F(tn, i - 1, j + 1) - F(tn, i, j - 1) - F(tn, i - 1, j - 1))
+                             + F(rbv, i, j) * (F(tn, i + 1, j + 1) + F(tn, i + 1, j) - F(tn, i - 1, j + 1) - F(tn, i - 1, j))
+                             - F(rbv, i, j - 1) * (F(tn, i + 1, j) + F(tn, i + 1, j - 1) - F(tn, i - 1, j) - F(tn, i - 1, j - 1))
+                             + F(rgv, i, j) * (F(tn, i, j + 1) - F(tn, i, j))
+                             - F(rgv, i, j - 1) * (F(tn, i, j) - F(tn, i, j - 1));
+            B(ind) = dk * F(s, i, j) * dHalf + dTwo * rkj * (-c + pe1 * d);
+        }
+    wd2o_alt_trid_lu(n, a, b);
+    /* second split step: the right-hand side is the first split's solution */
+    for (j = 2; j <= ny; j++)
+        for (i = 2; i <= nx; i++) {
+            const int ind = (j - 2) * (nx - 1) + i - 1;
+            const double rkj = dk * F(djc, i, j) * dHalf;
+            if (F(cv1, i, j) >= dZero) {
+                A(1, ind) = rkj * (-F(cv1, i, j - 1) - pe1 * F(rgv, i, j - 1));
+                A(2, ind) = dOne + rkj * (F(cv1, i, j) + pe1 * (F(rgv, i, j) + F(rgv, i, j - 1)));
+                A(3, ind) = rkj * (-pe1 * F(rgv, i, j));
+            } else {
+                A(1, ind) = rkj * (-pe1 * F(rgv, i, j - 1));
+                A(2, ind) = dOne + rkj * (-F(cv1, i, j) + pe1 * (F(rgv, i, j) + F(rgv, i, j - 1)));
+                A(3, ind) = rkj * (F(cv1, i, j + 1) - pe1 * F(rgv, i, j));
+            }
+        }
+    /* interior of fixed-temperature regions */
+    for (jr = 1; jr <= r->nReg[1]; jr++)
+        for (ir = 1; ir <= r->nReg[0]; ir++) {
+            if (trtype(th, r, ir, jr) != RT_TEMPER) continue;
+            int iW = brd(r, ir, jr, WEST), iE = brd(r, ir, jr, EAST);
+            int jS = brd(r, ir, jr, SOUTH), jN = brd(r, ir, jr, NORTH);
+            for (j = jS + 1; j <= jN; j++)
+                for (i = iW + 1; i <= iE; i++) {
+                    const int ind = (j - 2) * (nx - 1) + i - 1;
+                    A(1, ind) = dZero;
+                    A(2, ind) = dOne;
+                    A(3, ind) = dZero;
+                    B(ind) = dZero;
+                }
+        }
+    wd2o_alt_trid_lu(n, a, b);
+    for (j = 2; j <= ny; j++)
+        for (i = 2; i <= nx; i++) {
+            const int ind = (j - 2) * (nx - 1) + i - 1;
+            F(t, i, j) = F(t, i, j) + B(ind);
+        }
+#undef A
+#undef B
+out:
+    free(cu1); free(cun); free(cv1); free(cvn); free(s); free(a); free(b);
+    return st;
+}
+
+/* thermal.f:EqState 280-328 */
+void wd2o_eq_state(int nx, int ny, int ld, double uref, double densref, double tmax, double tref,
+                   double rconst, const double *p, const double *t, double *den)
+{
+    int i, j;
+    const double dZero = 0.0, dOne = 1.0;
+    const double pref = densref * rconst * tref;
+    for (j = 2; j <= ny; j++)
+        for (i = 2; i <= nx; i++) {
+            const double c1 = F(p, i, j) * densref * (uref * uref) + pref;
+            const double c2 = densref * rconst * (F(t, i, j) * (tmax - tref) + tref);
+            F(den, i, j) = c1 / c2 - dOne;
+            if (fabs(F(den, i, j)) < 1.e-10) F(den, i, j) = dZero;
+        }
+}
+
+/* utility.f:Filter 31-247 */
+int wd2o_filter(const wd2o_regions *r, const wd2o_thermal *th, int ncomp, double fp, double *qu)
+{
+    const int nx = r->nx, ny = r->ny, ld = r->ld;
+    const double dFour = 4.0;
+    int ir, jr, i, j;
+    if (ncomp != VU && ncomp != VV && ncomp != VT) {
+        fprintf(stderr, "Wrong ncomp flag passed to Filter\n");
+        return ERR_FLAG;
+    }
+    double *qh = zalloc((size_t)ld * (ny + 2));
+#define FILT(i, j) ((F(qu, i, j - 1) + F(qu, i - 1, j) + F(qu, i, j + 1) + F(qu, i + 1, j) + fp * F(qu, i, j)) / (fp + dFour))
+    for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) F(qh, i, j) = F(qu, i, j);
+    for (jr = 1; jr <= r->nReg[1]; jr++)
+        for (ir = 1; ir <= r->nReg[0]; ir++) {
+            int iW = brd(r, ir, jr, WEST), iE = brd(r, ir, jr, EAST);
+            int jS = brd(r, ir, jr, SOUTH), jN = brd(r, ir, jr, NORTH);
+            if (ncomp == VU) {
+                if (rtype(r, ir, jr) == RM_BLOCKG) continue;
+                for (j = jS + 1; j <= jN; j++)
+                    for (i = iW + 1; i <= iE - 1; i++) F(qh, i, j) = FILT(i, j);
+                int tw = btp(r, ir, jr, WEST), te = btp(r, ir, jr, EAST);
+                if (tw == BM_OUTLT1)
+                    for (j = jS + 1; j <= jN; j++) F(qh, iW, j) = FILT(iW, j);
+                else if (tw < BM_INTERN || tw > BM_OUTLT2)
+                    fprintf(stderr, "Wrong nBdTypeW flag in region %d,%d\n", ir, jr);
+                if (te == BM_INTERN || te == BM_OUTLT1)
+                    for (j = jS + 1; j <= jN; j++) F(qh, iE, j) = FILT(iE, j);
+                else if (te < BM_INTERN || te > BM_OUTLT2)
+                    fprintf(stderr, "Wrong nBdTypeE flag in region %d,%d\n", ir, jr);
+            } else if (ncomp == VV) {
+                if (rtype(r, ir, jr) == RM_BLOCKG) continue;
+                for (j = jS + 1; j <= jN - 1; j++)
+                    for (i = iW + 1; i <= iE; i++) F(qh, i, j) = FILT(i, j);
+                int ts = btp(r, ir, jr, SOUTH), tn = btp(r, ir, jr, NORTH);
+                if (ts == BM_OUTLT1)
+                    for (i = iW + 1; i <= iE; i++) F(qh, i, jS) = FILT(i, jS);
+                else if (ts < BM_INTERN || ts > BM_OUTLT2)
+                    fprintf(stderr, "Wrong nBdTypeS flag in region %d,%d\n", ir, jr);
+                if (tn == BM_INTERN || tn == BM_OUTLT1)
+                    for (i = iW + 1; i <= iE; i++) F(qh, i, jN) = FILT(i, jN);
+                else if (tn < BM_INTERN || tn > BM_OUTLT2)
+                    fprintf(stderr, "Wrong nBdTypeN flag in region %d,%d\n", ir, jr);
+            } else {
+                /* utility.f:209 compares the thermal region type with BT_TEMPER
+                 * (= RT_HEATGN): heat-generating regions are the ones skipped */
+                if (trtype(th, r, ir, jr) == BT_TEMPER) continue;
+                for (j = jS + 1; j <= jN; j++)
+                    for (i = iW + 1; i <= iE; i++) F(qh, i, j) = FILT(i, j);
+            }
+        }
+#undef FILT
+    for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) F(qu, i, j) = F(qh, i, j);
+    free(qh);
+    return 0;
+}

@@ -142,6 +142,36 @@ int wd2o_cold_start(const wd2o_regions *r, const wd2o_metrics *m, int lCartesGri
                     int nPpeSolver, int msorit, double sortol, double sorrel, double dk,
                     double *u, double *v, double *p, int *nSorConv);
 
+/* Thermal region tables (bound_cond.f, parse.f): nTRgType(mgri,mgrj) region
+ * types RT_*, nTemBdTp(mgri,mgrj,4) temperature boundary types BT_*,
+ * dTRgVal(mgri,mgrj) fixed region temperatures, dHGSTval(mgri,mgrj) heat
+ * generation values.  Boundary temperatures are dBCVal(.,.,.,4) of the
+ * region tables. */
+typedef struct {
+    const int *nTRgType;
+    const int *nTemBdTp;
+    const double *dTRgVal;
+    const double *dHGSTval;
+} wd2o_thermal;
+
+/* bound_cond.f:TempBoundCond 1030-1190 */
+int wd2o_temp_bound_cond(const wd2o_regions *r, const wd2o_thermal *th, double *t);
+
+/* thermal.f:ThermEnergy 24-274: two ADI splits of the energy equation on
+ * t (in place); un, vn, tn are the time-level-n fields, u, v the current
+ * velocities. */
+int wd2o_therm_energy(const wd2o_regions *r, const wd2o_thermal *th, double dk, double pe,
+                      const wd2o_metrics *m, const double *un, const double *vn,
+                      const double *u, const double *v, const double *tn, double *t);
+
+/* thermal.f:EqState 280-328 */
+void wd2o_eq_state(int nx, int ny, int ld, double uref, double densref, double tmax, double tref,
+                   double rconst, const double *p, const double *t, double *den);
+
+/* utility.f:Filter 31-247 (ncomp 1 u, 2 v, 4 t); returns ERR_FLAG for a bad
+ * ncomp, where the reference STOPs. */
+int wd2o_filter(const wd2o_regions *r, const wd2o_thermal *th, int ncomp, double fp, double *qu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,39 @@ RUNS = {
 }
 
 
+# Thermal runs (thermal_energy, buoyancy, filters, momentum/energy
+# iterations): the drop-in link's parity cases.  Kept out of run_* because
+# the device-resident context and the oracle's step are the cold-flow path.
+THERMAL_RUNS = {
+    "therm_cavity_32": (33, 33, 1.0, 1.0, dict(
+        nts=60, dt=1e-3, mu=0.01, ppe_solver="rb_sor", nreg=(2, 1), i_borders=(17,), gas_constant=287.0,
+        extra_input=["thermal_energy", "buoyancy", "max_me_iter 3", "me_tolerance 1e-9", "filter_u 200.0",
+                     "filter_v 200.0", "filter_t 50.0", "ref_temperature 300.0", "max_ref_temperature 310.0"],
+        bc_lines=["wall 1 1 west temper 310.0", "wall 2 1 east temper 300.0", "wall 1 1 north tangent_vel 0.5",
+                  "heat_generation_region 2 1 40.0"])),
+}
+
+
+def thermal_run_fixture(name: str, out_dir: str) -> None:
+    nx, ny, lx, ly, kw = THERMAL_RUNS[name]
+    x, y = case.uniform_nodes(nx, ny, lx, ly)
+    with tempfile.TemporaryDirectory(prefix=f"wd2fx_{name}_") as rd:
+        d = decks.Deck(**kw)
+        d.write(rd, x, y)
+        stdout = run_reference_program(rd)
+        rst = decks.read_restart(os.path.join(rd, "out.rst"))
+    lines = [ln for ln in stdout.splitlines() if decks._DIFF_RE.match(ln)]
+    sl = np.s_[: ny + 2, : nx + 2]
+    np.savez_compressed(
+        os.path.join(out_dir, f"thermrun_{name}.npz"),
+        x=x, y=y, nts=kw["nts"], dt=kw["dt"], mu=kw["mu"], rho=1.0, gas_constant=kw["gas_constant"],
+        deck=d.input_text(),
+        k=rst["k"], dtime=rst["dtime"], diff_lines="\n".join(lines),
+        p=rst["p"][sl], u=rst["u"][sl], v=rst["v"][sl], t=rst["t"][sl],
+    )
+    print(f"{name}: steps={rst['k']} rows={len(lines)} last: {lines[-1].strip()}")
+
+
 def run_fixture(name: str, out_dir: str) -> None:
     nx, ny, lx, ly, kw = RUNS[name]
     if lx is None:
@@ -172,15 +205,79 @@ def kernel_fixtures(out_dir: str) -> None:
         print(f"kernels_{tag}: {len(out)} arrays; naux n = {out['naux1_n']}, {out['naux30_n']}")
 
 
+def thermal_layouts():
+    """The two kernel grids with thermal tables: heat generation, a fixed-
+    temperature region, fixed-temperature and heat-flux walls."""
+    nx, ny, x, y, reg = kernel_case()
+    th = case.make_thermal(reg, heat=[(2, 1, 0.5)], fixed=[(1, 2, 0.3)],
+                           walls=[(1, 1, "west", "temper", 1.0), (2, 2, "north", "flux", 0.05)])
+    yield "skew", nx, ny, x, y, reg, th
+    nx, ny, x, y, reg = blockage_case()
+    th = case.make_thermal(reg, heat=[(3, 3, 0.8)], fixed=[(1, 3, -0.2)],
+                           walls=[(2, 2, sd, "flux", 0.0) for sd in ("west", "east", "south", "north")]
+                           + [(1, 1, "south", "temper", 1.0), (3, 1, "east", "temper", 0.0)])
+    yield "block", nx, ny, x, y, reg, th
+
+
+def thermal_fixtures(out_dir: str) -> None:
+    """TempBoundCond, ThermEnergy, EqState and Filter (u, v, t) of the
+    reference on seeded fields, as thermal_<layout>.npz."""
+    for tag, nx, ny, x, y, reg, th in thermal_layouts():
+        R = Reference.for_grid(("thermal", tag, nx, ny))
+        gx, gy = R.field(), R.field()
+        gx[1:ny + 1, 1:nx + 1] = x
+        gy[1:ny + 1, 1:nx + 1] = y
+        g = R.grid(nx, ny, gx, gy, {})
+        rng = np.random.default_rng(4242)
+        sl = np.s_[: ny + 2, : nx + 2]
+
+        def rf(scale=1.0):
+            f = R.field()
+            f[sl] = scale * rng.standard_normal((ny + 2, nx + 2))
+            return f
+
+        out = dict(nx=nx, ny=ny, x=x, y=y)
+        for nm in ("nreg", "nRegBrd", "nRegType", "nMomBdTp", "dBCVal", "dPRporos", "dPRporc1", "dPRporc2"):
+            out[f"reg_{nm}"] = getattr(reg, nm)
+        for nm in ("nTRgType", "nTemBdTp", "dTRgVal", "dHGSTval"):
+            out[f"th_{nm}"] = getattr(th, nm)
+        t = rf()
+        out["tbc_in"] = t[sl].copy()
+        R.temp_bound_cond(reg, th, t)
+        out["tbc_out"] = t[sl].copy()
+        un, vn, u, v, tn, t = rf(0.3), rf(0.3), rf(0.3), rf(0.3), rf(), rf()
+        for nm, a in (("un", un), ("vn", vn), ("u", u), ("v", v), ("tn", tn), ("t", t)):
+            out[f"te_{nm}"] = a[sl].copy()
+        out["te_dk"], out["te_pe"] = 0.01, 70.0
+        R.therm_energy(reg, th, g, 0.01, 70.0, un, vn, u, v, tn, t)
+        out["te_out"] = t[sl].copy()
+        p, t, den = rf(0.1), rf(0.5), R.field()
+        out["eqs_p"], out["eqs_t"] = p[sl].copy(), t[sl].copy()
+        out["eqs_args"] = np.array([1.5, 1.2, 400.0, 300.0, 287.0])
+        R.eq_state(nx, ny, *out["eqs_args"], p, t, den)
+        out["eqs_out"] = den[sl].copy()
+        for comp, nm in ((1, "u"), (2, "v"), (4, "t")):
+            q = rf()
+            out[f"filt_{nm}_in"] = q[sl].copy()
+            R.filter(reg, th, comp, 20.0, q)
+            out[f"filt_{nm}_out"] = q[sl].copy()
+        np.savez_compressed(os.path.join(out_dir, f"thermal_{tag}.npz"), **out)
+        print(f"thermal_{tag}: {len(out)} arrays")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*")
     ap.add_argument("--out", default=HERE)
     args = ap.parse_args()
-    names = args.only or (["kernels"] + list(RUNS))
+    names = args.only or (["kernels", "thermal"] + list(RUNS) + list(THERMAL_RUNS))
     for nm in names:
         if nm == "kernels":
             kernel_fixtures(args.out)
+        elif nm == "thermal":
+            thermal_fixtures(args.out)
+        elif nm in THERMAL_RUNS:
+            thermal_run_fixture(nm, args.out)
         else:
             run_fixture(nm, args.out)
 

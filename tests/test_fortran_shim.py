@@ -23,7 +23,8 @@ CONFIG = """\
 EXPORTS = {"nauxmomentum_": "wd2_naux_momentum", "ppe_": "wd2_ppe", "project_": "wd2_project",
            "velboundcond_": "wd2_vel_bound_cond", "presboundcond_": "wd2_pres_bound_cond",
            "veloutflowbcs_": "wd2_vel_outflow_bcs", "dmaxnorm_": "wd2_dmax_norm",
-           "diffmaxnorm_": "wd2_diff_max_norm"}
+           "diffmaxnorm_": "wd2_diff_max_norm", "tempboundcond_": "wd2_temp_bound_cond",
+           "thermenergy_": "wd2_therm_energy", "eqstate_": "wd2_eq_state", "filter_": "wd2_filter"}
 
 
 @pytest.mark.skipif(FC is None, reason="no Fortran compiler (amdflang) in this image")

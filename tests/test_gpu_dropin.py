@@ -1,8 +1,9 @@
 """The drop-in boundary end to end: the reference's own driver (main.f and
 every routine it calls) linked against libwolfd2_amd through the Fortran shim
 wolfd2_amd/fortran/wd2_shim.f, so that nAuxMomentum, Ppe, Project,
-VelBoundCond, PresBoundCond, VelOutflowBCs, DMaxNorm and DiffMaxNorm run on
-the GPU (oracle/Makefile, target dropin).  It reads the same input deck, grid
+VelBoundCond, PresBoundCond, VelOutflowBCs, DMaxNorm, DiffMaxNorm and the
+thermal routines TempBoundCond, ThermEnergy, EqState and Filter run on the GPU
+(oracle/Makefile, target dropin).  It reads the same input deck, grid
 and fluid table as the reference program that wrote the golden fixture, and
 its restart file and PrintDiff table must equal the fixture's bit for bit.
 """
@@ -19,7 +20,7 @@ import refdecks
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "oracle", "_ref", "dropin", "wolfd2")
 SHIM_SYMS = ["nauxmomentum_", "ppe_", "project_", "velboundcond_", "presboundcond_", "veloutflowbcs_",
-             "dmaxnorm_", "diffmaxnorm_"]
+             "dmaxnorm_", "diffmaxnorm_", "tempboundcond_", "thermenergy_", "eqstate_", "filter_"]
 
 needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in executable not built (needs the reference "
                                                                 "sources: make -C oracle dropin)")
@@ -37,7 +38,8 @@ def test_dropin_binds_the_shim():
     base = {defs[s] - sdefs[s] for s in SHIM_SYMS}
     assert len(base) == 1, "entry points do not all come from the shim object"
     for c in ("wd2_naux_momentum", "wd2_ppe", "wd2_project", "wd2_vel_bound_cond", "wd2_pres_bound_cond",
-              "wd2_vel_outflow_bcs", "wd2_dmax_norm", "wd2_diff_max_norm"):
+              "wd2_vel_outflow_bcs", "wd2_dmax_norm", "wd2_diff_max_norm", "wd2_temp_bound_cond",
+              "wd2_therm_energy", "wd2_eq_state", "wd2_filter"):
         assert f" U {c}\n" in nm, c
 
 
@@ -67,4 +69,30 @@ def test_reference_driver_on_gpu_bitwise(name):
     assert np.array_equal(tab, ref), "PrintDiff table differs"
     assert rst["k"] == int(fx["k"]) and rst["dtime"] == float(fx["dtime"])
     for f in ("u", "v", "p"):
+        assert np.array_equal(rst[f], fx[f]), f
+
+
+@pytest.mark.gpu
+@needs_exe
+def test_reference_driver_thermal_on_gpu_bitwise():
+    """A heated cavity with the energy equation, buoyancy (EqState feeding the
+    y-momentum source), velocity and temperature filters, a heat-generating
+    region and up to three momentum/energy iterations per step: the PrintDiff
+    lines (four norms) and the restart fields u, v, p, t equal the reference
+    program's."""
+    fx = helpers.load("thermrun_therm_cavity_32.npz")
+    print(f"drop-in thermal run steps={int(fx['nts'])}", flush=True)
+    with tempfile.TemporaryDirectory(prefix="wd2_dropin_") as rd:
+        refdecks.write_grid(os.path.join(rd, "grid.xyz"), fx["x"], fx["y"])
+        refdecks.write_fluid_table(os.path.join(rd, "fluid.dat"), "fl01", float(fx["rho"]), float(fx["mu"]),
+                                   float(fx["gas_constant"]))
+        with open(os.path.join(rd, "input.dat"), "w") as f:
+            f.write(str(fx["deck"]))
+        r = subprocess.run([EXE], cwd=rd, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        rst = refdecks.read_restart(os.path.join(rd, "out.rst"))
+    lines = [ln for ln in r.stdout.splitlines() if refdecks._DIFF_RE.match(ln)]
+    assert lines == str(fx["diff_lines"]).split("\n"), "PrintDiff lines differ"
+    assert rst["k"] == int(fx["k"]) and rst["dtime"] == float(fx["dtime"])
+    for f in ("u", "v", "p", "t"):
         assert np.array_equal(rst[f], fx[f]), f

@@ -3,7 +3,8 @@
 Each function mirrors one subroutine of the reference with its argument
 meaning (momentum.f:nAuxMomentum/XMomentum/YMomentum/AltTridLU,
 pressure.f:Ppe/Divergence, utility.f:Project/DMaxNorm/DiffMaxNorm,
-bound_cond.f:VelBoundCond/PresBoundCond/VelOutflowBCs): fields are numpy
+bound_cond.f:VelBoundCond/PresBoundCond/VelOutflowBCs/TempBoundCond,
+thermal.f:ThermEnergy/EqState, utility.f:Filter): fields are numpy
 float64 arrays of shape (mny+1, mnx+1) indexed [j, i] (the reference's
 (0:mnx,0:mny) column-major arrays), updated in place; region tables and
 metrics come from :mod:`wolfd2_amd.case`.  Every call runs on the GPU through
@@ -24,6 +25,7 @@ from ._lib import Params, Wd2Error, check, dp, fp, ip, lib
 
 __all__ = ["vel_bound_cond", "pres_bound_cond", "vel_outflow_bcs", "project", "divergence", "ppe",
            "x_momentum", "y_momentum", "naux_momentum", "alt_trid_lu", "dmax_norm", "diff_max_norm",
+           "temp_bound_cond", "therm_energy", "eq_state", "filter",
            "Solver", "Wd2Error", "make_params"]
 
 
@@ -50,6 +52,32 @@ def project(reg: case.Regions, g: case.Grid, dk: float, p, u, v) -> None:
     check(lib().wd2_project(reg.nx, reg.ny, g.ld, reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
                             ip(reg.nRegType), ip(reg.nMomBdTp), C.c_double(dk), dp(g.dju), dp(g.djv), dp(g.yeu),
                             dp(g.xzv), dp(g.yzu), dp(g.xev), dp(p), dp(u), dp(v)))
+
+
+def temp_bound_cond(reg: case.Regions, th: case.Thermal, t: np.ndarray) -> None:
+    check(lib().wd2_temp_bound_cond(reg.nx, reg.ny, _ld(t), reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                                    ip(th.nTRgType), ip(th.nTemBdTp), fp(th.dTRgVal), fp(reg.dBCVal), dp(t)))
+
+
+def therm_energy(reg: case.Regions, th: case.Thermal, g: case.Grid, dk: float, pe: float, un, vn, u, v, tn,
+                 t) -> None:
+    check(lib().wd2_therm_energy(reg.nx, reg.ny, g.ld, reg.mgri, reg.mgrj, ip(reg.nreg), ip(reg.nRegBrd),
+                                 ip(th.nTRgType), ip(th.nTemBdTp), C.c_double(dk), C.c_double(pe),
+                                 fp(th.dTRgVal), fp(th.dHGSTval), fp(reg.dBCVal),
+                                 *[dp(g.metrics[k]) for k in ("rau", "rbu", "rbv", "rgv", "djc", "xeu", "yeu",
+                                                              "xzv", "yzv", "xec", "yec", "xzc", "yzc")],
+                                 dp(un), dp(vn), dp(u), dp(v), dp(tn), dp(t)))
+
+
+def eq_state(g: case.Grid, uref: float, densref: float, tmax: float, tref: float, rconst: float, p, t,
+             den) -> None:
+    check(lib().wd2_eq_state(g.nx, g.ny, g.ld, *[C.c_double(x) for x in (uref, densref, tmax, tref, rconst)],
+                             dp(p), dp(t), dp(den)))
+
+
+def filter(reg: case.Regions, th: case.Thermal, ncomp: int, fpar: float, q: np.ndarray) -> None:
+    check(lib().wd2_filter(reg.nx, reg.ny, _ld(q), reg.mgri, reg.mgrj, ncomp, ip(reg.nreg), ip(reg.nRegBrd),
+                           ip(reg.nRegType), ip(reg.nMomBdTp), ip(th.nTRgType), C.c_double(fpar), dp(q)))
 
 
 def divergence(g: case.Grid, nloc: int, u, v, div) -> None:

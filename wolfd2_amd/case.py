@@ -298,6 +298,64 @@ def sine_profile(s: np.ndarray, amplitudes) -> np.ndarray:
     return out
 
 
+# wolfd2.h:24-31
+RT_NOSRCE, RT_HEATGN, RT_TEMPER = 0, 1, 2
+BT_INTERN, BT_TEMPER, BT_HTFLUX = 0, 1, 2
+
+
+@dataclass
+class Thermal:
+    """Thermal region tables (parse.f:lReadBCFile, bound_cond.f:HeatGenST):
+    nTRgType(mgri,mgrj) region types RT_*, nTemBdTp(mgri,mgrj,4) temperature
+    boundary types BT_*, dTRgVal(mgri,mgrj) region temperatures, dHGSTval
+    (mgri,mgrj) non-dimensional heat generation.  Boundary temperatures are
+    the regions' dBCVal(.,.,side,VAR_T)."""
+    nTRgType: np.ndarray
+    nTemBdTp: np.ndarray
+    dTRgVal: np.ndarray
+    dHGSTval: np.ndarray
+
+
+def make_thermal(reg: Regions, heat=(), fixed=(), walls=()) -> Thermal:
+    """Thermal tables over ``reg`` (whose dBCVal receives the boundary
+    temperatures): every region RT_NOSRCE with BT_INTERN sides and adiabatic
+    (BT_HTFLUX, zero flux) outer walls (parse.f:InitBCFlags 2350-2375), then
+    ``heat`` items (ireg, jreg, q) with q the non-dimensional source,
+    ``fixed`` items (ireg, jreg, T) for fixed-temperature regions (their sides
+    and the facing sides of neighbours become BT_TEMPER at T, parse.f:
+    1805-1860), and ``walls`` items (ireg, jreg, side, 'temper'|'flux', value)."""
+    mgri, mgrj = reg.mgri, reg.mgrj
+    ni, nj = int(reg.nreg[0]), int(reg.nreg[1])
+    f = dict(order="F")
+    rt = np.zeros((mgri, mgrj), dtype=np.int32, **f)
+    bt = np.zeros((mgri, mgrj, 4), dtype=np.int32, **f)
+    tv = np.zeros((mgri, mgrj), dtype=np.float64, **f)
+    hg = np.zeros((mgri, mgrj), dtype=np.float64, **f)
+    bt[0, :nj, WEST - 1] = BT_HTFLUX
+    bt[ni - 1, :nj, EAST - 1] = BT_HTFLUX
+    bt[:ni, 0, SOUTH - 1] = BT_HTFLUX
+    bt[:ni, nj - 1, NORTH - 1] = BT_HTFLUX
+    for ir, jr, q in heat:
+        rt[ir - 1, jr - 1] = RT_HEATGN
+        hg[ir - 1, jr - 1] = q
+    for ir, jr, temp in fixed:
+        rt[ir - 1, jr - 1] = RT_TEMPER
+        tv[ir - 1, jr - 1] = temp
+        for side in (WEST, EAST, SOUTH, NORTH):
+            bt[ir - 1, jr - 1, side - 1] = BT_TEMPER
+            reg.dBCVal[ir - 1, jr - 1, side - 1, VAR_T - 1] = temp
+        for di, dj, side in ((-1, 0, EAST), (1, 0, WEST), (0, -1, NORTH), (0, 1, SOUTH)):
+            a, b = ir + di, jr + dj
+            if 1 <= a <= ni and 1 <= b <= nj:
+                bt[a - 1, b - 1, side - 1] = BT_TEMPER
+                reg.dBCVal[a - 1, b - 1, side - 1, VAR_T - 1] = temp
+    for ir, jr, side, kind, value in walls:
+        sd = SIDES[side] if isinstance(side, str) else side
+        bt[ir - 1, jr - 1, sd - 1] = BT_TEMPER if kind == "temper" else BT_HTFLUX
+        reg.dBCVal[ir - 1, jr - 1, sd - 1, VAR_T - 1] = value
+    return Thermal(nTRgType=rt, nTemBdTp=bt, dTRgVal=tv, dHGSTval=hg)
+
+
 def make_regions(nx: int, ny: int, nreg=(1, 1), i_borders=(), j_borders=(),
                  walls=(), inlets=(), outlets=(), blockages=(), porous=(),
                  mgri: int = MGRI, mgrj: int = MGRJ) -> Regions:

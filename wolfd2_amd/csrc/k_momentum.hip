@@ -1,7 +1,8 @@
 // k_momentum.hip -- the Douglas-Gunn ADI momentum step (momentum.f:199-1226).
 //
 // One kernel per reference stage, each a single pass over the grid:
-//   ConvCoef (864-981)            face convective coefficients, cases 1,2,4,5
+//   ConvCoef (864-981)            face convective coefficients: cases 1,2,4,5, and
+//                                 3,6 for the thermal energy equation
 //   porous division (295-324)     cj*, c*s, c*n divided by the region porosity
 //   DConvU/V + DDiffU/V           convection and full-tensor diffusion of the
 //   (987-1109)                    star and n levels, four outputs per point
@@ -54,6 +55,18 @@ __global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, do
             cc1[k] = (M(yet, i, j) * (U(i, j + 1) + U(i, j)) - djac * M(xet, i, j) * (V(i + 1, j) + V(i, j))) * dHalf;
         if (i >= 1 && j >= 1)
             cc2[k] = (djac * M(xzi, i, j) * (V(i, j) + V(i, j - 1)) - M(yzi, i, j) * (U(i, j) + U(i - 1, j))) * dHalf;
+    } else if (NCOMP == 3) {   // thermal.f:110-112 (time-level-n face coefficients)
+        if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
+            cc1[k] = (M(yet, i, j) * U(i, j) -
+                      M(xet, i, j) * (V(i + 1, j) + V(i, j) + V(i + 1, j - 1) + V(i, j - 1)) / dFour) * dHalf;
+            cc2[k] = (M(xzi, i, j) * V(i, j) -
+                      M(yzi, i, j) * (U(i, j + 1) + U(i - 1, j + 1) + U(i, j) + U(i - 1, j)) / dFour) * dHalf;
+        }
+    } else if (NCOMP == 6) {   // thermal.f:107-109 (cell-centre coefficients for the upwinded rows)
+        if (i >= 1 && j >= 1) {
+            cc1[k] = (M(yet, i, j) * (U(i, j) + U(i - 1, j)) - M(xet, i, j) * (V(i, j) + V(i, j - 1))) * dHalf;
+            cc2[k] = (M(xzi, i, j) * (V(i, j) + V(i, j - 1)) - M(yzi, i, j) * (U(i, j) + U(i - 1, j))) * dHalf;
+        }
     } else if (NCOMP == 4) {
         if (i <= nx && j >= 1 && j <= ny) {
             cc1[k] = djac * M(yet, i, j) * U(i, j) -
@@ -85,6 +98,8 @@ void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const doubl
     case 2: hipLaunchKernelGGL((k_conv_coef<2, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
     case 4: hipLaunchKernelGGL((k_conv_coef<4, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
     case 5: hipLaunchKernelGGL((k_conv_coef<5, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
+    case 3: hipLaunchKernelGGL((k_conv_coef<3, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
+    case 6: hipLaunchKernelGGL((k_conv_coef<6, T>), g, b, 0, s, nx, ny, ld, djac, xzi, xet, yzi, yet, u, v, cc1, cc2); break;
     default: break;
     }
 }

@@ -61,6 +61,12 @@ struct wd2_ctx {
     // boundary profiles by their index in the (mgri,mgrj,4,3) offset table,
     // and the device copies of that table and of the values
     std::map<int, std::vector<double>> h_prof;
+    // thermal region tables: host copies, device copies and their view
+    std::vector<int> h_trt, h_tbt;
+    std::vector<double> h_trv, h_hg;
+    int *d_thi = nullptr;
+    double *d_thd = nullptr;
+    Thermal TH{};
     int *d_poff = nullptr;
     double *d_pval = nullptr;
     size_t pval_cap = 0;
@@ -73,6 +79,7 @@ struct wd2_ctx {
     // fields
     double *store = nullptr;
     double *u, *v, *p, *un, *vn, *pn, *us, *vs, *dus, *dvs, *d, *dn, *div, *rhs, *beff, *pscr;
+    double *t, *tn, *ts, *qh;      // temperature levels and the filter's copy
     double *pc[5];
     MomTemps T{};
     Chain ch{};
@@ -134,6 +141,8 @@ static void ctx_free(wd2_ctx *c) {
     (void)hipFree(c->d_int);
     (void)hipFree(c->d_dbl);
     (void)hipFree(c->d_poff);
+    (void)hipFree(c->d_thi);
+    (void)hipFree(c->d_thd);
     (void)hipFree(c->d_pval);
     (void)hipFree(c->prow);
     (void)hipFree(c->metric_store);
@@ -161,7 +170,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
                              hipHostMallocDefault));
         HIPCHK(hipEventCreateWithFlags(&c->sev[k], hipEventDisableTiming));
     }
-    const int NFIELDS = 16 + 5 + 13;
+    const int NFIELDS = 16 + 4 + 5 + 13;
     const size_t nchain = (size_t)nx * ny + 64;
     const size_t total = NFIELDS * c->nf + 5 * nchain;
     HIPCHK(hipMalloc(&c->store, total * sizeof(double)));
@@ -169,7 +178,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     double *q = c->store;
     auto take = [&](size_t n) { double *r = q; q += n; return r; };
     double **f[] = {&c->u, &c->v, &c->p, &c->un, &c->vn, &c->pn, &c->us, &c->vs, &c->dus, &c->dvs,
-                    &c->d, &c->dn, &c->div, &c->rhs, &c->beff, &c->pscr};
+                    &c->d, &c->dn, &c->div, &c->rhs, &c->beff, &c->pscr, &c->t, &c->tn, &c->ts, &c->qh};
     for (double **pp : f) *pp = take(c->nf);
     for (int k = 0; k < 5; k++) c->pc[k] = take(c->nf);
     double **t[] = {&c->T.cj1, &c->T.cj2, &c->T.c1s, &c->T.c2s, &c->T.c1n, &c->T.c2n, &c->T.cnvs,
@@ -211,7 +220,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     M.ran = c->met[M_RAN]; M.rbn = c->met[M_RBN]; M.rgn = c->met[M_RGN];
     M.rac = c->met[M_RAC]; M.rbc = c->met[M_RBC]; M.rgc = c->met[M_RGC];
     M.rau = c->met[M_RAU]; M.rbu = c->met[M_RBU]; M.rbv = c->met[M_RBV]; M.rgv = c->met[M_RGV];
-    M.dju = c->met[M_DJU]; M.djv = c->met[M_DJV];
+    M.dju = c->met[M_DJU]; M.djv = c->met[M_DJV]; M.djc = c->met[M_DJC];
     M.xen = c->met[M_XEN]; M.yen = c->met[M_YEN]; M.xzn = c->met[M_XZN]; M.yzn = c->met[M_YZN];
     M.xec = c->met[M_XEC]; M.yec = c->met[M_YEC]; M.xzc = c->met[M_XZC]; M.yzc = c->met[M_YZC];
     M.xeu = c->met[M_XEU]; M.yeu = c->met[M_YEU]; M.xzv = c->met[M_XZV]; M.yzv = c->met[M_YZV];
@@ -257,7 +266,7 @@ static bool valid_btype(int t) { return t >= BM_INTERN && t <= BM_OUTLT2; }
 static int set_regions(wd2_ctx *c, int mgri, int mgrj, const int *nReg, const int *nRegBrd, const int *nRegType,
                        const int *nMomBdTp, const double *dBCVal, const double *por, const double *pc1,
                        const double *pc2) {
-    if (!nReg || !nRegBrd || !nMomBdTp) return fail(WD2_ERR_ARG, "region tables missing");
+    if (!nReg || !nRegBrd) return fail(WD2_ERR_ARG, "region tables missing");
     const int ni = nReg[0], nj = nReg[1];
     if (mgri < 1 || mgrj < 1 || ni < 1 || nj < 1 || ni > mgri || nj > mgrj)
         return fail(WD2_ERR_ARG, "bad region counts %d x %d (tables %d x %d)", ni, nj, mgri, mgrj);
@@ -268,7 +277,8 @@ static int set_regions(wd2_ctx *c, int mgri, int mgrj, const int *nReg, const in
     c->h_brd.assign(nRegBrd, nRegBrd + 4 * n2);
     c->h_rtype.assign(n2, RM_INTERN);
     if (nRegType) c->h_rtype.assign(nRegType, nRegType + n2);
-    c->h_btp.assign(nMomBdTp, nMomBdTp + 4 * n2);
+    c->h_btp.assign(4 * n2, BM_INTERN);   // callers without momentum boundary types: all internal
+    if (nMomBdTp) c->h_btp.assign(nMomBdTp, nMomBdTp + 4 * n2);
     c->h_bcv.assign(16 * n2, 0.0);
     if (dBCVal) c->h_bcv.assign(dBCVal, dBCVal + 16 * n2);
     c->h_por.assign(n2, 1.0);
@@ -342,6 +352,36 @@ static int set_regions(wd2_ctx *c, int mgri, int mgrj, const int *nReg, const in
     return WD2_OK;
 }
 
+// Thermal region tables (any may be null: zeros).  Call after set_regions.
+static int set_thermal(wd2_ctx *c, const int *nTRgType, const int *nTemBdTp, const double *dTRgVal,
+                       const double *dHGSTval) {
+    const size_t n2 = (size_t)c->mgri * c->mgrj;
+    c->h_trt.assign(n2, RT_NOSRCE);
+    c->h_tbt.assign(4 * n2, BT_INTERN);
+    c->h_trv.assign(n2, 0.0);
+    c->h_hg.assign(n2, 0.0);
+    if (nTRgType) c->h_trt.assign(nTRgType, nTRgType + n2);
+    if (nTemBdTp) c->h_tbt.assign(nTemBdTp, nTemBdTp + 4 * n2);
+    if (dTRgVal) c->h_trv.assign(dTRgVal, dTRgVal + n2);
+    if (dHGSTval) c->h_hg.assign(dHGSTval, dHGSTval + n2);
+    if (!c->d_thi) {
+        HIPCHK(hipMalloc(&c->d_thi, 5 * n2 * sizeof(int)));
+        HIPCHK(hipMalloc(&c->d_thd, 2 * n2 * sizeof(double)));
+    }
+    HIPCHK(hipMemcpyAsync(c->d_thi, c->h_trt.data(), n2 * sizeof(int), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_thi + n2, c->h_tbt.data(), 4 * n2 * sizeof(int), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_thd, c->h_trv.data(), n2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_thd + n2, c->h_hg.data(), n2 * sizeof(double), hipMemcpyHostToDevice, c->s));
+    Thermal &TH = c->TH;
+    TH.mgri = c->mgri;
+    TH.mgrj = c->mgrj;
+    TH.trtype = c->d_thi;
+    TH.tbtp = c->d_thi + n2;
+    TH.trval = c->d_thd;
+    TH.hgval = c->d_thd + n2;
+    return WD2_OK;
+}
+
 static int check_err(wd2_ctx *c) {
     int e = 0;
     HIPCHK(hipMemcpyAsync(&e, &c->sc->err, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -349,6 +389,7 @@ static int check_err(wd2_ctx *c) {
     if (e) {
         const int z = 0;
         HIPCHK(hipMemcpy(&c->sc->err, &z, sizeof(int), hipMemcpyHostToDevice));
+        if (e >= 2000) return fail(WD2_ERR_FLAG, "Wrong nTemBdTp flag on side %d", e - 2000);
         return fail(WD2_ERR_FLAG, "bad boundary flag (code %d)", e);
     }
     return WD2_OK;
@@ -918,6 +959,74 @@ static int norm_common(int nx, int ny, int ld, const double *a, const double *b,
 int wd2_dmax_norm(int nx, int ny, int ld, const double *u, double *out) { return norm_common(nx, ny, ld, u, nullptr, out); }
 int wd2_diff_max_norm(int nx, int ny, int ld, const double *un, const double *u, double *out) {
     return norm_common(nx, ny, ld, un, u, out);
+}
+
+// ---------------------------------------------------------------- thermal
+int wd2_temp_bound_cond(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                        const int *nTRgType, const int *nTemBdTp, const double *dTRgVal, const double *dBCVal,
+                        double *t) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nullptr, nullptr, dBCVal, nullptr, nullptr, nullptr)) ||
+        (st = set_thermal(c, nTRgType, nTemBdTp, dTRgVal, nullptr)))
+        return st;
+    if ((st = up(c, t, ld, c->t))) return st;
+    launch_temp_bc(c->R, c->TH, c->ld, c->t, &c->sc->err, c->s);
+    if ((st = check_err(c))) return st;
+    return down(c, c->t, t, ld);
+}
+
+int wd2_therm_energy(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
+                     const int *nTRgType, const int *nTemBdTp, double dk, double pe, const double *dTRgVal,
+                     const double *dHGSTval, const double *dBCVal, const double *rau, const double *rbu,
+                     const double *rbv, const double *rgv, const double *djc, const double *xeu, const double *yeu,
+                     const double *xzv, const double *yzv, const double *xec, const double *yec, const double *xzc,
+                     const double *yzc, const double *un, const double *vn, const double *u, const double *v,
+                     const double *tn, double *t) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nullptr, nullptr, dBCVal, nullptr, nullptr, nullptr)) ||
+        (st = set_thermal(c, nTRgType, nTemBdTp, dTRgVal, dHGSTval)))
+        return st;
+    const int ks[] = {M_RAU, M_RBU, M_RBV, M_RGV, M_DJC, M_XEU, M_YEU, M_XZV, M_YZV, M_XEC, M_YEC, M_XZC, M_YZC};
+    const double *hs[] = {rau, rbu, rbv, rgv, djc, xeu, yeu, xzv, yzv, xec, yec, xzc, yzc};
+    for (int k = 0; k < 13; k++)
+        if ((st = up_metric(ks[k], hs[k], ld))) return st;
+    if ((st = up(c, un, ld, c->un)) || (st = up(c, vn, ld, c->vn)) || (st = up(c, u, ld, c->us)) ||
+        (st = up(c, v, ld, c->vs)) || (st = up(c, tn, ld, c->tn)) || (st = up(c, t, ld, c->t)))
+        return st;
+    const ThermTemps w{c->T.cj1, c->T.cj2, c->T.c1n, c->T.c2n};
+    launch_therm_energy(c->R, c->TH, c->ld, dk, pe, c->M, c->un, c->vn, c->us, c->vs, c->tn, c->t, w, c->ch, c->tw,
+                        &c->sc->err, c->s);
+    if ((st = check_err(c))) return st;
+    return down(c, c->t, t, ld);
+}
+
+int wd2_eq_state(int nx, int ny, int ld, double uref, double densref, double tmax, double tref, double rconst,
+                 const double *p, const double *t, double *den) {
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    // den is written on the interior only: its other entries come back as sent
+    if ((st = up(c, p, ld, c->p)) || (st = up(c, t, ld, c->t)) || (st = up(c, den, ld, c->d))) return st;
+    launch_eq_state(nx, ny, c->ld, uref, densref, tmax, tref, rconst, c->p, c->t, c->d, c->s);
+    return down(c, c->d, den, ld);
+}
+
+int wd2_filter(int nx, int ny, int ld, int mgri, int mgrj, int ncomp, const int *nReg, const int *nRegBrd,
+               const int *nRegType, const int *nMomBdTp, const int *nTRgType, double fp, double *qu) {
+    if (ncomp != VU && ncomp != VV && ncomp != VT) return fail(WD2_ERR_FLAG, "Wrong ncomp flag passed to Filter");
+    int st;
+    if ((st = dropin_ctx(nx, ny, ld))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((st = set_regions(c, mgri, mgrj, nReg, nRegBrd, nRegType, nMomBdTp, nullptr, nullptr, nullptr, nullptr)) ||
+        (st = set_thermal(c, nTRgType, nullptr, nullptr, nullptr)))
+        return st;
+    if ((st = up(c, qu, ld, c->t))) return st;
+    launch_filter(c->R, c->TH, c->ld, ncomp, fp, c->t, c->qh, c->s);
+    return down(c, c->t, qu, ld);
 }
 
 // ------------------------------------------------------------ context API

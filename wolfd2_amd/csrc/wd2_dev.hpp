@@ -58,10 +58,29 @@ struct Regions {
     }
 };
 
+// wolfd2.h:24-31: thermal region and boundary types.
+enum { RT_NOSRCE = 0, RT_HEATGN = 1, RT_TEMPER = 2 };
+enum { BT_INTERN = 0, BT_TEMPER = 1, BT_HTFLUX = 2 };
+
+// Thermal region tables on the device (Fortran-ordered, extents of Regions):
+// nTRgType(mgri,mgrj), nTemBdTp(mgri,mgrj,4), dTRgVal(mgri,mgrj),
+// dHGSTval(mgri,mgrj).  Boundary temperatures are Regions::bc(.., VT).
+struct Thermal {
+    int mgri, mgrj;
+    const int *trtype, *tbtp;
+    const double *trval, *hgval;
+    __host__ __device__ int rtype(int ir, int jr) const { return trtype[(ir - 1) + mgri * (jr - 1)]; }
+    __host__ __device__ int btype(int ir, int jr, int side) const {
+        return tbtp[(ir - 1) + mgri * ((jr - 1) + mgrj * (side - 1))];
+    }
+    __host__ __device__ double tval(int ir, int jr) const { return trval[(ir - 1) + mgri * (jr - 1)]; }
+    __host__ __device__ double hg(int ir, int jr) const { return hgval[(ir - 1) + mgri * (jr - 1)]; }
+};
+
 // Metric arrays used on the hot path (grid.f:Metric names).
 struct Metrics {
     const double *ran, *rbn, *rgn, *rac, *rbc, *rgc, *rau, *rbu, *rbv, *rgv;
-    const double *dju, *djv, *xen, *yen, *xzn, *yzn, *xec, *yec, *xzc, *yzc;
+    const double *dju, *djv, *djc, *xen, *yen, *xzn, *yzn, *xec, *yec, *xzc, *yzc;
     const double *xeu, *yeu, *xzv, *yzv, *xzu, *yzu, *xev, *yev;
 };
 

@@ -152,4 +152,19 @@ void launch_scatter(int nx, int ny, int ld, int comp, const T *x, double *out, h
 void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const double *dus, const double *dvs,
                          double *partials, Scalars *sc, hipStream_t s);
 
+// Thermal energy (k_thermal.hip).  err receives 2000+side for a bad
+// temperature boundary flag.
+struct ThermTemps {
+    double *cu1, *cv1, *cun, *cvn;   // ConvCoef case 6 of (u, v), case 3 of (un, vn)
+};
+void launch_temp_bc(const Regions &R, const Thermal &TH, int ld, double *t, int *err, hipStream_t s);
+void launch_therm_energy(const Regions &R, const Thermal &TH, int ld, double dk, double pe, const Metrics &m,
+                         const double *un, const double *vn, const double *u, const double *v, const double *tn,
+                         double *t, const ThermTemps &w, const ChainT<double> &ch, const TridWork &tw, int *err,
+                         hipStream_t s);
+void launch_eq_state(int nx, int ny, int ld, double uref, double densref, double tmax, double tref, double rconst,
+                     const double *p, const double *t, double *den, hipStream_t s);
+void launch_filter(const Regions &R, const Thermal &TH, int ld, int ncomp, double fp, double *qu, double *qh,
+                   hipStream_t s);
+
 }  // namespace wd2

@@ -13,6 +13,10 @@ C       PresBoundCond  bound_cond.f:853-856
 C       VelOutflowBCs  bound_cond.f:1656-1659
 C       DiffMaxNorm    utility.f:446
 C       DMaxNorm       utility.f:479
+C       TempBoundCond  bound_cond.f:1030-1033
+C       ThermEnergy    thermal.f:24-32
+C       EqState        thermal.f:280-282
+C       Filter         utility.f:31-34
 C
 C     Each one calls the C ABI of libwolfd2_amd (include/wolfd2_amd.h)
 C     through ISO_C_BINDING interfaces.  The caller's compile-time extents
@@ -133,6 +137,57 @@ C=======================================================================
           INTEGER(C_INT) :: wd2_diff_max_norm
           INTEGER(C_INT), VALUE :: nx, ny, ld
           REAL(C_DOUBLE) :: un(*), u(*), res
+        END FUNCTION
+
+        FUNCTION wd2_temp_bound_cond(nx, ny, ld, mgri, mgrj, nReg,
+     &      nRegBrd, nTRgType, nTemBdTp, dTRgVal, dBCVal, t)
+     &      BIND(C, NAME='wd2_temp_bound_cond')
+          IMPORT :: C_INT, C_DOUBLE
+          INTEGER(C_INT) :: wd2_temp_bound_cond
+          INTEGER(C_INT), VALUE :: nx, ny, ld, mgri, mgrj
+          INTEGER(C_INT) :: nReg(*), nRegBrd(*), nTRgType(*)
+          INTEGER(C_INT) :: nTemBdTp(*)
+          REAL(C_DOUBLE) :: dTRgVal(*), dBCVal(*), t(*)
+        END FUNCTION
+
+        FUNCTION wd2_therm_energy(nx, ny, ld, mgri, mgrj, nReg,
+     &      nRegBrd, nTRgType, nTemBdTp, dk, pe, dTRgVal, dHGSTval,
+     &      dBCVal, rau, rbu, rbv, rgv, djc, xeu, yeu, xzv, yzv,
+     &      xec, yec, xzc, yzc, un, vn, u, v, tn, t)
+     &      BIND(C, NAME='wd2_therm_energy')
+          IMPORT :: C_INT, C_DOUBLE
+          INTEGER(C_INT) :: wd2_therm_energy
+          INTEGER(C_INT), VALUE :: nx, ny, ld, mgri, mgrj
+          INTEGER(C_INT) :: nReg(*), nRegBrd(*), nTRgType(*)
+          INTEGER(C_INT) :: nTemBdTp(*)
+          REAL(C_DOUBLE), VALUE :: dk, pe
+          REAL(C_DOUBLE) :: dTRgVal(*), dHGSTval(*), dBCVal(*)
+          REAL(C_DOUBLE) :: rau(*), rbu(*), rbv(*), rgv(*), djc(*)
+          REAL(C_DOUBLE) :: xeu(*), yeu(*), xzv(*), yzv(*)
+          REAL(C_DOUBLE) :: xec(*), yec(*), xzc(*), yzc(*)
+          REAL(C_DOUBLE) :: un(*), vn(*), u(*), v(*), tn(*), t(*)
+        END FUNCTION
+
+        FUNCTION wd2_eq_state(nx, ny, ld, uref, densref, tmax, tref,
+     &      rconst, p, t, den)
+     &      BIND(C, NAME='wd2_eq_state')
+          IMPORT :: C_INT, C_DOUBLE
+          INTEGER(C_INT) :: wd2_eq_state
+          INTEGER(C_INT), VALUE :: nx, ny, ld
+          REAL(C_DOUBLE), VALUE :: uref, densref, tmax, tref, rconst
+          REAL(C_DOUBLE) :: p(*), t(*), den(*)
+        END FUNCTION
+
+        FUNCTION wd2_filter(nx, ny, ld, mgri, mgrj, ncomp, nReg,
+     &      nRegBrd, nRegType, nMomBdTp, nTRgType, fp, qu)
+     &      BIND(C, NAME='wd2_filter')
+          IMPORT :: C_INT, C_DOUBLE
+          INTEGER(C_INT) :: wd2_filter
+          INTEGER(C_INT), VALUE :: nx, ny, ld, mgri, mgrj, ncomp
+          INTEGER(C_INT) :: nReg(*), nRegBrd(*), nRegType(*)
+          INTEGER(C_INT) :: nMomBdTp(*), nTRgType(*)
+          REAL(C_DOUBLE), VALUE :: fp
+          REAL(C_DOUBLE) :: qu(*)
         END FUNCTION
       END INTERFACE
 
@@ -303,4 +358,74 @@ C-----------------------------------------------------------------------
       ist = wd2_diff_max_norm(nx, ny, mnx + 1, un, u, r)
       CALL wd2_check(ist, 'DiffMaxNorm')
       DiffMaxNorm = r
+      END
+
+C-----------------------------------------------------------------------
+      SUBROUTINE TempBoundCond(nx, ny, nReg, nRegBrd, nTRgType,
+     &    nTemBdTp, dTRgVal, dBCVal, t)
+      USE wd2_iface
+      IMPLICIT NONE
+      INCLUDE 'config.f'
+      INTEGER nx, ny
+      INTEGER nReg(*), nRegBrd(*), nTRgType(*), nTemBdTp(*)
+      REAL*8  dTRgVal(*), dBCVal(*), t(*)
+      INTEGER(C_INT) ist
+      ist = wd2_temp_bound_cond(nx, ny, mnx + 1, mgri, mgrj, nReg,
+     &    nRegBrd, nTRgType, nTemBdTp, dTRgVal, dBCVal, t)
+      CALL wd2_check(ist, 'TempBoundCond')
+      END
+
+C-----------------------------------------------------------------------
+      SUBROUTINE ThermEnergy(nx, ny, nReg, nRegBrd, nTRgType,
+     &    nTemBdTp, dk, pe, dTRgVal, dHGSTval, dBCVal,
+     &    rau, rbu, rbv, rgv, djc, xeu, yeu, xzv, yzv,
+     &    xec, yec, xzc, yzc, un, vn, u, v, tn, t)
+      USE wd2_iface
+      IMPLICIT NONE
+      INCLUDE 'config.f'
+      INTEGER nx, ny
+      INTEGER nReg(*), nRegBrd(*), nTRgType(*), nTemBdTp(*)
+      REAL*8  dk, pe
+      REAL*8  dTRgVal(*), dHGSTval(*), dBCVal(*)
+      REAL*8  rau(*), rbu(*), rbv(*), rgv(*), djc(*)
+      REAL*8  xeu(*), yeu(*), xzv(*), yzv(*)
+      REAL*8  xec(*), yec(*), xzc(*), yzc(*)
+      REAL*8  un(*), vn(*), u(*), v(*), tn(*), t(*)
+      INTEGER(C_INT) ist
+      ist = wd2_therm_energy(nx, ny, mnx + 1, mgri, mgrj, nReg,
+     &    nRegBrd, nTRgType, nTemBdTp, dk, pe, dTRgVal, dHGSTval,
+     &    dBCVal, rau, rbu, rbv, rgv, djc, xeu, yeu, xzv, yzv,
+     &    xec, yec, xzc, yzc, un, vn, u, v, tn, t)
+      CALL wd2_check(ist, 'ThermEnergy')
+      END
+
+C-----------------------------------------------------------------------
+      SUBROUTINE EqState(nx, ny, uref, densref, tmax, tref, rconst,
+     &    p, t, den)
+      USE wd2_iface
+      IMPLICIT NONE
+      INCLUDE 'config.f'
+      INTEGER nx, ny
+      REAL*8  uref, densref, tmax, tref, rconst
+      REAL*8  p(*), t(*), den(*)
+      INTEGER(C_INT) ist
+      ist = wd2_eq_state(nx, ny, mnx + 1, uref, densref, tmax, tref,
+     &    rconst, p, t, den)
+      CALL wd2_check(ist, 'EqState')
+      END
+
+C-----------------------------------------------------------------------
+      SUBROUTINE Filter(nx, ny, ncomp, nReg, nRegBrd, nRegType,
+     &    nMomBdTp, nTRgType, fp, qu)
+      USE wd2_iface
+      IMPLICIT NONE
+      INCLUDE 'config.f'
+      INTEGER nx, ny, ncomp
+      INTEGER nReg(*), nRegBrd(*), nRegType(*), nMomBdTp(*)
+      INTEGER nTRgType(*)
+      REAL*8  fp, qu(*)
+      INTEGER(C_INT) ist
+      ist = wd2_filter(nx, ny, mnx + 1, mgri, mgrj, ncomp, nReg,
+     &    nRegBrd, nRegType, nMomBdTp, nTRgType, fp, qu)
+      CALL wd2_check(ist, 'Filter')
       END
