@@ -199,10 +199,37 @@ int wd2_ctx_set_fields(wd2_ctx *ctx, int ld, const double *u, const double *v, c
 int wd2_ctx_get_fields(wd2_ctx *ctx, int ld, double *u, double *v, double *p);
 /* main.f:606-641: VelBoundCond, Ppe, PresBoundCond, Project, VelBoundCond. */
 int wd2_ctx_cold_start(wd2_ctx *ctx, const wd2_params *prm, int *nSorConv);
-/* nsteps of main.f:690-972 (cold flow, one momentum/energy iteration).
- * Per step k (if the pointers are non-null): ql_iter[k], ppe_iter[k], and
- * dif[3k..3k+2] = DiffMaxNorm of p, u, v against the previous step. */
+/* nsteps of main.f:690-972 with the context's step options (default: cold
+ * flow, one momentum/energy iteration, no filters).  Per step k (if the
+ * pointers are non-null): ql_iter[k], ppe_iter[k] of the step's last
+ * momentum/energy iteration, and dif[3k..3k+2] = DiffMaxNorm of p, u, v
+ * against the previous step. */
 int wd2_ctx_step(wd2_ctx *ctx, const wd2_params *prm, int nsteps, int *ql_iter, int *ppe_iter, double *dif);
+/* As wd2_ctx_step, with dif[4k..4k+3] = DiffMaxNorm of p, u, v, t (the four
+ * columns of PrintDiff when the energy equation is on). */
+int wd2_ctx_step4(wd2_ctx *ctx, const wd2_params *prm, int nsteps, int *ql_iter, int *ppe_iter, double *dif);
+
+/* Step options: the rest of main.f:690-972 (input deck keywords of
+ * parse.f:117-193 in brackets). */
+typedef struct {
+    int nthermen;      /* [thermal_energy] ThermEnergy in every iteration, TempBoundCond at step end */
+    int neqstate;      /* [buoyancy] EqState after ThermEnergy: the density d of YMomentum */
+    int nmeiter;       /* [max_me_iter] momentum/energy iterations (1 when nthermen = 0) */
+    int nfiltu, nfiltv, nfiltt;   /* [filter_u/v/t] Shuman filters of us, vs, t */
+    double dmeittol;   /* [me_tolerance] stop when l > 1 and max(dif) < dmeittol */
+    double fpu, fpv, fpt;         /* filter parameters */
+    double pe;         /* Peclet number (file_manip.f:158) */
+    double uref, densref, tmax, tref, rconst;   /* EqState arguments */
+} wd2_step_options;
+int wd2_ctx_set_options(wd2_ctx *ctx, const wd2_step_options *opt);
+/* Thermal region tables of the context (see wd2_therm_energy); the boundary
+ * temperatures are the context's dBCVal(.,.,side,4). */
+int wd2_ctx_set_thermal(wd2_ctx *ctx, const int *nTRgType, const int *nTemBdTp, const double *dTRgVal,
+                        const double *dHGSTval);
+/* Temperature t and density d of the context (host, leading dimension ld;
+ * a null pointer is skipped). */
+int wd2_ctx_set_thermal_fields(wd2_ctx *ctx, int ld, const double *t, const double *d);
+int wd2_ctx_get_thermal_fields(wd2_ctx *ctx, int ld, double *t, double *d);
 /* Device pointers of the context's u, v, p fields and their leading
  * dimension (the device layout pads rows to a multiple of 16 elements). */
 int wd2_ctx_device_fields(wd2_ctx *ctx, double **u, double **v, double **p, int *ld);

@@ -73,7 +73,7 @@ def test_restart_round_trip(tmp_path):
 
 def test_refused_features(tmp_path):
     p = tmp_path / "input.dat"
-    p.write_text("section input_parameters\ngrid_file g.xyz\nthermal_energy on\nend section\n")
+    p.write_text("section input_parameters\ngrid_file g.xyz\nsmall_scale\nend section\n")
     with pytest.raises(deck.DeckError, match="outside the GPU hot path"):
         deck.read_input(str(p))
     p.write_text("section input_parameters\ngrid_file g.xyz\nend section\nsection boundary_conditions\n"
@@ -104,3 +104,58 @@ def test_print_diff_lines_match_the_reference_program():
         r = refdecks.parse_print_diff(ln)[0]
         mine = deck.print_diff_row(int(r[0]), r[1], int(r[2]), int(r[3]), r[4:7])
         assert mine.rstrip() == ln.rstrip(), (mine, ln)
+
+
+THERMAL_DECK = dict(nts=12, dt=1e-3, mu=0.01, ppe_solver="rb_sor", nreg=(2, 1), i_borders=(9,), gas_constant=287.0,
+                    extra_input=["thermal_energy", "buoyancy", "max_me_iter 2", "filter_t 30.0",
+                                 "ref_temperature 300.0", "max_ref_temperature 320.0", "print_bann_freq 5"],
+                    bc_lines=["wall 1 1 west temper 320.0", "fixed_temperature_wall 2 1 east 310.0",
+                              "heat_flux_wall 1 1 south", "heat_generation_region 2 1 25.0",
+                              "wall 1 1 north tangent_vel 0.5"])
+
+
+@pytest.mark.skipif(not os.path.exists(REF_EXE), reason="oracle/_ref not built (needs the reference sources)")
+def test_print_diff_thermal_lines_match_the_reference_program():
+    """With the energy equation the reference prints a 78-column rule, four
+    a12 headings and rows of four e12.5 norms; the banner and rows of
+    deck.PRINTDIFF_*_T and deck.print_diff_row(thermal=True) match them."""
+    nx = ny = 17
+    x, y = refdecks.uniform_grid(nx, ny)
+    d = refdecks.Deck(**THERMAL_DECK)
+    with tempfile.TemporaryDirectory() as rd:
+        d.write(rd, x, y)
+        out = subprocess.run([REF_EXE], cwd=rd, capture_output=True, text=True, timeout=60, check=True).stdout
+    lines = out.splitlines()
+    assert deck.PRINTDIFF_HEAD_T.rstrip() in [ln.rstrip() for ln in lines]
+    assert sum(ln == deck.PRINTDIFF_RULE_T for ln in lines) == 2 * 3   # banners before steps 1, 6, 11
+    rows = [ln for ln in lines if refdecks._DIFF_RE.match(ln)]
+    assert len(rows) == 12
+    for ln in rows:
+        w = ln.split()
+        k, t, ql = int(w[0]), float(w[1]), int(w[2].rstrip("*"))
+        mine = deck.print_diff_row(k, t, ql, int(w[3]), [float(v) for v in w[4:8]], thermal=True)
+        assert mine == ln, (mine, ln)
+
+
+def test_thermal_deck_tables(tmp_path):
+    """The thermal tables the deck builds: defaults (adiabatic outer walls),
+    wall temperatures scaled by (T - tref)/(tmax - tref), a fixed-temperature
+    wall, heat generation made non-dimensional by NDHGenCoef."""
+    nx = ny = 17
+    x, y = refdecks.uniform_grid(nx, ny)
+    dk = refdecks.Deck(**THERMAL_DECK)
+    dk.write(str(tmp_path), x, y)
+    d = deck.read_input(str(tmp_path / "input.dat"))
+    s = d.settings
+    assert (s["nthermen"], s["neqstate"], s["nmeiter"], s["nfiltt"], s["fpt"]) == (1, 1, 2, 1, 30.0)
+    assert (s["tref"], s["tmax"]) == (300.0, 320.0)
+    assert deck.read_gas_constant(str(tmp_path / "fluid.dat"), "fl01") == 287.0
+    reg = deck.build_case(d, 1.0, nx, ny)
+    th = deck.build_thermal(d, reg, 0.025, 70.0)
+    W, E, S, N = case.WEST - 1, case.EAST - 1, case.SOUTH - 1, case.NORTH - 1
+    assert th.nTemBdTp[0, 0, W] == case.BT_TEMPER and reg.dBCVal[0, 0, W, 3] == (320.0 - 300.0) / 20.0
+    assert th.nTemBdTp[1, 0, E] == case.BT_TEMPER and reg.dBCVal[1, 0, E, 3] == (310.0 - 300.0) / 20.0
+    assert th.nTemBdTp[0, 0, S] == case.BT_HTFLUX and th.nTemBdTp[1, 0, N] == case.BT_HTFLUX
+    assert th.nTemBdTp[0, 0, E] == case.BT_INTERN and th.nTemBdTp[1, 0, W] == case.BT_INTERN
+    assert th.nTRgType[1, 0] == case.RT_HEATGN and th.nTRgType[0, 0] == case.RT_NOSRCE
+    assert th.dHGSTval[1, 0] == (25.0 * (1.0 * 1.0)) / (70.0 * 0.025 * (320.0 - 300.0))

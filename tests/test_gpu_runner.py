@@ -107,3 +107,24 @@ def test_runner_bc_profile_deck_bitwise_vs_oracle(oracle, tmp_path):
     S = np.s_[: ny + 2, : nx + 2]
     for f, a in zip("uvp", (u, v, p)):
         assert np.array_equal(rst[f], a[S]), f
+
+
+def test_runner_thermal_reproduces_the_reference_program(tmp_path):
+    """A heated cavity deck (energy equation, buoyancy, filters, momentum/
+    energy iterations, heat generation) on the device-resident context: the
+    PrintDiff lines (four norms) and the restart fields u, v, p, t equal the
+    reference program's."""
+    fx = helpers.load("thermrun_therm_cavity_32.npz")
+    refdecks.write_grid(str(tmp_path / "grid.xyz"), fx["x"], fx["y"])
+    refdecks.write_fluid_table(str(tmp_path / "fluid.dat"), "fl01", float(fx["rho"]), float(fx["mu"]),
+                               float(fx["gas_constant"]))
+    (tmp_path / "input.dat").write_text(str(fx["deck"]))
+    print(f"runner thermal cavity steps={int(fx['nts'])}", flush=True)
+    buf = io.StringIO()
+    run_case(str(tmp_path), out=buf)
+    lines = [ln for ln in buf.getvalue().splitlines() if refdecks._DIFF_RE.match(ln)]
+    assert lines == str(fx["diff_lines"]).split("\n"), "PrintDiff lines differ"
+    rst = deck.read_restart(str(tmp_path / "out.rst"))
+    assert rst["k"] == int(fx["k"]) and rst["dtime"] == float(fx["dtime"])
+    for f in ("u", "v", "p", "t"):
+        assert np.array_equal(rst[f], fx[f]), f

@@ -31,6 +31,14 @@ class Params(C.Structure):
                 ("dk", C.c_double), ("re", C.c_double), ("fr", C.c_double)]
 
 
+class StepOptions(C.Structure):
+    """wd2_step_options of include/wolfd2_amd.h."""
+    _fields_ = [("nthermen", C.c_int), ("neqstate", C.c_int), ("nmeiter", C.c_int), ("nfiltu", C.c_int),
+                ("nfiltv", C.c_int), ("nfiltt", C.c_int), ("dmeittol", C.c_double), ("fpu", C.c_double),
+                ("fpv", C.c_double), ("fpt", C.c_double), ("pe", C.c_double), ("uref", C.c_double),
+                ("densref", C.c_double), ("tmax", C.c_double), ("tref", C.c_double), ("rconst", C.c_double)]
+
+
 _lib = None
 
 

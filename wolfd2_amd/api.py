@@ -21,7 +21,7 @@ import ctypes as C
 import numpy as np
 
 from . import case
-from ._lib import Params, Wd2Error, check, dp, fp, ip, lib
+from ._lib import Params, StepOptions, Wd2Error, check, dp, fp, ip, lib
 
 __all__ = ["vel_bound_cond", "pres_bound_cond", "vel_outflow_bcs", "project", "divergence", "ppe",
            "x_momentum", "y_momentum", "naux_momentum", "alt_trid_lu", "dmax_norm", "diff_max_norm",
@@ -173,6 +173,7 @@ class Solver:
                                    ip(reg.nRegBrd), ip(reg.nRegType), ip(reg.nMomBdTp), fp(reg.dBCVal),
                                    fp(reg.dPRporos), fp(reg.dPRporc1), fp(reg.dPRporc2), mets))
         self.params = make_params(prm)
+        self.thermal = False
         for (ir, jr, side, var), values in reg.profiles.items():
             self.set_bc_profile(ir, jr, side, var, values)
 
@@ -202,6 +203,26 @@ class Solver:
         self.prm = prm
         self.params = make_params(prm)
 
+    def set_options(self, opt: case.StepOptions):
+        """Energy equation, buoyancy, momentum/energy iterations and filters
+        of the following steps (main.f:735-895)."""
+        o = StepOptions(*[getattr(opt, f) for f, _ in StepOptions._fields_])
+        check(lib().wd2_ctx_set_options(self.ctx, C.byref(o)))
+        self.thermal = opt.nthermen == 1
+
+    def set_thermal(self, th: case.Thermal):
+        check(lib().wd2_ctx_set_thermal(self.ctx, ip(th.nTRgType), ip(th.nTemBdTp), fp(th.dTRgVal),
+                                        fp(th.dHGSTval)))
+
+    def set_thermal_fields(self, t=None, d=None):
+        check(lib().wd2_ctx_set_thermal_fields(self.ctx, self.g.ld, dp(t), dp(d)))
+
+    def get_thermal_fields(self):
+        g = self.g
+        t, d = np.zeros((g.ncol, g.ld)), np.zeros((g.ncol, g.ld))
+        check(lib().wd2_ctx_get_thermal_fields(self.ctx, g.ld, dp(t), dp(d)))
+        return t, d
+
     def set_precision(self, precision: str):
         """"fp64" (the reference's arithmetic, bit for bit) or "mixed" (FP64
         fields; FP32 momentum stencils and ADI chains; red-black SOR as FP64
@@ -223,13 +244,16 @@ class Solver:
         return n.value
 
     def step(self, nsteps: int = 1) -> np.ndarray:
-        """Advance nsteps; returns rows (ql_iter, ppe_iter, |p1-pn|, |u1-un|, |v1-vn|)."""
+        """Advance nsteps; returns rows (ql_iter, ppe_iter, |p1-pn|, |u1-un|,
+        |v1-vn|), and |t1-tn| as a sixth column with the energy equation on."""
+        nd = 4 if self.thermal else 3
         ql = (C.c_int * nsteps)()
         pi = (C.c_int * nsteps)()
-        dif = (C.c_double * (3 * nsteps))()
-        check(lib().wd2_ctx_step(self.ctx, C.byref(self.params), nsteps, ql, pi, dif))
-        out = np.zeros((nsteps, 5))
+        dif = (C.c_double * (nd * nsteps))()
+        fn = lib().wd2_ctx_step4 if self.thermal else lib().wd2_ctx_step
+        check(fn(self.ctx, C.byref(self.params), nsteps, ql, pi, dif))
+        out = np.zeros((nsteps, 2 + nd))
         out[:, 0] = np.frombuffer(ql, dtype=np.int32)
         out[:, 1] = np.frombuffer(pi, dtype=np.int32)
-        out[:, 2:] = np.frombuffer(dif, dtype=np.float64).reshape(nsteps, 3)
+        out[:, 2:] = np.frombuffer(dif, dtype=np.float64).reshape(nsteps, nd)
         return out

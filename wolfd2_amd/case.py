@@ -322,8 +322,9 @@ def make_thermal(reg: Regions, heat=(), fixed=(), walls=()) -> Thermal:
     (BT_HTFLUX, zero flux) outer walls (parse.f:InitBCFlags 2350-2375), then
     ``heat`` items (ireg, jreg, q) with q the non-dimensional source,
     ``fixed`` items (ireg, jreg, T) for fixed-temperature regions (their sides
-    and the facing sides of neighbours become BT_TEMPER at T, parse.f:
-    1805-1860), and ``walls`` items (ireg, jreg, side, 'temper'|'flux', value)."""
+    and the facing sides of neighbours become BT_TEMPER at T, as a 'blockage
+    ... temper' line does, parse.f:1357-1392), and ``walls`` items (ireg,
+    jreg, side, 'temper'|'flux', value)."""
     mgri, mgrj = reg.mgri, reg.mgrj
     ni, nj = int(reg.nreg[0]), int(reg.nreg[1])
     f = dict(order="F")
@@ -499,6 +500,29 @@ class Params:
         re = dlref * uref / dkinvisc               # file_manip.f:157
         fr = uref ** 2.0 / (dlref * 9.81)          # file_manip.f:159
         return Params(dk=dk, re=re, fr=fr, **kw)
+
+
+@dataclass
+class StepOptions:
+    """The rest of a time step's switches (parse.f:117-193 defaults): the
+    energy equation, the equation of state (buoyancy), the momentum/energy
+    iterations and the Shuman filters, with the physical constants they use."""
+    nthermen: int = 0
+    neqstate: int = 0
+    nmeiter: int = 1
+    nfiltu: int = 0
+    nfiltv: int = 0
+    nfiltt: int = 0
+    dmeittol: float = 1.0e-4
+    fpu: float = 5.0e2
+    fpv: float = 5.0e2
+    fpt: float = 5.0
+    pe: float = 1.0
+    uref: float = 1.0
+    densref: float = 1.0
+    tmax: float = 1.0e3
+    tref: float = 2.8357e2
+    rconst: float = 0.0
 
 
 PPE_SOLVERS = {"sor": 1, "lsor": 2, "rb_lsor": 3, "par_rb_lsor": 4, "rb_sor": 5, "par_rb_sor": 6}

@@ -67,6 +67,8 @@ struct wd2_ctx {
     int *d_thi = nullptr;
     double *d_thd = nullptr;
     Thermal TH{};
+    // step options (parse.f:117-193 defaults)
+    wd2_step_options opt{0, 0, 1, 0, 0, 0, 1.0e-4, 5.0e2, 5.0e2, 5.0, 1.0, 1.0, 1.0, 1.0e3, 2.8357e2, 0.0};
     int *d_poff = nullptr;
     double *d_pval = nullptr;
     size_t pval_cap = 0;
@@ -623,38 +625,87 @@ static int cold_start(wd2_ctx *c, const wd2_params *P, int *nsor) {
     return check_err(c);
 }
 
-static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double *dif3) {
+// One time step of main.f:690-972: the time-level copies, the momentum/energy
+// iterations (nAuxMomentum, filters, boundary fills, Ppe, Project and, with
+// the energy equation, ThermEnergy and EqState), the temperature filter, the
+// step-end fills and the PrintDiff norms dif[0..3] = |p-pn|, |u-un|, |v-vn|,
+// |t-tn| (dif may be null; only its first three are written without the
+// energy equation).
+static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double *dif) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
+    const wd2_step_options &O = c->opt;
+    const bool therm = O.nthermen == 1;
     hipStream_t s = c->s;
-    // main.f:696-704, 741-747
+    // main.f:696-704
     launch_copy(ld, 0, nx + 1, 0, ny + 1, c->p, c->pn, s);
     launch_copy(ld, 0, nx + 1, 0, ny + 1, c->u, c->un, s);
     launch_copy(ld, 0, nx + 1, 0, ny + 1, c->v, c->vn, s);
+    if (therm) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->t, c->tn, s);
     launch_copy(ld, 0, nx + 1, 0, ny + 1, c->d, c->dn, s);
-    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->u, c->us, s);
-    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->v, c->vs, s);
-    int st = naux_momentum(c, P->mqiter, P->dk, P->re, P->fr, P->qtol, nql);
-    if (st) return st;
-    launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
-    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
-    st = ppe(c, P->lCartesGrid, P->nPpeSolver, P->msorit, P->dk, P->sortol, P->sorrel, c->us, c->vs, c->p, nsor);
-    if (st) return st;
-    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
-    launch_project(c->R, ld, P->dk, c->M, c->p, c->us, c->vs, s);
-    launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
-    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
-    // main.f:864-870: u = us, v = vs (swap the buffers: same values)
-    std::swap(c->u, c->us);
-    std::swap(c->v, c->vs);
+    // main.f:735: one iteration without the energy equation
+    const int nme = (!therm && O.nmeiter > 0) ? 1 : O.nmeiter;
+    const ThermTemps tt{c->T.cj1, c->T.cj2, c->T.c1n, c->T.c2n};
+    int st;
+    for (int l = 1; l <= nme; l++) {
+        // main.f:741-747
+        launch_copy(ld, 0, nx + 1, 0, ny + 1, c->u, c->us, s);
+        launch_copy(ld, 0, nx + 1, 0, ny + 1, c->v, c->vs, s);
+        if (therm) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->t, c->ts, s);
+        if ((st = naux_momentum(c, P->mqiter, P->dk, P->re, P->fr, P->qtol, nql))) return st;
+        // main.f:783-791
+        if (O.nfiltu == 1) launch_filter(c->R, c->TH, ld, VU, O.fpu, c->us, c->qh, s);
+        if (O.nfiltv == 1) launch_filter(c->R, c->TH, ld, VV, O.fpv, c->vs, c->qh, s);
+        launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
+        launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+        st = ppe(c, P->lCartesGrid, P->nPpeSolver, P->msorit, P->dk, P->sortol, P->sorrel, c->us, c->vs, c->p, nsor);
+        if (st) return st;
+        launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+        launch_project(c->R, ld, P->dk, c->M, c->p, c->us, c->vs, s);
+        launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
+        launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+        // main.f:840-855
+        if (therm)
+            launch_therm_energy(c->R, c->TH, ld, P->dk, O.pe, c->M, c->un, c->vn, c->us, c->vs, c->tn, c->ts, tt,
+                                c->ch, c->tw, &c->sc->err, s);
+        if (O.neqstate == 1)
+            launch_eq_state(nx, ny, ld, O.uref, O.densref, O.tmax, O.tref, O.rconst, c->p, therm ? c->ts : c->t,
+                            c->d, s);
+        // main.f:857-875: the iteration test needs DiffMaxNorm of (u,us),
+        // (v,vs), (t,ts) from the second iteration on (after the last one the
+        // loop ends either way)
+        const bool test = l > 1 && l < nme;
+        if (test) {
+            launch_max_norm(nx, ny, ld, c->u, c->us, c->partials, c->sc, 5, s);
+            launch_max_norm(nx, ny, ld, c->v, c->vs, c->partials + RED_MAX_BLOCKS, c->sc, 6, s);
+            launch_max_norm(nx, ny, ld, therm ? c->t : c->u, therm ? c->ts : c->u, c->partials + 2 * RED_MAX_BLOCKS,
+                            c->sc, 7, s);
+        }
+        // u = us, v = vs, t = ts (swap the buffers: same values)
+        std::swap(c->u, c->us);
+        std::swap(c->v, c->vs);
+        if (therm) std::swap(c->t, c->ts);
+        if (test) {
+            double dm[3];
+            HIPCHK(hipMemcpyAsync(dm, &c->sc->val[5], 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            // main.f:873: difmax = max(dif(1),dif(2),dif(3)) (Fortran MAX)
+            double dmax = dm[0];
+            if (dm[1] > dmax) dmax = dm[1];
+            if (dm[2] > dmax) dmax = dm[2];
+            if (dmax < O.dmeittol) break;
+        }
+    }
+    // main.f:890-895
+    if (therm && O.nfiltt == 1) launch_filter(c->R, c->TH, ld, VT, O.fpt, c->t, c->qh, s);
     launch_vel_bc<double>(c->R, ld, c->u, c->v, false, &c->sc->err, s);
     launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    if (therm) launch_temp_bc(c->R, c->TH, ld, c->t, &c->sc->err, s);
     // main.f:962-965
     launch_max_norm(nx, ny, ld, c->pn, c->p, c->partials, c->sc, 2, s);
     launch_max_norm(nx, ny, ld, c->un, c->u, c->partials + RED_MAX_BLOCKS, c->sc, 3, s);
     launch_max_norm(nx, ny, ld, c->vn, c->v, c->partials + 2 * RED_MAX_BLOCKS, c->sc, 4, s);
-    if (dif3) {
-        HIPCHK(hipMemcpyAsync(dif3, &c->sc->val[2], 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-    }
+    if (therm) launch_max_norm(nx, ny, ld, c->tn, c->t, c->partials + 3 * RED_MAX_BLOCKS, c->sc, 5, s);
+    if (dif) HIPCHK(hipMemcpyAsync(dif, &c->sc->val[2], (therm ? 4 : 3) * sizeof(double), hipMemcpyDeviceToHost, s));
     return check_err(c);
 }
 
@@ -1042,6 +1093,10 @@ int wd2_ctx_create(wd2_ctx **out, int nx, int ny, int ld, int mgri, int mgrj, co
         ctx_free(c);
         return st;
     }
+    if ((st = set_thermal(c, nullptr, nullptr, nullptr, nullptr))) {
+        ctx_free(c);
+        return st;
+    }
     for (int k = 0; k < M_COUNT; k++)
         if ((st = up(c, metrics[k], ld, c->met[k]))) {
             ctx_free(c);
@@ -1084,15 +1139,60 @@ int wd2_ctx_cold_start(wd2_ctx *c, const wd2_params *P, int *nSorConv) {
     return st;
 }
 
-int wd2_ctx_step(wd2_ctx *c, const wd2_params *P, int nsteps, int *ql_iter, int *ppe_iter, double *dif) {
+static int ctx_steps(wd2_ctx *c, const wd2_params *P, int nsteps, int *ql_iter, int *ppe_iter, double *dif,
+                     int ndif) {
+    double d4[4] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < nsteps; k++) {
         int nql = 0, nsor = 0;
-        int st = one_step(c, P, &nql, &nsor, dif ? dif + 3 * k : nullptr);
+        int st = one_step(c, P, &nql, &nsor, dif ? d4 : nullptr);
         if (st) return st;
+        if (dif) {
+            HIPCHK(hipStreamSynchronize(c->s));   // d4 is written by the step's last copy
+            for (int q = 0; q < ndif; q++) dif[ndif * k + q] = d4[q];
+        }
         if (ql_iter) ql_iter[k] = nql;
         if (ppe_iter) ppe_iter[k] = nsor;
     }
     HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_ctx_step(wd2_ctx *c, const wd2_params *P, int nsteps, int *ql_iter, int *ppe_iter, double *dif) {
+    return ctx_steps(c, P, nsteps, ql_iter, ppe_iter, dif, 3);
+}
+
+int wd2_ctx_step4(wd2_ctx *c, const wd2_params *P, int nsteps, int *ql_iter, int *ppe_iter, double *dif) {
+    return ctx_steps(c, P, nsteps, ql_iter, ppe_iter, dif, 4);
+}
+
+int wd2_ctx_set_options(wd2_ctx *c, const wd2_step_options *opt) {
+    if (!c || !opt) return fail(WD2_ERR_ARG, "wd2_ctx_set_options: null argument");
+    if (opt->nthermen == 1 && !(opt->pe > 0.0))
+        return fail(WD2_ERR_ARG, "wd2_ctx_set_options: the energy equation needs a Peclet number > 0");
+    c->opt = *opt;
+    return WD2_OK;
+}
+
+int wd2_ctx_set_thermal(wd2_ctx *c, const int *nTRgType, const int *nTemBdTp, const double *dTRgVal,
+                        const double *dHGSTval) {
+    if (!c) return fail(WD2_ERR_ARG, "wd2_ctx_set_thermal: null context");
+    int st = set_thermal(c, nTRgType, nTemBdTp, dTRgVal, dHGSTval);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_ctx_set_thermal_fields(wd2_ctx *c, int ld, const double *t, const double *d) {
+    int st;
+    if ((st = up(c, t, ld, c->t)) || (st = up(c, d, ld, c->d))) return st;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_ctx_get_thermal_fields(wd2_ctx *c, int ld, double *t, double *d) {
+    int st;
+    if (t && (st = down(c, c->t, t, ld))) return st;
+    if (d && (st = down(c, c->d, d, ld))) return st;
     return WD2_OK;
 }
 

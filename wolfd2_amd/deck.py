@@ -10,14 +10,14 @@ Reads what a wolfd2 run reads:
 Writes what it writes: the restart file (file_manip.f:SaveRestart 364-401) and
 the per-step PrintDiff table (string.f:PrintDiff 467-570).
 
-Only the hot path's options are accepted. These features of the reference are
-outside that path, and a deck that turns one on is refused:
+The thermal keywords (energy equation, buoyancy, momentum/energy iterations,
+filters, temperature walls and regions, heat generation) are read as the
+reference reads them.  These features of the reference are outside the GPU
+path, and a deck that turns one on is refused:
 
-- thermal energy, buoyancy;
 - small scales, trajectories;
 - porous regions, whose coefficients come from the reference's permeability
-  correlations;
-- thermal boundary regions.
+  correlations.
 
 The reference program linked against this library through the Fortran binding
 serves those cases.
@@ -38,17 +38,19 @@ DEFAULTS = dict(nts=0, mqiter=20, nPpeSolver=1, msorit=2000, qtol=1.0e-4, sortol
                 dlref=1.0, uref=1.0, tref=2.8357e2, tmax=1.0e3, dk=0.0,
                 flPropFn="fluidprop.dat", flName="air01", resReadFl="restart.rst", resWriteFl="restart.rst",
                 nrestart=0, nreswrite=0, nWrtRstFq=100, lPrntDiff=0, nPrDfFreq=1, lPrDfBann=0, nPrDfBnFq=20,
-                cartesian=False, grid_file=None)
+                cartesian=False, grid_file=None,
+                nthermen=0, neqstate=0, nmeiter=1, dmeittol=1.0e-4, nfiltu=0, nfiltv=0, nfiltt=0,
+                fpu=5.0e2, fpv=5.0e2, fpt=5.0)
 
-# keywords that switch on physics outside the hot path (parse.f:905-973)
-REFUSED = {"thermal_energy", "buoyancy", "small_scale", "trajectories"}
-# keywords of outputs this runner does not produce; accepted and ignored
-IGNORED = {"max_me_iter", "me_tolerance", "filter_u", "filter_v", "filter_t", "time_average_mode",
+# keywords that switch on physics outside the GPU path (parse.f:905-973)
+REFUSED = {"small_scale", "trajectories"}
+# keywords of outputs this runner does not produce, and the small-scale
+# model's parameters (inert without small_scale); accepted and ignored
+IGNORED = {"time_average_mode",
            "output_format", "output_prefix", "snapshots_prefix", "snapshots_freq", "snapshots_first",
            "time_series_prefix", "time_series_freq", "time_series_point", "atd_cu0", "atd_tscf", "atd_hscf",
            "atd_bnumcrit", "atd_rmapmax", "atd_rmapexp", "atd_temcf", "atd_ppe_solver", "atd_max_sor_iter",
-           "atd_sor_tolerance", "atd_sor_relaxation", "atd_filter_u", "atd_filter_v", "atd_filter_t",
-           "max_ref_temperature"}
+           "atd_sor_tolerance", "atd_sor_relaxation", "atd_filter_u", "atd_filter_v", "atd_filter_t"}
 
 
 class DeckError(ValueError):
@@ -103,12 +105,24 @@ def read_input(path: str) -> Deck:
 
 def _input_keyword(s: dict, key: str, a: list) -> None:
     if key in REFUSED:
-        if a and a[0].lower() in ("off", "no", "0", "false"):
-            return
         raise DeckError(f"'{key}' is outside the GPU hot path (use the Fortran drop-in build for it)")
     if key in IGNORED:
         return
-    if key == "grid_file":
+    if key == "thermal_energy":            # parse.f:334-340: the keyword alone switches it on
+        s["nthermen"] = 1
+    elif key == "buoyancy":
+        s["neqstate"] = 1
+    elif key == "max_me_iter":
+        s["nmeiter"] = int(a[0])
+    elif key == "me_tolerance":
+        s["dmeittol"] = float(a[0])
+    elif key in ("filter_u", "filter_v", "filter_t"):   # parse.f:404-438
+        c = key[-1]
+        s[f"nfilt{c}"] = 1
+        s[f"fp{c}"] = float(a[0])
+    elif key == "max_ref_temperature":
+        s["tmax"] = float(a[0])
+    elif key == "grid_file":
         s["grid_file"] = a[0]
         s["cartesian"] = len(a) > 1 and a[1].lower() == "cartesian_grid"
     elif key == "n_time_steps":
@@ -169,13 +183,16 @@ def _bc_keyword(d: Deck, key: str, a: list) -> None:
     elif key in ("wall", "inlet", "outlet"):
         d.bc.append((key, int(a[0]), int(a[1]), a[2].lower(), [x.lower() for x in a[3:]]))
     elif key == "blockage":
-        d.bc.append((key, int(a[0]), int(a[1]), None, []))
+        d.bc.append((key, int(a[0]), int(a[1]), None, [x.lower() for x in a[2:]]))
+    elif key in ("heat_generation_region", "fixed_temperature_region"):
+        d.bc.append((key, int(a[0]), int(a[1]), None, [float(a[2])]))
+    elif key in ("fixed_temperature_wall", "heat_flux_wall"):
+        d.bc.append((key, int(a[0]), int(a[1]), a[2].lower(), [float(x) for x in a[3:4]]))
     elif key == "bc_profile":
         if a[2].lower() not in case.SIDES or a[3].lower() not in PROFILE_VARS or a[4].lower() not in PROFILE_SHAPES:
             raise ValueError("expected bc_profile IREG JREG SIDE u|v|p parabolic|sine|file ...")
         d.profiles.append((int(a[0]), int(a[1]), a[2].lower(), a[3].lower(), [a[4].lower()] + a[5:]))
-    elif key in ("porous_region", "heat_generation_region", "fixed_temperature_region", "fixed_temperature_wall",
-                 "heat_flux_wall"):
+    elif key == "porous_region":
         raise DeckError(f"'{key}' is outside the GPU hot path (use the Fortran drop-in build for it)")
     else:
         raise ValueError(f"unknown keyword '{key}'")
@@ -254,6 +271,8 @@ def build_case(d: Deck, rho: float, nx: int, ny: int, x=None, y=None, case_dir: 
         if key == "blockage":
             blocks.append((ir, jr))
             continue
+        if key in THERMAL_ONLY:
+            continue
         if side not in case.SIDES:
             raise DeckError(f"bad side '{side}' for {key} {ir} {jr}")
         press = _opt(w, ("press", "p"))
@@ -284,6 +303,93 @@ def build_case(d: Deck, rho: float, nx: int, ny: int, x=None, y=None, case_dir: 
             _apply_profile(reg, x, y, ir, jr, case.SIDES[side], PROFILE_VARS[var], w,
                            pscale if var == "p" else uref, case_dir)
     return reg
+
+
+# ------------------------------------------------------------ thermal tables
+THERMAL_ONLY = ("heat_generation_region", "fixed_temperature_region", "fixed_temperature_wall", "heat_flux_wall")
+
+
+def build_thermal(d: Deck, reg: case.Regions, dconduct: float, pe: float) -> case.Thermal:
+    """Thermal region tables of the boundary_conditions section, line by line
+    in deck order as parse.f:lReadBCFile applies them, after the defaults of
+    parse.f:InitBCFlags 2350-2375 (no source, internal sides, adiabatic outer
+    walls).  Temperatures are scaled as (T - tref)/(tmax - tref) into
+    dBCVal(.,.,side,4) of ``reg``; heat generation is made non-dimensional by
+    bound_cond.f:NDHGenCoef 1995-2030."""
+    s = d.settings
+    tref, tmax, dlref = s["tref"], s["tmax"], s["dlref"]
+    ni, nj = int(reg.nreg[0]), int(reg.nreg[1])
+    th = case.make_thermal(reg)
+
+    def scaled(x):
+        return (x - tref) / (tmax - tref)
+
+    def set_side(ir, jr, side, bt, value=None):
+        th.nTemBdTp[ir - 1, jr - 1, side - 1] = bt
+        if value is not None:
+            reg.dBCVal[ir - 1, jr - 1, side - 1, case.VAR_T - 1] = value
+
+    def neighbours(ir, jr):
+        for di, dj, side in ((-1, 0, case.EAST), (1, 0, case.WEST), (0, -1, case.NORTH), (0, 1, case.SOUTH)):
+            a, b = ir + di, jr + dj
+            if 1 <= a <= ni and 1 <= b <= nj:
+                yield a, b, side
+
+    for key, ir, jr, side, w in d.bc:
+        if key == "blockage":                       # parse.f:1318-1392
+            for sd in (case.WEST, case.EAST, case.SOUTH, case.NORTH):
+                set_side(ir, jr, sd, case.BT_HTFLUX)
+            for a, b, sd in neighbours(ir, jr):
+                set_side(a, b, sd, case.BT_HTFLUX)
+            tv = _opt(w, ("temper", "t"))
+            if tv is not None:
+                dum = scaled(tv)
+                th.nTRgType[ir - 1, jr - 1] = case.RT_TEMPER
+                th.dTRgVal[ir - 1, jr - 1] = dum
+                for sd in (case.WEST, case.EAST, case.SOUTH, case.NORTH):
+                    set_side(ir, jr, sd, case.BT_TEMPER, dum)
+                for a, b, sd in neighbours(ir, jr):
+                    set_side(a, b, sd, case.BT_TEMPER, dum)
+        elif key in ("wall", "inlet", "outlet"):   # parse.f:1553-1566, 1655-1668, 1738-1751
+            tv = _opt(w, ("temper", "t"))
+            if tv is not None:
+                set_side(ir, jr, case.SIDES[side], case.BT_TEMPER, scaled(tv))
+        elif key == "heat_generation_region":      # parse.f:1768-1800
+            th.nTRgType[ir - 1, jr - 1] = case.RT_HEATGN
+            th.dTRgVal[ir - 1, jr - 1] = w[0]
+        elif key == "fixed_temperature_region":    # parse.f:1805-1845
+            dum = scaled(w[0])
+            th.nTRgType[ir - 1, jr - 1] = case.RT_TEMPER
+            th.dTRgVal[ir - 1, jr - 1] = dum
+            for sd in (case.WEST, case.EAST, case.SOUTH, case.NORTH):
+                set_side(ir, jr, sd, case.BT_TEMPER, dum)
+        elif key == "fixed_temperature_wall":      # parse.f:1847-1882
+            if side not in case.SIDES or not w:
+                raise DeckError(f"bad fixed_temperature_wall {ir} {jr}")
+            set_side(ir, jr, case.SIDES[side], case.BT_TEMPER, scaled(w[0]))
+        elif key == "heat_flux_wall":              # parse.f:1886-1926 (adiabatic)
+            if side not in case.SIDES:
+                raise DeckError(f"bad heat_flux_wall {ir} {jr}")
+            set_side(ir, jr, case.SIDES[side], case.BT_HTFLUX)
+    for jr in range(1, nj + 1):                    # bound_cond.f:NDHGenCoef
+        for ir in range(1, ni + 1):
+            if th.nTRgType[ir - 1, jr - 1] == case.RT_HEATGN:
+                th.dHGSTval[ir - 1, jr - 1] = (th.dTRgVal[ir - 1, jr - 1] * (dlref * dlref)) / \
+                    (pe * dconduct * (tmax - tref))
+    return th
+
+
+def read_gas_constant(path: str, name: str) -> float:
+    """The gas constant of a fluid table entry, 'fluid NAME gasconstant R'
+    (parse.f:2130-2145); 0 when the entry gives none."""
+    with open(path) as f:
+        for raw in f:
+            w = raw.split()
+            if len(w) >= 2 and w[0].lower() == "fluid" and w[1] == name:
+                if len(w) >= 4 and w[2] == "gasconstant":
+                    return float(w[3])
+                return 0.0
+    raise DeckError(f"fluid '{name}' not found in {path}")
 
 
 # ------------------------------------------------------- boundary profiles
@@ -392,8 +498,17 @@ PRINTDIFF_HEAD = " " + f"{'Tstp':>6}{'Time   ':>13}{'QlIt':>5}{'PpeIt':>7}{'|p1-
     "".join(f"{h:>13}" for h in ("|u1-un|", "|v1-vn|"))
 
 
-def print_diff_row(k: int, time: float, nql: int, nsor: int, dif) -> str:
-    """string.f:PrintDiff row '(1x,i6,e13.6,i4,a1,i6,4(e13.6))' (no thermal)."""
+# with the energy equation: a character*78 rule and four a12 norm headings
+PRINTDIFF_RULE_T = " " + "-" * 40 + " " * 38
+PRINTDIFF_HEAD_T = " " + f"{'Tstp':>6}{'Time   ':>13}{'QlIt':>5}{'PpeIt':>7}{'|p1-pn|':>10}" + \
+    "".join(f"{h:>12}" for h in ("|u1-un|", "|v1-vn|", "|t1-tn|"))
+
+
+def print_diff_row(k: int, time: float, nql: int, nsor: int, dif, thermal: bool = False) -> str:
+    """string.f:PrintDiff row '(1x,i6,e13.6,i4,a1,i6,4(e13.6))' with three
+    norms, or '(1x,i6,e13.6,i4,a1,i6,4(e12.5))' with four (energy equation)."""
     mark = "*" if nql < 0 else " "
-    return (" " + f"{k:6d}" + fortran_e(time, 13, 6) + f"{nql:4d}" + mark + f"{nsor:6d}" +
-            "".join(fortran_e(float(v), 13, 6) for v in dif[:3]))
+    head = " " + f"{k:6d}" + fortran_e(time, 13, 6) + f"{nql:4d}" + mark + f"{nsor:6d}"
+    if thermal:
+        return head + "".join(fortran_e(float(v), 12, 5) for v in dif[:4])
+    return head + "".join(fortran_e(float(v), 13, 6) for v in dif[:3])

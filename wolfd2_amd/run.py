@@ -8,6 +8,10 @@ program flow on the device-resident context:
 2. the time loop of main.f:690-972;
 3. the checkpoint and final restart writes (main.f:1040-1046, 1310-1313).
 
+With thermal_energy the steps carry the energy equation, buoyancy, the
+momentum/energy iterations and the filters (main.f:735-895), and the table has
+the fourth norm |t1-tn|.
+
 It prints the PrintDiff table (string.f:PrintDiff) on standard output and
 stops on divergence (main.f:966-970), as the reference program does.
 """
@@ -35,22 +39,36 @@ def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) 
     s = d.settings
     x, y = deck.read_grid(os.path.join(case_dir, s["grid_file"]))
     ny, nx = x.shape
-    fl = deck.read_fluid(os.path.join(case_dir, s["flPropFn"]), s["flName"], s["tref"])
-    rho, mu = float(fl[1]), float(fl[3])
+    fl_path = os.path.join(case_dir, s["flPropFn"])
+    fl = deck.read_fluid(fl_path, s["flName"], s["tref"])
+    rho, mu, dconduct, prandtl = float(fl[1]), float(fl[3]), float(fl[4]), float(fl[5])
     g = case.compute_grid(x, y, dlref=s["dlref"])
     reg = deck.build_case(d, rho, nx, ny, x=x, y=y, case_dir=case_dir)
     prm = case.Params.from_physical(s["dk"], rho, mu, dlref=s["dlref"], uref=s["uref"], mqiter=s["mqiter"],
                                     qtol=s["qtol"], nPpeSolver=s["nPpeSolver"], msorit=s["msorit"],
                                     sortol=s["sortol"], sorrel=s["sorrel"], cartesian=s["cartesian"])
+    thermal = s["nthermen"] == 1
+    pe = prm.re * prandtl                          # file_manip.f:158
+    th = deck.build_thermal(d, reg, dconduct, pe)
+    opt = case.StepOptions(nthermen=s["nthermen"], neqstate=s["neqstate"], nmeiter=s["nmeiter"],
+                           nfiltu=s["nfiltu"], nfiltv=s["nfiltv"], nfiltt=s["nfiltt"], dmeittol=s["dmeittol"],
+                           fpu=s["fpu"], fpv=s["fpv"], fpt=s["fpt"], pe=pe, uref=s["uref"], densref=rho,
+                           tmax=s["tmax"], tref=s["tref"], rconst=deck.read_gas_constant(fl_path, s["flName"]))
     sol = api.Solver(g, reg, prm)
+    # the small-scale fields of a restart file pass through unchanged
+    carry = {}
     try:
         sol.set_precision(precision)
+        sol.set_thermal(th)
+        sol.set_options(opt)
         if s["nrestart"]:
             r = deck.read_restart(os.path.join(case_dir, s["resReadFl"]))
             if (r["nx"], r["ny"]) != (nx, ny):
                 raise deck.DeckError(f"restart file is {r['nx']}x{r['ny']}, grid is {nx}x{ny}")
             ks, dtime = r["k"], r["dtime"]
             sol.set_fields(*(_embed(r[f], g.ld, g.ncol) for f in ("u", "v", "p")))
+            sol.set_thermal_fields(t=_embed(r["t"], g.ld, g.ncol))
+            carry = {f: r[f] for f in ("pss", "uss", "vss", "tss")}
         else:
             ks, dtime = 0, 0.0
             z = np.zeros((g.ncol, g.ld))
@@ -63,9 +81,10 @@ def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) 
 
         def save(kk, dt):
             u, v, p = sol.get_fields()
+            t, _ = sol.get_thermal_fields()
             S = np.s_[: ny + 2, : nx + 2]
             deck.write_restart(os.path.join(case_dir, s["resWriteFl"]), kk, dt, nx, ny,
-                               {"u": u[S], "v": v[S], "p": p[S]})
+                               {"u": u[S], "v": v[S], "p": p[S], "t": t[S], **carry})
 
         while k < ks + nts and not diverged:
             n = min(chunk, ks + nts - k)
@@ -75,7 +94,7 @@ def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) 
             for m in range(n):
                 k += 1
                 dtime = dtime + prm.dk             # main.f:693
-                ql, nsor, dif = int(tab[m, 0]), int(tab[m, 1]), tab[m, 2:5]
+                ql, nsor, dif = int(tab[m, 0]), int(tab[m, 1]), tab[m, 2:]
                 rows.append([k, dtime, ql, nsor, *dif])
                 if max(dif) > 1.0e12:              # main.f:966-970
                     print("* Solution diverged. Please reduce CFL number.", file=out)
@@ -84,14 +103,15 @@ def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) 
                 if s["lPrntDiff"] == 1 and (k - ks) % s["nPrDfFreq"] == 0:
                     if s["lPrDfBann"] == 1:
                         if ban == 0 or ban >= s["nPrDfBnFq"]:
-                            print(deck.PRINTDIFF_RULE, file=out)
-                            print(deck.PRINTDIFF_HEAD, file=out)
-                            print(deck.PRINTDIFF_RULE, file=out)
+                            rule = deck.PRINTDIFF_RULE_T if thermal else deck.PRINTDIFF_RULE
+                            print(rule, file=out)
+                            print(deck.PRINTDIFF_HEAD_T if thermal else deck.PRINTDIFF_HEAD, file=out)
+                            print(rule, file=out)
                             ban = 1
                         else:
                             ban += 1
                     rdim = dtime * s["dlref"] / s["uref"]
-                    print(deck.print_diff_row(k, rdim, ql, nsor, dif), file=out)
+                    print(deck.print_diff_row(k, rdim, ql, nsor, dif, thermal), file=out)
             if diverged:
                 break
             if s["nreswrite"] == 2 and (k - ks) % s["nWrtRstFq"] == 0:
@@ -99,9 +119,11 @@ def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) 
         if not diverged and s["nreswrite"] == 1:
             save(k, dtime)                         # main.f:1312, k-1 after the loop
         u, v, p = sol.get_fields()
+        t, _ = sol.get_thermal_fields()
     finally:
         sol.close()
-    return {"rows": np.array(rows), "u": u, "v": v, "p": p, "k": k, "dtime": dtime, "diverged": diverged}
+    return {"rows": np.array(rows), "u": u, "v": v, "p": p, "t": t, "k": k, "dtime": dtime,
+            "diverged": diverged}
 
 
 def main(argv=None) -> int:
