@@ -61,6 +61,12 @@ class _Thermal(C.Structure):
     _fields_ = [("nTRgType", _ip), ("nTemBdTp", _ip), ("dTRgVal", _dp), ("dHGSTval", _dp)]
 
 
+class _StepOptions(C.Structure):
+    _fields_ = [(f, C.c_int) for f in ("nthermen", "neqstate", "nmeiter", "nfiltu", "nfiltv", "nfiltt")] + \
+        [(f, C.c_double) for f in ("dmeittol", "fpu", "fpv", "fpt", "pe", "uref", "densref", "tmax", "tref",
+                                   "rconst")]
+
+
 class _Metrics(C.Structure):
     _fields_ = [(nm, _dp) for nm in METRIC_ORDER]
 
@@ -208,6 +214,28 @@ class Oracle:
             assert st == 0, st
             rows.append((it[0], it[1], dif[0], dif[1], dif[2]))
         return np.array(rows, dtype=np.float64)
+
+
+def run_ex(oracle, reg, th, grid, prm, opt, u, v, p, t, d, nsteps: int):
+    """Advance nsteps of main.f:690-972 with step options ``opt``
+    (case.StepOptions) in place; returns rows (ql, ppe, dp, du, dv, dt)."""
+    shape = u.shape
+    work = [np.zeros(shape) for _ in range(7)]
+    dn = np.zeros(shape)
+    it = (C.c_int * 2)()
+    dif = (C.c_double * 4)()
+    rs, ms, ts = oracle.regions(reg, grid.ld), oracle.metrics(grid), oracle.thermal(th)
+    o = _StepOptions(*[getattr(opt, f) for f, _ in _StepOptions._fields_])
+    rows = []
+    for _ in range(nsteps):
+        st = oracle.L.wd2o_step_ex(C.byref(rs), C.byref(ts), C.byref(ms), C.byref(o), C.c_int(int(prm.cartesian)),
+                                   C.c_int(prm.mqiter), C.c_double(prm.qtol), C.c_int(prm.nPpeSolver),
+                                   C.c_int(prm.msorit), C.c_double(prm.sortol), C.c_double(prm.sorrel),
+                                   C.c_double(prm.dk), C.c_double(prm.re), C.c_double(prm.fr),
+                                   _d(d), _d(dn), _d(u), _d(v), _d(p), _d(t), *[_d(w) for w in work], it, dif)
+        assert st == 0, st
+        rows.append((it[0], it[1], dif[0], dif[1], dif[2], dif[3]))
+    return np.array(rows, dtype=np.float64)
 
 
 class Reference:

@@ -1660,3 +1660,79 @@ int wd2o_filter(const wd2o_regions *r, const wd2o_thermal *th, int ncomp, double
     free(qh);
     return 0;
 }
+
+/* One time step of main.f:690-972 with the energy equation, buoyancy,
+ * momentum/energy iterations and filters (wd2o_step_options).  Work arrays
+ * (pn, un, vn, tn, us, vs, ts: ld*(ny+2) doubles each) are caller-owned;
+ * d is the density (EqState's output), dn its time-level-n copy.
+ * out_iter = {nQLiter, nSorConv} of the last iteration; out_dif = the four
+ * PrintDiff norms {|p1-pn|, |u1-un|, |v1-vn|, |t1-tn|}. */
+int wd2o_step_ex(const wd2o_regions *r, const wd2o_thermal *th, const wd2o_metrics *m,
+                 const wd2o_step_options *o, int lCartesGrid, int mqiter, double qtol,
+                 int nPpeSolver, int msorit, double sortol, double sorrel, double dk, double re,
+                 double fr, double *d, double *dn, double *u, double *v, double *p, double *t,
+                 double *pn, double *un, double *vn, double *tn, double *us, double *vs, double *ts,
+                 int *out_iter, double *out_dif)
+{
+    int nx = r->nx, ny = r->ny, ld = r->ld, i, j, l, st, nql = 0, nsor = 0;
+    int nmeiter = o->nmeiter;
+    for (j = 0; j <= ny + 1; j++)
+        for (i = 0; i <= nx + 1; i++) {
+            F(pn, i, j) = F(p, i, j);
+            F(un, i, j) = F(u, i, j);
+            F(vn, i, j) = F(v, i, j);
+            F(tn, i, j) = F(t, i, j);
+            F(dn, i, j) = F(d, i, j);
+        }
+    if (o->nthermen != 1 && nmeiter > 0) nmeiter = 1;   /* main.f:735 */
+    for (l = 1; l <= nmeiter; l++) {
+        double dif1, dif2, dif3, difmax;
+        for (j = 0; j <= ny + 1; j++)
+            for (i = 0; i <= nx + 1; i++) {
+                F(us, i, j) = F(u, i, j);
+                F(vs, i, j) = F(v, i, j);
+                F(ts, i, j) = F(t, i, j);
+            }
+        nql = wd2o_naux_momentum(r, mqiter, dk, re, fr, qtol, m, d, dn, un, vn, us, vs);
+        if (nql <= ERR_FLAG) return nql;
+        if (o->nfiltu == 1 && (st = wd2o_filter(r, th, VU, o->fpu, us)) != 0) return st;
+        if (o->nfiltv == 1 && (st = wd2o_filter(r, th, VV, o->fpv, vs)) != 0) return st;
+        if ((st = wd2o_vel_bound_cond(r, us, vs)) != 0) return st;
+        if ((st = wd2o_pres_bound_cond(r, p)) != 0) return st;
+        if ((st = wd2o_ppe(r, lCartesGrid, nPpeSolver, msorit, &nsor, dk, sortol, sorrel, m, us, vs, p)) != 0)
+            return st;
+        if ((st = wd2o_pres_bound_cond(r, p)) != 0) return st;
+        if ((st = wd2o_project(r, dk, m, p, us, vs)) != 0) return st;
+        if ((st = wd2o_vel_bound_cond(r, us, vs)) != 0) return st;
+        if ((st = wd2o_pres_bound_cond(r, p)) != 0) return st;
+        if (o->nthermen == 1 && (st = wd2o_therm_energy(r, th, dk, o->pe, m, un, vn, us, vs, tn, ts)) != 0)
+            return st;
+        if (o->neqstate == 1)
+            wd2o_eq_state(nx, ny, ld, o->uref, o->densref, o->tmax, o->tref, o->rconst, p, ts, d);
+        dif1 = wd2o_diff_max_norm(nx, ny, ld, u, us);
+        dif2 = wd2o_diff_max_norm(nx, ny, ld, v, vs);
+        dif3 = wd2o_diff_max_norm(nx, ny, ld, t, ts);
+        for (j = 0; j <= ny + 1; j++)
+            for (i = 0; i <= nx + 1; i++) {
+                F(u, i, j) = F(us, i, j);
+                F(v, i, j) = F(vs, i, j);
+                F(t, i, j) = F(ts, i, j);
+            }
+        /* main.f:873-877, Fortran MAX of the three */
+        difmax = dif1;
+        if (dif2 > difmax) difmax = dif2;
+        if (dif3 > difmax) difmax = dif3;
+        if (l > 1 && difmax < o->dmeittol) break;
+    }
+    if (o->nthermen == 1 && o->nfiltt == 1 && (st = wd2o_filter(r, th, VT, o->fpt, t)) != 0) return st;
+    if ((st = wd2o_vel_bound_cond(r, u, v)) != 0) return st;
+    if ((st = wd2o_pres_bound_cond(r, p)) != 0) return st;
+    if (o->nthermen == 1 && (st = wd2o_temp_bound_cond(r, th, t)) != 0) return st;
+    out_iter[0] = nql;
+    out_iter[1] = nsor;
+    out_dif[0] = wd2o_diff_max_norm(nx, ny, ld, pn, p);
+    out_dif[1] = wd2o_diff_max_norm(nx, ny, ld, un, u);
+    out_dif[2] = wd2o_diff_max_norm(nx, ny, ld, vn, v);
+    out_dif[3] = wd2o_diff_max_norm(nx, ny, ld, tn, t);
+    return 0;
+}

@@ -172,6 +172,22 @@ void wd2o_eq_state(int nx, int ny, int ld, double uref, double densref, double t
  * ncomp, where the reference STOPs. */
 int wd2o_filter(const wd2o_regions *r, const wd2o_thermal *th, int ncomp, double fp, double *qu);
 
+/* Step options of main.f:690-972 beyond cold flow (the fields of
+ * wd2_step_options in include/wolfd2_amd.h). */
+typedef struct {
+    int nthermen, neqstate, nmeiter, nfiltu, nfiltv, nfiltt;
+    double dmeittol, fpu, fpv, fpt, pe, uref, densref, tmax, tref, rconst;
+} wd2o_step_options;
+
+/* main.f:690-972 with the energy equation, buoyancy, momentum/energy
+ * iterations and filters; see wd2_oracle.c for the arguments. */
+int wd2o_step_ex(const wd2o_regions *r, const wd2o_thermal *th, const wd2o_metrics *m,
+                 const wd2o_step_options *o, int lCartesGrid, int mqiter, double qtol,
+                 int nPpeSolver, int msorit, double sortol, double sorrel, double dk, double re,
+                 double fr, double *d, double *dn, double *u, double *v, double *p, double *t,
+                 double *pn, double *un, double *vn, double *tn, double *us, double *vs, double *ts,
+                 int *out_iter, double *out_dif);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,3 +97,54 @@ def test_thermal_bad_flags_raise():
     th = case.make_thermal(reg, walls=[(1, 1, "west", "temper", 1.0)])
     api.temp_bound_cond(reg, th, t)
     assert np.all(t[2:ny + 1, 1] == 2.0)
+
+
+def _thermal_context_case(nx, ny):
+    x, y = case.skewed_nodes(nx, ny, seed=2)
+    g = case.compute_grid(x, y)
+    reg = case.make_regions(nx, ny, nreg=(3, 2), i_borders=(nx // 3, 2 * nx // 3), j_borders=(ny // 2,),
+                            walls=[(2, 2, "north", "no_slip", 1.0, 0.0)])
+    th = case.make_thermal(reg, heat=[(2, 1, 3.0)], fixed=[(3, 2, 0.2)],
+                           walls=[(1, 1, "west", "temper", 1.0), (1, 2, "west", "temper", 1.0)])
+    prm = case.Params.from_physical(5e-4, 1.0, 1e-2, nPpeSolver=5, cartesian=False, msorit=300, sorrel=1.6)
+    opt = case.StepOptions(nthermen=1, neqstate=1, nmeiter=3, dmeittol=1e-7, nfiltu=1, nfiltv=1, nfiltt=1,
+                           fpu=300.0, fpv=300.0, fpt=40.0, pe=prm.re * 0.7, densref=1.0, tmax=320.0, tref=300.0,
+                           rconst=287.0)
+    return g, reg, th, prm, opt
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_thermal_context_steps_vs_oracle(oracle, precision):
+    """Device-resident steps with the energy equation, buoyancy, filters and
+    up to three momentum/energy iterations on a skewed 3x2-region grid: FP64
+    bit for bit against the oracle's step; mixed precision within 1e-6
+    relative L2 per field."""
+    from pyoracle import run_ex
+    nx, ny = 129, 97
+    g, reg, th, prm, opt = _thermal_context_case(nx, ny)
+    print(f"thermal context {nx}x{ny} {precision}", flush=True)
+    sol = api.Solver(g, reg, prm)
+    sol.set_precision(precision)
+    sol.set_thermal(th)
+    sol.set_options(opt)
+    z = np.zeros((g.ncol, g.ld))
+    sol.set_fields(z, z, z)
+    sol.cold_start()
+    tg = sol.step(8)
+    ug, vg, pg = sol.get_fields()
+    t_g, d_g = sol.get_thermal_fields()
+    sol.close()
+    u, v, p, t, d = (np.zeros((g.ncol, g.ld)) for _ in range(5))
+    oracle.cold_start(reg, g, prm, u, v, p)
+    to = run_ex(oracle, reg, th, g, prm, opt, u, v, p, t, d, 8)
+    assert np.abs(t).max() > 0.1 and np.abs(d).max() > 1e-3    # the energy equation and buoyancy act
+    if precision == "fp64":
+        assert np.array_equal(tg, to)
+        for name, a, b in zip(("u", "v", "p", "t", "d"), (ug, vg, pg, t_g, d_g), (u, v, p, t, d)):
+            assert np.array_equal(a, b), name
+    else:
+        S = np.s_[: g.ny + 2, : g.nx + 2]
+        for name, a, b in zip(("u", "v", "p", "t"), (ug, vg, pg, t_g), (u, v, p, t)):
+            err = np.linalg.norm(a[S] - b[S]) / np.linalg.norm(b[S])
+            print(f"  {name}: rel L2 {err:.2e}", flush=True)
+            assert err <= 1e-6, name

@@ -5,6 +5,8 @@ Pinned against:
 * the golden fixtures the reference's own compiled subroutines wrote
   (tests/golden/thermal_*.npz, tests/golden/make_fixtures.py), bit for bit;
 * the live compiled reference (oracle/_ref) on further seeded inputs;
+* the reference program's own heated-cavity run (energy equation, buoyancy,
+  filters, momentum/energy iterations): PrintDiff lines and restart fields;
 * what the mathematics fixes: a uniform temperature at rest stays uniform,
   the heat generation term alone raises the temperature by exactly the ADI
   solves of dk q / 2, the filter leaves a constant field unchanged and
@@ -114,11 +116,10 @@ def test_uniform_temperature_at_rest_stays_uniform(oracle):
 
 
 def test_heat_generation_raises_by_the_split_solves(oracle):
-    """At rest with an adiabatic box and t = tn = 0, the first split solves
-    (I + rkj A_x) x1 = dk q / 2 and the second (I + rkj A_y) x2 = x1: for a
-    uniform q the chains keep x1, x2 uniform, so t rises by exactly dk q / 2
-    where the Neumann rows conserve it (the interior away from the box ends
-    of each chain is checked by the dense solve)."""
+    """At rest with an adiabatic box and t = tn = 0 the explicit terms vanish:
+    the first split solves (I + rkj A_x) x1 = dk q / 2 and the second
+    (I + rkj A_y) x2 = x1, as one chain each in the reference's row order.
+    Dense solves of those two chains give the temperature rise."""
     nx, ny = 9, 7
     g, reg = _box(nx, ny)
     q = 2.0
@@ -184,3 +185,52 @@ def test_eq_state_reference_state_is_zero(oracle):
     t[:] = 1.0                  # tmax: rho_ref Tref / Tmax - 1
     oracle.eq_state(g, 1.0, 1.2, 400.0, 300.0, 287.0, p, t, den)
     np.testing.assert_allclose(den[2:g.ny + 1, 2:g.nx + 1], 300.0 / 400.0 - 1.0, rtol=1e-15)
+
+
+def _thermal_case(tmp_path):
+    import refdecks
+    from wolfd2_amd.run import Case
+    fx = helpers.load("thermrun_therm_cavity_32.npz")
+    refdecks.write_grid(str(tmp_path / "grid.xyz"), fx["x"], fx["y"])
+    refdecks.write_fluid_table(str(tmp_path / "fluid.dat"), "fl01", float(fx["rho"]), float(fx["mu"]),
+                               float(fx["gas_constant"]))
+    (tmp_path / "input.dat").write_text(str(fx["deck"]))
+    return fx, Case(str(tmp_path))
+
+
+def test_oracle_thermal_run_matches_reference_program(oracle, tmp_path):
+    """The oracle's step with the energy equation (wd2o_step_ex) on the heated
+    cavity deck: the reference program's PrintDiff lines and restart fields
+    u, v, p, t, bit for bit."""
+    from pyoracle import run_ex
+    from wolfd2_amd import deck
+    fx, cs = _thermal_case(tmp_path)
+    g = cs.grid
+    u, v, p, t, d = (np.zeros((g.ncol, g.ld)) for _ in range(5))
+    oracle.cold_start(cs.regions, g, cs.params, u, v, p)
+    tab = run_ex(oracle, cs.regions, cs.thermal, g, cs.params, cs.options, u, v, p, t, d, int(fx["nts"]))
+    dt = 0.0
+    lines = []
+    for k, row in enumerate(tab, start=1):
+        dt = dt + cs.params.dk
+        lines.append(deck.print_diff_row(k, dt, int(row[0]), int(row[1]), row[2:], thermal=True))
+    assert lines == str(fx["diff_lines"]).split("\n")
+    S = np.s_[: g.ny + 2, : g.nx + 2]
+    for f, a in zip("uvpt", (u, v, p, t)):
+        assert np.array_equal(a[S], fx[f]), f
+
+
+def test_oracle_step_ex_cold_flow_equals_step(oracle):
+    """Without the energy equation and filters, wd2o_step_ex is wd2o_step."""
+    from pyoracle import run_ex
+    fx = helpers.load("run_chan_64x16.npz")
+    g, reg, prm = helpers.run_case(fx)
+    th = case.make_thermal(reg)
+    a = [np.zeros((g.ncol, g.ld)) for _ in range(3)]
+    oracle.cold_start(reg, g, prm, *a)
+    b = [x.copy() for x in a]
+    t1 = oracle.run(reg, g, prm, *a, 30)
+    t2 = run_ex(oracle, reg, th, g, prm, case.StepOptions(nmeiter=4), *b, np.zeros_like(a[0]), np.zeros_like(a[0]), 30)
+    assert np.array_equal(t1, t2[:, :5]) and np.all(t2[:, 5] == 0.0)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

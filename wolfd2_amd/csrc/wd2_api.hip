@@ -82,6 +82,10 @@ struct wd2_ctx {
     double *store = nullptr;
     double *u, *v, *p, *un, *vn, *pn, *us, *vs, *dus, *dvs, *d, *dn, *div, *rhs, *beff, *pscr;
     double *t, *tn, *ts, *qh;      // temperature levels and the filter's copy
+    // ThermEnergy's face coefficients: buffers of their own, since the mixed
+    // path's FP32 momentum temporaries alias the FP64 ones and read the
+    // entries they never write as zero
+    ThermTemps tt{};
     double *pc[5];
     MomTemps T{};
     Chain ch{};
@@ -172,7 +176,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
                              hipHostMallocDefault));
         HIPCHK(hipEventCreateWithFlags(&c->sev[k], hipEventDisableTiming));
     }
-    const int NFIELDS = 16 + 4 + 5 + 13;
+    const int NFIELDS = 16 + 4 + 4 + 5 + 13;
     const size_t nchain = (size_t)nx * ny + 64;
     const size_t total = NFIELDS * c->nf + 5 * nchain;
     HIPCHK(hipMalloc(&c->store, total * sizeof(double)));
@@ -182,6 +186,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     double **f[] = {&c->u, &c->v, &c->p, &c->un, &c->vn, &c->pn, &c->us, &c->vs, &c->dus, &c->dvs,
                     &c->d, &c->dn, &c->div, &c->rhs, &c->beff, &c->pscr, &c->t, &c->tn, &c->ts, &c->qh};
     for (double **pp : f) *pp = take(c->nf);
+    c->tt = ThermTemps{take(c->nf), take(c->nf), take(c->nf), take(c->nf)};
     for (int k = 0; k < 5; k++) c->pc[k] = take(c->nf);
     double **t[] = {&c->T.cj1, &c->T.cj2, &c->T.c1s, &c->T.c2s, &c->T.c1n, &c->T.c2n, &c->T.cnvs,
                     &c->T.cnvn, &c->T.difs, &c->T.difn, &c->T.cps, &c->T.cpn, &c->T.cpj};
@@ -644,7 +649,6 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
     launch_copy(ld, 0, nx + 1, 0, ny + 1, c->d, c->dn, s);
     // main.f:735: one iteration without the energy equation
     const int nme = (!therm && O.nmeiter > 0) ? 1 : O.nmeiter;
-    const ThermTemps tt{c->T.cj1, c->T.cj2, c->T.c1n, c->T.c2n};
     int st;
     for (int l = 1; l <= nme; l++) {
         // main.f:741-747
@@ -665,7 +669,7 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
         launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
         // main.f:840-855
         if (therm)
-            launch_therm_energy(c->R, c->TH, ld, P->dk, O.pe, c->M, c->un, c->vn, c->us, c->vs, c->tn, c->ts, tt,
+            launch_therm_energy(c->R, c->TH, ld, P->dk, O.pe, c->M, c->un, c->vn, c->us, c->vs, c->tn, c->ts, c->tt,
                                 c->ch, c->tw, &c->sc->err, s);
         if (O.neqstate == 1)
             launch_eq_state(nx, ny, ld, O.uref, O.densref, O.tmax, O.tref, O.rconst, c->p, therm ? c->ts : c->t,
@@ -1048,9 +1052,8 @@ int wd2_therm_energy(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg
     if ((st = up(c, un, ld, c->un)) || (st = up(c, vn, ld, c->vn)) || (st = up(c, u, ld, c->us)) ||
         (st = up(c, v, ld, c->vs)) || (st = up(c, tn, ld, c->tn)) || (st = up(c, t, ld, c->t)))
         return st;
-    const ThermTemps w{c->T.cj1, c->T.cj2, c->T.c1n, c->T.c2n};
-    launch_therm_energy(c->R, c->TH, c->ld, dk, pe, c->M, c->un, c->vn, c->us, c->vs, c->tn, c->t, w, c->ch, c->tw,
-                        &c->sc->err, c->s);
+    launch_therm_energy(c->R, c->TH, c->ld, dk, pe, c->M, c->un, c->vn, c->us, c->vs, c->tn, c->t, c->tt, c->ch,
+                        c->tw, &c->sc->err, c->s);
     if ((st = check_err(c))) return st;
     return down(c, c->t, t, ld);
 }

@@ -32,28 +32,42 @@ def _embed(a: np.ndarray, ld: int, ncol: int) -> np.ndarray:
     return f
 
 
+class Case:
+    """Everything a run of a case directory needs: the deck's settings, the
+    grid and its metrics, the region and thermal tables, the run parameters
+    and step options (file_manip.f:ReadParam, parse.f, grid.f, bound_cond.f)."""
+
+    def __init__(self, case_dir: str):
+        d = deck.read_input(os.path.join(case_dir, "input.dat"))
+        s = d.settings
+        x, y = deck.read_grid(os.path.join(case_dir, s["grid_file"]))
+        ny, nx = x.shape
+        fl_path = os.path.join(case_dir, s["flPropFn"])
+        fl = deck.read_fluid(fl_path, s["flName"], s["tref"])
+        rho, mu, dconduct, prandtl = float(fl[1]), float(fl[3]), float(fl[4]), float(fl[5])
+        self.deck, self.settings, self.nx, self.ny = d, s, nx, ny
+        self.grid = case.compute_grid(x, y, dlref=s["dlref"])
+        self.regions = deck.build_case(d, rho, nx, ny, x=x, y=y, case_dir=case_dir)
+        self.params = case.Params.from_physical(s["dk"], rho, mu, dlref=s["dlref"], uref=s["uref"],
+                                                mqiter=s["mqiter"], qtol=s["qtol"], nPpeSolver=s["nPpeSolver"],
+                                                msorit=s["msorit"], sortol=s["sortol"], sorrel=s["sorrel"],
+                                                cartesian=s["cartesian"])
+        pe = self.params.re * prandtl                  # file_manip.f:158
+        self.thermal = deck.build_thermal(d, self.regions, dconduct, pe)
+        self.options = case.StepOptions(
+            nthermen=s["nthermen"], neqstate=s["neqstate"], nmeiter=s["nmeiter"], nfiltu=s["nfiltu"],
+            nfiltv=s["nfiltv"], nfiltt=s["nfiltt"], dmeittol=s["dmeittol"], fpu=s["fpu"], fpv=s["fpv"],
+            fpt=s["fpt"], pe=pe, uref=s["uref"], densref=rho, tmax=s["tmax"], tref=s["tref"],
+            rconst=deck.read_gas_constant(fl_path, s["flName"]))
+
+
 def run_case(case_dir: str, precision: str = "fp64", out=None, chunk: int = 50) -> dict:
     """Run the deck in case_dir; returns the PrintDiff rows and final state."""
     out = out or sys.stdout
-    d = deck.read_input(os.path.join(case_dir, "input.dat"))
-    s = d.settings
-    x, y = deck.read_grid(os.path.join(case_dir, s["grid_file"]))
-    ny, nx = x.shape
-    fl_path = os.path.join(case_dir, s["flPropFn"])
-    fl = deck.read_fluid(fl_path, s["flName"], s["tref"])
-    rho, mu, dconduct, prandtl = float(fl[1]), float(fl[3]), float(fl[4]), float(fl[5])
-    g = case.compute_grid(x, y, dlref=s["dlref"])
-    reg = deck.build_case(d, rho, nx, ny, x=x, y=y, case_dir=case_dir)
-    prm = case.Params.from_physical(s["dk"], rho, mu, dlref=s["dlref"], uref=s["uref"], mqiter=s["mqiter"],
-                                    qtol=s["qtol"], nPpeSolver=s["nPpeSolver"], msorit=s["msorit"],
-                                    sortol=s["sortol"], sorrel=s["sorrel"], cartesian=s["cartesian"])
+    cs = Case(case_dir)
+    s, nx, ny = cs.settings, cs.nx, cs.ny
+    g, reg, prm, th, opt = cs.grid, cs.regions, cs.params, cs.thermal, cs.options
     thermal = s["nthermen"] == 1
-    pe = prm.re * prandtl                          # file_manip.f:158
-    th = deck.build_thermal(d, reg, dconduct, pe)
-    opt = case.StepOptions(nthermen=s["nthermen"], neqstate=s["neqstate"], nmeiter=s["nmeiter"],
-                           nfiltu=s["nfiltu"], nfiltv=s["nfiltv"], nfiltt=s["nfiltt"], dmeittol=s["dmeittol"],
-                           fpu=s["fpu"], fpv=s["fpv"], fpt=s["fpt"], pe=pe, uref=s["uref"], densref=rho,
-                           tmax=s["tmax"], tref=s["tref"], rconst=deck.read_gas_constant(fl_path, s["flName"]))
     sol = api.Solver(g, reg, prm)
     # the small-scale fields of a restart file pass through unchanged
     carry = {}
