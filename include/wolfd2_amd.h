@@ -50,6 +50,12 @@ int wd2_set_tiled_sor(int enable);
  * default); 0 selects the tiled wavefront, one launch per sweep.  Both give
  * the reference's iterates bit for bit.  Returns the previous setting. */
 int wd2_set_lex_rows(int enable);
+/* Non-zero (default): on a uniform Cartesian grid, where rau and rgv each hold
+ * one value over the entries the red-black SOR reads, the tiled SOR kernels
+ * take the two values as arguments instead of loading the arrays (the same
+ * values, so the same bits); 0 always loads them.  Returns the previous
+ * setting. */
+int wd2_set_uniform_coefficients(int enable);
 /* Non-zero (default): print the reference's warnings on standard output
  * (Ppe non-convergence, bad solver flag).  Returns the previous setting. */
 int wd2_set_verbose(int enable);

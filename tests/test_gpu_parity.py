@@ -199,3 +199,62 @@ def test_full_size_cavity_steps_bitwise_vs_oracle(oracle, solver, sor_tiling):
     to = oracle.run(reg, g, prm, u, v, p, 2)
     assert np.array_equal(tg[:, :2], to[:, :2])
     assert np.array_equal(u1, u) and np.array_equal(v1, v) and np.array_equal(p1, p)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_uniform_coefficient_path_equals_array_path(precision):
+    """On a uniform Cartesian grid the tiled red-black SOR kernels take rau and
+    rgv as two values; loading the arrays instead gives the same bits."""
+    from wolfd2_amd._lib import lib
+    nx = ny = 1025
+    x, y = case.uniform_nodes(nx, ny)
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(nx, ny)
+    prm = case.Params.from_physical(2e-4, 1.0, 1e-3, nPpeSolver=5, msorit=60)
+    print(f"uniform vs array coefficients {precision}", flush=True)
+    out = []
+    old = lib().wd2_set_uniform_coefficients(1)
+    try:
+        for flag in (1, 0):
+            lib().wd2_set_uniform_coefficients(flag)
+            sol = api.Solver(g, reg, prm)
+            sol.set_precision(precision)
+            z = np.zeros((g.ncol, g.ld))
+            sol.set_fields(z, z, z)
+            sol.cold_start()
+            tab = sol.step(2)
+            out.append((tab, *sol.get_fields()))
+            sol.close()
+    finally:
+        lib().wd2_set_uniform_coefficients(old)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+def test_stretched_cartesian_grid_tiled_sor_vs_oracle(oracle):
+    """A stretched Cartesian grid (rau, rgv vary, so the kernels load them):
+    tiled red-black SOR steps against the oracle, bit for bit."""
+    nx, ny = 161, 129
+    s = np.arange(nx) / (nx - 1.0)
+    t = np.arange(ny) / (ny - 1.0)
+    xs = s + 0.12 * np.sin(np.pi * s) * s
+    yt = t - 0.08 * np.sin(2 * np.pi * t)
+    x = np.broadcast_to(xs[None, :], (ny, nx)).copy()
+    y = np.broadcast_to(yt[:, None], (ny, nx)).copy()
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(nx, ny)
+    prm = case.Params.from_physical(5e-4, 1.0, 1e-2, nPpeSolver=5, msorit=200, sorrel=1.7, cartesian=True)
+    print("stretched Cartesian grid, tiled SOR", flush=True)
+    sol = api.Solver(g, reg, prm)
+    z = np.zeros((g.ncol, g.ld))
+    sol.set_fields(z, z, z)
+    sol.cold_start()
+    tg = sol.step(3)
+    fg = sol.get_fields()
+    sol.close()
+    u, v, p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
+    oracle.cold_start(reg, g, prm, u, v, p)
+    to = oracle.run(reg, g, prm, u, v, p, 3)
+    assert np.array_equal(tg, to)
+    for a, b in zip(fg, (u, v, p)):
+        assert np.array_equal(a, b)

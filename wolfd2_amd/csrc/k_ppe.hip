@@ -222,7 +222,8 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
                                                     double *__restrict__ p_out, unsigned *__restrict__ masks,
                                                     const unsigned *__restrict__ prev_masks, int prev_nsweep,
                                                     int prev_m0, int prev_launch, double sortol, int msorit,
-                                                    int ntx, int ntiles, Scalars *sc, int force) {
+                                                    int ntx, int ntiles, Scalars *sc, int force, int uni,
+                                                    double rau_c, double rgv_c) {
     if (!force && sc->sor_done) return;
     // p split by colour: PC[c][y*TK + k] is the point of colour c ((i+j) & 1)
     // in pair k of tile row y, so that the neighbours a colour reads are
@@ -273,10 +274,16 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
         double aw = 0.0;
         if (upd0 || upd1) {
             bv = *reinterpret_cast<const double2 *>(b + WD2_IX(i, j));
-            g0 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j - 1));
-            g1 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j));
-            av = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
-            aw = rau[WD2_IX(i - 1, j)];
+            if (uni) {   // every rau / rgv entry an updated point reads has this value
+                g0 = g1 = make_double2(rgv_c, rgv_c);
+                av = make_double2(rau_c, rau_c);
+                aw = rau_c;
+            } else {
+                g0 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j - 1));
+                g1 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j));
+                av = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
+                aw = rau[WD2_IX(i - 1, j)];
+            }
         }
         int f = 0;
 #pragma unroll
@@ -379,12 +386,13 @@ int sor_tiled_sweeps() { return KT; }
 void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
                       const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
                       int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
-                      int force, hipStream_t s) {
+                      int force, const double *uni, hipStream_t s) {
     int ntx = 0;
     const int nt = sor_tiled_tiles(nx, ny, &ntx);
     const int grid = 8 * ((nt + 7) / 8);
     hipLaunchKernelGGL(k_sor_tiled, dim3(grid), dim3(TP_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel, nsweep, p_in,
-                       p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit, ntx, nt, sc, force);
+                       p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit, ntx, nt, sc, force,
+                       uni ? 1 : 0, uni ? uni[0] : 0.0, uni ? uni[1] : 0.0);
 }
 
 void launch_sor_tiled_check(int ntiles, int nsweep, int m0, int launch, int msorit, const unsigned *masks,

@@ -33,11 +33,32 @@ namespace wd2 {
 
 constexpr int KM = 8, MH = 2 * KM, MW = 96, MT = MW - 2 * MH;   // 64 owned per side
 constexpr int MX_T = 768;                                         // 12 wavefronts
-constexpr int MX_PAIRS = MW * MW / 2 / MX_T;                      // 6 pairs per lane
 constexpr int MLD = MW + 1;                                       // padded LDS row of p
 constexpr int MK = MW / 2;                                        // pairs per tile row
-static_assert(MW * MW / 2 == MX_PAIRS * MX_T, "tile pairs must divide evenly");
+constexpr int SR = 6;                                             // rows of a lane's strip
+constexpr int NB = MW / SR;                                       // strips per pair column
+// Row stride of the colour-split correction: 6 strips of rows apart (the
+// lanes k = 32..47 of one strip and k = 0..15 of the next share a half
+// wavefront) the two halves fall on disjoint LDS banks: 6 * ES = 16 mod 32.
+constexpr int ES = 56;
+static_assert(MK * NB == MX_T, "one lane per (pair column, strip)");
+static_assert(NB * SR == MW, "strips cover the tile");
+static_assert((SR * ES) % 32 == 16 && ES >= MK, "conflict-free strip rows");
 
+// Lane t owns pair column k = t % MK (points x = 2k, 2k+1) on the strip of
+// rows y = SR*(t / MK) .. +SR-1.  Tile origins are even, so the colour
+// (i + j) & 1 of the left point of row y is y & 1 = r & 1 (r the row within
+// the strip): which point of a pair a colour updates is known at compile time.
+// A point's four neighbours are of the other colour: its pair partner and,
+// inside the strip, its north and south neighbours are in the lane's own
+// registers; the other W/E neighbour (lane k -+ 1) and the N/S neighbours at
+// the strip ends are read from LDS.  The FP64 residual reads p the same way.
+//
+// UNI: every rau and rgv entry the updated points read (rau on i = 1..nx,
+// j = 2..ny; rgv on i = 2..nx, j = 1..ny) equals rau_c / rgv_c (a uniform
+// Cartesian grid): the two values replace the loads (the same values, so the
+// same arithmetic), and the sweep coefficients are the same at every point.
+template <bool UNI>
 __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, const double *__restrict__ rau,
                                                        const double *__restrict__ rgv, const double *__restrict__ b,
                                                        double sorrel, int nsweep,
@@ -45,13 +66,13 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
                                                        unsigned *__restrict__ masks,
                                                        const unsigned *__restrict__ prev_masks, int prev_nsweep,
                                                        int prev_m0, int prev_launch, double sortol, int msorit,
-                                                       int ntx, int ntiles, Scalars *sc, int force) {
+                                                       int ntx, int ntiles, Scalars *sc, int force, double rau_c,
+                                                       double rgv_c) {
     if (!force && sc->sor_done) return;
     __shared__ double P[MW * MLD];
-    // the correction e split by colour: EC[c][y*MK + k] is the point of colour
-    // c ((i+j) & 1) in pair k of tile row y, so that the neighbours a colour
-    // reads are unit-stride across the lanes of a wavefront
-    __shared__ float EC[2][MW * MK];
+    // the correction e split by colour: EC[c][y*ES + k] is the point of colour
+    // c ((i+j) & 1) in pair k of tile row y
+    __shared__ float EC[2][MW * ES];
     __shared__ unsigned sh[MX_T / 64];
     if (!force && prev_nsweep > 0) {
         int conv = 0;
@@ -75,76 +96,111 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     const int TI = tile % ntx, TJ = tile / ntx;
     const int i0 = 2 + TI * MT - MH, j0 = 2 + TJ * MT - MH;   // tile origin (even: i0 + x is 16-byte aligned)
     const int t = threadIdx.x;
-#define QX(r) (2 * ((t + MX_T * (r)) % MK))
-#define QY(r) ((t + MX_T * (r)) / MK)
-#define QB(r) (t + MX_T * (r))   // y*MK + k of the lane's pair r
+    const int k = t % MK, y0 = SR * (t / MK), x = 2 * k;
+    const int i = i0 + x;
 
-    // p of the tile into LDS (FP64; 0 outside the array) as aligned pairs, e = 0
+    // which points of the strip are updated / owned (bit 2r + h)
+    unsigned upd = 0, own = 0;
 #pragma unroll
-    for (int r = 0; r < MX_PAIRS; r++) {
-        const int x = QX(r), y = QY(r);
-        const int c = y * MLD + x;
-        const double2 pv = load_pair(p_in, ld, nx, ny, i0 + x, j0 + y);
-        P[c] = pv.x;
-        P[c + 1] = pv.y;
-        EC[0][QB(r)] = 0.f;
-        EC[1][QB(r)] = 0.f;
-    }
-    __syncthreads();
-
-    // FP64 residual and FP32 coefficients of the lane's points
-    float cr[MX_PAIRS][2], c1[MX_PAIRS][2], c5[MX_PAIRS][2], ci3[MX_PAIRS][2];
-    float cw[MX_PAIRS], cc[MX_PAIRS], ce[MX_PAIRS];   // rau at i-1, i, i+1 of the pair
-    int flags[MX_PAIRS];
-#pragma unroll
-    for (int r = 0; r < MX_PAIRS; r++) {
-        const int x = QX(r), y = QY(r);
-        const int i = i0 + x, j = j0 + y;
-        const int c0 = y * MLD + x;
-        int f = 0;
+    for (int r = 0; r < SR; r++) {
+        const int y = y0 + r, j = j0 + y;
         const bool rowin = j >= 2 && j <= ny && y >= 1 && y <= MW - 2;
-        const bool upd0 = rowin && i >= 2 && i <= nx && x >= 1;
-        const bool upd1 = rowin && i + 1 >= 2 && i + 1 <= nx && x + 1 <= MW - 2;
-        double ra[3] = {0.0, 0.0, 0.0};
-        double2 bv = make_double2(0.0, 0.0), g0 = bv, g1 = bv;
-        if (upd0 || upd1) {
-            // i >= 1 and i + 1 <= nx + 1 here: the pairs are inside the array
-            ra[0] = rau[WD2_IX(i - 1, j)];
-            const double2 av = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
-            ra[1] = av.x;
-            ra[2] = av.y;
-            bv = *reinterpret_cast<const double2 *>(b + WD2_IX(i, j));
-            g0 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j - 1));
-            g1 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j));
-        }
-        cw[r] = (float)ra[0];
-        cc[r] = (float)ra[1];
-        ce[r] = (float)ra[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const bool upd = h ? upd1 : upd0;
-            const bool own = upd && x + h >= MH && x + h < MH + MT && y >= MH && y < MH + MT;
-            float rr = 0.f, a1f = 0.f, a5f = 0.f, i3f = 0.f;
-            if (upd) {
-                const double a1 = h ? g0.y : g0.x, a5 = h ? g1.y : g1.x;
-                const double a2 = ra[h], a4 = ra[h + 1];
-                const double a3 = -a4 - a2 - a5 - a1;
-                const int c = c0 + h;
-                const double res = (h ? bv.y : bv.x) - a1 * P[c - MLD] - a2 * P[c - 1] - a3 * P[c] -
-                                   a4 * P[c + 1] - a5 * P[c + MLD];
-                rr = (float)res;
-                a1f = (float)a1;
-                a5f = (float)a5;
-                i3f = 1.0f / (float)a3;   // only scales the FP32 correction sweeps
-            }
-            cr[r][h] = rr; c1[r][h] = a1f; c5[r][h] = a5f; ci3[r][h] = i3f;
-            f |= (upd ? 1 : 0) << h;
-            f |= (own ? 1 : 0) << (2 + h);
+            const bool u = rowin && i + h >= 2 && i + h <= nx && x + h >= 1 && x + h <= MW - 2;
+            const bool o = u && x + h >= MH && x + h < MH + MT && y >= MH && y < MH + MT;
+            upd |= (u ? 1u : 0u) << (2 * r + h);
+            own |= (o ? 1u : 0u) << (2 * r + h);
         }
-        f |= ((i + j) & 1) << 4;   // black iff (i + j) even
-        flags[r] = f;
     }
 
+    // All loads are issued before the barrier, so their latencies overlap:
+    // p of the strip (into registers and the LDS tile; 0 outside the array),
+    // b and, unless UNI, rau of the rows with points to update, and the rgv
+    // pairs of the strip's rows and the row below (a row's a1 is the row
+    // below's a5).
+    double2 pr[SR], bvr[SR], rav[SR], g[SR + 1];
+    double ram[SR];
+#pragma unroll
+    for (int r = 0; r < SR; r++) {
+        const int j = j0 + y0 + r;
+        pr[r] = load_pair(p_in, ld, nx, ny, i, j);
+        bvr[r] = make_double2(0.0, 0.0);
+        ram[r] = rau_c;
+        rav[r] = make_double2(rau_c, rau_c);
+        if ((upd >> (2 * r)) & 3) {
+            // i >= 1 and i + 1 <= nx + 1 here: the pairs are inside the array
+            bvr[r] = *reinterpret_cast<const double2 *>(b + WD2_IX(i, j));
+            if (!UNI) {
+                ram[r] = rau[WD2_IX(i - 1, j)];
+                rav[r] = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q <= SR; q++)
+        g[q] = UNI ? make_double2(rgv_c, rgv_c) : load_pair(rgv, ld, nx, ny, i, j0 + y0 + q - 1);
+#pragma unroll
+    for (int r = 0; r < SR; r++) {
+        const int c = (y0 + r) * MLD + x;
+        P[c] = pr[r].x;
+        P[c + 1] = pr[r].y;
+        EC[0][(y0 + r) * ES + k] = 0.f;
+        EC[1][(y0 + r) * ES + k] = 0.f;
+    }
+    __syncthreads();
+    // p at the strip's ends (the rows below and above), from LDS
+    double2 pS = make_double2(0.0, 0.0), pN = make_double2(0.0, 0.0);
+    if (y0 > 0) pS = make_double2(P[(y0 - 1) * MLD + x], P[(y0 - 1) * MLD + x + 1]);
+    if (y0 + SR < MW) pN = make_double2(P[(y0 + SR) * MLD + x], P[(y0 + SR) * MLD + x + 1]);
+
+    // FP64 residual and the FP32 sweep coefficients, all scaled by 1/a3 in
+    // FP32: the update is then four fused multiply-adds,
+    // s = r/a3 - (a1/a3) e_S - (a2/a3) e_W - (a4/a3) e_E - (a5/a3) e_N - e.
+    // These FP32 sweeps only produce the correction e; the iteration
+    // converges to the fixed point of the FP64 residual.
+    float cr[SR][2];
+    float n1[UNI ? 1 : SR][2], n2[UNI ? 1 : SR][2], n4[UNI ? 1 : SR][2], n5[UNI ? 1 : SR][2];
+    if (UNI) {
+        const double a3 = -rau_c - rau_c - rgv_c - rgv_c;
+        const float i3 = 1.0f / (float)a3;
+        n1[0][0] = n1[0][1] = n5[0][0] = n5[0][1] = -(float)rgv_c * i3;
+        n2[0][0] = n2[0][1] = n4[0][0] = n4[0][1] = -(float)rau_c * i3;
+    }
+#pragma unroll
+    for (int r = 0; r < SR; r++) {
+        const int c0 = (y0 + r) * MLD + x;
+        const double ra[3] = {ram[r], rav[r].x, rav[r].y};
+        const double2 ps = r > 0 ? pr[r - 1] : pS, pn = r < SR - 1 ? pr[r + 1] : pN;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            float rr = 0.f;
+            if ((upd >> (2 * r + h)) & 1) {
+                const double a1 = h ? g[r].y : g[r].x, a5 = h ? g[r + 1].y : g[r + 1].x;
+                const double a2 = ra[h], a4 = ra[h + 1];
+                const double a3 = -a4 - a2 - a5 - a1;
+                const double pw = h ? pr[r].x : P[c0 - 1];
+                const double pe = h ? P[c0 + 2] : pr[r].y;
+                const double res = (h ? bvr[r].y : bvr[r].x) - a1 * (h ? ps.y : ps.x) - a2 * pw -
+                                   a3 * (h ? pr[r].y : pr[r].x) - a4 * pe - a5 * (h ? pn.y : pn.x);
+                const float i3 = 1.0f / (float)a3;
+                rr = (float)res * i3;
+                if (!UNI) {
+                    n1[UNI ? 0 : r][h] = -(float)a1 * i3;
+                    n2[UNI ? 0 : r][h] = -(float)a2 * i3;
+                    n4[UNI ? 0 : r][h] = -(float)a4 * i3;
+                    n5[UNI ? 0 : r][h] = -(float)a5 * i3;
+                }
+            } else if (!UNI) {
+                n1[UNI ? 0 : r][h] = n2[UNI ? 0 : r][h] = n4[UNI ? 0 : r][h] = n5[UNI ? 0 : r][h] = 0.f;
+            }
+            cr[r][h] = rr;
+        }
+    }
+
+    float e[SR][2];
+#pragma unroll
+    for (int r = 0; r < SR; r++) e[r][0] = e[r][1] = 0.f;
     const float w = (float)sorrel;
     const float tol = (float)sortol;
     unsigned below = 0;   // bit s: this lane's owned max |s| of sweep s is < sortol
@@ -155,21 +211,25 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
             float *__restrict__ dst = EC[color];
             const float *__restrict__ src = EC[1 - color];
 #pragma unroll
-            for (int r = 0; r < MX_PAIRS; r++) {
-                // h: the pair's point of this colour (0 left, 1 right); its west
-                // neighbour is the other colour in pair k-1+h, its east one in k+h
-                const int h = ((flags[r] >> 4) & 1) ^ color;
-                if ((flags[r] >> h) & 1) {
-                    const int q = QB(r);
-                    // register selects (a runtime index into the arrays would go to scratch)
-                    const float a1 = h ? c1[r][1] : c1[r][0], a5 = h ? c5[r][1] : c5[r][0];
-                    const float a2 = h ? cc[r] : cw[r], a4 = h ? ce[r] : cc[r];
-                    const float rr = h ? cr[r][1] : cr[r][0], i3 = h ? ci3[r][1] : ci3[r][0];
-                    const float e = dst[q];
-                    const float sum =
-                        (rr - a1 * src[q - MK] - a2 * src[q - 1 + h] - a4 * src[q + h] - a5 * src[q + MK]) * i3 - e;
-                    dst[q] = e + w * sum;
-                    if ((flags[r] >> (2 + h)) & 1) mx = fmaxf(mx, fabsf(sum));
+            for (int r = 0; r < SR; r++) {
+                const int h = (r & 1) ^ color;   // the row's point of this colour
+                const int rc = UNI ? 0 : r;
+                if ((upd >> (2 * r + h)) & 1) {
+                    const int q = (y0 + r) * ES + k;
+                    // updated points are never on the tile's outer ring, so
+                    // every LDS neighbour index below is inside the tile
+                    const float eS = r > 0 ? e[r - 1][h] : src[q - ES];
+                    const float eN = r < SR - 1 ? e[r + 1][h] : src[q + ES];
+                    const float eW = h ? e[r][0] : src[q - 1];
+                    const float eE = h ? src[q + 1] : e[r][1];
+                    float acc = fmaf(n1[rc][h], eS, cr[r][h]);
+                    acc = fmaf(n2[rc][h], eW, acc);
+                    acc = fmaf(n4[rc][h], eE, acc);
+                    acc = fmaf(n5[rc][h], eN, acc);
+                    const float sum = acc - e[r][h];
+                    e[r][h] = fmaf(w, sum, e[r][h]);
+                    dst[q] = e[r][h];
+                    if ((own >> (2 * r + h)) & 1) mx = fmaxf(mx, fabsf(sum));
                 }
             }
             __syncthreads();
@@ -179,25 +239,17 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
 
     // p + e on the owned points
 #pragma unroll
-    for (int r = 0; r < MX_PAIRS; r++) {
-        const int x = QX(r), y = QY(r);
-        const int lc = (flags[r] >> 4) & 1;   // colour of the pair's left point
+    for (int r = 0; r < SR; r++)
 #pragma unroll
         for (int h = 0; h < 2; h++)
-            if ((flags[r] >> (2 + h)) & 1) {
-                const int c = y * MLD + x + h;
-                p_out[WD2_IX(i0 + x + h, j0 + y)] = P[c] + (double)EC[lc ^ h][QB(r)];
-            }
-    }
-#undef QX
-#undef QY
-#undef QB
+            if ((own >> (2 * r + h)) & 1)
+                p_out[WD2_IX(i + h, j0 + y0 + r)] = P[(y0 + r) * MLD + x + h] + (double)e[r][h];
     for (int off = 32; off > 0; off >>= 1) below &= (unsigned)__shfl_xor((int)below, off);
     if ((t & 63) == 0) sh[t >> 6] = below;
     __syncthreads();
     if (t == 0) {
         unsigned r = ~0u;
-        for (int k = 0; k < MX_T / 64; k++) r &= sh[k];
+        for (int q = 0; q < MX_T / 64; q++) r &= sh[q];
         masks[tile] = r;
     }
 }
@@ -212,13 +264,18 @@ int sor_tiled_mx_sweeps() { return KM; }
 void launch_sor_tiled_mx(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
                          const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
                          int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
-                         int force, hipStream_t s) {
+                         int force, const double *uni, hipStream_t s) {
     int ntx = 0;
     const int nt = sor_tiled_mx_tiles(nx, ny, &ntx);
     const int grid = 8 * ((nt + 7) / 8);
-    hipLaunchKernelGGL(k_sor_tiled_mx, dim3(grid), dim3(MX_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel, nsweep,
-                       p_in, p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit, ntx, nt,
-                       sc, force);
+    if (uni)
+        hipLaunchKernelGGL(k_sor_tiled_mx<true>, dim3(grid), dim3(MX_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel,
+                           nsweep, p_in, p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit,
+                           ntx, nt, sc, force, uni[0], uni[1]);
+    else
+        hipLaunchKernelGGL(k_sor_tiled_mx<false>, dim3(grid), dim3(MX_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel,
+                           nsweep, p_in, p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit,
+                           ntx, nt, sc, force, 0.0, 0.0);
 }
 
 }  // namespace wd2

@@ -23,6 +23,7 @@ static int g_fallbacks = 0;
 static int g_tiled_sor = 1;   // temporally blocked SorRB when the grid allows it
 static int g_verbose = 1;     // print the reference's warnings (pressure.f:242-246, 234)
 static int g_lex_rows = 1;    // lexicographic Sor as one single-workgroup launch when the rows fit
+static int g_uniform = 1;     // tiled SorRB takes rau/rgv as two values on uniform Cartesian grids
 
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -110,6 +111,11 @@ struct wd2_ctx {
     hipEvent_t sev[2] = {nullptr, nullptr};
     int stage_i = 0;
     std::vector<std::vector<double>> met_shadow;
+    int met_version = 0, uni_version = -1;
+    // {rau, rgv} when each is one value over the entries the tiled red-black
+    // SOR reads (a uniform Cartesian grid); uni_ok false otherwise
+    bool uni_ok = false;
+    double uni[2] = {0.0, 0.0};
     // optional HIP-event timing of the red-black SOR kernels: event pairs,
     // each around kt_cnt[pair] back-to-back launches
     bool kt_on = false;
@@ -389,6 +395,24 @@ static int set_thermal(wd2_ctx *c, const int *nTRgType, const int *nTemBdTp, con
     return WD2_OK;
 }
 
+// Uniform-coefficient scan: rau on i = 1..nx, j = 2..ny and rgv on
+// i = 2..nx, j = 1..ny (host arrays, leading dimension ld_h).
+static void scan_uniform(wd2_ctx *c, const double *rau, const double *rgv, int ld_h) {
+    const int nx = c->nx, ny = c->ny;
+    auto same = [&](const double *a, int i0, int i1, int j0, int j1) {
+        const double v = a[(size_t)i0 + (size_t)ld_h * j0];
+        for (int j = j0; j <= j1; j++)
+            for (int i = i0; i <= i1; i++)
+                if (std::memcmp(&a[(size_t)i + (size_t)ld_h * j], &v, sizeof v) != 0) return false;
+        return true;
+    };
+    c->uni_ok = rau && rgv && same(rau, 1, nx, 2, ny) && same(rgv, 2, nx, 1, ny);
+    if (c->uni_ok) {
+        c->uni[0] = rau[1 + (size_t)ld_h * 2];
+        c->uni[1] = rgv[2 + (size_t)ld_h * 1];
+    }
+}
+
 static int check_err(wd2_ctx *c) {
     int e = 0;
     HIPCHK(hipMemcpyAsync(&e, &c->sc->err, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -496,6 +520,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         // mixed precision: FP64 residual + FP32 sweeps + FP64 accumulation
         const bool mx = c->precision == WD2_PREC_MIXED;
         auto sweep = mx ? launch_sor_tiled_mx : launch_sor_tiled;
+        const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
         const int K = mx ? sor_tiled_mx_sweeps() : sor_tiled_sweeps();
         launch_copy(ld, 0, nx + 1, 0, ny + 1, p, c->pscr, s);
         int m0 = 0, L = 0;
@@ -521,7 +546,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
                     const double *in = (L % 2 == 0) ? p : c->pscr;
                     double *out = (L % 2 == 0) ? c->pscr : p;
                     sweep(nx, ny, ld, c->M, b, sorrel, kr, in, out, part[L % 2], part[(L + 1) % 2], prev_k, prev_m0,
-                          L - 1, sortol, msorit, c->sc, 0, s);
+                          L - 1, sortol, msorit, c->sc, 0, uni, s);
                     prev_k = kr;
                     prev_m0 = m0;
                     m0 += kr;
@@ -552,7 +577,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         if (L > 0) {
             if (keep < krd)   // stopped inside the launch: redo it with exactly `keep` sweeps
                 sweep(nx, ny, ld, c->M, b, sorrel, keep, in, out, part[Ld % 2], nullptr, 0, 0, 0, sortol, msorit,
-                      c->sc, 1, s);
+                      c->sc, 1, uni, s);
             if (out != p) launch_copy(ld, 0, nx + 1, 0, ny + 1, out, p, s);
         }
         int nconv = h.sor_nconv;
@@ -751,6 +776,7 @@ static int up_metric(int k, const double *h, int ld) {
     }
     sh.resize(w * rows);
     for (size_t j = 0; j < rows; j++) std::memcpy(sh.data() + j * w, h + j * (size_t)ld, w * sizeof(double));
+    c->met_version++;
     return up(c, h, ld, c->met[k]);
 }
 
@@ -764,6 +790,12 @@ int wd2_sor_mixed_sweeps_per_launch(void) { return sor_tiled_mx_sweeps(); }
 int wd2_set_verbose(int enable) {
     const int old = g_verbose;
     g_verbose = enable ? 1 : 0;
+    return old;
+}
+
+int wd2_set_uniform_coefficients(int enable) {
+    const int old = g_uniform;
+    g_uniform = enable ? 1 : 0;
     return old;
 }
 
@@ -878,6 +910,10 @@ int wd2_ppe(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const i
         (st = up_metric(M_RGV, rgv, ld)) || (st = up_metric(M_XEU, xeu, ld)) || (st = up_metric(M_YEU, yeu, ld)) ||
         (st = up_metric(M_XZV, xzv, ld)) || (st = up_metric(M_YZV, yzv, ld)))
         return st;
+    if (c->met_version != c->uni_version) {
+        scan_uniform(c, rau, rgv, ld);
+        c->uni_version = c->met_version;
+    }
     if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v)) || (st = up(c, p, ld, c->p))) return st;
     if ((st = ppe(c, lCartesGrid != 0, nPpeSolver, msorit, dk, sortol, sorrel, c->u, c->v, c->p, nSorConv)))
         return st;
@@ -1109,6 +1145,7 @@ int wd2_ctx_create(wd2_ctx **out, int nx, int ny, int ld, int mgri, int mgrj, co
         ctx_free(c);
         return fail(WD2_ERR_HIP, "metric upload failed");
     }
+    scan_uniform(c, metrics[M_RAU], metrics[M_RGV], ld);
     *out = c;
     return WD2_OK;
 }

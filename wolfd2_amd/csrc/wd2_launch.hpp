@@ -87,14 +87,17 @@ int sor_tiled_sweeps();
 void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
                       const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
                       int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
-                      int force, hipStream_t s);
+                      int force, const double *uni, hipStream_t s);
+// uni: null, or {rau, rgv} when every rau entry on i = 1..nx, j = 2..ny and
+// every rgv entry on i = 2..nx, j = 1..ny has that one value (a uniform
+// Cartesian grid): the kernels then use the values instead of loading them.
 // Mixed precision (FP64 residual and accumulation, FP32 sweeps), same contract.
 int sor_tiled_mx_tiles(int nx, int ny, int *ntx);
 int sor_tiled_mx_sweeps();
 void launch_sor_tiled_mx(int nx, int ny, int ld, const Metrics &m, const double *b, double sorrel, int nsweep,
                          const double *p_in, double *p_out, unsigned *masks, const unsigned *prev_masks,
                          int prev_nsweep, int prev_m0, int prev_launch, double sortol, int msorit, Scalars *sc,
-                         int force, hipStream_t s);
+                         int force, const double *uni, hipStream_t s);
 // Stopping test of a launch over ntiles tile masks (the last launch of a batch).
 void launch_sor_tiled_check(int ntiles, int nsweep, int m0, int launch, int msorit, const unsigned *masks,
                             Scalars *sc, hipStream_t s);
