@@ -105,30 +105,56 @@ __device__ inline unsigned seg_kinds(long seg0, long L) {
 // the TR_W rows beyond the boundary from a zero state.  The recurrences
 // contract, so the guess is usually the exact state bit for bit; it is always
 // verified (and repaired if not) against the neighbour block's own result.
+// The rows a warm-up reads are first loaded by TR_W lanes at once into LDS
+// (w[0..3][q] = A, D, C, B of row k0 + q, and D of row k0 + q + 1 in w[4]),
+// so the one lane that runs the recurrence waits for one load, not TR_W.
 constexpr int TR_W = 16;
 template <typename T>
-__device__ inline void warm_fwd(const T *__restrict__ A, const T *__restrict__ D, const T *__restrict__ Cc,
-                                const T *__restrict__ B, long blk0, long L, T &cpv, T &bpv) {
+__device__ inline void warm_fwd_load(const T *__restrict__ A, const T *__restrict__ D, const T *__restrict__ Cc,
+                                     const T *__restrict__ B, long blk0, T (*w)[TR_W]) {
+    const long k0 = blk0 > TR_W ? blk0 - TR_W : 0;
+    const int q = threadIdx.x;
+    if (q < TR_W && k0 + q < blk0) {
+        w[0][q] = A[k0 + q];
+        w[1][q] = D[k0 + q];
+        w[2][q] = Cc[k0 + q];
+        w[3][q] = B[k0 + q];
+        w[4][q] = D[k0 + q + 1];   // row k0 + q + 1 <= blk0 < n
+    }
+}
+template <typename T>
+__device__ inline void warm_fwd(long blk0, long L, const T (*w)[TR_W], T &cpv, T &bpv) {
     const long k0 = blk0 > TR_W ? blk0 - TR_W : 0;
     long r = k0 % L;
     T c = T(0), bb = T(0);
-    for (long k = k0; k < blk0; k++, r = (r + 1 == L) ? 0 : r + 1) {
+    for (int q = 0; k0 + q < blk0; q++, r = (r + 1 == L) ? 0 : r + 1) {
         const int kind = r == 0 ? 1 : (r == L - 1 ? 2 : 0);
         T ck, bk;
-        fwd_row<T>(kind, A[k], D[k], Cc[k], B[k], kind == 1 ? D[k + 1] : T(1), c, bb, ck, bk);
+        fwd_row<T>(kind, w[0][q], w[1][q], w[2][q], w[3][q], kind == 1 ? w[4][q] : T(1), c, bb, ck, bk);
         c = ck;
         bb = bk;
     }
     cpv = c;
     bpv = bb;
 }
+// backward warm-up over the rows k1 .. k1+TR_W-1 (cp in w[0], bp in w[1])
 template <typename T>
-__device__ inline T warm_bwd(const T *CP, const T *BP, long n, long k1, long L) {
+__device__ inline void warm_bwd_load(const T *CP, const T *BP, long n, long k1, T (*w)[TR_W]) {
+    const int q = threadIdx.x;
+    if (q < TR_W && k1 + q < n) {
+        w[0][q] = CP[k1 + q];
+        w[1][q] = BP[k1 + q];
+    }
+}
+template <typename T>
+__device__ inline T warm_bwd(const T (*w)[TR_W], long n, long k1, long L) {
     const long k2 = min(n, k1 + TR_W);
     long r = (k2 - 1) % L;
     T xw = T(0);
-    for (long k = k2 - 1; k >= k1; k--, r = (r == 0) ? L - 1 : r - 1)
-        xw = (r == L - 1 && r != 0) ? BP[k] : BP[k] - (CP[k] * xw);
+    for (long k = k2 - 1; k >= k1; k--, r = (r == 0) ? L - 1 : r - 1) {
+        const int q = (int)(k - k1);
+        xw = (r == L - 1 && r != 0) ? w[1][q] : w[1][q] - (w[0][q] * xw);
+    }
     return xw;
 }
 
@@ -139,6 +165,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
                                                    T *__restrict__ blk_b) {
     __shared__ T sbuf[TR_T * TR_P];
     __shared__ T sc[TR_T], sb[TR_T];
+    __shared__ T sw[5][TR_W];
     __shared__ int changed;
     const int t = threadIdx.x;
     const long blk0 = (long)blockIdx.x * TR_R;
@@ -146,6 +173,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
     const long seg0 = blk0 + (long)t * TR_S;
     const int len = max(0, min(TR_S, nrow - t * TR_S));
     T a[TR_S], d[TR_S], c[TR_S], b[TR_S], cp[TR_S], bp[TR_S];
+    if (blk0 > 0) warm_fwd_load<T>(A, D, Cc, B, blk0, sw);   // made visible by stage_in's barriers
     stage_in(A, blk0, nrow, sbuf, a);
     stage_in(D, blk0, nrow, sbuf, d);
     stage_in(Cc, blk0, nrow, sbuf, c);
@@ -156,7 +184,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
     // speculative run: lane 0 from the warmed-up state entering the block,
     // the others from the zero state
     T cpv = T(0), bpv = T(0);
-    if (t == 0 && blk0 > 0) warm_fwd<T>(A, D, Cc, B, blk0, L, cpv, bpv);
+    if (t == 0 && blk0 > 0) warm_fwd<T>(blk0, L, sw, cpv, bpv);
 #pragma unroll
     for (int s = 0; s < TR_S; s++) {
         if (s < len) {
@@ -217,6 +245,11 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
                                                    int *__restrict__ fail) {
     __shared__ T scp[TR_T * TR_P], sbp[TR_T * TR_P];
     __shared__ T sx[TR_T];
+    __shared__ T sw[2][TR_W];
+    // the head rows a repair usually needs (A, D, C, B of rows blk0 .. +TR_H-1
+    // and D of the row after each), loaded by TR_H lanes at once
+    constexpr int TR_H = 64;
+    __shared__ T sh[5][TR_H];
     __shared__ int changed;
     const int t = threadIdx.x;
     const long blk0 = (long)blockIdx.x * TR_R;
@@ -231,6 +264,15 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
             sbp[(k / TR_S) * TR_P + k % TR_S] = BP[blk0 + k];
         }
     }
+    if (blockIdx.x > 0 && t < TR_H && t < nrow) {
+        const long g = blk0 + t;
+        sh[0][t] = A[g];
+        sh[1][t] = D[g];
+        sh[2][t] = Cc[g];
+        sh[3][t] = B[g];
+        sh[4][t] = g + 1 < n ? D[g + 1] : T(1);
+    }
+    if (blk0 + nrow < n) warm_bwd_load<T>(CP, BP, n, blk0 + nrow, sw);
     __syncthreads();
     if (t == 0 && blockIdx.x > 0) {
         long r = blk0 % L;   // position of the block's first row in its chain
@@ -240,9 +282,11 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
             for (; k < nrow; k++, r = (r + 1 == L) ? 0 : r + 1) {
                 const int kind = r == 0 ? 1 : (r == L - 1 ? 2 : 0);
                 const long g = blk0 + k;
-                const T dn = (kind == 1) ? D[g + 1] : T(1);
+                const bool inl = k < TR_H;
+                const T dn = (kind == 1) ? (inl ? sh[4][k] : D[g + 1]) : T(1);
                 T ck, bk;
-                fwd_row<T>(kind, A[g], D[g], Cc[g], B[g], dn, cpv, bpv, ck, bk);
+                fwd_row<T>(kind, inl ? sh[0][k] : A[g], inl ? sh[1][k] : D[g], inl ? sh[2][k] : Cc[g],
+                           inl ? sh[3][k] : B[g], dn, cpv, bpv, ck, bk);
                 const int q = (k / TR_S) * TR_P + k % TR_S;
                 if (same(ck, scp[q]) && same(bk, sbp[q])) break;
                 scp[q] = ck; sbp[q] = bk; CP[g] = ck; BP[g] = bk;
@@ -260,7 +304,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
     // next block's first row (recorded for the check in k_trid_finish)
     T xn = T(0);
     if (len > 0 && t * TR_S + len == nrow && blk0 + nrow < n) {
-        xn = warm_bwd<T>(CP, BP, n, blk0 + nrow, L);
+        xn = warm_bwd<T>(sw, n, blk0 + nrow, L);
         blk_xe[blockIdx.x] = xn;
     }
 #pragma unroll
