@@ -20,11 +20,13 @@
 // before using it, and a one-workgroup kernel repairs each block's backward
 // tail from its successor's first value.  A repair that never coalesces raises
 // a flag and the same kernel recomputes the whole system sequentially, so the
-// answer is exact in every case.  Segments are loaded and stored through LDS
-// so that global accesses are unit-stride.
+// answer is exact in every case.  A thread loads and stores its segment as
+// 16-byte vectors.
 //
 // A batch of independent chains of equal length L is solved at once: row k
 // starts a chain when k % L == 0 and ends one when k % L == L-1.
+#include <type_traits>
+
 #include "wd2_dev.hpp"
 #include "wd2_launch.hpp"
 
@@ -56,35 +58,49 @@ __device__ inline int row_kind(long k, long L) {
     return r == 0 ? 1 : (r == L - 1 ? 2 : 0);
 }
 
-// Segment loads and stores go through LDS so that global accesses are
-// unit-stride across the workgroup (row blk0 + t + TR_T*m in pass m), while
-// each thread works on TR_S consecutive rows.
+// A thread's TR_S consecutive rows are read and written as 16-byte vectors
+// (chain arrays start 128-byte aligned and a segment starts at a multiple of
+// TR_S rows): a wavefront covers one contiguous span, every load in flight at
+// once.  A partial last segment goes row by row.
 template <typename T>
-__device__ inline void stage_in(const T *__restrict__ G, long blk0, int nrow, T *sbuf, T (&r)[TR_S]) {
-    const int t = threadIdx.x;
+__device__ inline void seg_load(const T *__restrict__ G, long seg0, int len, T (&r)[TR_S]) {
+    if (len == TR_S) {
+        constexpr int V = 16 / sizeof(T);
+        using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+        const V4 *g = reinterpret_cast<const V4 *>(G + seg0);
 #pragma unroll
-    for (int m = 0; m < TR_S; m++) {
-        const int k = t + TR_T * m;
-        if (k < nrow) sbuf[(k / TR_S) * TR_P + k % TR_S] = G[blk0 + k];
+        for (int q = 0; q < TR_S / V; q++) {
+            const V4 v = g[q];
+            const T *e = reinterpret_cast<const T *>(&v);
+#pragma unroll
+            for (int u = 0; u < V; u++) r[q * V + u] = e[u];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < TR_S; q++) r[q] = q < len ? G[seg0 + q] : T(0);
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < TR_S; q++) r[q] = sbuf[t * TR_P + q];
-    __syncthreads();
 }
 template <typename T>
-__device__ inline void stage_out(T *__restrict__ G, long blk0, int nrow, T *sbuf, const T (&r)[TR_S]) {
-    const int t = threadIdx.x;
+__device__ inline void seg_store(T *__restrict__ G, long seg0, int len, const T (&r)[TR_S]) {
+    if (len == TR_S) {
+        constexpr int V = 16 / sizeof(T);
+        using V4 = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+        V4 *g = reinterpret_cast<V4 *>(G + seg0);
 #pragma unroll
-    for (int q = 0; q < TR_S; q++) sbuf[t * TR_P + q] = r[q];
-    __syncthreads();
+        for (int q = 0; q < TR_S / V; q++) {
+            V4 v;
+            T *e = reinterpret_cast<T *>(&v);
 #pragma unroll
-    for (int m = 0; m < TR_S; m++) {
-        const int k = t + TR_T * m;
-        if (k < nrow) G[blk0 + k] = sbuf[(k / TR_S) * TR_P + k % TR_S];
+            for (int u = 0; u < V; u++) e[u] = r[q * V + u];
+            g[q] = v;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < TR_S; q++)
+            if (q < len) G[seg0 + q] = r[q];
     }
-    __syncthreads();
 }
+
 
 // Kinds of the TR_S rows from seg0 on, two bits per row (one 64-bit modulo).
 __device__ inline unsigned seg_kinds(long seg0, long L) {
@@ -163,7 +179,6 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
                                                    const T *__restrict__ Cc, const T *__restrict__ B,
                                                    T *__restrict__ CP, T *__restrict__ BP, T *__restrict__ blk_c,
                                                    T *__restrict__ blk_b) {
-    __shared__ T sbuf[TR_T * TR_P];
     __shared__ T sc[TR_T], sb[TR_T];
     __shared__ T sw[5][TR_W];
     __shared__ int changed;
@@ -173,11 +188,12 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
     const long seg0 = blk0 + (long)t * TR_S;
     const int len = max(0, min(TR_S, nrow - t * TR_S));
     T a[TR_S], d[TR_S], c[TR_S], b[TR_S], cp[TR_S], bp[TR_S];
-    if (blk0 > 0) warm_fwd_load<T>(A, D, Cc, B, blk0, sw);   // made visible by stage_in's barriers
-    stage_in(A, blk0, nrow, sbuf, a);
-    stage_in(D, blk0, nrow, sbuf, d);
-    stage_in(Cc, blk0, nrow, sbuf, c);
-    stage_in(B, blk0, nrow, sbuf, b);
+    if (blk0 > 0) warm_fwd_load<T>(A, D, Cc, B, blk0, sw);
+    seg_load(A, seg0, len, a);
+    seg_load(D, seg0, len, d);
+    seg_load(Cc, seg0, len, c);
+    seg_load(B, seg0, len, b);
+    __syncthreads();   // sw is complete
     const unsigned kinds = seg_kinds(seg0, L);
     // the first-pivot quirk needs the original diagonal of the row after a chain start
 #define DNEXT(s) (SKIND(kinds, s) == 1 ? ((s) + 1 < len ? d[((s) + 1) % TR_S] : D[seg0 + (s) + 1]) : T(1))
@@ -227,8 +243,8 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
         blk_c[blockIdx.x] = cp[len - 1];
         blk_b[blockIdx.x] = bp[len - 1];
     }
-    stage_out(CP, blk0, nrow, sbuf, cp);
-    stage_out(BP, blk0, nrow, sbuf, bp);
+    seg_store(CP, seg0, len, cp);
+    seg_store(BP, seg0, len, bp);
 }
 
 // Backward pass.  A workgroup first repairs the head of its block's forward
@@ -338,7 +354,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
         __syncthreads();
     }
     if (t == 0 && len > 0) blk_x[blockIdx.x] = x[0];
-    stage_out(X, blk0, nrow, scp, x);
+    seg_store(X, seg0, len, x);
 }
 
 // One workgroup: repair each block's tail from its successor's first value,

@@ -167,6 +167,10 @@ static void ctx_free(wd2_ctx *c) {
     delete c;
 }
 
+// Rows of each chain buffer: every chain of the grid fits, and a multiple of
+// 16 keeps each buffer 128-byte aligned (the chain solver reads 16-byte vectors).
+static size_t chain_rows(int nx, int ny) { return ((size_t)nx * ny + 64 + 15) / 16 * 16; }
+
 static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     if (nx < 5 || ny < 5) return fail(WD2_ERR_ARG, "grid %dx%d too small (need nx, ny >= 5)", nx, ny);
     wd2_ctx *c = new wd2_ctx();
@@ -183,7 +187,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
         HIPCHK(hipEventCreateWithFlags(&c->sev[k], hipEventDisableTiming));
     }
     const int NFIELDS = 16 + 4 + 4 + 5 + 13;
-    const size_t nchain = (size_t)nx * ny + 64;
+    const size_t nchain = chain_rows(nx, ny);
     const size_t total = NFIELDS * c->nf + 5 * nchain;
     HIPCHK(hipMalloc(&c->store, total * sizeof(double)));
     HIPCHK(hipMemsetAsync(c->store, 0, total * sizeof(double), c->s));
@@ -1254,7 +1258,7 @@ int wd2_ctx_set_precision(wd2_ctx *c, int precision) {
         // The FP32 momentum views alias the FP64 temporaries and chains.  Entries
         // the momentum kernels never write are read as 0 (as in the reference),
         // so clear the whole block when the arithmetic changes.
-        const size_t nchain = (size_t)c->nx * c->ny + 64;
+        const size_t nchain = chain_rows(c->nx, c->ny);
         HIPCHK(hipMemsetAsync(c->T.cj1, 0, (13 * c->nf + 5 * nchain) * sizeof(double), c->s));
         HIPCHK(hipStreamSynchronize(c->s));
     }
