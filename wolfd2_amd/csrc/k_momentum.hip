@@ -33,17 +33,16 @@ template <typename T>
 __device__ inline T cv(double x) { return (T)x; }
 
 template <int NCOMP, typename T>
-__global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, double djac_,
-                                                      const double *__restrict__ xzi, const double *__restrict__ xet,
-                                                      const double *__restrict__ yzi, const double *__restrict__ yet,
-                                                      const double *__restrict__ u, const double *__restrict__ v,
-                                                      T *__restrict__ cc1, T *__restrict__ cc2) {
+__global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, double djac_, MArr xzi, MArr xet,
+                                                      MArr yzi, MArr yet, const double *__restrict__ u,
+                                                      const double *__restrict__ v, T *__restrict__ cc1,
+                                                      T *__restrict__ cc2) {
     const int i = blockIdx.x * MX + threadIdx.x, j = blockIdx.y * MY + threadIdx.y;   // 0-based cover
     if (i > nx + 1 || j > ny + 1) return;
     const T dHalf = T(0.5), dFour = T(4.0), djac = cv<T>(djac_);
 #define U(a, b) cv<T>(u[WD2_IX(a, b)])
 #define V(a, b) cv<T>(v[WD2_IX(a, b)])
-#define M(f, a, b) cv<T>(f[WD2_IX(a, b)])
+#define M(f, a, b) cv<T>(mget(f, ld, a, b))
     const size_t k = WD2_IX(i, j);
     if (NCOMP == 1) {
         if (i >= 1 && j >= 1)
@@ -88,8 +87,8 @@ __global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, do
 }
 
 template <typename T>
-void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const double *xzi, const double *xet,
-                      const double *yzi, const double *yet, const double *u, const double *v, T *cc1, T *cc2,
+void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const MArr &xzi, const MArr &xet,
+                      const MArr &yzi, const MArr &yet, const double *u, const double *v, T *cc1, T *cc2,
                       hipStream_t s) {
     const double djac = njacob == 1 ? 2.0 : 1.0;
     const dim3 g = mgrid(nx + 2, ny + 2), b(MX, MY);
@@ -103,11 +102,10 @@ void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const doubl
     default: break;
     }
 }
-template void launch_conv_coef<double>(int, int, int, int, int, const double *, const double *, const double *,
-                                       const double *, const double *, const double *, double *, double *,
-                                       hipStream_t);
-template void launch_conv_coef<float>(int, int, int, int, int, const double *, const double *, const double *,
-                                      const double *, const double *, const double *, float *, float *, hipStream_t);
+template void launch_conv_coef<double>(int, int, int, int, int, const MArr &, const MArr &, const MArr &,
+                                       const MArr &, const double *, const double *, double *, double *, hipStream_t);
+template void launch_conv_coef<float>(int, int, int, int, int, const MArr &, const MArr &, const MArr &, const MArr &,
+                                      const double *, const double *, float *, float *, hipStream_t);
 
 // --------------------------------------------------- porous-region division
 template <typename T>
@@ -224,9 +222,9 @@ __device__ inline T dconv_v(int ld, int i, int j, const T *c1, const T *c2, cons
 #undef C2
 }
 template <typename T>
-__device__ inline T ddiff_u(int ld, int i, int j, const double *ac, const double *bc, const double *bn,
-                            const double *gn, const double *u) {
-#define F(f, a, b) cv<T>(f[WD2_IX(a, b)])
+__device__ inline T ddiff_u(int ld, int i, int j, const MArr &ac, const MArr &bc, const MArr &bn, const MArr &gn,
+                            const double *u) {
+#define F(f, a, b) cv<T>(mget(f, ld, a, b))
     const T s1 = F(ac, i + 1, j) * (U(i + 1, j) - U(i, j)) - F(ac, i, j) * (U(i, j) - U(i - 1, j)) +
                  F(bc, i + 1, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
                  F(bc, i, j) * (U(i, j + 1) + U(i - 1, j + 1) - U(i, j - 1) - U(i - 1, j - 1));
@@ -236,8 +234,8 @@ __device__ inline T ddiff_u(int ld, int i, int j, const double *ac, const double
     return s1 + s2;
 }
 template <typename T>
-__device__ inline T ddiff_v(int ld, int i, int j, const double *an, const double *bc, const double *bn,
-                            const double *gc, const double *u) {
+__device__ inline T ddiff_v(int ld, int i, int j, const MArr &an, const MArr &bc, const MArr &bn, const MArr &gc,
+                            const double *u) {
     const T s1 = F(an, i, j) * (U(i + 1, j) - U(i, j)) - F(an, i - 1, j) * (U(i, j) - U(i - 1, j)) +
                  F(bn, i, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
                  F(bn, i - 1, j) * (U(i, j + 1) + U(i - 1, j + 1) - U(i, j - 1) - U(i - 1, j - 1));
@@ -259,16 +257,16 @@ __global__ __launch_bounds__(MX *MY) void k_conv_diff(int nx, int ny, int ld, in
         const size_t k = WD2_IX(i, j);
         t.cnvs[k] = dconv_u<T>(ld, i, j, t.c1s, t.c2s, fs);
         t.cnvn[k] = dconv_u<T>(ld, i, j, t.c1n, t.c2n, fn);
-        t.difs[k] = ddiff_u<T>(ld, i, j, m.rac, m.rbc, m.rbn, m.rgn, fs);
-        t.difn[k] = ddiff_u<T>(ld, i, j, m.rac, m.rbc, m.rbn, m.rgn, fn);
+        t.difs[k] = ddiff_u<T>(ld, i, j, MARR(m, rac), MARR(m, rbc), MARR(m, rbn), MARR(m, rgn), fs);
+        t.difn[k] = ddiff_u<T>(ld, i, j, MARR(m, rac), MARR(m, rbc), MARR(m, rbn), MARR(m, rgn), fn);
     } else {
         const int i = 2 + blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
         if (i > nx || j > ny) return;
         const size_t k = WD2_IX(i, j);
         t.cnvs[k] = dconv_v<T>(ld, i, j, t.c1s, t.c2s, fs);
         t.cnvn[k] = dconv_v<T>(ld, i, j, t.c1n, t.c2n, fn);
-        t.difs[k] = ddiff_v<T>(ld, i, j, m.ran, m.rbc, m.rbn, m.rgc, fs);
-        t.difn[k] = ddiff_v<T>(ld, i, j, m.ran, m.rbc, m.rbn, m.rgc, fn);
+        t.difs[k] = ddiff_v<T>(ld, i, j, MARR(m, ran), MARR(m, rbc), MARR(m, rbn), MARR(m, rgc), fs);
+        t.difn[k] = ddiff_v<T>(ld, i, j, MARR(m, ran), MARR(m, rbc), MARR(m, rbn), MARR(m, rgc), fn);
     }
 }
 
@@ -311,10 +309,10 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
     const T dk = cv<T>(dk_), dk2 = cv<T>(dk2_), re1 = cv<T>(re1_), fr = cv<T>(fr_);
     const size_t p = WD2_IX(i, j);
 #define TT(f, a, b) t.f[WD2_IX(a, b)]
-#define MM(f, a, b) cv<T>(m.f[WD2_IX(a, b)])
+#define MM(f, a, b) cv<T>(MGET(m, f, a, b))
     T a1, a2, a3;
     if (comp == 1) {
-        const T rkj = dk2 * cv<T>(m.dju[p]);
+        const T rkj = dk2 * cv<T>(MGET(m, dju, i, j));
         if (split == 1) {
             if (t.cj1[p] >= dZero) {
                 a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(rac, i, j));
@@ -354,7 +352,7 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
             ch.b[k] = bb;
         }
     } else {
-        const T rkj = dk2 * cv<T>(m.djv[p]);
+        const T rkj = dk2 * cv<T>(MGET(m, djv, i, j));
         if (split == 1) {
             if (t.cj1[p] >= dZero) {
                 a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(ran, i - 1, j));

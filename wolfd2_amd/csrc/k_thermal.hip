@@ -126,33 +126,34 @@ __global__ __launch_bounds__(TX *TY) void k_therm_x(Regions R, Thermal TH, int l
     const long ind = (long)(j - 2) * (nx - 1) + (i - 2);
     const double dZero = 0.0, dOne = 1.0, dTwo = 2.0, dHalf = 0.5;
 #define F(a, p, q) a[WD2_IX(p, q)]
-    const double *rau = m.rau, *rbu = m.rbu, *rbv = m.rbv, *rgv = m.rgv;
-    const double rkj = dk * F(m.djc, i, j) * dHalf;
+#define G(f, p, q) MGET(m, f, p, q)   // a metric, through its constant box
+    const double rkj = dk * G(djc, i, j) * dHalf;
     double a1, a2, a3;
     if (F(cu1, i, j) >= dZero) {
-        a1 = rkj * (-F(cu1, i - 1, j) - pe1 * F(rau, i - 1, j));
-        a2 = dOne + rkj * (F(cu1, i, j) + pe1 * (F(rau, i, j) + F(rau, i - 1, j)));
-        a3 = rkj * (-pe1 * F(rau, i, j));
+        a1 = rkj * (-F(cu1, i - 1, j) - pe1 * G(rau, i - 1, j));
+        a2 = dOne + rkj * (F(cu1, i, j) + pe1 * (G(rau, i, j) + G(rau, i - 1, j)));
+        a3 = rkj * (-pe1 * G(rau, i, j));
     } else {
-        a1 = rkj * (-pe1 * F(rau, i - 1, j));
-        a2 = dOne + rkj * (-F(cu1, i, j) + pe1 * (F(rau, i, j) + F(rau, i - 1, j)));
-        a3 = rkj * (F(cu1, i + 1, j) - pe1 * F(rau, i, j));
+        a1 = rkj * (-pe1 * G(rau, i - 1, j));
+        a2 = dOne + rkj * (-F(cu1, i, j) + pe1 * (G(rau, i, j) + G(rau, i - 1, j)));
+        a3 = rkj * (F(cu1, i + 1, j) - pe1 * G(rau, i, j));
     }
     const double c = -F(cvn, i, j - 1) * F(tn, i, j - 1) - F(cun, i - 1, j) * F(tn, i - 1, j) +
                      (F(cun, i, j) - F(cun, i - 1, j) + F(cvn, i, j) - F(cvn, i, j - 1)) * F(tn, i, j) +
                      F(cun, i, j) * F(tn, i + 1, j) + F(cvn, i, j) * F(tn, i, j + 1);
     const double d =
-        F(rau, i, j) * (F(tn, i + 1, j) - F(tn, i, j)) - F(rau, i - 1, j) * (F(tn, i, j) - F(tn, i - 1, j)) +
-        F(rbu, i, j) * (F(tn, i + 1, j + 1) + F(tn, i, j + 1) - F(tn, i + 1, j - 1) - F(tn, i, j - 1)) -
-        F(rbu, i - 1, j) * (F(tn, i, j + 1) + F(tn, i - 1, j + 1) - F(tn, i, j - 1) - F(tn, i - 1, j - 1)) +
-        F(rbv, i, j) * (F(tn, i + 1, j + 1) + F(tn, i + 1, j) - F(tn, i - 1, j + 1) - F(tn, i - 1, j)) -
-        F(rbv, i, j - 1) * (F(tn, i + 1, j) + F(tn, i + 1, j - 1) - F(tn, i - 1, j) - F(tn, i - 1, j - 1)) +
-        F(rgv, i, j) * (F(tn, i, j + 1) - F(tn, i, j)) - F(rgv, i, j - 1) * (F(tn, i, j) - F(tn, i, j - 1));
+        G(rau, i, j) * (F(tn, i + 1, j) - F(tn, i, j)) - G(rau, i - 1, j) * (F(tn, i, j) - F(tn, i - 1, j)) +
+        G(rbu, i, j) * (F(tn, i + 1, j + 1) + F(tn, i, j + 1) - F(tn, i + 1, j - 1) - F(tn, i, j - 1)) -
+        G(rbu, i - 1, j) * (F(tn, i, j + 1) + F(tn, i - 1, j + 1) - F(tn, i, j - 1) - F(tn, i - 1, j - 1)) +
+        G(rbv, i, j) * (F(tn, i + 1, j + 1) + F(tn, i + 1, j) - F(tn, i - 1, j + 1) - F(tn, i - 1, j)) -
+        G(rbv, i, j - 1) * (F(tn, i + 1, j) + F(tn, i + 1, j - 1) - F(tn, i - 1, j) - F(tn, i - 1, j - 1)) +
+        G(rgv, i, j) * (F(tn, i, j + 1) - F(tn, i, j)) - G(rgv, i, j - 1) * (F(tn, i, j) - F(tn, i, j - 1));
     A[ind] = a1;
     D[ind] = a2;
     C[ind] = a3;
     B[ind] = dk * heat_source(R, TH, i, j) * dHalf + dTwo * rkj * (-c + pe1 * d);
 #undef F
+#undef G
 }
 
 // Second split (thermal.f:210-252): y-direction upwinded rows in the same
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(TX *TY) void k_therm_y(Regions R, Thermal TH, int l
     const long ind = (long)(j - 2) * (nx - 1) + (i - 2);
     const double dZero = 0.0, dOne = 1.0, dHalf = 0.5;
 #define F(a, p, q) a[WD2_IX(p, q)]
-    const double *rgv = m.rgv;
+#define G(f, p, q) MGET(m, f, p, q)   // a metric, through its constant box
     if (fixed_temp_point(R, TH, i, j)) {
         A[ind] = dZero;
         D[ind] = dOne;
@@ -177,22 +178,23 @@ __global__ __launch_bounds__(TX *TY) void k_therm_y(Regions R, Thermal TH, int l
         B[ind] = dZero;
         return;
     }
-    const double rkj = dk * F(m.djc, i, j) * dHalf;
+    const double rkj = dk * G(djc, i, j) * dHalf;
     double a1, a2, a3;
     if (F(cv1, i, j) >= dZero) {
-        a1 = rkj * (-F(cv1, i, j - 1) - pe1 * F(rgv, i, j - 1));
-        a2 = dOne + rkj * (F(cv1, i, j) + pe1 * (F(rgv, i, j) + F(rgv, i, j - 1)));
-        a3 = rkj * (-pe1 * F(rgv, i, j));
+        a1 = rkj * (-F(cv1, i, j - 1) - pe1 * G(rgv, i, j - 1));
+        a2 = dOne + rkj * (F(cv1, i, j) + pe1 * (G(rgv, i, j) + G(rgv, i, j - 1)));
+        a3 = rkj * (-pe1 * G(rgv, i, j));
     } else {
-        a1 = rkj * (-pe1 * F(rgv, i, j - 1));
-        a2 = dOne + rkj * (-F(cv1, i, j) + pe1 * (F(rgv, i, j) + F(rgv, i, j - 1)));
-        a3 = rkj * (F(cv1, i, j + 1) - pe1 * F(rgv, i, j));
+        a1 = rkj * (-pe1 * G(rgv, i, j - 1));
+        a2 = dOne + rkj * (-F(cv1, i, j) + pe1 * (G(rgv, i, j) + G(rgv, i, j - 1)));
+        a3 = rkj * (F(cv1, i, j + 1) - pe1 * G(rgv, i, j));
     }
     A[ind] = a1;
     D[ind] = a2;
     C[ind] = a3;
     B[ind] = x1[ind];
 #undef F
+#undef G
 }
 
 // t(i,j) += x2(ind) (thermal.f:262-268)
@@ -210,8 +212,10 @@ void launch_therm_energy(const Regions &R, const Thermal &TH, int ld, double dk,
     const int nx = R.nx, ny = R.ny;
     const double pe1 = 1.0 / pe;
     launch_temp_bc(R, TH, ld, t, err, s);
-    launch_conv_coef<double>(nx, ny, ld, 6, 0, m.xzc, m.xec, m.yzc, m.yec, u, v, w.cu1, w.cv1, s);
-    launch_conv_coef<double>(nx, ny, ld, 3, 0, m.xzv, m.xeu, m.yzv, m.yeu, un, vn, w.cun, w.cvn, s);
+    launch_conv_coef<double>(nx, ny, ld, 6, 0, MARR(m, xzc), MARR(m, xec), MARR(m, yzc), MARR(m, yec), u, v, w.cu1,
+                             w.cv1, s);
+    launch_conv_coef<double>(nx, ny, ld, 3, 0, MARR(m, xzv), MARR(m, xeu), MARR(m, yzv), MARR(m, yeu), un, vn, w.cun,
+                             w.cvn, s);
     const long n = (long)(nx - 1) * (ny - 1);
     const dim3 g = tgrid(nx - 1, ny - 1), b(TX, TY);
     hipLaunchKernelGGL(k_therm_x, g, b, 0, s, R, TH, ld, dk, pe1, m, w.cu1, w.cun, w.cvn, tn, ch.a, ch.d, ch.c,

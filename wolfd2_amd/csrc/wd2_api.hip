@@ -41,12 +41,6 @@ static int fail(int code, const char *fmt, ...) {
         if (e_ != hipSuccess) return fail(WD2_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));   \
     } while (0)
 
-// Metric order of grid.f:Grid (and of wd2_ctx_create's metrics argument).
-enum {
-    M_RAU, M_RBU, M_RBV, M_RGV, M_RAN, M_RBN, M_RGN, M_RAC, M_RBC, M_RGC, M_DJU, M_DJV, M_DJC, M_DJN,
-    M_XEN, M_YEN, M_XZN, M_YZN, M_XEC, M_YEC, M_XZC, M_YZC, M_XEU, M_YEU, M_XZV, M_YZV, M_XZU, M_YZU,
-    M_XEV, M_YEV, M_COUNT
-};
 
 struct wd2_ctx {
     int nx = 0, ny = 0, ld = 0;
@@ -242,6 +236,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     M.xec = c->met[M_XEC]; M.yec = c->met[M_YEC]; M.xzc = c->met[M_XZC]; M.yzc = c->met[M_YZC];
     M.xeu = c->met[M_XEU]; M.yeu = c->met[M_YEU]; M.xzv = c->met[M_XZV]; M.yzv = c->met[M_YZV];
     M.xzu = c->met[M_XZU]; M.yzu = c->met[M_YZU]; M.xev = c->met[M_XEV]; M.yev = c->met[M_YEV];
+    for (int k = 0; k < M_COUNT; k++) M.box[k] = MBox{0.0, 1, 0, 1, 0};
     HIPCHK(hipStreamSynchronize(c->s));
     *out = c;
     return WD2_OK;
@@ -399,6 +394,44 @@ static int set_thermal(wd2_ctx *c, const int *nTRgType, const int *nTemBdTp, con
     return WD2_OK;
 }
 
+// The constant box of a metric array (host, leading dimension ldh): one value
+// over [2..nx-1] x [2..ny-1], grown a row or column at a time while the new
+// line holds the same value (bit for bit); empty when the core is not one value.
+static MBox scan_box(const double *h, int ldh, int nx, int ny) {
+    const MBox empty{0.0, 1, 0, 1, 0};
+    if (!h || nx < 4 || ny < 4) return empty;
+    auto bits = [&](int i, int j) {
+        uint64_t b;
+        std::memcpy(&b, h + (size_t)i + (size_t)ldh * j, sizeof b);
+        return b;
+    };
+    const uint64_t v = bits(2, 2);
+    for (int j = 2; j <= ny - 1; j++)
+        for (int i = 2; i <= nx - 1; i++)
+            if (bits(i, j) != v) return empty;
+    int i0 = 2, i1 = nx - 1, j0 = 2, j1 = ny - 1;
+    auto col = [&](int i) {
+        for (int j = j0; j <= j1; j++)
+            if (bits(i, j) != v) return false;
+        return true;
+    };
+    auto row = [&](int j) {
+        for (int i = i0; i <= i1; i++)
+            if (bits(i, j) != v) return false;
+        return true;
+    };
+    for (bool grew = true; grew;) {
+        grew = false;
+        if (i0 > 0 && col(i0 - 1)) { i0--; grew = true; }
+        if (i1 < nx + 1 && col(i1 + 1)) { i1++; grew = true; }
+        if (j0 > 0 && row(j0 - 1)) { j0--; grew = true; }
+        if (j1 < ny + 1 && row(j1 + 1)) { j1++; grew = true; }
+    }
+    double c;
+    std::memcpy(&c, &v, sizeof c);
+    return MBox{c, i0, i1, j0, j1};
+}
+
 // Uniform-coefficient scan: rau on i = 1..nx, j = 2..ny and rgv on
 // i = 2..nx, j = 1..ny (host arrays, leading dimension ld_h).
 static void scan_uniform(wd2_ctx *c, const double *rau, const double *rgv, int ld_h) {
@@ -441,13 +474,13 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
     const Metrics &M = c->M;
     hipStream_t s = c->s;
     if (comp == 1) {
-        launch_conv_coef<T>(nx, ny, ld, 4, 1, M.xzu, M.xeu, M.yzu, M.yeu, us, vs, t.cj1, t.cj2, s);
-        launch_conv_coef<T>(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, us, vs, t.c1s, t.c2s, s);
-        launch_conv_coef<T>(nx, ny, ld, 1, 0, M.xzn, M.xec, M.yzn, M.yec, un, vn, t.c1n, t.c2n, s);
+        launch_conv_coef<T>(nx, ny, ld, 4, 1, MARR(M, xzu), MARR(M, xeu), MARR(M, yzu), MARR(M, yeu), us, vs, t.cj1, t.cj2, s);
+        launch_conv_coef<T>(nx, ny, ld, 1, 0, MARR(M, xzn), MARR(M, xec), MARR(M, yzn), MARR(M, yec), us, vs, t.c1s, t.c2s, s);
+        launch_conv_coef<T>(nx, ny, ld, 1, 0, MARR(M, xzn), MARR(M, xec), MARR(M, yzn), MARR(M, yec), un, vn, t.c1n, t.c2n, s);
     } else {
-        launch_conv_coef<T>(nx, ny, ld, 5, 1, M.xzv, M.xev, M.yzv, M.yev, us, vs, t.cj1, t.cj2, s);
-        launch_conv_coef<T>(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, us, vs, t.c1s, t.c2s, s);
-        launch_conv_coef<T>(nx, ny, ld, 2, 0, M.xzc, M.xen, M.yzc, M.yen, un, vn, t.c1n, t.c2n, s);
+        launch_conv_coef<T>(nx, ny, ld, 5, 1, MARR(M, xzv), MARR(M, xev), MARR(M, yzv), MARR(M, yev), us, vs, t.cj1, t.cj2, s);
+        launch_conv_coef<T>(nx, ny, ld, 2, 0, MARR(M, xzc), MARR(M, xen), MARR(M, yzc), MARR(M, yen), us, vs, t.c1s, t.c2s, s);
+        launch_conv_coef<T>(nx, ny, ld, 2, 0, MARR(M, xzc), MARR(M, xen), MARR(M, yzc), MARR(M, yen), un, vn, t.c1n, t.c2n, s);
     }
     if (c->any_porous) launch_porous_div<T>(c->R, ld, comp, t, s);
     const double *fs = comp == 1 ? us : vs, *fn = comp == 1 ? un : vn;
@@ -781,6 +814,7 @@ static int up_metric(int k, const double *h, int ld) {
     sh.resize(w * rows);
     for (size_t j = 0; j < rows; j++) std::memcpy(sh.data() + j * w, h + j * (size_t)ld, w * sizeof(double));
     c->met_version++;
+    c->M.box[k] = scan_box(h, ld, c->nx, c->ny);
     return up(c, h, ld, c->met[k]);
 }
 
@@ -1150,6 +1184,7 @@ int wd2_ctx_create(wd2_ctx **out, int nx, int ny, int ld, int mgri, int mgrj, co
         return fail(WD2_ERR_HIP, "metric upload failed");
     }
     scan_uniform(c, metrics[M_RAU], metrics[M_RGV], ld);
+    for (int k = 0; k < M_COUNT; k++) c->M.box[k] = scan_box(metrics[k], ld, nx, ny);
     *out = c;
     return WD2_OK;
 }

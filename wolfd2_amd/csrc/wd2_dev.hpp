@@ -77,12 +77,50 @@ struct Thermal {
     __host__ __device__ double hg(int ir, int jr) const { return hgval[(ir - 1) + mgri * (jr - 1)]; }
 };
 
-// Metric arrays used on the hot path (grid.f:Metric names).
+// Metric order of grid.f:Grid (and of wd2_ctx_create's metrics argument),
+// with lower-case aliases matching the Metrics member names.
+enum {
+    M_RAU, M_RBU, M_RBV, M_RGV, M_RAN, M_RBN, M_RGN, M_RAC, M_RBC, M_RGC, M_DJU, M_DJV, M_DJC, M_DJN,
+    M_XEN, M_YEN, M_XZN, M_YZN, M_XEC, M_YEC, M_XZC, M_YZC, M_XEU, M_YEU, M_XZV, M_YZV, M_XZU, M_YZU,
+    M_XEV, M_YEV, M_COUNT
+};
+enum {
+    MI_rau, MI_rbu, MI_rbv, MI_rgv, MI_ran, MI_rbn, MI_rgn, MI_rac, MI_rbc, MI_rgc, MI_dju, MI_djv, MI_djc, MI_djn,
+    MI_xen, MI_yen, MI_xzn, MI_yzn, MI_xec, MI_yec, MI_xzc, MI_yzc, MI_xeu, MI_yeu, MI_xzv, MI_yzv, MI_xzu, MI_yzu,
+    MI_xev, MI_yev
+};
+
+// The constant part of a metric array: every entry (i,j) with i0 <= i <= i1
+// and j0 <= j <= j1 holds c, bit for bit (found by scanning the array on the
+// host), so a kernel reads c there instead of loading it -- the same value.
+// On a uniform Cartesian grid every metric is one value over the whole
+// grid, and the stencils then load only the fields.  Empty: i0 > i1.
+struct MBox {
+    double c;
+    int i0, i1, j0, j1;
+};
+
+// Metric arrays used on the hot path (grid.f:Metric names) and their boxes.
 struct Metrics {
     const double *ran, *rbn, *rgn, *rac, *rbc, *rgc, *rau, *rbu, *rbv, *rgv;
     const double *dju, *djv, *djc, *xen, *yen, *xzn, *yzn, *xec, *yec, *xzc, *yzc;
     const double *xeu, *yeu, *xzv, *yzv, *xzu, *yzu, *xev, *yev;
+    MBox box[M_COUNT];
 };
+
+// A metric array with its box, as passed to kernels that take single arrays.
+struct MArr {
+    const double *p;
+    MBox b;
+};
+
+__device__ inline double mget(const double *__restrict__ p, const MBox &b, int ld, int i, int j) {
+    if (i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1) return b.c;
+    return p[(size_t)i + (size_t)ld * (size_t)j];
+}
+__device__ inline double mget(const MArr &a, int ld, int i, int j) { return mget(a.p, a.b, ld, i, j); }
+// m.f(i,j) of a Metrics m through its box
+#define MGET(m, f, i, j) mget((m).f, (m).box[MI_##f], ld, (i), (j))
 
 #define WD2_IX(i, j) ((size_t)(i) + (size_t)ld * (size_t)(j))
 

@@ -134,9 +134,11 @@ struct ChainT {
 };
 using MomTemps = MomTempsT<double>;
 using Chain = ChainT<double>;
+// metric f of a Metrics m with its box
+#define MARR(m, f) (MArr{(m).f, (m).box[MI_##f]})
 template <typename T>
-void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const double *xzi, const double *xet,
-                      const double *yzi, const double *yet, const double *u, const double *v, T *cc1, T *cc2,
+void launch_conv_coef(int nx, int ny, int ld, int ncomp, int njacob, const MArr &xzi, const MArr &xet,
+                      const MArr &yzi, const MArr &yet, const double *u, const double *v, T *cc1, T *cc2,
                       hipStream_t s);
 template <typename T>
 void launch_porous_div(const Regions &R, int ld, int comp, const MomTempsT<T> &t, hipStream_t s);
