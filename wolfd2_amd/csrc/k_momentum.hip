@@ -292,7 +292,8 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
                                                      double dk2_, double re1_, double fr_, Metrics m, MomTempsT<T> t,
                                                      const double *__restrict__ fs, const double *__restrict__ fn,
                                                      const double *__restrict__ d, const double *__restrict__ dn,
-                                                     const T *__restrict__ xprev, ChainT<T> ch, int *err) {
+                                                     const T *__restrict__ xprev, ChainT<T> ch, int *err,
+                                                     int live) {
     const int nx = R.nx, ny = R.ny;
     int i, j;
     long k;
@@ -310,29 +311,34 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
     const size_t p = WD2_IX(i, j);
 #define TT(f, a, b) t.f[WD2_IX(a, b)]
 #define MM(f, a, b) cv<T>(MGET(m, f, a, b))
+    // Without porous regions the Dupuit-Forchheimer coefficients are the +0 the
+    // arrays hold, and without a density field d and dn are +0: the values are
+    // then used without loading them (the same arithmetic).
+    const bool por = live & 1, den = live & 2;
+    const T cpj = por ? t.cpj[p] : T(0), cps = por ? t.cps[p] : T(0), cpn = por ? t.cpn[p] : T(0);
     T a1, a2, a3;
     if (comp == 1) {
         const T rkj = dk2 * cv<T>(MGET(m, dju, i, j));
         if (split == 1) {
             if (t.cj1[p] >= dZero) {
                 a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(rac, i, j));
-                a2 = dOne + rkj * (TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * t.cpj[p];
+                a2 = dOne + rkj * (TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * cpj;
                 a3 = rkj * (-re1 * MM(rac, i + 1, j));
             } else {
                 a1 = rkj * (-re1 * MM(rac, i, j));
-                a2 = dOne + rkj * (-TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * t.cpj[p];
+                a2 = dOne + rkj * (-TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * cpj;
                 a3 = rkj * (TT(cj1, i + 1, j) - re1 * MM(rac, i + 1, j));
             }
             ch.b[k] = cv<T>(fn[p] - fs[p]) + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
-                      dk2 * (t.cps[p] * cv<T>(fs[p]) + t.cpn[p] * cv<T>(fn[p]));
+                      dk2 * (cps * cv<T>(fs[p]) + cpn * cv<T>(fn[p]));
         } else {
             if (t.cj2[p] >= dZero) {
                 a1 = rkj * (-TT(cj2, i, j - 1) - re1 * MM(rgn, i, j - 1));
-                a2 = dOne + rkj * (TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * t.cpj[p];
+                a2 = dOne + rkj * (TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * cpj;
                 a3 = rkj * (-re1 * MM(rgn, i, j));
             } else {
                 a1 = rkj * (-re1 * MM(rgn, i, j - 1));
-                a2 = dOne + rkj * (-TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * t.cpj[p];
+                a2 = dOne + rkj * (-TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * cpj;
                 a3 = rkj * (TT(cj2, i, j + 1) - re1 * MM(rgn, i, j));
             }
             T bb = xprev[k];
@@ -356,25 +362,26 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
         if (split == 1) {
             if (t.cj1[p] >= dZero) {
                 a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(ran, i - 1, j));
-                a2 = rkj * (TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * t.cpj[p] + dOne;
+                a2 = rkj * (TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * cpj + dOne;
                 a3 = rkj * (-re1 * MM(ran, i, j));
             } else {
                 a1 = rkj * (-re1 * MM(ran, i - 1, j));
-                a2 = rkj * (-TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * t.cpj[p] + dOne;
+                a2 = rkj * (-TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * cpj + dOne;
                 a3 = rkj * (TT(cj1, i + 1, j) - re1 * MM(ran, i, j));
             }
-            const T buoy = dk * (cv<T>(d[WD2_IX(i, j + 1)]) + cv<T>(d[p]) + cv<T>(dn[WD2_IX(i, j + 1)]) + cv<T>(dn[p])) /
-                           (dFour * fr);
+            const T d1 = den ? cv<T>(d[WD2_IX(i, j + 1)]) : T(0), d0 = den ? cv<T>(d[p]) : T(0);
+            const T n1 = den ? cv<T>(dn[WD2_IX(i, j + 1)]) : T(0), n0 = den ? cv<T>(dn[p]) : T(0);
+            const T buoy = dk * (d1 + d0 + n1 + n0) / (dFour * fr);
             ch.b[k] = cv<T>(fn[p] - fs[p]) + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
-                      dk2 * (t.cps[p] * cv<T>(fs[p]) + t.cpn[p] * cv<T>(fn[p])) - buoy;
+                      dk2 * (cps * cv<T>(fs[p]) + cpn * cv<T>(fn[p])) - buoy;
         } else {
             if (t.cj2[p] >= dZero) {
                 a1 = rkj * (-TT(cj2, i, j - 1) - re1 * MM(rgc, i, j));
-                a2 = rkj * (TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * t.cpj[p] + dOne;
+                a2 = rkj * (TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * cpj + dOne;
                 a3 = rkj * (-re1 * MM(rgc, i, j + 1));
             } else {
                 a1 = rkj * (-re1 * MM(rgc, i, j));
-                a2 = rkj * (-TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * t.cpj[p] + dOne;
+                a2 = rkj * (-TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * cpj + dOne;
                 a3 = rkj * (TT(cj2, i, j + 1) - re1 * MM(rgc, i, j + 1));
             }
             T bb = xprev[k];
@@ -404,18 +411,19 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
 template <typename T>
 void launch_assemble(const Regions &R, int ld, int comp, int split, double dk, double re, double fr,
                      const Metrics &m, const MomTempsT<T> &t, const double *fs, const double *fn, const double *d,
-                     const double *dn, const T *xprev, const ChainT<T> &ch, hipStream_t s, int *err) {
+                     const double *dn, const T *xprev, const ChainT<T> &ch, hipStream_t s, int *err, int live) {
     const double dk2 = dk * 0.5, re1 = 1.0 / re;
     const dim3 g = comp == 1 ? mgrid(R.nx, R.ny - 1) : mgrid(R.nx - 1, R.ny);
     hipLaunchKernelGGL(k_assemble<T>, g, dim3(MX, MY), 0, s, R, ld, comp, split, dk, dk2, re1, fr, m, t, fs, fn, d,
-                       dn, xprev, ch, err);
+                       dn, xprev, ch, err, live);
 }
 template void launch_assemble<double>(const Regions &, int, int, int, double, double, double, const Metrics &,
                                       const MomTempsT<double> &, const double *, const double *, const double *,
-                                      const double *, const double *, const ChainT<double> &, hipStream_t, int *);
+                                      const double *, const double *, const ChainT<double> &, hipStream_t, int *,
+                                      int);
 template void launch_assemble<float>(const Regions &, int, int, int, double, double, double, const Metrics &,
                                      const MomTempsT<float> &, const double *, const double *, const double *,
-                                     const double *, const float *, const ChainT<float> &, hipStream_t, int *);
+                                     const double *, const float *, const ChainT<float> &, hipStream_t, int *, int);
 
 // ------------------------------------------------------------- scatter
 template <typename T>

@@ -94,6 +94,9 @@ struct wd2_ctx {
     int *tile_flags = nullptr;
     int sweep_id = 0;
     bool coef_ready = false;
+    // d (and dn) may be non-zero: set once EqState runs or a caller sends d;
+    // until then both hold +0 and the momentum assembly does not load them
+    bool d_live = false;
     int precision = WD2_PREC_FP64;
     // pinned copies of the stop flag and events of the SOR batches in flight
     int *h_flag = nullptr;
@@ -487,9 +490,10 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
     launch_conv_diff<T>(nx, ny, ld, comp, M, fs, fn, t, s);
     if (c->any_porous) launch_poros_coef<T>(c->R, ld, comp, us, vs, un, vn, t, s);
     const long n = comp == 1 ? (long)nx * (ny - 1) : (long)(nx - 1) * ny;
-    launch_assemble<T>(c->R, ld, comp, 1, dk, re, fr, M, t, fs, fn, c->d, c->dn, nullptr, ch, s, &c->sc->err);
+    const int live = (c->any_porous ? 1 : 0) | (c->d_live ? 2 : 0);
+    launch_assemble<T>(c->R, ld, comp, 1, dk, re, fr, M, t, fs, fn, c->d, c->dn, nullptr, ch, s, &c->sc->err, live);
     launch_trid<T>(tw, n, n, ch.a, ch.d, ch.c, ch.b, ch.x, s);
-    launch_assemble<T>(c->R, ld, comp, 2, dk, re, fr, M, t, fs, fn, c->d, c->dn, ch.x, ch, s, &c->sc->err);
+    launch_assemble<T>(c->R, ld, comp, 2, dk, re, fr, M, t, fs, fn, c->d, c->dn, ch.x, ch, s, &c->sc->err, live);
     launch_trid<T>(tw, n, n, ch.a, ch.d, ch.c, ch.b, ch.x, s);
     launch_scatter<T>(nx, ny, ld, comp, ch.x, out, s);
     return WD2_OK;
@@ -1008,6 +1012,7 @@ int wd2_y_momentum(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, 
         (st = up(c, vn, ld, c->vn)) || (st = up(c, d, ld, c->d)) || (st = up(c, dn, ld, c->dn)) ||
         (st = up(c, dvs, ld, c->dvs)))
         return st;
+    if (d || dn) c->d_live = true;
     if ((st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs))) return st;
     if ((st = check_err(c))) return st;
     if ((st = down(c, c->dvs, dvs, ld))) return st;
@@ -1040,6 +1045,7 @@ int wd2_naux_momentum(int nx, int ny, int ld, int mgri, int mgrj, int mqiter, co
     if ((st = up(c, d, ld, c->d)) || (st = up(c, dn, ld, c->dn)) || (st = up(c, un, ld, c->un)) ||
         (st = up(c, vn, ld, c->vn)) || (st = up(c, us, ld, c->us)) || (st = up(c, vs, ld, c->vs)))
         return st;
+    if (d || dn) c->d_live = true;
     // nAuxMomentum zeroes the increments on 0..nx+1 x 0..ny+1 each iteration
     launch_fill(c->ld, 0, nx + 1, 0, ny + 1, 0.0, c->dus, c->s);
     launch_fill(c->ld, 0, nx + 1, 0, ny + 1, 0.0, c->dvs, c->s);
@@ -1139,6 +1145,7 @@ int wd2_eq_state(int nx, int ny, int ld, double uref, double densref, double tma
     wd2_ctx *c = g_ctx;
     // den is written on the interior only: its other entries come back as sent
     if ((st = up(c, p, ld, c->p)) || (st = up(c, t, ld, c->t)) || (st = up(c, den, ld, c->d))) return st;
+    c->d_live = true;
     launch_eq_state(nx, ny, c->ld, uref, densref, tmax, tref, rconst, c->p, c->t, c->d, c->s);
     return down(c, c->d, den, ld);
 }
@@ -1249,6 +1256,7 @@ int wd2_ctx_set_options(wd2_ctx *c, const wd2_step_options *opt) {
     if (opt->nthermen == 1 && !(opt->pe > 0.0))
         return fail(WD2_ERR_ARG, "wd2_ctx_set_options: the energy equation needs a Peclet number > 0");
     c->opt = *opt;
+    if (opt->neqstate == 1) c->d_live = true;   // EqState writes d
     return WD2_OK;
 }
 
@@ -1264,6 +1272,7 @@ int wd2_ctx_set_thermal(wd2_ctx *c, const int *nTRgType, const int *nTemBdTp, co
 int wd2_ctx_set_thermal_fields(wd2_ctx *c, int ld, const double *t, const double *d) {
     int st;
     if ((st = up(c, t, ld, c->t)) || (st = up(c, d, ld, c->d))) return st;
+    if (d) c->d_live = true;
     HIPCHK(hipStreamSynchronize(c->s));
     return WD2_OK;
 }

@@ -148,10 +148,13 @@ void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, con
 template <typename T>
 void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *un,
                       const MomTempsT<T> &t, hipStream_t s);
+// live: bit 0, porous regions (load the Dupuit-Forchheimer coefficients);
+// bit 1, a density field (load d, dn); otherwise those values are +0.
 template <typename T>
 void launch_assemble(const Regions &R, int ld, int comp, int split, double dk, double re, double fr,
                      const Metrics &m, const MomTempsT<T> &t, const double *us, const double *un, const double *d,
-                     const double *dn, const T *xprev, const ChainT<T> &ch, hipStream_t s, int *err);
+                     const double *dn, const T *xprev, const ChainT<T> &ch, hipStream_t s, int *err,
+                     int live);
 template <typename T>
 void launch_scatter(int nx, int ny, int ld, int comp, const T *x, double *out, hipStream_t s);
 void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const double *dus, const double *dvs,
