@@ -32,6 +32,52 @@ static inline dim3 mgrid(int ni, int nj) { return dim3((ni + MX - 1) / MX, (nj +
 template <typename T>
 __device__ inline T cv(double x) { return (T)x; }
 
+// The centred face coefficients of ConvCoef cases 1 (cc_x1, cc_x2) and 2
+// (cc_y1, cc_y2) at one point (momentum.f:901-925), shared by k_conv_coef and
+// the fused right-hand-side kernels, which evaluate them on the fly: one
+// expression, so the same bits either way.
+#define U(a, b) cv<T>(u[WD2_IX(a, b)])
+#define V(a, b) cv<T>(v[WD2_IX(a, b)])
+#define M(f, a, b) cv<T>(mget(f, ld, a, b))
+template <typename T>
+__device__ inline T cc_x1(int ld, int i, int j, T djac, const MArr &xet, const MArr &yet,
+                          const double *__restrict__ u, const double *__restrict__ v) {
+    return (djac * M(yet, i, j) * (U(i, j) + U(i - 1, j)) - M(xet, i, j) * (V(i, j) + V(i, j - 1))) * T(0.5);
+}
+template <typename T>
+__device__ inline T cc_x2(int ld, int i, int j, T djac, const MArr &xzi, const MArr &yzi,
+                          const double *__restrict__ u, const double *__restrict__ v) {
+    return (M(xzi, i, j) * (V(i + 1, j) + V(i, j)) - djac * M(yzi, i, j) * (U(i, j + 1) + U(i, j))) * T(0.5);
+}
+template <typename T>
+__device__ inline T cc_y1(int ld, int i, int j, T djac, const MArr &xet, const MArr &yet,
+                          const double *__restrict__ u, const double *__restrict__ v) {
+    return (M(yet, i, j) * (U(i, j + 1) + U(i, j)) - djac * M(xet, i, j) * (V(i + 1, j) + V(i, j))) * T(0.5);
+}
+template <typename T>
+__device__ inline T cc_y2(int ld, int i, int j, T djac, const MArr &xzi, const MArr &yzi,
+                          const double *__restrict__ u, const double *__restrict__ v) {
+    return (djac * M(xzi, i, j) * (V(i, j) + V(i, j - 1)) - M(yzi, i, j) * (U(i, j) + U(i - 1, j))) * T(0.5);
+}
+// ConvCoef cases 4 and 5 (upwind Jacobian, momentum.f:927-961)
+template <typename T>
+__device__ inline void cc_4(int ld, int i, int j, T djac, const MArr &xzi, const MArr &xet, const MArr &yzi,
+                            const MArr &yet, const double *__restrict__ u, const double *__restrict__ v, T &c1, T &c2) {
+    const T dFour = T(4.0);
+    c1 = djac * M(yet, i, j) * U(i, j) - M(xet, i, j) * (V(i + 1, j) + V(i, j) + V(i + 1, j - 1) + V(i, j - 1)) / dFour;
+    c2 = M(xzi, i, j) * (V(i + 1, j) + V(i, j) + V(i + 1, j - 1) + V(i, j - 1)) / dFour - djac * M(yzi, i, j) * U(i, j);
+}
+template <typename T>
+__device__ inline void cc_5(int ld, int i, int j, T djac, const MArr &xzi, const MArr &xet, const MArr &yzi,
+                            const MArr &yet, const double *__restrict__ u, const double *__restrict__ v, T &c1, T &c2) {
+    const T dFour = T(4.0);
+    c1 = M(yet, i, j) * (U(i, j + 1) + U(i - 1, j + 1) + U(i, j) + U(i - 1, j)) / dFour - djac * M(xet, i, j) * V(i, j);
+    c2 = djac * M(xzi, i, j) * V(i, j) - M(yzi, i, j) * (U(i, j + 1) + U(i - 1, j + 1) + U(i, j) + U(i - 1, j)) / dFour;
+}
+#undef U
+#undef V
+#undef M
+
 template <int NCOMP, typename T>
 __global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, double djac_, MArr xzi, MArr xet,
                                                       MArr yzi, MArr yet, const double *__restrict__ u,
@@ -45,15 +91,11 @@ __global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, do
 #define M(f, a, b) cv<T>(mget(f, ld, a, b))
     const size_t k = WD2_IX(i, j);
     if (NCOMP == 1) {
-        if (i >= 1 && j >= 1)
-            cc1[k] = (djac * M(yet, i, j) * (U(i, j) + U(i - 1, j)) - M(xet, i, j) * (V(i, j) + V(i, j - 1))) * dHalf;
-        if (i >= 1 && i <= nx && j >= 1 && j <= ny)
-            cc2[k] = (M(xzi, i, j) * (V(i + 1, j) + V(i, j)) - djac * M(yzi, i, j) * (U(i, j + 1) + U(i, j))) * dHalf;
+        if (i >= 1 && j >= 1) cc1[k] = cc_x1<T>(ld, i, j, djac, xet, yet, u, v);
+        if (i >= 1 && i <= nx && j >= 1 && j <= ny) cc2[k] = cc_x2<T>(ld, i, j, djac, xzi, yzi, u, v);
     } else if (NCOMP == 2) {
-        if (i >= 1 && i <= nx && j >= 1 && j <= ny)
-            cc1[k] = (M(yet, i, j) * (U(i, j + 1) + U(i, j)) - djac * M(xet, i, j) * (V(i + 1, j) + V(i, j))) * dHalf;
-        if (i >= 1 && j >= 1)
-            cc2[k] = (djac * M(xzi, i, j) * (V(i, j) + V(i, j - 1)) - M(yzi, i, j) * (U(i, j) + U(i - 1, j))) * dHalf;
+        if (i >= 1 && i <= nx && j >= 1 && j <= ny) cc1[k] = cc_y1<T>(ld, i, j, djac, xet, yet, u, v);
+        if (i >= 1 && j >= 1) cc2[k] = cc_y2<T>(ld, i, j, djac, xzi, yzi, u, v);
     } else if (NCOMP == 3) {   // thermal.f:110-112 (time-level-n face coefficients)
         if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
             cc1[k] = (M(yet, i, j) * U(i, j) -
@@ -67,19 +109,9 @@ __global__ __launch_bounds__(MX *MY) void k_conv_coef(int nx, int ny, int ld, do
             cc2[k] = (M(xzi, i, j) * (V(i, j) + V(i, j - 1)) - M(yzi, i, j) * (U(i, j) + U(i - 1, j))) * dHalf;
         }
     } else if (NCOMP == 4) {
-        if (i <= nx && j >= 1 && j <= ny) {
-            cc1[k] = djac * M(yet, i, j) * U(i, j) -
-                     M(xet, i, j) * (V(i + 1, j) + V(i, j) + V(i + 1, j - 1) + V(i, j - 1)) / dFour;
-            cc2[k] = M(xzi, i, j) * (V(i + 1, j) + V(i, j) + V(i + 1, j - 1) + V(i, j - 1)) / dFour -
-                     djac * M(yzi, i, j) * U(i, j);
-        }
+        if (i <= nx && j >= 1 && j <= ny) cc_4<T>(ld, i, j, djac, xzi, xet, yzi, yet, u, v, cc1[k], cc2[k]);
     } else if (NCOMP == 5) {
-        if (i >= 1 && i <= nx && j <= ny) {
-            cc1[k] = M(yet, i, j) * (U(i, j + 1) + U(i - 1, j + 1) + U(i, j) + U(i - 1, j)) / dFour -
-                     djac * M(xet, i, j) * V(i, j);
-            cc2[k] = djac * M(xzi, i, j) * V(i, j) -
-                     M(yzi, i, j) * (U(i, j + 1) + U(i - 1, j + 1) + U(i, j) + U(i - 1, j)) / dFour;
-        }
+        if (i >= 1 && i <= nx && j <= ny) cc_5<T>(ld, i, j, djac, xzi, xet, yzi, yet, u, v, cc1[k], cc2[k]);
     }
 #undef U
 #undef V
@@ -204,22 +236,23 @@ template void launch_poros_coef<float>(const Regions &, int, int, const double *
                                        const double *, const MomTempsT<float> &, hipStream_t);
 
 // ---------------------------------------------------- convection/diffusion
+// DConvU / DConvV (momentum.f:987-1030) at one point, from the four face
+// coefficients around it: west/east c1 and south/north c2 (DConvU: c1(i,j),
+// c1(i+1,j), c2(i,j-1), c2(i,j); DConvV: c1(i-1,j), c1(i,j), c2(i,j),
+// c2(i,j+1)).
+#define U(a, b) cv<T>(u[WD2_IX(a, b)])
+template <typename T>
+__device__ inline T dconv(int ld, int i, int j, T c1w, T c1e, T c2s, T c2n, const double *u) {
+    return -c2s * U(i, j - 1) - c1w * U(i - 1, j) + (c1e - c1w + c2n - c2s) * U(i, j) + c1e * U(i + 1, j) +
+           c2n * U(i, j + 1);
+}
 template <typename T>
 __device__ inline T dconv_u(int ld, int i, int j, const T *c1, const T *c2, const double *u) {
-#define C1(a, b) c1[WD2_IX(a, b)]
-#define C2(a, b) c2[WD2_IX(a, b)]
-#define U(a, b) cv<T>(u[WD2_IX(a, b)])
-    return -C2(i, j - 1) * U(i, j - 1) - C1(i, j) * U(i - 1, j) +
-           (C1(i + 1, j) - C1(i, j) + C2(i, j) - C2(i, j - 1)) * U(i, j) + C1(i + 1, j) * U(i + 1, j) +
-           C2(i, j) * U(i, j + 1);
+    return dconv<T>(ld, i, j, c1[WD2_IX(i, j)], c1[WD2_IX(i + 1, j)], c2[WD2_IX(i, j - 1)], c2[WD2_IX(i, j)], u);
 }
 template <typename T>
 __device__ inline T dconv_v(int ld, int i, int j, const T *c1, const T *c2, const double *u) {
-    return -C2(i, j) * U(i, j - 1) - C1(i - 1, j) * U(i - 1, j) +
-           (C1(i, j) - C1(i - 1, j) + C2(i, j + 1) - C2(i, j)) * U(i, j) + C1(i, j) * U(i + 1, j) +
-           C2(i, j + 1) * U(i, j + 1);
-#undef C1
-#undef C2
+    return dconv<T>(ld, i, j, c1[WD2_IX(i - 1, j)], c1[WD2_IX(i, j)], c2[WD2_IX(i, j)], c2[WD2_IX(i, j + 1)], u);
 }
 template <typename T>
 __device__ inline T ddiff_u(int ld, int i, int j, const MArr &ac, const MArr &bc, const MArr &bn, const MArr &gn,
@@ -280,6 +313,104 @@ template void launch_conv_diff<double>(int, int, int, int, const Metrics &, cons
                                        const MomTempsT<double> &, hipStream_t);
 template void launch_conv_diff<float>(int, int, int, int, const Metrics &, const double *, const double *,
                                       const MomTempsT<float> &, hipStream_t);
+
+// Fused right-hand-side terms for grids without porous regions.  The
+// coefficients ConvCoef writes (cases 1 and 2 with njacob 0) are only read
+// by DConvU / DConvV, so these kernels evaluate them where they are needed
+// instead of storing them; each is the same expression as in k_conv_coef, so
+// the bits are the same.  Every coefficient read lies in ConvCoef's write
+// range (DConvU: c1 at i = 1..nx+1, c2 at j = 1..ny; DConvV: c1 at
+// i = 1..nx, c2 at j = 1..ny+1).
+//
+// k_mom_n: the time-level-n terms cnvn and difn of one component, which
+// depend only on un, vn and the metrics; the q-l iterations of one
+// nAuxMomentum call reuse them.
+template <int COMP, typename T>
+__global__ __launch_bounds__(MX *MY) void k_mom_n(int nx, int ny, int ld, Metrics m, const double *__restrict__ un,
+                                                  const double *__restrict__ vn, T *__restrict__ cnvn,
+                                                  T *__restrict__ difn) {
+    const T one = cv<T>(1.0);
+    if (COMP == 1) {
+        const int i = 1 + blockIdx.x * MX + threadIdx.x, j = 2 + blockIdx.y * MY + threadIdx.y;
+        if (i > nx || j > ny) return;
+        const MArr xet = MARR(m, xec), yet = MARR(m, yec), xzi = MARR(m, xzn), yzi = MARR(m, yzn);
+        const T c1w = cc_x1<T>(ld, i, j, one, xet, yet, un, vn), c1e = cc_x1<T>(ld, i + 1, j, one, xet, yet, un, vn);
+        const T c2s = cc_x2<T>(ld, i, j - 1, one, xzi, yzi, un, vn), c2n = cc_x2<T>(ld, i, j, one, xzi, yzi, un, vn);
+        const size_t k = WD2_IX(i, j);
+        cnvn[k] = dconv<T>(ld, i, j, c1w, c1e, c2s, c2n, un);
+        difn[k] = ddiff_u<T>(ld, i, j, MARR(m, rac), MARR(m, rbc), MARR(m, rbn), MARR(m, rgn), un);
+    } else {
+        const int i = 2 + blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
+        if (i > nx || j > ny) return;
+        const MArr xet = MARR(m, xen), yet = MARR(m, yen), xzi = MARR(m, xzc), yzi = MARR(m, yzc);
+        const T c1w = cc_y1<T>(ld, i - 1, j, one, xet, yet, un, vn), c1e = cc_y1<T>(ld, i, j, one, xet, yet, un, vn);
+        const T c2s = cc_y2<T>(ld, i, j, one, xzi, yzi, un, vn), c2n = cc_y2<T>(ld, i, j + 1, one, xzi, yzi, un, vn);
+        const size_t k = WD2_IX(i, j);
+        cnvn[k] = dconv<T>(ld, i, j, c1w, c1e, c2s, c2n, vn);
+        difn[k] = ddiff_v<T>(ld, i, j, MARR(m, ran), MARR(m, rbc), MARR(m, rbn), MARR(m, rgc), vn);
+    }
+}
+
+// k_mom_s: the upwind-Jacobian coefficients cj1, cj2 (ConvCoef case 4 / 5,
+// njacob 1, on its write range) and the terms cnvs, difs of the current
+// iterate, in one pass over us and vs.
+template <int COMP, typename T>
+__global__ __launch_bounds__(MX *MY) void k_mom_s(int nx, int ny, int ld, Metrics m, const double *__restrict__ us,
+                                                  const double *__restrict__ vs, MomTempsT<T> t) {
+    const T one = cv<T>(1.0), two = cv<T>(2.0);
+    if (COMP == 1) {
+        // cover i = 0..nx, j = 1..ny: case 4's range; DConvU / DDiffU on i >= 1, j >= 2
+        const int i = blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
+        if (i > nx || j > ny) return;
+        const size_t k = WD2_IX(i, j);
+        cc_4<T>(ld, i, j, two, MARR(m, xzu), MARR(m, xeu), MARR(m, yzu), MARR(m, yeu), us, vs, t.cj1[k], t.cj2[k]);
+        if (i >= 1 && j >= 2) {
+            const MArr xet = MARR(m, xec), yet = MARR(m, yec), xzi = MARR(m, xzn), yzi = MARR(m, yzn);
+            const T c1w = cc_x1<T>(ld, i, j, one, xet, yet, us, vs), c1e = cc_x1<T>(ld, i + 1, j, one, xet, yet, us, vs);
+            const T c2s = cc_x2<T>(ld, i, j - 1, one, xzi, yzi, us, vs), c2n = cc_x2<T>(ld, i, j, one, xzi, yzi, us, vs);
+            t.cnvs[k] = dconv<T>(ld, i, j, c1w, c1e, c2s, c2n, us);
+            t.difs[k] = ddiff_u<T>(ld, i, j, MARR(m, rac), MARR(m, rbc), MARR(m, rbn), MARR(m, rgn), us);
+        }
+    } else {
+        // cover i = 1..nx, j = 0..ny: case 5's range; DConvV / DDiffV on i >= 2, j >= 1
+        const int i = 1 + blockIdx.x * MX + threadIdx.x, j = blockIdx.y * MY + threadIdx.y;
+        if (i > nx || j > ny) return;
+        const size_t k = WD2_IX(i, j);
+        cc_5<T>(ld, i, j, two, MARR(m, xzv), MARR(m, xev), MARR(m, yzv), MARR(m, yev), us, vs, t.cj1[k], t.cj2[k]);
+        if (i >= 2 && j >= 1) {
+            const MArr xet = MARR(m, xen), yet = MARR(m, yen), xzi = MARR(m, xzc), yzi = MARR(m, yzc);
+            const T c1w = cc_y1<T>(ld, i - 1, j, one, xet, yet, us, vs), c1e = cc_y1<T>(ld, i, j, one, xet, yet, us, vs);
+            const T c2s = cc_y2<T>(ld, i, j, one, xzi, yzi, us, vs), c2n = cc_y2<T>(ld, i, j + 1, one, xzi, yzi, us, vs);
+            t.cnvs[k] = dconv<T>(ld, i, j, c1w, c1e, c2s, c2n, vs);
+            t.difs[k] = ddiff_v<T>(ld, i, j, MARR(m, ran), MARR(m, rbc), MARR(m, rbn), MARR(m, rgc), vs);
+        }
+    }
+}
+
+template <typename T>
+void launch_mom_n(int nx, int ny, int ld, int comp, const Metrics &m, const double *un, const double *vn, T *cnvn,
+                  T *difn, hipStream_t s) {
+    if (comp == 1)
+        hipLaunchKernelGGL((k_mom_n<1, T>), mgrid(nx, ny - 1), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn, difn);
+    else
+        hipLaunchKernelGGL((k_mom_n<2, T>), mgrid(nx - 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn, difn);
+}
+template <typename T>
+void launch_mom_s(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *vs,
+                  const MomTempsT<T> &t, hipStream_t s) {
+    if (comp == 1)
+        hipLaunchKernelGGL((k_mom_s<1, T>), mgrid(nx + 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, us, vs, t);
+    else
+        hipLaunchKernelGGL((k_mom_s<2, T>), mgrid(nx, ny + 1), dim3(MX, MY), 0, s, nx, ny, ld, m, us, vs, t);
+}
+template void launch_mom_n<double>(int, int, int, int, const Metrics &, const double *, const double *, double *,
+                                   double *, hipStream_t);
+template void launch_mom_n<float>(int, int, int, int, const Metrics &, const double *, const double *, float *,
+                                  float *, hipStream_t);
+template void launch_mom_s<double>(int, int, int, int, const Metrics &, const double *, const double *,
+                                   const MomTempsT<double> &, hipStream_t);
+template void launch_mom_s<float>(int, int, int, int, const Metrics &, const double *, const double *,
+                                  const MomTempsT<float> &, hipStream_t);
 
 // ------------------------------------------------------------ assembly
 // comp 1 (x-momentum): rows (i=1..nx, j=2..ny), k = (j-2)*nx + i-1

@@ -183,7 +183,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
                              hipHostMallocDefault));
         HIPCHK(hipEventCreateWithFlags(&c->sev[k], hipEventDisableTiming));
     }
-    const int NFIELDS = 16 + 4 + 4 + 5 + 13;
+    const int NFIELDS = 16 + 4 + 4 + 5 + 15;
     const size_t nchain = chain_rows(nx, ny);
     const size_t total = NFIELDS * c->nf + 5 * nchain;
     HIPCHK(hipMalloc(&c->store, total * sizeof(double)));
@@ -196,7 +196,8 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->tt = ThermTemps{take(c->nf), take(c->nf), take(c->nf), take(c->nf)};
     for (int k = 0; k < 5; k++) c->pc[k] = take(c->nf);
     double **t[] = {&c->T.cj1, &c->T.cj2, &c->T.c1s, &c->T.c2s, &c->T.c1n, &c->T.c2n, &c->T.cnvs,
-                    &c->T.cnvn, &c->T.difs, &c->T.difn, &c->T.cps, &c->T.cpn, &c->T.cpj};
+                    &c->T.cnvn, &c->T.difs, &c->T.difn, &c->T.cps, &c->T.cpn, &c->T.cpj, &c->T.cnvn2,
+                    &c->T.difn2};
     for (double **pp : t) *pp = take(c->nf);
     c->ch.a = take(nchain);
     c->ch.d = take(nchain);
@@ -215,7 +216,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     auto F = [](double *q) { return reinterpret_cast<float *>(q); };
     c->Tf = MomTempsT<float>{F(c->T.cj1),  F(c->T.cj2),  F(c->T.c1s),  F(c->T.c2s), F(c->T.c1n),
                              F(c->T.c2n),  F(c->T.cnvs), F(c->T.cnvn), F(c->T.difs), F(c->T.difn),
-                             F(c->T.cps),  F(c->T.cpn),  F(c->T.cpj)};
+                             F(c->T.cps),  F(c->T.cpn),  F(c->T.cpj),  F(c->T.cnvn2), F(c->T.difn2)};
     c->chf = ChainT<float>{F(c->ch.a), F(c->ch.d), F(c->ch.c), F(c->ch.b), F(c->ch.x)};
     c->twf = TridWorkT<float>{F(c->tw.cp),    F(c->tw.bp),     F(c->tw.blk_c), F(c->tw.blk_b),
                               F(c->tw.blk_x), F(c->tw.blk_xe), c->tw.fail,     c->tw.nfallback};
@@ -469,14 +470,26 @@ static int check_err(wd2_ctx *c) {
 // -------------------------------------------------------------- momentum
 // XMomentum (comp 1) / YMomentum (comp 2) on temporaries and chains of type T
 // (double: the reference's arithmetic; float: the mixed-precision momentum).
+// n_ready: this component's time-level-n terms (cnvn, difn) are already in
+// place from an earlier call on the same un, vn and metrics.
 template <typename T>
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
-                    const double *un, const double *vn, const MomTempsT<T> &t, const ChainT<T> &ch,
-                    const TridWorkT<T> &tw, double *out) {
+                    const double *un, const double *vn, const MomTempsT<T> &t0, const ChainT<T> &ch,
+                    const TridWorkT<T> &tw, bool n_ready, double *out) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     const Metrics &M = c->M;
     hipStream_t s = c->s;
-    if (comp == 1) {
+    MomTempsT<T> t = t0;
+    if (comp == 2) {
+        t.cnvn = t0.cnvn2;
+        t.difn = t0.difn2;
+    }
+    const double *fs = comp == 1 ? us : vs, *fn = comp == 1 ? un : vn;
+    if (!c->any_porous) {
+        // fused: the centred coefficients are evaluated where they are read
+        if (!n_ready) launch_mom_n<T>(nx, ny, ld, comp, M, un, vn, t.cnvn, t.difn, s);
+        launch_mom_s<T>(nx, ny, ld, comp, M, us, vs, t, s);
+    } else if (comp == 1) {
         launch_conv_coef<T>(nx, ny, ld, 4, 1, MARR(M, xzu), MARR(M, xeu), MARR(M, yzu), MARR(M, yeu), us, vs, t.cj1, t.cj2, s);
         launch_conv_coef<T>(nx, ny, ld, 1, 0, MARR(M, xzn), MARR(M, xec), MARR(M, yzn), MARR(M, yec), us, vs, t.c1s, t.c2s, s);
         launch_conv_coef<T>(nx, ny, ld, 1, 0, MARR(M, xzn), MARR(M, xec), MARR(M, yzn), MARR(M, yec), un, vn, t.c1n, t.c2n, s);
@@ -485,10 +498,11 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
         launch_conv_coef<T>(nx, ny, ld, 2, 0, MARR(M, xzc), MARR(M, xen), MARR(M, yzc), MARR(M, yen), us, vs, t.c1s, t.c2s, s);
         launch_conv_coef<T>(nx, ny, ld, 2, 0, MARR(M, xzc), MARR(M, xen), MARR(M, yzc), MARR(M, yen), un, vn, t.c1n, t.c2n, s);
     }
-    if (c->any_porous) launch_porous_div<T>(c->R, ld, comp, t, s);
-    const double *fs = comp == 1 ? us : vs, *fn = comp == 1 ? un : vn;
-    launch_conv_diff<T>(nx, ny, ld, comp, M, fs, fn, t, s);
-    if (c->any_porous) launch_poros_coef<T>(c->R, ld, comp, us, vs, un, vn, t, s);
+    if (c->any_porous) {
+        launch_porous_div<T>(c->R, ld, comp, t, s);
+        launch_conv_diff<T>(nx, ny, ld, comp, M, fs, fn, t, s);
+        launch_poros_coef<T>(c->R, ld, comp, us, vs, un, vn, t, s);
+    }
     const long n = comp == 1 ? (long)nx * (ny - 1) : (long)(nx - 1) * ny;
     const int live = (c->any_porous ? 1 : 0) | (c->d_live ? 2 : 0);
     launch_assemble<T>(c->R, ld, comp, 1, dk, re, fr, M, t, fs, fn, c->d, c->dn, nullptr, ch, s, &c->sc->err, live);
@@ -500,10 +514,10 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
 }
 
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
-                    const double *un, const double *vn, double *out) {
+                    const double *un, const double *vn, double *out, bool n_ready = false) {
     if (c->precision == WD2_PREC_MIXED)
-        return momentum<float>(c, comp, dk, re, fr, us, vs, un, vn, c->Tf, c->chf, c->twf, out);
-    return momentum<double>(c, comp, dk, re, fr, us, vs, un, vn, c->T, c->ch, c->tw, out);
+        return momentum<float>(c, comp, dk, re, fr, us, vs, un, vn, c->Tf, c->chf, c->twf, n_ready, out);
+    return momentum<double>(c, comp, dk, re, fr, us, vs, un, vn, c->T, c->ch, c->tw, n_ready, out);
 }
 
 // momentum.f:nAuxMomentum 33-193 on the context's un, vn -> us, vs.
@@ -514,9 +528,11 @@ static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr
     *nql = -1;
     for (int m = 1; m <= mqiter; m++) {
         if (c->any_outlet) launch_vel_bc<double>(c->R, ld, c->us, c->vs, true, &c->sc->err, c->s);
-        int st = momentum(c, 1, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dus);
+        // un, vn and the metrics are fixed during the iterations: the
+        // time-level-n terms of the first iteration serve them all
+        int st = momentum(c, 1, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dus, m > 1);
         if (st) return st;
-        st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs);
+        st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs, m > 1);
         if (st) return st;
         launch_update_norms(nx, ny, ld, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, c->s);
         double nv[2];
@@ -1303,7 +1319,7 @@ int wd2_ctx_set_precision(wd2_ctx *c, int precision) {
         // the momentum kernels never write are read as 0 (as in the reference),
         // so clear the whole block when the arithmetic changes.
         const size_t nchain = chain_rows(c->nx, c->ny);
-        HIPCHK(hipMemsetAsync(c->T.cj1, 0, (13 * c->nf + 5 * nchain) * sizeof(double), c->s));
+        HIPCHK(hipMemsetAsync(c->T.cj1, 0, (15 * c->nf + 5 * nchain) * sizeof(double), c->s));
         HIPCHK(hipStreamSynchronize(c->s));
     }
     c->precision = precision;

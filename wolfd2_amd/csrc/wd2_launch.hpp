@@ -124,9 +124,11 @@ void launch_slor_rbp_relax(int nx, int ny, int ld, int first_line, double sorrel
 
 // Momentum (momentum.f).  T: type of the temporaries, chains and arithmetic
 // (double: the reference's; float: the mixed-precision path).
+// cnvn / difn hold the x-momentum's time-level-n terms, cnvn2 / difn2 the
+// y-momentum's, so both survive the q-l iterations of one nAuxMomentum call.
 template <typename T>
 struct MomTempsT {
-    T *cj1, *cj2, *c1s, *c2s, *c1n, *c2n, *cnvs, *cnvn, *difs, *difn, *cps, *cpn, *cpj;
+    T *cj1, *cj2, *c1s, *c2s, *c1n, *c2n, *cnvs, *cnvn, *difs, *difn, *cps, *cpn, *cpj, *cnvn2, *difn2;
 };
 template <typename T>
 struct ChainT {
@@ -148,6 +150,14 @@ void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, con
 template <typename T>
 void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *un,
                       const MomTempsT<T> &t, hipStream_t s);
+// Fused terms for grids without porous regions: k_mom_n writes cnvn / difn of
+// one component from un, vn; k_mom_s writes cj1, cj2, cnvs, difs from us, vs.
+template <typename T>
+void launch_mom_n(int nx, int ny, int ld, int comp, const Metrics &m, const double *un, const double *vn, T *cnvn,
+                  T *difn, hipStream_t s);
+template <typename T>
+void launch_mom_s(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *vs,
+                  const MomTempsT<T> &t, hipStream_t s);
 // live: bit 0, porous regions (load the Dupuit-Forchheimer coefficients);
 // bit 1, a density field (load d, dn); otherwise those values are +0.
 template <typename T>
