@@ -121,6 +121,44 @@ def test_alt_trid_lu_bitwise_vs_oracle(oracle, n, weak):
     assert np.array_equal(x_gpu, x_ref)
 
 
+def _two_fixed_points(rng):
+    """Constant rows (a, d, c, b) whose rounded forward recurrence has two
+    fixed points: a run from the zero state and a run from another state
+    settle on different values and never coincide bit for bit."""
+    def run(a, d, c, b, cp, bp, n):
+        out = []
+        for _ in range(n):
+            dp = d - (a * cp)
+            cp = c / dp
+            bp = (b - a * bp) / dp
+            out.append((cp, bp))
+        return out
+    while True:
+        a, c = -rng.uniform(0.05, 0.5), -rng.uniform(0.05, 0.5)
+        d, b = 1.0 + rng.uniform(0.0, 0.5), rng.uniform(-1.0, 1.0)
+        r1, r2 = run(a, d, c, b, 0.0, 0.0, 200), run(a, d, c, b, -0.3, 0.7, 200)
+        if all(p != q for p, q in zip(r1, r2)):
+            return a, d, c, b
+
+
+def test_alt_trid_lu_rounding_cycles_bitwise(oracle):
+    """Stretches of identical rows on which speculative runs never coincide
+    with the exact one (two fixed points of the rounded recurrence): inside a
+    2048-row block, across block boundaries and longer than a block.  The
+    forward sweep's serial walk and the cross-block repairs must still give
+    the sequential sweep's bits."""
+    rng = np.random.default_rng(7)
+    a, b = _chain(30_000, rng)
+    row = _two_fixed_points(rng)
+    for k0, length in [(100, 300), (1500, 1200), (5000, 40), (7000, 2600), (12_000, 5000), (20_000, 900)]:
+        a[k0:k0 + length] = row[:3]
+        b[k0:k0 + length] = row[3]
+    print("chain with rounding-cycle stretches", flush=True)
+    x_gpu = api.alt_trid_lu(a, b)
+    _, x_ref = oracle.alt_trid_lu(a, b)
+    assert np.array_equal(x_gpu, x_ref)
+
+
 def test_alt_trid_lu_sequential_fallback_bitwise(oracle):
     """The 1-D Laplacian chain (d = 2, a = c = -1) barely contracts, so the
     speculative segments never coincide bit for bit with the repaired ones:

@@ -351,42 +351,6 @@ __global__ __launch_bounds__(MX *MY) void k_mom_n(int nx, int ny, int ld, Metric
     }
 }
 
-// k_mom_s: the upwind-Jacobian coefficients cj1, cj2 (ConvCoef case 4 / 5,
-// njacob 1, on its write range) and the terms cnvs, difs of the current
-// iterate, in one pass over us and vs.
-template <int COMP, typename T>
-__global__ __launch_bounds__(MX *MY) void k_mom_s(int nx, int ny, int ld, Metrics m, const double *__restrict__ us,
-                                                  const double *__restrict__ vs, MomTempsT<T> t) {
-    const T one = cv<T>(1.0), two = cv<T>(2.0);
-    if (COMP == 1) {
-        // cover i = 0..nx, j = 1..ny: case 4's range; DConvU / DDiffU on i >= 1, j >= 2
-        const int i = blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
-        if (i > nx || j > ny) return;
-        const size_t k = WD2_IX(i, j);
-        cc_4<T>(ld, i, j, two, MARR(m, xzu), MARR(m, xeu), MARR(m, yzu), MARR(m, yeu), us, vs, t.cj1[k], t.cj2[k]);
-        if (i >= 1 && j >= 2) {
-            const MArr xet = MARR(m, xec), yet = MARR(m, yec), xzi = MARR(m, xzn), yzi = MARR(m, yzn);
-            const T c1w = cc_x1<T>(ld, i, j, one, xet, yet, us, vs), c1e = cc_x1<T>(ld, i + 1, j, one, xet, yet, us, vs);
-            const T c2s = cc_x2<T>(ld, i, j - 1, one, xzi, yzi, us, vs), c2n = cc_x2<T>(ld, i, j, one, xzi, yzi, us, vs);
-            t.cnvs[k] = dconv<T>(ld, i, j, c1w, c1e, c2s, c2n, us);
-            t.difs[k] = ddiff_u<T>(ld, i, j, MARR(m, rac), MARR(m, rbc), MARR(m, rbn), MARR(m, rgn), us);
-        }
-    } else {
-        // cover i = 1..nx, j = 0..ny: case 5's range; DConvV / DDiffV on i >= 2, j >= 1
-        const int i = 1 + blockIdx.x * MX + threadIdx.x, j = blockIdx.y * MY + threadIdx.y;
-        if (i > nx || j > ny) return;
-        const size_t k = WD2_IX(i, j);
-        cc_5<T>(ld, i, j, two, MARR(m, xzv), MARR(m, xev), MARR(m, yzv), MARR(m, yev), us, vs, t.cj1[k], t.cj2[k]);
-        if (i >= 2 && j >= 1) {
-            const MArr xet = MARR(m, xen), yet = MARR(m, yen), xzi = MARR(m, xzc), yzi = MARR(m, yzc);
-            const T c1w = cc_y1<T>(ld, i - 1, j, one, xet, yet, us, vs), c1e = cc_y1<T>(ld, i, j, one, xet, yet, us, vs);
-            const T c2s = cc_y2<T>(ld, i, j, one, xzi, yzi, us, vs), c2n = cc_y2<T>(ld, i, j + 1, one, xzi, yzi, us, vs);
-            t.cnvs[k] = dconv<T>(ld, i, j, c1w, c1e, c2s, c2n, vs);
-            t.difs[k] = ddiff_v<T>(ld, i, j, MARR(m, ran), MARR(m, rbc), MARR(m, rbn), MARR(m, rgc), vs);
-        }
-    }
-}
-
 template <typename T>
 void launch_mom_n(int nx, int ny, int ld, int comp, const Metrics &m, const double *un, const double *vn, T *cnvn,
                   T *difn, hipStream_t s) {
@@ -395,22 +359,10 @@ void launch_mom_n(int nx, int ny, int ld, int comp, const Metrics &m, const doub
     else
         hipLaunchKernelGGL((k_mom_n<2, T>), mgrid(nx - 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn, difn);
 }
-template <typename T>
-void launch_mom_s(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *vs,
-                  const MomTempsT<T> &t, hipStream_t s) {
-    if (comp == 1)
-        hipLaunchKernelGGL((k_mom_s<1, T>), mgrid(nx + 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, us, vs, t);
-    else
-        hipLaunchKernelGGL((k_mom_s<2, T>), mgrid(nx, ny + 1), dim3(MX, MY), 0, s, nx, ny, ld, m, us, vs, t);
-}
 template void launch_mom_n<double>(int, int, int, int, const Metrics &, const double *, const double *, double *,
                                    double *, hipStream_t);
 template void launch_mom_n<float>(int, int, int, int, const Metrics &, const double *, const double *, float *,
                                   float *, hipStream_t);
-template void launch_mom_s<double>(int, int, int, int, const Metrics &, const double *, const double *,
-                                   const MomTempsT<double> &, hipStream_t);
-template void launch_mom_s<float>(int, int, int, int, const Metrics &, const double *, const double *,
-                                  const MomTempsT<float> &, hipStream_t);
 
 // ------------------------------------------------------------ assembly
 // comp 1 (x-momentum): rows (i=1..nx, j=2..ny), k = (j-2)*nx + i-1
@@ -418,6 +370,66 @@ template void launch_mom_s<float>(int, int, int, int, const Metrics &, const dou
 // split 1: first split step with its right-hand side; split 2: second split
 // step (y-direction coefficients on the same x-ordered chain), right-hand
 // side = first-split solution, wall and blockage identity rows.
+//
+// One chain row's coefficients (XMomentum 350-496 / YMomentum 675-821) from
+// the upwind-Jacobian coefficient cj on the row's point (cj0) and its two
+// neighbours along the split direction (cjm below, cjp above) and the two
+// metric values of the split (mlo, mhi: rac(i,j), rac(i+1,j) / rgn(i,j-1),
+// rgn(i,j) for comp 1; ran(i-1,j), ran(i,j) / rgc(i,j), rgc(i,j+1) for
+// comp 2).  The two components sum the diagonal in different orders.
+template <int COMP, typename T>
+__device__ inline void asm_coef(T cjm, T cj0, T cjp, T mlo, T mhi, T rkj, T re1, T dk2, T cpj, T &a1, T &a2,
+                                T &a3) {
+    const T dZero = T(0.0), dOne = T(1.0);
+    if (cj0 >= dZero) {
+        a1 = rkj * (-cjm - re1 * mlo);
+        a2 = COMP == 1 ? dOne + rkj * (cj0 + re1 * (mhi + mlo)) + dk2 * cpj
+                       : rkj * (cj0 + re1 * (mhi + mlo)) + dk2 * cpj + dOne;
+        a3 = rkj * (-re1 * mhi);
+    } else {
+        a1 = rkj * (-re1 * mlo);
+        a2 = COMP == 1 ? dOne + rkj * (-cj0 + re1 * (mhi + mlo)) + dk2 * cpj
+                       : rkj * (-cj0 + re1 * (mhi + mlo)) + dk2 * cpj + dOne;
+        a3 = rkj * (cjp - re1 * mhi);
+    }
+}
+// The first split's right-hand side (buoy: the y-momentum's buoyancy term).
+template <int COMP, typename T>
+__device__ inline T asm_rhs(double fsp, double fnp, T rkj, T re1, T dk2, T cnvs, T cnvn, T difs, T difn, T cps,
+                            T cpn, T buoy) {
+    const T r = cv<T>(fnp - fsp) + rkj * (-cnvs - cnvn) + rkj * re1 * (difs + difn) -
+                dk2 * (cps * cv<T>(fsp) + cpn * cv<T>(fnp));
+    return COMP == 1 ? r : r - buoy;
+}
+// Whether the second split's row at (i, j) is an identity row (a wall or
+// inlet face on its split side, or a blocked region); a boundary type out of
+// range sets err (2001 / 2003).
+template <int COMP>
+__device__ inline bool asm_ident(const Regions &R, int i, int j, int *err) {
+    bool ident = false;
+    for (int jr = 1; jr <= R.nj; jr++)
+        for (int ir = 1; ir <= R.ni; ir++) {
+            const int iW = R.border(ir, jr, WEST), iE = R.border(ir, jr, EAST);
+            const int jS = R.border(ir, jr, SOUTH), jN = R.border(ir, jr, NORTH);
+            if (COMP == 1) {
+                const bool jin = j >= jS + 1 && j <= jN;
+                if (R.type(ir, jr) == RM_BLOCKG && jin && i >= iW && i <= iE) ident = true;
+                const int tw = R.btype(ir, jr, WEST), te = R.btype(ir, jr, EAST);
+                if ((tw == BM_WALL1 || tw == BM_WALL2 || tw == BM_INLET) && jin && i == iW) ident = true;
+                if ((te == BM_WALL1 || te == BM_WALL2 || te == BM_INLET) && jin && i == iE) ident = true;
+                if (tw < BM_INTERN || tw > BM_OUTLT2 || te < BM_INTERN || te > BM_OUTLT2) *err = 2001;
+            } else {
+                const bool iin = i >= iW + 1 && i <= iE;
+                if (R.type(ir, jr) == RM_BLOCKG && iin && j >= jS && j <= jN) ident = true;
+                const int ts = R.btype(ir, jr, SOUTH), tn = R.btype(ir, jr, NORTH);
+                if ((ts == BM_WALL1 || ts == BM_WALL2 || ts == BM_INLET) && iin && j == jS) ident = true;
+                if ((tn == BM_WALL1 || tn == BM_WALL2 || tn == BM_INLET) && iin && j == jN) ident = true;
+                if (ts < BM_INTERN || ts > BM_OUTLT2 || tn < BM_INTERN || tn > BM_OUTLT2) *err = 2003;
+            }
+        }
+    return ident;
+}
+
 template <typename T>
 __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp, int split, double dk_,
                                                      double dk2_, double re1_, double fr_, Metrics m, MomTempsT<T> t,
@@ -451,84 +463,32 @@ __global__ __launch_bounds__(MX *MY) void k_assemble(Regions R, int ld, int comp
     if (comp == 1) {
         const T rkj = dk2 * cv<T>(MGET(m, dju, i, j));
         if (split == 1) {
-            if (t.cj1[p] >= dZero) {
-                a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(rac, i, j));
-                a2 = dOne + rkj * (TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * cpj;
-                a3 = rkj * (-re1 * MM(rac, i + 1, j));
-            } else {
-                a1 = rkj * (-re1 * MM(rac, i, j));
-                a2 = dOne + rkj * (-TT(cj1, i, j) + re1 * (MM(rac, i + 1, j) + MM(rac, i, j))) + dk2 * cpj;
-                a3 = rkj * (TT(cj1, i + 1, j) - re1 * MM(rac, i + 1, j));
-            }
-            ch.b[k] = cv<T>(fn[p] - fs[p]) + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
-                      dk2 * (cps * cv<T>(fs[p]) + cpn * cv<T>(fn[p]));
+            asm_coef<1, T>(TT(cj1, i - 1, j), TT(cj1, i, j), TT(cj1, i + 1, j), MM(rac, i, j), MM(rac, i + 1, j), rkj,
+                           re1, dk2, cpj, a1, a2, a3);
+            ch.b[k] = asm_rhs<1, T>(fs[p], fn[p], rkj, re1, dk2, t.cnvs[p], t.cnvn[p], t.difs[p], t.difn[p], cps, cpn,
+                                    dZero);
         } else {
-            if (t.cj2[p] >= dZero) {
-                a1 = rkj * (-TT(cj2, i, j - 1) - re1 * MM(rgn, i, j - 1));
-                a2 = dOne + rkj * (TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * cpj;
-                a3 = rkj * (-re1 * MM(rgn, i, j));
-            } else {
-                a1 = rkj * (-re1 * MM(rgn, i, j - 1));
-                a2 = dOne + rkj * (-TT(cj2, i, j) + re1 * (MM(rgn, i, j) + MM(rgn, i, j - 1))) + dk2 * cpj;
-                a3 = rkj * (TT(cj2, i, j + 1) - re1 * MM(rgn, i, j));
-            }
+            asm_coef<1, T>(TT(cj2, i, j - 1), TT(cj2, i, j), TT(cj2, i, j + 1), MM(rgn, i, j - 1), MM(rgn, i, j), rkj,
+                           re1, dk2, cpj, a1, a2, a3);
             T bb = xprev[k];
-            bool ident = false;
-            for (int jr = 1; jr <= R.nj; jr++)
-                for (int ir = 1; ir <= R.ni; ir++) {
-                    const int iW = R.border(ir, jr, WEST), iE = R.border(ir, jr, EAST);
-                    const int jS = R.border(ir, jr, SOUTH), jN = R.border(ir, jr, NORTH);
-                    const bool jin = j >= jS + 1 && j <= jN;
-                    if (R.type(ir, jr) == RM_BLOCKG && jin && i >= iW && i <= iE) ident = true;
-                    const int tw = R.btype(ir, jr, WEST), te = R.btype(ir, jr, EAST);
-                    if ((tw == BM_WALL1 || tw == BM_WALL2 || tw == BM_INLET) && jin && i == iW) ident = true;
-                    if ((te == BM_WALL1 || te == BM_WALL2 || te == BM_INLET) && jin && i == iE) ident = true;
-                    if (tw < BM_INTERN || tw > BM_OUTLT2 || te < BM_INTERN || te > BM_OUTLT2) *err = 2001;
-                }
-            if (ident) { a1 = dZero; a2 = dOne; a3 = dZero; bb = dZero; }
+            if (asm_ident<1>(R, i, j, err)) { a1 = dZero; a2 = dOne; a3 = dZero; bb = dZero; }
             ch.b[k] = bb;
         }
     } else {
         const T rkj = dk2 * cv<T>(MGET(m, djv, i, j));
         if (split == 1) {
-            if (t.cj1[p] >= dZero) {
-                a1 = rkj * (-TT(cj1, i - 1, j) - re1 * MM(ran, i - 1, j));
-                a2 = rkj * (TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * cpj + dOne;
-                a3 = rkj * (-re1 * MM(ran, i, j));
-            } else {
-                a1 = rkj * (-re1 * MM(ran, i - 1, j));
-                a2 = rkj * (-TT(cj1, i, j) + re1 * (MM(ran, i, j) + MM(ran, i - 1, j))) + dk2 * cpj + dOne;
-                a3 = rkj * (TT(cj1, i + 1, j) - re1 * MM(ran, i, j));
-            }
+            asm_coef<2, T>(TT(cj1, i - 1, j), TT(cj1, i, j), TT(cj1, i + 1, j), MM(ran, i - 1, j), MM(ran, i, j), rkj,
+                           re1, dk2, cpj, a1, a2, a3);
             const T d1 = den ? cv<T>(d[WD2_IX(i, j + 1)]) : T(0), d0 = den ? cv<T>(d[p]) : T(0);
             const T n1 = den ? cv<T>(dn[WD2_IX(i, j + 1)]) : T(0), n0 = den ? cv<T>(dn[p]) : T(0);
             const T buoy = dk * (d1 + d0 + n1 + n0) / (dFour * fr);
-            ch.b[k] = cv<T>(fn[p] - fs[p]) + rkj * (-t.cnvs[p] - t.cnvn[p]) + rkj * re1 * (t.difs[p] + t.difn[p]) -
-                      dk2 * (cps * cv<T>(fs[p]) + cpn * cv<T>(fn[p])) - buoy;
+            ch.b[k] = asm_rhs<2, T>(fs[p], fn[p], rkj, re1, dk2, t.cnvs[p], t.cnvn[p], t.difs[p], t.difn[p], cps, cpn,
+                                    buoy);
         } else {
-            if (t.cj2[p] >= dZero) {
-                a1 = rkj * (-TT(cj2, i, j - 1) - re1 * MM(rgc, i, j));
-                a2 = rkj * (TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * cpj + dOne;
-                a3 = rkj * (-re1 * MM(rgc, i, j + 1));
-            } else {
-                a1 = rkj * (-re1 * MM(rgc, i, j));
-                a2 = rkj * (-TT(cj2, i, j) + re1 * (MM(rgc, i, j + 1) + MM(rgc, i, j))) + dk2 * cpj + dOne;
-                a3 = rkj * (TT(cj2, i, j + 1) - re1 * MM(rgc, i, j + 1));
-            }
+            asm_coef<2, T>(TT(cj2, i, j - 1), TT(cj2, i, j), TT(cj2, i, j + 1), MM(rgc, i, j), MM(rgc, i, j + 1), rkj,
+                           re1, dk2, cpj, a1, a2, a3);
             T bb = xprev[k];
-            bool ident = false;
-            for (int jr = 1; jr <= R.nj; jr++)
-                for (int ir = 1; ir <= R.ni; ir++) {
-                    const int iW = R.border(ir, jr, WEST), iE = R.border(ir, jr, EAST);
-                    const int jS = R.border(ir, jr, SOUTH), jN = R.border(ir, jr, NORTH);
-                    const bool iin = i >= iW + 1 && i <= iE;
-                    if (R.type(ir, jr) == RM_BLOCKG && iin && j >= jS && j <= jN) ident = true;
-                    const int ts = R.btype(ir, jr, SOUTH), tn = R.btype(ir, jr, NORTH);
-                    if ((ts == BM_WALL1 || ts == BM_WALL2 || ts == BM_INLET) && iin && j == jS) ident = true;
-                    if ((tn == BM_WALL1 || tn == BM_WALL2 || tn == BM_INLET) && iin && j == jN) ident = true;
-                    if (ts < BM_INTERN || ts > BM_OUTLT2 || tn < BM_INTERN || tn > BM_OUTLT2) *err = 2003;
-                }
-            if (ident) { a1 = dZero; a2 = dOne; a3 = dZero; bb = dZero; }
+            if (asm_ident<2>(R, i, j, err)) { a1 = dZero; a2 = dOne; a3 = dZero; bb = dZero; }
             ch.b[k] = bb;
         }
     }
@@ -555,6 +515,122 @@ template void launch_assemble<double>(const Regions &, int, int, int, double, do
 template void launch_assemble<float>(const Regions &, int, int, int, double, double, double, const Metrics &,
                                      const MomTempsT<float> &, const double *, const double *, const double *,
                                      const double *, const float *, const ChainT<float> &, hipStream_t, int *, int);
+
+// Fused assembly for grids without porous regions: the work of k_conv_coef,
+// k_conv_diff (current iterate) and both k_assemble splits of one component
+// in one pass over its rows.
+//   - cj (ConvCoef case 4 / 5, njacob 1) is evaluated on the row's point and
+//     its two neighbours along each split direction.  Where case 4 / 5 writes
+//     nothing (cj1 at i = nx+1, cj2 at j = ny+1) the arrays hold +0, and so
+//     does this kernel.
+//   - cnvs and difs are evaluated at the row's point from the centred
+//     coefficients around it, as k_mom_n does; cnvn and difn come from
+//     k_mom_n.
+//   - ch1 receives the first split's rows, ch2 the second split's
+//     coefficients; zm flags the second split's identity rows, whose
+//     right-hand side is +0 where the others take the first split's solution
+//     (written into ch2.b by the first split's chain solve).
+// Every value is the same expression as in k_conv_coef / k_conv_diff /
+// k_assemble, so the chains are bit for bit those of the separate kernels.
+template <int COMP, typename T>
+__global__ __launch_bounds__(MX *MY) void k_mom_asm(Regions R, int ld, double dk_, double dk2_, double re1_,
+                                                    double fr_, Metrics m, MomTempsT<T> t,
+                                                    const double *__restrict__ us, const double *__restrict__ vs,
+                                                    const double *__restrict__ un, const double *__restrict__ vn,
+                                                    const double *__restrict__ d, const double *__restrict__ dn,
+                                                    ChainT<T> ch1, ChainT<T> ch2, unsigned char *__restrict__ zm,
+                                                    int *err, int live) {
+    const int nx = R.nx, ny = R.ny;
+    int i, j;
+    long k;
+    if (COMP == 1) {
+        i = 1 + blockIdx.x * MX + threadIdx.x; j = 2 + blockIdx.y * MY + threadIdx.y;
+        if (i > nx || j > ny) return;
+        k = (long)(j - 2) * nx + (i - 1);
+    } else {
+        i = 2 + blockIdx.x * MX + threadIdx.x; j = 1 + blockIdx.y * MY + threadIdx.y;
+        if (i > nx || j > ny) return;
+        k = (long)(j - 1) * (nx - 1) + (i - 2);
+    }
+    const T dZero = T(0.0), dOne = T(1.0), dFour = T(4.0);
+    const T dk = cv<T>(dk_), dk2 = cv<T>(dk2_), re1 = cv<T>(re1_), fr = cv<T>(fr_);
+    const T one = cv<T>(1.0), two = cv<T>(2.0);
+    const size_t p = WD2_IX(i, j);
+#define MM(f, a, b) cv<T>(MGET(m, f, a, b))
+    T a1, a2, a3;
+    if (COMP == 1) {
+        const MArr xzi = MARR(m, xzu), xet = MARR(m, xeu), yzi = MARR(m, yzu), yet = MARR(m, yeu);
+        T c1m, c10, c1p = dZero, c2m, c20, c2p = dZero, junk;
+        cc_4<T>(ld, i - 1, j, two, xzi, xet, yzi, yet, us, vs, c1m, junk);
+        cc_4<T>(ld, i, j, two, xzi, xet, yzi, yet, us, vs, c10, c20);
+        if (i + 1 <= nx) cc_4<T>(ld, i + 1, j, two, xzi, xet, yzi, yet, us, vs, c1p, junk);
+        cc_4<T>(ld, i, j - 1, two, xzi, xet, yzi, yet, us, vs, junk, c2m);
+        if (j + 1 <= ny) cc_4<T>(ld, i, j + 1, two, xzi, xet, yzi, yet, us, vs, junk, c2p);
+        const MArr cxe = MARR(m, xec), cye = MARR(m, yec), cxz = MARR(m, xzn), cyz = MARR(m, yzn);
+        const T f1w = cc_x1<T>(ld, i, j, one, cxe, cye, us, vs), f1e = cc_x1<T>(ld, i + 1, j, one, cxe, cye, us, vs);
+        const T f2s = cc_x2<T>(ld, i, j - 1, one, cxz, cyz, us, vs), f2n = cc_x2<T>(ld, i, j, one, cxz, cyz, us, vs);
+        const T cnvs = dconv<T>(ld, i, j, f1w, f1e, f2s, f2n, us);
+        const T difs = ddiff_u<T>(ld, i, j, MARR(m, rac), MARR(m, rbc), MARR(m, rbn), MARR(m, rgn), us);
+        const T rkj = dk2 * cv<T>(MGET(m, dju, i, j));
+        asm_coef<1, T>(c1m, c10, c1p, MM(rac, i, j), MM(rac, i + 1, j), rkj, re1, dk2, dZero, a1, a2, a3);
+        ch1.a[k] = a1; ch1.d[k] = a2; ch1.c[k] = a3;
+        ch1.b[k] = asm_rhs<1, T>(us[p], un[p], rkj, re1, dk2, cnvs, t.cnvn[p], difs, t.difn[p], dZero, dZero, dZero);
+        asm_coef<1, T>(c2m, c20, c2p, MM(rgn, i, j - 1), MM(rgn, i, j), rkj, re1, dk2, dZero, a1, a2, a3);
+        const bool id = asm_ident<1>(R, i, j, err);
+        if (id) { a1 = dZero; a2 = dOne; a3 = dZero; }
+        ch2.a[k] = a1; ch2.d[k] = a2; ch2.c[k] = a3;
+        zm[k] = id ? 1 : 0;
+    } else {
+        const MArr xzi = MARR(m, xzv), xet = MARR(m, xev), yzi = MARR(m, yzv), yet = MARR(m, yev);
+        T c1m, c10, c1p = dZero, c2m, c20, c2p = dZero, junk;
+        cc_5<T>(ld, i - 1, j, two, xzi, xet, yzi, yet, us, vs, c1m, junk);
+        cc_5<T>(ld, i, j, two, xzi, xet, yzi, yet, us, vs, c10, c20);
+        if (i + 1 <= nx) cc_5<T>(ld, i + 1, j, two, xzi, xet, yzi, yet, us, vs, c1p, junk);
+        cc_5<T>(ld, i, j - 1, two, xzi, xet, yzi, yet, us, vs, junk, c2m);
+        if (j + 1 <= ny) cc_5<T>(ld, i, j + 1, two, xzi, xet, yzi, yet, us, vs, junk, c2p);
+        const MArr cxe = MARR(m, xen), cye = MARR(m, yen), cxz = MARR(m, xzc), cyz = MARR(m, yzc);
+        const T f1w = cc_y1<T>(ld, i - 1, j, one, cxe, cye, us, vs), f1e = cc_y1<T>(ld, i, j, one, cxe, cye, us, vs);
+        const T f2s = cc_y2<T>(ld, i, j, one, cxz, cyz, us, vs), f2n = cc_y2<T>(ld, i, j + 1, one, cxz, cyz, us, vs);
+        const T cnvs = dconv<T>(ld, i, j, f1w, f1e, f2s, f2n, vs);
+        const T difs = ddiff_v<T>(ld, i, j, MARR(m, ran), MARR(m, rbc), MARR(m, rbn), MARR(m, rgc), vs);
+        const T rkj = dk2 * cv<T>(MGET(m, djv, i, j));
+        asm_coef<2, T>(c1m, c10, c1p, MM(ran, i - 1, j), MM(ran, i, j), rkj, re1, dk2, dZero, a1, a2, a3);
+        ch1.a[k] = a1; ch1.d[k] = a2; ch1.c[k] = a3;
+        const bool den = live & 2;
+        const T d1 = den ? cv<T>(d[WD2_IX(i, j + 1)]) : T(0), d0 = den ? cv<T>(d[p]) : T(0);
+        const T n1 = den ? cv<T>(dn[WD2_IX(i, j + 1)]) : T(0), n0 = den ? cv<T>(dn[p]) : T(0);
+        const T buoy = dk * (d1 + d0 + n1 + n0) / (dFour * fr);
+        ch1.b[k] = asm_rhs<2, T>(vs[p], vn[p], rkj, re1, dk2, cnvs, t.cnvn[p], difs, t.difn[p], dZero, dZero, buoy);
+        asm_coef<2, T>(c2m, c20, c2p, MM(rgc, i, j), MM(rgc, i, j + 1), rkj, re1, dk2, dZero, a1, a2, a3);
+        const bool id = asm_ident<2>(R, i, j, err);
+        if (id) { a1 = dZero; a2 = dOne; a3 = dZero; }
+        ch2.a[k] = a1; ch2.d[k] = a2; ch2.c[k] = a3;
+        zm[k] = id ? 1 : 0;
+    }
+#undef MM
+}
+
+template <typename T>
+void launch_mom_asm(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
+                    const MomTempsT<T> &t, const double *us, const double *vs, const double *un, const double *vn,
+                    const double *d, const double *dn, const ChainT<T> &ch1, const ChainT<T> &ch2, unsigned char *zm,
+                    hipStream_t s, int *err, int live) {
+    const double dk2 = dk * 0.5, re1 = 1.0 / re;
+    if (comp == 1)
+        hipLaunchKernelGGL((k_mom_asm<1, T>), mgrid(R.nx, R.ny - 1), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
+                           t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live);
+    else
+        hipLaunchKernelGGL((k_mom_asm<2, T>), mgrid(R.nx - 1, R.ny), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
+                           t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live);
+}
+template void launch_mom_asm<double>(const Regions &, int, int, double, double, double, const Metrics &,
+                                     const MomTempsT<double> &, const double *, const double *, const double *,
+                                     const double *, const double *, const double *, const ChainT<double> &,
+                                     const ChainT<double> &, unsigned char *, hipStream_t, int *, int);
+template void launch_mom_asm<float>(const Regions &, int, int, double, double, double, const Metrics &,
+                                    const MomTempsT<float> &, const double *, const double *, const double *,
+                                    const double *, const double *, const double *, const ChainT<float> &,
+                                    const ChainT<float> &, unsigned char *, hipStream_t, int *, int);
 
 // ------------------------------------------------------------- scatter
 template <typename T>

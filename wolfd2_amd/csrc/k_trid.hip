@@ -118,12 +118,13 @@ __device__ inline unsigned seg_kinds(long seg0, long L) {
 
 // Speculated states at block boundaries: the forward state entering row blk0
 // and the backward value at row k1 are guessed by running the recurrence over
-// the TR_W rows beyond the boundary from a zero state.  The recurrences
-// contract, so the guess is usually the exact state bit for bit; it is always
-// verified (and repaired if not) against the neighbour block's own result.
-// The rows a warm-up reads are first loaded by TR_W lanes at once into LDS
-// (w[0..3][q] = A, D, C, B of row k0 + q, and D of row k0 + q + 1 in w[4]),
-// so the one lane that runs the recurrence waits for one load, not TR_W.
+// the TR_W (forward) or TR_WB (backward) rows beyond the boundary from a zero
+// state.  The recurrences contract, so the guess is usually the exact state
+// bit for bit; it is always verified (and repaired if not) against the
+// neighbour block's own result.  The rows a warm-up reads are first loaded by
+// as many lanes at once into LDS (forward: w[0..3][q] = A, D, C, B of row
+// k0 + q, and D of row k0 + q + 1 in w[4]), so the one lane that runs the
+// recurrence waits for one load, not one per row.
 constexpr int TR_W = 16;
 template <typename T>
 __device__ inline void warm_fwd_load(const T *__restrict__ A, const T *__restrict__ D, const T *__restrict__ Cc,
@@ -153,18 +154,19 @@ __device__ inline void warm_fwd(long blk0, long L, const T (*w)[TR_W], T &cpv, T
     cpv = c;
     bpv = bb;
 }
-// backward warm-up over the rows k1 .. k1+TR_W-1 (cp in w[0], bp in w[1])
+// Backward warm-up over the rows k1 .. k1+TR_WB-1 (cp in w[0], bp in w[1]).
+constexpr int TR_WB = 16;
 template <typename T>
-__device__ inline void warm_bwd_load(const T *CP, const T *BP, long n, long k1, T (*w)[TR_W]) {
+__device__ inline void warm_bwd_load(const T *CP, const T *BP, long n, long k1, T (*w)[TR_WB]) {
     const int q = threadIdx.x;
-    if (q < TR_W && k1 + q < n) {
+    if (q < TR_WB && k1 + q < n) {
         w[0][q] = CP[k1 + q];
         w[1][q] = BP[k1 + q];
     }
 }
 template <typename T>
-__device__ inline T warm_bwd(const T (*w)[TR_W], long n, long k1, long L) {
-    const long k2 = min(n, k1 + TR_W);
+__device__ inline T warm_bwd(const T (*w)[TR_WB], long n, long k1, long L) {
+    const long k2 = min(n, k1 + TR_WB);
     long r = (k2 - 1) % L;
     T xw = T(0);
     for (long k = k2 - 1; k >= k1; k--, r = (r == 0) ? L - 1 : r - 1) {
@@ -258,10 +260,11 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
                                                    T *__restrict__ CP, T *__restrict__ BP,
                                                    const T *__restrict__ blk_c, const T *__restrict__ blk_b,
                                                    T *__restrict__ X, T *__restrict__ blk_x, T *__restrict__ blk_xe,
-                                                   int *__restrict__ fail) {
+                                                   int *__restrict__ fail, T *__restrict__ B2,
+                                                   const unsigned char *__restrict__ Z) {
     __shared__ T scp[TR_T * TR_P], sbp[TR_T * TR_P];
     __shared__ T sx[TR_T];
-    __shared__ T sw[2][TR_W];
+    __shared__ T sw[2][TR_WB];
     // the head rows a repair usually needs (A, D, C, B of rows blk0 .. +TR_H-1
     // and D of the row after each), loaded by TR_H lanes at once
     constexpr int TR_H = 64;
@@ -355,6 +358,18 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
     }
     if (t == 0 && len > 0) blk_x[blockIdx.x] = x[0];
     seg_store(X, seg0, len, x);
+    if (B2) {
+        unsigned long long zb = 0;   // the segment's flags, one byte per row
+        if (len == TR_S) {
+            zb = *reinterpret_cast<const unsigned long long *>(Z + seg0);
+        } else {
+            for (int q = 0; q < len; q++) zb |= (unsigned long long)Z[seg0 + q] << (8 * q);
+        }
+#pragma unroll
+        for (int q = 0; q < TR_S; q++)
+            if ((zb >> (8 * q)) & 0xffu) x[q] = T(0);
+        seg_store(B2, seg0, len, x);
+    }
 }
 
 // One workgroup: repair each block's tail from its successor's first value,
@@ -367,7 +382,8 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
                                                        const T *__restrict__ B, T *__restrict__ CP,
                                                        T *__restrict__ BP, T *__restrict__ X,
                                                        const T *__restrict__ blk_x, const T *__restrict__ blk_xe,
-                                                       int *__restrict__ fail, int *__restrict__ nfallback) {
+                                                       int *__restrict__ fail, int *__restrict__ nfallback,
+                                                       T *__restrict__ B2, const unsigned char *__restrict__ Z) {
     for (int blk = threadIdx.x; blk < nblk - 1; blk += TRF_T) {
         const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
         long r = (k1 - 1) % L;   // position of row k in its chain
@@ -379,6 +395,7 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
             const T xk = (r == L - 1 && r != 0) ? BP[k] : BP[k] - (CP[k] * xv);
             if (same(xk, X[k])) break;
             X[k] = xk;
+            if (B2) B2[k] = Z[k] ? T(0) : xk;
             xv = xk;
         }
         if (k < k0) atomicOr(fail, 1);
@@ -397,6 +414,7 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
         for (long k = c0 + L - 1; k >= c0; k--) {
             X[k] = (row_kind(k, L) == 2) ? BP[k] : BP[k] - (CP[k] * xv);
             xv = X[k];
+            if (B2) B2[k] = Z[k] ? T(0) : xv;
         }
     }
     *fail = 0;
@@ -407,17 +425,18 @@ int trid_blocks(long n) { return (int)((n + TR_R - 1) / TR_R); }
 
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
-                 hipStream_t s) {
+                 hipStream_t s, T *B2, const unsigned char *Z) {
     const int nb = trid_blocks(n);
     hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b);
     hipLaunchKernelGGL(k_trid_bwd<T>, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
-                       X, w.blk_x, w.blk_xe, w.fail);
+                       X, w.blk_x, w.blk_xe, w.fail, B2, Z);
     hipLaunchKernelGGL(k_trid_finish<T>, dim3(1), dim3(TRF_T), 0, s, n, L, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
-                       w.blk_xe, w.fail, w.nfallback);
+                       w.blk_xe, w.fail, w.nfallback, B2, Z);
 }
 template void launch_trid<double>(const TridWorkT<double> &, long, long, const double *, const double *,
-                                  const double *, const double *, double *, hipStream_t);
+                                  const double *, const double *, double *, hipStream_t, double *,
+                                  const unsigned char *);
 template void launch_trid<float>(const TridWorkT<float> &, long, long, const float *, const float *, const float *,
-                                 const float *, float *, hipStream_t);
+                                 const float *, float *, hipStream_t, float *, const unsigned char *);
 
 }  // namespace wd2

@@ -83,11 +83,12 @@ struct wd2_ctx {
     ThermTemps tt{};
     double *pc[5];
     MomTemps T{};
-    Chain ch{};
+    Chain ch{}, ch2{};   // ch2: the second split's chain of the fused momentum path
+    unsigned char *zm = nullptr;   // its identity-row flags
     TridWork tw{};
     // FP32 views of the same buffers for the mixed-precision momentum
     MomTempsT<float> Tf{};
-    ChainT<float> chf{};
+    ChainT<float> chf{}, chf2{};
     TridWorkT<float> twf{};
     double *partials = nullptr;
     Scalars *sc = nullptr;
@@ -160,6 +161,7 @@ static void ctx_free(wd2_ctx *c) {
     (void)hipFree(c->sc);
     (void)hipFree(c->tile_flags);
     (void)hipFree(c->tw.cp);
+    (void)hipFree(c->zm);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
 }
@@ -185,7 +187,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     }
     const int NFIELDS = 16 + 4 + 4 + 5 + 15;
     const size_t nchain = chain_rows(nx, ny);
-    const size_t total = NFIELDS * c->nf + 5 * nchain;
+    const size_t total = NFIELDS * c->nf + 10 * nchain;
     HIPCHK(hipMalloc(&c->store, total * sizeof(double)));
     HIPCHK(hipMemsetAsync(c->store, 0, total * sizeof(double), c->s));
     double *q = c->store;
@@ -204,6 +206,8 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->ch.c = take(nchain);
     c->ch.b = take(nchain);
     c->ch.x = take(nchain);
+    c->ch2 = Chain{take(nchain), take(nchain), take(nchain), take(nchain), take(nchain)};
+    HIPCHK(hipMalloc(&c->zm, nchain));
     const int nb = trid_blocks((long)nchain) + 2;
     HIPCHK(hipMalloc(&c->tw.cp, (2 * nchain + 4 * nb) * sizeof(double) + 2 * sizeof(int) + 64));
     c->tw.bp = c->tw.cp + nchain;
@@ -218,6 +222,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
                              F(c->T.c2n),  F(c->T.cnvs), F(c->T.cnvn), F(c->T.difs), F(c->T.difn),
                              F(c->T.cps),  F(c->T.cpn),  F(c->T.cpj),  F(c->T.cnvn2), F(c->T.difn2)};
     c->chf = ChainT<float>{F(c->ch.a), F(c->ch.d), F(c->ch.c), F(c->ch.b), F(c->ch.x)};
+    c->chf2 = ChainT<float>{F(c->ch2.a), F(c->ch2.d), F(c->ch2.c), F(c->ch2.b), F(c->ch2.x)};
     c->twf = TridWorkT<float>{F(c->tw.cp),    F(c->tw.bp),     F(c->tw.blk_c), F(c->tw.blk_b),
                               F(c->tw.blk_x), F(c->tw.blk_xe), c->tw.fail,     c->tw.nfallback};
     HIPCHK(hipMemsetAsync(c->tw.fail, 0, 2 * sizeof(int), c->s));
@@ -475,7 +480,7 @@ static int check_err(wd2_ctx *c) {
 template <typename T>
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
                     const double *un, const double *vn, const MomTempsT<T> &t0, const ChainT<T> &ch,
-                    const TridWorkT<T> &tw, bool n_ready, double *out) {
+                    const ChainT<T> &ch2, const TridWorkT<T> &tw, bool n_ready, double *out) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     const Metrics &M = c->M;
     hipStream_t s = c->s;
@@ -485,11 +490,21 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
         t.difn = t0.difn2;
     }
     const double *fs = comp == 1 ? us : vs, *fn = comp == 1 ? un : vn;
+    const long n = comp == 1 ? (long)nx * (ny - 1) : (long)(nx - 1) * ny;
+    const int live = (c->any_porous ? 1 : 0) | (c->d_live ? 2 : 0);
     if (!c->any_porous) {
-        // fused: the centred coefficients are evaluated where they are read
+        // fused: the convection coefficients are evaluated where they are
+        // read; the first split's solve writes the second split's right-hand
+        // side (identity rows zeroed)
         if (!n_ready) launch_mom_n<T>(nx, ny, ld, comp, M, un, vn, t.cnvn, t.difn, s);
-        launch_mom_s<T>(nx, ny, ld, comp, M, us, vs, t, s);
-    } else if (comp == 1) {
+        launch_mom_asm<T>(c->R, ld, comp, dk, re, fr, M, t, us, vs, un, vn, c->d, c->dn, ch, ch2, c->zm, s,
+                          &c->sc->err, live);
+        launch_trid<T>(tw, n, n, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm);
+        launch_trid<T>(tw, n, n, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s);
+        launch_scatter<T>(nx, ny, ld, comp, ch2.x, out, s);
+        return WD2_OK;
+    }
+    if (comp == 1) {
         launch_conv_coef<T>(nx, ny, ld, 4, 1, MARR(M, xzu), MARR(M, xeu), MARR(M, yzu), MARR(M, yeu), us, vs, t.cj1, t.cj2, s);
         launch_conv_coef<T>(nx, ny, ld, 1, 0, MARR(M, xzn), MARR(M, xec), MARR(M, yzn), MARR(M, yec), us, vs, t.c1s, t.c2s, s);
         launch_conv_coef<T>(nx, ny, ld, 1, 0, MARR(M, xzn), MARR(M, xec), MARR(M, yzn), MARR(M, yec), un, vn, t.c1n, t.c2n, s);
@@ -498,13 +513,9 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
         launch_conv_coef<T>(nx, ny, ld, 2, 0, MARR(M, xzc), MARR(M, xen), MARR(M, yzc), MARR(M, yen), us, vs, t.c1s, t.c2s, s);
         launch_conv_coef<T>(nx, ny, ld, 2, 0, MARR(M, xzc), MARR(M, xen), MARR(M, yzc), MARR(M, yen), un, vn, t.c1n, t.c2n, s);
     }
-    if (c->any_porous) {
-        launch_porous_div<T>(c->R, ld, comp, t, s);
-        launch_conv_diff<T>(nx, ny, ld, comp, M, fs, fn, t, s);
-        launch_poros_coef<T>(c->R, ld, comp, us, vs, un, vn, t, s);
-    }
-    const long n = comp == 1 ? (long)nx * (ny - 1) : (long)(nx - 1) * ny;
-    const int live = (c->any_porous ? 1 : 0) | (c->d_live ? 2 : 0);
+    launch_porous_div<T>(c->R, ld, comp, t, s);
+    launch_conv_diff<T>(nx, ny, ld, comp, M, fs, fn, t, s);
+    launch_poros_coef<T>(c->R, ld, comp, us, vs, un, vn, t, s);
     launch_assemble<T>(c->R, ld, comp, 1, dk, re, fr, M, t, fs, fn, c->d, c->dn, nullptr, ch, s, &c->sc->err, live);
     launch_trid<T>(tw, n, n, ch.a, ch.d, ch.c, ch.b, ch.x, s);
     launch_assemble<T>(c->R, ld, comp, 2, dk, re, fr, M, t, fs, fn, c->d, c->dn, ch.x, ch, s, &c->sc->err, live);
@@ -516,8 +527,8 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
                     const double *un, const double *vn, double *out, bool n_ready = false) {
     if (c->precision == WD2_PREC_MIXED)
-        return momentum<float>(c, comp, dk, re, fr, us, vs, un, vn, c->Tf, c->chf, c->twf, n_ready, out);
-    return momentum<double>(c, comp, dk, re, fr, us, vs, un, vn, c->T, c->ch, c->tw, n_ready, out);
+        return momentum<float>(c, comp, dk, re, fr, us, vs, un, vn, c->Tf, c->chf, c->chf2, c->twf, n_ready, out);
+    return momentum<double>(c, comp, dk, re, fr, us, vs, un, vn, c->T, c->ch, c->ch2, c->tw, n_ready, out);
 }
 
 // momentum.f:nAuxMomentum 33-193 on the context's un, vn -> us, vs.
@@ -1319,7 +1330,7 @@ int wd2_ctx_set_precision(wd2_ctx *c, int precision) {
         // the momentum kernels never write are read as 0 (as in the reference),
         // so clear the whole block when the arithmetic changes.
         const size_t nchain = chain_rows(c->nx, c->ny);
-        HIPCHK(hipMemsetAsync(c->T.cj1, 0, (15 * c->nf + 5 * nchain) * sizeof(double), c->s));
+        HIPCHK(hipMemsetAsync(c->T.cj1, 0, (15 * c->nf + 10 * nchain) * sizeof(double), c->s));
         HIPCHK(hipStreamSynchronize(c->s));
     }
     c->precision = precision;

@@ -18,9 +18,11 @@ struct TridWorkT {
 };
 using TridWork = TridWorkT<double>;
 int trid_blocks(long n);
+// B2, Z (optional): also write B2[k] = Z[k] ? +0 : X[k], the right-hand side
+// of a following system whose rows flagged in Z are identity rows.
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
-                 hipStream_t s);
+                 hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr);
 
 // Boundary fills (one workgroup).  err receives 1000+side for a bad flag.
 template <typename T>
@@ -150,14 +152,18 @@ void launch_poros_coef(const Regions &R, int ld, int comp, const double *us, con
 template <typename T>
 void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *un,
                       const MomTempsT<T> &t, hipStream_t s);
-// Fused terms for grids without porous regions: k_mom_n writes cnvn / difn of
-// one component from un, vn; k_mom_s writes cj1, cj2, cnvs, difs from us, vs.
+// Fused kernels for grids without porous regions: k_mom_n writes cnvn / difn
+// of one component from un, vn; k_mom_asm assembles both splits of one
+// component from us, vs (ch1: the first split's chain; ch2: the second
+// split's coefficients; zm: its identity rows).
 template <typename T>
 void launch_mom_n(int nx, int ny, int ld, int comp, const Metrics &m, const double *un, const double *vn, T *cnvn,
                   T *difn, hipStream_t s);
 template <typename T>
-void launch_mom_s(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *vs,
-                  const MomTempsT<T> &t, hipStream_t s);
+void launch_mom_asm(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
+                    const MomTempsT<T> &t, const double *us, const double *vs, const double *un, const double *vn,
+                    const double *d, const double *dn, const ChainT<T> &ch1, const ChainT<T> &ch2, unsigned char *zm,
+                    hipStream_t s, int *err, int live);
 // live: bit 0, porous regions (load the Dupuit-Forchheimer coefficients);
 // bit 1, a density field (load d, dn); otherwise those values are +0.
 template <typename T>
