@@ -123,8 +123,10 @@ def test_alt_trid_lu_bitwise_vs_oracle(oracle, n, weak):
 
 def _two_fixed_points(rng):
     """Constant rows (a, d, c, b) whose rounded forward recurrence has two
-    fixed points: a run from the zero state and a run from another state
-    settle on different values and never coincide bit for bit."""
+    fixed points (a run from the zero state and a run from another state
+    settle on different values and never coincide bit for bit), and whose
+    backward recurrence x = b' - c' x on either settled (c', b') has two
+    fixed points as well."""
     def run(a, d, c, b, cp, bp, n):
         out = []
         for _ in range(n):
@@ -133,11 +135,21 @@ def _two_fixed_points(rng):
             bp = (b - a * bp) / dp
             out.append((cp, bp))
         return out
+
+    def back(cp, bp, x, n):
+        out = []
+        for _ in range(n):
+            x = bp - (cp * x)
+            out.append(x)
+        return out
+
+    def apart(r1, r2):
+        return all(p != q for p, q in zip(r1, r2))
     while True:
         a, c = -rng.uniform(0.05, 0.5), -rng.uniform(0.05, 0.5)
         d, b = 1.0 + rng.uniform(0.0, 0.5), rng.uniform(-1.0, 1.0)
         r1, r2 = run(a, d, c, b, 0.0, 0.0, 200), run(a, d, c, b, -0.3, 0.7, 200)
-        if all(p != q for p, q in zip(r1, r2)):
+        if apart(r1, r2) and all(apart(back(*r[-1], 0.0, 100), back(*r[-1], 3.0, 100)) for r in (r1, r2)):
             return a, d, c, b
 
 

@@ -176,6 +176,62 @@ __device__ inline T warm_bwd(const T (*w)[TR_WB], long n, long k1, long L) {
     return xw;
 }
 
+// Bit queries on a workgroup's per-lane flags (TR_T bits in 64-bit words).
+__device__ inline int lowest_bit_in(const unsigned long long *m, int lo, int hi) {   // -1 if none
+    for (int w = lo >> 6; lo <= hi && w <= (hi >> 6); w++) {
+        unsigned long long x = m[w];
+        if (w == (lo >> 6)) x &= ~0ull << (lo & 63);
+        if (w == (hi >> 6) && (hi & 63) != 63) x &= (1ull << ((hi & 63) + 1)) - 1;
+        if (x) return (w << 6) + __ffsll((long long)x) - 1;
+    }
+    return -1;
+}
+__device__ inline int highest_clear_at_or_below(const unsigned long long *m, int hi) {   // -1 if none
+    for (int w = hi >> 6; w >= 0; w--) {
+        unsigned long long x = ~m[w];
+        if (w == (hi >> 6) && (hi & 63) != 63) x &= (1ull << ((hi & 63) + 1)) - 1;
+        if (x) return (w << 6) + 63 - __clzll((long long)x);
+    }
+    return -1;
+}
+__device__ inline int highest_bit_in(const unsigned long long *m, int lo, int hi) {   // -1 if none
+    for (int w = hi >> 6; lo <= hi && w >= (lo >> 6); w--) {
+        unsigned long long x = m[w];
+        if (w == (lo >> 6)) x &= ~0ull << (lo & 63);
+        if (w == (hi >> 6) && (hi & 63) != 63) x &= (1ull << ((hi & 63) + 1)) - 1;
+        if (x) return (w << 6) + 63 - __clzll((long long)x);
+    }
+    return -1;
+}
+__device__ inline int lowest_clear_at_or_above(const unsigned long long *m, int lo, int n) {   // n if none
+    for (int w = lo >> 6; w < n / 64; w++) {
+        unsigned long long x = ~m[w];
+        if (w == (lo >> 6)) x &= ~0ull << (lo & 63);
+        if (x) return (w << 6) + __ffsll((long long)x) - 1;
+    }
+    return n;
+}
+__device__ inline bool bit_at(const unsigned long long *m, int i) { return (m[i >> 6] >> (i & 63)) & 1ull; }
+
+// Rounding cycles.  Over a stretch of rows whose coefficients repeat with
+// period two (identical rows included), the rounded forward map has fixed
+// points or two-cycles, and runs from different states can settle on
+// different ones and never coincide: the fix-up rounds would then carry the
+// exact state through the stretch one segment per round.  Call a lane
+// "constant-even" (CE) if its TR_S rows are interior rows whose coefficients
+// equal those of the row two before (rows -2 and -1 being its predecessor's
+// last two rows).  If lanes v+1 .. t-1 are CE and lane v's end state S is on
+// the cycle of their row pair (f7(f6(S)) == S for lane v's last two rows,
+// which are the pair of every row pair after them), each of those lanes maps
+// S to itself (an even number of rows), so the exact state entering lane t
+// is S.  After TR_ACC rounds a block still changing marks its CE lanes, and
+// from then on a lane whose predecessor is CE takes its entering state from
+// the first lane v of that CE run (or the lane before it) whose end state is
+// on the cycle.  At convergence every lane's entering state then equals its
+// predecessor's end state (by induction along the run), so the result is the
+// sequential sweep's; only the number of rounds changes.
+constexpr int TR_ACC = 3;
+
 template <typename T>
 __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__restrict__ A, const T *__restrict__ D,
                                                    const T *__restrict__ Cc, const T *__restrict__ B,
@@ -184,6 +240,8 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
     __shared__ T sc[TR_T], sb[TR_T];
     __shared__ T sw[5][TR_W];
     __shared__ int changed;
+    __shared__ unsigned long long ce_mask[TR_T / 64], cyc_mask[TR_T / 64];
+    __shared__ T s67[8][TR_T];   // each lane's last two rows (A, D, C, B), for the CE test
     const int t = threadIdx.x;
     const long blk0 = (long)blockIdx.x * TR_R;
     const int nrow = (int)min((long)TR_R, n - blk0);
@@ -214,12 +272,30 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
     }
     // fix-up rounds inside the workgroup
     const bool dep = len > 0 && t > 0 && SKIND(kinds, 0) != 1;
+    bool accel = false;   // the CE shortcut is on (uniform)
+    bool cen = false;     // lane t+1 is CE
+    int runlo = -1;       // lane t-1 is CE: the first lane v may take (else -1)
     for (int round = 0; round <= TR_T; round++) {
         if (len > 0) { sc[t] = cp[len - 1]; sb[t] = bp[len - 1]; }
         if (t == 0) changed = 0;
+        if (accel) {
+            bool on = false;
+            if (cen) {
+                T c6, b6, c7, b7;
+                fwd_row<T>(0, a[6], d[6], c[6], b[6], T(1), cp[7], bp[7], c6, b6);
+                fwd_row<T>(0, a[7], d[7], c[7], b[7], T(1), c6, b6, c7, b7);
+                on = same(c7, cp[7]) && same(b7, bp[7]);
+            }
+            const unsigned long long m = __ballot(on);
+            if ((t & 63) == 0) cyc_mask[t >> 6] = m;
+        }
         __syncthreads();
         if (dep) {
             T sc0 = sc[t - 1], sb0 = sb[t - 1];
+            if (runlo >= 0) {
+                const int v = lowest_bit_in(cyc_mask, runlo, t - 2);
+                if (v >= 0) { sc0 = sc[v]; sb0 = sb[v]; }
+            }
             bool live = true;
 #pragma unroll
             for (int s = 0; s < TR_S; s++) {
@@ -237,6 +313,30 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
         }
         __syncthreads();
         if (!changed) break;
+        if (round == TR_ACC) {
+            // mark the CE lanes (rows -2, -1 from the predecessor lane; lane 0
+            // of a block is never CE)
+            if (len == TR_S) {
+                s67[0][t] = a[6]; s67[1][t] = d[6]; s67[2][t] = c[6]; s67[3][t] = b[6];
+                s67[4][t] = a[7]; s67[5][t] = d[7]; s67[6][t] = c[7]; s67[7][t] = b[7];
+            }
+            __syncthreads();
+            bool ce = len == TR_S && kinds == 0u && t > 0 && SKIND(seg_kinds(seg0 - TR_S, L), 6) == 0 &&
+                      SKIND(seg_kinds(seg0 - TR_S, L), 7) == 0;
+#pragma unroll
+            for (int s = 2; s < TR_S; s++)
+                ce = ce && same(a[s], a[s - 2]) && same(d[s], d[s - 2]) && same(c[s], c[s - 2]) &&
+                     same(b[s], b[s - 2]);
+            ce = ce && same(a[0], s67[0][t - 1]) && same(d[0], s67[1][t - 1]) && same(c[0], s67[2][t - 1]) &&
+                 same(b[0], s67[3][t - 1]) && same(a[1], s67[4][t - 1]) && same(d[1], s67[5][t - 1]) &&
+                 same(c[1], s67[6][t - 1]) && same(b[1], s67[7][t - 1]);
+            const unsigned long long m = __ballot(ce);
+            if ((t & 63) == 0) ce_mask[t >> 6] = m;
+            __syncthreads();
+            cen = t + 1 < TR_T && bit_at(ce_mask, t + 1);
+            if (t >= 2 && bit_at(ce_mask, t - 1)) runlo = max(0, highest_clear_at_or_below(ce_mask, t - 1));
+            accel = true;
+        }
         __syncthreads();
     }
 #undef DNEXT
@@ -264,6 +364,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
                                                    const unsigned char *__restrict__ Z) {
     __shared__ T scp[TR_T * TR_P], sbp[TR_T * TR_P];
     __shared__ T sx[TR_T];
+    __shared__ unsigned long long ce_mask[TR_T / 64], cyc_mask[TR_T / 64];
     __shared__ T sw[2][TR_WB];
     // the head rows a repair usually needs (A, D, C, B of rows blk0 .. +TR_H-1
     // and D of the row after each), loaded by TR_H lanes at once
@@ -335,13 +436,35 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
             x[s] = T(0);
         }
     }
+    // Fix-up rounds, with the CE shortcut of the forward pass mirrored: a lane
+    // is CE if its rows are interior rows whose (c', b') equal those of the
+    // row two after (rows 8, 9 being its successor's first two rows); if
+    // lanes t+1 .. v-1 are CE and lane v's first value x0 is on the cycle of
+    // its first two rows (g0(g1(x0)) == x0, g_s(x) = b'_s - c'_s x), each of
+    // those lanes maps x0 to itself, so the exact value after lane t is x0.
     const bool dep = len > 0 && SKIND(kinds, len - 1) != 2 && t * TR_S + len < nrow;
+    bool accel = false;   // the CE shortcut is on (uniform)
+    bool cpv = false;     // lane t-1 is CE
+    int runhi = -1;       // lane t+1 is CE: the last lane v may take (else -1)
     for (int round = 0; round <= TR_T; round++) {
         if (len > 0) sx[t] = x[0];
         if (t == 0) changed = 0;
+        if (accel) {
+            bool on = false;
+            if (cpv) {
+                const T y = bp[1] - (cp[1] * x[0]);
+                on = same(bp[0] - (cp[0] * y), x[0]);
+            }
+            const unsigned long long m = __ballot(on);
+            if ((t & 63) == 0) cyc_mask[t >> 6] = m;
+        }
         __syncthreads();
         if (dep) {
             T xv = sx[t + 1];
+            if (runhi >= 0) {
+                const int v = highest_bit_in(cyc_mask, t + 2, runhi);
+                if (v >= 0) xv = sx[v];
+            }
             bool live = true;
 #pragma unroll
             for (int s = TR_S - 1; s >= 0; s--) {
@@ -354,6 +477,24 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
         }
         __syncthreads();
         if (!changed) break;
+        if (round == TR_ACC) {
+            // mark the CE lanes (rows 8, 9 from the successor's staged c', b')
+            bool ce = len == TR_S && kinds == 0u && (t + 1) * TR_S + 2 <= nrow;
+            if (ce) {
+                const unsigned kn = seg_kinds(seg0 + TR_S, L);
+                ce = SKIND(kn, 0) == 0 && SKIND(kn, 1) == 0;
+            }
+#pragma unroll
+            for (int s = 0; s + 2 < TR_S; s++) ce = ce && same(cp[s], cp[s + 2]) && same(bp[s], bp[s + 2]);
+            ce = ce && same(cp[6], scp[(t + 1) * TR_P]) && same(bp[6], sbp[(t + 1) * TR_P]) &&
+                 same(cp[7], scp[(t + 1) * TR_P + 1]) && same(bp[7], sbp[(t + 1) * TR_P + 1]);
+            const unsigned long long m = __ballot(ce);
+            if ((t & 63) == 0) ce_mask[t >> 6] = m;
+            __syncthreads();
+            cpv = t >= 1 && bit_at(ce_mask, t - 1);
+            if (t + 1 < TR_T && bit_at(ce_mask, t + 1)) runhi = min(TR_T - 1, lowest_clear_at_or_above(ce_mask, t + 1, TR_T));
+            accel = true;
+        }
         __syncthreads();
     }
     if (t == 0 && len > 0) blk_x[blockIdx.x] = x[0];
