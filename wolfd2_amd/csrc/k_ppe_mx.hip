@@ -161,11 +161,13 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     // converges to the fixed point of the FP64 residual.
     float cr[SR][2];
     float n1[UNI ? 1 : SR][2], n2[UNI ? 1 : SR][2], n4[UNI ? 1 : SR][2], n5[UNI ? 1 : SR][2];
+    // UNI: every updated point has the same diagonal a3 (the same expression
+    // of the same two values) and so the same FP32 reciprocal
+    const double a3u = -rau_c - rau_c - rgv_c - rgv_c;
+    const float i3u = 1.0f / (float)a3u;
     if (UNI) {
-        const double a3 = -rau_c - rau_c - rgv_c - rgv_c;
-        const float i3 = 1.0f / (float)a3;
-        n1[0][0] = n1[0][1] = n5[0][0] = n5[0][1] = -(float)rgv_c * i3;
-        n2[0][0] = n2[0][1] = n4[0][0] = n4[0][1] = -(float)rau_c * i3;
+        n1[0][0] = n1[0][1] = n5[0][0] = n5[0][1] = -(float)rgv_c * i3u;
+        n2[0][0] = n2[0][1] = n4[0][0] = n4[0][1] = -(float)rau_c * i3u;
     }
 #pragma unroll
     for (int r = 0; r < SR; r++) {
@@ -178,12 +180,12 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
             if ((upd >> (2 * r + h)) & 1) {
                 const double a1 = h ? g[r].y : g[r].x, a5 = h ? g[r + 1].y : g[r + 1].x;
                 const double a2 = ra[h], a4 = ra[h + 1];
-                const double a3 = -a4 - a2 - a5 - a1;
+                const double a3 = UNI ? a3u : -a4 - a2 - a5 - a1;
                 const double pw = h ? pr[r].x : P[c0 - 1];
                 const double pe = h ? P[c0 + 2] : pr[r].y;
                 const double res = (h ? bvr[r].y : bvr[r].x) - a1 * (h ? ps.y : ps.x) - a2 * pw -
                                    a3 * (h ? pr[r].y : pr[r].x) - a4 * pe - a5 * (h ? pn.y : pn.x);
-                const float i3 = 1.0f / (float)a3;
+                const float i3 = UNI ? i3u : 1.0f / (float)a3;
                 rr = (float)res * i3;
                 if (!UNI) {
                     n1[UNI ? 0 : r][h] = -(float)a1 * i3;
@@ -203,9 +205,28 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     for (int r = 0; r < SR; r++) e[r][0] = e[r][1] = 0.f;
     const float w = (float)sorrel;
     const float tol = (float)sortol;
-    unsigned below = 0;   // bit s: this lane's owned max |s| of sweep s is < sortol
+    // The sweeps are straight-line code: every point of the strip is
+    // evaluated with its own relaxation factor wm, sorrel where the point is
+    // updated and 0 elsewhere, so a point that is not updated keeps e = +0
+    // (fmaf(0, s, +0) is +0 for finite s).  The neighbour indices that would
+    // leave the tile at its outer ring are clamped to the lane's own slot, so
+    // every read is of a written (finite) slot.  The stopping test flags an
+    // owned point with |s| >= sortol, which is max |s| >= sortol over the
+    // owned points; it is gathered per wavefront in scalar lane masks.
+    float wm[SR][2];
+    unsigned long long owm[SR][2];
+#pragma unroll
+    for (int r = 0; r < SR; r++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            wm[r][h] = ((upd >> (2 * r + h)) & 1) ? w : 0.f;
+            owm[r][h] = __ballot((own >> (2 * r + h)) & 1);
+        }
+    const int qs0 = y0 > 0 ? -ES : 0, qn0 = y0 + SR < MW ? ES : 0;
+    const int qw0 = k > 0 ? -1 : 0, qe0 = k < MK - 1 ? 1 : 0;
+    unsigned below = 0;   // bit s: the wavefront's owned max |s| of sweep s is < sortol
     for (int s = 0; s < nsweep; s++) {
-        float mx = 0.f;
+        unsigned long long big = 0;   // lanes of the wavefront with such a point
 #pragma unroll
         for (int color = 0; color < 2; color++) {
             float *__restrict__ dst = EC[color];
@@ -214,27 +235,23 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
             for (int r = 0; r < SR; r++) {
                 const int h = (r & 1) ^ color;   // the row's point of this colour
                 const int rc = UNI ? 0 : r;
-                if ((upd >> (2 * r + h)) & 1) {
-                    const int q = (y0 + r) * ES + k;
-                    // updated points are never on the tile's outer ring, so
-                    // every LDS neighbour index below is inside the tile
-                    const float eS = r > 0 ? e[r - 1][h] : src[q - ES];
-                    const float eN = r < SR - 1 ? e[r + 1][h] : src[q + ES];
-                    const float eW = h ? e[r][0] : src[q - 1];
-                    const float eE = h ? src[q + 1] : e[r][1];
-                    float acc = fmaf(n1[rc][h], eS, cr[r][h]);
-                    acc = fmaf(n2[rc][h], eW, acc);
-                    acc = fmaf(n4[rc][h], eE, acc);
-                    acc = fmaf(n5[rc][h], eN, acc);
-                    const float sum = acc - e[r][h];
-                    e[r][h] = fmaf(w, sum, e[r][h]);
-                    dst[q] = e[r][h];
-                    if ((own >> (2 * r + h)) & 1) mx = fmaxf(mx, fabsf(sum));
-                }
+                const int q = (y0 + r) * ES + k;
+                const float eS = r > 0 ? e[r - 1][h] : src[q + qs0];
+                const float eN = r < SR - 1 ? e[r + 1][h] : src[q + qn0];
+                const float eW = h ? e[r][0] : src[q + qw0];
+                const float eE = h ? src[q + qe0] : e[r][1];
+                float acc = fmaf(n1[rc][h], eS, cr[r][h]);
+                acc = fmaf(n2[rc][h], eW, acc);
+                acc = fmaf(n4[rc][h], eE, acc);
+                acc = fmaf(n5[rc][h], eN, acc);
+                const float sum = acc - e[r][h];
+                e[r][h] = fmaf(wm[r][h], sum, e[r][h]);
+                dst[q] = e[r][h];
+                big |= __ballot(fabsf(sum) >= tol) & owm[r][h];
             }
             __syncthreads();
         }
-        if (mx < tol) below |= 1u << s;
+        if (big == 0) below |= 1u << s;
     }
 
     // p + e on the owned points
