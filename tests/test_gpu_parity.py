@@ -157,8 +157,8 @@ def test_alt_trid_lu_rounding_cycles_bitwise(oracle):
     """Stretches of identical rows on which speculative runs never coincide
     with the exact one (two fixed points of the rounded recurrence): inside a
     2048-row block, across block boundaries and longer than a block.  The
-    forward sweep's serial walk and the cross-block repairs must still give
-    the sequential sweep's bits."""
+    fix-up rounds' pass through constant-even segments (forward and backward)
+    and the cross-block repairs must still give the sequential sweep's bits."""
     rng = np.random.default_rng(7)
     a, b = _chain(30_000, rng)
     row = _two_fixed_points(rng)
