@@ -177,14 +177,31 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             float rr = 0.f;
-            if ((upd >> (2 * r + h)) & 1) {
+            if (UNI) {
+                // straight-line: every point is evaluated and a point that is
+                // not updated gets r = 0 by a select (the west index is kept
+                // inside the tile; what is read there is discarded)
+                const double pw = h ? pr[r].x : P[c0 - (x > 0 ? 1 : 0)];
+                const double pe = h ? P[c0 + 2] : pr[r].y;
+                double res = fma(-rgv_c, h ? ps.y : ps.x, h ? bvr[r].y : bvr[r].x);
+                res = fma(-rau_c, pw, res);
+                res = fma(-a3u, h ? pr[r].y : pr[r].x, res);
+                res = fma(-rau_c, pe, res);
+                res = fma(-rgv_c, h ? pn.y : pn.x, res);
+                rr = ((upd >> (2 * r + h)) & 1) ? (float)res * i3u : 0.f;
+            } else if ((upd >> (2 * r + h)) & 1) {
                 const double a1 = h ? g[r].y : g[r].x, a5 = h ? g[r + 1].y : g[r + 1].x;
                 const double a2 = ra[h], a4 = ra[h + 1];
                 const double a3 = UNI ? a3u : -a4 - a2 - a5 - a1;
                 const double pw = h ? pr[r].x : P[c0 - 1];
                 const double pe = h ? P[c0 + 2] : pr[r].y;
-                const double res = (h ? bvr[r].y : bvr[r].x) - a1 * (h ? ps.y : ps.x) - a2 * pw -
-                                   a3 * (h ? pr[r].y : pr[r].x) - a4 * pe - a5 * (h ? pn.y : pn.x);
+                // the residual as one FMA chain (the defect only needs to be
+                // accurate, not the reference's rounding)
+                double res = fma(-a1, h ? ps.y : ps.x, h ? bvr[r].y : bvr[r].x);
+                res = fma(-a2, pw, res);
+                res = fma(-a3, h ? pr[r].y : pr[r].x, res);
+                res = fma(-a4, pe, res);
+                res = fma(-a5, h ? pn.y : pn.x, res);
                 const float i3 = UNI ? i3u : 1.0f / (float)a3;
                 rr = (float)res * i3;
                 if (!UNI) {
