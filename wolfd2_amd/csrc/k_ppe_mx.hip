@@ -58,6 +58,17 @@ static_assert((SR * ES) % 32 == 16 && ES >= MK, "conflict-free strip rows");
 // j = 2..ny; rgv on i = 2..nx, j = 1..ny) equals rau_c / rgv_c (a uniform
 // Cartesian grid): the two values replace the loads (the same values, so the
 // same arithmetic), and the sweep coefficients are the same at every point.
+// f(i,j), f(i+1,j) for even i, 0 for points outside the array (i outside
+// 0..nx+1 or j outside 0..ny+1), as one 16-byte load from a clamped address
+// and two selects: the clamped pair lies inside the ld x (ny+2) field (ld is
+// even and at least nx+2), so no branch is needed.
+__device__ inline double2 load_pair_sel(const double *__restrict__ f, int ld, int nx, int ny, int i, int j) {
+    const int ic = min(max(i, 0), ld - 2), jc = min(max(j, 0), ny + 1);
+    const double2 v = *reinterpret_cast<const double2 *>(f + (size_t)ic + (size_t)ld * jc);
+    const bool jin = j >= 0 && j <= ny + 1;
+    return make_double2(jin && i >= 0 && i <= nx + 1 ? v.x : 0.0, jin && i + 1 >= 0 && i + 1 <= nx + 1 ? v.y : 0.0);
+}
+
 template <bool UNI>
 __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, const double *__restrict__ rau,
                                                        const double *__restrict__ rgv, const double *__restrict__ b,
@@ -115,26 +126,23 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     }
 
     // All loads are issued before the barrier, so their latencies overlap:
-    // p of the strip (into registers and the LDS tile; 0 outside the array),
-    // b and, unless UNI, rau of the rows with points to update, and the rgv
-    // pairs of the strip's rows and the row below (a row's a1 is the row
-    // below's a5).
+    // p and b of the strip (p into registers and the LDS tile; 0 outside the
+    // array; b is read wherever it is, only updated points use it), unless
+    // UNI the rau of the rows with points to update, and the rgv pairs of the
+    // strip's rows and the row below (a row's a1 is the row below's a5).
     double2 pr[SR], bvr[SR], rav[SR], g[SR + 1];
     double ram[SR];
 #pragma unroll
     for (int r = 0; r < SR; r++) {
         const int j = j0 + y0 + r;
-        pr[r] = load_pair(p_in, ld, nx, ny, i, j);
-        bvr[r] = make_double2(0.0, 0.0);
+        pr[r] = load_pair_sel(p_in, ld, nx, ny, i, j);
+        bvr[r] = load_pair_sel(b, ld, nx, ny, i, j);
         ram[r] = rau_c;
         rav[r] = make_double2(rau_c, rau_c);
-        if ((upd >> (2 * r)) & 3) {
+        if (!UNI && ((upd >> (2 * r)) & 3)) {
             // i >= 1 and i + 1 <= nx + 1 here: the pairs are inside the array
-            bvr[r] = *reinterpret_cast<const double2 *>(b + WD2_IX(i, j));
-            if (!UNI) {
-                ram[r] = rau[WD2_IX(i - 1, j)];
-                rav[r] = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
-            }
+            ram[r] = rau[WD2_IX(i - 1, j)];
+            rav[r] = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
         }
     }
 #pragma unroll
