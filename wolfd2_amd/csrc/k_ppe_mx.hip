@@ -286,8 +286,7 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
         for (int h = 0; h < 2; h++)
             if ((own >> (2 * r + h)) & 1)
                 p_out[WD2_IX(i + h, j0 + y0 + r)] = P[(y0 + r) * MLD + x + h] + (double)e[r][h];
-    for (int off = 32; off > 0; off >>= 1) below &= (unsigned)__shfl_xor((int)below, off);
-    if ((t & 63) == 0) sh[t >> 6] = below;
+    if ((t & 63) == 0) sh[t >> 6] = below;   // the same in every lane of the wavefront
     __syncthreads();
     if (t == 0) {
         unsigned r = ~0u;

@@ -229,6 +229,32 @@ __global__ __launch_bounds__(SX *SY) void k_fill(int ld, int i0, int i1, int j0,
     dst[WD2_IX(i, j)] = val;
 }
 
+// Up to six fields copied in one pass (the time-level copies of a step).
+constexpr int COPY_SET_MAX = 6;
+struct CopySet {
+    const double *src[COPY_SET_MAX];
+    double *dst[COPY_SET_MAX];
+    int n;
+};
+__global__ __launch_bounds__(SX *SY) void k_copy_set(int ld, int i0, int i1, int j0, int j1, CopySet cs) {
+    const int i = i0 + blockIdx.x * SX + threadIdx.x, j = j0 + blockIdx.y * SY + threadIdx.y;
+    if (i > i1 || j > j1) return;
+    const size_t k = WD2_IX(i, j);
+#pragma unroll
+    for (int q = 0; q < COPY_SET_MAX; q++)
+        if (q < cs.n) cs.dst[q][k] = cs.src[q][k];
+}
+void launch_copy_set(int ld, int i0, int i1, int j0, int j1, const double *const *src, double *const *dst, int n,
+                     hipStream_t s) {
+    CopySet cs{};
+    cs.n = n < COPY_SET_MAX ? n : COPY_SET_MAX;
+    for (int q = 0; q < cs.n; q++) {
+        cs.src[q] = src[q];
+        cs.dst[q] = dst[q];
+    }
+    hipLaunchKernelGGL(k_copy_set, grid2(i1 - i0 + 1, j1 - j0 + 1), dim3(SX, SY), 0, s, ld, i0, i1, j0, j1, cs);
+}
+
 void launch_copy(int ld, int i0, int i1, int j0, int j1, const double *src, double *dst, hipStream_t s) {
     hipLaunchKernelGGL(k_copy, grid2(i1 - i0 + 1, j1 - j0 + 1), dim3(SX, SY), 0, s, ld, i0, i1, j0, j1, src, dst);
 }

@@ -534,8 +534,11 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
 // momentum.f:nAuxMomentum 33-193 on the context's un, vn -> us, vs.
 static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr, double qtol, int *nql) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
-    launch_copy(ld, 1, nx + 1, 1, ny + 1, c->un, c->us, c->s);
-    launch_copy(ld, 1, nx + 1, 1, ny + 1, c->vn, c->vs, c->s);
+    {
+        const double *src[2] = {c->un, c->vn};
+        double *dst[2] = {c->us, c->vs};
+        launch_copy_set(ld, 1, nx + 1, 1, ny + 1, src, dst, 2, c->s);
+    }
     *nql = -1;
     for (int m = 1; m <= mqiter; m++) {
         if (c->any_outlet) launch_vel_bc<double>(c->R, ld, c->us, c->vs, true, &c->sc->err, c->s);
@@ -734,19 +737,27 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
     const wd2_step_options &O = c->opt;
     const bool therm = O.nthermen == 1;
     hipStream_t s = c->s;
-    // main.f:696-704
-    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->p, c->pn, s);
-    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->u, c->un, s);
-    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->v, c->vn, s);
-    if (therm) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->t, c->tn, s);
-    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->d, c->dn, s);
+    // main.f:696-704 and the first iteration's us = u, vs = v (main.f:741-747)
+    // in one launch.  Without a density field d and dn are both still all +0
+    // (nothing has written d), so that copy is skipped.
+    {
+        const double *src[6] = {c->p, c->u, c->v, c->u, c->v, nullptr};
+        double *dst[6] = {c->pn, c->un, c->vn, c->us, c->vs, nullptr};
+        int n = 5;
+        if (c->d_live) { src[n] = c->d; dst[n] = c->dn; n++; }
+        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, n, s);
+        if (therm) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->t, c->tn, s);
+    }
     // main.f:735: one iteration without the energy equation
     const int nme = (!therm && O.nmeiter > 0) ? 1 : O.nmeiter;
     int st;
     for (int l = 1; l <= nme; l++) {
-        // main.f:741-747
-        launch_copy(ld, 0, nx + 1, 0, ny + 1, c->u, c->us, s);
-        launch_copy(ld, 0, nx + 1, 0, ny + 1, c->v, c->vs, s);
+        // main.f:741-747 (the first iteration's us, vs come with the copies above)
+        if (l > 1) {
+            const double *src[2] = {c->u, c->v};
+            double *dst[2] = {c->us, c->vs};
+            launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 2, s);
+        }
         if (therm) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->t, c->ts, s);
         if ((st = naux_momentum(c, P->mqiter, P->dk, P->re, P->fr, P->qtol, nql))) return st;
         // main.f:783-791

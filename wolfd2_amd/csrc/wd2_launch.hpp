@@ -63,6 +63,9 @@ void launch_max_norm(int nx, int ny, int ld, const double *a, const double *b, d
                      int slot, hipStream_t s);
 // dst = src on i in [i0,i1], j in [j0,j1]
 void launch_copy(int ld, int i0, int i1, int j0, int j1, const double *src, double *dst, hipStream_t s);
+// dst[q] = src[q] over the same box for q < n (n <= 6), one launch
+void launch_copy_set(int ld, int i0, int i1, int j0, int j1, const double *const *src, double *const *dst, int n,
+                     hipStream_t s);
 void launch_fill(int ld, int i0, int i1, int j0, int j1, double val, double *dst, hipStream_t s);
 
 // SOR family.  coef: null -> coefficients from rau/rgv; else five fields.
