@@ -252,105 +252,111 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
     const int TI = tile % ntx, TJ = tile / ntx;
     const int i0 = 2 + TI * TT - TH, j0 = 2 + TJ * TT - TH;   // tile origin
     const int t = threadIdx.x;
-    double cb[TP_PAIRS][2], c1[TP_PAIRS][2], c2[TP_PAIRS][2], c4[TP_PAIRS][2], c5[TP_PAIRS][2];
-    int flags[TP_PAIRS];
+    // Per pair r and colour c, everything the sweeps need for the pair's point
+    // of that colour (h = its side, 0 left or 1 right): b and a1, a2, a4, a5
+    // (+0, +0, +0, +0, 1 where the point is not updated, so a3 = -1 there),
+    // a3 = -a4 - a2 - a5 - a1 (the reference's expression, evaluated once),
+    // the LDS offsets of its west and east neighbours (q - 1 + h and q + h)
+    // and of its south and north ones, clamped to the point's own slot where
+    // they would leave the tile (the outer ring is never updated: what such a
+    // point reads is discarded), and whether the point is updated / owned.
+    double cb[TP_PAIRS][2], c1[TP_PAIRS][2], c2[TP_PAIRS][2], c3[TP_PAIRS][2], c4[TP_PAIRS][2], c5[TP_PAIRS][2];
+    int ow[TP_PAIRS][2], oe[TP_PAIRS][2], os[TP_PAIRS], on[TP_PAIRS];
+    bool up[TP_PAIRS][2];
+    unsigned long long ownm[TP_PAIRS][2];
+    int lcs[TP_PAIRS];
 #define PX(r) (2 * ((t + TP_T * (r)) % TK))
 #define PY(r) ((t + TP_T * (r)) / TK)
 #define PB(r) (t + TP_T * (r))   // y*TK + k of the lane's pair r
 #pragma unroll
     for (int r = 0; r < TP_PAIRS; r++) {
-        const int x = PX(r), y = PY(r);
+        const int x = PX(r), y = PY(r), kk = x / 2;
         const int i = i0 + x, j = j0 + y;   // i is even: the pair (i, i+1) is 16-byte aligned
-        const double2 pv = load_pair(p_in, ld, nx, ny, i, j);
+        const double2 pv = load_pair_sel(p_in, ld, nx, ny, i, j);
         const int lc = (i + j) & 1;   // colour of the left point
+        lcs[r] = lc;
         PC[lc][PB(r)] = pv.x;
         PC[lc ^ 1][PB(r)] = pv.y;
         const bool rowin = j >= 2 && j <= ny && y >= 1 && y <= TW - 2;
         const bool upd0 = rowin && i >= 2 && i <= nx && x >= 1;
         const bool upd1 = rowin && i + 1 >= 2 && i + 1 <= nx && x + 1 <= TW - 2;
-        // coefficients as aligned pairs (i >= 2 and i + 1 <= nx + 1 when a point updates)
-        double2 bv = make_double2(0.0, 0.0), g0 = make_double2(0.0, 0.0), g1 = make_double2(1.0, 1.0);
-        double2 av = make_double2(0.0, 0.0);
-        double aw = 0.0;
-        if (upd0 || upd1) {
-            bv = *reinterpret_cast<const double2 *>(b + WD2_IX(i, j));
-            if (uni) {   // every rau / rgv entry an updated point reads has this value
-                g0 = g1 = make_double2(rgv_c, rgv_c);
-                av = make_double2(rau_c, rau_c);
-                aw = rau_c;
-            } else {
-                g0 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j - 1));
-                g1 = *reinterpret_cast<const double2 *>(rgv + WD2_IX(i, j));
-                av = *reinterpret_cast<const double2 *>(rau + WD2_IX(i, j));
-                aw = rau[WD2_IX(i - 1, j)];
-            }
+        // coefficients as aligned pairs, read from clamped addresses and used
+        // only where a point updates (i >= 2 and i + 1 <= nx + 1 there)
+        const double2 bv = load_pair_sel(b, ld, nx, ny, i, j);
+        double2 g0, g1, av;
+        double aw;
+        if (uni) {   // every rau / rgv entry an updated point reads has this value
+            g0 = g1 = make_double2(rgv_c, rgv_c);
+            av = make_double2(rau_c, rau_c);
+            aw = rau_c;
+        } else {
+            g0 = load_pair_sel(rgv, ld, nx, ny, i, j - 1);
+            g1 = load_pair_sel(rgv, ld, nx, ny, i, j);
+            av = load_pair_sel(rau, ld, nx, ny, i, j);
+            aw = rau[(size_t)min(max(i - 1, 0), ld - 1) + (size_t)ld * (size_t)min(max(j, 0), ny + 1)];
         }
-        int f = 0;
+        os[r] = y > 0 ? -TK : 0;
+        on[r] = y < TW - 1 ? TK : 0;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const bool upd = h ? upd1 : upd0;
-            const bool own = upd && x + h >= TH && x + h < TH + TT && y >= TH && y < TH + TT;
-            double bb = 0, a1 = 0, a2 = 0, a4 = 0, a5 = 1;
-            if (upd) {
-                bb = h ? bv.y : bv.x;
-                a1 = h ? g0.y : g0.x;
-                a2 = h ? av.x : aw;
-                a4 = h ? av.y : av.x;
-                a5 = h ? g1.y : g1.x;
-            }
-            cb[r][h] = bb; c1[r][h] = a1; c2[r][h] = a2; c4[r][h] = a4; c5[r][h] = a5;
-            f |= (upd ? 1 : 0) << h;
-            f |= (own ? 1 : 0) << (2 + h);
+        for (int c = 0; c < 2; c++) {
+            const int h = lc ^ c;
+            const bool u = h ? upd1 : upd0;
+            const bool own = u && x + h >= TH && x + h < TH + TT && y >= TH && y < TH + TT;
+            const double bb = u ? (h ? bv.y : bv.x) : 0.0;
+            const double a1 = u ? (h ? g0.y : g0.x) : 0.0;
+            const double a2 = u ? (h ? av.x : aw) : 0.0;
+            const double a4 = u ? (h ? av.y : av.x) : 0.0;
+            const double a5 = u ? (h ? g1.y : g1.x) : 1.0;
+            cb[r][c] = bb; c1[r][c] = a1; c2[r][c] = a2; c4[r][c] = a4; c5[r][c] = a5;
+            c3[r][c] = -a4 - a2 - a5 - a1;
+            ow[r][c] = h ? 0 : (kk > 0 ? -1 : 0);
+            oe[r][c] = h ? (kk < TK - 1 ? 1 : 0) : 0;
+            up[r][c] = u;
+            ownm[r][c] = __ballot(own);
         }
-        f |= ((i + j) & 1) << 4;   // black iff (i + j) even
-        flags[r] = f;
     }
     __syncthreads();
-    unsigned below = 0;   // bit s: this lane's owned max |s| of sweep s is < sortol
+    // The sweeps are straight-line code: every point is evaluated and a point
+    // that is not updated keeps its value by a select.  The stopping test
+    // flags an owned point whose |s| is not below sortol (a NaN counts as not
+    // converged), which is max |s| < sortol over the owned points otherwise;
+    // it is gathered per wavefront as ballots.
+    unsigned below = 0;   // bit s: the wavefront's owned max |s| of sweep s is < sortol
     for (int s = 0; s < nsweep; s++) {
-        double mx = 0.0;
+        unsigned long long big = 0;
 #pragma unroll
         for (int color = 0; color < 2; color++) {
             double *__restrict__ dst = PC[color];
             const double *__restrict__ src = PC[1 - color];
 #pragma unroll
             for (int r = 0; r < TP_PAIRS; r++) {
-                // h: the pair's point of this colour (0 left, 1 right); its west
-                // neighbour is the other colour in pair k-1+h, its east one in k+h
-                const int h = ((flags[r] >> 4) & 1) ^ color;
-                if ((flags[r] >> h) & 1) {
-                    const int q = PB(r);
-                    // register selects (a runtime index into the arrays would go to scratch)
-                    const double a1 = h ? c1[r][1] : c1[r][0], a2 = h ? c2[r][1] : c2[r][0];
-                    const double a4 = h ? c4[r][1] : c4[r][0], a5 = h ? c5[r][1] : c5[r][0];
-                    const double bb = h ? cb[r][1] : cb[r][0];
-                    const double a3 = -a4 - a2 - a5 - a1;
-                    const double pij = dst[q];
-                    double sum = bb - a1 * src[q - TK] - a2 * src[q - 1 + h] - a4 * src[q + h] - a5 * src[q + TK];
-                    sum = sum / a3 - pij;
-                    dst[q] = pij + sorrel * sum;
-                    if ((flags[r] >> (2 + h)) & 1) mx = dmaxf(mx, fabs(sum));
-                }
+                const int q = PB(r);
+                const double pij = dst[q];
+                // the reference's order: b - a1 pS - a2 pW - a4 pE - a5 pN
+                double sum = cb[r][color] - c1[r][color] * src[q + os[r]] - c2[r][color] * src[q + ow[r][color]] -
+                             c4[r][color] * src[q + oe[r][color]] - c5[r][color] * src[q + on[r]];
+                sum = sum / c3[r][color] - pij;
+                const double pn = pij + sorrel * sum;
+                dst[q] = up[r][color] ? pn : pij;
+                big |= __ballot(!(fabs(sum) < sortol)) & ownm[r][color];
             }
             __syncthreads();
         }
-        if (mx < sortol) below |= 1u << s;
+        if (big == 0) below |= 1u << s;
     }
 #pragma unroll
     for (int r = 0; r < TP_PAIRS; r++) {
-        const int lc = (flags[r] >> 4) & 1;
 #pragma unroll
-        for (int h = 0; h < 2; h++)
-            if ((flags[r] >> (2 + h)) & 1) {
-                const int x = PX(r) + h, y = PY(r);
-                p_out[WD2_IX(i0 + x, j0 + y)] = PC[lc ^ h][PB(r)];
+        for (int c = 0; c < 2; c++)
+            if ((ownm[r][c] >> (t & 63)) & 1ull) {
+                const int x = PX(r) + (lcs[r] ^ c), y = PY(r);
+                p_out[WD2_IX(i0 + x, j0 + y)] = PC[c][PB(r)];
             }
     }
 #undef PX
 #undef PY
 #undef PB
-    // tile mask = AND over lanes
-    for (int off = 32; off > 0; off >>= 1) below &= (unsigned)__shfl_xor((int)below, off);
+    // tile mask = AND over the wavefronts (below is the same in every lane of one)
     if ((t & 63) == 0) sh[t >> 6] = below;
     __syncthreads();
     if (t == 0) {

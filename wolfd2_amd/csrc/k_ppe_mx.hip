@@ -58,17 +58,6 @@ static_assert((SR * ES) % 32 == 16 && ES >= MK, "conflict-free strip rows");
 // j = 2..ny; rgv on i = 2..nx, j = 1..ny) equals rau_c / rgv_c (a uniform
 // Cartesian grid): the two values replace the loads (the same values, so the
 // same arithmetic), and the sweep coefficients are the same at every point.
-// f(i,j), f(i+1,j) for even i, 0 for points outside the array (i outside
-// 0..nx+1 or j outside 0..ny+1), as one 16-byte load from a clamped address
-// and two selects: the clamped pair lies inside the ld x (ny+2) field (ld is
-// even and at least nx+2), so no branch is needed.
-__device__ inline double2 load_pair_sel(const double *__restrict__ f, int ld, int nx, int ny, int i, int j) {
-    const int ic = min(max(i, 0), ld - 2), jc = min(max(j, 0), ny + 1);
-    const double2 v = *reinterpret_cast<const double2 *>(f + (size_t)ic + (size_t)ld * jc);
-    const bool jin = j >= 0 && j <= ny + 1;
-    return make_double2(jin && i >= 0 && i <= nx + 1 ? v.x : 0.0, jin && i + 1 >= 0 && i + 1 <= nx + 1 ? v.y : 0.0);
-}
-
 template <bool UNI>
 __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, const double *__restrict__ rau,
                                                        const double *__restrict__ rgv, const double *__restrict__ b,

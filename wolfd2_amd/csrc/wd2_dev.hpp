@@ -141,6 +141,16 @@ __device__ inline double2 load_pair(const double *__restrict__ f, int ld, int nx
     return v;
 }
 
+// The same as one 16-byte load from a clamped address and two selects: the
+// clamped pair lies inside the ld x (ny+2) field (ld is even and at least
+// nx+2), so no branch is needed.
+__device__ inline double2 load_pair_sel(const double *__restrict__ f, int ld, int nx, int ny, int i, int j) {
+    const int ic = min(max(i, 0), ld - 2), jc = min(max(j, 0), ny + 1);
+    const double2 v = *reinterpret_cast<const double2 *>(f + (size_t)ic + (size_t)ld * jc);
+    const bool jin = j >= 0 && j <= ny + 1;
+    return make_double2(jin && i >= 0 && i <= nx + 1 ? v.x : 0.0, jin && i + 1 >= 0 && i + 1 <= nx + 1 ? v.y : 0.0);
+}
+
 // Load that bypasses the (non-coherent) per-CU cache: used to read values that
 // other workgroups of the same kernel published before an atomic ticket.
 __device__ inline double ld_coherent(const double *p) {
