@@ -98,6 +98,7 @@ struct wd2_ctx {
     // d (and dn) may be non-zero: set once EqState runs or a caller sends d;
     // until then both hold +0 and the momentum assembly does not load them
     bool d_live = false;
+    bool sor_capped = false;   // the last tiled SOR solve ran to msorit
     int precision = WD2_PREC_FP64;
     // pinned copies of the stop flag and events of the SOR batches in flight
     int *h_flag = nullptr;
@@ -602,11 +603,14 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
         // batches in flight: the host reads batch n's flag (copied to pinned
         // memory) while batch n+1 runs.  A batch issued after the solve
         // stopped returns at entry in every launch.  Batches grow from 8 to 64
-        // launches, so a short solve is not overrun and a long one rarely waits.
+        // launches, so a short solve is not overrun and a long one rarely
+        // waits; after a solve that ran to msorit they are 128 launches from
+        // the start (fewer checks; a solve that then stops early overruns by
+        // at most two batches of launches that return at entry).
         int issued = 0, checked = 0, stop = 0;
         while (!stop) {
             if (m0 < msorit && issued - checked < 2) {
-                const int Q = std::min(64, 8 << std::min(issued, 3));
+                const int Q = c->sor_capped ? 128 : std::min(64, 8 << std::min(issued, 3));
                 // timing: one event pair around the batch's launches (no marker
                 // between two launches, so their durations are not inflated)
                 hipEvent_t *ev = kt_events(c, 1);
@@ -652,6 +656,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
             if (out != p) launch_copy(ld, 0, nx + 1, 0, ny + 1, out, p, s);
         }
         int nconv = h.sor_nconv;
+        c->sor_capped = nconv < 1;
         if (nconv < 1) {
             if (g_verbose) fprintf(stdout, " Warning: SOR iterations did not converge after %6d iterations.\n", msorit);
             nconv = msorit;
