@@ -17,7 +17,7 @@
 
 namespace wd2 {
 
-constexpr int BC_THREADS = 256;
+constexpr int BC_THREADS = 1024;
 
 #define PAR(var, lo, hi) for (int var = (lo) + (int)threadIdx.x; var <= (hi); var += BC_THREADS)
 #define SER(var, lo, hi) if (threadIdx.x == 0) for (int var = (lo); var <= (hi); var++)

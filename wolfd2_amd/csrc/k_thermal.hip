@@ -26,7 +26,7 @@ namespace wd2 {
 
 constexpr int TX = 64, TY = 4;
 static inline dim3 tgrid(int ni, int nj) { return dim3((ni + TX - 1) / TX, (nj + TY - 1) / TY); }
-constexpr int TBC_THREADS = 256;
+constexpr int TBC_THREADS = 1024;
 
 #define PAR(var, lo, hi) for (int var = (lo) + (int)threadIdx.x; var <= (hi); var += TBC_THREADS)
 
