@@ -23,8 +23,9 @@
 // answer is exact in every case.  A thread loads and stores its segment as
 // 16-byte vectors.
 //
-// A batch of independent chains of equal length L is solved at once: row k
-// starts a chain when k % L == 0 and ends one when k % L == L-1.
+// A batch of independent chains is solved at once: rows [0, n1) are chains
+// of length L1 and rows [n1, n) chains of length L2 (Chains).  No state
+// crosses a chain start, so a batch gives each chain the bits it has alone.
 #include <type_traits>
 
 #include "wd2_dev.hpp"
@@ -53,9 +54,36 @@ __device__ inline void fwd_row(int kind, T a, T d, T c, T b, T dnext, T cpv, T b
     }
 }
 
-__device__ inline int row_kind(long k, long L) {
-    const long r = k % L;
-    return r == 0 ? 1 : (r == L - 1 ? 2 : 0);
+// Position r of row k in its chain, and that chain's length Lk.
+__device__ inline long chain_pos(const Chains &cs, long k, long &Lk) {
+    if (k < cs.n1) {
+        Lk = cs.L1;
+        return k % cs.L1;
+    }
+    Lk = cs.L2;
+    return (k - cs.n1) % cs.L2;
+}
+// (r, Lk) of row k + 1 from those of row k, and of row k - 1.
+__device__ inline void chain_up(const Chains &cs, long k, long &r, long &Lk) {
+    if (++r == Lk) {
+        r = 0;
+        if (k + 1 == cs.n1) Lk = cs.L2;
+    }
+}
+__device__ inline void chain_down(const Chains &cs, long k, long &r, long &Lk) {
+    if (r == 0) {
+        if (k == cs.n1) Lk = cs.L1;
+        r = Lk - 1;
+    } else {
+        r--;
+    }
+}
+__device__ inline int kind_at(long r, long Lk) { return r == 0 ? 1 : (r == Lk - 1 ? 2 : 0); }
+__device__ inline bool chain_end(long r, long Lk) { return r == Lk - 1 && r != 0; }
+__device__ inline int row_kind(long k, const Chains &cs) {
+    long Lk;
+    const long r = chain_pos(cs, k, Lk);
+    return kind_at(r, Lk);
 }
 
 // A thread's TR_S consecutive rows are read and written as 16-byte vectors
@@ -103,14 +131,14 @@ __device__ inline void seg_store(T *__restrict__ G, long seg0, int len, const T 
 
 
 // Kinds of the TR_S rows from seg0 on, two bits per row (one 64-bit modulo).
-__device__ inline unsigned seg_kinds(long seg0, long L) {
-    long r = seg0 % L;
+__device__ inline unsigned seg_kinds(long seg0, const Chains &cs) {
+    long Lk;
+    long r = chain_pos(cs, seg0, Lk);
     unsigned m = 0;
 #pragma unroll
     for (int s = 0; s < TR_S; s++) {
-        const unsigned k = r == 0 ? 1u : (r == L - 1 ? 2u : 0u);
-        m |= k << (2 * s);
-        if (++r == L) r = 0;
+        m |= (unsigned)kind_at(r, Lk) << (2 * s);
+        chain_up(cs, seg0 + s, r, Lk);
     }
     return m;
 }
@@ -140,12 +168,13 @@ __device__ inline void warm_fwd_load(const T *__restrict__ A, const T *__restric
     }
 }
 template <typename T>
-__device__ inline void warm_fwd(long blk0, long L, const T (*w)[TR_W], T &cpv, T &bpv) {
+__device__ inline void warm_fwd(long blk0, const Chains &cs, const T (*w)[TR_W], T &cpv, T &bpv) {
     const long k0 = blk0 > TR_W ? blk0 - TR_W : 0;
-    long r = k0 % L;
+    long Lk;
+    long r = chain_pos(cs, k0, Lk);
     T c = T(0), bb = T(0);
-    for (int q = 0; k0 + q < blk0; q++, r = (r + 1 == L) ? 0 : r + 1) {
-        const int kind = r == 0 ? 1 : (r == L - 1 ? 2 : 0);
+    for (int q = 0; k0 + q < blk0; chain_up(cs, k0 + q, r, Lk), q++) {
+        const int kind = kind_at(r, Lk);
         T ck, bk;
         fwd_row<T>(kind, w[0][q], w[1][q], w[2][q], w[3][q], kind == 1 ? w[4][q] : T(1), c, bb, ck, bk);
         c = ck;
@@ -165,13 +194,14 @@ __device__ inline void warm_bwd_load(const T *CP, const T *BP, long n, long k1, 
     }
 }
 template <typename T>
-__device__ inline T warm_bwd(const T (*w)[TR_WB], long n, long k1, long L) {
+__device__ inline T warm_bwd(const T (*w)[TR_WB], long n, long k1, const Chains &cs) {
     const long k2 = min(n, k1 + TR_WB);
-    long r = (k2 - 1) % L;
+    long Lk;
+    long r = chain_pos(cs, k2 - 1, Lk);
     T xw = T(0);
-    for (long k = k2 - 1; k >= k1; k--, r = (r == 0) ? L - 1 : r - 1) {
+    for (long k = k2 - 1; k >= k1; chain_down(cs, k, r, Lk), k--) {
         const int q = (int)(k - k1);
-        xw = (r == L - 1 && r != 0) ? w[1][q] : w[1][q] - (w[0][q] * xw);
+        xw = chain_end(r, Lk) ? w[1][q] : w[1][q] - (w[0][q] * xw);
     }
     return xw;
 }
@@ -233,7 +263,7 @@ __device__ inline bool bit_at(const unsigned long long *m, int i) { return (m[i 
 constexpr int TR_ACC = 3;
 
 template <typename T>
-__global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__restrict__ A, const T *__restrict__ D,
+__global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, Chains cs, const T *__restrict__ A, const T *__restrict__ D,
                                                    const T *__restrict__ Cc, const T *__restrict__ B,
                                                    T *__restrict__ CP, T *__restrict__ BP, T *__restrict__ blk_c,
                                                    T *__restrict__ blk_b) {
@@ -254,13 +284,13 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
     seg_load(Cc, seg0, len, c);
     seg_load(B, seg0, len, b);
     __syncthreads();   // sw is complete
-    const unsigned kinds = seg_kinds(seg0, L);
+    const unsigned kinds = seg_kinds(seg0, cs);
     // the first-pivot quirk needs the original diagonal of the row after a chain start
 #define DNEXT(s) (SKIND(kinds, s) == 1 ? ((s) + 1 < len ? d[((s) + 1) % TR_S] : D[seg0 + (s) + 1]) : T(1))
     // speculative run: lane 0 from the warmed-up state entering the block,
     // the others from the zero state
     T cpv = T(0), bpv = T(0);
-    if (t == 0 && blk0 > 0) warm_fwd<T>(blk0, L, sw, cpv, bpv);
+    if (t == 0 && blk0 > 0) warm_fwd<T>(blk0, cs, sw, cpv, bpv);
 #pragma unroll
     for (int s = 0; s < TR_S; s++) {
         if (s < len) {
@@ -321,8 +351,8 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
                 s67[4][t] = a[7]; s67[5][t] = d[7]; s67[6][t] = c[7]; s67[7][t] = b[7];
             }
             __syncthreads();
-            bool ce = len == TR_S && kinds == 0u && t > 0 && SKIND(seg_kinds(seg0 - TR_S, L), 6) == 0 &&
-                      SKIND(seg_kinds(seg0 - TR_S, L), 7) == 0;
+            bool ce = len == TR_S && kinds == 0u && t > 0 && SKIND(seg_kinds(seg0 - TR_S, cs), 6) == 0 &&
+                      SKIND(seg_kinds(seg0 - TR_S, cs), 7) == 0;
 #pragma unroll
             for (int s = 2; s < TR_S; s++)
                 ce = ce && same(a[s], a[s - 2]) && same(d[s], d[s - 2]) && same(c[s], c[s - 2]) &&
@@ -355,7 +385,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, long L, const T *__re
 // repaired rows are written back), then back-substitutes its segments with
 // the same speculation and in-workgroup fix-up rounds.
 template <typename T>
-__global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__restrict__ A, const T *__restrict__ D,
+__global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, Chains cs, const T *__restrict__ A, const T *__restrict__ D,
                                                    const T *__restrict__ Cc, const T *__restrict__ B,
                                                    T *__restrict__ CP, T *__restrict__ BP,
                                                    const T *__restrict__ blk_c, const T *__restrict__ blk_b,
@@ -395,12 +425,13 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
     if (blk0 + nrow < n) warm_bwd_load<T>(CP, BP, n, blk0 + nrow, sw);
     __syncthreads();
     if (t == 0 && blockIdx.x > 0) {
-        long r = blk0 % L;   // position of the block's first row in its chain
+        long Lk;
+        long r = chain_pos(cs, blk0, Lk);   // position of the block's first row in its chain
         if (r != 0) {
             T cpv = blk_c[blockIdx.x - 1], bpv = blk_b[blockIdx.x - 1];
             int k = 0;
-            for (; k < nrow; k++, r = (r + 1 == L) ? 0 : r + 1) {
-                const int kind = r == 0 ? 1 : (r == L - 1 ? 2 : 0);
+            for (; k < nrow; chain_up(cs, blk0 + k, r, Lk), k++) {
+                const int kind = kind_at(r, Lk);
                 const long g = blk0 + k;
                 const bool inl = k < TR_H;
                 const T dn = (kind == 1) ? (inl ? sh[4][k] : D[g + 1]) : T(1);
@@ -419,12 +450,12 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
     T cp[TR_S], bp[TR_S], x[TR_S];
 #pragma unroll
     for (int q = 0; q < TR_S; q++) { cp[q] = scp[t * TR_P + q]; bp[q] = sbp[t * TR_P + q]; }
-    const unsigned kinds = seg_kinds(seg0, L);
+    const unsigned kinds = seg_kinds(seg0, cs);
     // the lane with the block's last row starts from the warmed-up value of the
     // next block's first row (recorded for the check in k_trid_finish)
     T xn = T(0);
     if (len > 0 && t * TR_S + len == nrow && blk0 + nrow < n) {
-        xn = warm_bwd<T>(sw, n, blk0 + nrow, L);
+        xn = warm_bwd<T>(sw, n, blk0 + nrow, cs);
         blk_xe[blockIdx.x] = xn;
     }
 #pragma unroll
@@ -481,7 +512,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
             // mark the CE lanes (rows 8, 9 from the successor's staged c', b')
             bool ce = len == TR_S && kinds == 0u && (t + 1) * TR_S + 2 <= nrow;
             if (ce) {
-                const unsigned kn = seg_kinds(seg0 + TR_S, L);
+                const unsigned kn = seg_kinds(seg0 + TR_S, cs);
                 ce = SKIND(kn, 0) == 0 && SKIND(kn, 1) == 0;
             }
 #pragma unroll
@@ -518,7 +549,7 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, long L, const T *__re
 // sequentially (exact in every case).
 constexpr int TRF_T = 1024;
 template <typename T>
-__global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk, const T *__restrict__ A,
+__global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nblk, const T *__restrict__ A,
                                                        const T *__restrict__ D, const T *__restrict__ Cc,
                                                        const T *__restrict__ B, T *__restrict__ CP,
                                                        T *__restrict__ BP, T *__restrict__ X,
@@ -527,13 +558,14 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
                                                        T *__restrict__ B2, const unsigned char *__restrict__ Z) {
     for (int blk = threadIdx.x; blk < nblk - 1; blk += TRF_T) {
         const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
-        long r = (k1 - 1) % L;   // position of row k in its chain
-        if (r == L - 1 && r != 0) continue;
+        long Lk;
+        long r = chain_pos(cs, k1 - 1, Lk);   // position of row k in its chain
+        if (chain_end(r, Lk)) continue;
         T xv = blk_x[blk + 1];
         if (same(xv, blk_xe[blk])) continue;   // the block started from the exact value
         long k = k1 - 1;
-        for (; k >= k0; k--, r = (r == 0) ? L - 1 : r - 1) {
-            const T xk = (r == L - 1 && r != 0) ? BP[k] : BP[k] - (CP[k] * xv);
+        for (; k >= k0; chain_down(cs, k, r, Lk), k--) {
+            const T xk = chain_end(r, Lk) ? BP[k] : BP[k] - (CP[k] * xv);
             if (same(xk, X[k])) break;
             X[k] = xk;
             if (B2) B2[k] = Z[k] ? T(0) : xk;
@@ -544,19 +576,21 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
     __syncthreads();
     if (threadIdx.x != 0) return;
     if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-    for (long c0 = 0; c0 < n; c0 += L) {
+    for (long c0 = 0; c0 < n;) {
+        const long L = c0 < cs.n1 ? cs.L1 : cs.L2;
         T cpv = T(0), bpv = T(0);
         for (long k = c0; k < c0 + L; k++) {
-            const int kind = row_kind(k, L);
+            const int kind = row_kind(k, cs);
             fwd_row<T>(kind, A[k], D[k], Cc[k], B[k], kind == 1 ? D[k + 1] : T(1), cpv, bpv, CP[k], BP[k]);
             cpv = CP[k]; bpv = BP[k];
         }
         T xv = T(0);
         for (long k = c0 + L - 1; k >= c0; k--) {
-            X[k] = (row_kind(k, L) == 2) ? BP[k] : BP[k] - (CP[k] * xv);
+            X[k] = (row_kind(k, cs) == 2) ? BP[k] : BP[k] - (CP[k] * xv);
             xv = X[k];
             if (B2) B2[k] = Z[k] ? T(0) : xv;
         }
+        c0 += L;
     }
     *fail = 0;
     *nfallback += 1;
@@ -565,19 +599,29 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, long L, int nblk,
 int trid_blocks(long n) { return (int)((n + TR_R - 1) / TR_R); }
 
 template <typename T>
+void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
+                 T *X, hipStream_t s, T *B2, const unsigned char *Z) {
+    const int nb = trid_blocks(n);
+    hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b);
+    hipLaunchKernelGGL(k_trid_bwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
+                       X, w.blk_x, w.blk_xe, w.fail, B2, Z);
+    hipLaunchKernelGGL(k_trid_finish<T>, dim3(1), dim3(TRF_T), 0, s, n, cs, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
+                       w.blk_xe, w.fail, w.nfallback, B2, Z);
+}
+template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
                  hipStream_t s, T *B2, const unsigned char *Z) {
-    const int nb = trid_blocks(n);
-    hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b);
-    hipLaunchKernelGGL(k_trid_bwd<T>, dim3(nb), dim3(TR_T), 0, s, n, L, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
-                       X, w.blk_x, w.blk_xe, w.fail, B2, Z);
-    hipLaunchKernelGGL(k_trid_finish<T>, dim3(1), dim3(TRF_T), 0, s, n, L, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
-                       w.blk_xe, w.fail, w.nfallback, B2, Z);
+    launch_trid<T>(w, n, Chains{n, L, L}, A, D, C, B, X, s, B2, Z);
 }
 template void launch_trid<double>(const TridWorkT<double> &, long, long, const double *, const double *,
                                   const double *, const double *, double *, hipStream_t, double *,
                                   const unsigned char *);
 template void launch_trid<float>(const TridWorkT<float> &, long, long, const float *, const float *, const float *,
                                  const float *, float *, hipStream_t, float *, const unsigned char *);
+template void launch_trid<double>(const TridWorkT<double> &, long, const Chains &, const double *, const double *,
+                                  const double *, const double *, double *, hipStream_t, double *,
+                                  const unsigned char *);
+template void launch_trid<float>(const TridWorkT<float> &, long, const Chains &, const float *, const float *,
+                                 const float *, const float *, float *, hipStream_t, float *, const unsigned char *);
 
 }  // namespace wd2

@@ -169,7 +169,9 @@ static void ctx_free(wd2_ctx *c) {
 
 // Rows of each chain buffer: every chain of the grid fits, and a multiple of
 // 16 keeps each buffer 128-byte aligned (the chain solver reads 16-byte vectors).
-static size_t chain_rows(int nx, int ny) { return ((size_t)nx * ny + 64 + 15) / 16 * 16; }
+// Chain capacity: the X- and Y-momentum chains of one q-l iteration side by
+// side (nx(ny-1) + (nx-1)ny rows), padded.
+static size_t chain_rows(int nx, int ny) { return (2 * (size_t)nx * ny + 64 + 15) / 16 * 16; }
 
 static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     if (nx < 5 || ny < 5) return fail(WD2_ERR_ARG, "grid %dx%d too small (need nx, ny >= 5)", nx, ny);
@@ -525,6 +527,37 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
     return WD2_OK;
 }
 
+// XMomentum and YMomentum of one q-l iteration without porous regions.  The
+// two read the same us, vs, un, vn and write separate increments, so their
+// chains are assembled side by side (X rows first) and each split is one
+// solve of a batch of two chains, which gives each chain the bits it has alone.
+template <typename T>
+static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const MomTempsT<T> &t, const ChainT<T> &ch,
+                          const ChainT<T> &ch2, const TridWorkT<T> &tw, bool n_ready) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    const Metrics &M = c->M;
+    hipStream_t s = c->s;
+    const long n1 = (long)nx * (ny - 1), n2 = (long)(nx - 1) * ny;
+    const int live = c->d_live ? 2 : 0;
+    MomTempsT<T> t2 = t;
+    t2.cnvn = t.cnvn2;
+    t2.difn = t.difn2;
+    auto shift = [](const ChainT<T> &q, long o) { return ChainT<T>{q.a + o, q.d + o, q.c + o, q.b + o, q.x + o}; };
+    if (!n_ready) {
+        launch_mom_n<T>(nx, ny, ld, 1, M, c->un, c->vn, t.cnvn, t.difn, s);
+        launch_mom_n<T>(nx, ny, ld, 2, M, c->un, c->vn, t2.cnvn, t2.difn, s);
+    }
+    launch_mom_asm<T>(c->R, ld, 1, dk, re, fr, M, t, c->us, c->vs, c->un, c->vn, c->d, c->dn, ch, ch2, c->zm, s,
+                      &c->sc->err, live);
+    launch_mom_asm<T>(c->R, ld, 2, dk, re, fr, M, t2, c->us, c->vs, c->un, c->vn, c->d, c->dn, shift(ch, n1),
+                      shift(ch2, n1), c->zm + n1, s, &c->sc->err, live);
+    const Chains cs{n1, n1, n2};
+    launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm);
+    launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s);
+    launch_scatter<T>(nx, ny, ld, 1, ch2.x, c->dus, s);
+    launch_scatter<T>(nx, ny, ld, 2, ch2.x + n1, c->dvs, s);
+}
+
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
                     const double *un, const double *vn, double *out, bool n_ready = false) {
     if (c->precision == WD2_PREC_MIXED)
@@ -545,10 +578,17 @@ static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr
         if (c->any_outlet) launch_vel_bc<double>(c->R, ld, c->us, c->vs, true, &c->sc->err, c->s);
         // un, vn and the metrics are fixed during the iterations: the
         // time-level-n terms of the first iteration serve them all
-        int st = momentum(c, 1, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dus, m > 1);
-        if (st) return st;
-        st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs, m > 1);
-        if (st) return st;
+        if (!c->any_porous) {
+            if (c->precision == WD2_PREC_MIXED)
+                momentum_pair<float>(c, dk, re, fr, c->Tf, c->chf, c->chf2, c->twf, m > 1);
+            else
+                momentum_pair<double>(c, dk, re, fr, c->T, c->ch, c->ch2, c->tw, m > 1);
+        } else {
+            int st = momentum(c, 1, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dus, m > 1);
+            if (st) return st;
+            st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs, m > 1);
+            if (st) return st;
+        }
         launch_update_norms(nx, ny, ld, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, c->s);
         double nv[2];
         HIPCHK(hipMemcpyAsync(nv, c->sc->val, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));

@@ -18,11 +18,21 @@ struct TridWorkT {
 };
 using TridWork = TridWorkT<double>;
 int trid_blocks(long n);
+// Row layout of a batch of independent chains: rows [0, n1) are chains of
+// length L1, rows [n1, n) chains of length L2 (n1 a multiple of L1).
+struct Chains {
+    long n1, L1, L2;
+};
+// Chains of equal length L over rows [0, n).
 // B2, Z (optional): also write B2[k] = Z[k] ? +0 : X[k], the right-hand side
 // of a following system whose rows flagged in Z are identity rows.
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
                  hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr);
+// Any layout (n rows in all).
+template <typename T>
+void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
+                 T *X, hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr);
 
 // Boundary fills (one workgroup).  err receives 1000+side for a bad flag.
 template <typename T>
