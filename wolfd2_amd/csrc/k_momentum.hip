@@ -709,4 +709,61 @@ void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const d
     hipLaunchKernelGGL(k_update_norms_fold, dim3(1), dim3(256), 0, s, ld, dus, dvs, partials, gx * gy, sc);
 }
 
+// k_scatter (comp 1 from x1, comp 2 from x2) and k_update_norms per point:
+// dus is written where the X scatter writes it (j >= 2) and read elsewhere,
+// dvs likewise (i >= 2), so every stored value and every sum is the one the
+// separate kernels give.  A wavefront covers one row j.
+template <typename T>
+__global__ __launch_bounds__(MX *MY) void k_scatter_update(int nx, int ny, int ld, const T *__restrict__ x1,
+                                                           const T *__restrict__ x2, double *__restrict__ us,
+                                                           double *__restrict__ vs, double *__restrict__ dus,
+                                                           double *__restrict__ dvs, double *__restrict__ partials) {
+    __shared__ double sh[MX * MY / 64];
+    const int tid = threadIdx.x + MX * threadIdx.y;
+    double m1 = 0.0, m2 = 0.0;
+    const int i = 1 + blockIdx.x * MX + threadIdx.x;
+    for (int j = 1 + blockIdx.y * MY + threadIdx.y; j <= ny; j += gridDim.y * MY) {
+        if (i <= nx) {
+            const size_t k = WD2_IX(i, j);
+            double du, dv;
+            if (j >= 2) {
+                du = (double)x1[(long)(j - 2) * nx + (i - 1)];
+                dus[k] = du;
+            } else {
+                du = dus[k];
+            }
+            if (i >= 2) {
+                dv = (double)x2[(long)(j - 1) * (nx - 1) + (i - 2)];
+                dvs[k] = dv;
+            } else {
+                dv = dvs[k];
+            }
+            us[k] = us[k] + du;
+            vs[k] = vs[k] + dv;
+            if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1) {
+                m1 = dmaxf(m1, fabs(du));
+                m2 = dmaxf(m2, fabs(dv));
+            }
+        }
+    }
+    m1 = block_max<MX * MY>(m1, sh, tid);
+    m2 = block_max<MX * MY>(m2, sh, tid);
+    const unsigned int bid = blockIdx.x + gridDim.x * blockIdx.y;
+    if (tid == 0) { partials[2 * bid] = m1; partials[2 * bid + 1] = m2; }
+}
+
+template <typename T>
+void launch_scatter_update(int nx, int ny, int ld, const T *x1, const T *x2, double *us, double *vs, double *dus,
+                           double *dvs, double *partials, Scalars *sc, hipStream_t s) {
+    const int gx = (nx + MX - 1) / MX;
+    const int gy = min((ny + MY - 1) / MY, max(1, 1024 / gx));
+    hipLaunchKernelGGL(k_scatter_update<T>, dim3(gx, gy), dim3(MX, MY), 0, s, nx, ny, ld, x1, x2, us, vs, dus, dvs,
+                       partials);
+    hipLaunchKernelGGL(k_update_norms_fold, dim3(1), dim3(256), 0, s, ld, dus, dvs, partials, gx * gy, sc);
+}
+template void launch_scatter_update<double>(int, int, int, const double *, const double *, double *, double *,
+                                            double *, double *, double *, Scalars *, hipStream_t);
+template void launch_scatter_update<float>(int, int, int, const float *, const float *, double *, double *,
+                                           double *, double *, double *, Scalars *, hipStream_t);
+
 }  // namespace wd2

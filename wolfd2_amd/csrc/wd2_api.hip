@@ -527,9 +527,10 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
     return WD2_OK;
 }
 
-// XMomentum and YMomentum of one q-l iteration without porous regions.  The
-// two read the same us, vs, un, vn and write separate increments, so their
-// chains are assembled side by side (X rows first) and each split is one
+// XMomentum and YMomentum of one q-l iteration without porous regions, then
+// us += dus, vs += dvs and the increment norms (momentum.f:171-180).  The two
+// components read the same us, vs, un, vn and write separate increments, so
+// their chains are assembled side by side (X rows first) and each split is one
 // solve of a batch of two chains, which gives each chain the bits it has alone.
 template <typename T>
 static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const MomTempsT<T> &t, const ChainT<T> &ch,
@@ -554,8 +555,7 @@ static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const Mom
     const Chains cs{n1, n1, n2};
     launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm);
     launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s);
-    launch_scatter<T>(nx, ny, ld, 1, ch2.x, c->dus, s);
-    launch_scatter<T>(nx, ny, ld, 2, ch2.x + n1, c->dvs, s);
+    launch_scatter_update<T>(nx, ny, ld, ch2.x, ch2.x + n1, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, s);
 }
 
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
@@ -588,8 +588,8 @@ static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr
             if (st) return st;
             st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs, m > 1);
             if (st) return st;
+            launch_update_norms(nx, ny, ld, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, c->s);
         }
-        launch_update_norms(nx, ny, ld, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, c->s);
         double nv[2];
         HIPCHK(hipMemcpyAsync(nv, c->sc->val, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
         HIPCHK(hipStreamSynchronize(c->s));

@@ -188,6 +188,11 @@ template <typename T>
 void launch_scatter(int nx, int ny, int ld, int comp, const T *x, double *out, hipStream_t s);
 void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const double *dus, const double *dvs,
                          double *partials, Scalars *sc, hipStream_t s);
+// launch_scatter of both components (x1: the X chain solution, x2: the Y
+// chain solution) followed by launch_update_norms, in one pass.
+template <typename T>
+void launch_scatter_update(int nx, int ny, int ld, const T *x1, const T *x2, double *us, double *vs, double *dus,
+                           double *dvs, double *partials, Scalars *sc, hipStream_t s);
 
 // Thermal energy (k_thermal.hip).  err receives 2000+side for a bad
 // temperature boundary flag.
