@@ -27,7 +27,8 @@ constexpr int BC_THREADS = 1024;
 template <typename T>
 __global__ __launch_bounds__(BC_THREADS) void k_vel_bc(Regions R, int ld, T *__restrict__ u,
                                                        T *__restrict__ v, int outflow_only,
-                                                       int *__restrict__ err) {
+                                                       int *__restrict__ err, const int *stop) {
+    if (stop && *stop) return;
     const T dTwo = 2, dThree = 3, dFour = 4, dFive = 5, dEight = 8;
     for (int jr = 1; jr <= R.nj; jr++)
         for (int ir = 1; ir <= R.ni; ir++) {
@@ -227,15 +228,17 @@ __global__ __launch_bounds__(BC_THREADS) void k_pres_bc(Regions R, int ld, T *__
 }
 
 template <typename T>
-void launch_vel_bc(const Regions &R, int ld, T *u, T *v, bool outflow_only, int *err, hipStream_t s) {
-    hipLaunchKernelGGL(k_vel_bc<T>, dim3(1), dim3(BC_THREADS), 0, s, R, ld, u, v, outflow_only ? 1 : 0, err);
+void launch_vel_bc(const Regions &R, int ld, T *u, T *v, bool outflow_only, int *err, hipStream_t s,
+                   const int *stop) {
+    hipLaunchKernelGGL(k_vel_bc<T>, dim3(1), dim3(BC_THREADS), 0, s, R, ld, u, v, outflow_only ? 1 : 0, err, stop);
 }
 template <typename T>
 void launch_pres_bc(const Regions &R, int ld, T *p, int *err, hipStream_t s) {
     hipLaunchKernelGGL(k_pres_bc<T>, dim3(1), dim3(BC_THREADS), 0, s, R, ld, p, err);
 }
-template void launch_vel_bc<double>(const Regions &, int, double *, double *, bool, int *, hipStream_t);
-template void launch_vel_bc<float>(const Regions &, int, float *, float *, bool, int *, hipStream_t);
+template void launch_vel_bc<double>(const Regions &, int, double *, double *, bool, int *, hipStream_t,
+                                    const int *);
+template void launch_vel_bc<float>(const Regions &, int, float *, float *, bool, int *, hipStream_t, const int *);
 template void launch_pres_bc<double>(const Regions &, int, double *, int *, hipStream_t);
 
 }  // namespace wd2

@@ -539,7 +539,8 @@ __global__ __launch_bounds__(MX *MY) void k_mom_asm(Regions R, int ld, double dk
                                                     const double *__restrict__ un, const double *__restrict__ vn,
                                                     const double *__restrict__ d, const double *__restrict__ dn,
                                                     ChainT<T> ch1, ChainT<T> ch2, unsigned char *__restrict__ zm,
-                                                    int *err, int live) {
+                                                    int *err, int live, const int *stop) {
+    if (stop && *stop) return;
     const int nx = R.nx, ny = R.ny;
     int i, j;
     long k;
@@ -614,23 +615,23 @@ template <typename T>
 void launch_mom_asm(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
                     const MomTempsT<T> &t, const double *us, const double *vs, const double *un, const double *vn,
                     const double *d, const double *dn, const ChainT<T> &ch1, const ChainT<T> &ch2, unsigned char *zm,
-                    hipStream_t s, int *err, int live) {
+                    hipStream_t s, int *err, int live, const int *stop) {
     const double dk2 = dk * 0.5, re1 = 1.0 / re;
     if (comp == 1)
         hipLaunchKernelGGL((k_mom_asm<1, T>), mgrid(R.nx, R.ny - 1), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
-                           t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live);
+                           t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live, stop);
     else
         hipLaunchKernelGGL((k_mom_asm<2, T>), mgrid(R.nx - 1, R.ny), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
-                           t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live);
+                           t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live, stop);
 }
 template void launch_mom_asm<double>(const Regions &, int, int, double, double, double, const Metrics &,
                                      const MomTempsT<double> &, const double *, const double *, const double *,
                                      const double *, const double *, const double *, const ChainT<double> &,
-                                     const ChainT<double> &, unsigned char *, hipStream_t, int *, int);
+                                     const ChainT<double> &, unsigned char *, hipStream_t, int *, int, const int *);
 template void launch_mom_asm<float>(const Regions &, int, int, double, double, double, const Metrics &,
                                     const MomTempsT<float> &, const double *, const double *, const double *,
                                     const double *, const double *, const double *, const ChainT<float> &,
-                                    const ChainT<float> &, unsigned char *, hipStream_t, int *, int);
+                                    const ChainT<float> &, unsigned char *, hipStream_t, int *, int, const int *);
 
 // ------------------------------------------------------------- scatter
 template <typename T>
@@ -717,7 +718,9 @@ template <typename T>
 __global__ __launch_bounds__(MX *MY) void k_scatter_update(int nx, int ny, int ld, const T *__restrict__ x1,
                                                            const T *__restrict__ x2, double *__restrict__ us,
                                                            double *__restrict__ vs, double *__restrict__ dus,
-                                                           double *__restrict__ dvs, double *__restrict__ partials) {
+                                                           double *__restrict__ dvs, double *__restrict__ partials,
+                                                           const int *stop) {
+    if (*stop) return;
     __shared__ double sh[MX * MY / 64];
     const int tid = threadIdx.x + MX * threadIdx.y;
     double m1 = 0.0, m2 = 0.0;
@@ -752,18 +755,44 @@ __global__ __launch_bounds__(MX *MY) void k_scatter_update(int nx, int ny, int l
     if (tid == 0) { partials[2 * bid] = m1; partials[2 * bid + 1] = m2; }
 }
 
+// k_update_norms_fold, then the q-l stopping test of iteration m
+// (momentum.f:182-188: stop once max(DMaxNorm(dus), DMaxNorm(dvs)) <= qtol).
+__global__ __launch_bounds__(256) void k_ql_fold(int ld, const double *__restrict__ dus,
+                                                 const double *__restrict__ dvs, const double *__restrict__ partials,
+                                                 int nb, Scalars *sc, double qtol, int m) {
+    if (sc->ql_stop) return;
+    __shared__ double sh[4];
+    double r1 = 0.0, r2 = 0.0;
+    for (int q = threadIdx.x; q < nb; q += 256) {
+        r1 = dmaxf(r1, partials[2 * q]);
+        r2 = dmaxf(r2, partials[2 * q + 1]);
+    }
+    r1 = block_max<256>(r1, sh, threadIdx.x);
+    r2 = block_max<256>(r2, sh, threadIdx.x);
+    if (threadIdx.x == 0) {
+        const double v0 = dmaxf(fabs(dus[WD2_IX(5, 5)]), r1), v1 = dmaxf(fabs(dvs[WD2_IX(5, 5)]), r2);
+        sc->val[0] = v0;
+        sc->val[1] = v1;
+        const double difmax = v0 > v1 ? v0 : v1;
+        if (difmax <= qtol) sc->ql_stop = m;
+    }
+}
+
+__global__ void k_ql_reset(Scalars *sc) { sc->ql_stop = 0; }
+void launch_ql_reset(Scalars *sc, hipStream_t s) { hipLaunchKernelGGL(k_ql_reset, dim3(1), dim3(64), 0, s, sc); }
+
 template <typename T>
 void launch_scatter_update(int nx, int ny, int ld, const T *x1, const T *x2, double *us, double *vs, double *dus,
-                           double *dvs, double *partials, Scalars *sc, hipStream_t s) {
+                           double *dvs, double *partials, Scalars *sc, double qtol, int m, hipStream_t s) {
     const int gx = (nx + MX - 1) / MX;
     const int gy = min((ny + MY - 1) / MY, max(1, 1024 / gx));
     hipLaunchKernelGGL(k_scatter_update<T>, dim3(gx, gy), dim3(MX, MY), 0, s, nx, ny, ld, x1, x2, us, vs, dus, dvs,
-                       partials);
-    hipLaunchKernelGGL(k_update_norms_fold, dim3(1), dim3(256), 0, s, ld, dus, dvs, partials, gx * gy, sc);
+                       partials, &sc->ql_stop);
+    hipLaunchKernelGGL(k_ql_fold, dim3(1), dim3(256), 0, s, ld, dus, dvs, partials, gx * gy, sc, qtol, m);
 }
 template void launch_scatter_update<double>(int, int, int, const double *, const double *, double *, double *,
-                                            double *, double *, double *, Scalars *, hipStream_t);
+                                            double *, double *, double *, Scalars *, double, int, hipStream_t);
 template void launch_scatter_update<float>(int, int, int, const float *, const float *, double *, double *,
-                                           double *, double *, double *, Scalars *, hipStream_t);
+                                           double *, double *, double *, Scalars *, double, int, hipStream_t);
 
 }  // namespace wd2

@@ -266,7 +266,8 @@ template <typename T>
 __global__ __launch_bounds__(TR_T) void k_trid_fwd(long n, Chains cs, const T *__restrict__ A, const T *__restrict__ D,
                                                    const T *__restrict__ Cc, const T *__restrict__ B,
                                                    T *__restrict__ CP, T *__restrict__ BP, T *__restrict__ blk_c,
-                                                   T *__restrict__ blk_b) {
+                                                   T *__restrict__ blk_b, const int *stop) {
+    if (stop && *stop) return;
     __shared__ T sc[TR_T], sb[TR_T];
     __shared__ T sw[5][TR_W];
     __shared__ int changed;
@@ -391,7 +392,8 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, Chains cs, const T *_
                                                    const T *__restrict__ blk_c, const T *__restrict__ blk_b,
                                                    T *__restrict__ X, T *__restrict__ blk_x, T *__restrict__ blk_xe,
                                                    int *__restrict__ fail, T *__restrict__ B2,
-                                                   const unsigned char *__restrict__ Z) {
+                                                   const unsigned char *__restrict__ Z, const int *stop) {
+    if (stop && *stop) return;
     __shared__ T scp[TR_T * TR_P], sbp[TR_T * TR_P];
     __shared__ T sx[TR_T];
     __shared__ unsigned long long ce_mask[TR_T / 64], cyc_mask[TR_T / 64];
@@ -555,7 +557,9 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nb
                                                        T *__restrict__ BP, T *__restrict__ X,
                                                        const T *__restrict__ blk_x, const T *__restrict__ blk_xe,
                                                        int *__restrict__ fail, int *__restrict__ nfallback,
-                                                       T *__restrict__ B2, const unsigned char *__restrict__ Z) {
+                                                       T *__restrict__ B2, const unsigned char *__restrict__ Z,
+                                                       const int *stop) {
+    if (stop && *stop) return;
     for (int blk = threadIdx.x; blk < nblk - 1; blk += TRF_T) {
         const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
         long Lk;
@@ -600,13 +604,14 @@ int trid_blocks(long n) { return (int)((n + TR_R - 1) / TR_R); }
 
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
-                 T *X, hipStream_t s, T *B2, const unsigned char *Z) {
+                 T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop) {
     const int nb = trid_blocks(n);
-    hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b);
+    hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
+                       stop);
     hipLaunchKernelGGL(k_trid_bwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
-                       X, w.blk_x, w.blk_xe, w.fail, B2, Z);
+                       X, w.blk_x, w.blk_xe, w.fail, B2, Z, stop);
     hipLaunchKernelGGL(k_trid_finish<T>, dim3(1), dim3(TRF_T), 0, s, n, cs, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
-                       w.blk_xe, w.fail, w.nfallback, B2, Z);
+                       w.blk_xe, w.fail, w.nfallback, B2, Z, stop);
 }
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
@@ -620,8 +625,9 @@ template void launch_trid<float>(const TridWorkT<float> &, long, long, const flo
                                  const float *, float *, hipStream_t, float *, const unsigned char *);
 template void launch_trid<double>(const TridWorkT<double> &, long, const Chains &, const double *, const double *,
                                   const double *, const double *, double *, hipStream_t, double *,
-                                  const unsigned char *);
+                                  const unsigned char *, const int *);
 template void launch_trid<float>(const TridWorkT<float> &, long, const Chains &, const float *, const float *,
-                                 const float *, const float *, float *, hipStream_t, float *, const unsigned char *);
+                                 const float *, const float *, float *, hipStream_t, float *, const unsigned char *,
+                                 const int *);
 
 }  // namespace wd2

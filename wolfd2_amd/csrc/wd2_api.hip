@@ -101,8 +101,10 @@ struct wd2_ctx {
     bool sor_capped = false;   // the last tiled SOR solve ran to msorit
     int precision = WD2_PREC_FP64;
     // pinned copies of the stop flag and events of the SOR batches in flight
+    // (h_flag[0..1]) and of the q-l iterations in flight (h_flag[2..3])
     int *h_flag = nullptr;
     hipEvent_t bev[2] = {nullptr, nullptr};
+    hipEvent_t qev[2] = {nullptr, nullptr};
     // host transfers: two pinned staging buffers of one packed field each (the
     // caller's arrays are pageable and strided), and the last uploaded content
     // of every metric array of the drop-in calls (re-sent only when it changes)
@@ -144,6 +146,8 @@ static void ctx_free(wd2_ctx *c) {
     for (hipEvent_t e : c->kt_ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->bev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->qev)
+        if (e) (void)hipEventDestroy(e);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
     for (int k = 0; k < 2; k++) {
         if (c->sev[k]) (void)hipEventDestroy(c->sev[k]);
@@ -181,8 +185,9 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->ld = round_up(nx + 2, 16);
     c->nf = (size_t)c->ld * (ny + 2);
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-    HIPCHK(hipHostMalloc((void **)&c->h_flag, 2 * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&c->h_flag, 4 * sizeof(int), hipHostMallocDefault));
     for (hipEvent_t &e : c->bev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (hipEvent_t &e : c->qev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (int k = 0; k < 2; k++) {
         HIPCHK(hipHostMalloc((void **)&c->h_stage[k], (size_t)(nx + 2) * (ny + 2) * sizeof(double),
                              hipHostMallocDefault));
@@ -532,9 +537,14 @@ static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const
 // components read the same us, vs, un, vn and write separate increments, so
 // their chains are assembled side by side (X rows first) and each split is one
 // solve of a batch of two chains, which gives each chain the bits it has alone.
+// Iteration m's kernels return at entry once an earlier iteration has met
+// qtol (sc->ql_stop != 0), and its fold sets sc->ql_stop = m when it does, so
+// the host can queue iteration m before it reads iteration m-1's test.
 template <typename T>
 static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const MomTempsT<T> &t, const ChainT<T> &ch,
-                          const ChainT<T> &ch2, const TridWorkT<T> &tw, bool n_ready) {
+                          const ChainT<T> &ch2, const TridWorkT<T> &tw, int m, double qtol) {
+    const bool n_ready = m > 1;
+    const int *stop = &c->sc->ql_stop;
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     const Metrics &M = c->M;
     hipStream_t s = c->s;
@@ -549,13 +559,14 @@ static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const Mom
         launch_mom_n<T>(nx, ny, ld, 2, M, c->un, c->vn, t2.cnvn, t2.difn, s);
     }
     launch_mom_asm<T>(c->R, ld, 1, dk, re, fr, M, t, c->us, c->vs, c->un, c->vn, c->d, c->dn, ch, ch2, c->zm, s,
-                      &c->sc->err, live);
+                      &c->sc->err, live, stop);
     launch_mom_asm<T>(c->R, ld, 2, dk, re, fr, M, t2, c->us, c->vs, c->un, c->vn, c->d, c->dn, shift(ch, n1),
-                      shift(ch2, n1), c->zm + n1, s, &c->sc->err, live);
+                      shift(ch2, n1), c->zm + n1, s, &c->sc->err, live, stop);
     const Chains cs{n1, n1, n2};
-    launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm);
-    launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s);
-    launch_scatter_update<T>(nx, ny, ld, ch2.x, ch2.x + n1, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, s);
+    launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop);
+    launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop);
+    launch_scatter_update<T>(nx, ny, ld, ch2.x, ch2.x + n1, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, qtol,
+                             m, s);
 }
 
 static int momentum(wd2_ctx *c, int comp, double dk, double re, double fr, const double *us, const double *vs,
@@ -574,22 +585,45 @@ static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr
         launch_copy_set(ld, 1, nx + 1, 1, ny + 1, src, dst, 2, c->s);
     }
     *nql = -1;
+    if (!c->any_porous) {
+        if (mqiter < 1) return check_err(c);
+        // Pipelined by one iteration: iteration m is queued before the host
+        // reads iteration m-1's stopping test.  Once a test is met, every
+        // later iteration's kernels return at entry, so the fields are those
+        // of the sequential loop.  un, vn and the metrics are fixed during
+        // the iterations: the time-level-n terms of the first serve them all.
+        hipStream_t s = c->s;
+        const int *stop = &c->sc->ql_stop;
+        launch_ql_reset(c->sc, s);
+        int m = 1, last = 0;   // last: the iteration whose test was read last
+        for (; m <= mqiter; m++) {
+            if (c->any_outlet) launch_vel_bc<double>(c->R, ld, c->us, c->vs, true, &c->sc->err, s, stop);
+            if (c->precision == WD2_PREC_MIXED)
+                momentum_pair<float>(c, dk, re, fr, c->Tf, c->chf, c->chf2, c->twf, m, qtol);
+            else
+                momentum_pair<double>(c, dk, re, fr, c->T, c->ch, c->ch2, c->tw, m, qtol);
+            HIPCHK(hipMemcpyAsync(&c->h_flag[2 + m % 2], &c->sc->ql_stop, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipEventRecord(c->qev[m % 2], s));
+            if (m >= 2) {
+                HIPCHK(hipEventSynchronize(c->qev[(m - 1) % 2]));
+                last = m - 1;
+                if (c->h_flag[2 + last % 2]) break;
+            }
+        }
+        if (m > mqiter) {
+            HIPCHK(hipEventSynchronize(c->qev[mqiter % 2]));
+            last = mqiter;
+        }
+        if (c->h_flag[2 + last % 2]) *nql = c->h_flag[2 + last % 2];
+        return check_err(c);
+    }
     for (int m = 1; m <= mqiter; m++) {
         if (c->any_outlet) launch_vel_bc<double>(c->R, ld, c->us, c->vs, true, &c->sc->err, c->s);
-        // un, vn and the metrics are fixed during the iterations: the
-        // time-level-n terms of the first iteration serve them all
-        if (!c->any_porous) {
-            if (c->precision == WD2_PREC_MIXED)
-                momentum_pair<float>(c, dk, re, fr, c->Tf, c->chf, c->chf2, c->twf, m > 1);
-            else
-                momentum_pair<double>(c, dk, re, fr, c->T, c->ch, c->ch2, c->tw, m > 1);
-        } else {
-            int st = momentum(c, 1, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dus, m > 1);
-            if (st) return st;
-            st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs, m > 1);
-            if (st) return st;
-            launch_update_norms(nx, ny, ld, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, c->s);
-        }
+        int st = momentum(c, 1, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dus, m > 1);
+        if (st) return st;
+        st = momentum(c, 2, dk, re, fr, c->us, c->vs, c->un, c->vn, c->dvs, m > 1);
+        if (st) return st;
+        launch_update_norms(nx, ny, ld, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, c->s);
         double nv[2];
         HIPCHK(hipMemcpyAsync(nv, c->sc->val, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
         HIPCHK(hipStreamSynchronize(c->s));

@@ -29,14 +29,16 @@ struct Chains {
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
                  hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr);
-// Any layout (n rows in all).
+// Any layout (n rows in all).  stop (optional): every kernel returns at entry
+// while *stop != 0.
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
-                 T *X, hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr);
+                 T *X, hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr, const int *stop = nullptr);
 
 // Boundary fills (one workgroup).  err receives 1000+side for a bad flag.
 template <typename T>
-void launch_vel_bc(const Regions &R, int ld, T *u, T *v, bool outflow_only, int *err, hipStream_t s);
+void launch_vel_bc(const Regions &R, int ld, T *u, T *v, bool outflow_only, int *err, hipStream_t s,
+                   const int *stop = nullptr);   // stop: as for launch_trid
 template <typename T>
 void launch_pres_bc(const Regions &R, int ld, T *p, int *err, hipStream_t s);
 
@@ -48,7 +50,8 @@ struct Scalars {
     double sor_dif;
     int err;                       // bad-flag code from a kernel (0 = none)
     int redo_launch, redo_sweeps;  // temporally blocked SOR: launch to redo, sweeps to keep
-    int pad1[13];
+    int ql_stop;                   // q-l iteration that met qtol (0 = none yet)
+    int pad1[12];
 };
 
 constexpr int RED_MAX_BLOCKS = 4096;   // partial-maximum slots per reduction
@@ -176,7 +179,7 @@ template <typename T>
 void launch_mom_asm(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
                     const MomTempsT<T> &t, const double *us, const double *vs, const double *un, const double *vn,
                     const double *d, const double *dn, const ChainT<T> &ch1, const ChainT<T> &ch2, unsigned char *zm,
-                    hipStream_t s, int *err, int live);
+                    hipStream_t s, int *err, int live, const int *stop = nullptr);
 // live: bit 0, porous regions (load the Dupuit-Forchheimer coefficients);
 // bit 1, a density field (load d, dn); otherwise those values are +0.
 template <typename T>
@@ -189,10 +192,14 @@ void launch_scatter(int nx, int ny, int ld, int comp, const T *x, double *out, h
 void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const double *dus, const double *dvs,
                          double *partials, Scalars *sc, hipStream_t s);
 // launch_scatter of both components (x1: the X chain solution, x2: the Y
-// chain solution) followed by launch_update_norms, in one pass.
+// chain solution) followed by launch_update_norms, in one pass.  Both kernels
+// return at entry while sc->ql_stop != 0; the fold then sets sc->ql_stop = m
+// when max(val[0], val[1]) <= qtol (the q-l stopping test of iteration m).
 template <typename T>
 void launch_scatter_update(int nx, int ny, int ld, const T *x1, const T *x2, double *us, double *vs, double *dus,
-                           double *dvs, double *partials, Scalars *sc, hipStream_t s);
+                           double *dvs, double *partials, Scalars *sc, double qtol, int m, hipStream_t s);
+// sc->ql_stop = 0 (a kernel, not a memset).
+void launch_ql_reset(Scalars *sc, hipStream_t s);
 
 // Thermal energy (k_thermal.hip).  err receives 2000+side for a bad
 // temperature boundary flag.
