@@ -33,8 +33,8 @@ namespace wd2 {
 
 constexpr int KM = 8, MH = 2 * KM, MW = 96, MT = MW - 2 * MH;   // 64 owned per side
 constexpr int MX_T = 768;                                         // 12 wavefronts
-constexpr int MLD = MW + 1;                                       // padded LDS row of p
 constexpr int MK = MW / 2;                                        // pairs per tile row
+constexpr int PS = MK + 1, PR = 2 * PS;                           // LDS row of p: even, then odd columns
 constexpr int SR = 6;                                             // rows of a lane's strip
 constexpr int NB = MW / SR;                                       // strips per pair column
 // Row stride of the colour-split correction: 6 strips of rows apart (the
@@ -69,7 +69,10 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
                                                        int ntx, int ntiles, Scalars *sc, int force, double rau_c,
                                                        double rgv_c) {
     if (!force && sc->sor_done) return;
-    __shared__ double P[MW * MLD];
+    // p by column parity within a row: column 2k + h of tile row y is
+    // P[y*PR + h*PS + k], so the lanes of a row access consecutive doubles and
+    // a lane's pair is one two-address access
+    __shared__ double P[MW * PR];
     // the correction e split by colour: EC[c][y*ES + k] is the point of colour
     // c ((i+j) & 1) in pair k of tile row y
     __shared__ float EC[2][MW * ES];
@@ -139,17 +142,20 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
         g[q] = UNI ? make_double2(rgv_c, rgv_c) : load_pair(rgv, ld, nx, ny, i, j0 + y0 + q - 1);
 #pragma unroll
     for (int r = 0; r < SR; r++) {
-        const int c = (y0 + r) * MLD + x;
+        const int c = (y0 + r) * PR + k;
         P[c] = pr[r].x;
-        P[c + 1] = pr[r].y;
+        P[c + PS] = pr[r].y;
         EC[0][(y0 + r) * ES + k] = 0.f;
         EC[1][(y0 + r) * ES + k] = 0.f;
     }
     __syncthreads();
     // p at the strip's ends (the rows below and above), from LDS
     double2 pS = make_double2(0.0, 0.0), pN = make_double2(0.0, 0.0);
-    if (y0 > 0) pS = make_double2(P[(y0 - 1) * MLD + x], P[(y0 - 1) * MLD + x + 1]);
-    if (y0 + SR < MW) pN = make_double2(P[(y0 + SR) * MLD + x], P[(y0 + SR) * MLD + x + 1]);
+    if (y0 > 0) pS = make_double2(P[(y0 - 1) * PR + k], P[(y0 - 1) * PR + PS + k]);
+    if (y0 + SR < MW) pN = make_double2(P[(y0 + SR) * PR + k], P[(y0 + SR) * PR + PS + k]);
+    // the west neighbour of column x and the east neighbour of column x + 1,
+    // kept inside the tile at its outer ring (those points are not updated)
+    const int kw = k > 0 ? k - 1 : k, ke = k < MK - 1 ? k + 1 : k;
 
     // FP64 residual and the FP32 sweep coefficients, all scaled by 1/a3 in
     // FP32: the update is then four fused multiply-adds,
@@ -168,7 +174,7 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     }
 #pragma unroll
     for (int r = 0; r < SR; r++) {
-        const int c0 = (y0 + r) * MLD + x;
+        const int c0 = (y0 + r) * PR;
         const double ra[3] = {ram[r], rav[r].x, rav[r].y};
         const double2 ps = r > 0 ? pr[r - 1] : pS, pn = r < SR - 1 ? pr[r + 1] : pN;
 #pragma unroll
@@ -176,10 +182,10 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
             float rr = 0.f;
             if (UNI) {
                 // straight-line: every point is evaluated and a point that is
-                // not updated gets r = 0 by a select (the west index is kept
-                // inside the tile; what is read there is discarded)
-                const double pw = h ? pr[r].x : P[c0 - (x > 0 ? 1 : 0)];
-                const double pe = h ? P[c0 + 2] : pr[r].y;
+                // not updated gets r = 0 by a select (what the clamped west and
+                // east indices read at the ring is discarded)
+                const double pw = h ? pr[r].x : P[c0 + PS + kw];
+                const double pe = h ? P[c0 + ke] : pr[r].y;
                 double res = fma(-rgv_c, h ? ps.y : ps.x, h ? bvr[r].y : bvr[r].x);
                 res = fma(-rau_c, pw, res);
                 res = fma(-a3u, h ? pr[r].y : pr[r].x, res);
@@ -190,8 +196,9 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
                 const double a1 = h ? g[r].y : g[r].x, a5 = h ? g[r + 1].y : g[r + 1].x;
                 const double a2 = ra[h], a4 = ra[h + 1];
                 const double a3 = UNI ? a3u : -a4 - a2 - a5 - a1;
-                const double pw = h ? pr[r].x : P[c0 - 1];
-                const double pe = h ? P[c0 + 2] : pr[r].y;
+                // an updated point has x >= 2 (h = 0) or x + 1 <= MW - 2 (h = 1)
+                const double pw = h ? pr[r].x : P[c0 + PS + k - 1];
+                const double pe = h ? P[c0 + k + 1] : pr[r].y;
                 // the residual as one FMA chain (the defect only needs to be
                 // accurate, not the reference's rounding)
                 double res = fma(-a1, h ? ps.y : ps.x, h ? bvr[r].y : bvr[r].x);
@@ -274,7 +281,7 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
 #pragma unroll
         for (int h = 0; h < 2; h++)
             if ((own >> (2 * r + h)) & 1)
-                p_out[WD2_IX(i + h, j0 + y0 + r)] = P[(y0 + r) * MLD + x + h] + (double)e[r][h];
+                p_out[WD2_IX(i + h, j0 + y0 + r)] = P[(y0 + r) * PR + h * PS + k] + (double)e[r][h];
     if ((t & 63) == 0) sh[t >> 6] = below;   // the same in every lane of the wavefront
     __syncthreads();
     if (t == 0) {
