@@ -45,7 +45,7 @@ int wd2_trid_fallbacks(void);
  * sweep.  Both give the reference's iterates bit for bit.  Returns the
  * previous setting. */
 int wd2_set_tiled_sor(int enable);
-/* Lexicographic Sor (nPpeSolver 1) on grids of up to 1024 pressure rows runs
+/* Lexicographic Sor (nPpeSolver 1) on grids of up to 512 pressure rows runs
  * as one single-workgroup launch per solve, a grid row per lane (on by
  * default); 0 selects the tiled wavefront, one launch per sweep.  Both give
  * the reference's iterates bit for bit.  Returns the previous setting. */
@@ -106,9 +106,17 @@ int wd2_y_momentum(int nx, int ny, int ld, int mgri, int mgrj,
 int wd2_alt_trid_lu(int n, const double *a, double *b);
 
 /* pressure.f:Ppe 30-38.  lCartesGrid is a Fortran logical (non-zero = true).
- * nPpeSolver 1 sor, 2 lsor, 3 rb_lsor, 4 par_rb_lsor, 5 rb_sor, 6 par_rb_sor.
- * *nSorConv receives the iteration count (msorit when not converged, after the
- * reference's warning). */
+ * nPpeSolver 1 sor, 2 lsor, 3 rb_lsor, 4 par_rb_lsor, 5 rb_sor, 6 par_rb_sor
+ * (the reference's solvers, bit for bit), and 7 mg, which has no counterpart
+ * in the reference: multigrid V-cycles (red-black Gauss-Seidel smoothing on
+ * LDS tiles, cell-centred coarsening, FP64 or, with WD2_PREC_MIXED, FP32)
+ * inside FP64 defect correction on the same frozen-ghost system
+ * (pressure.f:97-106 with the cross terms of RhsPpe on non-Cartesian grids),
+ * until the divergence the projection leaves, max |dk (b - A p)| over the
+ * interior, is below sortol; msorit caps the V-cycles and sorrel is unused.
+ * Grids with blockages return WD2_ERR_UNSUPPORTED for solver 7.
+ * *nSorConv receives the iteration count (sweeps, or V-cycles for solver 7;
+ * msorit when not converged, after the reference's warning). */
 int wd2_ppe(int nx, int ny, int ld, int mgri, int mgrj,
             const int *nReg, const int *nRegBrd, const int *nRegType,
             int lCartesGrid, int nPpeSolver, int msorit, int *nSorConv,
@@ -266,14 +274,21 @@ int wd2_ctx_set_precision(wd2_ctx *ctx, int precision);
  * a profile of constant values gives the constant's results bit for bit.
  * values = NULL removes the profile.  Applies to this context's later steps. */
 int wd2_ctx_set_bc_profile(wd2_ctx *ctx, int ireg, int jreg, int side, int var, const double *values, int n);
-/* Timing of the red-black SOR kernels with HIP events on the context's
- * stream.  enable=1 starts recording: one event pair around each batch of
- * back-to-back launches of the tiled multi-sweep kernel, or around each
- * colour kernel of the untiled path.  enable=0 stops and returns the mean
+/* Timing of one Poisson kernel with HIP events on the context's stream.
+ * enable=1 starts recording the red-black SOR kernels: one event pair around
+ * each batch of back-to-back launches of the tiled multi-sweep kernel, or
+ * around each colour kernel of the untiled path (launches after a solve has
+ * converged exit early and are counted too).  enable=2 / 3 records the
+ * multigrid solver's level-0 down / up kernel, one pair per launch, keeping
+ * only the launches that ran in full.  enable=0 stops and returns the mean
  * launch duration (microseconds: the bracketed time over the launches it
- * holds) and the launch count.  Launches after a solve has converged exit
- * early and are counted too. */
+ * holds) and the launch count. */
 int wd2_ctx_kernel_timing(wd2_ctx *ctx, int enable, double *avg_us, long *launches);
+/* The multigrid solver's (nPpeSolver 7) stopping measure: max |dk (b - A p)|
+ * over the interior when the last solve stopped (*last) and the largest such
+ * value since the last reset (*max_since); reset != 0 then clears the
+ * maximum.  No counterpart in the reference. */
+int wd2_ctx_ppe_divergence(wd2_ctx *ctx, double *last, double *max_since, int reset);
 
 #ifdef __cplusplus
 }

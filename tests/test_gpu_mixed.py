@@ -141,3 +141,20 @@ def test_mixed_full_size_cavity_steps():
         e = rel_l2(a[S], b[S])
         print(f"  {name}: rel L2 {e:.3e}", flush=True)
         assert e <= TOL_L2, name
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nonfinite_rhs_is_not_converged(precision):
+    """A NaN in the Poisson right-hand side never counts as converged: the
+    solve runs to msorit and reports it (pressure.f:242-246), in both
+    precisions (the stopping test compares !(|s| < sortol))."""
+    g, reg, prm = _cavity(64, 100.0, 1e-3, msorit=40)
+    u = np.zeros((g.ncol, g.ld))
+    u[30, 30] = np.nan
+    z = np.zeros((g.ncol, g.ld))
+    sol = api.Solver(g, reg, prm)
+    sol.set_precision(precision)
+    sol.set_fields(u, z, z)
+    n = sol.cold_start()
+    sol.close()
+    assert n == prm.msorit

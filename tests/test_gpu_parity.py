@@ -304,7 +304,87 @@ def test_stretched_cartesian_grid_tiled_sor_vs_oracle(oracle):
     sol.close()
     u, v, p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
     oracle.cold_start(reg, g, prm, u, v, p)
-    to = oracle.run(reg, g, prm, u, v, p, 3)
+    to = oracle.run(reg, g, prm, u, v, p, 2)
+    assert np.all(np.isfinite(to)) and np.all(np.abs(u) < 10.0), "the case must stay bounded to compare"
+    assert np.array_equal(tg, to)
+    for a, b in zip(fg, (u, v, p)):
+        assert np.array_equal(a, b)
+
+
+def _stretched_case(nx=161, ny=129, **kw):
+    s = np.arange(nx) / (nx - 1.0)
+    t = np.arange(ny) / (ny - 1.0)
+    xs = s + 0.12 * np.sin(np.pi * s) * s
+    yt = t - 0.08 * np.sin(2 * np.pi * t)
+    x = np.broadcast_to(xs[None, :], (ny, nx)).copy()
+    y = np.broadcast_to(yt[:, None], (ny, nx)).copy()
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(nx, ny)
+    prm = case.Params.from_physical(5e-4, 1.0, 1e-2, cartesian=True, **kw)
+    return g, reg, prm
+
+
+def test_stretched_cartesian_grid_mixed_follows_fp64():
+    """The mixed kernel's per-point coefficient path (rau, rgv loaded per
+    point, distinct a1/a5 and a2/a4 at every point of a stretched grid): a
+    swapped or shifted coefficient changes the fixed point, which the FP64
+    path (bit-exact against the oracle above) pins.  Bar: sweep counts within
+    one of FP64's, fields within 1e-6 relative L2."""
+    g, reg, prm = _stretched_case(nPpeSolver=5, msorit=200, sorrel=1.7)
+    print("stretched Cartesian grid, mixed vs fp64", flush=True)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        sol = api.Solver(g, reg, prm)
+        sol.set_precision(prec)
+        z = np.zeros((g.ncol, g.ld))
+        sol.set_fields(z, z, z)
+        sol.cold_start()
+        res[prec] = (sol.step(3), sol.get_fields())
+        sol.close()
+    (t64, f64), (tmx, fmx) = res["fp64"], res["mixed"]
+    assert np.array_equal(t64[:, 0], tmx[:, 0])
+    assert np.all(np.abs(t64[:, 1] - tmx[:, 1]) <= 1)
+    S = np.s_[: g.ny + 2, : g.nx + 2]
+    for name, a, b in zip("uvp", fmx, f64):
+        e = np.linalg.norm(a[S] - b[S]) / np.linalg.norm(b[S])
+        print(f"  {name}: rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-6, name
+
+
+@pytest.mark.parametrize("grid", ["uniform", "skewed"])
+def test_lex_rows_several_wavefronts_bitwise_vs_oracle(oracle, grid):
+    """Lexicographic Sor as one single-workgroup launch with 300 pressure rows
+    (five wavefronts: the cross-wavefront ring hand-off, the skew between
+    wavefronts and the barriers between them), on a uniform Cartesian grid
+    and on a skewed non-Cartesian one (RhsPpe inside the sweeps)."""
+    nx, ny = 161, 301
+    if grid == "uniform":
+        x, y = case.uniform_nodes(nx, ny, lx=0.6)
+        cart = True
+    else:
+        x, y = case.skewed_nodes(nx, ny, seed=5)
+        x = 0.55 * x   # cells about as wide as they are high
+        cart = False
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(nx, ny)
+    prm = case.Params.from_physical(2e-4, 1.0, 1e-2, nPpeSolver=1, msorit=1500, sorrel=1.8, cartesian=cart)
+    print(f"lexicographic Sor, {ny - 1} rows, {grid}", flush=True)
+    from wolfd2_amd._lib import lib
+    old = lib().wd2_set_lex_rows(1)
+    try:
+        sol = api.Solver(g, reg, prm)
+        z = np.zeros((g.ncol, g.ld))
+        sol.set_fields(z, z, z)
+        sol.cold_start()
+        tg = sol.step(2)
+        fg = sol.get_fields()
+        sol.close()
+    finally:
+        lib().wd2_set_lex_rows(old)
+    u, v, p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
+    oracle.cold_start(reg, g, prm, u, v, p)
+    to = oracle.run(reg, g, prm, u, v, p, 2)
+    assert np.all(np.isfinite(to)) and np.all(np.abs(u) < 10.0), "the case must stay bounded to compare"
     assert np.array_equal(tg, to)
     for a, b in zip(fg, (u, v, p)):
         assert np.array_equal(a, b)

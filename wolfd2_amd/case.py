@@ -525,4 +525,6 @@ class StepOptions:
     rconst: float = 0.0
 
 
-PPE_SOLVERS = {"sor": 1, "lsor": 2, "rb_lsor": 3, "par_rb_lsor": 4, "rb_sor": 5, "par_rb_sor": 6}
+# the ppe_solver keywords (parse.f:446-466); "mg" (7) is the multigrid solver of this path,
+# which the reference does not have
+PPE_SOLVERS = {"sor": 1, "lsor": 2, "rb_lsor": 3, "par_rb_lsor": 4, "rb_sor": 5, "par_rb_sor": 6, "mg": 7}

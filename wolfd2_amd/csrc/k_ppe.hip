@@ -216,6 +216,11 @@ constexpr int TP_T = 512;
 constexpr int TP_PAIRS = TW * TW / 2 / TP_T;   // 4
 constexpr int TK = TW / 2;                     // pairs per tile row
 
+// RCP: every diagonal a3 of the launch is a power of two (the uniform-grid
+// value, and -1 at points that are not updated), so x / a3 == x * (1 / a3)
+// bit for bit (both are the correctly rounded value of the same real, and
+// 1 / a3 is exact): the sweeps multiply by the reciprocal instead of dividing.
+template <bool RCP>
 __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, const double *__restrict__ rau,
                                                     const double *__restrict__ rgv, const double *__restrict__ b,
                                                     double sorrel, int nsweep, const double *__restrict__ p_in,
@@ -309,6 +314,7 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
             const double a5 = u ? (h ? g1.y : g1.x) : 1.0;
             cb[r][c] = bb; c1[r][c] = a1; c2[r][c] = a2; c4[r][c] = a4; c5[r][c] = a5;
             c3[r][c] = -a4 - a2 - a5 - a1;
+            if (RCP) c3[r][c] = 1.0 / c3[r][c];
             ow[r][c] = h ? 0 : (kk > 0 ? -1 : 0);
             oe[r][c] = h ? (kk < TK - 1 ? 1 : 0) : 0;
             up[r][c] = u;
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(TP_T) void k_sor_tiled(int nx, int ny, int ld, cons
                 // the reference's order: b - a1 pS - a2 pW - a4 pE - a5 pN
                 double sum = cb[r][color] - c1[r][color] * src[q + os[r]] - c2[r][color] * src[q + ow[r][color]] -
                              c4[r][color] * src[q + oe[r][color]] - c5[r][color] * src[q + on[r]];
-                sum = sum / c3[r][color] - pij;
+                sum = (RCP ? sum * c3[r][color] : sum / c3[r][color]) - pij;
                 const double pn = pij + sorrel * sum;
                 dst[q] = up[r][color] ? pn : pij;
                 big |= __ballot(!(fabs(sum) < sortol)) & ownm[r][color];
@@ -396,9 +402,22 @@ void launch_sor_tiled(int nx, int ny, int ld, const Metrics &m, const double *b,
     int ntx = 0;
     const int nt = sor_tiled_tiles(nx, ny, &ntx);
     const int grid = 8 * ((nt + 7) / 8);
-    hipLaunchKernelGGL(k_sor_tiled, dim3(grid), dim3(TP_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel, nsweep, p_in,
-                       p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit, ntx, nt, sc, force,
-                       uni ? 1 : 0, uni ? uni[0] : 0.0, uni ? uni[1] : 0.0);
+    // the diagonal of an updated point on the uniform path, in the kernel's
+    // order of operations (-a4 - a2 - a5 - a1); non-updated points have -1
+    bool rcp = false;
+    if (uni) {
+        const double a3 = -uni[0] - uni[0] - uni[1] - uni[1];
+        int e2 = 0;
+        rcp = std::isfinite(a3) && a3 != 0.0 && std::frexp(a3, &e2) == -0.5;
+    }
+    if (rcp)
+        hipLaunchKernelGGL(k_sor_tiled<true>, dim3(grid), dim3(TP_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel, nsweep,
+                           p_in, p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit, ntx, nt,
+                           sc, force, 1, uni[0], uni[1]);
+    else
+        hipLaunchKernelGGL(k_sor_tiled<false>, dim3(grid), dim3(TP_T), 0, s, nx, ny, ld, m.rau, m.rgv, b, sorrel,
+                           nsweep, p_in, p_out, masks, prev_masks, prev_nsweep, prev_m0, prev_launch, sortol, msorit,
+                           ntx, nt, sc, force, uni ? 1 : 0, uni ? uni[0] : 0.0, uni ? uni[1] : 0.0);
 }
 
 void launch_sor_tiled_check(int ntiles, int nsweep, int m0, int launch, int msorit, const unsigned *masks,

@@ -232,8 +232,10 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
     // (fmaf(0, s, +0) is +0 for finite s).  The neighbour indices that would
     // leave the tile at its outer ring are clamped to the lane's own slot, so
     // every read is of a written (finite) slot.  The stopping test flags an
-    // owned point with |s| >= sortol, which is max |s| >= sortol over the
-    // owned points; it is gathered per wavefront in scalar lane masks.
+    // owned point whose |s| is not below sortol (a NaN counts as not
+    // converged, as in the FP64 kernel), which is max |s| >= sortol over the
+    // owned points otherwise; it is gathered per wavefront in scalar lane
+    // masks.
     float wm[SR][2];
     unsigned long long owm[SR][2];
 #pragma unroll
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(MX_T) void k_sor_tiled_mx(int nx, int ny, int ld, c
                 const float sum = acc - e[r][h];
                 e[r][h] = fmaf(wm[r][h], sum, e[r][h]);
                 dst[q] = e[r][h];
-                big |= __ballot(fabsf(sum) >= tol) & owm[r][h];
+                big |= __ballot(!(fabsf(sum) < tol)) & owm[r][h];
             }
             __syncthreads();
         }

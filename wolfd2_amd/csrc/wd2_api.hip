@@ -117,17 +117,30 @@ struct wd2_ctx {
     // SOR reads (a uniform Cartesian grid); uni_ok false otherwise
     bool uni_ok = false;
     double uni[2] = {0.0, 0.0};
-    // optional HIP-event timing of the red-black SOR kernels: event pairs,
+    // multigrid Poisson solver (nPpeSolver 7): its levels and their storage,
+    // the per-tile residual maxima, the precision and metric version its
+    // coefficients were built for, the last solve's V-cycle count (the size
+    // of the next solve's first batch), and max |dk (b - A p)| at the end of
+    // the last solve and its maximum since the last query
+    MgHierarchy mg{};
+    double *mg_store = nullptr;
+    size_t mg_bytes = 0;
+    double *mg_part = nullptr;
+    int mg_prec = -1, mg_ver = -1, mg_last = 4;
+    double mg_div = 0.0, mg_div_max = 0.0;
+    // optional HIP-event timing of one kernel (kt_on: 1 the red-black SOR
+    // kernel, 2 / 3 the multigrid level-0 down / up kernel): event pairs,
     // each around kt_cnt[pair] back-to-back launches
-    bool kt_on = false;
+    int kt_on = 0;
     std::vector<hipEvent_t> kt_ev;
     std::vector<int> kt_cnt;
     size_t kt_used = 0;
 };
 
-// npairs new event pairs for the kernel timing (nullptr when it is off)
-static hipEvent_t *kt_events(wd2_ctx *c, int npairs) {
-    if (!c->kt_on) return nullptr;
+// npairs new event pairs for the timing of kernel `which` (nullptr when the
+// timing is off or times another kernel)
+static hipEvent_t *kt_events(wd2_ctx *c, int npairs, int which = 1) {
+    if (c->kt_on != which) return nullptr;
     while (c->kt_ev.size() < c->kt_used + 2 * npairs) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
@@ -165,6 +178,8 @@ static void ctx_free(wd2_ctx *c) {
     (void)hipFree(c->partials);
     (void)hipFree(c->sc);
     (void)hipFree(c->tile_flags);
+    (void)hipFree(c->mg_store);
+    (void)hipFree(c->mg_part);
     (void)hipFree(c->tw.cp);
     (void)hipFree(c->zm);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -637,11 +652,135 @@ static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr
 }
 
 // ------------------------------------------------------------------- Ppe
+// Levels of the multigrid solver: level 0 is the (nx-1) x (ny-1) pressure
+// grid, each next level ceil-halves both sides down to at most 2 x 2 cells;
+// the levels at most mg_bottom_side() cells wide go to the bottom kernel.
+static int mg_setup(wd2_ctx *c) {
+    if (c->mg_store) return WD2_OK;
+    MgHierarchy &H = c->mg;
+    int NX = c->nx - 1, NY = c->ny - 1;
+    size_t off[MG_MAXLEV], total = 0;
+    H.nlev = 0;
+    for (;;) {
+        if (H.nlev >= MG_MAXLEV) return fail(WD2_ERR_ARG, "multigrid: too many levels for %dx%d", c->nx, c->ny);
+        MgLevel &L = H.lev[H.nlev];
+        L.nx = NX;
+        L.ny = NY;
+        L.ld = round_up(NX + 2, 16);
+        off[H.nlev] = total;
+        total += 5 * (size_t)L.ld * (NY + 2);
+        H.nlev++;
+        if (std::max(NX, NY) <= 2) break;
+        NX = (NX + 1) / 2;
+        NY = (NY + 1) / 2;
+    }
+    H.ntiled = 1;
+    while (H.ntiled < H.nlev && std::max(H.lev[H.ntiled].nx, H.lev[H.ntiled].ny) > mg_bottom_side()) H.ntiled++;
+    size_t cells = 0;
+    for (int l = H.ntiled; l < H.nlev; l++) cells += (size_t)(H.lev[l].nx + 2) * (H.lev[l].ny + 2);
+    if (H.ntiled >= H.nlev || cells > (size_t)mg_bottom_cells())
+        return fail(WD2_ERR_ARG, "multigrid: bottom levels do not fit (%zu cells)", cells);
+    HIPCHK(hipMalloc(&c->mg_store, total * sizeof(double)));
+    c->mg_bytes = total * sizeof(double);
+    HIPCHK(hipMemsetAsync(c->mg_store, 0, c->mg_bytes, c->s));
+    for (int l = 0; l < H.nlev; l++) {
+        MgLevel &L = H.lev[l];
+        const size_t n = (size_t)L.ld * (L.ny + 2);
+        double *b = c->mg_store + off[l];
+        L.e = b;
+        L.r = b + n;
+        L.ax = b + 2 * n;
+        L.ay = b + 3 * n;
+        L.c = b + 4 * n;
+    }
+    const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, nullptr);
+    HIPCHK(hipMalloc(&c->mg_part, nt * sizeof(double)));
+    return WD2_OK;
+}
+
+// nPpeSolver 7: multigrid V-cycles (FP64 or FP32 by the context's precision)
+// in FP64 defect correction, until max |dk (b - A p)| < tol (the divergence
+// Project leaves) or maxcyc cycles; *nconv_out = the V-cycles applied.
+static int ppe_mg(wd2_ctx *c, int cart, int maxcyc, double dk, double tol, double *p, int *nconv_out) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    if (c->any_blockage)
+        return fail(WD2_ERR_UNSUPPORTED, "the multigrid Poisson solver (nPpeSolver 7) does not take blockages");
+    int st = mg_setup(c);
+    if (st) return st;
+    const bool mx = c->precision == WD2_PREC_MIXED;
+    const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
+    if (c->mg_prec != c->precision || c->mg_ver != c->met_version) {
+        // float and double views share the storage: clear it (zero rings)
+        if (c->mg_prec != -1 && c->mg_prec != c->precision) HIPCHK(hipMemsetAsync(c->mg_store, 0, c->mg_bytes, s));
+        if (mx)
+            launch_mg_coef<float>(c->mg, nx, ny, ld, c->M.rau, c->M.rgv, uni, s);
+        else
+            launch_mg_coef<double>(c->mg, nx, ny, ld, c->M.rau, c->M.rgv, uni, s);
+        c->mg_prec = c->precision;
+        c->mg_ver = c->met_version;
+    }
+    launch_mg_reset(c->sc, s);
+    const double *b = cart ? c->rhs : c->beff;
+    const size_t kt0 = c->kt_used;
+    // Batches of V-cycles with a stopping check at the end of each, two in
+    // flight (as the SOR solves): the first batch is the last solve's cycle
+    // count plus the final check, so a steady run rarely overruns
+    int cyc = 0, issued = 0, checked = 0, stop = 0;
+    const int total = maxcyc + 1;   // the last cycle only tests
+    while (!stop) {
+        if (cyc < total && issued - checked < 2) {
+            const int Q = issued == 0 ? std::max(1, c->mg_last + 1) : std::min(16, 2 << std::min(issued, 3));
+            for (int q = 0; q < Q && cyc < total; q++) {
+                cyc++;
+                if (!cart) launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s);
+                hipEvent_t *ed = kt_events(c, 1, 2), *eu = kt_events(c, 1, 3);
+                if (mx)
+                    launch_mg_cycle<float>(c->mg, nx, ny, ld, b, p, c->M.rau, c->M.rgv, uni, dk, tol, cyc, maxcyc,
+                                           c->mg_part, c->sc, s, ed, eu);
+                else
+                    launch_mg_cycle<double>(c->mg, nx, ny, ld, b, p, c->M.rau, c->M.rgv, uni, dk, tol, cyc, maxcyc,
+                                            c->mg_part, c->sc, s, ed, eu);
+            }
+            HIPCHK(hipMemcpyAsync(&c->h_flag[issued % 2], &c->sc->mg_done, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipEventRecord(c->bev[issued % 2], s));
+            issued++;
+            continue;
+        }
+        if (checked == issued) break;
+        HIPCHK(hipEventSynchronize(c->bev[checked % 2]));
+        stop = c->h_flag[checked % 2];
+        checked++;
+    }
+    Scalars h;
+    HIPCHK(hipMemcpy(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    int nconv = h.mg_nconv;
+    c->mg_div = h.val[7];
+    if (!(c->mg_div <= c->mg_div_max)) c->mg_div_max = c->mg_div;
+    // the timing keeps the launches that ran in full: the down kernel of every
+    // applied cycle and of the final test, the up kernel of the applied ones
+    if (c->kt_on == 2 || c->kt_on == 3) {
+        const int applied = nconv < 0 ? maxcyc : nconv;
+        const size_t keep = kt0 + 2 * (size_t)(c->kt_on == 2 ? applied + 1 : applied);
+        if (c->kt_used > keep) {
+            c->kt_used = keep;
+            c->kt_cnt.resize(keep / 2);
+        }
+    }
+    if (nconv < 0) {
+        if (g_verbose) fprintf(stdout, " Warning: multigrid did not converge after %6d V-cycles.\n", maxcyc);
+        nconv = maxcyc;
+    }
+    c->mg_last = std::min(nconv, 64);
+    *nconv_out = nconv;
+    return WD2_OK;
+}
+
 static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double sortol, double sorrel,
                const double *u, const double *v, double *p, int *nconv_out) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     hipStream_t s = c->s;
-    if (solver < 1 || solver > 6) {
+    if (solver < 1 || solver > 7) {
         // pressure.f:234-236 prints and leaves p untouched; Ppe then reports msorit
         if (g_verbose) fprintf(stdout, " Wrong nPpeSolver flag passed to Ppe\n");
         *nconv_out = msorit;
@@ -649,6 +788,7 @@ static int ppe(wd2_ctx *c, int cart, int solver, int msorit, double dk, double s
     }
     launch_divergence(nx, ny, ld, 1, c->M, u, v, c->div, s);
     launch_ppe_rhs(nx, ny, ld, dk, c->div, c->any_blockage ? c->prow : nullptr, c->rhs, s);
+    if (solver == 7) return ppe_mg(c, cart, msorit, dk, sortol, p, nconv_out);
     const PpeCoef cf{c->pc[0], c->pc[1], c->pc[2], c->pc[3], c->pc[4]};
     const PpeCoef *coef = nullptr;
     if (c->any_blockage || solver == 2 || solver == 3 || solver == 4) {
@@ -1473,9 +1613,17 @@ int wd2_ctx_set_bc_profile(wd2_ctx *c, int ireg, int jreg, int side, int var, co
     return WD2_OK;
 }
 
+int wd2_ctx_ppe_divergence(wd2_ctx *c, double *last, double *max_since, int reset) {
+    if (last) *last = c->mg_div;
+    if (max_since) *max_since = c->mg_div_max;
+    if (reset) c->mg_div_max = 0.0;
+    return WD2_OK;
+}
+
 int wd2_ctx_kernel_timing(wd2_ctx *c, int enable, double *avg_us, long *launches) {
     if (enable) {
-        c->kt_on = true;
+        if (enable < 1 || enable > 3) return fail(WD2_ERR_ARG, "kernel timing selector %d (1..3)", enable);
+        c->kt_on = enable;
         c->kt_used = 0;
         c->kt_cnt.clear();
         return WD2_OK;
@@ -1489,7 +1637,7 @@ int wd2_ctx_kernel_timing(wd2_ctx *c, int enable, double *avg_us, long *launches
         tot += ms;
         n += c->kt_cnt[k / 2];
     }
-    c->kt_on = false;
+    c->kt_on = 0;
     c->kt_used = 0;
     c->kt_cnt.clear();
     if (avg_us) *avg_us = n ? 1e3 * tot / n : 0.0;

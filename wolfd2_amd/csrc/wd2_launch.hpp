@@ -51,7 +51,10 @@ struct Scalars {
     int err;                       // bad-flag code from a kernel (0 = none)
     int redo_launch, redo_sweeps;  // temporally blocked SOR: launch to redo, sweeps to keep
     int ql_stop;                   // q-l iteration that met qtol (0 = none yet)
-    int pad1[12];
+    int mg_done;                   // multigrid solve stopped
+    int mg_nconv;                  // V-cycles applied when it converged, -1 not converged
+    int mg_cycles;                 // V-cycles applied so far
+    int pad1[9];
 };
 
 constexpr int RED_MAX_BLOCKS = 4096;   // partial-maximum slots per reduction
@@ -215,5 +218,37 @@ void launch_eq_state(int nx, int ny, int ld, double uref, double densref, double
                      const double *p, const double *t, double *den, hipStream_t s);
 void launch_filter(const Regions &R, const Thermal &TH, int ld, int ncomp, double fp, double *qu, double *qh,
                    hipStream_t s);
+
+// Multigrid Poisson solver (k_mg.hip).  A level of nx x ny cells: arrays of
+// the arithmetic type (float or double) with row stride ld; on cells with a
+// zero ring e (the correction after pre-smoothing), r (right-hand side) and
+// c (the correction after the up pass, which the next finer level
+// prolongates: a separate array, since an up launch reads e in its halos
+// while it writes); ax / ay its x- and y-face coefficients.  Levels
+// 0..ntiled-1 run as LDS tiles, the rest in the one-workgroup bottom kernel.
+constexpr int MG_MAXLEV = 16;
+struct MgLevel {
+    int nx, ny, ld;
+    void *e, *r, *ax, *ay, *c;
+};
+struct MgHierarchy {
+    int nlev, ntiled;
+    MgLevel lev[MG_MAXLEV];
+};
+int mg_tiles(int NX, int NY, int *ntx);
+int mg_bottom_side();     // widest level the bottom kernel takes
+int mg_bottom_cells();    // LDS cells per array of the bottom kernel
+int mg_sweeps();          // smoothing sweeps before and after
+template <typename T>
+void launch_mg_coef(const MgHierarchy &H, int nx, int ny, int ld, const double *rau, const double *rgv,
+                    const double *uni, hipStream_t s);
+// One V-cycle of defect correction on b (1-based cycle count); part: one
+// double per level-0 tile.  ev_down / ev_up (optional): an event pair around
+// the level-0 down / up kernel.
+template <typename T>
+void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double *b, double *p, const double *rau,
+                     const double *rgv, const double *uni, double dk, double tol, int cycle, int maxcyc,
+                     double *part, Scalars *sc, hipStream_t s, hipEvent_t *ev_down, hipEvent_t *ev_up);
+void launch_mg_reset(Scalars *sc, hipStream_t s);
 
 }  // namespace wd2
