@@ -175,10 +175,18 @@ def run_path(sol, precision: str, solver: str, fields0, warmup: int, steps: int,
         # level-0 up: read e, r, the coarse correction (s / 4) and p, write p
         bpl = (16.0 + 2.25 * s) * cells
         t = "float" if precision == "mixed" else "double"
+        reps = 50
         for which, name in ((2, "down"), (3, "up")):
-            us, nl = _kernel_time(sol, which, steps)
+            # back-to-back launches on the last solve's state (an event pair
+            # around a single launch inside a V-cycle also holds the gap
+            # between launches); the in-cycle figure is kept beside it
+            us = C.c_double(0)
+            L.wd2_ctx_mg_kernel_time(sol.ctx, which, reps, C.byref(us))
+            us_cycle, nl = _kernel_time(sol, which, 2)
             kern = f"k_mg_{name}<{t}, level 0> (2 red-black Gauss-Seidel sweeps on LDS tiles, FP64 pressure)"
-            roof[name] = _roofline(kern, bpl, us, nl, _traffic(f"k_mg_{name}_{precision}_cavity1024_traffic.json"))
+            roof[name] = _roofline(kern, bpl, us.value, reps,
+                                   _traffic(f"k_mg_{name}_{precision}_cavity1024_traffic.json"))
+            roof[name]["avg_launch_us_in_cycles"] = round(us_cycle, 3)
     else:
         us, nl = _kernel_time(sol, 1, steps)
         if precision == "mixed":

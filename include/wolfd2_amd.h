@@ -289,6 +289,12 @@ int wd2_ctx_kernel_timing(wd2_ctx *ctx, int enable, double *avg_us, long *launch
  * value since the last reset (*max_since); reset != 0 then clears the
  * maximum.  No counterpart in the reference. */
 int wd2_ctx_ppe_divergence(wd2_ctx *ctx, double *last, double *max_since, int reset);
+/* Mean duration (microseconds) of the multigrid solver's level-0 down
+ * (which = 2) or up (which = 3) kernel over `reps` back-to-back launches on
+ * the context's last solve (event pairs around single launches in the
+ * solve's own cycles include the gap between launches); p is restored
+ * afterwards.  No counterpart in the reference. */
+int wd2_ctx_mg_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
 
 #ifdef __cplusplus
 }

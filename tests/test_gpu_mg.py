@@ -172,7 +172,7 @@ def test_mg_cycle_count_is_grid_independent(oracle, n):
 def test_mg_tier_b_run_matches_reference_program():
     """The reference program's tier-B 64^2 cavity (q-l 1e-13, red-black SOR
     to 1e-13 with sorrel 1.9, 200 steps), rerun with multigrid Poisson solves
-    (divergence tolerance 1e-18).  Both runs are converged solves of the same
+    (divergence tolerance 1e-18), FP64.  Both runs are converged solves of the same
     discrete equations: u and v agree to 1e-11 relative L2, and p does after
     removing its mean.  The mean of p is the system's weakest mode (the
     frozen ghosts pin the level only through the boundary coupling), which
@@ -186,7 +186,9 @@ def test_mg_tier_b_run_matches_reference_program():
     prm.sortol = 1e-18
     S = np.s_[: g.ny + 2, : g.nx + 2]
     I = np.s_[2:g.ny + 1, 2:g.nx + 1]
-    for precision in ("fp64", "mixed"):
+    # FP64 only: FP32 momentum increments cannot meet a q-l tolerance of
+    # 1e-13, so the mixed path runs every step to max_ql_iter there
+    for precision in ("fp64",):
         sol = api.Solver(g, reg, prm)
         sol.set_precision(precision)
         z = np.zeros((g.ncol, g.ld))

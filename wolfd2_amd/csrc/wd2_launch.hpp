@@ -250,5 +250,11 @@ void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double 
                      const double *rgv, const double *uni, double dk, double tol, int cycle, int maxcyc,
                      double *part, Scalars *sc, hipStream_t s, hipEvent_t *ev_down, hipEvent_t *ev_up);
 void launch_mg_reset(Scalars *sc, hipStream_t s);
+// reps back-to-back launches of the level-0 down (which = 2) or up (3) kernel
+// between events e0 and e1 (clears the stop flag; the up kernel changes p).
+template <typename T>
+void launch_mg_kernel_bench(const MgHierarchy &H, int which, int reps, int nx, int ny, int ld, const double *b,
+                            double *p, const double *rau, const double *rgv, const double *uni, double *part,
+                            Scalars *sc, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 
 }  // namespace wd2
