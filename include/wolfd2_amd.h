@@ -105,6 +105,19 @@ int wd2_y_momentum(int nx, int ny, int ld, int mgri, int mgrj,
  * b the right-hand side, overwritten by the solution.  a is not modified. */
 int wd2_alt_trid_lu(int n, const double *a, double *b);
 
+/* No counterpart in the reference: AltTridLU (momentum.f:1307-1339, the same
+ * a(3,n) and b(n) arguments and the same bits as wd2_alt_trid_lu) for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  a and b are DEVICE
+ * pointers; b is overwritten by the solution.  The work is queued on `stream`
+ * (a hipStream_t; NULL = the default stream) and the call returns without
+ * synchronising.  It calls no allocating routine: the solver's scratch is a
+ * static device array of each GPU (held until the process ends), so
+ * n <= WD2_TRID_NOALLOC_MAX_ROWS (larger n returns WD2_ERR_ARG), and calls
+ * that overlap in time on one GPU race on that array: serialise them. */
+#define WD2_TRID_NOALLOC_MAX_ROWS 266240
+int wd2_alt_trid_lu_noalloc(int n, const double *a, double *b, void *stream);
+
 /* pressure.f:Ppe 30-38.  lCartesGrid is a Fortran logical (non-zero = true).
  * nPpeSolver 1 sor, 2 lsor, 3 rb_lsor, 4 par_rb_lsor, 5 rb_sor, 6 par_rb_sor
  * (the reference's solvers, bit for bit), and 7 mg, which has no counterpart

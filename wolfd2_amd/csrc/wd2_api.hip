@@ -1036,6 +1036,11 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
     return check_err(c);
 }
 
+namespace wd2 {
+int api_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+int noalloc_trid_fallbacks();
+}  // namespace wd2
+
 static int fallbacks(wd2_ctx *c) {
     int n = 0;
     if (hipMemcpy(&n, c->tw.nfallback, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return 0;
@@ -1110,7 +1115,7 @@ int wd2_set_tiled_sor(int enable) {
     return old;
 }
 
-int wd2_trid_fallbacks(void) { return g_fallbacks + (g_ctx ? fallbacks(g_ctx) : 0); }
+int wd2_trid_fallbacks(void) { return g_fallbacks + (g_ctx ? fallbacks(g_ctx) : 0) + noalloc_trid_fallbacks(); }
 
 int wd2_vel_bound_cond(int nx, int ny, int ld, int mgri, int mgrj, const int *nReg, const int *nRegBrd,
                        const int *nMomBdTp, const double *dBCVal, double *u, double *v) {
