@@ -156,6 +156,8 @@ def run_path(sol, precision: str, solver: str, fields0, warmup: int, steps: int,
     L = lib()
     sol.set_precision(precision)
     sol.set_fields(*fields0)
+    tune = sol.mg_autotune(5) if solver == "mg" else None   # kernel shapes for this grid and precision
+    sol.set_fields(*fields0)
     if warmup:
         sol.step(warmup)
     warm = sol.get_fields()
@@ -207,7 +209,7 @@ def run_path(sol, precision: str, solver: str, fields0, warmup: int, steps: int,
         roof["sor"] = _roofline(kern, bpl, us, nl, traffic)
     return {"value": cells * steps / secs / 1e6, "ms_per_step": 1e3 * secs / steps, "fields": fields,
             "warm": warm, "ql": float(np.mean(tab[:, 0])), "ppe": float(np.mean(tab[:, 1])),
-            "div_max": div_max.value if solver == "mg" else None,
+            "div_max": div_max.value if solver == "mg" else None, "mg_tune": tune,
             "trid_fallbacks": int(L.wd2_trid_fallbacks() - fb0), "roofline": roof}
 
 
@@ -351,7 +353,8 @@ def main() -> int:
                    "divergence_max": {"mixed": mx["div_max"], "fp64": f64["div_max"], "tolerance": 1e-10},
                    "ql_iter_mean": mx["ql"], "vcycles_mean": mx["ppe"],
                    "fp64_ql_iter_mean": f64["ql"], "fp64_vcycles_mean": f64["ppe"],
-                   "trid_sequential_fallbacks": {"mixed": mx["trid_fallbacks"], "fp64": f64["trid_fallbacks"]}},
+                   "trid_sequential_fallbacks": {"mixed": mx["trid_fallbacks"], "fp64": f64["trid_fallbacks"]},
+                   "mg_autotuned_shape": {"mixed": mx["mg_tune"], "fp64": f64["mg_tune"]}},
         "roofline": mx["roofline"]["down"],
         "roofline_up": mx["roofline"]["up"],
         "fp64_roofline": f64["roofline"]["down"],

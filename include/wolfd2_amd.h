@@ -308,6 +308,44 @@ int wd2_ctx_ppe_divergence(wd2_ctx *ctx, double *last, double *max_since, int re
  * solve's own cycles include the gap between launches); p is restored
  * afterwards.  No counterpart in the reference. */
 int wd2_ctx_mg_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
+/* Kernel shapes of the multigrid solver (no counterpart in the reference;
+ * none changes the arithmetic, so results are the same bits): nt0 threads
+ * per level-0 tile workgroup (256 or 512), stage 1 to form the level-0
+ * residual from an LDS copy of p (0: from memory), bside the widest level
+ * the one-workgroup bottom kernel takes (32 or 64). */
+int wd2_ctx_mg_set_tune(wd2_ctx *ctx, int nt0, int stage, int bside);
+int wd2_ctx_mg_get_tune(wd2_ctx *ctx, int *nt0, int *stage, int *bside);
+/* Autotuner: times `cycles` full V-cycles of every shape above on the
+ * context's grid, precision and current right-hand side (p is restored),
+ * keeps the fastest and returns its time per V-cycle in *best_us. */
+int wd2_ctx_mg_autotune(wd2_ctx *ctx, double dk, int cycles, double *best_us);
+/* hipGraph steps (no counterpart in the reference).  enable != 0: each
+ * time step of a context on the multigrid path (nPpeSolver 7, no porous
+ * regions or blockages, no energy equation or equation of state) is one
+ * replay of a captured graph: the q-l iterations and V-cycles are queued to
+ * a budget (the last step's counts plus margin; work after a stopping test
+ * has fired returns at entry, so the fields are those of the host-driven
+ * step, bit for bit), and a step whose budget ran out is restored and redone
+ * host-driven.  Graphs are cached per budget.  Other setups step host-driven.
+ * wd2_ctx_graph_stats: steps replayed and steps redone so far. */
+int wd2_ctx_set_graph(wd2_ctx *ctx, int enable);
+int wd2_ctx_graph_stats(wd2_ctx *ctx, long *steps, long *redos);
+
+/* An ensemble of independent contexts (any sizes) advanced together (no
+ * counterpart in the reference, which runs one case per process,
+ * main.f:690-1212): every member's step is queued on its own stream before
+ * any is waited for, so the members run side by side on the GPU (graph mode
+ * members as one replay each).  prm holds one wd2_params per member; per
+ * step k and member i, ql_iter[k*n+i], ppe_iter[k*n+i] and dif[3(k*n+i)..]
+ * as for wd2_ctx_step.  The ensemble does not own the contexts. */
+typedef struct wd2_ens wd2_ens;
+int wd2_ens_create(wd2_ens **ens, int n, wd2_ctx *const *ctxs);
+int wd2_ens_destroy(wd2_ens *ens);
+int wd2_ens_step(wd2_ens *ens, const wd2_params *prm, int nsteps, int *ql_iter, int *ppe_iter, double *dif);
+
+/* Mean time of one full V-cycle in the current shape over `cycles` cycles on
+ * the context's current right-hand side (p is restored). */
+int wd2_ctx_mg_cycle_time(wd2_ctx *ctx, double dk, int cycles, double *us);
 
 #ifdef __cplusplus
 }

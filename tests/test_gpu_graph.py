@@ -1,0 +1,83 @@
+"""hipGraph steps and ensembles (no counterpart in the reference).
+
+* A step replayed as a captured graph equals the host-driven step bit for
+  bit, in FP64 and in mixed precision, including steps whose iteration
+  budget runs out and are redone host-driven (the budget follows the last
+  step's counts, so the first steps of a cold start rise past it).
+* An ensemble of cavities of uneven sizes and Reynolds numbers, advanced
+  together (graph mode and host-driven), gives every member the bits it has
+  when it runs alone.
+"""
+import numpy as np
+import pytest
+
+from wolfd2_amd import api, case, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(g, reg, prm, precision, graph, fields=None):
+    sol = api.Solver(g, reg, prm)
+    sol.set_precision(precision)
+    sol.set_graph(graph)
+    if fields is None:
+        z = np.zeros((g.ncol, g.ld))
+        sol.set_fields(z, z, z)
+        sol.cold_start()
+    else:
+        sol.set_fields(*fields)
+    return sol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_steps_equal_host_driven_steps(precision):
+    g, reg, prm = workloads.cavity(128, 1000.0, 1e-3, nPpeSolver=7, msorit=60, sortol=1e-10)
+    out = {}
+    for graph in (False, True):
+        sol = _solver(g, reg, prm, precision, graph)
+        tab = sol.step(30)
+        out[graph] = (tab, sol.get_fields(), sol.graph_stats())
+        sol.close()
+    (t0, f0, _), (t1, f1, st) = out[False], out[True]
+    print(f"  {precision}: graph steps {st}, ql {t1[:6, 0].tolist()} cycles {t1[:6, 1].tolist()}", flush=True)
+    assert st["replayed"] > 0 and st["replayed"] + st["redone"] == 30
+    assert st["redone"] >= 1, "a cold start's rising counts should exceed the first budgets"
+    assert np.array_equal(t0, t1)
+    for a, b in zip(f0, f1):
+        assert np.array_equal(a, b)
+
+
+def test_graph_steps_channel_with_profile_and_outlet():
+    g, reg, prm, x, y = workloads.channel(256, 64, outlet="mass_cons", dt=5e-4, nPpeSolver=7, msorit=60,
+                                          sortol=1e-10)
+    f0 = workloads.poiseuille_noise(g, reg, x, y)
+    out = {}
+    for graph in (False, True):
+        sol = _solver(g, reg, prm, "fp64", graph, fields=f0)
+        out[graph] = (sol.step(20), sol.get_fields())
+        sol.close()
+    assert np.array_equal(out[False][0], out[True][0])
+    for a, b in zip(out[False][1], out[True][1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_ensemble_members_equal_their_single_runs(graph):
+    specs = workloads.ensemble_specs(6, seed=7, sides=(64, 96, 128, 192))
+    precisions = ["mixed", "fp64"] * 3
+    cases = [workloads.cavity(n, re, 5e-4, nPpeSolver=7, msorit=60, sortol=1e-10) for n, re in specs]
+    print(f"  ensemble members (side, Re): {[(n, round(r)) for n, r in specs]}", flush=True)
+    alone = []
+    for (g, reg, prm), prec in zip(cases, precisions):
+        sol = _solver(g, reg, prm, prec, graph)
+        alone.append((sol.step(12), sol.get_fields()))
+        sol.close()
+    members = [_solver(g, reg, prm, prec, graph) for (g, reg, prm), prec in zip(cases, precisions)]
+    ens = api.Ensemble(members)
+    tabs = ens.step(12)
+    ens.close()
+    for i, m in enumerate(members):
+        assert np.array_equal(tabs[i], alone[i][0]), i
+        for a, b in zip(m.get_fields(), alone[i][1]):
+            assert np.array_equal(a, b), i
+        m.close()

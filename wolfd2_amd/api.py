@@ -26,7 +26,7 @@ from ._lib import Params, StepOptions, Wd2Error, check, dp, fp, ip, lib
 __all__ = ["vel_bound_cond", "pres_bound_cond", "vel_outflow_bcs", "project", "divergence", "ppe",
            "x_momentum", "y_momentum", "naux_momentum", "alt_trid_lu", "dmax_norm", "diff_max_norm",
            "temp_bound_cond", "therm_energy", "eq_state", "filter",
-           "Solver", "Wd2Error", "make_params"]
+           "Solver", "Ensemble", "Wd2Error", "make_params"]
 
 
 def _ld(a: np.ndarray) -> int:
@@ -229,6 +229,35 @@ class Solver:
         defect correction around FP32 sweeps)."""
         check(lib().wd2_ctx_set_precision(self.ctx, {"fp64": 0, "mixed": 1}[precision]))
 
+    def set_graph(self, enable: bool = True) -> None:
+        """Step as hipGraph replays where the setup allows (wd2_ctx_set_graph)."""
+        check(lib().wd2_ctx_set_graph(self.ctx, 1 if enable else 0))
+
+    def graph_stats(self) -> dict:
+        a, b = C.c_long(), C.c_long()
+        check(lib().wd2_ctx_graph_stats(self.ctx, C.byref(a), C.byref(b)))
+        return {"replayed": a.value, "redone": b.value}
+
+    def mg_autotune(self, cycles: int = 5) -> dict:
+        """Pick the multigrid solver's fastest kernel shapes for this grid and
+        precision (wd2_ctx_mg_autotune); returns the shape and its V-cycle time."""
+        us = C.c_double(0)
+        check(lib().wd2_ctx_mg_autotune(self.ctx, C.c_double(self.prm.dk), cycles, C.byref(us)))
+        return {**self.mg_tune(), "vcycle_us": us.value}
+
+    def mg_tune(self) -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().wd2_ctx_mg_get_tune(self.ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return {"nt0": a.value, "stage": b.value, "bside": c.value}
+
+    def set_mg_tune(self, nt0: int, stage: int, bside: int) -> None:
+        check(lib().wd2_ctx_mg_set_tune(self.ctx, nt0, stage, bside))
+
+    def mg_cycle_time(self, cycles: int = 5) -> float:
+        us = C.c_double(0)
+        check(lib().wd2_ctx_mg_cycle_time(self.ctx, C.c_double(self.prm.dk), cycles, C.byref(us)))
+        return us.value
+
     def set_fields(self, u, v, p):
         check(lib().wd2_ctx_set_fields(self.ctx, self.g.ld, dp(u), dp(v), dp(p)))
 
@@ -257,3 +286,46 @@ class Solver:
         out[:, 1] = np.frombuffer(pi, dtype=np.int32)
         out[:, 2:] = np.frombuffer(dif, dtype=np.float64).reshape(nsteps, nd)
         return out
+
+
+class Ensemble:
+    """Independent cases advanced together (wd2_ens_*): each member's step is
+    queued on its own stream before any is waited for.  Members are Solver
+    objects, which keep their own fields, parameters and precision."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        self.ens = C.c_void_p()
+        check(lib().wd2_ens_create(C.byref(self.ens), len(self.members), arr))
+
+    def close(self):
+        if self.ens:
+            lib().wd2_ens_destroy(self.ens)
+            self.ens = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps: int = 1) -> list:
+        """Advance every member nsteps; returns one (nsteps, 5) table per
+        member: ql_iter, ppe_iter, |p1-pn|, |u1-un|, |v1-vn|."""
+        n = len(self.members)
+        prm = (Params * n)(*[m.params for m in self.members])
+        ql = (C.c_int * (n * nsteps))()
+        pi = (C.c_int * (n * nsteps))()
+        dif = (C.c_double * (3 * n * nsteps))()
+        check(lib().wd2_ens_step(self.ens, prm, nsteps, ql, pi, dif))
+        q = np.frombuffer(ql, dtype=np.int32).reshape(nsteps, n)
+        pp = np.frombuffer(pi, dtype=np.int32).reshape(nsteps, n)
+        d = np.frombuffer(dif, dtype=np.float64).reshape(nsteps, n, 3)
+        out = []
+        for i in range(n):
+            t = np.zeros((nsteps, 5))
+            t[:, 0], t[:, 1], t[:, 2:] = q[:, i], pp[:, i], d[:, i, :]
+            out.append(t)
+        return out
+

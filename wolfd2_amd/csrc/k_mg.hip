@@ -51,9 +51,8 @@ constexpr int MG_HU = 2 * MG_NU;               // up halo
 constexpr int MG_TD = MG_OT + 2 * MG_HD;       // down tile side (44)
 constexpr int MG_TU = MG_OT + 2 * MG_HU;       // up tile side (40)
 constexpr int MG_TC = MG_TU / 2 + 2;           // coarse cells of the up tile (22)
-constexpr int MG_NT0 = 256;                    // threads of a level-0 tile workgroup (many tiles)
 constexpr int MG_NT1 = 1024;                   // threads of a coarser tile workgroup (few tiles: latency)
-constexpr int MG_BSIDE = 64;                   // the bottom kernel takes levels at most this wide
+constexpr int MG_BSIDE = 64;                   // widest level the bottom kernel can hold
 constexpr int MG_BT = 1024;                    // threads of the bottom workgroup
 constexpr int MG_BCAP = 6400;                  // LDS cells per array (e, r) of the bottom levels
 constexpr int MG_NCOARSE = 6;                  // sweeps on the coarsest grid (at most 2 x 2 cells)
@@ -208,7 +207,7 @@ __device__ inline int mg_tile(int ntiles) {
 // tile and one more ring) and b (ghosts frozen), stored in T for the up pass,
 // and max |r| of the owned points goes to part[tile].  D.nt > 0: this launch
 // first evaluates the stopping test of the level-0 kernel before it.
-template <typename T, bool L0, int NT>
+template <typename T, bool L0, int NT, bool STAGE>
 __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, int ntiles, int nx, int ny, int ld,
                                                 const double *__restrict__ p, const double *__restrict__ b,
                                                 const double *__restrict__ rau, const double *__restrict__ rgv,
@@ -217,7 +216,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
     if (mg_stopped(sc)) return;
     constexpr int TW = MG_TD, PW = TW + 2;
     __shared__ T E[TW * TW], R[TW * TW], AX[TW * (TW + 1)], AY[(TW + 1) * TW];
-    __shared__ double PS[L0 ? PW * PW : 1];
+    __shared__ double PS[(L0 && STAGE) ? PW * PW : 1];
     __shared__ double red[NT / 64];
     if (D.nt > 0 && mg_decide<NT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, sc, red)) return;
     const int tile = mg_tile(ntiles);
@@ -227,7 +226,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
     const int NX = L.nx, NY = L.ny;
     T *e = (T *)L.e, *r = (T *)L.r;
     tile_faces<T, TW, NT>(L, gx0, gy0, AX, AY);
-    if (L0) {
+    if (L0 && STAGE) {
         // p on the tile and one more ring: field (i, j) = (I + 2, J + 2) for
         // I = gx0 - 1 .., 0 outside the field
         for (int q = threadIdx.x; q < PW * PW; q += NT) {
@@ -258,9 +257,15 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
                     a5 = rgv[WD2_IX(i, j)];
                 }
                 const double a3 = -a4 - a2 - a5 - a1;
-                const int k = (y + 1) * PW + x + 1;
-                const double res =
-                    b[WD2_IX(i, j)] - (a1 * PS[k - PW] + a2 * PS[k - 1] + a3 * PS[k] + a4 * PS[k + 1] + a5 * PS[k + PW]);
+                double res;
+                if (STAGE) {
+                    const int k = (y + 1) * PW + x + 1;
+                    res = b[WD2_IX(i, j)] -
+                          (a1 * PS[k - PW] + a2 * PS[k - 1] + a3 * PS[k] + a4 * PS[k + 1] + a5 * PS[k + PW]);
+                } else {
+                    res = b[WD2_IX(i, j)] - (a1 * p[WD2_IX(i, j - 1)] + a2 * p[WD2_IX(i - 1, j)] +
+                                             a3 * p[WD2_IX(i, j)] + a4 * p[WD2_IX(i + 1, j)] + a5 * p[WD2_IX(i, j + 1)]);
+                }
                 rv = (T)res;
                 const bool own = x >= MG_HD && x < MG_HD + MG_OT && y >= MG_HD && y < MG_HD + MG_OT;
                 // a NaN residual wins the maximum (never converged)
@@ -527,22 +532,45 @@ void launch_mg_coef(const MgHierarchy &H, int nx, int ny, int ld, const double *
     }
 }
 
+template <typename T, int NT, bool STAGE>
+static void down0_as(const MgHierarchy &H, int nx, int ny, int ld, const double *b, const double *p,
+                     const double *rau, const double *rgv, const double *uni, double *part, Scalars *sc,
+                     hipStream_t s) {
+    int ntx = 0;
+    const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
+    hipLaunchKernelGGL((k_mg_down<T, true, NT, STAGE>), dim3(8 * ((nt + 7) / 8)), dim3(NT), 0, s, H.lev[0],
+                       H.lev[1], ntx, nt, nx, ny, ld, p, b, rau, rgv, uni ? 1 : 0, uni ? uni[0] : 0.0,
+                       uni ? uni[1] : 0.0, part, MgDecide{nullptr, 0, 0, 0, 0.0, 0.0}, sc);
+}
+
+// the level-0 down kernel in the shape the hierarchy's tuning selects
 template <typename T>
 static void down0(const MgHierarchy &H, int nx, int ny, int ld, const double *b, const double *p, const double *rau,
                   const double *rgv, const double *uni, double *part, Scalars *sc, hipStream_t s) {
-    int ntx = 0;
-    const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
-    hipLaunchKernelGGL((k_mg_down<T, true, MG_NT0>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT0), 0, s, H.lev[0],
-                       H.lev[1], ntx, nt, nx, ny, ld, p, b, rau, rgv, uni ? 1 : 0, uni ? uni[0] : 0.0,
-                       uni ? uni[1] : 0.0, part, MgDecide{nullptr, 0, 0, 0, 0.0, 0.0}, sc);
+    const bool big = H.tune.nt0 >= 512;
+    if (H.tune.stage) {
+        if (big)
+            down0_as<T, 512, true>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
+        else
+            down0_as<T, 256, true>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
+    } else {
+        if (big)
+            down0_as<T, 512, false>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
+        else
+            down0_as<T, 256, false>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
+    }
 }
 
 template <typename T>
 static void up0(const MgHierarchy &H, int ld, double *p, Scalars *sc, hipStream_t s) {
     int ntx = 0;
     const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
-    hipLaunchKernelGGL((k_mg_up<T, true, MG_NT0>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT0), 0, s, H.lev[0], H.lev[1],
-                       ntx, nt, ld, p, sc);
+    if (H.tune.nt0 >= 512)
+        hipLaunchKernelGGL((k_mg_up<T, true, 512>), dim3(8 * ((nt + 7) / 8)), dim3(512), 0, s, H.lev[0], H.lev[1],
+                           ntx, nt, ld, p, sc);
+    else
+        hipLaunchKernelGGL((k_mg_up<T, true, 256>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, s, H.lev[0], H.lev[1],
+                           ntx, nt, ld, p, sc);
 }
 
 // One V-cycle of defect correction (cycle is 1-based), every kernel
@@ -561,7 +589,7 @@ void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double 
     const MgDecide none{nullptr, 0, 0, 0, 0.0, 0.0};
     for (int l = 1; l < H.ntiled; l++) {
         const int nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
-        hipLaunchKernelGGL((k_mg_down<T, false, MG_NT1>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
+        hipLaunchKernelGGL((k_mg_down<T, false, MG_NT1, false>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
                            H.lev[l + 1], ntx, nt, nx, ny, ld, p, b, rau, rgv, 0, 0.0, 0.0, part, l == 1 ? D : none,
                            sc);
     }

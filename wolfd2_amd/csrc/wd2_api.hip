@@ -128,6 +128,22 @@ struct wd2_ctx {
     double *mg_part = nullptr;
     int mg_prec = -1, mg_ver = -1, mg_last = 4;
     double mg_div = 0.0, mg_div_max = 0.0;
+    // hipGraph mode (wd2_ctx_set_graph): captured steps keyed by the q-l
+    // iterations and V-cycles queued and the buffer u is in (u and us swap
+    // every step), the pre-step u, v, p for a step that is redone eagerly,
+    // the pinned copy of the scalars each graph ends with, the last step's
+    // q-l count, and counts of replayed and redone steps
+    struct GraphEntry {
+        int q, cyc;
+        const double *u;
+        hipGraphExec_t exec;
+    };
+    bool graph_on = false;
+    std::vector<GraphEntry> graphs;
+    double *gsave = nullptr;
+    Scalars *h_sc = nullptr;
+    int ql_last = 4, g_q = 0, g_c = 0;
+    long graph_steps = 0, graph_redos = 0;
     // optional HIP-event timing of one kernel (kt_on: 1 the red-black SOR
     // kernel, 2 / 3 the multigrid level-0 down / up kernel): event pairs,
     // each around kt_cnt[pair] back-to-back launches
@@ -180,6 +196,9 @@ static void ctx_free(wd2_ctx *c) {
     (void)hipFree(c->tile_flags);
     (void)hipFree(c->mg_store);
     (void)hipFree(c->mg_part);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    (void)hipFree(c->gsave);
+    if (c->h_sc) (void)hipHostFree(c->h_sc);
     (void)hipFree(c->tw.cp);
     (void)hipFree(c->zm);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -652,9 +671,21 @@ static int naux_momentum(wd2_ctx *c, int mqiter, double dk, double re, double fr
 }
 
 // ------------------------------------------------------------------- Ppe
+// The levels at most tune.bside cells wide (and at most mg_bottom_side())
+// go to the bottom kernel; level 0 is always tiled.
+static int mg_retile(MgHierarchy &H) {
+    const int side = std::min(H.tune.bside, mg_bottom_side());
+    H.ntiled = 1;
+    while (H.ntiled < H.nlev && std::max(H.lev[H.ntiled].nx, H.lev[H.ntiled].ny) > side) H.ntiled++;
+    size_t cells = 0;
+    for (int l = H.ntiled; l < H.nlev; l++) cells += (size_t)(H.lev[l].nx + 2) * (H.lev[l].ny + 2);
+    if (H.ntiled >= H.nlev || cells > (size_t)mg_bottom_cells())
+        return fail(WD2_ERR_ARG, "multigrid: bottom levels do not fit (%zu cells)", cells);
+    return WD2_OK;
+}
+
 // Levels of the multigrid solver: level 0 is the (nx-1) x (ny-1) pressure
-// grid, each next level ceil-halves both sides down to at most 2 x 2 cells;
-// the levels at most mg_bottom_side() cells wide go to the bottom kernel.
+// grid, each next level ceil-halves both sides down to at most 2 x 2 cells.
 static int mg_setup(wd2_ctx *c) {
     if (c->mg_store) return WD2_OK;
     MgHierarchy &H = c->mg;
@@ -674,12 +705,9 @@ static int mg_setup(wd2_ctx *c) {
         NX = (NX + 1) / 2;
         NY = (NY + 1) / 2;
     }
-    H.ntiled = 1;
-    while (H.ntiled < H.nlev && std::max(H.lev[H.ntiled].nx, H.lev[H.ntiled].ny) > mg_bottom_side()) H.ntiled++;
-    size_t cells = 0;
-    for (int l = H.ntiled; l < H.nlev; l++) cells += (size_t)(H.lev[l].nx + 2) * (H.lev[l].ny + 2);
-    if (H.ntiled >= H.nlev || cells > (size_t)mg_bottom_cells())
-        return fail(WD2_ERR_ARG, "multigrid: bottom levels do not fit (%zu cells)", cells);
+    H.tune = MgTune{256, 0, 64};
+    int st = mg_retile(H);
+    if (st) return st;
     HIPCHK(hipMalloc(&c->mg_store, total * sizeof(double)));
     c->mg_bytes = total * sizeof(double);
     HIPCHK(hipMemsetAsync(c->mg_store, 0, c->mg_bytes, c->s));
@@ -698,6 +726,58 @@ static int mg_setup(wd2_ctx *c) {
     return WD2_OK;
 }
 
+// Levels allocated and the face coefficients built for the context's
+// precision and metrics.
+static int mg_prepare(wd2_ctx *c) {
+    int st = mg_setup(c);
+    if (st) return st;
+    if (c->mg_prec != c->precision || c->mg_ver != c->met_version) {
+        const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
+        // float and double views share the storage: clear it (zero rings)
+        if (c->mg_prec != -1 && c->mg_prec != c->precision)
+            HIPCHK(hipMemsetAsync(c->mg_store, 0, c->mg_bytes, c->s));
+        if (c->precision == WD2_PREC_MIXED)
+            launch_mg_coef<float>(c->mg, c->nx, c->ny, c->ld, c->M.rau, c->M.rgv, uni, c->s);
+        else
+            launch_mg_coef<double>(c->mg, c->nx, c->ny, c->ld, c->M.rau, c->M.rgv, uni, c->s);
+        c->mg_prec = c->precision;
+        c->mg_ver = c->met_version;
+    }
+    return WD2_OK;
+}
+
+// Mean time of a full V-cycle (microseconds) over `cycles` cycles on the
+// context's current right-hand side and p (tolerance 0, so none stops
+// early); p is restored afterwards.
+static int mg_cycle_time(wd2_ctx *c, int cycles, double dk, double *us) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    hipEvent_t e[2];
+    HIPCHK(hipEventCreate(&e[0]));
+    HIPCHK(hipEventCreate(&e[1]));
+    const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->p, c->pscr, s);
+    launch_mg_reset(c->sc, s);
+    HIPCHK(hipEventRecord(e[0], s));
+    for (int k = 1; k <= cycles; k++) {
+        if (c->precision == WD2_PREC_MIXED)
+            launch_mg_cycle<float>(c->mg, nx, ny, ld, c->rhs, c->p, c->M.rau, c->M.rgv, uni, dk, 0.0, k, cycles + 1,
+                                   c->mg_part, c->sc, s, nullptr, nullptr);
+        else
+            launch_mg_cycle<double>(c->mg, nx, ny, ld, c->rhs, c->p, c->M.rau, c->M.rgv, uni, dk, 0.0, k, cycles + 1,
+                                    c->mg_part, c->sc, s, nullptr, nullptr);
+    }
+    HIPCHK(hipEventRecord(e[1], s));
+    launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pscr, c->p, s);
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    *us = 1e3 * ms / cycles;
+    return WD2_OK;
+}
+
 // nPpeSolver 7: multigrid V-cycles (FP64 or FP32 by the context's precision)
 // in FP64 defect correction, until max |dk (b - A p)| < tol (the divergence
 // Project leaves) or maxcyc cycles; *nconv_out = the V-cycles applied.
@@ -706,20 +786,10 @@ static int ppe_mg(wd2_ctx *c, int cart, int maxcyc, double dk, double tol, doubl
     hipStream_t s = c->s;
     if (c->any_blockage)
         return fail(WD2_ERR_UNSUPPORTED, "the multigrid Poisson solver (nPpeSolver 7) does not take blockages");
-    int st = mg_setup(c);
+    int st = mg_prepare(c);
     if (st) return st;
     const bool mx = c->precision == WD2_PREC_MIXED;
     const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
-    if (c->mg_prec != c->precision || c->mg_ver != c->met_version) {
-        // float and double views share the storage: clear it (zero rings)
-        if (c->mg_prec != -1 && c->mg_prec != c->precision) HIPCHK(hipMemsetAsync(c->mg_store, 0, c->mg_bytes, s));
-        if (mx)
-            launch_mg_coef<float>(c->mg, nx, ny, ld, c->M.rau, c->M.rgv, uni, s);
-        else
-            launch_mg_coef<double>(c->mg, nx, ny, ld, c->M.rau, c->M.rgv, uni, s);
-        c->mg_prec = c->precision;
-        c->mg_ver = c->met_version;
-    }
     launch_mg_reset(c->sc, s);
     const double *b = cart ? c->rhs : c->beff;
     const size_t kt0 = c->kt_used;
@@ -1040,6 +1110,151 @@ namespace wd2 {
 int api_fail(int code, const char *msg) { return fail(code, "%s", msg); }
 int noalloc_trid_fallbacks();
 }  // namespace wd2
+
+// ------------------------------------------------------ hipGraph steps
+// A time step on the multigrid path as one graph replay.  The host-driven
+// step reads two stopping tests (the q-l loop's and the Poisson solve's)
+// while it queues work; in a graph both loops are queued to a budget instead
+// (the last step's count plus margin): iterations after a stopping test has
+// fired return at entry, exactly as in the host-driven step, so the fields
+// are the same.  If a budget ran out before its test fired, the step is
+// restored from the copies the graph took first and redone host-driven.
+static bool graph_ok(const wd2_ctx *c, const wd2_params *P) {
+    return c->graph_on && P->nPpeSolver == 7 && P->mqiter >= 1 && !c->any_porous && !c->any_blockage &&
+           c->opt.nthermen != 1 && c->opt.neqstate != 1 && !c->d_live && c->kt_on == 0;
+}
+
+// The step's work on the stream (the body of a capture): main.f:690-972 with
+// Q q-l iterations and C V-cycles queued.
+static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C) {
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    const wd2_step_options &O = c->opt;
+    {
+        const double *src[3] = {c->u, c->v, c->p};
+        double *dst[3] = {c->gsave, c->gsave + c->nf, c->gsave + 2 * c->nf};
+        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 3, s);
+    }
+    {
+        const double *src[5] = {c->p, c->u, c->v, c->u, c->v};
+        double *dst[5] = {c->pn, c->un, c->vn, c->us, c->vs};
+        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 5, s);
+    }
+    {   // momentum.f:114-119
+        const double *src[2] = {c->un, c->vn};
+        double *dst[2] = {c->us, c->vs};
+        launch_copy_set(ld, 1, nx + 1, 1, ny + 1, src, dst, 2, s);
+    }
+    const int *stop = &c->sc->ql_stop;
+    launch_ql_reset(c->sc, s);
+    for (int m = 1; m <= Q; m++) {
+        if (c->any_outlet) launch_vel_bc<double>(c->R, ld, c->us, c->vs, true, &c->sc->err, s, stop);
+        if (c->precision == WD2_PREC_MIXED)
+            momentum_pair<float>(c, P->dk, P->re, P->fr, c->Tf, c->chf, c->chf2, c->twf, m, P->qtol);
+        else
+            momentum_pair<double>(c, P->dk, P->re, P->fr, c->T, c->ch, c->ch2, c->tw, m, P->qtol);
+    }
+    if (O.nfiltu == 1) launch_filter(c->R, c->TH, ld, VU, O.fpu, c->us, c->qh, s);
+    if (O.nfiltv == 1) launch_filter(c->R, c->TH, ld, VV, O.fpv, c->vs, c->qh, s);
+    launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    // Ppe (pressure.f:30) with the multigrid solver
+    launch_divergence(nx, ny, ld, 1, c->M, c->us, c->vs, c->div, s);
+    launch_ppe_rhs(nx, ny, ld, P->dk, c->div, nullptr, c->rhs, s);
+    const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
+    const double *b = P->lCartesGrid ? c->rhs : c->beff;
+    launch_mg_reset(c->sc, s);
+    for (int cyc = 1; cyc <= C; cyc++) {
+        if (!P->lCartesGrid) launch_rhs_cross(nx, ny, ld, c->M, c->rhs, c->p, c->beff, s);
+        if (c->precision == WD2_PREC_MIXED)
+            launch_mg_cycle<float>(c->mg, nx, ny, ld, b, c->p, c->M.rau, c->M.rgv, uni, P->dk, P->sortol, cyc,
+                                   P->msorit, c->mg_part, c->sc, s, nullptr, nullptr);
+        else
+            launch_mg_cycle<double>(c->mg, nx, ny, ld, b, c->p, c->M.rau, c->M.rgv, uni, P->dk, P->sortol, cyc,
+                                    P->msorit, c->mg_part, c->sc, s, nullptr, nullptr);
+    }
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    launch_project(c->R, ld, P->dk, c->M, c->p, c->us, c->vs, s);
+    launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    // u = us, v = vs: the host swaps the buffers after the step, so the rest
+    // of the step works on us, vs as the new u, v (main.f:946-965)
+    launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
+    launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    launch_max_norm(nx, ny, ld, c->pn, c->p, c->partials, c->sc, 2, s);
+    launch_max_norm(nx, ny, ld, c->un, c->us, c->partials + RED_MAX_BLOCKS, c->sc, 3, s);
+    launch_max_norm(nx, ny, ld, c->vn, c->vs, c->partials + 2 * RED_MAX_BLOCKS, c->sc, 4, s);
+    (void)hipMemcpyAsync(c->h_sc, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, s);
+}
+
+// Queue one graph step (capturing it first if no graph of this shape is
+// cached); graph_end completes it.
+static int graph_begin(wd2_ctx *c, const wd2_params *P) {
+    int st = mg_prepare(c);
+    if (st) return st;
+    if (!c->gsave) HIPCHK(hipMalloc(&c->gsave, 3 * c->nf * sizeof(double)));
+    if (!c->h_sc) HIPCHK(hipHostMalloc((void **)&c->h_sc, sizeof(Scalars), hipHostMallocDefault));
+    const int Q = std::min(P->mqiter, c->ql_last + 2);
+    const int C = std::min(P->msorit + 1, c->mg_last + 2);
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : c->graphs)
+        if (g.q == Q && g.cyc == C && g.u == c->u) exec = g.exec;
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        step_body(c, P, Q, C);
+        const hipError_t ec = hipStreamEndCapture(c->s, &graph);
+        if (ec != hipSuccess) return fail(WD2_ERR_HIP, "step capture: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(WD2_ERR_HIP, "step graph instantiation: %s", hipGetErrorString(ei));
+        if (c->graphs.size() >= 16) {
+            (void)hipGraphExecDestroy(c->graphs.front().exec);
+            c->graphs.erase(c->graphs.begin());
+        }
+        c->graphs.push_back({Q, C, c->u, exec});
+    }
+    HIPCHK(hipGraphLaunch(exec, c->s));
+    c->g_q = Q;
+    c->g_c = C;
+    return WD2_OK;
+}
+
+static int graph_end(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double *dif) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    const Scalars h = *c->h_sc;
+    if (h.err) return check_err(c);
+    const bool ql_short = h.ql_stop == 0 && c->g_q < P->mqiter;
+    const bool mg_short = h.mg_done == 0;
+    if (ql_short || mg_short) {
+        // a budget ran out: restore the step's start and redo it host-driven
+        const double *src[3] = {c->gsave, c->gsave + c->nf, c->gsave + 2 * c->nf};
+        double *dst[3] = {c->u, c->v, c->p};
+        launch_copy_set(c->ld, 0, c->nx + 1, 0, c->ny + 1, src, dst, 3, c->s);
+        c->graph_redos++;
+        int st = one_step(c, P, nql, nsor, dif);
+        if (st) return st;
+        HIPCHK(hipStreamSynchronize(c->s));
+        c->ql_last = *nql > 0 ? *nql : P->mqiter;
+        return WD2_OK;
+    }
+    std::swap(c->u, c->us);
+    std::swap(c->v, c->vs);
+    *nql = h.ql_stop ? h.ql_stop : -1;
+    int nconv = h.mg_nconv;
+    c->mg_div = h.val[7];
+    if (!(c->mg_div <= c->mg_div_max)) c->mg_div_max = c->mg_div;
+    if (nconv < 0) {
+        if (g_verbose) fprintf(stdout, " Warning: multigrid did not converge after %6d V-cycles.\n", P->msorit);
+        nconv = P->msorit;
+    }
+    *nsor = nconv;
+    c->mg_last = std::min(nconv, 64);
+    c->ql_last = *nql > 0 ? *nql : P->mqiter;
+    if (dif) for (int q = 0; q < 3; q++) dif[q] = h.val[2 + q];
+    c->graph_steps++;
+    return WD2_OK;
+}
 
 static int fallbacks(wd2_ctx *c) {
     int n = 0;
@@ -1492,8 +1707,12 @@ static int ctx_steps(wd2_ctx *c, const wd2_params *P, int nsteps, int *ql_iter, 
     double d4[4] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < nsteps; k++) {
         int nql = 0, nsor = 0;
-        int st = one_step(c, P, &nql, &nsor, dif ? d4 : nullptr);
-        if (st) return st;
+        int st;
+        if (ndif == 3 && graph_ok(c, P)) {
+            if ((st = graph_begin(c, P)) || (st = graph_end(c, P, &nql, &nsor, d4))) return st;
+        } else if ((st = one_step(c, P, &nql, &nsor, dif ? d4 : nullptr))) {
+            return st;
+        }
         if (dif) {
             HIPCHK(hipStreamSynchronize(c->s));   // d4 is written by the step's last copy
             for (int q = 0; q < ndif; q++) dif[ndif * k + q] = d4[q];
@@ -1618,6 +1837,64 @@ int wd2_ctx_set_bc_profile(wd2_ctx *c, int ireg, int jreg, int side, int var, co
     return WD2_OK;
 }
 
+int wd2_ctx_mg_set_tune(wd2_ctx *c, int nt0, int stage, int bside) {
+    if ((nt0 != 256 && nt0 != 512) || (stage != 0 && stage != 1) || (bside != 32 && bside != 64))
+        return fail(WD2_ERR_ARG, "multigrid shape: nt0 256|512, stage 0|1, bside 32|64");
+    int st = mg_setup(c);
+    if (st) return st;
+    const MgTune old = c->mg.tune;
+    c->mg.tune = MgTune{nt0, stage, bside};
+    if ((st = mg_retile(c->mg))) {
+        c->mg.tune = old;
+        (void)mg_retile(c->mg);
+        return st;
+    }
+    return WD2_OK;
+}
+
+int wd2_ctx_mg_get_tune(wd2_ctx *c, int *nt0, int *stage, int *bside) {
+    int st = mg_setup(c);
+    if (st) return st;
+    if (nt0) *nt0 = c->mg.tune.nt0;
+    if (stage) *stage = c->mg.tune.stage;
+    if (bside) *bside = c->mg.tune.bside;
+    return WD2_OK;
+}
+
+int wd2_ctx_mg_cycle_time(wd2_ctx *c, double dk, int cycles, double *us) {
+    if (cycles < 1) return fail(WD2_ERR_ARG, "cycle timing needs cycles >= 1");
+    int st = mg_prepare(c);
+    if (st) return st;
+    return mg_cycle_time(c, cycles, dk, us);
+}
+
+int wd2_ctx_mg_autotune(wd2_ctx *c, double dk, int cycles, double *best_us) {
+    if (cycles < 1) return fail(WD2_ERR_ARG, "autotune needs cycles >= 1");
+    int st = mg_prepare(c);
+    if (st) return st;
+    MgTune best = c->mg.tune;
+    double tbest = 0.0;
+    bool have = false;
+    for (int nt0 : {256, 512})
+        for (int stage : {0, 1})
+            for (int bside : {32, 64}) {
+                c->mg.tune = MgTune{nt0, stage, bside};
+                if (mg_retile(c->mg)) continue;
+                double us = 0.0;
+                (void)mg_cycle_time(c, 1, dk, &us);   // warm
+                if ((st = mg_cycle_time(c, cycles, dk, &us))) return st;
+                if (!have || us < tbest) {
+                    tbest = us;
+                    best = c->mg.tune;
+                    have = true;
+                }
+            }
+    c->mg.tune = best;
+    if ((st = mg_retile(c->mg))) return st;
+    if (best_us) *best_us = tbest;
+    return WD2_OK;
+}
+
 int wd2_ctx_mg_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     if (!c->mg_store || c->mg_prec < 0) return fail(WD2_ERR_ARG, "no multigrid solve has run on this context");
     if ((which != 2 && which != 3) || reps < 1) return fail(WD2_ERR_ARG, "mg kernel timing: which %d reps %d", which, reps);
@@ -1641,6 +1918,66 @@ int wd2_ctx_mg_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     (void)hipEventDestroy(e[0]);
     (void)hipEventDestroy(e[1]);
     if (avg_us) *avg_us = 1e3 * ms / reps;
+    return WD2_OK;
+}
+
+int wd2_ctx_set_graph(wd2_ctx *c, int enable) {
+    c->graph_on = enable != 0;
+    return WD2_OK;
+}
+
+int wd2_ctx_graph_stats(wd2_ctx *c, long *steps, long *redos) {
+    if (steps) *steps = c->graph_steps;
+    if (redos) *redos = c->graph_redos;
+    return WD2_OK;
+}
+
+// ------------------------------------------------------------ ensembles
+struct wd2_ens {
+    std::vector<wd2_ctx *> m;
+};
+
+int wd2_ens_create(wd2_ens **out, int n, wd2_ctx *const *ctxs) {
+    if (!out || n < 1 || !ctxs) return fail(WD2_ERR_ARG, "ensemble needs n >= 1 contexts");
+    wd2_ens *e = new wd2_ens();
+    e->m.assign(ctxs, ctxs + n);
+    *out = e;
+    return WD2_OK;
+}
+
+int wd2_ens_destroy(wd2_ens *e) {
+    delete e;
+    return WD2_OK;
+}
+
+// Every member's step is queued on its own stream before any is waited for,
+// so the members' graphs run side by side on the GPU; a member whose setup
+// has no graph path steps host-driven after the others are queued.
+int wd2_ens_step(wd2_ens *e, const wd2_params *P, int nsteps, int *ql_iter, int *ppe_iter, double *dif) {
+    const int n = (int)e->m.size();
+    std::vector<char> g(n);
+    for (int k = 0; k < nsteps; k++) {
+        for (int i = 0; i < n; i++) {
+            g[i] = graph_ok(e->m[i], &P[i]);
+            if (g[i]) {
+                int st = graph_begin(e->m[i], &P[i]);
+                if (st) return st;
+            }
+        }
+        for (int i = 0; i < n; i++) {
+            wd2_ctx *c = e->m[i];
+            int nql = 0, nsor = 0;
+            double d3[3] = {0.0, 0.0, 0.0};
+            int st = g[i] ? graph_end(c, &P[i], &nql, &nsor, d3) : one_step(c, &P[i], &nql, &nsor, d3);
+            if (st) return st;
+            if (!g[i]) HIPCHK(hipStreamSynchronize(c->s));
+            const size_t r = (size_t)k * n + i;
+            if (ql_iter) ql_iter[r] = nql;
+            if (ppe_iter) ppe_iter[r] = nsor;
+            if (dif)
+                for (int q = 0; q < 3; q++) dif[3 * r + q] = d3[q];
+        }
+    }
     return WD2_OK;
 }
 

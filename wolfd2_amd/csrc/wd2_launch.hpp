@@ -231,12 +231,20 @@ struct MgLevel {
     int nx, ny, ld;
     void *e, *r, *ax, *ay, *c;
 };
+// Shapes an autotuner chooses among (wd2_ctx_mg_autotune): threads of a
+// level-0 tile workgroup (256 or 512), whether the level-0 residual reads p
+// from an LDS copy of the tile (1) or straight from memory (0), and the
+// widest level the one-workgroup bottom kernel takes (32 or 64).
+struct MgTune {
+    int nt0, stage, bside;
+};
 struct MgHierarchy {
     int nlev, ntiled;
     MgLevel lev[MG_MAXLEV];
+    MgTune tune;
 };
 int mg_tiles(int NX, int NY, int *ntx);
-int mg_bottom_side();     // widest level the bottom kernel takes
+int mg_bottom_side();     // widest level the bottom kernel can take
 int mg_bottom_cells();    // LDS cells per array of the bottom kernel
 int mg_sweeps();          // smoothing sweeps before and after
 template <typename T>

@@ -89,3 +89,13 @@ def through_restart(g, u, v, p, k: int = 0, dtime: float = 0.0):
         f[S] = r[name]
         out.append(f)
     return tuple(out)
+
+
+def ensemble_specs(n: int = 64, seed: int = 20261202, sides=tuple(range(128, 769, 64)),
+                   re_range=(100.0, 2000.0)):
+    """The ensemble of cavities: n (side, Re) pairs, sides drawn uniformly from
+    {128, 192, ..., 768} cells and Re uniformly from [100, 2000], seeded."""
+    rng = np.random.default_rng(seed)
+    side = rng.choice(np.asarray(sides), size=n)
+    re = rng.uniform(re_range[0], re_range[1], size=n)
+    return [(int(a), float(b)) for a, b in zip(side, re)]
