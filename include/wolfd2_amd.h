@@ -347,6 +347,54 @@ int wd2_ens_step(wd2_ens *ens, const wd2_params *prm, int nsteps, int *ql_iter, 
  * the context's current right-hand side (p is restored). */
 int wd2_ctx_mg_cycle_time(wd2_ctx *ctx, double dk, int cycles, double *us);
 
+/* ---- Adjoint of a run of time steps (no counterpart in the reference).
+ * Reverse-mode gradient, FP64, of nsteps steps of wd2_ctx_step (main.f:690-972)
+ * with each q-l loop and Poisson solve taken at its converged solution, on a
+ * cavity: one fluid region covering the grid, no-slip walls (type 1) on every
+ * side with any boundary profiles (wd2_ctx_set_bc_profile), nPpeSolver 7,
+ * lCartesGrid set, no energy equation, density or filters.  The forward run
+ * should converge its q-l loops: the gradient is that of the converged map.
+ * A negative qtol runs every loop for max_ql_iter iterations, which avoids
+ * the reference's stopping norm (DiffMaxNorm over j = 2..ny-1) ending a loop
+ * whose increments sit on the rows next to the north or south wall (the
+ * first steps of a lid started from rest).
+ *
+ * wd2_adj_create binds an adjoint object to a context (checkpoint_every:
+ * steps between the saved states, e.g. 20; memory: 3 fields per checkpoint,
+ * 5 fields per step of one segment, 15 work fields).  wd2_adj_forward runs
+ * nsteps steps of the context from its current fields (as wd2_ctx_step),
+ * saving the checkpoints.  wd2_adj_backward takes the adjoint of the final
+ * u, v, p (host arrays with leading dimension ld; p_bar may be NULL) and
+ * returns the adjoint of the initial u, v, p (any output may be NULL) and of
+ * the boundary values: bc_bar[((side-1)*2 + (var-1))*W + k] for side 1..4
+ * (W, E, S, N), var 1..2 (u, v) and grid line k (j on W/E, i on S/N), W =
+ * max(nx, ny) + 2 -- for a side with a constant value, the gradient of that
+ * constant is the sum over its lines.  The context's fields are those after
+ * the forward run when it returns. */
+typedef struct wd2_adj wd2_adj;
+typedef struct {
+    int steps;               /* steps differentiated by the last backward pass */
+    int adj_iters_max;       /* most fixed-point iterations of one adjoint q-l loop */
+    double adj_iters_mean;
+    int ppe_cycles_max;      /* most V-cycles of one adjoint Poisson solve */
+    double ppe_cycles_mean;
+    int fwd_unconverged;     /* forward steps whose q-l loop (qtol > 0) or Poisson solve hit its cap */
+    int adj_unconverged;     /* adjoint q-l loops or Poisson solves that hit their caps */
+    double ms_forward;       /* wall time of the last wd2_adj_forward */
+    double ms_recompute;     /* backward pass: re-running the segments from their checkpoints */
+    double ms_reverse;       /* backward pass: the adjoint steps */
+} wd2_adj_stats;
+int wd2_adj_create(wd2_adj **adj, wd2_ctx *ctx, int checkpoint_every);
+int wd2_adj_destroy(wd2_adj *adj);
+/* fixed_point_tol: the adjoint q-l loop stops when max|update| <= tol *
+ * max|nu| (default 1e-12, at most max_iter iterations, default 200);
+ * poisson_tol: the adjoint Poisson solve stops when max|b - A lam| <= tol *
+ * max|b| (default 1e-13).  Non-positive values keep the current setting. */
+int wd2_adj_set_tolerances(wd2_adj *adj, double fixed_point_tol, int max_iter, double poisson_tol);
+int wd2_adj_forward(wd2_adj *adj, const wd2_params *prm, int nsteps, int *ql_iter, int *ppe_iter);
+int wd2_adj_backward(wd2_adj *adj, int ld, const double *u_bar, const double *v_bar, const double *p_bar,
+                     double *u0_bar, double *v0_bar, double *p0_bar, double *bc_bar, wd2_adj_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,169 @@
+"""The adjoint of a run of time steps (wd2_adj_*, no counterpart in the
+reference) against central finite differences of the forward run.
+
+The forward map is main.f's step with converged q-l loops and multigrid
+Poisson solves; the gradient pinned is that of this map with respect to the
+lid values (the north side's u profile), the initial velocity on the points a
+step updates, and (small cases) the initial pressure.  Each check takes a
+random direction d of the controls and compares <grad J, d> with
+(J(x + eps d) - J(x - eps d)) / (2 eps); a wrong sign, index, transposed
+stencil or a missing step piece shows as an O(1) relative error.
+"""
+import numpy as np
+import pytest
+import torch
+
+from wolfd2_amd import api, autograd, case, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _stretched(nx, ny):
+    s = np.arange(nx) / (nx - 1.0)
+    t = np.arange(ny) / (ny - 1.0)
+    xs = s + 0.12 * np.sin(np.pi * s) * s
+    yt = t - 0.08 * np.sin(2 * np.pi * t)
+    return np.broadcast_to(xs[None, :], (ny, nx)).copy(), np.broadcast_to(yt[:, None], (ny, nx)).copy()
+
+
+def _small_case(re=100.0, dt=2e-3):
+    x, y = _stretched(34, 30)
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(g.nx, g.ny)
+    prm = case.Params.from_physical(dt, 1.0, 1.0 / re, nPpeSolver=7, msorit=100, sortol=1e-15, qtol=1e-15,
+                                    mqiter=80)
+    return g, reg, prm, x, y
+
+
+def _base_state(g, reg, prm, lid, nwarm=4):
+    """A developed state: a few steps from rest under the lid profile."""
+    sol = api.Solver(g, reg, prm)
+    sol.set_bc_profile(1, 1, case.NORTH, case.VAR_U, lid)
+    z = np.zeros((g.ncol, g.ld))
+    sol.set_fields(z, z, z)
+    sol.step(nwarm)
+    f = sol.get_fields()
+    sol.close()
+    return f
+
+
+@pytest.mark.parametrize("nsteps,every", [(1, 1), (5, 2)])
+def test_adjoint_matches_central_differences_small(nsteps, every):
+    """Stretched 33x29 cavity, a linear functional of the final u, v, p: the
+    gradient with respect to the lid, the initial u, v (updated points) and
+    the initial p agrees with central differences on random directions to
+    1e-7 relative; segments of `every` steps re-run from checkpoints."""
+    g, reg, prm, x, y = _small_case()
+    lid = workloads.lid_modes(x, y, reg, seed=5)
+    u0, v0, p0 = _base_state(g, reg, prm, lid)
+    rng = np.random.default_rng(nsteps)
+    wu, wv, wp = (rng.standard_normal((g.ncol, g.ld)) for _ in range(3))
+    su, sv = workloads.interior_u(g), workloads.interior_v(g)
+    sp = np.s_[2:g.ny + 1, 2:g.nx + 1]
+    mask_u, mask_v, mask_p = (np.zeros((g.ncol, g.ld)) for _ in range(3))
+    mask_u[su] = 1
+    mask_v[sv] = 1
+    mask_p[sp] = 1
+    sol = api.Solver(g, reg, prm)
+    adj = api.Adjoint(sol, every)
+
+    def J(lid_, u_, v_, p_, with_tape=False):
+        sol.set_bc_profile(1, 1, case.NORTH, case.VAR_U, lid_)
+        sol.set_fields(u_, v_, p_)
+        tab = adj.forward(nsteps) if with_tape else sol.step(nsteps)[:, :2]
+        assert np.all(tab[:, 0] > 0), "q-l loop did not converge"
+        u, v, p = sol.get_fields()
+        return float(np.sum(wu * mask_u * u) + np.sum(wv * mask_v * v) + np.sum(wp * mask_p * p))
+
+    J0 = J(lid, u0, v0, p0, with_tape=True)
+    r = adj.backward(wu * mask_u, wv * mask_v, wp * mask_p)
+    st = r["stats"]
+    print(f"  {nsteps} steps (checkpoint every {every}): adjoint q-l iterations max {st['adj_iters_max']}, "
+          f"Poisson V-cycles max {st['ppe_cycles_max']}, unconverged fwd {st['fwd_unconverged']} "
+          f"adj {st['adj_unconverged']}", flush=True)
+    assert st["adj_unconverged"] == 0 and st["fwd_unconverged"] == 0
+    # the context's fields are the forward run's final ones again
+    J1 = J(lid, u0, v0, p0)
+    assert J1 == J0
+    glid = r["bc_bar"][case.NORTH - 1, case.VAR_U - 1, 2:g.nx + 1]
+    gu, gv, gp = r["u0_bar"], r["v0_bar"], r["p0_bar"]
+    worst = 0.0
+    for k in range(6):
+        dl = np.zeros(g.nx + 2)
+        dl[2:g.nx + 1] = rng.standard_normal(g.nx - 1)
+        du, dv, dpp = (rng.standard_normal((g.ncol, g.ld)) for _ in range(3))
+        du *= mask_u
+        dv *= mask_v
+        dpp *= mask_p
+        which = k % 3   # lid only, velocity only, everything
+        if which == 0:
+            du[:] = dv[:] = dpp[:] = 0.0
+        elif which == 1:
+            dl[:] = 0.0
+            dpp[:] = 0.0
+        eps = 1e-6
+        jp = J(lid + eps * dl, u0 + eps * du, v0 + eps * dv, p0 + eps * dpp)
+        jm = J(lid - eps * dl, u0 - eps * du, v0 - eps * dv, p0 - eps * dpp)
+        fd = (jp - jm) / (2 * eps)
+        ad = float(np.sum(glid * dl[2:g.nx + 1]) + np.sum(gu * du) + np.sum(gv * dv) + np.sum(gp * dpp))
+        rel = abs(fd - ad) / max(abs(fd), 1e-30)
+        worst = max(worst, rel)
+        print(f"    direction {k} ({['lid', 'velocity', 'all'][which]}): FD {fd:.12e} adjoint {ad:.12e} "
+              f"rel {rel:.2e}", flush=True)
+    adj.close()
+    sol.close()
+    assert worst < 1e-7
+
+
+def test_adjoint_cavity_512_lid_and_initial_velocity():
+    """The adjoint case at full size: 512^2 cavity, Re 400, 200 steps from
+    rest, checkpoint every 20 steps.  Objective: half the squared L2 misfit of
+    the final u, v to the target run's (a lid of 16 seeded Fourier modes,
+    amplitudes N(0, 0.1)); controls: the 512 lid values and the initial
+    velocity.  The gradient through the torch.autograd.Function agrees with
+    central differences on 32 random directions to 1e-5 relative."""
+    g, reg, prm, x, y = workloads.adjoint_cavity(512)
+    nsteps = 200
+    run = autograd.CavityRun(g, reg, prm, nsteps, checkpoint_every=20)
+    lid_t = workloads.lid_modes(x, y, reg)
+    nu_, nv_ = (g.ny - 1, g.nx - 2), (g.ny - 2, g.nx - 1)
+    zu, zv = np.zeros(nu_), np.zeros(nv_)
+    print("  target run", flush=True)
+    uT, vT = run.run(lid_t[2:g.nx + 1], zu, zv, checkpoints=False)
+    lid = torch.ones(g.nx - 1, dtype=torch.float64, requires_grad=True)
+    u0 = torch.zeros(nu_, dtype=torch.float64, requires_grad=True)
+    v0 = torch.zeros(nv_, dtype=torch.float64, requires_grad=True)
+    uN, vN = autograd.LidRun.apply(lid, u0, v0, run)
+    J = 0.5 * ((uN - torch.from_numpy(uT)) ** 2).sum() + 0.5 * ((vN - torch.from_numpy(vT)) ** 2).sum()
+    J.backward()
+    st = run.last_stats
+    print(f"  J = {J.item():.6e}; forward {st['ms_forward']:.0f} ms, backward recompute {st['ms_recompute']:.0f} ms "
+          f"+ reverse {st['ms_reverse']:.0f} ms; adjoint q-l iterations mean {st['adj_iters_mean']:.1f}, "
+          f"V-cycles mean {st['ppe_cycles_mean']:.1f}", flush=True)
+    assert st["fwd_unconverged"] == 0 and st["adj_unconverged"] == 0
+    gl, gu, gv = lid.grad.numpy(), u0.grad.numpy(), v0.grad.numpy()
+
+    def Jnp(l, a, b):
+        u, v = run.run(l, a, b, checkpoints=False)
+        return 0.5 * np.sum((u - uT) ** 2) + 0.5 * np.sum((v - vT) ** 2)
+
+    rng = np.random.default_rng(400)
+    worst = 0.0
+    for k in range(32):
+        dl = rng.standard_normal(g.nx - 1)
+        du, dv = rng.standard_normal(nu_), rng.standard_normal(nv_)
+        if k % 2 == 0:   # even directions: the lid alone
+            du[:] = 0.0
+            dv[:] = 0.0
+        else:            # odd: the initial velocity alone
+            dl[:] = 0.0
+        eps = 1e-5
+        l0 = np.ones(g.nx - 1)
+        fd = (Jnp(l0 + eps * dl, eps * du, eps * dv) - Jnp(l0 - eps * dl, -eps * du, -eps * dv)) / (2 * eps)
+        ad = float(gl @ dl + np.sum(gu * du) + np.sum(gv * dv))
+        rel = abs(fd - ad) / abs(fd)
+        worst = max(worst, rel)
+        print(f"    direction {k:2d} ({'lid' if k % 2 == 0 else 'u0,v0'}): FD {fd:.10e} adjoint {ad:.10e} "
+              f"rel {rel:.2e}", flush=True)
+    run.close()
+    assert worst < 1e-5

@@ -39,6 +39,14 @@ class StepOptions(C.Structure):
                 ("densref", C.c_double), ("tmax", C.c_double), ("tref", C.c_double), ("rconst", C.c_double)]
 
 
+class AdjStats(C.Structure):
+    """wd2_adj_stats of include/wolfd2_amd.h."""
+    _fields_ = [("steps", C.c_int), ("adj_iters_max", C.c_int), ("adj_iters_mean", C.c_double),
+                ("ppe_cycles_max", C.c_int), ("ppe_cycles_mean", C.c_double), ("fwd_unconverged", C.c_int),
+                ("adj_unconverged", C.c_int), ("ms_forward", C.c_double), ("ms_recompute", C.c_double),
+                ("ms_reverse", C.c_double)]
+
+
 _lib = None
 
 

@@ -21,12 +21,12 @@ import ctypes as C
 import numpy as np
 
 from . import case
-from ._lib import Params, StepOptions, Wd2Error, check, dp, fp, ip, lib
+from ._lib import AdjStats, Params, StepOptions, Wd2Error, check, dp, fp, ip, lib
 
 __all__ = ["vel_bound_cond", "pres_bound_cond", "vel_outflow_bcs", "project", "divergence", "ppe",
            "x_momentum", "y_momentum", "naux_momentum", "alt_trid_lu", "dmax_norm", "diff_max_norm",
            "temp_bound_cond", "therm_energy", "eq_state", "filter",
-           "Solver", "Ensemble", "Wd2Error", "make_params"]
+           "Solver", "Ensemble", "Adjoint", "Wd2Error", "make_params"]
 
 
 def _ld(a: np.ndarray) -> int:
@@ -329,3 +329,58 @@ class Ensemble:
             out.append(t)
         return out
 
+
+
+class Adjoint:
+    """Reverse-mode gradient of a run of a Solver's time steps (wd2_adj_*):
+    ``forward(nsteps)`` advances the solver from its current fields saving a
+    checkpoint every ``checkpoint_every`` steps; ``backward(u_bar, v_bar,
+    p_bar)`` returns the adjoint of the run's initial u, v, p and of the
+    boundary values, given the adjoint of its final fields.  The case must be
+    a one-region cavity with no-slip walls, multigrid Poisson solves, FP64;
+    the gradient is that of the converged q-l loops and Poisson solves."""
+
+    def __init__(self, solver: Solver, checkpoint_every: int = 20):
+        self.solver = solver
+        self.adj = C.c_void_p()
+        check(lib().wd2_adj_create(C.byref(self.adj), solver.ctx, checkpoint_every))
+
+    def close(self):
+        if self.adj:
+            lib().wd2_adj_destroy(self.adj)
+            self.adj = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tolerances(self, fixed_point_tol: float = 0.0, max_iter: int = 0, poisson_tol: float = 0.0) -> None:
+        check(lib().wd2_adj_set_tolerances(self.adj, C.c_double(fixed_point_tol), max_iter,
+                                           C.c_double(poisson_tol)))
+
+    def forward(self, nsteps: int) -> np.ndarray:
+        """Advance nsteps with checkpoints; rows (ql_iter, ppe_iter)."""
+        ql = (C.c_int * nsteps)()
+        pi = (C.c_int * nsteps)()
+        check(lib().wd2_adj_forward(self.adj, C.byref(self.solver.params), nsteps, ql, pi))
+        return np.stack([np.frombuffer(ql, dtype=np.int32), np.frombuffer(pi, dtype=np.int32)], axis=1)
+
+    def backward(self, u_bar, v_bar, p_bar=None) -> dict:
+        """Adjoints of the initial fields (u0_bar, v0_bar, p0_bar: (ncol, ld)
+        arrays), of the boundary values (bc_bar[side-1, var-1, k] for side
+        W, E, S, N, var u, v, grid line k) and the pass's statistics."""
+        g = self.solver.g
+        out = [np.zeros((g.ncol, g.ld)) for _ in range(3)]
+        W = max(g.nx, g.ny) + 2
+        bc = np.zeros(8 * W)
+        st = AdjStats()
+        ub = np.ascontiguousarray(u_bar, dtype=np.float64)
+        vb = np.ascontiguousarray(v_bar, dtype=np.float64)
+        pb = None if p_bar is None else np.ascontiguousarray(p_bar, dtype=np.float64)
+        check(lib().wd2_adj_backward(self.adj, g.ld, dp(ub), dp(vb), dp(pb), dp(out[0]), dp(out[1]), dp(out[2]),
+                                     dp(bc), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in AdjStats._fields_}
+        return {"u0_bar": out[0], "v0_bar": out[1], "p0_bar": out[2], "bc_bar": bc.reshape(4, 2, W),
+                "stats": stats}

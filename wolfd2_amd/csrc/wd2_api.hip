@@ -144,6 +144,9 @@ struct wd2_ctx {
     Scalars *h_sc = nullptr;
     int ql_last = 4, g_q = 0, g_c = 0;
     long graph_steps = 0, graph_redos = 0;
+    // adjoint tape (wd2_adj): where a step copies its converged q-l
+    // iterates us, vs (null: no copy)
+    double *adj_us = nullptr, *adj_vs = nullptr;
     // optional HIP-event timing of one kernel (kt_on: 1 the red-black SOR
     // kernel, 2 / 3 the multigrid level-0 down / up kernel): event pairs,
     // each around kt_cnt[pair] back-to-back launches
@@ -781,7 +784,9 @@ static int mg_cycle_time(wd2_ctx *c, int cycles, double dk, double *us) {
 // nPpeSolver 7: multigrid V-cycles (FP64 or FP32 by the context's precision)
 // in FP64 defect correction, until max |dk (b - A p)| < tol (the divergence
 // Project leaves) or maxcyc cycles; *nconv_out = the V-cycles applied.
-static int ppe_mg(wd2_ctx *c, int cart, int maxcyc, double dk, double tol, double *p, int *nconv_out) {
+// bsrc: the right-hand side (null: the Ppe right-hand side c->rhs / c->beff).
+static int ppe_mg(wd2_ctx *c, int cart, int maxcyc, double dk, double tol, double *p, int *nconv_out,
+                  const double *bsrc = nullptr) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     hipStream_t s = c->s;
     if (c->any_blockage)
@@ -791,7 +796,7 @@ static int ppe_mg(wd2_ctx *c, int cart, int maxcyc, double dk, double tol, doubl
     const bool mx = c->precision == WD2_PREC_MIXED;
     const double *uni = (g_uniform && c->uni_ok) ? c->uni : nullptr;
     launch_mg_reset(c->sc, s);
-    const double *b = cart ? c->rhs : c->beff;
+    const double *b = bsrc ? bsrc : cart ? c->rhs : c->beff;
     const size_t kt0 = c->kt_used;
     // Batches of V-cycles with a stopping check at the end of each, two in
     // flight (as the SOR solves): the first batch is the last solve's cycle
@@ -1049,6 +1054,11 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
         }
         if (therm) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->t, c->ts, s);
         if ((st = naux_momentum(c, P->mqiter, P->dk, P->re, P->fr, P->qtol, nql))) return st;
+        if (c->adj_us) {
+            const double *src[2] = {c->us, c->vs};
+            double *dst[2] = {c->adj_us, c->adj_vs};
+            launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 2, s);
+        }
         // main.f:783-791
         if (O.nfiltu == 1) launch_filter(c->R, c->TH, ld, VU, O.fpu, c->us, c->qh, s);
         if (O.nfiltv == 1) launch_filter(c->R, c->TH, ld, VV, O.fpv, c->vs, c->qh, s);
@@ -1260,6 +1270,247 @@ static int fallbacks(wd2_ctx *c) {
     int n = 0;
     if (hipMemcpy(&n, c->tw.nfallback, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return 0;
     return n;
+}
+
+// ================================================================ adjoint
+// Reverse-mode gradient of a run of time steps (wd2_adj_*), FP64, on a
+// one-region cavity with no-slip walls (boundary profiles allowed), the
+// multigrid Poisson solver and lCartesGrid set.  The map differentiated is
+// main.f's step with each q-l loop and Poisson solve at its converged
+// solution:
+// * q-l loop: the iterates stop where the second split's increment vanishes,
+//   Pa S1(w)^-1 b(w) = 0 (Pa: rows that are not identity rows of the second
+//   split, S1: the first split's matrix with the chain solver's first-row
+//   quirk, b: the assembled right-hand side).  Its adjoint is the fixed point
+//   of nu <- nu + S1^-T Pa S2^-T Pa (G^T nu + wbar), the transpose of the
+//   forward iteration (G = db/dw - dS1/dw x1, x1 = S1^-1 b); the converged
+//   nu gives the adjoints of un, vn and of the points the loop leaves alone;
+// * Ppe: p on the interior cells solves A p = div(us, vs)/dk - (frozen ghost
+//   couplings); A is symmetric on the Cartesian five-point form, so the
+//   adjoint solve A lam = pbar is another multigrid solve;
+// * boundary fills and Project are linear and replayed in reverse order.
+// Checkpoints of (u, v, p) every `every` steps bound the memory; the
+// backward pass re-runs each segment from its checkpoint, keeping (u, v, p)
+// and the converged (us, vs) of its steps, then walks the segment backwards.
+struct wd2_adj {
+    wd2_ctx *c = nullptr;
+    int every = 20, nsteps = 0, nseg_cap = 0;
+    bool have_fwd = false;
+    wd2_params P{};
+    double fp_tol = 1e-12, ppe_tol = 1e-13;
+    int fp_max = 200;
+    int mg_last = 4;   // V-cycles of the last adjoint Poisson solve (first batch size)
+    double *ckpt = nullptr;   // nseg_cap x (u, v, p)
+    double *tape = nullptr;   // every x (u, v, p, us, vs)
+    double *fin = nullptr;    // (u, v, p) after the run
+    // adjoint fields: the state's (ub, vb, pb); scratch tu, tv (w.r.t. us,
+    // vs), nub, nvb (w.r.t. un, vn); the Poisson adjoint lam (zero ghosts)
+    // and its right-hand side; the ConvCoef adjoints co (6 fields)
+    double *fstore = nullptr;
+    double *ub, *vb, *pb, *tu, *tv, *nub, *nvb, *lam, *arhs;
+    AdjCoefs co{};
+    double *bcb = nullptr;    // 8 x W boundary-value adjoints
+    int W = 0;
+    // chain vectors: S1^T and S2^T (A, D, C), nu, r, y, z, coefficient adjoints
+    double *cstore = nullptr;
+    double *At1, *Dt1, *Ct1, *At2, *Dt2, *Ct2, *nu, *r, *y, *z, *a1b, *d1b, *c1b;
+    unsigned long long *red = nullptr, *h_red = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    wd2_adj_stats st{};
+};
+
+static void adj_free(wd2_adj *a) {
+    if (!a) return;
+    (void)hipFree(a->ckpt);
+    (void)hipFree(a->tape);
+    (void)hipFree(a->fin);
+    (void)hipFree(a->fstore);
+    (void)hipFree(a->bcb);
+    (void)hipFree(a->cstore);
+    (void)hipFree(a->red);
+    if (a->h_red) (void)hipHostFree(a->h_red);
+    for (hipEvent_t e : a->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete a;
+}
+
+static int adj_check(const wd2_ctx *c, const wd2_params *P) {
+    if (c->precision != WD2_PREC_FP64) return fail(WD2_ERR_UNSUPPORTED, "adjoint: FP64 precision only");
+    if (c->R.ni != 1 || c->R.nj != 1 || c->h_rtype[0] != RM_INTERN)
+        return fail(WD2_ERR_UNSUPPORTED, "adjoint: one fluid region only");
+    const int want[4] = {1, c->nx, 1, c->ny};
+    for (int side = WEST; side <= NORTH; side++) {
+        if (c->R.border(1, 1, side) != want[side - 1])
+            return fail(WD2_ERR_UNSUPPORTED, "adjoint: the region must cover the grid");
+        if (c->R.btype(1, 1, side) != BM_WALL1)
+            return fail(WD2_ERR_UNSUPPORTED, "adjoint: no-slip walls (type 1) on every side only");
+    }
+    const wd2_step_options &O = c->opt;
+    if (c->d_live || O.nthermen == 1 || O.neqstate == 1 || O.nfiltu == 1 || O.nfiltv == 1)
+        return fail(WD2_ERR_UNSUPPORTED, "adjoint: no energy equation, density or filters");
+    if (P->nPpeSolver != 7 || !P->lCartesGrid || P->mqiter < 1)
+        return fail(WD2_ERR_UNSUPPORTED, "adjoint: nPpeSolver 7, lCartesGrid and max_ql_iter >= 1 only");
+    return WD2_OK;
+}
+
+static int adj_alloc(wd2_adj *a) {
+    wd2_ctx *c = a->c;
+    const size_t nf = c->nf, nch = chain_rows(c->nx, c->ny);
+    HIPCHK(hipMalloc(&a->tape, 5 * (size_t)a->every * nf * sizeof(double)));
+    HIPCHK(hipMalloc(&a->fin, 3 * nf * sizeof(double)));
+    HIPCHK(hipMalloc(&a->fstore, 15 * nf * sizeof(double)));
+    HIPCHK(hipMemsetAsync(a->fstore, 0, 15 * nf * sizeof(double), c->s));
+    double *f = a->fstore;
+    double **fp[9] = {&a->ub, &a->vb, &a->pb, &a->tu, &a->tv, &a->nub, &a->nvb, &a->lam, &a->arhs};
+    for (int k = 0; k < 9; k++) *fp[k] = f + k * nf;
+    a->co = AdjCoefs{f + 9 * nf, f + 10 * nf, f + 11 * nf, f + 12 * nf, f + 13 * nf, f + 14 * nf};
+    a->W = std::max(c->nx, c->ny) + 2;
+    HIPCHK(hipMalloc(&a->bcb, 8 * (size_t)a->W * sizeof(double)));
+    HIPCHK(hipMalloc(&a->cstore, 13 * nch * sizeof(double)));
+    HIPCHK(hipMemsetAsync(a->cstore, 0, 13 * nch * sizeof(double), c->s));
+    double **cp[13] = {&a->At1, &a->Dt1, &a->Ct1, &a->At2, &a->Dt2, &a->Ct2, &a->nu,
+                       &a->r,   &a->y,   &a->z,   &a->a1b, &a->d1b, &a->c1b};
+    for (int k = 0; k < 13; k++) *cp[k] = a->cstore + k * nch;
+    HIPCHK(hipMalloc(&a->red, 2 * sizeof(unsigned long long)));
+    HIPCHK(hipHostMalloc((void **)&a->h_red, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    for (auto &e : a->ev) HIPCHK(hipEventCreate(&e));
+    return WD2_OK;
+}
+
+static void adj_copy3(wd2_ctx *c, const double *u, const double *v, const double *p, double *du, double *dv,
+                      double *dp) {
+    const double *src[3] = {u, v, p};
+    double *dst[3] = {du, dv, dp};
+    launch_copy_set(c->ld, 0, c->nx + 1, 0, c->ny + 1, src, dst, 3, c->s);
+}
+
+// max |x| over n elements of up to two vectors, read back on the host
+static int adj_max2(wd2_adj *a, long n, const double *x0, const double *x1, double *m0, double *m1) {
+    hipStream_t s = a->c->s;
+    HIPCHK(hipMemsetAsync(a->red, 0, 2 * sizeof(unsigned long long), s));
+    launch_adj_maxabs(n, x0, a->red, 0, s);
+    if (x1) launch_adj_maxabs(n, x1, a->red, 1, s);
+    HIPCHK(hipMemcpyAsync(a->h_red, a->red, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::memcpy(m0, &a->h_red[0], sizeof(double));
+    if (m1) std::memcpy(m1, &a->h_red[1], sizeof(double));
+    return WD2_OK;
+}
+
+// tu, tv += G_s^T nu and (with_n) nub, nvb += G_n^T nu at the converged
+// iterate (us, vs) of the step from (un, vn); x1 in the first chain's x
+static void adj_gtrans(wd2_adj *a, const double *us, const double *vs, const double *un, const double *vn,
+                       int with_n) {
+    wd2_ctx *c = a->c;
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    const long n1 = (long)nx * (ny - 1), n2 = (long)(nx - 1) * ny, n = n1 + n2;
+    const Chains cs{n1, n1, n2};
+    launch_adj_coef(n, cs, c->ch.a, c->ch.d, c->ch.c, a->nu, c->ch.x, nullptr, a->a1b, a->d1b, a->c1b, s);
+    for (int comp = 1; comp <= 2; comp++) {
+        const long off = comp == 1 ? 0 : n1;
+        launch_adj_zero(6 * (long)c->nf, a->co.cj1, s);
+        launch_adj_asm(comp, nx, ny, ld, a->P.dk, a->P.re, c->M, us, vs, un, vn, a->a1b + off, a->d1b + off,
+                       a->c1b + off, a->nu + off, with_n, comp == 1 ? a->tu : a->tv, comp == 1 ? a->nub : a->nvb,
+                       a->co, s);
+        launch_adj_cc(comp, nx, ny, ld, c->M, a->co, a->tu, a->tv, a->nub, a->nvb, s);
+    }
+}
+
+// One step backwards: (ub, vb, pb) from the adjoint of the step's end state
+// to that of its start state; bcb accumulates the boundary values' adjoints.
+static int adj_step(wd2_adj *a, const double *un, const double *vn, const double *us, const double *vs) {
+    wd2_ctx *c = a->c;
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    const wd2_params &P = a->P;
+    const long n1 = (long)nx * (ny - 1), n2 = (long)(nx - 1) * ny, n = n1 + n2;
+    const Chains cs{n1, n1, n2};
+    // main.f:946-965 step-end fills, then (in reverse) PresBoundCond,
+    // Project, VelBoundCond, PresBoundCond after the solve
+    launch_adj_pbc(nx, ny, ld, a->pb, s);
+    launch_adj_vbc(c->R, ld, a->ub, a->vb, a->bcb, s);
+    launch_adj_pbc(nx, ny, ld, a->pb, s);
+    launch_adj_vbc(c->R, ld, a->ub, a->vb, a->bcb, s);
+    launch_adj_project(nx, ny, ld, P.dk, c->M, a->ub, a->vb, a->pb, s);
+    launch_adj_pbc(nx, ny, ld, a->pb, s);
+    // Ppe: lam = A^-1 pbar on the interior cells (warm-started from the last
+    // step's lam), then the divergence and the frozen ghosts take it
+    launch_copy(ld, 2, nx, 2, ny, a->pb, a->arhs, s);
+    launch_fill(ld, 2, nx, 2, ny, 0.0, a->pb, s);
+    double bmax = 0.0;
+    int st = adj_max2(a, (long)c->nf, a->arhs, nullptr, &bmax, nullptr);
+    if (st) return st;
+    int ncyc = 0;
+    if (bmax > 0.0) {
+        const int ml = c->mg_last;
+        const double md = c->mg_div, mdm = c->mg_div_max;
+        c->mg_last = a->mg_last;
+        st = ppe_mg(c, 1, P.msorit, 1.0, a->ppe_tol * bmax, a->lam, &ncyc, a->arhs);
+        a->mg_last = c->mg_last;
+        const bool conv = c->mg_div < a->ppe_tol * bmax;
+        c->mg_last = ml;
+        c->mg_div = md;
+        c->mg_div_max = mdm;
+        if (st) return st;
+        if (!conv) a->st.adj_unconverged++;
+        launch_adj_div(nx, ny, ld, P.dk, c->M, a->lam, a->ub, a->vb, a->pb, s);
+    } else {
+        launch_adj_zero((long)c->nf, a->lam, s);
+    }
+    a->st.ppe_cycles_max = std::max(a->st.ppe_cycles_max, ncyc);
+    a->st.ppe_cycles_mean += ncyc;
+    // PresBoundCond and VelBoundCond before the solve
+    launch_adj_pbc(nx, ny, ld, a->pb, s);
+    launch_adj_vbc(c->R, ld, a->ub, a->vb, a->bcb, s);
+    // q-l loop: the chains at the converged iterate, x1 = S1^-1 b
+    MomTemps t = c->T, t2 = c->T;
+    t2.cnvn = c->T.cnvn2;
+    t2.difn = c->T.difn2;
+    auto shift = [](const Chain &q, long o) { return Chain{q.a + o, q.d + o, q.c + o, q.b + o, q.x + o}; };
+    launch_mom_n<double>(nx, ny, ld, 1, c->M, un, vn, t.cnvn, t.difn, s);
+    launch_mom_n<double>(nx, ny, ld, 2, c->M, un, vn, t2.cnvn, t2.difn, s);
+    launch_mom_asm<double>(c->R, ld, 1, P.dk, P.re, P.fr, c->M, t, us, vs, un, vn, c->d, c->dn, c->ch, c->ch2,
+                           c->zm, s, &c->sc->err, 0);
+    launch_mom_asm<double>(c->R, ld, 2, P.dk, P.re, P.fr, c->M, t2, us, vs, un, vn, c->d, c->dn, shift(c->ch, n1),
+                           shift(c->ch2, n1), c->zm + n1, s, &c->sc->err, 0);
+    launch_trid<double>(c->tw, n, cs, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
+    launch_adj_tchain(n, cs, c->ch.a, c->ch.d, c->ch.c, a->At1, a->Dt1, a->Ct1, s);
+    launch_adj_tchain(n, cs, c->ch2.a, c->ch2.d, c->ch2.c, a->At2, a->Dt2, a->Ct2, s);
+    launch_adj_zero(n, a->nu, s);
+    const double *fsrc[2] = {a->ub, a->vb};
+    double *fdst[2] = {a->tu, a->tv};
+    int it = 1;
+    bool conv = false;
+    for (; it <= a->fp_max; it++) {
+        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, fsrc, fdst, 2, s);
+        if (it > 1) adj_gtrans(a, us, vs, un, vn, 0);
+        launch_adj_gather(nx, ny, ld, a->tu, a->tv, a->r, s);
+        launch_adj_zmask(n, c->zm, a->r, a->r, s);
+        launch_trid<double>(c->tw, n, cs, a->At2, a->Dt2, a->Ct2, a->r, a->y, s);
+        launch_adj_zmask(n, c->zm, a->y, a->y, s);
+        launch_trid<double>(c->tw, n, cs, a->At1, a->Dt1, a->Ct1, a->y, a->z, s);
+        launch_adj_axpy(n, 1.0, a->z, a->nu, s);
+        double zmax = 0.0, numax = 0.0;
+        if ((st = adj_max2(a, n, a->z, a->nu, &zmax, &numax))) return st;
+        if (zmax <= a->fp_tol * numax) {
+            conv = true;
+            break;
+        }
+    }
+    if (!conv) {
+        a->st.adj_unconverged++;
+        it = a->fp_max;
+    }
+    a->st.adj_iters_max = std::max(a->st.adj_iters_max, it);
+    a->st.adj_iters_mean += it;
+    // the adjoints of un, vn and of the points the loop leaves at un, vn
+    launch_copy_set(ld, 0, nx + 1, 0, ny + 1, fsrc, fdst, 2, s);
+    launch_adj_zero(2 * (long)c->nf, a->nub, s);   // nub, nvb are adjacent
+    adj_gtrans(a, us, vs, un, vn, 1);
+    launch_adj_combine(1, nx, ny, ld, a->tu, a->nub, a->ub, s);
+    launch_adj_combine(2, nx, ny, ld, a->tv, a->nvb, a->vb, s);
+    return check_err(c);
 }
 
 // ================================================================ C ABI
@@ -2010,6 +2261,140 @@ int wd2_ctx_kernel_timing(wd2_ctx *c, int enable, double *avg_us, long *launches
     c->kt_cnt.clear();
     if (avg_us) *avg_us = n ? 1e3 * tot / n : 0.0;
     if (launches) *launches = n;
+    return WD2_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------- adjoint C ABI
+extern "C" {
+
+int wd2_adj_create(wd2_adj **out, wd2_ctx *c, int every) {
+    if (!out || !c) return fail(WD2_ERR_ARG, "null argument");
+    if (every < 1) return fail(WD2_ERR_ARG, "checkpoint interval %d < 1", every);
+    wd2_adj *a = new wd2_adj;
+    a->c = c;
+    a->every = every;
+    int st = adj_alloc(a);
+    if (st) {
+        adj_free(a);
+        return st;
+    }
+    *out = a;
+    return WD2_OK;
+}
+
+int wd2_adj_destroy(wd2_adj *a) {
+    adj_free(a);
+    return WD2_OK;
+}
+
+int wd2_adj_set_tolerances(wd2_adj *a, double fp_tol, int max_iter, double ppe_tol) {
+    if (!a) return fail(WD2_ERR_ARG, "null argument");
+    if (fp_tol > 0.0) a->fp_tol = fp_tol;
+    if (max_iter > 0) a->fp_max = max_iter;
+    if (ppe_tol > 0.0) a->ppe_tol = ppe_tol;
+    return WD2_OK;
+}
+
+int wd2_adj_forward(wd2_adj *a, const wd2_params *P, int nsteps, int *ql_iter, int *ppe_iter) {
+    if (!a || !P) return fail(WD2_ERR_ARG, "null argument");
+    if (nsteps < 1) return fail(WD2_ERR_ARG, "nsteps %d < 1", nsteps);
+    wd2_ctx *c = a->c;
+    int st = adj_check(c, P);
+    if (st) return st;
+    const int nseg = (nsteps + a->every - 1) / a->every;
+    if (nseg > a->nseg_cap) {
+        (void)hipFree(a->ckpt);
+        a->ckpt = nullptr;
+        HIPCHK(hipMalloc(&a->ckpt, 3 * (size_t)nseg * c->nf * sizeof(double)));
+        a->nseg_cap = nseg;
+    }
+    a->P = *P;
+    a->have_fwd = false;
+    a->st.fwd_unconverged = 0;
+    HIPCHK(hipEventRecord(a->ev[0], c->s));
+    for (int k = 0; k < nsteps; k++) {
+        if (k % a->every == 0) {
+            double *ck = a->ckpt + 3 * (size_t)(k / a->every) * c->nf;
+            adj_copy3(c, c->u, c->v, c->p, ck, ck + c->nf, ck + 2 * c->nf);
+        }
+        int nql = 0, nsor = 0;
+        if ((st = one_step(c, P, &nql, &nsor, nullptr))) return st;
+        if ((P->qtol > 0.0 && nql < 0) || !(c->mg_div < P->sortol)) a->st.fwd_unconverged++;
+        if (ql_iter) ql_iter[k] = nql;
+        if (ppe_iter) ppe_iter[k] = nsor;
+    }
+    adj_copy3(c, c->u, c->v, c->p, a->fin, a->fin + c->nf, a->fin + 2 * c->nf);
+    HIPCHK(hipEventRecord(a->ev[1], c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
+    a->st.ms_forward = ms;
+    a->nsteps = nsteps;
+    a->have_fwd = true;
+    return WD2_OK;
+}
+
+int wd2_adj_backward(wd2_adj *a, int ld_h, const double *u_bar, const double *v_bar, const double *p_bar,
+                     double *u0_bar, double *v0_bar, double *p0_bar, double *bc_bar, wd2_adj_stats *stats) {
+    if (!a || !u_bar || !v_bar) return fail(WD2_ERR_ARG, "null argument");
+    if (!a->have_fwd) return fail(WD2_ERR_ARG, "wd2_adj_backward before wd2_adj_forward");
+    wd2_ctx *c = a->c;
+    if (ld_h < c->nx + 2) return fail(WD2_ERR_ARG, "ld=%d < nx+2=%d", ld_h, c->nx + 2);
+    int st = adj_check(c, &a->P);
+    if (st) return st;
+    hipStream_t s = c->s;
+    const size_t nf = c->nf;
+    launch_adj_zero(3 * (long)nf, a->ub, s);   // ub, vb, pb are adjacent
+    if ((st = up(c, u_bar, ld_h, a->ub)) || (st = up(c, v_bar, ld_h, a->vb))) return st;
+    if (p_bar && (st = up(c, p_bar, ld_h, a->pb))) return st;
+    launch_adj_zero(8 * (long)a->W, a->bcb, s);
+    launch_adj_zero((long)nf, a->lam, s);
+    const int fu = a->st.fwd_unconverged;
+    const double msf = a->st.ms_forward;
+    a->st = wd2_adj_stats{};
+    a->st.fwd_unconverged = fu;
+    a->st.ms_forward = msf;
+    const int nseg = (a->nsteps + a->every - 1) / a->every;
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        const int k0 = seg * a->every, k1 = std::min(a->nsteps, k0 + a->every);
+        const double *ck = a->ckpt + 3 * (size_t)seg * nf;
+        HIPCHK(hipEventRecord(a->ev[0], s));
+        adj_copy3(c, ck, ck + nf, ck + 2 * nf, c->u, c->v, c->p);
+        for (int k = k0; k < k1; k++) {
+            double *tp = a->tape + 5 * (size_t)(k - k0) * nf;
+            adj_copy3(c, c->u, c->v, c->p, tp, tp + nf, tp + 2 * nf);
+            c->adj_us = tp + 3 * nf;
+            c->adj_vs = tp + 4 * nf;
+            int nql = 0, nsor = 0;
+            st = one_step(c, &a->P, &nql, &nsor, nullptr);
+            c->adj_us = c->adj_vs = nullptr;
+            if (st) return st;
+        }
+        HIPCHK(hipEventRecord(a->ev[1], s));
+        for (int k = k1 - 1; k >= k0; k--) {
+            const double *tp = a->tape + 5 * (size_t)(k - k0) * nf;
+            if ((st = adj_step(a, tp, tp + nf, tp + 3 * nf, tp + 4 * nf))) return st;
+        }
+        HIPCHK(hipEventRecord(a->ev[2], s));
+        HIPCHK(hipStreamSynchronize(s));
+        float m1 = 0.f, m2 = 0.f;
+        HIPCHK(hipEventElapsedTime(&m1, a->ev[0], a->ev[1]));
+        HIPCHK(hipEventElapsedTime(&m2, a->ev[1], a->ev[2]));
+        a->st.ms_recompute += m1;
+        a->st.ms_reverse += m2;
+    }
+    adj_copy3(c, a->fin, a->fin + nf, a->fin + 2 * nf, c->u, c->v, c->p);
+    a->st.steps = a->nsteps;
+    a->st.adj_iters_mean /= a->nsteps;
+    a->st.ppe_cycles_mean /= a->nsteps;
+    if (u0_bar && (st = down(c, a->ub, u0_bar, ld_h))) return st;
+    if (v0_bar && (st = down(c, a->vb, v0_bar, ld_h))) return st;
+    if (p0_bar && (st = down(c, a->pb, p0_bar, ld_h))) return st;
+    if (bc_bar) HIPCHK(hipMemcpy(bc_bar, a->bcb, 8 * (size_t)a->W * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipStreamSynchronize(s));
+    if (stats) *stats = a->st;
     return WD2_OK;
 }
 

@@ -265,4 +265,42 @@ void launch_mg_kernel_bench(const MgHierarchy &H, int which, int reps, int nx, i
                             double *p, const double *rau, const double *rgv, const double *uni, double *part,
                             Scalars *sc, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 
+// Adjoint kernels (k_adjoint.hip), FP64.  See that file for the maps each
+// one transposes.  AdjCoefs: adjoints of one component's ConvCoef outputs
+// (the upwind Jacobian's cj1, cj2 and the centred fluxes of us and un).
+struct AdjCoefs {
+    double *cj1, *cj2, *c1s, *c2s, *c1n, *c2n;
+};
+// The transposed system of a chain solve, in the form the chain solver
+// (with its first-row quirk) solves exactly.
+void launch_adj_tchain(long n, const Chains &cs, const double *A, const double *D, const double *C, double *At,
+                       double *Dt, double *Ct, hipStream_t s);
+// Coefficient adjoints of S x = b given mu = S^-T xbar (Z: rows without).
+void launch_adj_coef(long n, const Chains &cs, const double *A, const double *D, const double *C, const double *mu,
+                     const double *x, const unsigned char *Z, double *Ab, double *Db, double *Cb, hipStream_t s);
+// Chain-ordered vector (X rows, then Y rows) from two fields.
+void launch_adj_gather(int nx, int ny, int ld, const double *ub, const double *vb, double *lam, hipStream_t s);
+void launch_adj_zmask(long n, const unsigned char *Z, const double *in, double *out, hipStream_t s);
+// One component's momentum assembly transposed (rows of that component's
+// chain); with_n = 0 skips the time-level-n terms.
+void launch_adj_asm(int comp, int nx, int ny, int ld, double dk, double re, const Metrics &m, const double *us,
+                    const double *vs, const double *un, const double *vn, const double *a1b, const double *d1b,
+                    const double *c1b, const double *bb, int with_n, double *fsb, double *fnb, const AdjCoefs &A,
+                    hipStream_t s);
+void launch_adj_cc(int comp, int nx, int ny, int ld, const Metrics &m, const AdjCoefs &A, double *usb, double *vsb,
+                   double *unb, double *vnb, hipStream_t s);
+// VelBoundCond / PresBoundCond transposed (one region, no-slip walls);
+// bcb: 8 lines of max(nx,ny)+2 values, [side-1][var-1][k].
+void launch_adj_vbc(const Regions &R, int ld, double *ub, double *vb, double *bcb, hipStream_t s);
+void launch_adj_pbc(int nx, int ny, int ld, double *pb, hipStream_t s);
+void launch_adj_project(int nx, int ny, int ld, double dk, const Metrics &m, const double *ub, const double *vb,
+                        double *pb, hipStream_t s);
+void launch_adj_div(int nx, int ny, int ld, double dk, const Metrics &m, const double *lam, double *ub, double *vb,
+                    double *pb, hipStream_t s);
+void launch_adj_axpy(long n, double alpha, const double *x, double *y, hipStream_t s);
+void launch_adj_zero(long n, double *y, hipStream_t s);
+void launch_adj_combine(int comp, int nx, int ny, int ld, const double *fsb, const double *fnb, double *out,
+                        hipStream_t s);
+void launch_adj_maxabs(long n, const double *x, unsigned long long *red, int slot, hipStream_t s);
+
 }  // namespace wd2

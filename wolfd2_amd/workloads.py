@@ -99,3 +99,47 @@ def ensemble_specs(n: int = 64, seed: int = 20261202, sides=tuple(range(128, 769
     side = rng.choice(np.asarray(sides), size=n)
     re = rng.uniform(re_range[0], re_range[1], size=n)
     return [(int(a), float(b)) for a, b in zip(side, re)]
+
+
+def lid_modes(x, y, reg, n_modes: int = 16, sigma: float = 0.1, seed: int = 20261203, base: float = 1.0):
+    """A sinusoidal lid profile: base + sum_m a_m sin(m pi s), m = 1..n_modes,
+    a_m ~ N(0, sigma^2) (seeded), s the fraction of the lid's length at each
+    u line of the north side (case.side_positions); returned for i = 0..nx+1
+    (the lid values are i = 2..nx)."""
+    s = case.side_positions(x, y, reg, 1, 1, case.NORTH, case.VAR_U)
+    a = np.random.default_rng(seed).normal(0.0, sigma, size=n_modes)
+    m = np.arange(1, n_modes + 1)
+    return base + np.sin(np.pi * np.outer(s, m)) @ a
+
+
+def adjoint_cavity(n: int = 512, re: float = 400.0, dt: float = 2e-4, qtol: float = -1.0, mqiter: int = 24,
+                   sortol: float = 1e-13, msorit: int = 100):
+    """The adjoint case: an n x n cavity (Re 400), FP64, every q-l loop run
+    for mqiter iterations (a negative qtol never stops a loop; the increments
+    reach round-off in 24 iterations at this dt) and multigrid Poisson solves
+    converged tightly, so the run is the converged map the adjoint
+    differentiates; returns (g, reg, prm, x, y).
+
+    No q-l tolerance is used because the reference's stopping norm
+    (DiffMaxNorm, utility.f) skips the rows next to the north and south
+    walls: in the first steps of an impulsively started lid the increments
+    sit on row ny (from rest they are exactly zero elsewhere, the wall rows
+    of the second split being identity rows), so the loop stops after one
+    iteration with that row unconverged, even at qtol 0."""
+    nx = ny = n + 1
+    x, y = case.uniform_nodes(nx, ny)
+    g = case.compute_grid(x, y)
+    reg = case.cavity_regions(nx, ny, lid=1.0)
+    prm = case.Params.from_physical(dt, 1.0, 1.0 / re, nPpeSolver=7, msorit=msorit, sortol=sortol, qtol=qtol,
+                                    mqiter=mqiter)
+    return g, reg, prm, x, y
+
+
+def interior_u(g):
+    """The u points a step updates (i = 2..nx-1, j = 2..ny) as a [j, i] slice."""
+    return np.s_[2:g.ny + 1, 2:g.nx]
+
+
+def interior_v(g):
+    """The v points a step updates (i = 2..nx, j = 2..ny-1) as a [j, i] slice."""
+    return np.s_[2:g.ny, 2:g.nx + 1]
