@@ -74,24 +74,31 @@ __global__ __launch_bounds__(AT) void k_adj_tchain(long n, Chains cs, const doub
     Ct[k] = sup;
 }
 
-// Coefficient adjoints of a solve S x = b with S^T mu = xbar solved: rows
-// flagged in Z (identity rows, constant coefficients) get none.
+// Coefficient adjoints -mu_k x_{k-1}, -mu_k x_k, -mu_k x_{k+1} of the
+// matrix S of a solve S x = b (S with the chain solver's first-row quirk
+// S(1,2) = c1 d1 / d2).  Z (optional): rows whose coefficients are constants
+// (identity rows) get none.  XK (optional): x is taken as +0 on the rows not
+// flagged in XK -- at a converged q-l fixed point the first split's x
+// vanishes on every row but the second split's identity rows, and only
+// those rows' columns of S carry a derivative.
 __global__ __launch_bounds__(AT) void k_adj_coef(long n, Chains cs, const double *__restrict__ A,
                                                  const double *__restrict__ D, const double *__restrict__ C,
                                                  const double *__restrict__ mu, const double *__restrict__ x,
-                                                 const unsigned char *__restrict__ Z, double *__restrict__ Ab,
+                                                 const unsigned char *__restrict__ Z,
+                                                 const unsigned char *__restrict__ XK, double *__restrict__ Ab,
                                                  double *__restrict__ Db, double *__restrict__ Cb) {
     const long k = (long)blockIdx.x * AT + threadIdx.x;
     if (k >= n) return;
     long L;
     const long r = apos(cs, k, L);
+    auto xv = [&](long q) { return (XK && !XK[q]) ? 0.0 : x[q]; };
     double ab = 0.0, db = 0.0, cb = 0.0;
     if (!(Z && Z[k])) {
         const double m = mu[k];
-        db = -m * x[k];
-        if (r > 0) ab = -m * x[k - 1];
+        db = -m * xv(k);
+        if (r > 0) ab = -m * xv(k - 1);
         if (r < L - 1) {
-            const double sb = -m * x[k + 1];   // adjoint of the entry S(k, k+1)
+            const double sb = -m * xv(k + 1);   // adjoint of the entry S(k, k+1)
             if (r == 0) {   // S(1,2) = c1 d1 / d2
                 cb = sb * D[k] / D[k + 1];
                 db += sb * C[k] / D[k + 1];
@@ -101,7 +108,7 @@ __global__ __launch_bounds__(AT) void k_adj_coef(long n, Chains cs, const double
         }
     }
     // row 2's diagonal also enters S(1,2): d(c1 d1 / d2)/d d2 = -c1 d1 / d2^2
-    if (r == 1 && !(Z && Z[k - 1])) db += mu[k - 1] * x[k] * C[k - 1] * D[k - 1] / (D[k] * D[k]);
+    if (r == 1 && !(Z && Z[k - 1])) db += mu[k - 1] * xv(k) * C[k - 1] * D[k - 1] / (D[k] * D[k]);
     Ab[k] = ab;
     Db[k] = db;
     Cb[k] = cb;
@@ -109,10 +116,21 @@ __global__ __launch_bounds__(AT) void k_adj_coef(long n, Chains cs, const double
 
 // lam in chain order from the adjoints of the increments (X rows i = 1..nx,
 // j = 2..ny, then Y rows i = 2..nx, j = 1..ny).
+// Rows flagged in Z (the second split's identity rows; null: none) get +0.
+// Thread 0 also clears red[slot], red[slot + 1] (red null: none).
 __global__ __launch_bounds__(AT) void k_adj_gather(int nx, int ny, int ld, const double *__restrict__ ub,
-                                                   const double *__restrict__ vb, double *__restrict__ lam) {
+                                                   const double *__restrict__ vb, const unsigned char *__restrict__ Z,
+                                                   double *__restrict__ lam, unsigned long long *red, int slot) {
     const long n1 = (long)nx * (ny - 1), n2 = (long)(nx - 1) * ny;
     const long k = (long)blockIdx.x * AT + threadIdx.x;
+    if (red && k == 0) {
+        red[slot] = 0ull;
+        red[slot + 1] = 0ull;
+    }
+    if (Z && k < n1 + n2 && Z[k]) {
+        lam[k] = 0.0;
+        return;
+    }
     if (k < n1) {
         const int i = 1 + (int)(k % nx), j = 2 + (int)(k / nx);
         lam[k] = ub[WD2_IX(i, j)];
@@ -335,18 +353,30 @@ __global__ __launch_bounds__(AT) void k_adj_asm_y(int nx, int ny, int ld, double
 
 // The ConvCoef outputs' adjoints of one component onto the fields (every
 // point of the written ranges; entries outside them were never written and
-// their adjoints are dropped).
+// their adjoints are dropped).  Each thread clears the adjoints it reads, so
+// A is all zero again for the next assembly transpose (with_n = 0: the
+// time-level-n ones are zero and not read).
 __global__ __launch_bounds__(AT) void k_adj_cc(int comp, int nx, int ny, int ld, Metrics m, AdjCoefs A,
-                                               double *usb, double *vsb, double *unb, double *vnb) {
+                                               int with_n, double *usb, double *vsb, double *unb, double *vnb) {
     const int W = nx + 2;
     const long q = (long)blockIdx.x * AT + threadIdx.x;
     if (q >= (long)W * (ny + 2)) return;
     const int i = (int)(q % W), j = (int)(q / W);
     const size_t p = WD2_IX(i, j);
+    const double gj1 = A.cj1[p], gj2 = A.cj2[p];
+    const double gc[2][2] = {{A.c1s[p], A.c2s[p]}, {with_n ? A.c1n[p] : 0.0, with_n ? A.c2n[p] : 0.0}};
+    A.cj1[p] = 0.0;
+    A.cj2[p] = 0.0;
+    A.c1s[p] = 0.0;
+    A.c2s[p] = 0.0;
+    if (with_n) {
+        A.c1n[p] = 0.0;
+        A.c2n[p] = 0.0;
+    }
     if (comp == 1) {
         // case 4 (i = 0..nx, j = 1..ny): cj1 = 2 yeu u - xeu (4 v)/4, cj2 = xzu (4 v)/4 - 2 yzu u
         if (i <= nx && j >= 1 && j <= ny) {
-            const double g1 = A.cj1[p], g2 = A.cj2[p];
+            const double g1 = gj1, g2 = gj2;
             aadd(&usb[p], 2.0 * m.yeu[p] * g1 - 2.0 * m.yzu[p] * g2);
             const double gv = (-m.xeu[p] * g1 + m.xzu[p] * g2) / 4.0;
             if (gv != 0.0) {
@@ -357,9 +387,9 @@ __global__ __launch_bounds__(AT) void k_adj_cc(int comp, int nx, int ny, int ld,
             }
         }
         // case 1: c1 (i = 1..nx+1, j = 1..ny+1), c2 (i = 1..nx, j = 1..ny)
-        for (int lev = 0; lev < 2; lev++) {
+        for (int lev = 0; lev < 1 + with_n; lev++) {
             double *ub = lev ? unb : usb, *vb = lev ? vnb : vsb;
-            const double g1 = lev ? A.c1n[p] : A.c1s[p], g2 = lev ? A.c2n[p] : A.c2s[p];
+            const double g1 = gc[lev][0], g2 = gc[lev][1];
             if (i >= 1 && j >= 1) {
                 aadd(&ub[WD2_IX(i, j)], 0.5 * m.yec[p] * g1);
                 aadd(&ub[WD2_IX(i - 1, j)], 0.5 * m.yec[p] * g1);
@@ -376,7 +406,7 @@ __global__ __launch_bounds__(AT) void k_adj_cc(int comp, int nx, int ny, int ld,
     } else {
         // case 5 (i = 1..nx, j = 0..ny): cj1 = yev (4 u)/4 - 2 xev v, cj2 = 2 xzv v - yzv (4 u)/4
         if (i >= 1 && i <= nx && j <= ny) {
-            const double g1 = A.cj1[p], g2 = A.cj2[p];
+            const double g1 = gj1, g2 = gj2;
             aadd(&vsb[p], -2.0 * m.xev[p] * g1 + 2.0 * m.xzv[p] * g2);
             const double gu = (m.yev[p] * g1 - m.yzv[p] * g2) / 4.0;
             if (gu != 0.0) {
@@ -387,9 +417,9 @@ __global__ __launch_bounds__(AT) void k_adj_cc(int comp, int nx, int ny, int ld,
             }
         }
         // case 2: c1 (i = 1..nx, j = 1..ny), c2 (i = 1..nx+1, j = 1..ny+1)
-        for (int lev = 0; lev < 2; lev++) {
+        for (int lev = 0; lev < 1 + with_n; lev++) {
             double *ub = lev ? unb : usb, *vb = lev ? vnb : vsb;
-            const double g1 = lev ? A.c1n[p] : A.c1s[p], g2 = lev ? A.c2n[p] : A.c2s[p];
+            const double g1 = gc[lev][0], g2 = gc[lev][1];
             if (i >= 1 && i <= nx && j >= 1 && j <= ny) {
                 aadd(&ub[WD2_IX(i, j + 1)], 0.5 * m.yen[p] * g1);
                 aadd(&ub[WD2_IX(i, j)], 0.5 * m.yen[p] * g1);
@@ -556,6 +586,38 @@ __global__ __launch_bounds__(AT) void k_adj_div(int nx, int ny, int ld, double d
     aadd(&ub[WD2_IX(i - 1, j - 1)], gu);
 }
 
+// nu += z over n elements with red[slot] = max |z| and red[slot + 1] =
+// max |nu| (the slots start at +0; a NaN wins).
+__global__ __launch_bounds__(AT) void k_adj_update(long n, const double *__restrict__ z, double *__restrict__ nu,
+                                                   unsigned long long *red, int slot) {
+    __shared__ double sh[2][AT / 64];
+    double mz = 0.0, mn = 0.0;
+    for (long k = (long)blockIdx.x * AT + threadIdx.x; k < n; k += (long)gridDim.x * AT) {
+        const double zk = z[k], v = nu[k] + zk;
+        nu[k] = v;
+        if (!(fabs(zk) <= mz)) mz = fabs(zk);
+        if (!(fabs(v) <= mn)) mn = fabs(v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double a = __shfl_xor(mz, off), b = __shfl_xor(mn, off);
+        if (!(a <= mz)) mz = a;
+        if (!(b <= mn)) mn = b;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = mz;
+        sh[1][threadIdx.x >> 6] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < AT / 64; w++) {
+            if (!(sh[0][w] <= mz)) mz = sh[0][w];
+            if (!(sh[1][w] <= mn)) mn = sh[1][w];
+        }
+        atomicMax(&red[slot], dbits(mz));
+        atomicMax(&red[slot + 1], dbits(mn));
+    }
+}
+
 // y += alpha x and y = 0 over n elements.
 __global__ __launch_bounds__(AT) void k_adj_axpy(long n, double alpha, const double *__restrict__ x, double *y) {
     const long k = (long)blockIdx.x * AT + threadIdx.x;
@@ -612,12 +674,18 @@ void launch_adj_tchain(long n, const Chains &cs, const double *A, const double *
     hipLaunchKernelGGL(k_adj_tchain, ag(n), dim3(AT), 0, s, n, cs, A, D, C, At, Dt, Ct);
 }
 void launch_adj_coef(long n, const Chains &cs, const double *A, const double *D, const double *C, const double *mu,
-                     const double *x, const unsigned char *Z, double *Ab, double *Db, double *Cb, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_coef, ag(n), dim3(AT), 0, s, n, cs, A, D, C, mu, x, Z, Ab, Db, Cb);
+                     const double *x, const unsigned char *Z, const unsigned char *XK, double *Ab, double *Db,
+                     double *Cb, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_coef, ag(n), dim3(AT), 0, s, n, cs, A, D, C, mu, x, Z, XK, Ab, Db, Cb);
 }
-void launch_adj_gather(int nx, int ny, int ld, const double *ub, const double *vb, double *lam, hipStream_t s) {
+void launch_adj_gather(int nx, int ny, int ld, const double *ub, const double *vb, const unsigned char *Z,
+                       double *lam, unsigned long long *red, int slot, hipStream_t s) {
     const long n = (long)nx * (ny - 1) + (long)(nx - 1) * ny;
-    hipLaunchKernelGGL(k_adj_gather, ag(n), dim3(AT), 0, s, nx, ny, ld, ub, vb, lam);
+    hipLaunchKernelGGL(k_adj_gather, ag(n), dim3(AT), 0, s, nx, ny, ld, ub, vb, Z, lam, red, slot);
+}
+void launch_adj_update(long n, const double *z, double *nu, unsigned long long *red, int slot, hipStream_t s) {
+    const long nb = std::min<long>(1024, (n + AT - 1) / AT);
+    hipLaunchKernelGGL(k_adj_update, dim3((unsigned)std::max<long>(nb, 1)), dim3(AT), 0, s, n, z, nu, red, slot);
 }
 void launch_adj_zmask(long n, const unsigned char *Z, const double *in, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_zmask, ag(n), dim3(AT), 0, s, n, Z, in, out);
@@ -633,10 +701,10 @@ void launch_adj_asm(int comp, int nx, int ny, int ld, double dk, double re, cons
         hipLaunchKernelGGL(k_adj_asm_y, ag((long)(nx - 1) * ny), dim3(AT), 0, s, nx, ny, ld, dk, re, m, us, vs, un,
                            vn, a1b, d1b, c1b, bb, with_n, fsb, fnb, A);
 }
-void launch_adj_cc(int comp, int nx, int ny, int ld, const Metrics &m, const AdjCoefs &A, double *usb, double *vsb,
-                   double *unb, double *vnb, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_cc, ag((long)(nx + 2) * (ny + 2)), dim3(AT), 0, s, comp, nx, ny, ld, m, A, usb, vsb,
-                       unb, vnb);
+void launch_adj_cc(int comp, int nx, int ny, int ld, const Metrics &m, const AdjCoefs &A, int with_n, double *usb,
+                   double *vsb, double *unb, double *vnb, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_cc, ag((long)(nx + 2) * (ny + 2)), dim3(AT), 0, s, comp, nx, ny, ld, m, A, with_n, usb,
+                       vsb, unb, vnb);
 }
 void launch_adj_vbc(const Regions &R, int ld, double *ub, double *vb, double *bcb, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_vbc, dim3(1), dim3(1024), 0, s, R, ld, ub, vb, bcb);

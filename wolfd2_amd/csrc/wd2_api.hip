@@ -1314,8 +1314,11 @@ struct wd2_adj {
     // chain vectors: S1^T and S2^T (A, D, C), nu, r, y, z, coefficient adjoints
     double *cstore = nullptr;
     double *At1, *Dt1, *Ct1, *At2, *Dt2, *Ct2, *nu, *r, *y, *z, *a1b, *d1b, *c1b;
+    // maxima of the adjoint q-l updates (two slots per iteration parity) and
+    // their pinned copies; qev: the events after each parity's copy
     unsigned long long *red = nullptr, *h_red = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t qev[2] = {nullptr, nullptr};
     wd2_adj_stats st{};
 };
 
@@ -1330,6 +1333,8 @@ static void adj_free(wd2_adj *a) {
     (void)hipFree(a->red);
     if (a->h_red) (void)hipHostFree(a->h_red);
     for (hipEvent_t e : a->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : a->qev)
         if (e) (void)hipEventDestroy(e);
     delete a;
 }
@@ -1371,9 +1376,10 @@ static int adj_alloc(wd2_adj *a) {
     double **cp[13] = {&a->At1, &a->Dt1, &a->Ct1, &a->At2, &a->Dt2, &a->Ct2, &a->nu,
                        &a->r,   &a->y,   &a->z,   &a->a1b, &a->d1b, &a->c1b};
     for (int k = 0; k < 13; k++) *cp[k] = a->cstore + k * nch;
-    HIPCHK(hipMalloc(&a->red, 2 * sizeof(unsigned long long)));
-    HIPCHK(hipHostMalloc((void **)&a->h_red, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&a->red, 4 * sizeof(unsigned long long)));
+    HIPCHK(hipHostMalloc((void **)&a->h_red, 4 * sizeof(unsigned long long), hipHostMallocDefault));
     for (auto &e : a->ev) HIPCHK(hipEventCreate(&e));
+    for (auto &e : a->qev) HIPCHK(hipEventCreate(&e));
     return WD2_OK;
 }
 
@@ -1406,14 +1412,14 @@ static void adj_gtrans(wd2_adj *a, const double *us, const double *vs, const dou
     hipStream_t s = c->s;
     const long n1 = (long)nx * (ny - 1), n2 = (long)(nx - 1) * ny, n = n1 + n2;
     const Chains cs{n1, n1, n2};
-    launch_adj_coef(n, cs, c->ch.a, c->ch.d, c->ch.c, a->nu, c->ch.x, nullptr, a->a1b, a->d1b, a->c1b, s);
+    launch_adj_coef(n, cs, c->ch.a, c->ch.d, c->ch.c, a->nu, c->ch.x, nullptr, c->zm, a->a1b, a->d1b, a->c1b, s);
     for (int comp = 1; comp <= 2; comp++) {
         const long off = comp == 1 ? 0 : n1;
-        launch_adj_zero(6 * (long)c->nf, a->co.cj1, s);
+        // co is all zero here: k_adj_cc clears what it reads
         launch_adj_asm(comp, nx, ny, ld, a->P.dk, a->P.re, c->M, us, vs, un, vn, a->a1b + off, a->d1b + off,
                        a->c1b + off, a->nu + off, with_n, comp == 1 ? a->tu : a->tv, comp == 1 ? a->nub : a->nvb,
                        a->co, s);
-        launch_adj_cc(comp, nx, ny, ld, c->M, a->co, a->tu, a->tv, a->nub, a->nvb, s);
+        launch_adj_cc(comp, nx, ny, ld, c->M, a->co, with_n, a->tu, a->tv, a->nub, a->nvb, s);
     }
 }
 
@@ -1480,30 +1486,45 @@ static int adj_step(wd2_adj *a, const double *un, const double *vn, const double
     launch_adj_zero(n, a->nu, s);
     const double *fsrc[2] = {a->ub, a->vb};
     double *fdst[2] = {a->tu, a->tv};
-    int it = 1;
+    // Pipelined by one iteration, as the forward q-l loop: iteration m is
+    // queued before the host reads iteration m-1's maxima; the extra
+    // iteration after the test is met only refines nu.
+    auto converged = [&](int m) {
+        double zmax, numax;
+        std::memcpy(&zmax, &a->h_red[2 * (m % 2)], sizeof(double));
+        std::memcpy(&numax, &a->h_red[2 * (m % 2) + 1], sizeof(double));
+        return zmax <= a->fp_tol * numax;
+    };
+    int m = 1;
     bool conv = false;
-    for (; it <= a->fp_max; it++) {
+    for (; m <= a->fp_max; m++) {
+        const int sl = 2 * (m % 2);
         launch_copy_set(ld, 0, nx + 1, 0, ny + 1, fsrc, fdst, 2, s);
-        if (it > 1) adj_gtrans(a, us, vs, un, vn, 0);
-        launch_adj_gather(nx, ny, ld, a->tu, a->tv, a->r, s);
-        launch_adj_zmask(n, c->zm, a->r, a->r, s);
+        if (m > 1) adj_gtrans(a, us, vs, un, vn, 0);
+        launch_adj_gather(nx, ny, ld, a->tu, a->tv, c->zm, a->r, a->red, sl, s);
         launch_trid<double>(c->tw, n, cs, a->At2, a->Dt2, a->Ct2, a->r, a->y, s);
         launch_adj_zmask(n, c->zm, a->y, a->y, s);
         launch_trid<double>(c->tw, n, cs, a->At1, a->Dt1, a->Ct1, a->y, a->z, s);
-        launch_adj_axpy(n, 1.0, a->z, a->nu, s);
-        double zmax = 0.0, numax = 0.0;
-        if ((st = adj_max2(a, n, a->z, a->nu, &zmax, &numax))) return st;
-        if (zmax <= a->fp_tol * numax) {
-            conv = true;
-            break;
+        launch_adj_update(n, a->z, a->nu, a->red, sl, s);
+        HIPCHK(hipMemcpyAsync(&a->h_red[sl], &a->red[sl], 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              s));
+        HIPCHK(hipEventRecord(a->qev[m % 2], s));
+        if (m >= 2) {
+            HIPCHK(hipEventSynchronize(a->qev[(m - 1) % 2]));
+            if (converged(m - 1)) {
+                conv = true;
+                break;
+            }
         }
     }
     if (!conv) {
-        a->st.adj_unconverged++;
-        it = a->fp_max;
+        m = std::min(m, a->fp_max);
+        HIPCHK(hipEventSynchronize(a->qev[m % 2]));
+        conv = converged(m);
+        if (!conv) a->st.adj_unconverged++;
     }
-    a->st.adj_iters_max = std::max(a->st.adj_iters_max, it);
-    a->st.adj_iters_mean += it;
+    a->st.adj_iters_max = std::max(a->st.adj_iters_max, m);
+    a->st.adj_iters_mean += m;
     // the adjoints of un, vn and of the points the loop leaves at un, vn
     launch_copy_set(ld, 0, nx + 1, 0, ny + 1, fsrc, fdst, 2, s);
     launch_adj_zero(2 * (long)c->nf, a->nub, s);   // nub, nvb are adjacent

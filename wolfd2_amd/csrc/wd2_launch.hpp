@@ -275,11 +275,17 @@ struct AdjCoefs {
 // (with its first-row quirk) solves exactly.
 void launch_adj_tchain(long n, const Chains &cs, const double *A, const double *D, const double *C, double *At,
                        double *Dt, double *Ct, hipStream_t s);
-// Coefficient adjoints of S x = b given mu = S^-T xbar (Z: rows without).
+// Coefficient adjoints of S x = b given mu = S^-T xbar (Z: rows without;
+// XK: x read as +0 on the rows it does not flag).
 void launch_adj_coef(long n, const Chains &cs, const double *A, const double *D, const double *C, const double *mu,
-                     const double *x, const unsigned char *Z, double *Ab, double *Db, double *Cb, hipStream_t s);
-// Chain-ordered vector (X rows, then Y rows) from two fields.
-void launch_adj_gather(int nx, int ny, int ld, const double *ub, const double *vb, double *lam, hipStream_t s);
+                     const double *x, const unsigned char *Z, const unsigned char *XK, double *Ab, double *Db,
+                     double *Cb, hipStream_t s);
+// Chain-ordered vector (X rows, then Y rows) from two fields, +0 on the rows
+// flagged in Z (null: none); also clears red[slot..slot+1] (red null: none).
+void launch_adj_gather(int nx, int ny, int ld, const double *ub, const double *vb, const unsigned char *Z,
+                       double *lam, unsigned long long *red, int slot, hipStream_t s);
+// nu += z, red[slot] = max |z|, red[slot + 1] = max |nu| (slots cleared before).
+void launch_adj_update(long n, const double *z, double *nu, unsigned long long *red, int slot, hipStream_t s);
 void launch_adj_zmask(long n, const unsigned char *Z, const double *in, double *out, hipStream_t s);
 // One component's momentum assembly transposed (rows of that component's
 // chain); with_n = 0 skips the time-level-n terms.
@@ -287,8 +293,9 @@ void launch_adj_asm(int comp, int nx, int ny, int ld, double dk, double re, cons
                     const double *vs, const double *un, const double *vn, const double *a1b, const double *d1b,
                     const double *c1b, const double *bb, int with_n, double *fsb, double *fnb, const AdjCoefs &A,
                     hipStream_t s);
-void launch_adj_cc(int comp, int nx, int ny, int ld, const Metrics &m, const AdjCoefs &A, double *usb, double *vsb,
-                   double *unb, double *vnb, hipStream_t s);
+// (clears A as it reads it; with_n = 0: the time-level-n parts are zero)
+void launch_adj_cc(int comp, int nx, int ny, int ld, const Metrics &m, const AdjCoefs &A, int with_n, double *usb,
+                   double *vsb, double *unb, double *vnb, hipStream_t s);
 // VelBoundCond / PresBoundCond transposed (one region, no-slip walls);
 // bcb: 8 lines of max(nx,ny)+2 values, [side-1][var-1][k].
 void launch_adj_vbc(const Regions &R, int ld, double *ub, double *vb, double *bcb, hipStream_t s);
