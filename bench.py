@@ -277,6 +277,121 @@ def dropin_rate(g, reg, prm, fields0, steps: int = 2) -> dict:
             "ms_per_step": round(1e3 * secs / steps, 3), "steps": steps, "path": "fp64 drop-in, host arrays"}
 
 
+def adjoint_block(n: int = 512, nsteps: int = 200, every: int = 20) -> dict:
+    """The adjoint case: n^2 cavity, Re 400, dt 2e-4, nsteps steps from rest
+    (24 q-l iterations per step, multigrid to 1e-13), checkpoints every
+    `every` steps; objective half the squared L2 misfit of the final u, v to
+    a target run's (lid of 16 seeded Fourier modes); gradient with respect to
+    the lid values and the initial velocity.  Reports the forward step time
+    (with checkpoints), the backward pass per step (segment re-runs plus
+    adjoint steps) and their ratio."""
+    from wolfd2_amd import autograd
+    g, reg, prm, x, y = workloads.adjoint_cavity(n)
+    run = autograd.CavityRun(g, reg, prm, nsteps, checkpoint_every=every)
+    zu, zv = np.zeros((g.ny - 1, g.nx - 2)), np.zeros((g.ny - 2, g.nx - 1))
+    lid_t = workloads.lid_modes(x, y, reg)[2:g.nx + 1]
+    uT, vT = run.run(lid_t, zu, zv, checkpoints=False)
+    lid = np.ones(g.nx - 1)
+    uN, vN = run.run(lid, zu, zv)
+    J = 0.5 * float(np.sum((uN - uT) ** 2) + np.sum((vN - vT) ** 2))
+    gl, gu, gv, st = run.grad(uN - uT, vN - vT)
+    run.close()
+    fwd = st["ms_forward"] / nsteps
+    bwd = (st["ms_recompute"] + st["ms_reverse"]) / nsteps
+    return {"workload": f"cavity {n}x{n}, Re 400, dt 2e-4, {nsteps} steps from rest, FP64, 24 q-l iterations per "
+                        f"step, multigrid to 1e-13; checkpoint every {every} steps; J = half the squared L2 misfit "
+                        "of the final u, v to a target run (lid of 16 Fourier modes, N(0, 0.1)); gradient w.r.t. "
+                        f"the {n} lid values and the initial velocity",
+            "J": J, "grad_lid_norm": float(np.linalg.norm(gl)),
+            "grad_u0_norm": float(np.sqrt(np.sum(gu ** 2) + np.sum(gv ** 2))),
+            "forward_ms_per_step": round(fwd, 4), "backward_ms_per_step": round(bwd, 4),
+            "recompute_ms_per_step": round(st["ms_recompute"] / nsteps, 4),
+            "reverse_ms_per_step": round(st["ms_reverse"] / nsteps, 4),
+            "backward_over_forward": round(bwd / fwd, 3),
+            "reverse_over_forward": round(st["ms_reverse"] / nsteps / fwd, 3),
+            "adjoint_iters_mean": st["adj_iters_mean"], "adjoint_vcycles_mean": st["ppe_cycles_mean"],
+            "unconverged": {"forward": st["fwd_unconverged"], "adjoint": st["adj_unconverged"]}}
+
+
+def ensemble_block(nmem: int = 64, steps: int = 20, warmup: int = 2, adj_steps: int = 10) -> dict:
+    """The ensemble: nmem cavities with sides drawn from {128, ..., 768}
+    cells and Re from U[100, 2000] (seeded), dt 2e-4, mixed precision
+    (FP32 bulk, FP64-refined multigrid to 1e-10), each member stepping as a
+    hipGraph replay on its own stream, the multigrid kernel shapes autotuned
+    once per grid size; then the adjoint of each member's final kinetic energy
+    with respect to its lid profile over adj_steps steps (FP64)."""
+    specs = workloads.ensemble_specs(nmem)
+    grids, tuned, sols = {}, {}, []
+    t_setup = time.perf_counter()
+    for side, re in specs:
+        if side not in grids:
+            grids[side] = workloads.cavity(side, re, 2e-4, nPpeSolver=7, msorit=100, sortol=1e-10)[0]
+        g = grids[side]
+        reg = case.cavity_regions(g.nx, g.ny)
+        prm = case.Params.from_physical(2e-4, 1.0, 1.0 / re, nPpeSolver=7, msorit=100, sortol=1e-10)
+        sol = api.Solver(g, reg, prm)
+        sol.set_precision("mixed")
+        z = np.zeros((g.ncol, g.ld))
+        sol.set_fields(z, z, z)
+        sol.cold_start()
+        if side not in tuned:
+            tuned[side] = sol.mg_autotune()
+        else:
+            t = tuned[side]
+            sol.set_mg_tune(t["nt0"], t["stage"], t["bside"])
+        sol.set_graph(True)
+        sols.append(sol)
+    t_setup = time.perf_counter() - t_setup
+    ens = api.Ensemble(sols)
+    ens.step(warmup)
+    t0 = time.perf_counter()
+    tabs = ens.step(steps)
+    secs = time.perf_counter() - t0
+    ens.close()
+    cells = sum(s * s for s, _ in specs)
+    gs = [s.graph_stats() for s in sols]
+    out = {"workload": f"{nmem} lid-driven cavities, sides from {{128..768}} and Re from U[100, 2000] (seeded), "
+                       "dt 2e-4, mixed precision, multigrid to 1e-10, hipGraph step per member, members on their "
+                       "own streams",
+           "cells_total": cells, "steps": steps, "value": round(cells * steps / secs / 1e6, 4),
+           "unit": "Mcell-steps/s", "ms_per_step": round(1e3 * secs / steps, 3),
+           "graph_steps": {"replayed": sum(x["replayed"] for x in gs), "redone": sum(x["redone"] for x in gs)},
+           "ql_iter_mean": round(float(np.mean([t[:, 0].mean() for t in tabs])), 2),
+           "vcycles_mean": round(float(np.mean([t[:, 1].mean() for t in tabs])), 2),
+           "mg_autotuned_shapes": {str(k): v for k, v in sorted(tuned.items())},
+           "setup_s": round(t_setup, 1)}
+    # adjoint of each member's kinetic energy w.r.t. its lid, FP64
+    t0 = time.perf_counter()
+    fwd = bwd = 0.0
+    norms = []
+    for sol, (side, re) in zip(sols, specs):
+        g = sol.g
+        sol.set_graph(False)
+        sol.set_precision("fp64")
+        prm = case.Params.from_physical(2e-4, 1.0, 1.0 / re, nPpeSolver=7, msorit=100, sortol=1e-13, qtol=-1.0,
+                                        mqiter=24)
+        sol.set_params(prm)
+        adj = api.Adjoint(sol, 20)
+        adj.forward(adj_steps)
+        u, v, _ = sol.get_fields()
+        su, sv = workloads.interior_u(g), workloads.interior_v(g)
+        ub, vb = np.zeros_like(u), np.zeros_like(v)
+        ub[su], vb[sv] = u[su], v[sv]
+        r = adj.backward(ub, vb)
+        st = r["stats"]
+        fwd += st["ms_forward"]
+        bwd += st["ms_recompute"] + st["ms_reverse"]
+        norms.append(float(np.linalg.norm(r["bc_bar"][case.NORTH - 1, case.VAR_U - 1, 2:g.nx + 1])))
+        adj.close()
+        sol.close()
+    out["adjoint"] = {"objective": "final kinetic energy 0.5 sum(u^2 + v^2) of each member",
+                      "control": "the member's lid profile", "steps": adj_steps,
+                      "forward_ms": round(fwd, 1), "backward_ms": round(bwd, 1),
+                      "backward_over_forward": round(bwd / fwd, 3), "wall_s": round(time.perf_counter() - t0, 1),
+                      "grad_norm_min_max": [min(norms), max(norms)]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +401,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-reference-tolerance", action="store_true",
                     help="skip the cavity at the reference's own Poisson settings (red-black SOR)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint case (512^2, 200 steps)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-cavity ensemble")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run) and exit")
@@ -328,6 +446,8 @@ def main() -> int:
                 ref["gpu_over_cpu"] = round(smx["value"] / cpu["value"], 1)
                 ref["fp64_gpu_over_cpu"] = round(s64["value"] / cpu["value"], 1)
 
+    adjoint = None if args.no_adjoint else adjoint_block()
+    ensemble = None if args.no_ensemble else ensemble_block(steps=args.ensemble_steps)
     out = {
         "metric": METRIC,
         "value": round(mx["value"], 4),
@@ -361,6 +481,8 @@ def main() -> int:
         "fp64_roofline_up": f64["roofline"]["up"],
         "reference_tolerance": ref,
         "dropin_pcie_inclusive": dropin,
+        "adjoint": adjoint,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
