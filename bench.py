@@ -23,6 +23,12 @@ The same cavity at the reference's own Poisson settings (red-black SOR,
 1e-8 / 2000 sweeps, which every step runs to its cap) is measured the same
 way under "reference_tolerance", with the roofline of its tiled SOR kernel.
 
+Beside it: "adjoint", the reverse-mode gradient of a 512^2 cavity run of 200
+steps (checkpoint every 20) with its cost against the forward steps, and
+"ensemble", 64 cavities of sizes 128..768 and Re 100..2000 stepped together
+(mixed precision, hipGraph steps, multigrid shapes autotuned per size) with
+each member's kinetic-energy gradient with respect to its lid.
+
 Prints one JSON line: metric, value, the roofline of the multigrid solver's
 level-0 kernels, and the CPU baseline: the reference program compiled at -O3
 on one host core, timed on steps of the cavity at its own settings.
