@@ -81,3 +81,40 @@ def test_ensemble_members_equal_their_single_runs(graph):
         for a, b in zip(m.get_fields(), alone[i][1]):
             assert np.array_equal(a, b), i
         m.close()
+
+
+def test_graph_cache_follows_precision_params_and_profiles():
+    """Cached step graphs bake in the precision, the step parameters and the
+    boundary tables: after each change the graph steps still equal host-driven
+    steps bit for bit, and they replay (a stale graph would fail its stopping
+    test and be redone, or give other fields)."""
+    g, reg, prm = workloads.cavity(96, 400.0, 5e-4, nPpeSolver=7, msorit=100, sortol=1e-11)
+    z = np.zeros((g.ncol, g.ld))
+    x, y = case.uniform_nodes(g.nx, g.ny)
+    lid2 = workloads.lid_modes(x, y, reg, seed=3)
+    prm2 = case.Params.from_physical(3e-4, 1.0, 1.0 / 700.0, nPpeSolver=7, msorit=100, sortol=1e-11)
+
+    def run(graph):
+        sol = api.Solver(g, reg, prm)
+        sol.set_graph(graph)
+        sol.set_fields(z, z, z)
+        tabs = [sol.step(6)]
+        sol.set_precision("mixed")
+        tabs.append(sol.step(6))
+        sol.set_params(prm2)
+        tabs.append(sol.step(6))
+        sol.set_bc_profile(1, 1, case.NORTH, case.VAR_U, lid2)
+        tabs.append(sol.step(6))
+        sol.set_precision("fp64")
+        tabs.append(sol.step(6))
+        out = sol.get_fields(), np.concatenate(tabs), sol.graph_stats()
+        sol.close()
+        return out
+
+    (fe, te, _), (fg, tg, st) = run(False), run(True)
+    print(f"  graph stats {st}", flush=True)
+    assert np.array_equal(te, tg)
+    for a, b in zip(fe, fg):
+        assert np.array_equal(a, b)
+    # each of the five phases redoes at most its first two steps
+    assert st["replayed"] >= 20

@@ -24,6 +24,7 @@ static int g_tiled_sor = 1;   // temporally blocked SorRB when the grid allows i
 static int g_verbose = 1;     // print the reference's warnings (pressure.f:242-246, 234)
 static int g_lex_rows = 1;    // lexicographic Sor as one single-workgroup launch when the rows fit
 static int g_uniform = 1;     // tiled SorRB takes rau/rgv as two values on uniform Cartesian grids
+static int g_graph_epoch = 0; // bumped by the global switches a captured step bakes in
 
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -133,9 +134,15 @@ struct wd2_ctx {
     // every step), the pre-step u, v, p for a step that is redone eagerly,
     // the pinned copy of the scalars each graph ends with, the last step's
     // q-l count, and counts of replayed and redone steps
+    // A graph bakes in the step's parameters, the precision, the multigrid
+    // shapes, the region and profile tables and the metrics: it is cached
+    // with the parameters and the global epoch, and every change of the rest
+    // clears the cache (graph_clear).
     struct GraphEntry {
         int q, cyc;
         const double *u;
+        wd2_params P;
+        int epoch;
         hipGraphExec_t exec;
     };
     bool graph_on = false;
@@ -172,6 +179,14 @@ static hipEvent_t *kt_events(wd2_ctx *c, int npairs, int which = 1) {
 }
 
 static inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// Drop every cached step graph (after the stream has drained).
+static void graph_clear(wd2_ctx *c) {
+    if (c->graphs.empty()) return;
+    (void)hipStreamSynchronize(c->s);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
 
 static void ctx_free(wd2_ctx *c) {
     if (!c) return;
@@ -1208,7 +1223,9 @@ static int graph_begin(wd2_ctx *c, const wd2_params *P) {
     const int C = std::min(P->msorit + 1, c->mg_last + 2);
     hipGraphExec_t exec = nullptr;
     for (auto &g : c->graphs)
-        if (g.q == Q && g.cyc == C && g.u == c->u) exec = g.exec;
+        if (g.q == Q && g.cyc == C && g.u == c->u && g.epoch == g_graph_epoch &&
+            std::memcmp(&g.P, P, sizeof(wd2_params)) == 0)
+            exec = g.exec;
     if (!exec) {
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
@@ -1222,7 +1239,7 @@ static int graph_begin(wd2_ctx *c, const wd2_params *P) {
             (void)hipGraphExecDestroy(c->graphs.front().exec);
             c->graphs.erase(c->graphs.begin());
         }
-        c->graphs.push_back({Q, C, c->u, exec});
+        c->graphs.push_back({Q, C, c->u, *P, g_graph_epoch, exec});
     }
     HIPCHK(hipGraphLaunch(exec, c->s));
     c->g_q = Q;
@@ -1567,6 +1584,7 @@ static int up_metric(int k, const double *h, int ld) {
     sh.resize(w * rows);
     for (size_t j = 0; j < rows; j++) std::memcpy(sh.data() + j * w, h + j * (size_t)ld, w * sizeof(double));
     c->met_version++;
+    graph_clear(c);
     c->M.box[k] = scan_box(h, ld, c->nx, c->ny);
     return up(c, h, ld, c->met[k]);
 }
@@ -1587,6 +1605,7 @@ int wd2_set_verbose(int enable) {
 int wd2_set_uniform_coefficients(int enable) {
     const int old = g_uniform;
     g_uniform = enable ? 1 : 0;
+    if (g_uniform != old) g_graph_epoch++;
     return old;
 }
 
@@ -2010,12 +2029,14 @@ int wd2_ctx_set_options(wd2_ctx *c, const wd2_step_options *opt) {
         return fail(WD2_ERR_ARG, "wd2_ctx_set_options: the energy equation needs a Peclet number > 0");
     c->opt = *opt;
     if (opt->neqstate == 1) c->d_live = true;   // EqState writes d
+    graph_clear(c);
     return WD2_OK;
 }
 
 int wd2_ctx_set_thermal(wd2_ctx *c, const int *nTRgType, const int *nTemBdTp, const double *dTRgVal,
                         const double *dHGSTval) {
     if (!c) return fail(WD2_ERR_ARG, "wd2_ctx_set_thermal: null context");
+    graph_clear(c);
     int st = set_thermal(c, nTRgType, nTemBdTp, dTRgVal, dHGSTval);
     if (st) return st;
     HIPCHK(hipStreamSynchronize(c->s));
@@ -2052,6 +2073,7 @@ int wd2_ctx_set_precision(wd2_ctx *c, int precision) {
     if (precision != WD2_PREC_FP64 && precision != WD2_PREC_MIXED)
         return fail(WD2_ERR_ARG, "wd2_ctx_set_precision: unknown precision %d", precision);
     if (precision != c->precision) {
+        graph_clear(c);
         // The FP32 momentum views alias the FP64 temporaries and chains.  Entries
         // the momentum kernels never write are read as 0 (as in the reference),
         // so clear the whole block when the arithmetic changes.
@@ -2075,8 +2097,10 @@ int wd2_ctx_set_bc_profile(wd2_ctx *c, int ireg, int jreg, int side, int var, co
     if (values && n != want)
         return fail(WD2_ERR_ARG, "wd2_ctx_set_bc_profile: side %d needs %d values (one per grid line), got %d", side,
                     want, n);
-    // kernels in flight may still read the current tables
+    // kernels in flight may still read the current tables; cached step graphs
+    // hold the table pointers
     HIPCHK(hipStreamSynchronize(c->s));
+    graph_clear(c);
     if (values)
         c->h_prof[key].assign(values, values + n);
     else
@@ -2114,6 +2138,7 @@ int wd2_ctx_mg_set_tune(wd2_ctx *c, int nt0, int stage, int bside) {
         return fail(WD2_ERR_ARG, "multigrid shape: nt0 256|512, stage 0|1, bside 32|64");
     int st = mg_setup(c);
     if (st) return st;
+    graph_clear(c);
     const MgTune old = c->mg.tune;
     c->mg.tune = MgTune{nt0, stage, bside};
     if ((st = mg_retile(c->mg))) {
@@ -2144,6 +2169,7 @@ int wd2_ctx_mg_autotune(wd2_ctx *c, double dk, int cycles, double *best_us) {
     if (cycles < 1) return fail(WD2_ERR_ARG, "autotune needs cycles >= 1");
     int st = mg_prepare(c);
     if (st) return st;
+    graph_clear(c);
     MgTune best = c->mg.tune;
     double tbest = 0.0;
     bool have = false;
