@@ -134,6 +134,28 @@ __device__ inline T tile_res(const T *E, const T *R, const T *AX, const T *AY, i
     return R[q] - (as * E[q - TW] + aw * E[q - 1] + a3 * E[q] + ae * E[q + 1] + an * E[q + TW]);
 }
 
+// The same two on a level whose faces all hold one x value ax and one y value
+// ay (level 0 of a uniform Cartesian grid: rau and rgv are constants, and the
+// faces to the frozen ghosts carry them too), with ia3 = 1 / a3: no face
+// arrays, a multiply in place of the division.
+template <typename T, int TW, int NT>
+__device__ inline void tile_colour_u(T *E, const T *R, T ax, T ay, T ia3, int c, int gx0, int gy0, int NX, int NY) {
+    constexpr int KW = TW / 2;
+    for (int h = threadIdx.x; h < TW * KW; h += NT) {
+        const int y = h / KW, x = 2 * (h % KW) + ((c ^ y) & 1);
+        const int I = gx0 + x, J = gy0 + y;
+        if (x < 1 || x > TW - 2 || y < 1 || y > TW - 2 || I < 0 || I >= NX || J < 0 || J >= NY) continue;
+        const int q = y * TW + x;
+        E[q] = (R[q] - (ay * (E[q - TW] + E[q + TW]) + ax * (E[q - 1] + E[q + 1]))) * ia3;
+    }
+}
+
+template <typename T, int TW>
+__device__ inline T tile_res_u(const T *E, const T *R, T ax, T ay, T a3, int x, int y) {
+    const int q = y * TW + x;
+    return R[q] - (ay * (E[q - TW] + E[q + TW]) + ax * (E[q - 1] + E[q + 1]) + a3 * E[q]);
+}
+
 // Faces of a tile of side TW at level L whose cell (0, 0) is (gx0, gy0).
 template <typename T, int TW, int NT>
 __device__ inline void tile_faces(const MgLevel &L, int gx0, int gy0, T *AX, T *AY) {
@@ -207,15 +229,17 @@ __device__ inline int mg_tile(int ntiles) {
 // tile and one more ring) and b (ghosts frozen), stored in T for the up pass,
 // and max |r| of the owned points goes to part[tile].  D.nt > 0: this launch
 // first evaluates the stopping test of the level-0 kernel before it.
-template <typename T, bool L0, int NT, bool STAGE>
+// UF: level 0 with uniform faces (rau_c, rgv_c), see tile_colour_u.
+template <typename T, bool L0, int NT, bool STAGE, bool UF>
 __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, int ntiles, int nx, int ny, int ld,
                                                 const double *__restrict__ p, const double *__restrict__ b,
                                                 const double *__restrict__ rau, const double *__restrict__ rgv,
                                                 int uni, double rau_c, double rgv_c, double *__restrict__ part,
                                                 MgDecide D, Scalars *sc) {
     if (mg_stopped(sc)) return;
+    static_assert(L0 || !UF, "uniform faces are a level-0 shape");
     constexpr int TW = MG_TD, PW = TW + 2;
-    __shared__ T E[TW * TW], R[TW * TW], AX[TW * (TW + 1)], AY[(TW + 1) * TW];
+    __shared__ T E[TW * TW], R[TW * TW], AX[UF ? 1 : TW * (TW + 1)], AY[UF ? 1 : (TW + 1) * TW];
     __shared__ double PS[(L0 && STAGE) ? PW * PW : 1];
     __shared__ double red[NT / 64];
     if (D.nt > 0 && mg_decide<NT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, sc, red)) return;
@@ -225,7 +249,8 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
     const int gx0 = I0 - MG_HD, gy0 = J0 - MG_HD;
     const int NX = L.nx, NY = L.ny;
     T *e = (T *)L.e, *r = (T *)L.r;
-    tile_faces<T, TW, NT>(L, gx0, gy0, AX, AY);
+    const T uax = (T)rau_c, uay = (T)rgv_c, ua3 = -(uax + uax + uay + uay), uia3 = T(1) / ua3;
+    if (!UF) tile_faces<T, TW, NT>(L, gx0, gy0, AX, AY);
     if (L0 && STAGE) {
         // p on the tile and one more ring: field (i, j) = (I + 2, J + 2) for
         // I = gx0 - 1 .., 0 outside the field
@@ -292,7 +317,10 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
     }
     for (int s = 0; s < MG_NU; s++)
         for (int c = 0; c < 2; c++) {
-            tile_colour<T, TW, NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
+            if (UF)
+                tile_colour_u<T, TW, NT>(E, R, uax, uay, uia3, c, gx0, gy0, NX, NY);
+            else
+                tile_colour<T, TW, NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
             __syncthreads();
         }
     // a coarse cell per thread: its children's residuals summed; e (and, on
@@ -309,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
                 const int x = MG_HD + 2 * cx + dx, y = MG_HD + 2 * cy + dy;
                 const int I = gx0 + x, J = gy0 + y;
                 if (I < NX && J < NY) {
-                    sum += tile_res<T, TW>(E, R, AX, AY, x, y);
+                    sum += UF ? tile_res_u<T, TW>(E, R, uax, uay, ua3, x, y) : tile_res<T, TW>(E, R, AX, AY, x, y);
                     e[mc(L.ld, I, J)] = E[y * TW + x];
                     if (L0) r[mc(L.ld, I, J)] = R[y * TW + x];
                 }
@@ -322,12 +350,14 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
 // Level l: c = e + prolongation of c_(l+1), then NU sweeps; c of the owned
 // points is written (L0: p += c in FP64 instead).  The input e of the halos
 // and the output c are separate arrays, so no tile reads what another writes.
-template <typename T, bool L0, int NT>
+// UF: level 0 with uniform faces ax, ay (see tile_colour_u).
+template <typename T, bool L0, int NT, bool UF>
 __global__ __launch_bounds__(NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, int ntiles, int ld,
-                                              double *__restrict__ p, const Scalars *sc) {
+                                              double *__restrict__ p, const Scalars *sc, double rau_c, double rgv_c) {
     if (mg_stopped(sc)) return;
+    static_assert(L0 || !UF, "uniform faces are a level-0 shape");
     constexpr int TW = MG_TU;
-    __shared__ T E[TW * TW], R[TW * TW], AX[TW * (TW + 1)], AY[(TW + 1) * TW], EC[MG_TC * MG_TC];
+    __shared__ T E[TW * TW], R[TW * TW], AX[UF ? 1 : TW * (TW + 1)], AY[UF ? 1 : (TW + 1) * TW], EC[MG_TC * MG_TC];
     const int tile = mg_tile(ntiles);
     if (tile >= ntiles) return;
     const int I0 = (tile % ntx) * MG_OT, J0 = (tile / ntx) * MG_OT;
@@ -335,7 +365,8 @@ __global__ __launch_bounds__(NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, int
     const int cgx0 = gx0 / 2 - 1, cgy0 = gy0 / 2 - 1;   // gx0 is even
     const int NX = L.nx, NY = L.ny;
     const T *e = (const T *)L.e, *r = (const T *)L.r, *ec = (const T *)C.c;
-    tile_faces<T, TW, NT>(L, gx0, gy0, AX, AY);
+    const T uax = (T)rau_c, uay = (T)rgv_c, uia3 = T(1) / -(uax + uax + uay + uay);
+    if (!UF) tile_faces<T, TW, NT>(L, gx0, gy0, AX, AY);
     for (int q = threadIdx.x; q < MG_TC * MG_TC; q += NT) {
         const int Ic = cgx0 + q % MG_TC, Jc = cgy0 + q / MG_TC;
         EC[q] = (Ic >= 0 && Ic < C.nx && Jc >= 0 && Jc < C.ny) ? ec[mc(C.ld, Ic, Jc)] : T(0);
@@ -360,7 +391,10 @@ __global__ __launch_bounds__(NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, int
     __syncthreads();
     for (int s = 0; s < MG_NU; s++)
         for (int c = 1; c >= 0; c--) {
-            tile_colour<T, TW, NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
+            if (UF)
+                tile_colour_u<T, TW, NT>(E, R, uax, uay, uia3, c, gx0, gy0, NX, NY);
+            else
+                tile_colour<T, TW, NT>(E, R, AX, AY, c, gx0, gy0, NX, NY);
             __syncthreads();
         }
     T *eo = (T *)L.c;
@@ -538,9 +572,13 @@ static void down0_as(const MgHierarchy &H, int nx, int ny, int ld, const double 
                      hipStream_t s) {
     int ntx = 0;
     const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
-    hipLaunchKernelGGL((k_mg_down<T, true, NT, STAGE>), dim3(8 * ((nt + 7) / 8)), dim3(NT), 0, s, H.lev[0],
-                       H.lev[1], ntx, nt, nx, ny, ld, p, b, rau, rgv, uni ? 1 : 0, uni ? uni[0] : 0.0,
-                       uni ? uni[1] : 0.0, part, MgDecide{nullptr, 0, 0, 0, 0.0, 0.0}, sc);
+    const MgDecide none{nullptr, 0, 0, 0, 0.0, 0.0};
+    if (uni)
+        hipLaunchKernelGGL((k_mg_down<T, true, NT, STAGE, true>), dim3(8 * ((nt + 7) / 8)), dim3(NT), 0, s, H.lev[0],
+                           H.lev[1], ntx, nt, nx, ny, ld, p, b, rau, rgv, 1, uni[0], uni[1], part, none, sc);
+    else
+        hipLaunchKernelGGL((k_mg_down<T, true, NT, STAGE, false>), dim3(8 * ((nt + 7) / 8)), dim3(NT), 0, s,
+                           H.lev[0], H.lev[1], ntx, nt, nx, ny, ld, p, b, rau, rgv, 0, 0.0, 0.0, part, none, sc);
 }
 
 // the level-0 down kernel in the shape the hierarchy's tuning selects
@@ -561,16 +599,24 @@ static void down0(const MgHierarchy &H, int nx, int ny, int ld, const double *b,
     }
 }
 
-template <typename T>
-static void up0(const MgHierarchy &H, int ld, double *p, Scalars *sc, hipStream_t s) {
+template <typename T, int NT>
+static void up0_as(const MgHierarchy &H, int ld, double *p, const double *uni, Scalars *sc, hipStream_t s) {
     int ntx = 0;
     const int nt = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
-    if (H.tune.nt0 >= 512)
-        hipLaunchKernelGGL((k_mg_up<T, true, 512>), dim3(8 * ((nt + 7) / 8)), dim3(512), 0, s, H.lev[0], H.lev[1],
-                           ntx, nt, ld, p, sc);
+    if (uni)
+        hipLaunchKernelGGL((k_mg_up<T, true, NT, true>), dim3(8 * ((nt + 7) / 8)), dim3(NT), 0, s, H.lev[0], H.lev[1],
+                           ntx, nt, ld, p, sc, uni[0], uni[1]);
     else
-        hipLaunchKernelGGL((k_mg_up<T, true, 256>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, s, H.lev[0], H.lev[1],
-                           ntx, nt, ld, p, sc);
+        hipLaunchKernelGGL((k_mg_up<T, true, NT, false>), dim3(8 * ((nt + 7) / 8)), dim3(NT), 0, s, H.lev[0],
+                           H.lev[1], ntx, nt, ld, p, sc, 0.0, 0.0);
+}
+
+template <typename T>
+static void up0(const MgHierarchy &H, int ld, double *p, const double *uni, Scalars *sc, hipStream_t s) {
+    if (H.tune.nt0 >= 512)
+        up0_as<T, 512>(H, ld, p, uni, sc, s);
+    else
+        up0_as<T, 256>(H, ld, p, uni, sc, s);
 }
 
 // One V-cycle of defect correction (cycle is 1-based), every kernel
@@ -589,7 +635,7 @@ void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double 
     const MgDecide none{nullptr, 0, 0, 0, 0.0, 0.0};
     for (int l = 1; l < H.ntiled; l++) {
         const int nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
-        hipLaunchKernelGGL((k_mg_down<T, false, MG_NT1, false>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
+        hipLaunchKernelGGL((k_mg_down<T, false, MG_NT1, false, false>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
                            H.lev[l + 1], ntx, nt, nx, ny, ld, p, b, rau, rgv, 0, 0.0, 0.0, part, l == 1 ? D : none,
                            sc);
     }
@@ -599,11 +645,11 @@ void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double 
     hipLaunchKernelGGL(k_mg_bottom<T>, dim3(1), dim3(MG_BT), 0, s, B, H.ntiled == 1 ? D : none, sc);
     for (int l = H.ntiled - 1; l >= 1; l--) {
         const int nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
-        hipLaunchKernelGGL((k_mg_up<T, false, MG_NT1>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
-                           H.lev[l + 1], ntx, nt, ld, p, sc);
+        hipLaunchKernelGGL((k_mg_up<T, false, MG_NT1, false>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
+                           H.lev[l + 1], ntx, nt, ld, p, sc, 0.0, 0.0);
     }
     if (ev_up) (void)hipEventRecord(ev_up[0], s);
-    up0<T>(H, ld, p, sc, s);
+    up0<T>(H, ld, p, uni, sc, s);
     if (ev_up) (void)hipEventRecord(ev_up[1], s);
 }
 
@@ -621,7 +667,7 @@ void launch_mg_kernel_bench(const MgHierarchy &H, int which, int reps, int nx, i
         if (which == 2)
             down0<T>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
         else
-            up0<T>(H, ld, p, sc, s);
+            up0<T>(H, ld, p, uni, sc, s);
     }
     (void)hipEventRecord(e1, s);
 }
