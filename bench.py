@@ -29,9 +29,12 @@ steps (checkpoint every 20) with its cost against the forward steps, and
 (mixed precision, hipGraph steps, multigrid shapes autotuned per size) with
 each member's kinetic-energy gradient with respect to its lid.
 
-Prints one JSON line: metric, value, the roofline of the multigrid solver's
-level-0 kernels, and the CPU baseline: the reference program compiled at -O3
-on one host core, timed on steps of the cavity at its own settings.
+Prints one JSON line: metric, value, the roofline of the mixed step's
+dominant kernel (the window kernel of its FP32 chain solves, `roofline`) with
+the bit-exact chain solver's kernels and the multigrid solver's level-0
+kernels beside it (and the FP64 path's), and the CPU
+baseline: the reference program compiled at -O3 on one host core, timed on
+steps of the cavity at its own settings.
 """
 from __future__ import annotations
 
@@ -195,6 +198,23 @@ def run_path(sol, precision: str, solver: str, fields0, warmup: int, steps: int,
             roof[name] = _roofline(kern, bpl, us.value, reps,
                                    _traffic(f"k_mg_{name}_{precision}_cavity1024_traffic.json"))
             roof[name]["avg_launch_us_in_cycles"] = round(us_cycle, 3)
+        # the chain solver: the forward kernel reads a, d, c, b and writes the
+        # swept super-diagonal and right-hand side (4s in, 2s out per row);
+        # the backward kernel reads those two and writes x (2s in, s out);
+        # rows: the X and Y chains of one q-l iteration as one batch
+        rows = sol.g.nx * (sol.g.ny - 1) + (sol.g.nx - 1) * sol.g.ny
+        kinds = [(1, "trid_fwd", 6.0), (2, "trid_bwd", 3.0)]
+        if precision == "mixed":
+            # the mixed path's window solver: a, d, c, b read and x written
+            # per owned row (5s; the overlap of the windows is extra traffic)
+            kinds.append((3, "trid_window", 5.0))
+        for which, name, per in kinds:
+            us = C.c_double(0)
+            L.wd2_ctx_trid_kernel_time(sol.ctx, which, reps, C.byref(us))
+            kern = (f"k_{name}<{t}> (" + ("windowed partitioned Thomas + PCR" if which == 3 else "parallel AltTridLU")
+                    + f", {rows} rows: the X and Y chains of one q-l iteration)")
+            roof[name] = _roofline(kern, per * s * rows, us.value, reps,
+                                   _traffic(f"k_{name}_{precision}_cavity1024_traffic.json"))
     else:
         us, nl = _kernel_time(sol, 1, steps)
         if precision == "mixed":
@@ -481,10 +501,15 @@ def main() -> int:
                    "fp64_ql_iter_mean": f64["ql"], "fp64_vcycles_mean": f64["ppe"],
                    "trid_sequential_fallbacks": {"mixed": mx["trid_fallbacks"], "fp64": f64["trid_fallbacks"]},
                    "mg_autotuned_shape": {"mixed": mx["mg_tune"], "fp64": f64["mg_tune"]}},
-        "roofline": mx["roofline"]["down"],
-        "roofline_up": mx["roofline"]["up"],
-        "fp64_roofline": f64["roofline"]["down"],
-        "fp64_roofline_up": f64["roofline"]["up"],
+        "roofline": mx["roofline"]["trid_window"],
+        "roofline_trid_fwd": mx["roofline"]["trid_fwd"],
+        "roofline_trid_bwd": mx["roofline"]["trid_bwd"],
+        "roofline_mg_down": mx["roofline"]["down"],
+        "roofline_mg_up": mx["roofline"]["up"],
+        "fp64_roofline": f64["roofline"]["trid_fwd"],
+        "fp64_roofline_trid_bwd": f64["roofline"]["trid_bwd"],
+        "fp64_roofline_mg_down": f64["roofline"]["down"],
+        "fp64_roofline_mg_up": f64["roofline"]["up"],
         "reference_tolerance": ref,
         "dropin_pcie_inclusive": dropin,
         "adjoint": adjoint,

@@ -50,6 +50,13 @@ int wd2_set_tiled_sor(int enable);
  * default); 0 selects the tiled wavefront, one launch per sweep.  Both give
  * the reference's iterates bit for bit.  Returns the previous setting. */
 int wd2_set_lex_rows(int enable);
+/* Mixed precision only: solve the FP32 momentum chains of q-l iterations 2..
+ * with the windowed partitioned-Thomas/PCR solver (enable = 1, the default;
+ * results agree with the bit-exact FP32 solver to FP32 rounding; the first
+ * iteration of each step keeps the bit-exact solver) or every chain with the
+ * bit-exact speculative solver (0).  Returns the previous setting.  No
+ * counterpart in the reference. */
+int wd2_set_window_chains(int enable);
 /* Non-zero (default): on a uniform Cartesian grid, where rau and rgv each hold
  * one value over the entries the red-black SOR reads, the tiled SOR kernels
  * take the two values as arguments instead of loading the arrays (the same
@@ -104,6 +111,14 @@ int wd2_y_momentum(int nx, int ny, int ld, int mgri, int mgrj,
 /* momentum.f:AltTridLU 1307-1339: a is a(3,n) (sub, diagonal, super per row),
  * b the right-hand side, overwritten by the solution.  a is not modified. */
 int wd2_alt_trid_lu(int n, const double *a, double *b);
+/* The mixed path's FP32 chain solve on host arrays (no counterpart in the
+ * reference; for measurement and tests): a batch of chains, rows [0, n1) in
+ * chains of L1 rows and rows [n1, n) in chains of L2 rows, each solved as
+ * AltTridLU solves it (first-row quirk included), with the window solver and
+ * its bit-exact fallback (window = 1) or the bit-exact solver (0).  a, d, c,
+ * b: sub-diagonal, diagonal, super-diagonal, right-hand side; x: solution. */
+int wd2_trid_f32(long n, long n1, long L1, long L2, const float *a, const float *d, const float *c, const float *b,
+                 float *x, int window);
 
 /* No counterpart in the reference: AltTridLU (momentum.f:1307-1339, the same
  * a(3,n) and b(n) arguments and the same bits as wd2_alt_trid_lu) for callers
@@ -308,6 +323,14 @@ int wd2_ctx_ppe_divergence(wd2_ctx *ctx, double *last, double *max_since, int re
  * solve's own cycles include the gap between launches); p is restored
  * afterwards.  No counterpart in the reference. */
 int wd2_ctx_mg_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
+/* Mean time (microseconds) of the chain solver's forward (which = 1) or
+ * backward (2) kernel, or of the mixed path's window kernel (3, mixed
+ * precision only), over `reps` back-to-back launches on the context's
+ * stream, on the first-split chains of the context's last q-l iteration (the
+ * X and Y chains as one two-chain batch) in the context's precision.  The
+ * chains are solved once more afterwards; fields are unchanged.  No
+ * counterpart in the reference (measurement only). */
+int wd2_ctx_trid_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
 /* Kernel shapes of the multigrid solver (no counterpart in the reference;
  * none changes the arithmetic, so results are the same bits): nt0 threads
  * per level-0 tile workgroup (256 or 512), stage 1 to form the level-0

@@ -600,6 +600,208 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nb
     *nfallback += 1;
 }
 
+// ------------------------------------------------------------------------
+// Windowed chain solve for the mixed path (FP32, not bit-exact).  The FP32
+// chains need only agree with the FP64 solve to FP32 accuracy, and the
+// momentum chains are strongly diagonally dominant, so the coupling of a
+// row to rows far away decays geometrically.  Each workgroup solves a window
+// of TW_N rows with the coupling at the window's two edges dropped and keeps
+// the TW_O rows in its middle (TW_H rows from either edge): for a dominance
+// ratio (|a| + |c|) / |d| <= TW_DOM the dropped coupling changes the kept
+// rows by less than about 0.63^TW_H = 1e-13 relative.  The window system is
+// the chain's own matrix, with the first-row quirk (S(1,2) = c1 d1 / d2) and
+// no coupling across chain starts and ends.
+//
+// Inside the window (partitioned Thomas + PCR): each thread holds TW_S rows
+// in registers and expresses its interior rows through its first and last
+// row (three 6-row Thomas solves: particular, left and right spikes), which
+// leaves two interface rows per thread, a tridiagonal system of 2 TW_T rows;
+// parallel cyclic reduction in LDS solves it (log2(2 TW_T) levels, each row
+// combining with the rows 2^l away), and the interior rows follow from the
+// interface values.  A window whose dominance ratio exceeds TW_DOM clears
+// *keep, and the bit-exact speculative solver (launched after this kernel
+// with stop = keep) solves the chains instead.
+constexpr int TW_T = 256, TW_S = 8, TW_N = TW_T * TW_S, TW_H = 64, TW_O = TW_N - 2 * TW_H;
+constexpr int TW_M = 2 * TW_T;   // interface rows per window
+constexpr float TW_DOM = 0.9f;
+
+__global__ void k_trid_keep_set(int *keep) { *keep = 1; }
+
+// Hardware reciprocal (1 ulp) for the window solver, which need not round
+// like the reference.
+__device__ inline float wrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename T>
+__global__ __launch_bounds__(TW_T) void k_trid_window(long n, Chains cs, const T *__restrict__ A,
+                                                      const T *__restrict__ D, const T *__restrict__ C,
+                                                      const T *__restrict__ B, T *__restrict__ X, T *__restrict__ B2,
+                                                      const unsigned char *__restrict__ Z, const int *stop,
+                                                      int *keep) {
+    static_assert(std::is_same<T, float>::value, "the window solver serves the FP32 chains");
+    if (stop && *stop) return;
+    // interface rows in LDS: sub- and super-diagonal, right-hand side and
+    // the reciprocal of the diagonal (each row's diagonal stays with its thread)
+    __shared__ T ra[2][TW_M], rc[2][TW_M], rb[2][TW_M], rq[2][TW_M];
+    const int t = threadIdx.x;
+    const long own0 = (long)blockIdx.x * TW_O, w0 = own0 - TW_H;
+    T a[TW_S], d[TW_S], c[TW_S], b[TW_S];
+    const long k0 = w0 + (long)t * TW_S;   // a multiple of TW_S: 16-byte aligned segments
+    if (k0 >= 0 && k0 + TW_S <= n) {
+        seg_load(A, k0, TW_S, a);
+        seg_load(D, k0, TW_S, d);
+        seg_load(C, k0, TW_S, c);
+        seg_load(B, k0, TW_S, b);
+    } else {
+#pragma unroll
+        for (int s = 0; s < TW_S; s++) {
+            const long k = k0 + s;
+            const bool in = k >= 0 && k < n;   // outside the chains: x = 0, uncoupled
+            a[s] = in ? A[k] : T(0);
+            d[s] = in ? D[k] : T(1);
+            c[s] = in ? C[k] : T(0);
+            b[s] = in ? B[k] : T(0);
+        }
+    }
+    // chain starts (no sub-diagonal; the first-row quirk on the
+    // super-diagonal) and ends (no super-diagonal)
+    bool weak = false;
+    {
+        long Lk = 1, r = 0;
+        if (k0 >= 0 && k0 < n) r = chain_pos(cs, k0, Lk);
+#pragma unroll
+        for (int s = 0; s < TW_S; s++) {
+            const long k = k0 + s;
+            if (k >= 0 && k < n) {
+                if (r == 0) {
+                    a[s] = T(0);
+                    if (Lk > 1) c[s] = c[s] * d[s] / D[k + 1];
+                }
+                if (r == Lk - 1) c[s] = T(0);
+                if (!(fabsf(a[s]) + fabsf(c[s]) <= TW_DOM * fabsf(d[s]))) weak = true;
+                chain_up(cs, k, r, Lk);
+            } else if (k == -1) {
+                r = 0;
+                Lk = cs.n1 > 0 ? cs.L1 : cs.L2;
+            }
+        }
+    }
+    // the window's edges: drop the coupling out of the window
+    if (t == 0) a[0] = T(0);
+    if (t == TW_T - 1) c[TW_S - 1] = T(0);
+    if (__any(weak)) *keep = 0;
+    // interior rows 1..TW_S-2 through x0 and x7: y (particular), v (x0
+    // spike, from -a1 x0), w (x7 spike, from -c6 x7)
+    T cp[TW_S], y[TW_S], v[TW_S], w[TW_S];
+    {
+        T q = wrcp(d[1]);
+        cp[1] = c[1] * q;
+        y[1] = b[1] * q;
+        v[1] = -a[1] * q;
+        w[1] = T(0);
+#pragma unroll
+        for (int s = 2; s <= TW_S - 2; s++) {
+            q = wrcp(d[s] - a[s] * cp[s - 1]);
+            cp[s] = c[s] * q;
+            y[s] = (b[s] - a[s] * y[s - 1]) * q;
+            v[s] = -a[s] * v[s - 1] * q;
+            w[s] = ((s == TW_S - 2 ? -c[s] : T(0)) - a[s] * w[s - 1]) * q;
+        }
+#pragma unroll
+        for (int s = TW_S - 3; s >= 1; s--) {
+            y[s] -= cp[s] * y[s + 1];
+            v[s] -= cp[s] * v[s + 1];
+            w[s] -= cp[s] * w[s + 1];
+        }
+    }
+    // interface rows 2t (the segment's first row) and 2t+1 (its last)
+    T di[2];
+    {
+        const int i0 = 2 * t, i1 = 2 * t + 1;
+        di[0] = d[0] + c[0] * v[1];
+        di[1] = d[TW_S - 1] + a[TW_S - 1] * w[TW_S - 2];
+        ra[0][i0] = a[0];
+        rc[0][i0] = c[0] * w[1];
+        rb[0][i0] = b[0] - c[0] * y[1];
+        rq[0][i0] = wrcp(di[0]);
+        ra[0][i1] = a[TW_S - 1] * v[TW_S - 2];
+        rc[0][i1] = c[TW_S - 1];
+        rb[0][i1] = b[TW_S - 1] - a[TW_S - 1] * y[TW_S - 2];
+        rq[0][i1] = wrcp(di[1]);
+    }
+    __syncthreads();
+    // PCR on the interface rows: row i eliminates its coupling to rows i-h
+    // and i+h with those rows' equations
+    int cur = 0;
+    for (int h = 1; h < TW_M; h <<= 1) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int i = 2 * t + e;
+            const T ai = ra[cur][i], ci = rc[cur][i], bi = rb[cur][i];
+            T an = T(0), dn = di[e], cn = T(0), bn = bi;
+            if (i - h >= 0) {
+                const T k1 = ai * rq[cur][i - h];
+                an = -ra[cur][i - h] * k1;
+                dn -= rc[cur][i - h] * k1;
+                bn -= rb[cur][i - h] * k1;
+            }
+            if (i + h < TW_M) {
+                const T k2 = ci * rq[cur][i + h];
+                cn = -rc[cur][i + h] * k2;
+                dn -= ra[cur][i + h] * k2;
+                bn -= rb[cur][i + h] * k2;
+            }
+            di[e] = dn;
+            ra[cur ^ 1][i] = an;
+            rc[cur ^ 1][i] = cn;
+            rb[cur ^ 1][i] = bn;
+            rq[cur ^ 1][i] = wrcp(dn);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    const T x0 = rb[cur][2 * t] * rq[cur][2 * t], x7 = rb[cur][2 * t + 1] * rq[cur][2 * t + 1];
+#pragma unroll
+    for (int s = 0; s < TW_S; s++) {
+        const long k = w0 + (long)t * TW_S + s;
+        if (k < own0 || k >= own0 + TW_O || k >= n) continue;
+        const T xs = s == 0 ? x0 : (s == TW_S - 1 ? x7 : y[s] + v[s] * x0 + w[s] * x7);
+        X[k] = xs;
+        if (B2) B2[k] = (Z && Z[k]) ? T(0) : xs;
+    }
+}
+
+template <typename T>
+void launch_trid_window(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
+                        const T *B, T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop) {
+    const unsigned nb = (unsigned)((n + TW_O - 1) / TW_O);
+    int *keep = w.fail + 2;   // the solver's flag block: fail, nfallback, keep
+    hipLaunchKernelGGL(k_trid_keep_set, dim3(1), dim3(1), 0, s, keep);
+    hipLaunchKernelGGL(k_trid_window<T>, dim3(nb), dim3(TW_T), 0, s, n, cs, A, D, C, B, X, B2, Z, stop, keep);
+    // the bit-exact solver runs only where a window was too weakly dominant
+    // (keep cleared); it also returns at entry once the q-l loop has stopped,
+    // since then the window kernel left keep set
+    launch_trid<T>(w, n, cs, A, D, C, B, X, s, B2, Z, keep);
+}
+// reps back-to-back launches of the window kernel between events e0 and e1
+// (X written; no second right-hand side).
+template <typename T>
+void launch_trid_window_bench(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
+                              const T *B, T *X, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    const unsigned nb = (unsigned)((n + TW_O - 1) / TW_O);
+    int *keep = w.fail + 2;
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < reps; k++)
+        hipLaunchKernelGGL(k_trid_window<T>, dim3(nb), dim3(TW_T), 0, s, n, cs, A, D, C, B, X, nullptr, nullptr,
+                           nullptr, keep);
+    (void)hipEventRecord(e1, s);
+}
+template void launch_trid_window_bench<float>(const TridWorkT<float> &, long, const Chains &, const float *,
+                                              const float *, const float *, const float *, float *, int, hipStream_t,
+                                              hipEvent_t, hipEvent_t);
+template void launch_trid_window<float>(const TridWorkT<float> &, long, const Chains &, const float *, const float *,
+                                        const float *, const float *, float *, hipStream_t, float *,
+                                        const unsigned char *, const int *);
+
 int trid_blocks(long n) { return (int)((n + TR_R - 1) / TR_R); }
 
 template <typename T>
@@ -613,6 +815,34 @@ void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, co
     hipLaunchKernelGGL(k_trid_finish<T>, dim3(1), dim3(TRF_T), 0, s, n, cs, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
                        w.blk_xe, w.fail, w.nfallback, B2, Z, stop);
 }
+// reps back-to-back launches of the forward (which = 1) or backward (2)
+// kernel of one solve between events e0 and e1, after a full solve that puts
+// the forward kernel's outputs in place; a full solve afterwards leaves the
+// flags and X consistent.
+template <typename T>
+void launch_trid_kernel_bench(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
+                              const T *B, T *X, int which, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    const int nb = trid_blocks(n);
+    launch_trid<T>(w, n, cs, A, D, C, B, X, s);
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < reps; k++) {
+        if (which == 1)
+            hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c,
+                               w.blk_b, nullptr);
+        else
+            hipLaunchKernelGGL(k_trid_bwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c,
+                               w.blk_b, X, w.blk_x, w.blk_xe, w.fail, nullptr, nullptr, nullptr);
+    }
+    (void)hipEventRecord(e1, s);
+    launch_trid<T>(w, n, cs, A, D, C, B, X, s);
+}
+template void launch_trid_kernel_bench<double>(const TridWorkT<double> &, long, const Chains &, const double *,
+                                               const double *, const double *, const double *, double *, int, int,
+                                               hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_trid_kernel_bench<float>(const TridWorkT<float> &, long, const Chains &, const float *,
+                                              const float *, const float *, const float *, float *, int, int,
+                                              hipStream_t, hipEvent_t, hipEvent_t);
+
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
                  hipStream_t s, T *B2, const unsigned char *Z) {

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/wolfd2_amd.h"
@@ -25,6 +26,7 @@ static int g_verbose = 1;     // print the reference's warnings (pressure.f:242-
 static int g_lex_rows = 1;    // lexicographic Sor as one single-workgroup launch when the rows fit
 static int g_uniform = 1;     // tiled SorRB takes rau/rgv as two values on uniform Cartesian grids
 static int g_graph_epoch = 0; // bumped by the global switches a captured step bakes in
+static int g_window_chains = 1;   // mixed path: FP32 chains by the windowed PCR solver
 
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -269,7 +271,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->ch2 = Chain{take(nchain), take(nchain), take(nchain), take(nchain), take(nchain)};
     HIPCHK(hipMalloc(&c->zm, nchain));
     const int nb = trid_blocks((long)nchain) + 2;
-    HIPCHK(hipMalloc(&c->tw.cp, (2 * nchain + 4 * nb) * sizeof(double) + 2 * sizeof(int) + 64));
+    HIPCHK(hipMalloc(&c->tw.cp, (2 * nchain + 4 * nb) * sizeof(double) + 4 * sizeof(int) + 64));
     c->tw.bp = c->tw.cp + nchain;
     c->tw.blk_c = c->tw.bp + nchain;
     c->tw.blk_b = c->tw.blk_c + nb;
@@ -285,7 +287,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->chf2 = ChainT<float>{F(c->ch2.a), F(c->ch2.d), F(c->ch2.c), F(c->ch2.b), F(c->ch2.x)};
     c->twf = TridWorkT<float>{F(c->tw.cp),    F(c->tw.bp),     F(c->tw.blk_c), F(c->tw.blk_b),
                               F(c->tw.blk_x), F(c->tw.blk_xe), c->tw.fail,     c->tw.nfallback};
-    HIPCHK(hipMemsetAsync(c->tw.fail, 0, 2 * sizeof(int), c->s));
+    HIPCHK(hipMemsetAsync(c->tw.fail, 0, 4 * sizeof(int), c->s));
     const size_t npart = 4 * (size_t)RED_MAX_BLOCKS + 2 * (size_t)(c->nf / 256 + 64);
     HIPCHK(hipMalloc(&c->partials, npart * sizeof(double)));
     HIPCHK(hipMalloc(&c->sc, sizeof(Scalars)));
@@ -615,8 +617,25 @@ static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const Mom
     launch_mom_asm<T>(c->R, ld, 2, dk, re, fr, M, t2, c->us, c->vs, c->un, c->vn, c->d, c->dn, shift(ch, n1),
                       shift(ch2, n1), c->zm + n1, s, &c->sc->err, live, stop);
     const Chains cs{n1, n1, n2};
-    launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop);
-    launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop);
+    if constexpr (std::is_same<T, float>::value) {
+        // The window solver serves iterations 2.. of the q-l loop.  The first
+        // iteration's solve is the whole update from time level n and may be
+        // the step's result (the test after one iteration can be met, as on
+        // the first step from rest), so it takes the sequential solver.  Per
+        // solve the two have errors of one size, but on a 1024^2 cavity from
+        // rest the window solver in the first iteration leaves v 1.3e-6 from
+        // the FP64 path after two steps (relative L2), against 2.8e-7 this way.
+        if (g_window_chains && m > 1) {
+            launch_trid_window<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop);
+            launch_trid_window<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop);
+        } else {
+            launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop);
+            launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop);
+        }
+    } else {
+        launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop);
+        launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop);
+    }
     launch_scatter_update<T>(nx, ny, ld, ch2.x, ch2.x + n1, c->us, c->vs, c->dus, c->dvs, c->partials, c->sc, qtol,
                              m, s);
 }
@@ -1602,6 +1621,13 @@ int wd2_set_verbose(int enable) {
     return old;
 }
 
+int wd2_set_window_chains(int enable) {
+    const int old = g_window_chains;
+    g_window_chains = enable ? 1 : 0;
+    if (g_window_chains != old) g_graph_epoch++;
+    return old;
+}
+
 int wd2_set_uniform_coefficients(int enable) {
     const int old = g_uniform;
     g_uniform = enable ? 1 : 0;
@@ -1844,6 +1870,31 @@ int wd2_alt_trid_lu(int n, const double *a, double *b) {
     HIPCHK(hipMemcpyAsync(c->ch.b, b, n * 8, hipMemcpyHostToDevice, c->s));
     launch_trid(c->tw, n, n, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, c->s);
     HIPCHK(hipMemcpyAsync(b, c->ch.x, n * 8, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return WD2_OK;
+}
+
+int wd2_trid_f32(long n, long n1, long L1, long L2, const float *a, const float *d, const float *cc, const float *b,
+                 float *x, int window) {
+    if (n < 1 || n1 < 0 || n1 > n || L1 < 1 || L2 < 1 || (n1 > 0 && n1 % L1) || (n - n1) % L2)
+        return fail(WD2_ERR_ARG, "wd2_trid_f32: chains of %ld and %ld rows do not tile %ld and %ld rows", L1, L2, n1,
+                    n - n1);
+    int st;
+    const int side = (int)std::ceil(std::sqrt((double)n)) + 8;
+    if ((st = dropin_ctx(side, side, side + 2))) return st;
+    wd2_ctx *c = g_ctx;
+    if ((size_t)n > (size_t)c->nx * c->ny) return fail(WD2_ERR_ARG, "chain too long");
+    const ChainT<float> &ch = c->chf;
+    HIPCHK(hipMemcpyAsync(ch.a, a, n * 4, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(ch.d, d, n * 4, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(ch.c, cc, n * 4, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(ch.b, b, n * 4, hipMemcpyHostToDevice, c->s));
+    const Chains cs{n1, n1 > 0 ? L1 : L2, L2};
+    if (window)
+        launch_trid_window<float>(c->twf, n, cs, ch.a, ch.d, ch.c, ch.b, ch.x, c->s, nullptr, nullptr, nullptr);
+    else
+        launch_trid<float>(c->twf, n, cs, ch.a, ch.d, ch.c, ch.b, ch.x, c->s);
+    HIPCHK(hipMemcpyAsync(x, ch.x, n * 4, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return WD2_OK;
 }
@@ -2217,6 +2268,36 @@ int wd2_ctx_mg_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     (void)hipEventDestroy(e[1]);
     if (avg_us) *avg_us = 1e3 * ms / reps;
     return WD2_OK;
+}
+
+int wd2_ctx_trid_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
+    if (which < 1 || which > 3 || reps < 1)
+        return fail(WD2_ERR_ARG, "chain solver timing: which %d (1 forward, 2 backward, 3 window), reps %d", which,
+                    reps);
+    if (which == 3 && c->precision != WD2_PREC_MIXED)
+        return fail(WD2_ERR_ARG, "chain solver timing: the window solver serves the mixed path only");
+    if (c->any_porous) return fail(WD2_ERR_UNSUPPORTED, "chain solver timing: the two-chain batches only");
+    const long n1 = (long)c->nx * (c->ny - 1), n2 = (long)(c->nx - 1) * c->ny;
+    const Chains cs{n1, n1, n2};
+    hipEvent_t e[2];
+    HIPCHK(hipEventCreate(&e[0]));
+    HIPCHK(hipEventCreate(&e[1]));
+    if (which == 3)
+        launch_trid_window_bench<float>(c->twf, n1 + n2, cs, c->chf.a, c->chf.d, c->chf.c, c->chf.b, c->chf.x, reps,
+                                        c->s, e[0], e[1]);
+    else if (c->precision == WD2_PREC_MIXED)
+        launch_trid_kernel_bench<float>(c->twf, n1 + n2, cs, c->chf.a, c->chf.d, c->chf.c, c->chf.b, c->chf.x, which,
+                                        reps, c->s, e[0], e[1]);
+    else
+        launch_trid_kernel_bench<double>(c->tw, n1 + n2, cs, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, which,
+                                         reps, c->s, e[0], e[1]);
+    HIPCHK(hipStreamSynchronize(c->s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    if (avg_us) *avg_us = 1e3 * ms / reps;
+    return check_err(c);
 }
 
 int wd2_ctx_set_graph(wd2_ctx *c, int enable) {

@@ -35,6 +35,22 @@ template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
                  T *X, hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr, const int *stop = nullptr);
 
+// The mixed path's FP32 chain solve (not bit-exact): overlapping windows
+// solved by partitioned Thomas + PCR in LDS, the bit-exact solver as the
+// fallback for a window that is not diagonally dominant enough; same
+// arguments as launch_trid.
+template <typename T>
+void launch_trid_window(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
+                        const T *B, T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop);
+template <typename T>
+void launch_trid_window_bench(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
+                              const T *B, T *X, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+// reps back-to-back launches of the forward (which = 1) or backward (2)
+// kernel of a solve of these chains between events e0 and e1.
+template <typename T>
+void launch_trid_kernel_bench(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
+                              const T *B, T *X, int which, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+
 // Boundary fills (one workgroup).  err receives 1000+side for a bad flag.
 template <typename T>
 void launch_vel_bc(const Regions &R, int ld, T *u, T *v, bool outflow_only, int *err, hipStream_t s,
