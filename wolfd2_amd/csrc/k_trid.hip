@@ -616,14 +616,16 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nb
 // in registers and expresses its interior rows through its first and last
 // row (three 6-row Thomas solves: particular, left and right spikes), which
 // leaves two interface rows per thread, a tridiagonal system of 2 TW_T rows;
-// parallel cyclic reduction in LDS solves it (log2(2 TW_T) levels, each row
-// combining with the rows 2^l away), and the interior rows follow from the
+// parallel cyclic reduction in LDS solves it (each level l combines a row
+// with the rows 2^l away; at most log2(2 TW_T) levels, fewer once the
+// couplings left are negligible), and the interior rows follow from the
 // interface values.  A window whose dominance ratio exceeds TW_DOM clears
 // *keep, and the bit-exact speculative solver (launched after this kernel
 // with stop = keep) solves the chains instead.
 constexpr int TW_T = 256, TW_S = 8, TW_N = TW_T * TW_S, TW_H = 64, TW_O = TW_N - 2 * TW_H;
 constexpr int TW_M = 2 * TW_T;   // interface rows per window
 constexpr float TW_DOM = 0.9f;
+constexpr float TW_EPS = 0x1p-48f;   // PCR stops once every coupling is below this share of its diagonal
 
 __global__ void k_trid_keep_set(int *keep) { *keep = 1; }
 
@@ -730,9 +732,15 @@ __global__ __launch_bounds__(TW_T) void k_trid_window(long n, Chains cs, const T
     }
     __syncthreads();
     // PCR on the interface rows: row i eliminates its coupling to rows i-h
-    // and i+h with those rows' equations
+    // and i+h with those rows' equations.  Each level squares the couplings
+    // of a dominant system (the interface rows alternate the coupling of two
+    // adjacent rows with that across a segment, the product of TW_S - 1
+    // dominant rows), so the reduction stops at the first level after which
+    // every row's remaining coupling is below TW_EPS of its diagonal; the
+    // dropped terms change x by less than TW_EPS times x 2^l segments away.
     int cur = 0;
     for (int h = 1; h < TW_M; h <<= 1) {
+        bool coupled = false;
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const int i = 2 * t + e;
@@ -755,18 +763,34 @@ __global__ __launch_bounds__(TW_T) void k_trid_window(long n, Chains cs, const T
             rc[cur ^ 1][i] = cn;
             rb[cur ^ 1][i] = bn;
             rq[cur ^ 1][i] = wrcp(dn);
+            coupled |= !(fabsf(an) + fabsf(cn) <= TW_EPS * fabsf(dn));
         }
-        __syncthreads();
+        const int more = __syncthreads_or(coupled);
         cur ^= 1;
+        if (!more) break;
     }
+    // the halo is TW_H / TW_S whole segments at either end, so a lane's rows
+    // are all kept or all dropped; kept segments are stored as 16-byte vectors
+    if (t < TW_H / TW_S || t >= TW_T - TW_H / TW_S || k0 >= n) return;
     const T x0 = rb[cur][2 * t] * rq[cur][2 * t], x7 = rb[cur][2 * t + 1] * rq[cur][2 * t + 1];
+    T x[TW_S];
 #pragma unroll
-    for (int s = 0; s < TW_S; s++) {
-        const long k = w0 + (long)t * TW_S + s;
-        if (k < own0 || k >= own0 + TW_O || k >= n) continue;
-        const T xs = s == 0 ? x0 : (s == TW_S - 1 ? x7 : y[s] + v[s] * x0 + w[s] * x7);
-        X[k] = xs;
-        if (B2) B2[k] = (Z && Z[k]) ? T(0) : xs;
+    for (int s = 0; s < TW_S; s++) x[s] = s == 0 ? x0 : (s == TW_S - 1 ? x7 : y[s] + v[s] * x0 + w[s] * x7);
+    const int len = (int)(n - k0 < TW_S ? n - k0 : TW_S);
+    seg_store(X, k0, len, x);
+    if (B2) {
+        unsigned long long zb = 0;   // the segment's identity-row flags, one byte per row
+        if (Z) {
+            if (len == TW_S) {
+                zb = *reinterpret_cast<const unsigned long long *>(Z + k0);
+            } else {
+                for (int q = 0; q < len; q++) zb |= (unsigned long long)Z[k0 + q] << (8 * q);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TW_S; q++)
+            if ((zb >> (8 * q)) & 0xffu) x[q] = T(0);
+        seg_store(B2, k0, len, x);
     }
 }
 
