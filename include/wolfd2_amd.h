@@ -318,10 +318,11 @@ int wd2_ctx_kernel_timing(wd2_ctx *ctx, int enable, double *avg_us, long *launch
  * maximum.  No counterpart in the reference. */
 int wd2_ctx_ppe_divergence(wd2_ctx *ctx, double *last, double *max_since, int reset);
 /* Mean duration (microseconds) of the multigrid solver's level-0 down
- * (which = 2) or up (which = 3) kernel over `reps` back-to-back launches on
- * the context's last solve (event pairs around single launches in the
- * solve's own cycles include the gap between launches); p is restored
- * afterwards.  No counterpart in the reference. */
+ * (which = 2) or up (which = 3) kernel, or of its one-workgroup bottom kernel
+ * (4), over `reps` back-to-back launches on the context's last solve (event
+ * pairs around single launches in the solve's own cycles include the gap
+ * between launches); p is restored afterwards.  No counterpart in the
+ * reference. */
 int wd2_ctx_mg_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
 /* Mean time (microseconds) of the chain solver's forward (which = 1) or
  * backward (2) kernel, or of the mixed path's window kernel (3, mixed

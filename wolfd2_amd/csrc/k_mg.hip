@@ -411,103 +411,231 @@ __global__ __launch_bounds__(NT) void k_mg_up(MgLevel L, MgLevel C, int ntx, int
 
 // ------------------------------------------------------------ bottom
 // Levels lb..nlev-1 (each at most MG_BSIDE cells wide) in one workgroup: the
-// V-cycle below level lb with the corrections and right-hand sides in LDS and
-// the face coefficients read from their (small, cache-resident) level arrays.
-// Level lb's r comes from the down kernel above; its correction goes to c for
-// the up kernel above.  D.nt > 0: the stopping test is evaluated first.
+// V-cycle below level lb with the corrections and right-hand sides in LDS.
+// The face coefficients are copied into LDS first where they fit
+// (mg_bottom_faces_fit: compact rows of nx + 1 x-faces and nx y-faces per
+// level), so the ~50 sweep phases between barriers read only LDS; otherwise
+// they are read from the (small, cache-resident) level arrays.  Both give the
+// same bits.  Level lb's r comes from the down kernel above; its correction
+// goes to c for the up kernel above.  D.nt > 0: the stopping test is
+// evaluated first.
 struct MgBottom {
     MgLevel lev[MG_MAXLEV];
     int nlev;   // bottom levels
 };
 
-// One colour of a red-black Gauss-Seidel sweep of level L (E, R in LDS with
-// row stride nx + 2).
+// LDS face values per direction: every bottom level of a 64-wide bottom in
+// FP32, of a 32-wide one in FP64 (the FP64 corrections and right-hand sides
+// take the rest of the LDS).
 template <typename T>
-__device__ inline void bt_colour(T *E, const T *R, const MgLevel &L, int c) {
+constexpr int mg_bfcap() { return sizeof(T) == 4 ? 5632 : 1440; }
+
+// Faces of one bottom level: the LDS copy (LF) or the level arrays.
+template <typename T, bool LF>
+struct BtFaces {
+    const T *fx, *fy;   // LF: LDS rows of NX + 1 (x) and NX (y) values
+    const T *ax, *ay;   // the level arrays
+    int ld, NX;
+    __device__ T x(int I, int J) const { return LF ? fx[I + (NX + 1) * J] : ax[mfx(ld, I, J)]; }
+    __device__ T y(int I, int J) const { return LF ? fy[I + NX * J] : ay[mfy(ld, I, J)]; }
+};
+
+// Red-black Gauss-Seidel sweeps of level L (E, R in LDS with row stride
+// nx + 2): nsweep sweeps of colours 0 then 1 (rev: 1 then 0), a sync after
+// each colour, by S threads.  A thread's points (point q of a colour: row
+// q / half, position q % half; q = thread + p S, at most P per thread and
+// colour) keep their index, face
+// coefficients, diagonal and right-hand side in registers across the sweeps
+// (R does not change while a level is smoothed), so a phase reads only the
+// four neighbours in E.  The update is the same expression on the same
+// values as a point-by-point evaluation.
+template <typename T, bool LF, int P, int S, typename Sync>
+__device__ inline void bt_smooth(T *E, const T *R, const MgLevel &L, const BtFaces<T, LF> &F, int nsweep, bool rev,
+                                 Sync sync) {
     const int NX = L.nx, NY = L.ny, W = NX + 2;
-    const T *ax = (const T *)L.ax, *ay = (const T *)L.ay;
     const int half = (NX + 1) / 2;   // points of one colour per row, at most
-    for (int q = threadIdx.x; q < half * NY; q += MG_BT) {
-        const int J = q / half, I = 2 * (q % half) + ((c ^ J) & 1);
-        if (I >= NX) continue;
-        const T aw = ax[mfx(L.ld, I, J)], ae = ax[mfx(L.ld, I + 1, J)];
-        const T as = ay[mfy(L.ld, I, J)], an = ay[mfy(L.ld, I, J + 1)];
-        const T a3 = -(aw + ae + as + an);
-        const int k = (I + 1) + W * (J + 1);
-        E[k] = (R[k] - (as * E[k - W] + aw * E[k - 1] + ae * E[k + 1] + an * E[k + W])) / a3;
-    }
+    int kk[2][P];
+    T cw[2][P], ce[2][P], cs[2][P], cn[2][P], ca[2][P], cr[2][P];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int q = threadIdx.x + p * S;
+            const int J = q / half, I = 2 * (q % half) + ((c ^ J) & 1);
+            kk[c][p] = -1;
+            if (q < half * NY && I < NX) {
+                cw[c][p] = F.x(I, J);
+                ce[c][p] = F.x(I + 1, J);
+                cs[c][p] = F.y(I, J);
+                cn[c][p] = F.y(I, J + 1);
+                ca[c][p] = -(cw[c][p] + ce[c][p] + cs[c][p] + cn[c][p]);
+                kk[c][p] = (I + 1) + W * (J + 1);
+                cr[c][p] = R[kk[c][p]];
+            }
+        }
+    for (int s = 0; s < nsweep; s++)
+#pragma unroll
+        for (int cc = 0; cc < 2; cc++) {
+            const int c = rev ? 1 - cc : cc;
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const int k = kk[c][p];
+                if (k >= 0)
+                    E[k] = (cr[c][p] - (cs[c][p] * E[k - W] + cw[c][p] * E[k - 1] + ce[c][p] * E[k + 1] +
+                                        cn[c][p] * E[k + W])) /
+                           ca[c][p];
+            }
+            sync();
+        }
 }
 
-template <typename T>
+template <typename T, bool LF>
 __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, MgDecide D, Scalars *sc) {
     if (mg_stopped(sc)) return;
+    constexpr int FC = LF ? mg_bfcap<T>() : 1;
     __shared__ T E[MG_BCAP], R[MG_BCAP];
+    __shared__ T FX[FC], FY[FC];
     __shared__ double red[MG_BT / 64];
     if (D.nt > 0 && mg_decide<MG_BT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, sc, red)) return;
-    int off[MG_MAXLEV];
-    int o = 0;
-    for (int l = 0; l < B.nlev; l++) {
-        off[l] = o;
-        o += (B.lev[l].nx + 2) * (B.lev[l].ny + 2);
-    }
-    // r of the first level; zero e (and the rings) everywhere
-    for (int l = 0; l < B.nlev; l++) {
-        const MgLevel &L = B.lev[l];
-        const int W = L.nx + 2, n = W * (L.ny + 2);
-        const T *r = (const T *)L.r;
-        for (int q = threadIdx.x; q < n; q += MG_BT) {
-            const int a = q % W, bb = q / W;   // padded position
-            E[off[l] + q] = T(0);
-            R[off[l] + q] = (l == 0 && a >= 1 && a <= L.nx && bb >= 1 && bb <= L.ny) ? r[(size_t)a + (size_t)L.ld * bb]
-                                                                                   : T(0);
+    // each level's offset in E and R, and in FX and FY (in LDS: indexed by
+    // the level, they would otherwise go to scratch)
+    __shared__ int off[MG_MAXLEV], ofx[MG_MAXLEV], ofy[MG_MAXLEV];
+    if (threadIdx.x == 0) {
+        int o = 0, ox = 0, oy = 0;
+        for (int l = 0; l < B.nlev; l++) {
+            const int NX = B.lev[l].nx, NY = B.lev[l].ny;
+            off[l] = o;
+            ofx[l] = ox;
+            ofy[l] = oy;
+            o += (NX + 2) * (NY + 2);
+            ox += (NX + 1) * NY;
+            oy += NX * (NY + 1);
         }
     }
     __syncthreads();
-    // down
-    for (int l = 0; l + 1 < B.nlev; l++) {
+    auto faces = [&](int l) {
         const MgLevel &L = B.lev[l];
+        return BtFaces<T, LF>{FX + ofx[l], FY + ofy[l], (const T *)L.ax, (const T *)L.ay, L.ld, L.nx};
+    };
+    // Set-up: every global load first (r of the first level, the faces),
+    // then the LDS stores, so the loads' latencies overlap.  E is zeroed
+    // everywhere, R everywhere but the first level's interior.
+    constexpr int RB = (MG_BSIDE * MG_BSIDE + MG_BT - 1) / MG_BT;   // r values per thread
+    constexpr int FB = LF ? (mg_bfcap<T>() + MG_BT - 1) / MG_BT : 1;   // face values per thread and direction
+    const MgLevel &L0 = B.lev[0];
+    const int n0 = L0.nx * L0.ny;
+    T rv[RB], fxv[FB], fyv[FB];
+#pragma unroll
+    for (int it = 0; it < RB; it++) {
+        const int q = threadIdx.x + it * MG_BT;
+        if (q < n0) rv[it] = ((const T *)L0.r)[(size_t)(q % L0.nx + 1) + (size_t)L0.ld * (q / L0.nx + 1)];
+    }
+    const int nl = B.nlev - 1;
+    const int nfx = ofx[nl] + (B.lev[nl].nx + 1) * B.lev[nl].ny, nfy = ofy[nl] + B.lev[nl].nx * (B.lev[nl].ny + 1);
+    if (LF) {
+#pragma unroll
+        for (int it = 0; it < FB; it++) {
+            const int q = threadIdx.x + it * MG_BT;
+            if (q < nfx) {
+                int l = 0;
+                while (l < nl && q >= ofx[l + 1]) l++;
+                const MgLevel &L = B.lev[l];
+                const int u = q - ofx[l];
+                fxv[it] = ((const T *)L.ax)[mfx(L.ld, u % (L.nx + 1), u / (L.nx + 1))];
+            }
+            if (q < nfy) {
+                int l = 0;
+                while (l < nl && q >= ofy[l + 1]) l++;
+                const MgLevel &L = B.lev[l];
+                const int u = q - ofy[l];
+                fyv[it] = ((const T *)L.ay)[mfy(L.ld, u % L.nx, u / L.nx)];
+            }
+        }
+    }
+    for (int l = 0; l < B.nlev; l++) {
+        const MgLevel &L = B.lev[l];
+        const int W = L.nx + 2, n = W * (L.ny + 2);
+        for (int q = threadIdx.x; q < n; q += MG_BT) {
+            const int a = q % W, bb = q / W;   // padded position
+            E[off[l] + q] = T(0);
+            if (!(l == 0 && a >= 1 && a <= L.nx && bb >= 1 && bb <= L.ny)) R[off[l] + q] = T(0);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < RB; it++) {
+        const int q = threadIdx.x + it * MG_BT;
+        if (q < n0) R[off[0] + (q % L0.nx + 1) + (L0.nx + 2) * (q / L0.nx + 1)] = rv[it];
+    }
+    if (LF) {
+#pragma unroll
+        for (int it = 0; it < FB; it++) {
+            const int q = threadIdx.x + it * MG_BT;
+            if (q < nfx) FX[q] = fxv[it];
+            if (q < nfy) FY[q] = fyv[it];
+        }
+    }
+    __syncthreads();
+    // The levels from lw on (at most 64 points of a colour: one per lane; the
+    // levels shrink) run in wavefront 0 alone, whose phases need only a
+    // wavefront sync; the wider ones use the whole workgroup between
+    // barriers.  The operations and their order are the same either way.
+    int lw = B.nlev;
+    for (int l = B.nlev - 1; l >= 0 && ((B.lev[l].nx + 1) / 2) * B.lev[l].ny <= 64; l--) lw = l;
+    const bool w0 = threadIdx.x < 64;
+    auto sync = [](bool wave) {
+        if (wave) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    };
+    // a level's sweeps: in wavefront 0 (one point per lane and colour) or by the whole workgroup (at most MG_BSIDE^2 / 2
+    // points per colour: two per thread)
+    static_assert(MG_BSIDE * MG_BSIDE / 2 <= 2 * MG_BT, "two points per thread and colour");
+    auto smooth = [&](T *El, const T *Rl, const MgLevel &L, const BtFaces<T, LF> &F, int nsweep, bool rev,
+                      bool wave) {
+        if (wave)
+            bt_smooth<T, LF, 1, 64>(El, Rl, L, F, nsweep, rev, [&] { sync(true); });
+        else
+            bt_smooth<T, LF, 2, MG_BT>(El, Rl, L, F, nsweep, rev, [&] { sync(false); });
+    };
+    // pre-smoothing of level l and the restriction of its residual
+    auto down = [&](int l, bool wave) {
+        const int step = wave ? 64 : MG_BT;
+        const MgLevel &L = B.lev[l];
+        const BtFaces<T, LF> F = faces(l);
         const int NX = L.nx, NY = L.ny, W = NX + 2;
         T *El = E + off[l], *Rl = R + off[l];
-        for (int s = 0; s < MG_NU; s++)
-            for (int c = 0; c < 2; c++) {
-                bt_colour(El, Rl, L, c);
-                __syncthreads();
-            }
-        const T *ax = (const T *)L.ax, *ay = (const T *)L.ay;
+        smooth(El, Rl, L, F, MG_NU, false, wave);
         const int NXc = B.lev[l + 1].nx, NYc = B.lev[l + 1].ny, Wc = NXc + 2;
         T *Rc = R + off[l + 1];
-        for (int q = threadIdx.x; q < NXc * NYc; q += MG_BT) {
+        for (int q = threadIdx.x; q < NXc * NYc; q += step) {
             const int Ic = q % NXc, Jc = q / NXc;
             T sum = T(0);
             for (int dy = 0; dy < 2; dy++)
                 for (int dx = 0; dx < 2; dx++) {
                     const int I = 2 * Ic + dx, J = 2 * Jc + dy;
                     if (I >= NX || J >= NY) continue;
-                    const T aw = ax[mfx(L.ld, I, J)], ae = ax[mfx(L.ld, I + 1, J)];
-                    const T as = ay[mfy(L.ld, I, J)], an = ay[mfy(L.ld, I, J + 1)];
+                    const T aw = F.x(I, J), ae = F.x(I + 1, J);
+                    const T as = F.y(I, J), an = F.y(I, J + 1);
                     const T a3 = -(aw + ae + as + an);
                     const int k = (I + 1) + W * (J + 1);
                     sum += Rl[k] - (as * El[k - W] + aw * El[k - 1] + a3 * El[k] + ae * El[k + 1] + an * El[k + W]);
                 }
             Rc[(Ic + 1) + Wc * (Jc + 1)] = sum;
         }
-        __syncthreads();
-    }
-    {
-        const int l = B.nlev - 1;
-        for (int s = 0; s < MG_NCOARSE; s++)
-            for (int c = 0; c < 2; c++) {
-                bt_colour(E + off[l], R + off[l], B.lev[l], c);
-                __syncthreads();
-            }
-    }
-    // up
-    for (int l = B.nlev - 2; l >= 0; l--) {
+        sync(wave);
+    };
+    // the coarse correction of level l + 1 added to level l, post-smoothing
+    auto up = [&](int l, bool wave) {
+        const int step = wave ? 64 : MG_BT;
         const int NX = B.lev[l].nx, NY = B.lev[l].ny, W = NX + 2;
         const int NXc = B.lev[l + 1].nx, Wc = NXc + 2;
         T *El = E + off[l];
         const T *Ec = E + off[l + 1];
-        for (int q = threadIdx.x; q < NX * NY; q += MG_BT) {
+        for (int q = threadIdx.x; q < NX * NY; q += step) {
             const int I = q % NX, J = q / NX;
             const int ic = (I >> 1) + 1, jc = (J >> 1) + 1;   // padded coarse position
             const int dx = (I & 1) ? 1 : -1, dy = (J & 1) ? 1 : -1;
@@ -517,13 +645,27 @@ __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, MgDecide D, Sca
                          T(0.0625);
             El[(I + 1) + W * (J + 1)] += pc;
         }
+        sync(wave);
+        smooth(El, R + off[l], B.lev[l], faces(l), MG_NU, true, wave);
+    };
+    // sweeps on the coarsest level
+    auto coarsest = [&](bool wave) {
+        const int l = B.nlev - 1;
+        smooth(E + off[l], R + off[l], B.lev[l], faces(l), MG_NCOARSE, false, wave);
+    };
+    const int lb = lw < B.nlev ? lw : B.nlev - 1;   // levels below lb: whole workgroup
+    for (int l = 0; l < lb; l++) down(l, false);
+    if (lw < B.nlev) {
+        if (w0) {
+            for (int l = lw; l + 1 < B.nlev; l++) down(l, true);
+            coarsest(true);
+            for (int l = B.nlev - 2; l >= lw; l--) up(l, true);
+        }
         __syncthreads();
-        for (int s = 0; s < MG_NU; s++)
-            for (int c = 1; c >= 0; c--) {
-                bt_colour(El, R + off[l], B.lev[l], c);
-                __syncthreads();
-            }
+    } else {
+        coarsest(false);
     }
+    for (int l = lb - 1; l >= 0; l--) up(l, false);
     const MgLevel &L = B.lev[0];
     T *e = (T *)L.c;
     const int W = L.nx + 2;
@@ -531,6 +673,25 @@ __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, MgDecide D, Sca
         const int I = q % L.nx, J = q / L.nx;
         e[(size_t)(I + 1) + (size_t)L.ld * (J + 1)] = E[(I + 1) + W * (J + 1)];
     }
+}
+
+// Whether the face coefficients of the bottom levels fit the kernel's LDS copy.
+template <typename T>
+static bool mg_bottom_faces_fit(const MgBottom &B) {
+    int fx = 0, fy = 0;
+    for (int l = 0; l < B.nlev; l++) {
+        fx += (B.lev[l].nx + 1) * B.lev[l].ny;
+        fy += B.lev[l].nx * (B.lev[l].ny + 1);
+    }
+    return fx <= mg_bfcap<T>() && fy <= mg_bfcap<T>();
+}
+
+template <typename T>
+static void launch_mg_bottom(const MgBottom &B, const MgDecide &D, Scalars *sc, hipStream_t s) {
+    if (mg_bottom_faces_fit<T>(B))
+        hipLaunchKernelGGL((k_mg_bottom<T, true>), dim3(1), dim3(MG_BT), 0, s, B, D, sc);
+    else
+        hipLaunchKernelGGL((k_mg_bottom<T, false>), dim3(1), dim3(MG_BT), 0, s, B, D, sc);
 }
 
 // ------------------------------------------------------------ control
@@ -642,7 +803,7 @@ void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double 
     MgBottom B{};
     B.nlev = H.nlev - H.ntiled;
     for (int l = 0; l < B.nlev; l++) B.lev[l] = H.lev[H.ntiled + l];
-    hipLaunchKernelGGL(k_mg_bottom<T>, dim3(1), dim3(MG_BT), 0, s, B, H.ntiled == 1 ? D : none, sc);
+    launch_mg_bottom<T>(B, H.ntiled == 1 ? D : none, sc, s);
     for (int l = H.ntiled - 1; l >= 1; l--) {
         const int nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
         hipLaunchKernelGGL((k_mg_up<T, false, MG_NT1, false>), dim3(8 * ((nt + 7) / 8)), dim3(MG_NT1), 0, s, H.lev[l],
@@ -654,7 +815,9 @@ void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double 
 }
 
 // `reps` back-to-back launches of the level-0 down (which = 2) or up
-// (which = 3) kernel between one event pair, on the state of the last solve
+// (which = 3) kernel, or of the bottom kernel (4; it writes only the
+// correction c of its first level), between one event pair, on the state of
+// the last solve
 // (the stop flag is cleared first and set again after; the up kernel adds
 // the last correction to p each time, so the caller restores p).
 template <typename T>
@@ -662,12 +825,18 @@ void launch_mg_kernel_bench(const MgHierarchy &H, int which, int reps, int nx, i
                             double *p, const double *rau, const double *rgv, const double *uni, double *part,
                             Scalars *sc, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     launch_mg_reset(sc, s);
+    MgBottom B{};
+    B.nlev = H.nlev - H.ntiled;
+    for (int l = 0; l < B.nlev; l++) B.lev[l] = H.lev[H.ntiled + l];
+    const MgDecide none{nullptr, 0, 0, 0, 0.0, 0.0};
     (void)hipEventRecord(e0, s);
     for (int k = 0; k < reps; k++) {
         if (which == 2)
             down0<T>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
-        else
+        else if (which == 3)
             up0<T>(H, ld, p, uni, sc, s);
+        else
+            launch_mg_bottom<T>(B, none, sc, s);
     }
     (void)hipEventRecord(e1, s);
 }

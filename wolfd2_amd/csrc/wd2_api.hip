@@ -2246,7 +2246,7 @@ int wd2_ctx_mg_autotune(wd2_ctx *c, double dk, int cycles, double *best_us) {
 
 int wd2_ctx_mg_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     if (!c->mg_store || c->mg_prec < 0) return fail(WD2_ERR_ARG, "no multigrid solve has run on this context");
-    if ((which != 2 && which != 3) || reps < 1) return fail(WD2_ERR_ARG, "mg kernel timing: which %d reps %d", which, reps);
+    if (which < 2 || which > 4 || reps < 1) return fail(WD2_ERR_ARG, "mg kernel timing: which %d reps %d", which, reps);
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     hipStream_t s = c->s;
     hipEvent_t e[2];
