@@ -393,8 +393,10 @@ int wd2_ctx_mg_cycle_time(wd2_ctx *ctx, double dk, int cycles, double *us);
  * the boundary values: bc_bar[((side-1)*2 + (var-1))*W + k] for side 1..4
  * (W, E, S, N), var 1..2 (u, v) and grid line k (j on W/E, i on S/N), W =
  * max(nx, ny) + 2 -- for a side with a constant value, the gradient of that
- * constant is the sum over its lines.  The context's fields are those after
- * the forward run when it returns. */
+ * constant is the sum over its lines.  The backward pass re-runs the
+ * segments from their checkpoints with the boundary values in force when it
+ * is called: keep them as they were for wd2_adj_forward.  The context's
+ * fields are those after the forward run when it returns. */
 typedef struct wd2_adj wd2_adj;
 typedef struct {
     int steps;               /* steps differentiated by the last backward pass */
@@ -418,6 +420,22 @@ int wd2_adj_set_tolerances(wd2_adj *adj, double fixed_point_tol, int max_iter, d
 int wd2_adj_forward(wd2_adj *adj, const wd2_params *prm, int nsteps, int *ql_iter, int *ppe_iter);
 int wd2_adj_backward(wd2_adj *adj, int ld, const double *u_bar, const double *v_bar, const double *p_bar,
                      double *u0_bar, double *v0_bar, double *p0_bar, double *bc_bar, wd2_adj_stats *stats);
+/* Tangent-linear (forward-mode) derivative of the same run: the forward run
+ * of the last wd2_adj_forward again from its initial state, each step
+ * followed by its tangent, along the direction (du0, dv0, dp0: host arrays
+ * of the initial u, v, p with leading dimension ld, any may be NULL for
+ * zero; dbc: boundary values laid out as bc_bar, NULL for zero).  Returns
+ * the derivative of the final u, v, p (any output may be NULL).  The q-l
+ * loops' tangents stop on the adjoint's fixed-point tolerance and the
+ * Poisson solves on its Poisson tolerance (wd2_adj_set_tolerances); stats
+ * report them in the adjoint's fields (adj_iters_*, ppe_cycles_*,
+ * adj_unconverged) and the pass's wall time in ms_reverse.  No counterpart
+ * in the reference: a check of the adjoint (<J d, w> = <d, J^T w>).  Like
+ * the backward pass it re-runs the forward steps with the boundary values in
+ * force when it is called: keep them as they were for wd2_adj_forward.  The
+ * context's fields are those after the forward run when it returns. */
+int wd2_adj_tangent(wd2_adj *adj, int ld, const double *du0, const double *dv0, const double *dp0,
+                    const double *dbc, double *duN, double *dvN, double *dpN, wd2_adj_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -384,3 +384,21 @@ class Adjoint:
         stats = {f: getattr(st, f) for f, _ in AdjStats._fields_}
         return {"u0_bar": out[0], "v0_bar": out[1], "p0_bar": out[2], "bc_bar": bc.reshape(4, 2, W),
                 "stats": stats}
+
+    def tangent(self, du0=None, dv0=None, dp0=None, dbc=None) -> dict:
+        """Tangent-linear derivative of the last forward run along a direction
+        of its initial fields (du0, dv0, dp0: (ncol, ld) arrays or None) and
+        boundary values (dbc[side-1, var-1, k] as bc_bar, or None): the
+        derivatives of the final u, v, p (duN, dvN, dpN) and the pass's
+        statistics.  The run is repeated from its initial state."""
+        g = self.solver.g
+        out = [np.zeros((g.ncol, g.ld)) for _ in range(3)]
+        W = max(g.nx, g.ny) + 2
+        st = AdjStats()
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        d0 = [arr(du0), arr(dv0), arr(dp0)]
+        bc = None if dbc is None else np.ascontiguousarray(np.asarray(dbc, dtype=np.float64).reshape(8 * W))
+        check(lib().wd2_adj_tangent(self.adj, g.ld, dp(d0[0]), dp(d0[1]), dp(d0[2]), dp(bc), dp(out[0]),
+                                    dp(out[1]), dp(out[2]), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in AdjStats._fields_}
+        return {"duN": out[0], "dvN": out[1], "dpN": out[2], "stats": stats}

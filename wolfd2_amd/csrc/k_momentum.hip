@@ -29,8 +29,8 @@ constexpr int MX = 64, MY = 4;
 static inline dim3 mgrid(int ni, int nj) { return dim3((ni + MX - 1) / MX, (nj + MY - 1) / MY); }
 
 // ------------------------------------------------------------- ConvCoef
-template <typename T>
-__device__ inline T cv(double x) { return (T)x; }
+template <typename T, typename S>
+__device__ inline T cv(S x) { return (T)x; }
 
 // The centred face coefficients of ConvCoef cases 1 (cc_x1, cc_x2) and 2
 // (cc_y1, cc_y2) at one point (momentum.f:901-925), shared by k_conv_coef and
@@ -39,37 +39,37 @@ __device__ inline T cv(double x) { return (T)x; }
 #define U(a, b) cv<T>(u[WD2_IX(a, b)])
 #define V(a, b) cv<T>(v[WD2_IX(a, b)])
 #define M(f, a, b) cv<T>(mget(f, ld, a, b))
-template <typename T>
+template <typename T, typename E>
 __device__ inline T cc_x1(int ld, int i, int j, T djac, const MArr &xet, const MArr &yet,
-                          const double *__restrict__ u, const double *__restrict__ v) {
+                          const E *__restrict__ u, const E *__restrict__ v) {
     return (djac * M(yet, i, j) * (U(i, j) + U(i - 1, j)) - M(xet, i, j) * (V(i, j) + V(i, j - 1))) * T(0.5);
 }
-template <typename T>
+template <typename T, typename E>
 __device__ inline T cc_x2(int ld, int i, int j, T djac, const MArr &xzi, const MArr &yzi,
-                          const double *__restrict__ u, const double *__restrict__ v) {
+                          const E *__restrict__ u, const E *__restrict__ v) {
     return (M(xzi, i, j) * (V(i + 1, j) + V(i, j)) - djac * M(yzi, i, j) * (U(i, j + 1) + U(i, j))) * T(0.5);
 }
-template <typename T>
+template <typename T, typename E>
 __device__ inline T cc_y1(int ld, int i, int j, T djac, const MArr &xet, const MArr &yet,
-                          const double *__restrict__ u, const double *__restrict__ v) {
+                          const E *__restrict__ u, const E *__restrict__ v) {
     return (M(yet, i, j) * (U(i, j + 1) + U(i, j)) - djac * M(xet, i, j) * (V(i + 1, j) + V(i, j))) * T(0.5);
 }
-template <typename T>
+template <typename T, typename E>
 __device__ inline T cc_y2(int ld, int i, int j, T djac, const MArr &xzi, const MArr &yzi,
-                          const double *__restrict__ u, const double *__restrict__ v) {
+                          const E *__restrict__ u, const E *__restrict__ v) {
     return (djac * M(xzi, i, j) * (V(i, j) + V(i, j - 1)) - M(yzi, i, j) * (U(i, j) + U(i - 1, j))) * T(0.5);
 }
 // ConvCoef cases 4 and 5 (upwind Jacobian, momentum.f:927-961)
-template <typename T>
+template <typename T, typename E>
 __device__ inline void cc_4(int ld, int i, int j, T djac, const MArr &xzi, const MArr &xet, const MArr &yzi,
-                            const MArr &yet, const double *__restrict__ u, const double *__restrict__ v, T &c1, T &c2) {
+                            const MArr &yet, const E *__restrict__ u, const E *__restrict__ v, T &c1, T &c2) {
     const T dFour = T(4.0);
     c1 = djac * M(yet, i, j) * U(i, j) - M(xet, i, j) * (V(i + 1, j) + V(i, j) + V(i + 1, j - 1) + V(i, j - 1)) / dFour;
     c2 = M(xzi, i, j) * (V(i + 1, j) + V(i, j) + V(i + 1, j - 1) + V(i, j - 1)) / dFour - djac * M(yzi, i, j) * U(i, j);
 }
-template <typename T>
+template <typename T, typename E>
 __device__ inline void cc_5(int ld, int i, int j, T djac, const MArr &xzi, const MArr &xet, const MArr &yzi,
-                            const MArr &yet, const double *__restrict__ u, const double *__restrict__ v, T &c1, T &c2) {
+                            const MArr &yet, const E *__restrict__ u, const E *__restrict__ v, T &c1, T &c2) {
     const T dFour = T(4.0);
     c1 = M(yet, i, j) * (U(i, j + 1) + U(i - 1, j + 1) + U(i, j) + U(i - 1, j)) / dFour - djac * M(xet, i, j) * V(i, j);
     c2 = djac * M(xzi, i, j) * V(i, j) - M(yzi, i, j) * (U(i, j + 1) + U(i - 1, j + 1) + U(i, j) + U(i - 1, j)) / dFour;
@@ -173,9 +173,9 @@ template void launch_porous_div<float>(const Regions &, int, int, const MomTemps
 // -------------------------------------------------------------- PorosCoef
 // Each point takes the value of the last region (in loop order) whose write
 // range covers it: zero for a non-porous region, the drag formula otherwise.
-template <typename T>
+template <typename T, typename E>
 __device__ inline T poros_val(int comp, int njacob, int ld, int i, int j, T porc1, T porc2,
-                              const double *__restrict__ u, const double *__restrict__ v) {
+                              const E *__restrict__ u, const E *__restrict__ v) {
     const T dFour = T(4.0);
     T unorm, unrm1 = T(0.0);
     if (comp == 1) {
@@ -241,8 +241,8 @@ template void launch_poros_coef<float>(const Regions &, int, int, const double *
 // c1(i+1,j), c2(i,j-1), c2(i,j); DConvV: c1(i-1,j), c1(i,j), c2(i,j),
 // c2(i,j+1)).
 #define U(a, b) cv<T>(u[WD2_IX(a, b)])
-template <typename T>
-__device__ inline T dconv(int ld, int i, int j, T c1w, T c1e, T c2s, T c2n, const double *u) {
+template <typename T, typename E>
+__device__ inline T dconv(int ld, int i, int j, T c1w, T c1e, T c2s, T c2n, const E *u) {
     return -c2s * U(i, j - 1) - c1w * U(i - 1, j) + (c1e - c1w + c2n - c2s) * U(i, j) + c1e * U(i + 1, j) +
            c2n * U(i, j + 1);
 }
@@ -254,9 +254,9 @@ template <typename T>
 __device__ inline T dconv_v(int ld, int i, int j, const T *c1, const T *c2, const double *u) {
     return dconv<T>(ld, i, j, c1[WD2_IX(i - 1, j)], c1[WD2_IX(i, j)], c2[WD2_IX(i, j)], c2[WD2_IX(i, j + 1)], u);
 }
-template <typename T>
+template <typename T, typename E>
 __device__ inline T ddiff_u(int ld, int i, int j, const MArr &ac, const MArr &bc, const MArr &bn, const MArr &gn,
-                            const double *u) {
+                            const E *u) {
 #define F(f, a, b) cv<T>(mget(f, ld, a, b))
     const T s1 = F(ac, i + 1, j) * (U(i + 1, j) - U(i, j)) - F(ac, i, j) * (U(i, j) - U(i - 1, j)) +
                  F(bc, i + 1, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
@@ -266,9 +266,9 @@ __device__ inline T ddiff_u(int ld, int i, int j, const MArr &ac, const MArr &bc
                  F(gn, i, j) * (U(i, j + 1) - U(i, j)) - F(gn, i, j - 1) * (U(i, j) - U(i, j - 1));
     return s1 + s2;
 }
-template <typename T>
+template <typename T, typename E>
 __device__ inline T ddiff_v(int ld, int i, int j, const MArr &an, const MArr &bc, const MArr &bn, const MArr &gc,
-                            const double *u) {
+                            const E *u) {
     const T s1 = F(an, i, j) * (U(i + 1, j) - U(i, j)) - F(an, i - 1, j) * (U(i, j) - U(i - 1, j)) +
                  F(bn, i, j) * (U(i + 1, j + 1) + U(i, j + 1) - U(i + 1, j - 1) - U(i, j - 1)) -
                  F(bn, i - 1, j) * (U(i, j + 1) + U(i - 1, j + 1) - U(i, j - 1) - U(i - 1, j - 1));
@@ -325,9 +325,9 @@ template void launch_conv_diff<float>(int, int, int, int, const Metrics &, const
 // k_mom_n: the time-level-n terms cnvn and difn of one component, which
 // depend only on un, vn and the metrics; the q-l iterations of one
 // nAuxMomentum call reuse them.
-template <int COMP, typename T>
-__global__ __launch_bounds__(MX *MY) void k_mom_n(int nx, int ny, int ld, Metrics m, const double *__restrict__ un,
-                                                  const double *__restrict__ vn, T *__restrict__ cnvn,
+template <int COMP, typename T, typename E = double>
+__global__ __launch_bounds__(MX *MY) void k_mom_n(int nx, int ny, int ld, Metrics m, const E *__restrict__ un,
+                                                  const E *__restrict__ vn, T *__restrict__ cnvn,
                                                   T *__restrict__ difn) {
     const T one = cv<T>(1.0);
     if (COMP == 1) {
@@ -394,8 +394,8 @@ __device__ inline void asm_coef(T cjm, T cj0, T cjp, T mlo, T mhi, T rkj, T re1,
     }
 }
 // The first split's right-hand side (buoy: the y-momentum's buoyancy term).
-template <int COMP, typename T>
-__device__ inline T asm_rhs(double fsp, double fnp, T rkj, T re1, T dk2, T cnvs, T cnvn, T difs, T difn, T cps,
+template <int COMP, typename T, typename E>
+__device__ inline T asm_rhs(E fsp, E fnp, T rkj, T re1, T dk2, T cnvs, T cnvn, T difs, T difn, T cps,
                             T cpn, T buoy) {
     const T r = cv<T>(fnp - fsp) + rkj * (-cnvs - cnvn) + rkj * re1 * (difs + difn) -
                 dk2 * (cps * cv<T>(fsp) + cpn * cv<T>(fnp));
@@ -532,11 +532,11 @@ template void launch_assemble<float>(const Regions &, int, int, int, double, dou
 //     (written into ch2.b by the first split's chain solve).
 // Every value is the same expression as in k_conv_coef / k_conv_diff /
 // k_assemble, so the chains are bit for bit those of the separate kernels.
-template <int COMP, typename T>
+template <int COMP, typename T, typename E = double>
 __global__ __launch_bounds__(MX *MY) void k_mom_asm(Regions R, int ld, double dk_, double dk2_, double re1_,
                                                     double fr_, Metrics m, MomTempsT<T> t,
-                                                    const double *__restrict__ us, const double *__restrict__ vs,
-                                                    const double *__restrict__ un, const double *__restrict__ vn,
+                                                    const E *__restrict__ us, const E *__restrict__ vs,
+                                                    const E *__restrict__ un, const E *__restrict__ vn,
                                                     const double *__restrict__ d, const double *__restrict__ dn,
                                                     ChainT<T> ch1, ChainT<T> ch2, unsigned char *__restrict__ zm,
                                                     int *err, int live, const int *stop) {
@@ -632,6 +632,31 @@ template void launch_mom_asm<float>(const Regions &, int, int, double, double, d
                                     const MomTempsT<float> &, const double *, const double *, const double *,
                                     const double *, const double *, const double *, const ChainT<float> &,
                                     const ChainT<float> &, unsigned char *, hipStream_t, int *, int, const int *);
+
+// The tangent-linear step's assembly: the same two kernels on dual numbers
+// (each field value with its directional derivative), so the chains' values
+// are the FP64 kernels' bit for bit and their derivatives follow the same
+// expressions by the chain rule.  No density field, no porous regions.
+void launch_mom_n_tl(int nx, int ny, int ld, int comp, const Metrics &m, const Dual *un, const Dual *vn, Dual *cnvn,
+                     Dual *difn, hipStream_t s) {
+    if (comp == 1)
+        hipLaunchKernelGGL((k_mom_n<1, Dual, Dual>), mgrid(nx, ny - 1), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn,
+                           difn);
+    else
+        hipLaunchKernelGGL((k_mom_n<2, Dual, Dual>), mgrid(nx - 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn,
+                           difn);
+}
+void launch_mom_asm_tl(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
+                       const MomTempsT<Dual> &t, const Dual *us, const Dual *vs, const Dual *un, const Dual *vn,
+                       const ChainT<Dual> &ch1, const ChainT<Dual> &ch2, unsigned char *zm, hipStream_t s, int *err) {
+    const double dk2 = dk * 0.5, re1 = 1.0 / re;
+    if (comp == 1)
+        hipLaunchKernelGGL((k_mom_asm<1, Dual, Dual>), mgrid(R.nx, R.ny - 1), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1,
+                           fr, m, t, us, vs, un, vn, nullptr, nullptr, ch1, ch2, zm, err, 0, nullptr);
+    else
+        hipLaunchKernelGGL((k_mom_asm<2, Dual, Dual>), mgrid(R.nx - 1, R.ny), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1,
+                           fr, m, t, us, vs, un, vn, nullptr, nullptr, ch1, ch2, zm, err, 0, nullptr);
+}
 
 // ------------------------------------------------------------- scatter
 template <typename T>

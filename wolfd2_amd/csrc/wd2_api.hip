@@ -1350,6 +1350,15 @@ struct wd2_adj {
     // chain vectors: S1^T and S2^T (A, D, C), nu, r, y, z, coefficient adjoints
     double *cstore = nullptr;
     double *At1, *Dt1, *Ct1, *At2, *Dt2, *Ct2, *nu, *r, *y, *z, *a1b, *d1b, *c1b;
+    // tangent-linear pass (wd2_adj_tangent): dual fields us, vs, un, vn, the
+    // dual time-level-n terms of both components, the dual chains of both
+    // splits; the tangent boundary values (a Regions with zero constants and
+    // one profile per wall side and velocity component)
+    Dual *tl_store = nullptr;
+    Dual *tUS, *tVS, *tUN, *tVN, *tC1, *tD1, *tC2, *tD2;
+    ChainT<Dual> tch1{}, tch2{};
+    double *tl_bcv = nullptr, *tl_pval = nullptr;
+    int *tl_poff = nullptr;
     // maxima of the adjoint q-l updates (two slots per iteration parity) and
     // their pinned copies; qev: the events after each parity's copy
     unsigned long long *red = nullptr, *h_red = nullptr;
@@ -1366,6 +1375,10 @@ static void adj_free(wd2_adj *a) {
     (void)hipFree(a->fstore);
     (void)hipFree(a->bcb);
     (void)hipFree(a->cstore);
+    (void)hipFree(a->tl_store);
+    (void)hipFree(a->tl_bcv);
+    (void)hipFree(a->tl_pval);
+    (void)hipFree(a->tl_poff);
     (void)hipFree(a->red);
     if (a->h_red) (void)hipHostFree(a->h_red);
     for (hipEvent_t e : a->ev)
@@ -1567,6 +1580,156 @@ static int adj_step(wd2_adj *a, const double *un, const double *vn, const double
     adj_gtrans(a, us, vs, un, vn, 1);
     launch_adj_combine(1, nx, ny, ld, a->tu, a->nub, a->ub, s);
     launch_adj_combine(2, nx, ny, ld, a->tv, a->nvb, a->vb, s);
+    return check_err(c);
+}
+
+// ================================================================ tangent-linear
+// Forward-mode derivative of the same step (wd2_adj_tangent): the linear
+// pieces run as the forward kernels on the tangent fields (the boundary
+// fills with the tangent boundary values), the Poisson solve is a multigrid
+// solve of the tangent right-hand side, and the q-l loop's tangent is the
+// fixed point of dw <- dw + S2^-1 Pa S1^-1 (db - dS1 x1) at the converged
+// iterate, with db, dS1 from the assembly on dual numbers (k_tangent.hip).
+static int tl_alloc(wd2_adj *a) {
+    if (a->tl_store) return WD2_OK;
+    wd2_ctx *c = a->c;
+    const size_t nf = c->nf, nch = chain_rows(c->nx, c->ny);
+    HIPCHK(hipMalloc(&a->tl_store, (8 * nf + 10 * nch) * sizeof(Dual)));
+    HIPCHK(hipMemsetAsync(a->tl_store, 0, (8 * nf + 10 * nch) * sizeof(Dual), c->s));
+    Dual *f = a->tl_store;
+    Dual **fp[8] = {&a->tUS, &a->tVS, &a->tUN, &a->tVN, &a->tC1, &a->tD1, &a->tC2, &a->tD2};
+    for (int k = 0; k < 8; k++) *fp[k] = f + k * nf;
+    Dual *q = f + 8 * nf;
+    a->tch1 = ChainT<Dual>{q, q + nch, q + 2 * nch, q + 3 * nch, q + 4 * nch};
+    q += 5 * nch;
+    a->tch2 = ChainT<Dual>{q, q + nch, q + 2 * nch, q + 3 * nch, q + 4 * nch};
+    // tangent boundary values: zero constants, a profile on every side for u
+    // and v (offsets into tl_pval, laid out as bc_bar: [side][var][line]),
+    // none for p
+    const int mr = c->mgri * c->mgrj;
+    HIPCHK(hipMalloc(&a->tl_bcv, 16 * (size_t)mr * sizeof(double)));
+    HIPCHK(hipMemsetAsync(a->tl_bcv, 0, 16 * (size_t)mr * sizeof(double), c->s));
+    HIPCHK(hipMalloc(&a->tl_pval, 8 * (size_t)a->W * sizeof(double)));
+    HIPCHK(hipMemsetAsync(a->tl_pval, 0, 8 * (size_t)a->W * sizeof(double), c->s));
+    std::vector<int> off(12 * (size_t)mr, -1);
+    for (int side = WEST; side <= NORTH; side++)
+        for (int var = VU; var <= VV; var++)
+            off[(size_t)mr * ((side - 1) + 4 * (var - 1))] = ((side - 1) * 2 + (var - 1)) * a->W;
+    HIPCHK(hipMalloc(&a->tl_poff, off.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(a->tl_poff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    return WD2_OK;
+}
+
+// One step forwards: the tangent state (ub, vb, pb) of the step's start to
+// that of its end, given the forward step's start (un, vn) and converged
+// iterate (us, vs); Rt carries the tangent boundary values.
+static int tl_step(wd2_adj *a, const Regions &Rt, const double *un, const double *vn, const double *us,
+                   const double *vs) {
+    wd2_ctx *c = a->c;
+    const int nx = c->nx, ny = c->ny, ld = c->ld;
+    hipStream_t s = c->s;
+    const wd2_params &P = a->P;
+    const long n1 = (long)nx * (ny - 1), n2 = (long)(nx - 1) * ny, n = n1 + n2;
+    const Chains cs{n1, n1, n2};
+    const long nf = (long)c->nf;
+    double *du = a->ub, *dv = a->vb, *dp = a->pb;
+    double *dun = a->nub, *dvn = a->nvb, *dus = a->tu, *dvs = a->tv;
+    {
+        const double *src[4] = {du, dv, du, dv};
+        double *dst[4] = {dun, dvn, dus, dvs};
+        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 4, s);
+    }
+    // the chains' values at the converged iterate: S1, S2 and x1 = S1^-1 b
+    MomTemps t = c->T, t2 = c->T;
+    t2.cnvn = c->T.cnvn2;
+    t2.difn = c->T.difn2;
+    auto shift = [](const Chain &q, long o) { return Chain{q.a + o, q.d + o, q.c + o, q.b + o, q.x + o}; };
+    auto dshift = [](const ChainT<Dual> &q, long o) {
+        return ChainT<Dual>{q.a + o, q.d + o, q.c + o, q.b + o, q.x + o};
+    };
+    launch_mom_n<double>(nx, ny, ld, 1, c->M, un, vn, t.cnvn, t.difn, s);
+    launch_mom_n<double>(nx, ny, ld, 2, c->M, un, vn, t2.cnvn, t2.difn, s);
+    launch_mom_asm<double>(c->R, ld, 1, P.dk, P.re, P.fr, c->M, t, us, vs, un, vn, c->d, c->dn, c->ch, c->ch2,
+                           c->zm, s, &c->sc->err, 0);
+    launch_mom_asm<double>(c->R, ld, 2, P.dk, P.re, P.fr, c->M, t2, us, vs, un, vn, c->d, c->dn, shift(c->ch, n1),
+                           shift(c->ch2, n1), c->zm + n1, s, &c->sc->err, 0);
+    launch_trid<double>(c->tw, n, cs, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
+    // the time-level-n terms on dual numbers
+    launch_tl_pack(nf, un, dun, a->tUN, s);
+    launch_tl_pack(nf, vn, dvn, a->tVN, s);
+    launch_mom_n_tl(nx, ny, ld, 1, c->M, a->tUN, a->tVN, a->tC1, a->tD1, s);
+    launch_mom_n_tl(nx, ny, ld, 2, c->M, a->tUN, a->tVN, a->tC2, a->tD2, s);
+    MomTempsT<Dual> tt1{}, tt2{};
+    tt1.cnvn = a->tC1;
+    tt1.difn = a->tD1;
+    tt2.cnvn = a->tC2;
+    tt2.difn = a->tD2;
+    int m = 1;
+    bool conv = false;
+    for (; m <= a->fp_max; m++) {
+        launch_tl_pack(nf, us, dus, a->tUS, s);
+        launch_tl_pack(nf, vs, dvs, a->tVS, s);
+        launch_mom_asm_tl(c->R, ld, 1, P.dk, P.re, P.fr, c->M, tt1, a->tUS, a->tVS, a->tUN, a->tVN, a->tch1, a->tch2,
+                          c->zm, s, &c->sc->err);
+        launch_mom_asm_tl(c->R, ld, 2, P.dk, P.re, P.fr, c->M, tt2, a->tUS, a->tVS, a->tUN, a->tVN,
+                          dshift(a->tch1, n1), dshift(a->tch2, n1), c->zm + n1, s, &c->sc->err);
+        launch_tl_rhs(n, cs, a->tch1, c->ch.x, a->r, s);
+        launch_trid<double>(c->tw, n, cs, c->ch.a, c->ch.d, c->ch.c, a->r, a->y, s);
+        launch_adj_zmask(n, c->zm, a->y, a->y, s);
+        launch_trid<double>(c->tw, n, cs, c->ch2.a, c->ch2.d, c->ch2.c, a->y, a->z, s);
+        launch_tl_scatter(nx, ny, ld, a->z, dus, dvs, s);
+        double zmax = 0.0, umax = 0.0, vmax = 0.0;
+        int st = adj_max2(a, n, a->z, nullptr, &zmax, nullptr);
+        if (!st) st = adj_max2(a, nf, dus, dvs, &umax, &vmax);
+        if (st) return st;
+        if (zmax <= a->fp_tol * std::max(umax, vmax)) {
+            conv = true;
+            break;
+        }
+    }
+    if (!conv) {
+        m = a->fp_max;
+        a->st.adj_unconverged++;
+    }
+    a->st.adj_iters_max = std::max(a->st.adj_iters_max, m);
+    a->st.adj_iters_mean += m;
+    // main.f: VelBoundCond, PresBoundCond, Ppe, PresBoundCond, Project,
+    // VelBoundCond, PresBoundCond, then u = us and the step-end fills
+    launch_vel_bc<double>(Rt, ld, dus, dvs, false, &c->sc->err, s);
+    launch_pres_bc<double>(Rt, ld, dp, &c->sc->err, s);
+    launch_divergence(nx, ny, ld, 1, c->M, dus, dvs, c->div, s);
+    launch_ppe_rhs(nx, ny, ld, P.dk, c->div, nullptr, c->rhs, s);
+    double bmax = 0.0, pmax = 0.0;
+    int st = adj_max2(a, nf, c->rhs, dp, &bmax, &pmax);
+    if (st) return st;
+    int ncyc = 0;
+    if (bmax > 0.0 || pmax > 0.0) {
+        const int ml = c->mg_last;
+        const double md = c->mg_div, mdm = c->mg_div_max;
+        c->mg_last = a->mg_last;
+        const double tol = a->ppe_tol * std::fabs(P.dk) * (bmax > 0.0 ? bmax : pmax);
+        st = ppe_mg(c, 1, P.msorit, P.dk, tol, dp, &ncyc);
+        a->mg_last = c->mg_last;
+        const bool pconv = c->mg_div < tol;
+        c->mg_last = ml;
+        c->mg_div = md;
+        c->mg_div_max = mdm;
+        if (st) return st;
+        if (!pconv) a->st.adj_unconverged++;
+    }
+    a->st.ppe_cycles_max = std::max(a->st.ppe_cycles_max, ncyc);
+    a->st.ppe_cycles_mean += ncyc;
+    launch_pres_bc<double>(Rt, ld, dp, &c->sc->err, s);
+    launch_project(Rt, ld, P.dk, c->M, dp, dus, dvs, s);
+    launch_vel_bc<double>(Rt, ld, dus, dvs, false, &c->sc->err, s);
+    launch_pres_bc<double>(Rt, ld, dp, &c->sc->err, s);
+    {
+        const double *src[2] = {dus, dvs};
+        double *dst[2] = {du, dv};
+        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 2, s);
+    }
+    launch_vel_bc<double>(Rt, ld, du, dv, false, &c->sc->err, s);
+    launch_pres_bc<double>(Rt, ld, dp, &c->sc->err, s);
     return check_err(c);
 }
 
@@ -2522,6 +2685,62 @@ int wd2_adj_backward(wd2_adj *a, int ld_h, const double *u_bar, const double *v_
     if (p0_bar && (st = down(c, a->pb, p0_bar, ld_h))) return st;
     if (bc_bar) HIPCHK(hipMemcpy(bc_bar, a->bcb, 8 * (size_t)a->W * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipStreamSynchronize(s));
+    if (stats) *stats = a->st;
+    return WD2_OK;
+}
+
+int wd2_adj_tangent(wd2_adj *a, int ld_h, const double *du0, const double *dv0, const double *dp0, const double *dbc,
+                    double *duN, double *dvN, double *dpN, wd2_adj_stats *stats) {
+    if (!a) return fail(WD2_ERR_ARG, "null argument");
+    if (!a->have_fwd) return fail(WD2_ERR_ARG, "wd2_adj_tangent before wd2_adj_forward");
+    wd2_ctx *c = a->c;
+    if (ld_h < c->nx + 2) return fail(WD2_ERR_ARG, "ld=%d < nx+2=%d", ld_h, c->nx + 2);
+    int st = adj_check(c, &a->P);
+    if (st || (st = tl_alloc(a))) return st;
+    hipStream_t s = c->s;
+    const size_t nf = c->nf;
+    launch_adj_zero(3 * (long)nf, a->ub, s);   // the tangent state: ub, vb, pb are adjacent
+    if (du0 && (st = up(c, du0, ld_h, a->ub))) return st;
+    if (dv0 && (st = up(c, dv0, ld_h, a->vb))) return st;
+    if (dp0 && (st = up(c, dp0, ld_h, a->pb))) return st;
+    if (dbc)
+        HIPCHK(hipMemcpyAsync(a->tl_pval, dbc, 8 * (size_t)a->W * sizeof(double), hipMemcpyHostToDevice, s));
+    else
+        launch_adj_zero(8 * (long)a->W, a->tl_pval, s);
+    Regions Rt = c->R;
+    Rt.bcv = a->tl_bcv;
+    Rt.poff = a->tl_poff;
+    Rt.pval = a->tl_pval;
+    const int fu = a->st.fwd_unconverged;
+    const double msf = a->st.ms_forward;
+    a->st = wd2_adj_stats{};
+    a->st.fwd_unconverged = fu;
+    a->st.ms_forward = msf;
+    // the forward run again from its first checkpoint, each step followed by
+    // its tangent (the converged iterate kept in the tape's first slot)
+    HIPCHK(hipEventRecord(a->ev[0], s));
+    adj_copy3(c, a->ckpt, a->ckpt + nf, a->ckpt + 2 * nf, c->u, c->v, c->p);
+    double *tus = a->tape + 3 * nf, *tvs = a->tape + 4 * nf;
+    for (int k = 0; k < a->nsteps; k++) {
+        c->adj_us = tus;
+        c->adj_vs = tvs;
+        int nql = 0, nsor = 0;
+        st = one_step(c, &a->P, &nql, &nsor, nullptr);
+        c->adj_us = c->adj_vs = nullptr;
+        if (st || (st = tl_step(a, Rt, c->un, c->vn, tus, tvs))) return st;
+    }
+    adj_copy3(c, a->fin, a->fin + nf, a->fin + 2 * nf, c->u, c->v, c->p);
+    HIPCHK(hipEventRecord(a->ev[1], s));
+    if (duN && (st = down(c, a->ub, duN, ld_h))) return st;
+    if (dvN && (st = down(c, a->vb, dvN, ld_h))) return st;
+    if (dpN && (st = down(c, a->pb, dpN, ld_h))) return st;
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
+    a->st.ms_reverse = ms;
+    a->st.steps = a->nsteps;
+    a->st.adj_iters_mean /= a->nsteps;
+    a->st.ppe_cycles_mean /= a->nsteps;
     if (stats) *stats = a->st;
     return WD2_OK;
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "wd2_dev.hpp"
+#include "wd2_dual.hpp"
 
 namespace wd2 {
 
@@ -325,5 +326,20 @@ void launch_adj_zero(long n, double *y, hipStream_t s);
 void launch_adj_combine(int comp, int nx, int ny, int ld, const double *fsb, const double *fnb, double *out,
                         hipStream_t s);
 void launch_adj_maxabs(long n, const double *x, unsigned long long *red, int slot, hipStream_t s);
+
+// ------------------------------------------------------------ tangent-linear
+// The momentum assembly on dual numbers (k_momentum.hip): values and
+// directional derivatives of the time-level-n terms and of the chains.
+void launch_mom_n_tl(int nx, int ny, int ld, int comp, const Metrics &m, const Dual *un, const Dual *vn, Dual *cnvn,
+                     Dual *difn, hipStream_t s);
+void launch_mom_asm_tl(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
+                       const MomTempsT<Dual> &t, const Dual *us, const Dual *vs, const Dual *un, const Dual *vn,
+                       const ChainT<Dual> &ch1, const ChainT<Dual> &ch2, unsigned char *zm, hipStream_t s, int *err);
+// out = (v, d) element-wise
+void launch_tl_pack(long n, const double *v, const double *d, Dual *out, hipStream_t s);
+// r = db - dS1 x1 from the dual first-split chains (k_tangent.hip)
+void launch_tl_rhs(long n, const Chains &cs, const ChainT<Dual> &ch, const double *x1, double *r, hipStream_t s);
+// du, dv += the chain vector z on the chain rows' points
+void launch_tl_scatter(int nx, int ny, int ld, const double *z, double *du, double *dv, hipStream_t s);
 
 }  // namespace wd2
