@@ -310,7 +310,9 @@ def adjoint_block(n: int = 512, nsteps: int = 200, every: int = 20) -> dict:
     a target run's (lid of 16 seeded Fourier modes); gradient with respect to
     the lid values and the initial velocity.  Reports the forward step time
     (with checkpoints), the backward pass per step (segment re-runs plus
-    adjoint steps) and their ratio."""
+    adjoint steps) and their ratio, and the FP64 tangent-linear check: the
+    tangent-linear derivative of J along one seeded direction of the lid
+    and the initial velocity against the adjoint gradient's projection."""
     from wolfd2_amd import autograd
     g, reg, prm, x, y = workloads.adjoint_cavity(n)
     run = autograd.CavityRun(g, reg, prm, nsteps, checkpoint_every=every)
@@ -321,6 +323,17 @@ def adjoint_block(n: int = 512, nsteps: int = 200, every: int = 20) -> dict:
     uN, vN = run.run(lid, zu, zv)
     J = 0.5 * float(np.sum((uN - uT) ** 2) + np.sum((vN - vT) ** 2))
     gl, gu, gv, st = run.grad(uN - uT, vN - vT)
+    rng = np.random.default_rng(400)
+    dl, du, dv = rng.standard_normal(g.nx - 1), rng.standard_normal(zu.shape), rng.standard_normal(zv.shape)
+    W = max(g.nx, g.ny) + 2
+    dbc = np.zeros((4, 2, W))
+    dbc[case.NORTH - 1, case.VAR_U - 1, 2:g.nx + 1] = dl
+    fu, fv = np.zeros((g.ncol, g.ld)), np.zeros((g.ncol, g.ld))
+    fu[run.su] = du
+    fv[run.sv] = dv
+    t = run.adj.tangent(fu, fv, None, dbc)
+    dj_tl = float(np.sum((uN - uT) * t["duN"][run.su]) + np.sum((vN - vT) * t["dvN"][run.sv]))
+    dj_ad = float(gl @ dl + np.sum(gu * du) + np.sum(gv * dv))
     run.close()
     fwd = st["ms_forward"] / nsteps
     bwd = (st["ms_recompute"] + st["ms_reverse"]) / nsteps
@@ -336,7 +349,12 @@ def adjoint_block(n: int = 512, nsteps: int = 200, every: int = 20) -> dict:
             "backward_over_forward": round(bwd / fwd, 3),
             "reverse_over_forward": round(st["ms_reverse"] / nsteps / fwd, 3),
             "adjoint_iters_mean": st["adj_iters_mean"], "adjoint_vcycles_mean": st["ppe_cycles_mean"],
-            "unconverged": {"forward": st["fwd_unconverged"], "adjoint": st["adj_unconverged"]}}
+            "unconverged": {"forward": st["fwd_unconverged"], "adjoint": st["adj_unconverged"],
+                            "tangent": t["stats"]["adj_unconverged"]},
+            "tangent_linear_check": {"direction": "seeded N(0,1) on the lid values and the initial velocity",
+                                     "dJ_tangent": dj_tl, "dJ_adjoint": dj_ad,
+                                     "rel_diff": abs(dj_tl - dj_ad) / abs(dj_ad),
+                                     "tangent_pass_ms_per_step": round(t["stats"]["ms_reverse"] / nsteps, 4)}}
 
 
 def ensemble_block(nmem: int = 64, steps: int = 20, warmup: int = 2, adj_steps: int = 10) -> dict:
