@@ -27,6 +27,22 @@ namespace wd2 {
 
 constexpr int MX = 64, MY = 4;
 static inline dim3 mgrid(int ni, int nj) { return dim3((ni + MX - 1) / MX, (nj + MY - 1) / MY); }
+// XCD-aware order of the MX x MY tiles of an ni x nj range: workgroups go to
+// the 8 XCDs round-robin, so workgroup b takes tile (b % 8) * per + b / 8 of
+// the row-major tile grid, and each XCD works through a contiguous band of
+// tile rows whose shared halo rows stay in its own L2.  xgrid: a 1-D grid of
+// 8 * per workgroups; xtile: the workgroup's tile (false: none).
+static inline dim3 xgrid(int ni, int nj) {
+    const int nt = ((ni + MX - 1) / MX) * ((nj + MY - 1) / MY);
+    return dim3(8 * ((nt + 7) / 8));
+}
+__device__ inline bool xtile(int ni, int nj, int &bx, int &by) {
+    const int gx = (ni + MX - 1) / MX, nt = gx * ((nj + MY - 1) / MY), per = (nt + 7) / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    bx = t % gx;
+    by = t / gx;
+    return t < nt;
+}
 
 // ------------------------------------------------------------- ConvCoef
 template <typename T, typename S>
@@ -331,7 +347,9 @@ __global__ __launch_bounds__(MX *MY) void k_mom_n(int nx, int ny, int ld, Metric
                                                   T *__restrict__ difn) {
     const T one = cv<T>(1.0);
     if (COMP == 1) {
-        const int i = 1 + blockIdx.x * MX + threadIdx.x, j = 2 + blockIdx.y * MY + threadIdx.y;
+        int bx, by;
+        if (!xtile(nx, ny - 1, bx, by)) return;
+        const int i = 1 + bx * MX + threadIdx.x, j = 2 + by * MY + threadIdx.y;
         if (i > nx || j > ny) return;
         const MArr xet = MARR(m, xec), yet = MARR(m, yec), xzi = MARR(m, xzn), yzi = MARR(m, yzn);
         const T c1w = cc_x1<T>(ld, i, j, one, xet, yet, un, vn), c1e = cc_x1<T>(ld, i + 1, j, one, xet, yet, un, vn);
@@ -340,7 +358,9 @@ __global__ __launch_bounds__(MX *MY) void k_mom_n(int nx, int ny, int ld, Metric
         cnvn[k] = dconv<T>(ld, i, j, c1w, c1e, c2s, c2n, un);
         difn[k] = ddiff_u<T>(ld, i, j, MARR(m, rac), MARR(m, rbc), MARR(m, rbn), MARR(m, rgn), un);
     } else {
-        const int i = 2 + blockIdx.x * MX + threadIdx.x, j = 1 + blockIdx.y * MY + threadIdx.y;
+        int bx, by;
+        if (!xtile(nx - 1, ny, bx, by)) return;
+        const int i = 2 + bx * MX + threadIdx.x, j = 1 + by * MY + threadIdx.y;
         if (i > nx || j > ny) return;
         const MArr xet = MARR(m, xen), yet = MARR(m, yen), xzi = MARR(m, xzc), yzi = MARR(m, yzc);
         const T c1w = cc_y1<T>(ld, i - 1, j, one, xet, yet, un, vn), c1e = cc_y1<T>(ld, i, j, one, xet, yet, un, vn);
@@ -355,9 +375,9 @@ template <typename T>
 void launch_mom_n(int nx, int ny, int ld, int comp, const Metrics &m, const double *un, const double *vn, T *cnvn,
                   T *difn, hipStream_t s) {
     if (comp == 1)
-        hipLaunchKernelGGL((k_mom_n<1, T>), mgrid(nx, ny - 1), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn, difn);
+        hipLaunchKernelGGL((k_mom_n<1, T>), xgrid(nx, ny - 1), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn, difn);
     else
-        hipLaunchKernelGGL((k_mom_n<2, T>), mgrid(nx - 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn, difn);
+        hipLaunchKernelGGL((k_mom_n<2, T>), xgrid(nx - 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn, difn);
 }
 template void launch_mom_n<double>(int, int, int, int, const Metrics &, const double *, const double *, double *,
                                    double *, hipStream_t);
@@ -544,12 +564,15 @@ __global__ __launch_bounds__(MX *MY) void k_mom_asm(Regions R, int ld, double dk
     const int nx = R.nx, ny = R.ny;
     int i, j;
     long k;
+    int bx, by;
     if (COMP == 1) {
-        i = 1 + blockIdx.x * MX + threadIdx.x; j = 2 + blockIdx.y * MY + threadIdx.y;
+        if (!xtile(nx, ny - 1, bx, by)) return;
+        i = 1 + bx * MX + threadIdx.x; j = 2 + by * MY + threadIdx.y;
         if (i > nx || j > ny) return;
         k = (long)(j - 2) * nx + (i - 1);
     } else {
-        i = 2 + blockIdx.x * MX + threadIdx.x; j = 1 + blockIdx.y * MY + threadIdx.y;
+        if (!xtile(nx - 1, ny, bx, by)) return;
+        i = 2 + bx * MX + threadIdx.x; j = 1 + by * MY + threadIdx.y;
         if (i > nx || j > ny) return;
         k = (long)(j - 1) * (nx - 1) + (i - 2);
     }
@@ -618,10 +641,10 @@ void launch_mom_asm(const Regions &R, int ld, int comp, double dk, double re, do
                     hipStream_t s, int *err, int live, const int *stop) {
     const double dk2 = dk * 0.5, re1 = 1.0 / re;
     if (comp == 1)
-        hipLaunchKernelGGL((k_mom_asm<1, T>), mgrid(R.nx, R.ny - 1), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
+        hipLaunchKernelGGL((k_mom_asm<1, T>), xgrid(R.nx, R.ny - 1), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
                            t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live, stop);
     else
-        hipLaunchKernelGGL((k_mom_asm<2, T>), mgrid(R.nx - 1, R.ny), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
+        hipLaunchKernelGGL((k_mom_asm<2, T>), xgrid(R.nx - 1, R.ny), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1, fr, m,
                            t, us, vs, un, vn, d, dn, ch1, ch2, zm, err, live, stop);
 }
 template void launch_mom_asm<double>(const Regions &, int, int, double, double, double, const Metrics &,
@@ -640,10 +663,10 @@ template void launch_mom_asm<float>(const Regions &, int, int, double, double, d
 void launch_mom_n_tl(int nx, int ny, int ld, int comp, const Metrics &m, const Dual *un, const Dual *vn, Dual *cnvn,
                      Dual *difn, hipStream_t s) {
     if (comp == 1)
-        hipLaunchKernelGGL((k_mom_n<1, Dual, Dual>), mgrid(nx, ny - 1), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn,
+        hipLaunchKernelGGL((k_mom_n<1, Dual, Dual>), xgrid(nx, ny - 1), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn,
                            difn);
     else
-        hipLaunchKernelGGL((k_mom_n<2, Dual, Dual>), mgrid(nx - 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn,
+        hipLaunchKernelGGL((k_mom_n<2, Dual, Dual>), xgrid(nx - 1, ny), dim3(MX, MY), 0, s, nx, ny, ld, m, un, vn, cnvn,
                            difn);
 }
 void launch_mom_asm_tl(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
@@ -651,10 +674,10 @@ void launch_mom_asm_tl(const Regions &R, int ld, int comp, double dk, double re,
                        const ChainT<Dual> &ch1, const ChainT<Dual> &ch2, unsigned char *zm, hipStream_t s, int *err) {
     const double dk2 = dk * 0.5, re1 = 1.0 / re;
     if (comp == 1)
-        hipLaunchKernelGGL((k_mom_asm<1, Dual, Dual>), mgrid(R.nx, R.ny - 1), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1,
+        hipLaunchKernelGGL((k_mom_asm<1, Dual, Dual>), xgrid(R.nx, R.ny - 1), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1,
                            fr, m, t, us, vs, un, vn, nullptr, nullptr, ch1, ch2, zm, err, 0, nullptr);
     else
-        hipLaunchKernelGGL((k_mom_asm<2, Dual, Dual>), mgrid(R.nx - 1, R.ny), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1,
+        hipLaunchKernelGGL((k_mom_asm<2, Dual, Dual>), xgrid(R.nx - 1, R.ny), dim3(MX, MY), 0, s, R, ld, dk, dk2, re1,
                            fr, m, t, us, vs, un, vn, nullptr, nullptr, ch1, ch2, zm, err, 0, nullptr);
 }
 
