@@ -115,7 +115,8 @@ int wd2_alt_trid_lu(int n, const double *a, double *b);
  * reference; for measurement and tests): a batch of chains, rows [0, n1) in
  * chains of L1 rows and rows [n1, n) in chains of L2 rows, each solved as
  * AltTridLU solves it (first-row quirk included), with the window solver and
- * its bit-exact fallback (window = 1) or the bit-exact solver (0).  a, d, c,
+ * its bit-exact fallback (window = 1: the default shape; 128, 256 or 512: that
+ * shape, see wd2_ctx_trid_set_window) or the bit-exact solver (0).  a, d, c,
  * b: sub-diagonal, diagonal, super-diagonal, right-hand side; x: solution. */
 int wd2_trid_f32(long n, long n1, long L1, long L2, const float *a, const float *d, const float *c, const float *b,
                  float *x, int window);
@@ -332,6 +333,16 @@ int wd2_ctx_mg_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
  * chains are solved once more afterwards; fields are unchanged.  No
  * counterpart in the reference (measurement only). */
 int wd2_ctx_trid_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
+/* The mixed path's window shape (no counterpart in the reference): the
+ * lanes of one window's workgroup, 128, 256 (the default) or 512 -- windows
+ * of 1024, 2048 or 4096 rows, at most 8, 9 or 10 parallel-cyclic-reduction
+ * levels.  The shapes round differently (FP32 results differ in the last
+ * bits).  wd2_ctx_trid_autotune times `reps` launches of each on the
+ * chains of the context's last q-l iteration (mixed precision, after a
+ * step), keeps the fastest and returns it and the three times (us[3], may
+ * be NULL); wd2_ctx_trid_set_window sets one. */
+int wd2_ctx_trid_set_window(wd2_ctx *ctx, int window_threads);
+int wd2_ctx_trid_autotune(wd2_ctx *ctx, int reps, int *window_threads, double *us);
 /* Kernel shapes of the multigrid solver (no counterpart in the reference;
  * none changes the arithmetic, so results are the same bits): nt0 threads
  * per level-0 tile workgroup (256 or 512), stage 1 to form the level-0

@@ -130,7 +130,7 @@ def _solve32(a, d, c, b, n1, L1, L2, window):
     f = lambda q: q.ctypes.data_as(C.POINTER(C.c_float))
     x = np.zeros_like(b)
     rc = lib().wd2_trid_f32(C.c_long(b.size), C.c_long(n1), C.c_long(L1), C.c_long(L2), f(a), f(d), f(c), f(b), f(x),
-                            1 if window else 0)
+                            int(window))
     assert rc == 0
     return x
 
@@ -140,20 +140,49 @@ def test_window_solver_on_random_dominant_chains(L1, L2, nch1, nch2):
     """Batches of two chain lengths (one long chain each as the momentum
     batches have, or many short ones), dominance ratio 0.6: the window
     solver's x agrees with an FP64 Thomas solve of the same quirked matrices
-    to FP32 accuracy, as the bit-exact FP32 solver does."""
+    to FP32 accuracy, as the bit-exact FP32 solver does, in each window shape
+    (128, 256 and 512 lanes: windows of 1024, 2048 and 4096 rows)."""
     rng = np.random.default_rng(L1 + L2)
     a1, d1, c1, b1 = _chains(rng, nch1, L1, 0.6)
     a2, d2, c2, b2 = _chains(rng, nch2, L2, 0.6)
     a, d, c, b = (np.concatenate(q) for q in ((a1, a2), (d1, d2), (c1, c2), (b1, b2)))
     n1 = nch1 * L1
     ref = np.concatenate([_thomas64(a1, d1, c1, b1, L1), _thomas64(a2, d2, c2, b2, L2)])
-    xw = _solve32(a, d, c, b, n1, L1, L2, True)
-    xe = _solve32(a, d, c, b, n1, L1, L2, False)
-    ew = np.abs(xw - ref).max() / np.abs(ref).max()
+    xe = _solve32(a, d, c, b, n1, L1, L2, 0)
     ee = np.abs(xe - ref).max() / np.abs(ref).max()
-    print(f"  chains {nch1}x{L1} + {nch2}x{L2}: max error / max |x|: window {ew:.2e}, exact FP32 {ee:.2e}", flush=True)
-    assert ee < 1e-5 and ew < 1e-5
-    assert ew < 4 * ee + 1e-7
+    assert ee < 1e-5
+    for shape in (128, 256, 512):
+        xw = _solve32(a, d, c, b, n1, L1, L2, shape)
+        ew = np.abs(xw - ref).max() / np.abs(ref).max()
+        print(f"  chains {nch1}x{L1} + {nch2}x{L2}: max error / max |x|: window ({shape} lanes) {ew:.2e}, "
+              f"exact FP32 {ee:.2e}", flush=True)
+        assert ew < 1e-5
+        assert ew < 4 * ee + 1e-7
+
+
+def test_window_shape_autotune_on_the_cavity():
+    """The window-shape autotuner on the chains of a 512^2 cavity's last
+    q-l iteration returns one of the three shapes and their times; the mixed
+    run that continues with the chosen shape stays within 1e-6 of the FP64
+    run (relative L2 per field)."""
+    g, reg, prm = workloads.cavity(512, 1000.0, 2e-4, nPpeSolver=7, msorit=100, sortol=1e-10)
+    z = np.zeros((g.ncol, g.ld))
+    sol = api.Solver(g, reg, prm)
+    sol.set_precision("mixed")
+    sol.set_fields(z, z, z)
+    sol.cold_start()
+    sol.step(2)
+    t = sol.trid_autotune(10)
+    print(f"  window autotune at 512^2: {t}", flush=True)
+    assert t["window_threads"] in (128, 256, 512)
+    assert all(v > 0 for v in t["us"].values())
+    sol.step(8)
+    fm = sol.get_fields()
+    sol.close()
+    f64, _ = _run(g, reg, prm, "fp64", True, 10)
+    S = np.s_[: g.ny + 2, : g.nx + 2]
+    for name, a, b in zip("uvp", fm, f64):
+        assert rel_l2(a[S], b[S]) < 1e-6, name
 
 
 def test_window_solver_falls_back_on_weakly_dominant_chains():

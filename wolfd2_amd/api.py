@@ -253,6 +253,18 @@ class Solver:
     def set_mg_tune(self, nt0: int, stage: int, bside: int) -> None:
         check(lib().wd2_ctx_mg_set_tune(self.ctx, nt0, stage, bside))
 
+    def trid_autotune(self, reps: int = 20) -> dict:
+        """Pick the mixed path's fastest window shape (lanes per window: 128,
+        256 or 512) on the chains of the last q-l iteration
+        (wd2_ctx_trid_autotune); returns it and the three shapes' times."""
+        wt = C.c_int()
+        us = (C.c_double * 3)()
+        check(lib().wd2_ctx_trid_autotune(self.ctx, reps, C.byref(wt), us))
+        return {"window_threads": wt.value, "us": {128: us[0], 256: us[1], 512: us[2]}}
+
+    def set_trid_window(self, window_threads: int) -> None:
+        check(lib().wd2_ctx_trid_set_window(self.ctx, window_threads))
+
     def mg_cycle_time(self, cycles: int = 5) -> float:
         us = C.c_double(0)
         check(lib().wd2_ctx_mg_cycle_time(self.ctx, C.c_double(self.prm.dk), cycles, C.byref(us)))

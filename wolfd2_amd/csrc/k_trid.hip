@@ -622,8 +622,14 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nb
 // interface values.  A window whose dominance ratio exceeds TW_DOM clears
 // *keep, and the bit-exact speculative solver (launched after this kernel
 // with stop = keep) solves the chains instead.
-constexpr int TW_T = 256, TW_S = 8, TW_N = TW_T * TW_S, TW_H = 64, TW_O = TW_N - 2 * TW_H;
-constexpr int TW_M = 2 * TW_T;   // interface rows per window
+// Window shapes: TW_T lanes (128, 256 or 512: windows of 1024, 2048 or
+// 4096 rows, at most 8, 9 or 10 PCR levels), chosen by
+// wd2_ctx_trid_autotune; 256 unless tuned.
+constexpr int TW_S = 8, TW_H = 64;
+template <int TW_T>
+struct TwShape {
+    static constexpr int N = TW_T * TW_S, O = N - 2 * TW_H, M = 2 * TW_T;
+};
 constexpr float TW_DOM = 0.9f;
 constexpr float TW_EPS = 0x1p-48f;   // PCR stops once every coupling is below this share of its diagonal
 
@@ -633,13 +639,14 @@ __global__ void k_trid_keep_set(int *keep) { *keep = 1; }
 // like the reference.
 __device__ inline float wrcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-template <typename T>
+template <typename T, int TW_T>
 __global__ __launch_bounds__(TW_T) void k_trid_window(long n, Chains cs, const T *__restrict__ A,
                                                       const T *__restrict__ D, const T *__restrict__ C,
                                                       const T *__restrict__ B, T *__restrict__ X, T *__restrict__ B2,
                                                       const unsigned char *__restrict__ Z, const int *stop,
                                                       int *keep) {
     static_assert(std::is_same<T, float>::value, "the window solver serves the FP32 chains");
+    constexpr int TW_N = TwShape<TW_T>::N, TW_O = TwShape<TW_T>::O, TW_M = TwShape<TW_T>::M;
     if (stop && *stop) return;
     // interface rows in LDS: sub- and super-diagonal, right-hand side and
     // the reciprocal of the diagonal (each row's diagonal stays with its thread)
@@ -794,37 +801,51 @@ __global__ __launch_bounds__(TW_T) void k_trid_window(long n, Chains cs, const T
     }
 }
 
+template <typename T, int TW_T>
+static void launch_window_kernel(long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B, T *X,
+                                 hipStream_t s, T *B2, const unsigned char *Z, const int *stop, int *keep) {
+    const unsigned nb = (unsigned)((n + TwShape<TW_T>::O - 1) / TwShape<TW_T>::O);
+    hipLaunchKernelGGL((k_trid_window<T, TW_T>), dim3(nb), dim3(TW_T), 0, s, n, cs, A, D, C, B, X, B2, Z, stop, keep);
+}
+template <typename T>
+static void launch_window_shape(int wt, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
+                                T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop, int *keep) {
+    if (wt == 128)
+        launch_window_kernel<T, 128>(n, cs, A, D, C, B, X, s, B2, Z, stop, keep);
+    else if (wt == 512)
+        launch_window_kernel<T, 512>(n, cs, A, D, C, B, X, s, B2, Z, stop, keep);
+    else
+        launch_window_kernel<T, 256>(n, cs, A, D, C, B, X, s, B2, Z, stop, keep);
+}
+int trid_window_shape_ok(int wt) { return wt == 128 || wt == 256 || wt == 512; }
+
 template <typename T>
 void launch_trid_window(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
-                        const T *B, T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop) {
-    const unsigned nb = (unsigned)((n + TW_O - 1) / TW_O);
+                        const T *B, T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop, int wt) {
     int *keep = w.fail + 2;   // the solver's flag block: fail, nfallback, keep
     hipLaunchKernelGGL(k_trid_keep_set, dim3(1), dim3(1), 0, s, keep);
-    hipLaunchKernelGGL(k_trid_window<T>, dim3(nb), dim3(TW_T), 0, s, n, cs, A, D, C, B, X, B2, Z, stop, keep);
+    launch_window_shape<T>(wt, n, cs, A, D, C, B, X, s, B2, Z, stop, keep);
     // the bit-exact solver runs only where a window was too weakly dominant
     // (keep cleared); it also returns at entry once the q-l loop has stopped,
     // since then the window kernel left keep set
     launch_trid<T>(w, n, cs, A, D, C, B, X, s, B2, Z, keep);
 }
-// reps back-to-back launches of the window kernel between events e0 and e1
-// (X written; no second right-hand side).
+// reps back-to-back launches of the window kernel of shape wt between events
+// e0 and e1 (X written; no second right-hand side).
 template <typename T>
 void launch_trid_window_bench(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
-                              const T *B, T *X, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    const unsigned nb = (unsigned)((n + TW_O - 1) / TW_O);
+                              const T *B, T *X, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int wt) {
     int *keep = w.fail + 2;
     (void)hipEventRecord(e0, s);
-    for (int k = 0; k < reps; k++)
-        hipLaunchKernelGGL(k_trid_window<T>, dim3(nb), dim3(TW_T), 0, s, n, cs, A, D, C, B, X, nullptr, nullptr,
-                           nullptr, keep);
+    for (int k = 0; k < reps; k++) launch_window_shape<T>(wt, n, cs, A, D, C, B, X, s, nullptr, nullptr, nullptr, keep);
     (void)hipEventRecord(e1, s);
 }
 template void launch_trid_window_bench<float>(const TridWorkT<float> &, long, const Chains &, const float *,
                                               const float *, const float *, const float *, float *, int, hipStream_t,
-                                              hipEvent_t, hipEvent_t);
+                                              hipEvent_t, hipEvent_t, int);
 template void launch_trid_window<float>(const TridWorkT<float> &, long, const Chains &, const float *, const float *,
                                         const float *, const float *, float *, hipStream_t, float *,
-                                        const unsigned char *, const int *);
+                                        const unsigned char *, const int *, int);
 
 int trid_blocks(long n) { return (int)((n + TR_R - 1) / TR_R); }
 

@@ -103,6 +103,7 @@ struct wd2_ctx {
     bool d_live = false;
     bool sor_capped = false;   // the last tiled SOR solve ran to msorit
     int precision = WD2_PREC_FP64;
+    int win_wt = 256;   // the mixed path's window shape (wd2_ctx_trid_autotune)
     // pinned copies of the stop flag and events of the SOR batches in flight
     // (h_flag[0..1]) and of the q-l iterations in flight (h_flag[2..3])
     int *h_flag = nullptr;
@@ -626,8 +627,9 @@ static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const Mom
         // rest the window solver in the first iteration leaves v 1.3e-6 from
         // the FP64 path after two steps (relative L2), against 2.8e-7 this way.
         if (g_window_chains && m > 1) {
-            launch_trid_window<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop);
-            launch_trid_window<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop);
+            launch_trid_window<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop, c->win_wt);
+            launch_trid_window<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop,
+                                  c->win_wt);
         } else {
             launch_trid<T>(tw, n1 + n2, cs, ch.a, ch.d, ch.c, ch.b, ch.x, s, ch2.b, c->zm, stop);
             launch_trid<T>(tw, n1 + n2, cs, ch2.a, ch2.d, ch2.c, ch2.b, ch2.x, s, nullptr, nullptr, stop);
@@ -2042,6 +2044,8 @@ int wd2_trid_f32(long n, long n1, long L1, long L2, const float *a, const float 
     if (n < 1 || n1 < 0 || n1 > n || L1 < 1 || L2 < 1 || (n1 > 0 && n1 % L1) || (n - n1) % L2)
         return fail(WD2_ERR_ARG, "wd2_trid_f32: chains of %ld and %ld rows do not tile %ld and %ld rows", L1, L2, n1,
                     n - n1);
+    if (window != 0 && window != 1 && !trid_window_shape_ok(window))
+        return fail(WD2_ERR_ARG, "wd2_trid_f32: window %d (0, 1, or a window shape of 128, 256 or 512 lanes)", window);
     int st;
     const int side = (int)std::ceil(std::sqrt((double)n)) + 8;
     if ((st = dropin_ctx(side, side, side + 2))) return st;
@@ -2054,7 +2058,8 @@ int wd2_trid_f32(long n, long n1, long L1, long L2, const float *a, const float 
     HIPCHK(hipMemcpyAsync(ch.b, b, n * 4, hipMemcpyHostToDevice, c->s));
     const Chains cs{n1, n1 > 0 ? L1 : L2, L2};
     if (window)
-        launch_trid_window<float>(c->twf, n, cs, ch.a, ch.d, ch.c, ch.b, ch.x, c->s, nullptr, nullptr, nullptr);
+        launch_trid_window<float>(c->twf, n, cs, ch.a, ch.d, ch.c, ch.b, ch.x, c->s, nullptr, nullptr, nullptr,
+                                  window == 1 ? 256 : window);
     else
         launch_trid<float>(c->twf, n, cs, ch.a, ch.d, ch.c, ch.b, ch.x, c->s);
     HIPCHK(hipMemcpyAsync(x, ch.x, n * 4, hipMemcpyDeviceToHost, c->s));
@@ -2447,7 +2452,7 @@ int wd2_ctx_trid_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     HIPCHK(hipEventCreate(&e[1]));
     if (which == 3)
         launch_trid_window_bench<float>(c->twf, n1 + n2, cs, c->chf.a, c->chf.d, c->chf.c, c->chf.b, c->chf.x, reps,
-                                        c->s, e[0], e[1]);
+                                        c->s, e[0], e[1], c->win_wt);
     else if (c->precision == WD2_PREC_MIXED)
         launch_trid_kernel_bench<float>(c->twf, n1 + n2, cs, c->chf.a, c->chf.d, c->chf.c, c->chf.b, c->chf.x, which,
                                         reps, c->s, e[0], e[1]);
@@ -2461,6 +2466,49 @@ int wd2_ctx_trid_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     (void)hipEventDestroy(e[1]);
     if (avg_us) *avg_us = 1e3 * ms / reps;
     return check_err(c);
+}
+
+int wd2_ctx_trid_set_window(wd2_ctx *c, int wt) {
+    if (!c || !trid_window_shape_ok(wt)) return fail(WD2_ERR_ARG, "window shape %d (128, 256 or 512 lanes)", wt);
+    if (wt != c->win_wt) graph_clear(c);
+    c->win_wt = wt;
+    return WD2_OK;
+}
+
+int wd2_ctx_trid_autotune(wd2_ctx *c, int reps, int *wt_out, double *us_out) {
+    if (!c || reps < 1) return fail(WD2_ERR_ARG, "window autotune: reps %d", reps);
+    if (c->precision != WD2_PREC_MIXED) return fail(WD2_ERR_ARG, "window autotune: the mixed path only");
+    if (c->any_porous) return fail(WD2_ERR_UNSUPPORTED, "window autotune: the two-chain batches only");
+    const long n1 = (long)c->nx * (c->ny - 1), n2 = (long)(c->nx - 1) * c->ny;
+    const Chains cs{n1, n1, n2};
+    hipEvent_t e[2];
+    HIPCHK(hipEventCreate(&e[0]));
+    HIPCHK(hipEventCreate(&e[1]));
+    const int shapes[3] = {128, 256, 512};
+    int best = c->win_wt;
+    double tbest = 0.0;
+    for (int q = 0; q < 3; q++) {
+        // one launch first (code load, caches), then the timed ones
+        launch_trid_window_bench<float>(c->twf, n1 + n2, cs, c->chf.a, c->chf.d, c->chf.c, c->chf.b, c->chf.x, 1,
+                                        c->s, e[0], e[1], shapes[q]);
+        launch_trid_window_bench<float>(c->twf, n1 + n2, cs, c->chf.a, c->chf.d, c->chf.c, c->chf.b, c->chf.x, reps,
+                                        c->s, e[0], e[1], shapes[q]);
+        HIPCHK(hipStreamSynchronize(c->s));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
+        const double us = 1e3 * ms / reps;
+        if (us_out) us_out[q] = us;
+        if (q == 0 || us < tbest) {
+            tbest = us;
+            best = shapes[q];
+        }
+    }
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    if (best != c->win_wt) graph_clear(c);
+    c->win_wt = best;
+    if (wt_out) *wt_out = best;
+    return WD2_OK;
 }
 
 int wd2_ctx_set_graph(wd2_ctx *c, int enable) {

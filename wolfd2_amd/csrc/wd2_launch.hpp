@@ -40,12 +40,16 @@ void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, co
 // solved by partitioned Thomas + PCR in LDS, the bit-exact solver as the
 // fallback for a window that is not diagonally dominant enough; same
 // arguments as launch_trid.
+// wt: the window shape, the lanes of one window's workgroup (128, 256 or
+// 512: windows of 1024, 2048 or 4096 rows).
 template <typename T>
 void launch_trid_window(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
-                        const T *B, T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop);
+                        const T *B, T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop,
+                        int wt = 256);
 template <typename T>
 void launch_trid_window_bench(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
-                              const T *B, T *X, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+                              const T *B, T *X, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int wt = 256);
+int trid_window_shape_ok(int wt);
 // reps back-to-back launches of the forward (which = 1) or backward (2)
 // kernel of a solve of these chains between events e0 and e1.
 template <typename T>
