@@ -342,6 +342,15 @@ int wd2_ctx_trid_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
  * step), keeps the fastest and returns it and the three times (us[3], may
  * be NULL); wd2_ctx_trid_set_window sets one. */
 int wd2_ctx_trid_set_window(wd2_ctx *ctx, int window_threads);
+/* Multigrid Poisson solves (nPpeSolver 7) from the extrapolated initial
+ * guess 2 p - pprev on the interior cells (pprev: the pressure at the start
+ * of the step before; from the second step after the fields were set), enable
+ * = 1; off by default.  The stopping test and the frozen ghosts are the same,
+ * so the solve converges to the same solution in fewer V-cycles; the iterates,
+ * and so the last bits of the fields, differ from the plain start.  The
+ * adjoint's runs always use the plain start.  Returns the previous setting.
+ * No counterpart in the reference. */
+int wd2_ctx_set_mg_extrapolation(wd2_ctx *ctx, int enable);
 int wd2_ctx_trid_autotune(wd2_ctx *ctx, int reps, int *window_threads, double *us);
 /* Kernel shapes of the multigrid solver (no counterpart in the reference;
  * none changes the arithmetic, so results are the same bits): nt0 threads

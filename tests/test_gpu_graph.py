@@ -1,9 +1,10 @@
 """hipGraph steps and ensembles (no counterpart in the reference).
 
 * A step replayed as a captured graph equals the host-driven step bit for
-  bit, in FP64 and in mixed precision, including steps whose iteration
-  budget runs out and are redone host-driven (the budget follows the last
-  step's counts, so the first steps of a cold start rise past it).
+  bit, in FP64 and in mixed precision, with and without the multigrid
+  solve's extrapolated start, including steps whose iteration budget runs
+  out and are redone host-driven (the budget follows the last step's counts,
+  so the first steps of a cold start rise past it).
 * An ensemble of cavities of uneven sizes and Reynolds numbers, advanced
   together (graph mode and host-driven), gives every member the bits it has
   when it runs alone.
@@ -16,10 +17,11 @@ from wolfd2_amd import api, case, workloads
 pytestmark = pytest.mark.gpu
 
 
-def _solver(g, reg, prm, precision, graph, fields=None):
+def _solver(g, reg, prm, precision, graph, fields=None, extrap=False):
     sol = api.Solver(g, reg, prm)
     sol.set_precision(precision)
     sol.set_graph(graph)
+    sol.set_mg_extrapolation(extrap)
     if fields is None:
         z = np.zeros((g.ncol, g.ld))
         sol.set_fields(z, z, z)
@@ -29,17 +31,18 @@ def _solver(g, reg, prm, precision, graph, fields=None):
     return sol
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_graph_steps_equal_host_driven_steps(precision):
+@pytest.mark.parametrize("precision,extrap", [("fp64", False), ("mixed", False), ("fp64", True), ("mixed", True)])
+def test_graph_steps_equal_host_driven_steps(precision, extrap):
     g, reg, prm = workloads.cavity(128, 1000.0, 1e-3, nPpeSolver=7, msorit=60, sortol=1e-10)
     out = {}
     for graph in (False, True):
-        sol = _solver(g, reg, prm, precision, graph)
+        sol = _solver(g, reg, prm, precision, graph, extrap=extrap)
         tab = sol.step(30)
         out[graph] = (tab, sol.get_fields(), sol.graph_stats())
         sol.close()
     (t0, f0, _), (t1, f1, st) = out[False], out[True]
-    print(f"  {precision}: graph steps {st}, ql {t1[:6, 0].tolist()} cycles {t1[:6, 1].tolist()}", flush=True)
+    print(f"  {precision}{' extrapolated' if extrap else ''}: graph steps {st}, ql {t1[:6, 0].tolist()} "
+          f"cycles {t1[:6, 1].tolist()}", flush=True)
     assert st["replayed"] > 0 and st["replayed"] + st["redone"] == 30
     assert st["redone"] >= 1, "a cold start's rising counts should exceed the first budgets"
     assert np.array_equal(t0, t1)
@@ -118,3 +121,33 @@ def test_graph_cache_follows_precision_params_and_profiles():
         assert np.array_equal(a, b)
     # each of the five phases redoes at most its first two steps
     assert st["replayed"] >= 20
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mg_extrapolated_start_converges_to_the_same_solution(precision):
+    """With the extrapolated start (2 p - pprev) every step's multigrid solve
+    still meets the divergence tolerance, the cavity's fields stay close to
+    the plain start's after 30 steps (both solves stop at a divergence of
+    1e-10): u and v within 1e-7 of the velocity's norm, p within 1e-6 of its
+    own (relative L2); and the V-cycle count does not rise."""
+    g, reg, prm = workloads.cavity(128, 1000.0, 1e-3, nPpeSolver=7, msorit=60, sortol=1e-10)
+    out = {}
+    for extrap in (False, True):
+        sol = _solver(g, reg, prm, precision, False, extrap=extrap)
+        tab = sol.step(30)
+        out[extrap] = (tab, sol.get_fields())
+        sol.close()
+    (t0, f0), (t1, f1) = out[False], out[True]
+    print(f"  {precision}: V-cycles per step plain {t0[:, 1].mean():.2f}, extrapolated {t1[:, 1].mean():.2f}",
+          flush=True)
+    assert np.all(t1[:, 1] < 60), "a solve hit the cycle cap"
+    assert t1[:, 1].sum() <= t0[:, 1].sum()
+    S = np.s_[: g.ny + 2, : g.nx + 2]
+    # u and v against the norm of the velocity (v is small in the first
+    # steps from rest), p against its own
+    speed = np.sqrt(np.linalg.norm(f0[0][S]) ** 2 + np.linalg.norm(f0[1][S]) ** 2)
+    errs = {}
+    for name, a, b in zip("uvp", f1, f0):
+        errs[name] = np.linalg.norm(a[S] - b[S]) / (speed if name != "p" else np.linalg.norm(b[S]))
+        print(f"    {name}: extrapolated vs plain {errs[name]:.2e}", flush=True)
+    assert errs["u"] < 1e-7 and errs["v"] < 1e-7 and errs["p"] < 1e-6

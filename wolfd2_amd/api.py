@@ -265,6 +265,14 @@ class Solver:
     def set_trid_window(self, window_threads: int) -> None:
         check(lib().wd2_ctx_trid_set_window(self.ctx, window_threads))
 
+    def set_mg_extrapolation(self, enable: bool = True) -> bool:
+        """Start each multigrid solve from the extrapolated pressure 2 p - pprev
+        (wd2_ctx_set_mg_extrapolation); returns the previous setting."""
+        old = lib().wd2_ctx_set_mg_extrapolation(self.ctx, 1 if enable else 0)
+        if old < 0:
+            check(old)
+        return bool(old)
+
     def mg_cycle_time(self, cycles: int = 5) -> float:
         us = C.c_double(0)
         check(lib().wd2_ctx_mg_cycle_time(self.ctx, C.c_double(self.prm.dk), cycles, C.byref(us)))

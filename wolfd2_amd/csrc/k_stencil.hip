@@ -262,4 +262,18 @@ void launch_fill(int ld, int i0, int i1, int j0, int j1, double val, double *dst
     hipLaunchKernelGGL(k_fill, grid2(i1 - i0 + 1, j1 - j0 + 1), dim3(SX, SY), 0, s, ld, i0, i1, j0, j1, val, dst);
 }
 
+// The multigrid solve's extrapolated initial guess on the interior cells
+// (i = 2..nx, j = 2..ny): p = 2 p - pprev, the pressure of the two steps
+// before.  The ghosts keep the values PresBoundCond set.
+__global__ __launch_bounds__(SX *SY) void k_mg_extrap(int nx, int ny, int ld, double *__restrict__ p,
+                                                      const double *__restrict__ pprev) {
+    const int i = 2 + blockIdx.x * SX + threadIdx.x, j = 2 + blockIdx.y * SY + threadIdx.y;
+    if (i > nx || j > ny) return;
+    const size_t k = WD2_IX(i, j);
+    p[k] = 2.0 * p[k] - pprev[k];
+}
+void launch_mg_extrap(int nx, int ny, int ld, double *p, const double *pprev, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_extrap, grid2(nx - 1, ny - 1), dim3(SX, SY), 0, s, nx, ny, ld, p, pprev);
+}
+
 }  // namespace wd2

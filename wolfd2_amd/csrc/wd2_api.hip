@@ -146,9 +146,17 @@ struct wd2_ctx {
         const double *u;
         wd2_params P;
         int epoch;
+        bool xp;   // the step's multigrid solve starts from the extrapolated pressure
         hipGraphExec_t exec;
     };
     bool graph_on = false;
+    // Multigrid solves from the extrapolated initial guess 2 p - pprev
+    // (wd2_ctx_set_mg_extrapolation): pprev holds the pressure at the start of
+    // the step before, p_hist counts the steps since the fields were set,
+    // extrap_off suspends it (the adjoint's runs).
+    bool mg_extrap = false, extrap_off = false;
+    double *pprev = nullptr;
+    int p_hist = 0;
     std::vector<GraphEntry> graphs;
     double *gsave = nullptr;
     Scalars *h_sc = nullptr;
@@ -219,6 +227,7 @@ static void ctx_free(wd2_ctx *c) {
     (void)hipFree(c->mg_part);
     for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipFree(c->gsave);
+    (void)hipFree(c->pprev);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     (void)hipFree(c->tw.cp);
     (void)hipFree(c->zm);
@@ -1062,11 +1071,39 @@ static int cold_start(wd2_ctx *c, const wd2_params *P, int *nsor) {
 // step-end fills and the PrintDiff norms dif[0..3] = |p-pn|, |u-un|, |v-vn|,
 // |t-tn| (dif may be null; only its first three are written without the
 // energy equation).
+// Whether this step's multigrid solve starts from 2 p - pprev: asked for,
+// not suspended, the multigrid solver, and a step of history since the
+// fields were set.  The converged solve is the same either way (the ghosts
+// stay the values PresBoundCond set); only the number of V-cycles changes.
+static bool mg_extrap_active(const wd2_ctx *c, const wd2_params *P) {
+    return c->mg_extrap && !c->extrap_off && P->nPpeSolver == 7 && c->p_hist > 0;
+}
+static int mg_extrap_alloc(wd2_ctx *c) {
+    if (!c->pprev) HIPCHK(hipMalloc(&c->pprev, c->nf * sizeof(double)));
+    return WD2_OK;
+}
+
+// The adjoint's runs (forward, segment re-runs, tangent re-runs) solve from
+// the plain initial guess, so that a re-run from a checkpoint repeats the
+// forward run bit for bit without the extrapolation's history.
+struct ExtrapOff {
+    wd2_ctx *c;
+    bool old;
+    explicit ExtrapOff(wd2_ctx *c_) : c(c_), old(c_->extrap_off) { c->extrap_off = true; }
+    ~ExtrapOff() { c->extrap_off = old; }
+};
+
 static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double *dif) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     const wd2_step_options &O = c->opt;
     const bool therm = O.nthermen == 1;
     hipStream_t s = c->s;
+    const bool xp = mg_extrap_active(c, P);
+    if (xp) {
+        int st0 = mg_extrap_alloc(c);
+        if (st0) return st0;
+        launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pn, c->pprev, s);   // the pressure of the step before
+    }
     // main.f:696-704 and the first iteration's us = u, vs = v (main.f:741-747)
     // in one launch.  Without a density field d and dn are both still all +0
     // (nothing has written d), so that copy is skipped.
@@ -1100,6 +1137,7 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
         if (O.nfiltv == 1) launch_filter(c->R, c->TH, ld, VV, O.fpv, c->vs, c->qh, s);
         launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
         launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+        if (xp && l == 1) launch_mg_extrap(nx, ny, ld, c->p, c->pprev, s);
         st = ppe(c, P->lCartesGrid, P->nPpeSolver, P->msorit, P->dk, P->sortol, P->sorrel, c->us, c->vs, c->p, nsor);
         if (st) return st;
         launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
@@ -1149,6 +1187,7 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
     launch_max_norm(nx, ny, ld, c->vn, c->v, c->partials + 2 * RED_MAX_BLOCKS, c->sc, 4, s);
     if (therm) launch_max_norm(nx, ny, ld, c->tn, c->t, c->partials + 3 * RED_MAX_BLOCKS, c->sc, 5, s);
     if (dif) HIPCHK(hipMemcpyAsync(dif, &c->sc->val[2], (therm ? 4 : 3) * sizeof(double), hipMemcpyDeviceToHost, s));
+    c->p_hist++;
     return check_err(c);
 }
 
@@ -1172,15 +1211,16 @@ static bool graph_ok(const wd2_ctx *c, const wd2_params *P) {
 
 // The step's work on the stream (the body of a capture): main.f:690-972 with
 // Q q-l iterations and C V-cycles queued.
-static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C) {
+static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C, bool xp) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     hipStream_t s = c->s;
     const wd2_step_options &O = c->opt;
-    {
-        const double *src[3] = {c->u, c->v, c->p};
-        double *dst[3] = {c->gsave, c->gsave + c->nf, c->gsave + 2 * c->nf};
-        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 3, s);
+    {   // the step's start, for a redo (pn: the extrapolation's history)
+        const double *src[4] = {c->u, c->v, c->p, c->pn};
+        double *dst[4] = {c->gsave, c->gsave + c->nf, c->gsave + 2 * c->nf, c->gsave + 3 * c->nf};
+        launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 4, s);
     }
+    if (xp) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pn, c->pprev, s);
     {
         const double *src[5] = {c->p, c->u, c->v, c->u, c->v};
         double *dst[5] = {c->pn, c->un, c->vn, c->us, c->vs};
@@ -1204,6 +1244,7 @@ static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C) {
     if (O.nfiltv == 1) launch_filter(c->R, c->TH, ld, VV, O.fpv, c->vs, c->qh, s);
     launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
     launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
+    if (xp) launch_mg_extrap(nx, ny, ld, c->p, c->pprev, s);
     // Ppe (pressure.f:30) with the multigrid solver
     launch_divergence(nx, ny, ld, 1, c->M, c->us, c->vs, c->div, s);
     launch_ppe_rhs(nx, ny, ld, P->dk, c->div, nullptr, c->rhs, s);
@@ -1238,19 +1279,21 @@ static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C) {
 static int graph_begin(wd2_ctx *c, const wd2_params *P) {
     int st = mg_prepare(c);
     if (st) return st;
-    if (!c->gsave) HIPCHK(hipMalloc(&c->gsave, 3 * c->nf * sizeof(double)));
+    if (!c->gsave) HIPCHK(hipMalloc(&c->gsave, 4 * c->nf * sizeof(double)));
+    const bool xp = mg_extrap_active(c, P);
+    if (xp && (st = mg_extrap_alloc(c))) return st;
     if (!c->h_sc) HIPCHK(hipHostMalloc((void **)&c->h_sc, sizeof(Scalars), hipHostMallocDefault));
     const int Q = std::min(P->mqiter, c->ql_last + 2);
     const int C = std::min(P->msorit + 1, c->mg_last + 2);
     hipGraphExec_t exec = nullptr;
     for (auto &g : c->graphs)
-        if (g.q == Q && g.cyc == C && g.u == c->u && g.epoch == g_graph_epoch &&
+        if (g.q == Q && g.cyc == C && g.u == c->u && g.epoch == g_graph_epoch && g.xp == xp &&
             std::memcmp(&g.P, P, sizeof(wd2_params)) == 0)
             exec = g.exec;
     if (!exec) {
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-        step_body(c, P, Q, C);
+        step_body(c, P, Q, C, xp);
         const hipError_t ec = hipStreamEndCapture(c->s, &graph);
         if (ec != hipSuccess) return fail(WD2_ERR_HIP, "step capture: %s", hipGetErrorString(ec));
         const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -1260,7 +1303,7 @@ static int graph_begin(wd2_ctx *c, const wd2_params *P) {
             (void)hipGraphExecDestroy(c->graphs.front().exec);
             c->graphs.erase(c->graphs.begin());
         }
-        c->graphs.push_back({Q, C, c->u, *P, g_graph_epoch, exec});
+        c->graphs.push_back({Q, C, c->u, *P, g_graph_epoch, xp, exec});
     }
     HIPCHK(hipGraphLaunch(exec, c->s));
     c->g_q = Q;
@@ -1276,9 +1319,9 @@ static int graph_end(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, doubl
     const bool mg_short = h.mg_done == 0;
     if (ql_short || mg_short) {
         // a budget ran out: restore the step's start and redo it host-driven
-        const double *src[3] = {c->gsave, c->gsave + c->nf, c->gsave + 2 * c->nf};
-        double *dst[3] = {c->u, c->v, c->p};
-        launch_copy_set(c->ld, 0, c->nx + 1, 0, c->ny + 1, src, dst, 3, c->s);
+        const double *src[4] = {c->gsave, c->gsave + c->nf, c->gsave + 2 * c->nf, c->gsave + 3 * c->nf};
+        double *dst[4] = {c->u, c->v, c->p, c->pn};
+        launch_copy_set(c->ld, 0, c->nx + 1, 0, c->ny + 1, src, dst, 4, c->s);
         c->graph_redos++;
         int st = one_step(c, P, nql, nsor, dif);
         if (st) return st;
@@ -1288,6 +1331,7 @@ static int graph_end(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, doubl
     }
     std::swap(c->u, c->us);
     std::swap(c->v, c->vs);
+    c->p_hist++;
     *nql = h.ql_stop ? h.ql_stop : -1;
     int nconv = h.mg_nconv;
     c->mg_div = h.val[7];
@@ -2192,6 +2236,7 @@ int wd2_ctx_destroy(wd2_ctx *c) {
 int wd2_ctx_set_fields(wd2_ctx *c, int ld, const double *u, const double *v, const double *p) {
     int st;
     if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v)) || (st = up(c, p, ld, c->p))) return st;
+    c->p_hist = 0;
     HIPCHK(hipStreamSynchronize(c->s));
     return WD2_OK;
 }
@@ -2207,6 +2252,7 @@ int wd2_ctx_get_fields(wd2_ctx *c, int ld, double *u, double *v, double *p) {
 
 int wd2_ctx_cold_start(wd2_ctx *c, const wd2_params *P, int *nSorConv) {
     int n = 0;
+    c->p_hist = 0;
     int st = cold_start(c, P, &n);
     if (nSorConv) *nSorConv = n;
     return st;
@@ -2468,6 +2514,13 @@ int wd2_ctx_trid_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     return check_err(c);
 }
 
+int wd2_ctx_set_mg_extrapolation(wd2_ctx *c, int enable) {
+    if (!c) return fail(WD2_ERR_ARG, "wd2_ctx_set_mg_extrapolation: null context");
+    const int old = c->mg_extrap ? 1 : 0;
+    c->mg_extrap = enable != 0;
+    return old;
+}
+
 int wd2_ctx_trid_set_window(wd2_ctx *c, int wt) {
     if (!c || !trid_window_shape_ok(wt)) return fail(WD2_ERR_ARG, "window shape %d (128, 256 or 512 lanes)", wt);
     if (wt != c->win_wt) graph_clear(c);
@@ -2642,6 +2695,7 @@ int wd2_adj_forward(wd2_adj *a, const wd2_params *P, int nsteps, int *ql_iter, i
     wd2_ctx *c = a->c;
     int st = adj_check(c, P);
     if (st) return st;
+    const ExtrapOff plain(c);
     const int nseg = (nsteps + a->every - 1) / a->every;
     if (nseg > a->nseg_cap) {
         (void)hipFree(a->ckpt);
@@ -2683,6 +2737,7 @@ int wd2_adj_backward(wd2_adj *a, int ld_h, const double *u_bar, const double *v_
     if (ld_h < c->nx + 2) return fail(WD2_ERR_ARG, "ld=%d < nx+2=%d", ld_h, c->nx + 2);
     int st = adj_check(c, &a->P);
     if (st) return st;
+    const ExtrapOff plain(c);
     hipStream_t s = c->s;
     const size_t nf = c->nf;
     launch_adj_zero(3 * (long)nf, a->ub, s);   // ub, vb, pb are adjacent
@@ -2745,6 +2800,7 @@ int wd2_adj_tangent(wd2_adj *a, int ld_h, const double *du0, const double *dv0, 
     if (ld_h < c->nx + 2) return fail(WD2_ERR_ARG, "ld=%d < nx+2=%d", ld_h, c->nx + 2);
     int st = adj_check(c, &a->P);
     if (st || (st = tl_alloc(a))) return st;
+    const ExtrapOff plain(c);
     hipStream_t s = c->s;
     const size_t nf = c->nf;
     launch_adj_zero(3 * (long)nf, a->ub, s);   // the tangent state: ub, vb, pb are adjacent
