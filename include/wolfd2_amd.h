@@ -342,15 +342,18 @@ int wd2_ctx_trid_kernel_time(wd2_ctx *ctx, int which, int reps, double *avg_us);
  * step), keeps the fastest and returns it and the three times (us[3], may
  * be NULL); wd2_ctx_trid_set_window sets one. */
 int wd2_ctx_trid_set_window(wd2_ctx *ctx, int window_threads);
-/* Multigrid Poisson solves (nPpeSolver 7) from the extrapolated initial
- * guess 2 p - pprev on the interior cells (pprev: the pressure at the start
- * of the step before; from the second step after the fields were set), enable
- * = 1; off by default.  The stopping test and the frozen ghosts are the same,
- * so the solve converges to the same solution in fewer V-cycles; the iterates,
- * and so the last bits of the fields, differ from the plain start.  The
- * adjoint's runs always use the plain start.  Returns the previous setting.
- * No counterpart in the reference. */
-int wd2_ctx_set_mg_extrapolation(wd2_ctx *ctx, int enable);
+/* Multigrid Poisson solves (nPpeSolver 7) from an extrapolated initial guess
+ * on the interior cells, from the pressures p^(n-1), p^(n-2) at the start of
+ * the steps before: order 1, 2 p^n - p^(n-1) (from the second step after the
+ * fields were set); order 2, 3 p^n - 3 p^(n-1) + p^(n-2) (from the third; the
+ * second step is linear); order 0, the plain start (the default).  The
+ * stopping test and the frozen ghosts are the same, so the solve converges
+ * to the same solution in fewer V-cycles; the iterates, and so the last bits
+ * of the fields, differ from the plain start.  The adjoint's runs always use
+ * the plain start (and the step after them restarts the history).  Returns
+ * the previous order; WD2_ERR_ARG for an order outside 0..2.  No
+ * counterpart in the reference. */
+int wd2_ctx_set_mg_extrapolation(wd2_ctx *ctx, int order);
 int wd2_ctx_trid_autotune(wd2_ctx *ctx, int reps, int *window_threads, double *us);
 /* Kernel shapes of the multigrid solver (no counterpart in the reference;
  * none changes the arithmetic, so results are the same bits): nt0 threads

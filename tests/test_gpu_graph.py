@@ -2,7 +2,8 @@
 
 * A step replayed as a captured graph equals the host-driven step bit for
   bit, in FP64 and in mixed precision, with and without the multigrid
-  solve's extrapolated start, including steps whose iteration budget runs
+  solve's extrapolated start (linear and quadratic, also switched during a
+  run), including steps whose iteration budget runs
   out and are redone host-driven (the budget follows the last step's counts,
   so the first steps of a cold start rise past it).
 * An ensemble of cavities of uneven sizes and Reynolds numbers, advanced
@@ -17,7 +18,7 @@ from wolfd2_amd import api, case, workloads
 pytestmark = pytest.mark.gpu
 
 
-def _solver(g, reg, prm, precision, graph, fields=None, extrap=False):
+def _solver(g, reg, prm, precision, graph, fields=None, extrap=0):
     sol = api.Solver(g, reg, prm)
     sol.set_precision(precision)
     sol.set_graph(graph)
@@ -31,7 +32,7 @@ def _solver(g, reg, prm, precision, graph, fields=None, extrap=False):
     return sol
 
 
-@pytest.mark.parametrize("precision,extrap", [("fp64", False), ("mixed", False), ("fp64", True), ("mixed", True)])
+@pytest.mark.parametrize("precision,extrap", [("fp64", 0), ("mixed", 0), ("fp64", 2), ("mixed", 1), ("mixed", 2)])
 def test_graph_steps_equal_host_driven_steps(precision, extrap):
     g, reg, prm = workloads.cavity(128, 1000.0, 1e-3, nPpeSolver=7, msorit=60, sortol=1e-10)
     out = {}
@@ -41,7 +42,7 @@ def test_graph_steps_equal_host_driven_steps(precision, extrap):
         out[graph] = (tab, sol.get_fields(), sol.graph_stats())
         sol.close()
     (t0, f0, _), (t1, f1, st) = out[False], out[True]
-    print(f"  {precision}{' extrapolated' if extrap else ''}: graph steps {st}, ql {t1[:6, 0].tolist()} "
+    print(f"  {precision}, extrapolation order {extrap}: graph steps {st}, ql {t1[:6, 0].tolist()} "
           f"cycles {t1[:6, 1].tolist()}", flush=True)
     assert st["replayed"] > 0 and st["replayed"] + st["redone"] == 30
     assert st["redone"] >= 1, "a cold start's rising counts should exceed the first budgets"
@@ -88,9 +89,10 @@ def test_ensemble_members_equal_their_single_runs(graph):
 
 def test_graph_cache_follows_precision_params_and_profiles():
     """Cached step graphs bake in the precision, the step parameters and the
-    boundary tables: after each change the graph steps still equal host-driven
-    steps bit for bit, and they replay (a stale graph would fail its stopping
-    test and be redone, or give other fields)."""
+    boundary tables and the extrapolated start's order and history slot:
+    after each change the graph steps still equal host-driven steps bit for
+    bit, and they replay (a stale graph would fail its stopping test and be
+    redone, or give other fields)."""
     g, reg, prm = workloads.cavity(96, 400.0, 5e-4, nPpeSolver=7, msorit=100, sortol=1e-11)
     z = np.zeros((g.ncol, g.ld))
     x, y = case.uniform_nodes(g.nx, g.ny)
@@ -106,8 +108,10 @@ def test_graph_cache_follows_precision_params_and_profiles():
         tabs.append(sol.step(6))
         sol.set_params(prm2)
         tabs.append(sol.step(6))
+        sol.set_mg_extrapolation(2)
         sol.set_bc_profile(1, 1, case.NORTH, case.VAR_U, lid2)
         tabs.append(sol.step(6))
+        sol.set_mg_extrapolation(1)
         sol.set_precision("fp64")
         tabs.append(sol.step(6))
         out = sol.get_fields(), np.concatenate(tabs), sol.graph_stats()
@@ -123,23 +127,27 @@ def test_graph_cache_follows_precision_params_and_profiles():
     assert st["replayed"] >= 20
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_mg_extrapolated_start_converges_to_the_same_solution(precision):
-    """With the extrapolated start (2 p - pprev) every step's multigrid solve
-    still meets the divergence tolerance, the cavity's fields stay close to
-    the plain start's after 30 steps (both solves stop at a divergence of
-    1e-10): u and v within 1e-7 of the velocity's norm, p within 1e-6 of its
-    own (relative L2); and the V-cycle count does not rise."""
+@pytest.mark.parametrize("precision,order", [("fp64", 1), ("mixed", 1), ("fp64", 2), ("mixed", 2)])
+def test_mg_extrapolated_start_converges_to_the_same_solution(precision, order):
+    """With the extrapolated start (order 1: 2 p^n - p^(n-1), order 2:
+    3 p^n - 3 p^(n-1) + p^(n-2)) every step's multigrid solve still meets the
+    divergence tolerance, the cavity's fields stay close to the plain start's
+    after 30 steps (both solves stop at a divergence of 1e-10): u and v within
+    1e-7 of the velocity's norm, p within 1e-6 of its own (relative L2); and
+    the V-cycle count does not rise."""
     g, reg, prm = workloads.cavity(128, 1000.0, 1e-3, nPpeSolver=7, msorit=60, sortol=1e-10)
     out = {}
-    for extrap in (False, True):
+    for extrap in (0, order):
         sol = _solver(g, reg, prm, precision, False, extrap=extrap)
+        if extrap == 2:
+            with pytest.raises(api.Wd2Error):
+                sol.set_mg_extrapolation(3)
         tab = sol.step(30)
         out[extrap] = (tab, sol.get_fields())
         sol.close()
-    (t0, f0), (t1, f1) = out[False], out[True]
-    print(f"  {precision}: V-cycles per step plain {t0[:, 1].mean():.2f}, extrapolated {t1[:, 1].mean():.2f}",
-          flush=True)
+    (t0, f0), (t1, f1) = out[0], out[order]
+    print(f"  {precision}, order {order}: V-cycles per step plain {t0[:, 1].mean():.2f}, extrapolated "
+          f"{t1[:, 1].mean():.2f}", flush=True)
     assert np.all(t1[:, 1] < 60), "a solve hit the cycle cap"
     assert t1[:, 1].sum() <= t0[:, 1].sum()
     S = np.s_[: g.ny + 2, : g.nx + 2]

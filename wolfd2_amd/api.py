@@ -265,13 +265,15 @@ class Solver:
     def set_trid_window(self, window_threads: int) -> None:
         check(lib().wd2_ctx_trid_set_window(self.ctx, window_threads))
 
-    def set_mg_extrapolation(self, enable: bool = True) -> bool:
-        """Start each multigrid solve from the extrapolated pressure 2 p - pprev
-        (wd2_ctx_set_mg_extrapolation); returns the previous setting."""
-        old = lib().wd2_ctx_set_mg_extrapolation(self.ctx, 1 if enable else 0)
+    def set_mg_extrapolation(self, order: int = 1) -> int:
+        """Start each multigrid solve from the pressure extrapolated from the
+        steps before (wd2_ctx_set_mg_extrapolation): order 1, 2 p^n - p^(n-1);
+        2, 3 p^n - 3 p^(n-1) + p^(n-2); 0 (or False), the plain start.
+        Returns the previous order."""
+        old = lib().wd2_ctx_set_mg_extrapolation(self.ctx, int(order))
         if old < 0:
             check(old)
-        return bool(old)
+        return old
 
     def mg_cycle_time(self, cycles: int = 5) -> float:
         us = C.c_double(0)

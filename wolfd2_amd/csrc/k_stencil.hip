@@ -263,17 +263,19 @@ void launch_fill(int ld, int i0, int i1, int j0, int j1, double val, double *dst
 }
 
 // The multigrid solve's extrapolated initial guess on the interior cells
-// (i = 2..nx, j = 2..ny): p = 2 p - pprev, the pressure of the two steps
-// before.  The ghosts keep the values PresBoundCond set.
+// (i = 2..nx, j = 2..ny) from the pressure p = p^n and the pressures at the
+// start of the steps before, p1 = p^(n-1) and p2 = p^(n-2): linear, 2 p - p1,
+// when p2 is null, else quadratic, 3 (p - p1) + p2.  The ghosts keep the
+// values PresBoundCond set.
 __global__ __launch_bounds__(SX *SY) void k_mg_extrap(int nx, int ny, int ld, double *__restrict__ p,
-                                                      const double *__restrict__ pprev) {
+                                                      const double *__restrict__ p1, const double *__restrict__ p2) {
     const int i = 2 + blockIdx.x * SX + threadIdx.x, j = 2 + blockIdx.y * SY + threadIdx.y;
     if (i > nx || j > ny) return;
     const size_t k = WD2_IX(i, j);
-    p[k] = 2.0 * p[k] - pprev[k];
+    p[k] = p2 ? 3.0 * (p[k] - p1[k]) + p2[k] : 2.0 * p[k] - p1[k];
 }
-void launch_mg_extrap(int nx, int ny, int ld, double *p, const double *pprev, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_extrap, grid2(nx - 1, ny - 1), dim3(SX, SY), 0, s, nx, ny, ld, p, pprev);
+void launch_mg_extrap(int nx, int ny, int ld, double *p, const double *p1, const double *p2, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_extrap, grid2(nx - 1, ny - 1), dim3(SX, SY), 0, s, nx, ny, ld, p, p1, p2);
 }
 
 }  // namespace wd2

@@ -146,17 +146,22 @@ struct wd2_ctx {
         const double *u;
         wd2_params P;
         int epoch;
-        bool xp;   // the step's multigrid solve starts from the extrapolated pressure
+        int xp, xw;   // the extrapolated start's order (0: plain start) and history slot
         hipGraphExec_t exec;
     };
     bool graph_on = false;
-    // Multigrid solves from the extrapolated initial guess 2 p - pprev
-    // (wd2_ctx_set_mg_extrapolation): pprev holds the pressure at the start of
-    // the step before, p_hist counts the steps since the fields were set,
-    // extrap_off suspends it (the adjoint's runs).
-    bool mg_extrap = false, extrap_off = false;
-    double *pprev = nullptr;
-    int p_hist = 0;
+    // Multigrid solves from an extrapolated initial guess
+    // (wd2_ctx_set_mg_extrapolation): mg_extrap is the order asked for (0
+    // off, 1 linear, 2 quadratic), extrap_off suspends it (the adjoint's
+    // runs), p_hist counts the steps since the fields were set.  phist holds
+    // two history slots: a step writes the pressure at the start of the step
+    // before (pn) into slot xw, the other slot holding the one before that
+    // from the step before, and xw flips when the step completes; x_run counts
+    // the steps in a row that wrote a slot.
+    int mg_extrap = 0;
+    bool extrap_off = false;
+    double *phist = nullptr;
+    int p_hist = 0, xw = 0, x_run = 0, g_x = 0;
     std::vector<GraphEntry> graphs;
     double *gsave = nullptr;
     Scalars *h_sc = nullptr;
@@ -227,7 +232,7 @@ static void ctx_free(wd2_ctx *c) {
     (void)hipFree(c->mg_part);
     for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     (void)hipFree(c->gsave);
-    (void)hipFree(c->pprev);
+    (void)hipFree(c->phist);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     (void)hipFree(c->tw.cp);
     (void)hipFree(c->zm);
@@ -1071,16 +1076,36 @@ static int cold_start(wd2_ctx *c, const wd2_params *P, int *nsor) {
 // step-end fills and the PrintDiff norms dif[0..3] = |p-pn|, |u-un|, |v-vn|,
 // |t-tn| (dif may be null; only its first three are written without the
 // energy equation).
-// Whether this step's multigrid solve starts from 2 p - pprev: asked for,
-// not suspended, the multigrid solver, and a step of history since the
-// fields were set.  The converged solve is the same either way (the ghosts
-// stay the values PresBoundCond set); only the number of V-cycles changes.
-static bool mg_extrap_active(const wd2_ctx *c, const wd2_params *P) {
-    return c->mg_extrap && !c->extrap_off && P->nPpeSolver == 7 && c->p_hist > 0;
+// The order of this step's extrapolated start: 0 (the plain start) unless
+// asked for, not suspended, the multigrid solver and a step of history since
+// the fields were set; then 1, p = 2 p^n - p^(n-1) on the interior cells, or
+// 2, p = 3 p^n - 3 p^(n-1) + p^(n-2), when asked for and the step before
+// wrote its history slot.  The converged solve is the same either way (the
+// ghosts stay the values PresBoundCond set); only the number of V-cycles
+// changes.
+static int mg_extrap_mode(const wd2_ctx *c, const wd2_params *P) {
+    if (!(c->mg_extrap > 0 && !c->extrap_off && P->nPpeSolver == 7 && c->p_hist > 0)) return 0;
+    return (c->mg_extrap >= 2 && c->x_run > 0) ? 2 : 1;
 }
 static int mg_extrap_alloc(wd2_ctx *c) {
-    if (!c->pprev) HIPCHK(hipMalloc(&c->pprev, c->nf * sizeof(double)));
+    if (!c->phist) HIPCHK(hipMalloc(&c->phist, 2 * c->nf * sizeof(double)));
     return WD2_OK;
+}
+// The history slots of a step of order x: p^(n-1) goes to slot xw, p^(n-2)
+// is the other slot (order 2 only).
+static double *extrap_p1(const wd2_ctx *c) { return c->phist + (size_t)c->xw * c->nf; }
+static const double *extrap_p2(const wd2_ctx *c, int x) {
+    return x == 2 ? c->phist + (size_t)(c->xw ^ 1) * c->nf : nullptr;
+}
+// A step of order x completed: flip the slot, count the history.
+static void extrap_advance(wd2_ctx *c, int x) {
+    if (x) {
+        c->xw ^= 1;
+        c->x_run++;
+    } else {
+        c->x_run = 0;
+    }
+    c->p_hist++;
 }
 
 // The adjoint's runs (forward, segment re-runs, tangent re-runs) solve from
@@ -1098,11 +1123,11 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
     const wd2_step_options &O = c->opt;
     const bool therm = O.nthermen == 1;
     hipStream_t s = c->s;
-    const bool xp = mg_extrap_active(c, P);
+    const int xp = mg_extrap_mode(c, P);
     if (xp) {
         int st0 = mg_extrap_alloc(c);
         if (st0) return st0;
-        launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pn, c->pprev, s);   // the pressure of the step before
+        launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pn, extrap_p1(c), s);   // the pressure of the step before
     }
     // main.f:696-704 and the first iteration's us = u, vs = v (main.f:741-747)
     // in one launch.  Without a density field d and dn are both still all +0
@@ -1137,7 +1162,7 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
         if (O.nfiltv == 1) launch_filter(c->R, c->TH, ld, VV, O.fpv, c->vs, c->qh, s);
         launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
         launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
-        if (xp && l == 1) launch_mg_extrap(nx, ny, ld, c->p, c->pprev, s);
+        if (xp && l == 1) launch_mg_extrap(nx, ny, ld, c->p, extrap_p1(c), extrap_p2(c, xp), s);
         st = ppe(c, P->lCartesGrid, P->nPpeSolver, P->msorit, P->dk, P->sortol, P->sorrel, c->us, c->vs, c->p, nsor);
         if (st) return st;
         launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
@@ -1187,7 +1212,7 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
     launch_max_norm(nx, ny, ld, c->vn, c->v, c->partials + 2 * RED_MAX_BLOCKS, c->sc, 4, s);
     if (therm) launch_max_norm(nx, ny, ld, c->tn, c->t, c->partials + 3 * RED_MAX_BLOCKS, c->sc, 5, s);
     if (dif) HIPCHK(hipMemcpyAsync(dif, &c->sc->val[2], (therm ? 4 : 3) * sizeof(double), hipMemcpyDeviceToHost, s));
-    c->p_hist++;
+    extrap_advance(c, xp);
     return check_err(c);
 }
 
@@ -1211,7 +1236,7 @@ static bool graph_ok(const wd2_ctx *c, const wd2_params *P) {
 
 // The step's work on the stream (the body of a capture): main.f:690-972 with
 // Q q-l iterations and C V-cycles queued.
-static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C, bool xp) {
+static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C, int xp) {
     const int nx = c->nx, ny = c->ny, ld = c->ld;
     hipStream_t s = c->s;
     const wd2_step_options &O = c->opt;
@@ -1220,7 +1245,7 @@ static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C, bool xp) {
         double *dst[4] = {c->gsave, c->gsave + c->nf, c->gsave + 2 * c->nf, c->gsave + 3 * c->nf};
         launch_copy_set(ld, 0, nx + 1, 0, ny + 1, src, dst, 4, s);
     }
-    if (xp) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pn, c->pprev, s);
+    if (xp) launch_copy(ld, 0, nx + 1, 0, ny + 1, c->pn, extrap_p1(c), s);
     {
         const double *src[5] = {c->p, c->u, c->v, c->u, c->v};
         double *dst[5] = {c->pn, c->un, c->vn, c->us, c->vs};
@@ -1244,7 +1269,7 @@ static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C, bool xp) {
     if (O.nfiltv == 1) launch_filter(c->R, c->TH, ld, VV, O.fpv, c->vs, c->qh, s);
     launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
     launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
-    if (xp) launch_mg_extrap(nx, ny, ld, c->p, c->pprev, s);
+    if (xp) launch_mg_extrap(nx, ny, ld, c->p, extrap_p1(c), extrap_p2(c, xp), s);
     // Ppe (pressure.f:30) with the multigrid solver
     launch_divergence(nx, ny, ld, 1, c->M, c->us, c->vs, c->div, s);
     launch_ppe_rhs(nx, ny, ld, P->dk, c->div, nullptr, c->rhs, s);
@@ -1280,14 +1305,14 @@ static int graph_begin(wd2_ctx *c, const wd2_params *P) {
     int st = mg_prepare(c);
     if (st) return st;
     if (!c->gsave) HIPCHK(hipMalloc(&c->gsave, 4 * c->nf * sizeof(double)));
-    const bool xp = mg_extrap_active(c, P);
+    const int xp = mg_extrap_mode(c, P);
     if (xp && (st = mg_extrap_alloc(c))) return st;
     if (!c->h_sc) HIPCHK(hipHostMalloc((void **)&c->h_sc, sizeof(Scalars), hipHostMallocDefault));
     const int Q = std::min(P->mqiter, c->ql_last + 2);
     const int C = std::min(P->msorit + 1, c->mg_last + 2);
     hipGraphExec_t exec = nullptr;
     for (auto &g : c->graphs)
-        if (g.q == Q && g.cyc == C && g.u == c->u && g.epoch == g_graph_epoch && g.xp == xp &&
+        if (g.q == Q && g.cyc == C && g.u == c->u && g.epoch == g_graph_epoch && g.xp == xp && g.xw == c->xw &&
             std::memcmp(&g.P, P, sizeof(wd2_params)) == 0)
             exec = g.exec;
     if (!exec) {
@@ -1303,11 +1328,12 @@ static int graph_begin(wd2_ctx *c, const wd2_params *P) {
             (void)hipGraphExecDestroy(c->graphs.front().exec);
             c->graphs.erase(c->graphs.begin());
         }
-        c->graphs.push_back({Q, C, c->u, *P, g_graph_epoch, xp, exec});
+        c->graphs.push_back({Q, C, c->u, *P, g_graph_epoch, xp, c->xw, exec});
     }
     HIPCHK(hipGraphLaunch(exec, c->s));
     c->g_q = Q;
     c->g_c = C;
+    c->g_x = xp;
     return WD2_OK;
 }
 
@@ -1331,7 +1357,7 @@ static int graph_end(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, doubl
     }
     std::swap(c->u, c->us);
     std::swap(c->v, c->vs);
-    c->p_hist++;
+    extrap_advance(c, c->g_x);
     *nql = h.ql_stop ? h.ql_stop : -1;
     int nconv = h.mg_nconv;
     c->mg_div = h.val[7];
@@ -2236,7 +2262,7 @@ int wd2_ctx_destroy(wd2_ctx *c) {
 int wd2_ctx_set_fields(wd2_ctx *c, int ld, const double *u, const double *v, const double *p) {
     int st;
     if ((st = up(c, u, ld, c->u)) || (st = up(c, v, ld, c->v)) || (st = up(c, p, ld, c->p))) return st;
-    c->p_hist = 0;
+    c->p_hist = c->x_run = 0;
     HIPCHK(hipStreamSynchronize(c->s));
     return WD2_OK;
 }
@@ -2252,7 +2278,7 @@ int wd2_ctx_get_fields(wd2_ctx *c, int ld, double *u, double *v, double *p) {
 
 int wd2_ctx_cold_start(wd2_ctx *c, const wd2_params *P, int *nSorConv) {
     int n = 0;
-    c->p_hist = 0;
+    c->p_hist = c->x_run = 0;
     int st = cold_start(c, P, &n);
     if (nSorConv) *nSorConv = n;
     return st;
@@ -2514,10 +2540,11 @@ int wd2_ctx_trid_kernel_time(wd2_ctx *c, int which, int reps, double *avg_us) {
     return check_err(c);
 }
 
-int wd2_ctx_set_mg_extrapolation(wd2_ctx *c, int enable) {
+int wd2_ctx_set_mg_extrapolation(wd2_ctx *c, int order) {
     if (!c) return fail(WD2_ERR_ARG, "wd2_ctx_set_mg_extrapolation: null context");
-    const int old = c->mg_extrap ? 1 : 0;
-    c->mg_extrap = enable != 0;
+    if (order < 0 || order > 2) return fail(WD2_ERR_ARG, "extrapolation order %d (0, 1 or 2)", order);
+    const int old = c->mg_extrap;
+    c->mg_extrap = order;
     return old;
 }
 

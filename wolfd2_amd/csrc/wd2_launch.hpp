@@ -104,8 +104,9 @@ void launch_copy(int ld, int i0, int i1, int j0, int j1, const double *src, doub
 void launch_copy_set(int ld, int i0, int i1, int j0, int j1, const double *const *src, double *const *dst, int n,
                      hipStream_t s);
 void launch_fill(int ld, int i0, int i1, int j0, int j1, double val, double *dst, hipStream_t s);
-// p = 2 p - pprev on the pressure interior (the multigrid solve's initial guess)
-void launch_mg_extrap(int nx, int ny, int ld, double *p, const double *pprev, hipStream_t s);
+// p = 2 p - p1 (p2 null) or 3 (p - p1) + p2 on the pressure interior (the
+// multigrid solve's extrapolated initial guess)
+void launch_mg_extrap(int nx, int ny, int ld, double *p, const double *p1, const double *p2, hipStream_t s);
 
 // SOR family.  coef: null -> coefficients from rau/rgv; else five fields.
 struct PpeCoef {
