@@ -627,10 +627,8 @@ static void momentum_pair(wd2_ctx *c, double dk, double re, double fr, const Mom
         launch_mom_n<T>(nx, ny, ld, 1, M, c->un, c->vn, t.cnvn, t.difn, s);
         launch_mom_n<T>(nx, ny, ld, 2, M, c->un, c->vn, t2.cnvn, t2.difn, s);
     }
-    launch_mom_asm<T>(c->R, ld, 1, dk, re, fr, M, t, c->us, c->vs, c->un, c->vn, c->d, c->dn, ch, ch2, c->zm, s,
-                      &c->sc->err, live, stop);
-    launch_mom_asm<T>(c->R, ld, 2, dk, re, fr, M, t2, c->us, c->vs, c->un, c->vn, c->d, c->dn, shift(ch, n1),
-                      shift(ch2, n1), c->zm + n1, s, &c->sc->err, live, stop);
+    launch_mom_asm_pair<T>(c->R, ld, dk, re, fr, M, t, t2, c->us, c->vs, c->un, c->vn, c->d, c->dn, ch, ch2, c->zm,
+                           shift(ch, n1), shift(ch2, n1), c->zm + n1, s, &c->sc->err, live, stop);
     const Chains cs{n1, n1, n2};
     if constexpr (std::is_same<T, float>::value) {
         // The window solver serves iterations 2.. of the q-l loop.  The first
@@ -1597,10 +1595,8 @@ static int adj_step(wd2_adj *a, const double *un, const double *vn, const double
     auto shift = [](const Chain &q, long o) { return Chain{q.a + o, q.d + o, q.c + o, q.b + o, q.x + o}; };
     launch_mom_n<double>(nx, ny, ld, 1, c->M, un, vn, t.cnvn, t.difn, s);
     launch_mom_n<double>(nx, ny, ld, 2, c->M, un, vn, t2.cnvn, t2.difn, s);
-    launch_mom_asm<double>(c->R, ld, 1, P.dk, P.re, P.fr, c->M, t, us, vs, un, vn, c->d, c->dn, c->ch, c->ch2,
-                           c->zm, s, &c->sc->err, 0);
-    launch_mom_asm<double>(c->R, ld, 2, P.dk, P.re, P.fr, c->M, t2, us, vs, un, vn, c->d, c->dn, shift(c->ch, n1),
-                           shift(c->ch2, n1), c->zm + n1, s, &c->sc->err, 0);
+    launch_mom_asm_pair<double>(c->R, ld, P.dk, P.re, P.fr, c->M, t, t2, us, vs, un, vn, c->d, c->dn, c->ch, c->ch2,
+                                c->zm, shift(c->ch, n1), shift(c->ch2, n1), c->zm + n1, s, &c->sc->err, 0);
     launch_trid<double>(c->tw, n, cs, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
     launch_adj_tchain(n, cs, c->ch.a, c->ch.d, c->ch.c, a->At1, a->Dt1, a->Ct1, s);
     launch_adj_tchain(n, cs, c->ch2.a, c->ch2.d, c->ch2.c, a->At2, a->Dt2, a->Ct2, s);
@@ -1721,10 +1717,8 @@ static int tl_step(wd2_adj *a, const Regions &Rt, const double *un, const double
     };
     launch_mom_n<double>(nx, ny, ld, 1, c->M, un, vn, t.cnvn, t.difn, s);
     launch_mom_n<double>(nx, ny, ld, 2, c->M, un, vn, t2.cnvn, t2.difn, s);
-    launch_mom_asm<double>(c->R, ld, 1, P.dk, P.re, P.fr, c->M, t, us, vs, un, vn, c->d, c->dn, c->ch, c->ch2,
-                           c->zm, s, &c->sc->err, 0);
-    launch_mom_asm<double>(c->R, ld, 2, P.dk, P.re, P.fr, c->M, t2, us, vs, un, vn, c->d, c->dn, shift(c->ch, n1),
-                           shift(c->ch2, n1), c->zm + n1, s, &c->sc->err, 0);
+    launch_mom_asm_pair<double>(c->R, ld, P.dk, P.re, P.fr, c->M, t, t2, us, vs, un, vn, c->d, c->dn, c->ch, c->ch2,
+                                c->zm, shift(c->ch, n1), shift(c->ch2, n1), c->zm + n1, s, &c->sc->err, 0);
     launch_trid<double>(c->tw, n, cs, c->ch.a, c->ch.d, c->ch.c, c->ch.b, c->ch.x, s);
     // the time-level-n terms on dual numbers
     launch_tl_pack(nf, un, dun, a->tUN, s);
@@ -1741,10 +1735,8 @@ static int tl_step(wd2_adj *a, const Regions &Rt, const double *un, const double
     for (; m <= a->fp_max; m++) {
         launch_tl_pack(nf, us, dus, a->tUS, s);
         launch_tl_pack(nf, vs, dvs, a->tVS, s);
-        launch_mom_asm_tl(c->R, ld, 1, P.dk, P.re, P.fr, c->M, tt1, a->tUS, a->tVS, a->tUN, a->tVN, a->tch1, a->tch2,
-                          c->zm, s, &c->sc->err);
-        launch_mom_asm_tl(c->R, ld, 2, P.dk, P.re, P.fr, c->M, tt2, a->tUS, a->tVS, a->tUN, a->tVN,
-                          dshift(a->tch1, n1), dshift(a->tch2, n1), c->zm + n1, s, &c->sc->err);
+        launch_mom_asm_tl_pair(c->R, ld, P.dk, P.re, P.fr, c->M, tt1, tt2, a->tUS, a->tVS, a->tUN, a->tVN, a->tch1,
+                               a->tch2, c->zm, dshift(a->tch1, n1), dshift(a->tch2, n1), c->zm + n1, s, &c->sc->err);
         launch_tl_rhs(n, cs, a->tch1, c->ch.x, a->r, s);
         launch_trid<double>(c->tw, n, cs, c->ch.a, c->ch.d, c->ch.c, a->r, a->y, s);
         launch_adj_zmask(n, c->zm, a->y, a->y, s);

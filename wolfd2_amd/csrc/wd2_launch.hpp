@@ -207,6 +207,15 @@ void launch_mom_asm(const Regions &R, int ld, int comp, double dk, double re, do
                     const MomTempsT<T> &t, const double *us, const double *vs, const double *un, const double *vn,
                     const double *d, const double *dn, const ChainT<T> &ch1, const ChainT<T> &ch2, unsigned char *zm,
                     hipStream_t s, int *err, int live, const int *stop = nullptr);
+// Both components at once (x-momentum: t1, ch1x, ch2x, zmx; y-momentum: t2,
+// ch1y, ch2y, zmy), the same rows and bits as two launch_mom_asm calls.
+template <typename T>
+void launch_mom_asm_pair(const Regions &R, int ld, double dk, double re, double fr, const Metrics &m,
+                         const MomTempsT<T> &t1, const MomTempsT<T> &t2, const double *us, const double *vs,
+                         const double *un, const double *vn, const double *d, const double *dn,
+                         const ChainT<T> &ch1x, const ChainT<T> &ch2x, unsigned char *zmx, const ChainT<T> &ch1y,
+                         const ChainT<T> &ch2y, unsigned char *zmy, hipStream_t s, int *err, int live,
+                         const int *stop = nullptr);
 // live: bit 0, porous regions (load the Dupuit-Forchheimer coefficients);
 // bit 1, a density field (load d, dn); otherwise those values are +0.
 template <typename T>
@@ -339,9 +348,11 @@ void launch_adj_maxabs(long n, const double *x, unsigned long long *red, int slo
 // directional derivatives of the time-level-n terms and of the chains.
 void launch_mom_n_tl(int nx, int ny, int ld, int comp, const Metrics &m, const Dual *un, const Dual *vn, Dual *cnvn,
                      Dual *difn, hipStream_t s);
-void launch_mom_asm_tl(const Regions &R, int ld, int comp, double dk, double re, double fr, const Metrics &m,
-                       const MomTempsT<Dual> &t, const Dual *us, const Dual *vs, const Dual *un, const Dual *vn,
-                       const ChainT<Dual> &ch1, const ChainT<Dual> &ch2, unsigned char *zm, hipStream_t s, int *err);
+void launch_mom_asm_tl_pair(const Regions &R, int ld, double dk, double re, double fr, const Metrics &m,
+                            const MomTempsT<Dual> &t1, const MomTempsT<Dual> &t2, const Dual *us, const Dual *vs,
+                            const Dual *un, const Dual *vn, const ChainT<Dual> &ch1x, const ChainT<Dual> &ch2x,
+                            unsigned char *zmx, const ChainT<Dual> &ch1y, const ChainT<Dual> &ch2y,
+                            unsigned char *zmy, hipStream_t s, int *err);
 // out = (v, d) element-wise
 void launch_tl_pack(long n, const double *v, const double *d, Dual *out, hipStream_t s);
 // r = db - dS1 x1 from the dual first-split chains (k_tangent.hip)
