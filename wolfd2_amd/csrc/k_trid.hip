@@ -548,7 +548,9 @@ __global__ __launch_bounds__(TR_T) void k_trid_bwd(long n, Chains cs, const T *_
 
 // One workgroup: repair each block's tail from its successor's first value,
 // then, if any repair failed to coalesce inside its block, solve every chain
-// sequentially (exact in every case).
+// sequentially (exact in every case).  rearm (may be null) is set to 1 once
+// the solve has run: the window solver's keep flag, which is this solve's
+// stop flag, so it is set again for the next window solve.
 constexpr int TRF_T = 1024;
 template <typename T>
 __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nblk, const T *__restrict__ A,
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nb
                                                        const T *__restrict__ blk_x, const T *__restrict__ blk_xe,
                                                        int *__restrict__ fail, int *__restrict__ nfallback,
                                                        T *__restrict__ B2, const unsigned char *__restrict__ Z,
-                                                       const int *stop) {
+                                                       const int *stop, int *rearm) {
     if (stop && *stop) return;
     for (int blk = threadIdx.x; blk < nblk - 1; blk += TRF_T) {
         const long k0 = (long)blk * TR_R, k1 = min(k0 + TR_R, n);
@@ -579,6 +581,7 @@ __global__ __launch_bounds__(TRF_T) void k_trid_finish(long n, Chains cs, int nb
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
+    if (rearm) *rearm = 1;   // every thread read *stop at entry, before the barrier
     if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
     for (long c0 = 0; c0 < n;) {
         const long L = c0 < cs.n1 ? cs.L1 : cs.L2;
@@ -822,13 +825,17 @@ int trid_window_shape_ok(int wt) { return wt == 128 || wt == 256 || wt == 512; }
 template <typename T>
 void launch_trid_window(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C,
                         const T *B, T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop, int wt) {
-    int *keep = w.fail + 2;   // the solver's flag block: fail, nfallback, keep
-    hipLaunchKernelGGL(k_trid_keep_set, dim3(1), dim3(1), 0, s, keep);
+    // the solver's flag block: fail, nfallback, keep; keep is 1 between solves
+    // (set at allocation, and again by each fallback solve that runs)
+    int *keep = w.fail + 2;
     launch_window_shape<T>(wt, n, cs, A, D, C, B, X, s, B2, Z, stop, keep);
     // the bit-exact solver runs only where a window was too weakly dominant
-    // (keep cleared); it also returns at entry once the q-l loop has stopped,
-    // since then the window kernel left keep set
-    launch_trid<T>(w, n, cs, A, D, C, B, X, s, B2, Z, keep);
+    // (keep cleared) and then sets keep again; it also returns at entry once
+    // the q-l loop has stopped, since then the window kernel left keep set
+    launch_trid<T>(w, n, cs, A, D, C, B, X, s, B2, Z, keep, keep);
+}
+void launch_trid_keep_init(int *fail_block, hipStream_t s) {
+    hipLaunchKernelGGL(k_trid_keep_set, dim3(1), dim3(1), 0, s, fail_block + 2);
 }
 // reps back-to-back launches of the window kernel of shape wt between events
 // e0 and e1 (X written; no second right-hand side).
@@ -839,6 +846,8 @@ void launch_trid_window_bench(const TridWorkT<T> &w, long n, const Chains &cs, c
     (void)hipEventRecord(e0, s);
     for (int k = 0; k < reps; k++) launch_window_shape<T>(wt, n, cs, A, D, C, B, X, s, nullptr, nullptr, nullptr, keep);
     (void)hipEventRecord(e1, s);
+    // no fallback runs here: set keep again for the next solve
+    hipLaunchKernelGGL(k_trid_keep_set, dim3(1), dim3(1), 0, s, keep);
 }
 template void launch_trid_window_bench<float>(const TridWorkT<float> &, long, const Chains &, const float *,
                                               const float *, const float *, const float *, float *, int, hipStream_t,
@@ -851,14 +860,14 @@ int trid_blocks(long n) { return (int)((n + TR_R - 1) / TR_R); }
 
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
-                 T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop) {
+                 T *X, hipStream_t s, T *B2, const unsigned char *Z, const int *stop, int *rearm) {
     const int nb = trid_blocks(n);
     hipLaunchKernelGGL(k_trid_fwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
                        stop);
     hipLaunchKernelGGL(k_trid_bwd<T>, dim3(nb), dim3(TR_T), 0, s, n, cs, A, D, C, B, w.cp, w.bp, w.blk_c, w.blk_b,
                        X, w.blk_x, w.blk_xe, w.fail, B2, Z, stop);
     hipLaunchKernelGGL(k_trid_finish<T>, dim3(1), dim3(TRF_T), 0, s, n, cs, nb, A, D, C, B, w.cp, w.bp, X, w.blk_x,
-                       w.blk_xe, w.fail, w.nfallback, B2, Z, stop);
+                       w.blk_xe, w.fail, w.nfallback, B2, Z, stop, rearm);
 }
 // reps back-to-back launches of the forward (which = 1) or backward (2)
 // kernel of one solve between events e0 and e1, after a full solve that puts
@@ -900,9 +909,9 @@ template void launch_trid<float>(const TridWorkT<float> &, long, long, const flo
                                  const float *, float *, hipStream_t, float *, const unsigned char *);
 template void launch_trid<double>(const TridWorkT<double> &, long, const Chains &, const double *, const double *,
                                   const double *, const double *, double *, hipStream_t, double *,
-                                  const unsigned char *, const int *);
+                                  const unsigned char *, const int *, int *);
 template void launch_trid<float>(const TridWorkT<float> &, long, const Chains &, const float *, const float *,
                                  const float *, const float *, float *, hipStream_t, float *, const unsigned char *,
-                                 const int *);
+                                 const int *, int *);
 
 }  // namespace wd2

@@ -303,6 +303,7 @@ static int ctx_alloc(wd2_ctx **out, int nx, int ny) {
     c->twf = TridWorkT<float>{F(c->tw.cp),    F(c->tw.bp),     F(c->tw.blk_c), F(c->tw.blk_b),
                               F(c->tw.blk_x), F(c->tw.blk_xe), c->tw.fail,     c->tw.nfallback};
     HIPCHK(hipMemsetAsync(c->tw.fail, 0, 4 * sizeof(int), c->s));
+    launch_trid_keep_init(c->tw.fail, c->s);
     const size_t npart = 4 * (size_t)RED_MAX_BLOCKS + 2 * (size_t)(c->nf / 256 + 64);
     HIPCHK(hipMalloc(&c->partials, npart * sizeof(double)));
     HIPCHK(hipMalloc(&c->sc, sizeof(Scalars)));

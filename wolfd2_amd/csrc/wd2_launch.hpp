@@ -31,10 +31,14 @@ template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, long L, const T *A, const T *D, const T *C, const T *B, T *X,
                  hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr);
 // Any layout (n rows in all).  stop (optional): every kernel returns at entry
-// while *stop != 0.
+// while *stop != 0.  rearm (optional): set to 1 once the solve has run.
 template <typename T>
 void launch_trid(const TridWorkT<T> &w, long n, const Chains &cs, const T *A, const T *D, const T *C, const T *B,
-                 T *X, hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr, const int *stop = nullptr);
+                 T *X, hipStream_t s, T *B2 = nullptr, const unsigned char *Z = nullptr, const int *stop = nullptr,
+                 int *rearm = nullptr);
+// Set the window solver's keep flag (the third int of a solver's flag block
+// fail, nfallback, keep) to 1, its value between solves.
+void launch_trid_keep_init(int *fail_block, hipStream_t s);
 
 // The mixed path's FP32 chain solve (not bit-exact): overlapping windows
 // solved by partitioned Thomas + PCR in LDS, the bit-exact solver as the
