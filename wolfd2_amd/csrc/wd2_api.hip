@@ -849,12 +849,16 @@ static int ppe_mg(wd2_ctx *c, int cart, int maxcyc, double dk, double tol, doubl
     const size_t kt0 = c->kt_used;
     // Batches of V-cycles with a stopping check at the end of each, two in
     // flight (as the SOR solves): the first batch is the last solve's cycle
-    // count plus the final check, so a steady run rarely overruns
+    // count plus the final check, so a steady run rarely overruns; the next
+    // batches take 1, 2, 4, 8, then 16 cycles.  The one-cycle second batch is
+    // queued before the first batch's test is read, so the GPU has it to run
+    // (returning at entry once the solve has stopped) while the host reads
+    // the test and queues what follows the solve.
     int cyc = 0, issued = 0, checked = 0, stop = 0;
     const int total = maxcyc + 1;   // the last cycle only tests
     while (!stop) {
         if (cyc < total && issued - checked < 2) {
-            const int Q = issued == 0 ? std::max(1, c->mg_last + 1) : std::min(16, 2 << std::min(issued, 3));
+            const int Q = issued == 0 ? std::max(1, c->mg_last + 1) : 1 << std::min(issued - 1, 4);
             for (int q = 0; q < Q && cyc < total; q++) {
                 cyc++;
                 if (!cart) launch_rhs_cross(nx, ny, ld, c->M, c->rhs, p, c->beff, s);
