@@ -1035,9 +1035,12 @@ void launch_update_norms(int nx, int ny, int ld, double *us, double *vs, const d
 }
 
 // k_scatter (comp 1 from x1, comp 2 from x2) and k_update_norms per point:
-// dus is written where the X scatter writes it (j >= 2) and read elsewhere,
-// dvs likewise (i >= 2), so every stored value and every sum is the one the
-// separate kernels give.  A wavefront covers one row j.
+// dus is taken from x1 where the X scatter writes it (j >= 2) and read
+// elsewhere, dvs likewise (i >= 2), so every sum is the one the separate
+// kernels give.  Of the increments taken from the chains only those at
+// (5, 5) are stored: the fold reads the stored increments there, nothing
+// else reads them during the q-l loop, and every point outside the chain
+// rows keeps its stored value.  A wavefront covers one row j.
 template <typename T>
 __global__ __launch_bounds__(MX *MY) void k_scatter_update(int nx, int ny, int ld, const T *__restrict__ x1,
                                                            const T *__restrict__ x2, double *__restrict__ us,
@@ -1053,15 +1056,16 @@ __global__ __launch_bounds__(MX *MY) void k_scatter_update(int nx, int ny, int l
         if (i <= nx) {
             const size_t k = WD2_IX(i, j);
             double du, dv;
+            const bool at55 = i == 5 && j == 5;
             if (j >= 2) {
                 du = (double)x1[(long)(j - 2) * nx + (i - 1)];
-                dus[k] = du;
+                if (at55) dus[k] = du;
             } else {
                 du = dus[k];
             }
             if (i >= 2) {
                 dv = (double)x2[(long)(j - 1) * (nx - 1) + (i - 2)];
-                dvs[k] = dv;
+                if (at55) dvs[k] = dv;
             } else {
                 dv = dvs[k];
             }
