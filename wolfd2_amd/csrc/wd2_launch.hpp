@@ -200,9 +200,9 @@ template <typename T>
 void launch_conv_diff(int nx, int ny, int ld, int comp, const Metrics &m, const double *us, const double *un,
                       const MomTempsT<T> &t, hipStream_t s);
 // Fused kernels for grids without porous regions: k_mom_n writes cnvn / difn
-// of one component from un, vn; k_mom_asm assembles both splits of one
-// component from us, vs (ch1: the first split's chain; ch2: the second
-// split's coefficients; zm: its identity rows).
+// of one component from un, vn; k_mom_asm_cb and k_mom_asm_gen assemble both
+// splits of a component from us, vs (ch1: the first split's chain; ch2: the
+// second split's coefficients; zm: its identity rows).
 template <typename T>
 void launch_mom_n(int nx, int ny, int ld, int comp, const Metrics &m, const double *un, const double *vn, T *cnvn,
                   T *difn, hipStream_t s);
