@@ -349,7 +349,10 @@ int wd2_ctx_trid_set_window(wd2_ctx *ctx, int window_threads);
  * second step is linear); order 0, the plain start (the default).  The
  * stopping test and the frozen ghosts are the same, so the solve converges
  * to the same solution in fewer V-cycles; the iterates, and so the last bits
- * of the fields, differ from the plain start.  The adjoint's runs always use
+ * of the fields, differ from the plain start.  A solve from the extrapolated
+ * start applies at least two V-cycles (msorit permitting) before its
+ * tolerance may stop it: with one, the error the divergence test barely
+ * sees grows over thousands of steps.  The adjoint's runs always use
  * the plain start (and the step after them restarts the history).  Returns
  * the previous order; WD2_ERR_ARG for an order outside 0..2.  No
  * counterpart in the reference. */

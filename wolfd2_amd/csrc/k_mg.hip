@@ -176,13 +176,14 @@ __device__ inline bool mg_stopped(const Scalars *sc) { return sc->mg_done != 0; 
 
 // The stopping test before V-cycle `cycle` (1-based), evaluated by every
 // workgroup of the first kernel after the level-0 down kernel from that
-// kernel's per-tile maxima of |r|: stop when max |dk r| < tol (mg_nconv =
-// cycles applied), or when `maxcyc` cycles are applied (mg_nconv = -1, not
-// converged; a NaN never passes).  All workgroups reach the same answer;
-// workgroup 0 records it (val[7] = max |dk r|).  sh: NT / 64 doubles of LDS.
+// kernel's per-tile maxima of |r|: stop when max |dk r| < tol and at least
+// minc cycles are applied (mg_nconv = cycles applied), or when `maxcyc`
+// cycles are applied (mg_nconv = -1, not converged; a NaN never passes).
+// All workgroups reach the same answer; workgroup 0 records it (val[7] =
+// max |dk r|).  sh: NT / 64 doubles of LDS.
 template <int NT>
 __device__ inline bool mg_decide(const double *__restrict__ part, int nt, double dk, double tol, int cycle,
-                                 int maxcyc, Scalars *sc, double *sh) {
+                                 int maxcyc, int minc, Scalars *sc, double *sh) {
     double m = 0.0;
     for (int k = threadIdx.x; k < nt; k += NT)
         if (!(part[k] <= m)) m = part[k];
@@ -197,7 +198,7 @@ __device__ inline bool mg_decide(const double *__restrict__ part, int nt, double
         if (!(sh[w] <= m)) m = sh[w];
     __syncthreads();
     const double div = m * dk;
-    const bool conv = div < tol, cap = !conv && cycle - 1 >= maxcyc;
+    const bool conv = div < tol && cycle - 1 >= minc, cap = !conv && cycle - 1 >= maxcyc;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->val[7] = div;
         sc->mg_cycles = cycle - 1;
@@ -214,6 +215,7 @@ struct MgDecide {
     const double *part;
     int nt, cycle, maxcyc;   // nt = 0: this kernel does not decide
     double dk, tol;
+    int minc;                // cycles to apply before the tolerance may stop the solve
 };
 
 // XCD-aware tile order: consecutive workgroups go to different XCDs; each XCD
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(MgLevel L, MgLevel C, int ntx, i
     __shared__ T E[TW * TW], R[TW * TW], AX[UF ? 1 : TW * (TW + 1)], AY[UF ? 1 : (TW + 1) * TW];
     __shared__ double PS[(L0 && STAGE) ? PW * PW : 1];
     __shared__ double red[NT / 64];
-    if (D.nt > 0 && mg_decide<NT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, sc, red)) return;
+    if (D.nt > 0 && mg_decide<NT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, D.minc, sc, red)) return;
     const int tile = mg_tile(ntiles);
     if (tile >= ntiles) return;
     const int I0 = (tile % ntx) * MG_OT, J0 = (tile / ntx) * MG_OT;
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(MG_BT) void k_mg_bottom(MgBottom B, MgDecide D, Sca
     __shared__ T E[MG_BCAP], R[MG_BCAP];
     __shared__ T FX[FC], FY[FC];
     __shared__ double red[MG_BT / 64];
-    if (D.nt > 0 && mg_decide<MG_BT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, sc, red)) return;
+    if (D.nt > 0 && mg_decide<MG_BT>(D.part, D.nt, D.dk, D.tol, D.cycle, D.maxcyc, D.minc, sc, red)) return;
     // each level's offset in E and R, and in FX and FY (in LDS: indexed by
     // the level, they would otherwise go to scratch)
     __shared__ int off[MG_MAXLEV], ofx[MG_MAXLEV], ofy[MG_MAXLEV];
@@ -786,13 +788,13 @@ static void up0(const MgHierarchy &H, int ld, double *p, const double *uni, Scal
 template <typename T>
 void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double *b, double *p, const double *rau,
                      const double *rgv, const double *uni, double dk, double tol, int cycle, int maxcyc,
-                     double *part, Scalars *sc, hipStream_t s, hipEvent_t *ev_down, hipEvent_t *ev_up) {
+                     double *part, Scalars *sc, hipStream_t s, hipEvent_t *ev_down, hipEvent_t *ev_up, int minc) {
     int ntx = 0;
     const int nt0 = mg_tiles(H.lev[0].nx, H.lev[0].ny, &ntx);
     if (ev_down) (void)hipEventRecord(ev_down[0], s);
     down0<T>(H, nx, ny, ld, b, p, rau, rgv, uni, part, sc, s);
     if (ev_down) (void)hipEventRecord(ev_down[1], s);
-    const MgDecide D{part, nt0, cycle, maxcyc, dk, tol};
+    const MgDecide D{part, nt0, cycle, maxcyc, dk, tol, minc};
     const MgDecide none{nullptr, 0, 0, 0, 0.0, 0.0};
     for (int l = 1; l < H.ntiled; l++) {
         const int nt = mg_tiles(H.lev[l].nx, H.lev[l].ny, &ntx);
@@ -849,10 +851,10 @@ template void launch_mg_coef<double>(const MgHierarchy &, int, int, int, const d
                                      const double *, hipStream_t);
 template void launch_mg_cycle<float>(const MgHierarchy &, int, int, int, const double *, double *, const double *,
                                      const double *, const double *, double, double, int, int, double *, Scalars *,
-                                     hipStream_t, hipEvent_t *, hipEvent_t *);
+                                     hipStream_t, hipEvent_t *, hipEvent_t *, int);
 template void launch_mg_cycle<double>(const MgHierarchy &, int, int, int, const double *, double *, const double *,
                                       const double *, const double *, double, double, int, int, double *, Scalars *,
-                                      hipStream_t, hipEvent_t *, hipEvent_t *);
+                                      hipStream_t, hipEvent_t *, hipEvent_t *, int);
 template void launch_mg_kernel_bench<float>(const MgHierarchy &, int, int, int, int, int, const double *, double *,
                                             const double *, const double *, const double *, double *, Scalars *,
                                             hipStream_t, hipEvent_t, hipEvent_t);

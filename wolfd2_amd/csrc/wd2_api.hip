@@ -160,6 +160,9 @@ struct wd2_ctx {
     // the steps in a row that wrote a slot.
     int mg_extrap = 0;
     bool extrap_off = false;
+    // V-cycles a multigrid solve applies before its tolerance may stop it: 2
+    // for a solve from the extrapolated start (mg_min_cycles), else 0
+    int mg_minc = 0;
     double *phist = nullptr;
     int p_hist = 0, xw = 0, x_run = 0, g_x = 0;
     std::vector<GraphEntry> graphs;
@@ -865,10 +868,10 @@ static int ppe_mg(wd2_ctx *c, int cart, int maxcyc, double dk, double tol, doubl
                 hipEvent_t *ed = kt_events(c, 1, 2), *eu = kt_events(c, 1, 3);
                 if (mx)
                     launch_mg_cycle<float>(c->mg, nx, ny, ld, b, p, c->M.rau, c->M.rgv, uni, dk, tol, cyc, maxcyc,
-                                           c->mg_part, c->sc, s, ed, eu);
+                                           c->mg_part, c->sc, s, ed, eu, c->mg_minc);
                 else
                     launch_mg_cycle<double>(c->mg, nx, ny, ld, b, p, c->M.rau, c->M.rgv, uni, dk, tol, cyc, maxcyc,
-                                            c->mg_part, c->sc, s, ed, eu);
+                                            c->mg_part, c->sc, s, ed, eu, c->mg_minc);
             }
             HIPCHK(hipMemcpyAsync(&c->h_flag[issued % 2], &c->sc->mg_done, sizeof(int), hipMemcpyDeviceToHost, s));
             HIPCHK(hipEventRecord(c->bev[issued % 2], s));
@@ -1100,6 +1103,14 @@ static double *extrap_p1(const wd2_ctx *c) { return c->phist + (size_t)c->xw * c
 static const double *extrap_p2(const wd2_ctx *c, int x) {
     return x == 2 ? c->phist + (size_t)(c->xw ^ 1) * c->nf : nullptr;
 }
+// The V-cycles a solve from the extrapolated start applies before its
+// tolerance may stop it.  A guess so close that one cycle meets the
+// tolerance leaves the error in the smooth modes, which the divergence test
+// barely sees, damped by one cycle only; the next steps extrapolate that
+// error again, and over thousands of such steps it grows (a 1024^2 cavity:
+// mixed against FP64 p 2e-4 relative L2 after 5000 steps, from 1e-7).  Two
+// cycles damp it more than the extrapolation can amplify it.
+static int mg_min_cycles(const wd2_params *P) { return std::min(2, std::max(P->msorit, 0)); }
 // A step of order x completed: flip the slot, count the history.
 static void extrap_advance(wd2_ctx *c, int x) {
     if (x) {
@@ -1166,7 +1177,9 @@ static int one_step(wd2_ctx *c, const wd2_params *P, int *nql, int *nsor, double
         launch_vel_bc<double>(c->R, ld, c->us, c->vs, false, &c->sc->err, s);
         launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
         if (xp && l == 1) launch_mg_extrap(nx, ny, ld, c->p, extrap_p1(c), extrap_p2(c, xp), s);
+        c->mg_minc = (xp && l == 1) ? mg_min_cycles(P) : 0;
         st = ppe(c, P->lCartesGrid, P->nPpeSolver, P->msorit, P->dk, P->sortol, P->sorrel, c->us, c->vs, c->p, nsor);
+        c->mg_minc = 0;
         if (st) return st;
         launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
         launch_project(c->R, ld, P->dk, c->M, c->p, c->us, c->vs, s);
@@ -1283,10 +1296,10 @@ static void step_body(wd2_ctx *c, const wd2_params *P, int Q, int C, int xp) {
         if (!P->lCartesGrid) launch_rhs_cross(nx, ny, ld, c->M, c->rhs, c->p, c->beff, s);
         if (c->precision == WD2_PREC_MIXED)
             launch_mg_cycle<float>(c->mg, nx, ny, ld, b, c->p, c->M.rau, c->M.rgv, uni, P->dk, P->sortol, cyc,
-                                   P->msorit, c->mg_part, c->sc, s, nullptr, nullptr);
+                                   P->msorit, c->mg_part, c->sc, s, nullptr, nullptr, xp ? mg_min_cycles(P) : 0);
         else
             launch_mg_cycle<double>(c->mg, nx, ny, ld, b, c->p, c->M.rau, c->M.rgv, uni, P->dk, P->sortol, cyc,
-                                    P->msorit, c->mg_part, c->sc, s, nullptr, nullptr);
+                                    P->msorit, c->mg_part, c->sc, s, nullptr, nullptr, xp ? mg_min_cycles(P) : 0);
     }
     launch_pres_bc<double>(c->R, ld, c->p, &c->sc->err, s);
     launch_project(c->R, ld, P->dk, c->M, c->p, c->us, c->vs, s);

@@ -289,11 +289,12 @@ void launch_mg_coef(const MgHierarchy &H, int nx, int ny, int ld, const double *
                     const double *uni, hipStream_t s);
 // One V-cycle of defect correction on b (1-based cycle count); part: one
 // double per level-0 tile.  ev_down / ev_up (optional): an event pair around
-// the level-0 down / up kernel.
+// the level-0 down / up kernel.  minc: the tolerance stops the solve only
+// once at least minc cycles are applied.
 template <typename T>
 void launch_mg_cycle(const MgHierarchy &H, int nx, int ny, int ld, const double *b, double *p, const double *rau,
                      const double *rgv, const double *uni, double dk, double tol, int cycle, int maxcyc,
-                     double *part, Scalars *sc, hipStream_t s, hipEvent_t *ev_down, hipEvent_t *ev_up);
+                     double *part, Scalars *sc, hipStream_t s, hipEvent_t *ev_down, hipEvent_t *ev_up, int minc = 0);
 void launch_mg_reset(Scalars *sc, hipStream_t s);
 // reps back-to-back launches of the level-0 down (which = 2) or up (3) kernel
 // between events e0 and e1 (clears the stop flag; the up kernel changes p).
