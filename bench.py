@@ -515,7 +515,7 @@ def main() -> int:
         "data": "synthetic (cold-start lid-driven cavity)",
         "config": {"workload": f"lid-driven cavity {n}x{n} cells, Re=1000, dt=2e-4; q-l tolerance 1e-4/20; "
                                "Poisson by multigrid V-cycles to a divergence max|dk(b-Ap)| <= 1e-10, each solve from the "
-                               "extrapolated pressure 3p^n - 3p^(n-1) + p^(n-2); mixed path: "
+                               "extrapolated pressure 3p^n - 3p^(n-1) + p^(n-2) (two V-cycles at least); mixed path: "
                                "FP32 momentum stencils and ADI chain solves, FP32 V-cycles inside FP64 defect "
                                "correction, FP64 fields and pressure accumulation; FP64 path: all FP64",
                    "cells": cells, "path": "mixed",
